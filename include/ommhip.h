@@ -1,0 +1,248 @@
+/*
+ * ommhip.h — C ABI of the MI355X (gfx950) NonbondedForce/PME operator and the
+ * water-box MD step built around it.
+ *
+ * The reference operator is the Rust `impl<B: Backend> Force<B> for
+ * NonbondedForce` (crates/openmm-core/src/forces/nonbonded.rs:1053-1155),
+ * implementing `trait Force<B>` (crates/openmm-core/src/forces/mod.rs:142-246).
+ * Every entry point below names the reference item it stands in for, so a Rust
+ * `impl Force<B>` (or any other FFI host) can forward to it one call per method.
+ *
+ * Conventions (same as the reference):
+ *   - units nm, ps, kJ/mol, elementary charge, Da;
+ *   - positions/forces are N x 3 row-major doubles (x0 y0 z0 x1 ...);
+ *   - box vectors are 9 doubles, rows a, b, c, in reduced triclinic form
+ *     (crates/openmm-core/src/system.rs:16-39); NULL means non-periodic;
+ *   - forces are returned in a caller-owned buffer that is OVERWRITTEN (the
+ *     reference returns a new owned tensor that the Context sums).
+ *
+ * Errors: every function returns an omm_status; nothing unwinds across the
+ * ABI.  omm_last_error() returns a thread-local message for the last failure.
+ * A handle is used by one host thread at a time (calls on one handle are
+ * serialised by an internal mutex, matching the reference's Mutex-guarded
+ * caches, pme.rs:10, neighbors.rs:8).
+ */
+#ifndef OMMHIP_H
+#define OMMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMMHIP_ABI_VERSION 1
+
+typedef enum omm_status {
+    OMM_OK = 0,
+    OMM_ERR_INVALID = 1,     /* bad argument / descriptor (reference: assert!) */
+    OMM_ERR_CUTOFF_BOX = 2,  /* cutoff > half the box (nonbonded.rs:1080-1085, 806-812) */
+    OMM_ERR_NONFINITE = 3,   /* non-finite position or energy (neighbors.rs:108-111) */
+    OMM_ERR_OVERFLOW = 4,    /* fixed-point or neighbour-list capacity overflow */
+    OMM_ERR_DEVICE = 5,      /* HIP runtime error or no usable gfx950 device */
+    OMM_ERR_STATE = 6        /* call out of order (e.g. compute before initialize) */
+} omm_status;
+
+/* NonbondedMethod, nonbonded.rs:20-33. */
+typedef enum omm_nb_method {
+    OMM_NB_NO_CUTOFF = 0,
+    OMM_NB_CUTOFF_NON_PERIODIC = 1,
+    OMM_NB_CUTOFF_PERIODIC = 2,
+    OMM_NB_EWALD = 3,
+    OMM_NB_PME = 4,
+    OMM_NB_LJPME = 5
+} omm_nb_method;
+
+/* Arithmetic of the device path.
+ *   MIXED : FP32 pair/spline/FFT math, int64 fixed-point force/grid/energy
+ *           accumulation, FP64 positions, integrator and final reductions.
+ *   DOUBLE: the plain FP64 form — same algorithms in FP64 throughout, FP64
+ *           atomics, one launch sequence per system and no graph. */
+typedef enum omm_precision {
+    OMM_PRECISION_MIXED = 0,
+    OMM_PRECISION_DOUBLE = 1
+} omm_precision;
+
+/*
+ * Description of a NonbondedForce: the builder state of the reference struct
+ * (nonbonded.rs:75-108) as plain arrays.  Setter equivalents:
+ * add_particle (165), add_exception (197), set_nonbonded_method (502),
+ * set_cutoff_distance (520), set_reaction_field_dielectric (533),
+ * set_ewald_error_tolerance (545), set_exceptions_use_periodic_boundary_conditions
+ * (557), set_use_dispersion_correction (570), set_use_switching_function (576),
+ * set_switching_distance (583), set_pme_parameters (595), set_force_group
+ * (forces/mod.rs:58), set_reciprocal_space_force_group (115),
+ * set_include_direct_space (125).  The arrays are copied at create time.
+ */
+typedef struct omm_nb_desc {
+    int32_t num_particles;
+    const double* charge;    /* [num_particles] e */
+    const double* sigma;     /* [num_particles] nm */
+    const double* epsilon;   /* [num_particles] kJ/mol */
+    int32_t num_exceptions;
+    const int32_t* exception_particle1;  /* [num_exceptions] */
+    const int32_t* exception_particle2;
+    const double* exception_charge_prod;
+    const double* exception_sigma;
+    const double* exception_epsilon;
+    int32_t method;                      /* omm_nb_method */
+    double cutoff;                       /* nm, default 1.0 */
+    double reaction_field_dielectric;    /* default 78.3 */
+    double ewald_error_tolerance;        /* default 5e-4 */
+    double pme_alpha;                    /* 0 = derive from tolerance (nonbonded.rs:647-658) */
+    int32_t pme_grid[3];
+    int32_t use_dispersion_correction;   /* default 1 */
+    int32_t exceptions_use_pbc;          /* default 0 */
+    int32_t use_switching_function;      /* default 0 */
+    double switching_distance;           /* default -1 */
+    int32_t force_group;                 /* 0..31, default 0 */
+    int32_t reciprocal_force_group;      /* -1..31, default -1 (= force_group) */
+    int32_t include_direct_space;        /* default 1 */
+} omm_nb_desc;
+
+/* Fill *desc with the reference defaults of NonbondedForce::new (nonbonded.rs:129-162);
+ * array pointers are set to NULL and counts to 0. */
+void omm_nb_desc_defaults(omm_nb_desc* desc);
+
+typedef struct omm_nb omm_nb;
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char* omm_last_error(void);
+
+/* OMMHIP_ABI_VERSION of the loaded library. */
+int32_t omm_abi_version(void);
+
+/* Number of visible gfx950 devices (0 when none; never an error on a host
+ * without a GPU). */
+int32_t omm_device_count(void);
+
+/* Validates and copies the description; allocates no device memory yet.
+ * Mirrors NonbondedForce::new + the builders above; the parameter checks are
+ * validate_particle_parameters (nonbonded.rs:1252-1261) and the setter asserts. */
+omm_status omm_nb_create(const omm_nb_desc* desc, int32_t precision, omm_nb** out);
+
+/* Force::initialize (nonbonded.rs:1055-1107): checks exception indices and
+ * duplicates, switching distance, cutoff <= box/2, then freezes the PME alpha
+ * and grid from this box for the lifetime of the handle.  Allocates the device
+ * buffers.  box may be NULL for non-periodic methods. */
+omm_status omm_nb_initialize(omm_nb* nb, int32_t num_particles, const double* box);
+
+/* The frozen (alpha, grid) — NonbondedForce::pme_parameters after initialize
+ * (initialized_reciprocal, nonbonded.rs:1086-1102). */
+omm_status omm_nb_get_pme_parameters(const omm_nb* nb, double* alpha, int32_t grid[3]);
+
+/* Force::calc_forces_and_energy (nonbonded.rs:1125-1131): direct space if
+ * include_direct_space, reciprocal always.  pos: host N x 3; box: host 9 or
+ * NULL; forces_out: host N x 3 (overwritten); energy_out: kJ/mol. */
+omm_status omm_nb_calc_forces_and_energy(omm_nb* nb, const double* pos, const double* box,
+                                         double* forces_out, double* energy_out);
+
+/* Force::calc_forces_and_energy_groups (nonbonded.rs:1133-1150).  *evaluated
+ * is set to 0 (and outputs untouched) when neither the direct nor the
+ * reciprocal group is selected — the reference's `None`. */
+omm_status omm_nb_calc_forces_and_energy_groups(omm_nb* nb, const double* pos, const double* box,
+                                                uint32_t groups, double* forces_out,
+                                                double* energy_out, int32_t* evaluated);
+
+/* Energy split of the last evaluation (same call as above, with
+ * include_direct = include_reciprocal = 1):
+ *   parts[0] direct-space pairs (LJ + screened/plain Coulomb),
+ *   parts[1] exceptions incl. the erf reciprocal correction (nonbonded.rs:995-1036),
+ *   parts[2] reciprocal-space mesh energy (pme.rs:339-372),
+ *   parts[3] self + neutralising energy (nonbonded.rs:965-973),
+ *   parts[4] dispersion correction (nonbonded.rs:991-993).
+ * Their sum is the energy of omm_nb_calc_forces_and_energy. */
+omm_status omm_nb_energy_parts(omm_nb* nb, const double* pos, const double* box, double parts[5]);
+
+/* Diagnostics for neighbour/exclusion-set parity: the number of direct-space
+ * pairs (i<j, not excluded, r^2 < cutoff^2 with the reference's FP64
+ * minimum-image arithmetic, nonbonded.rs:842-864) that the device evaluated in
+ * the last call, and an order-independent 64-bit hash of the set: the wrapping
+ * sum over pairs of omm_pair_key(i, j). */
+omm_status omm_nb_pair_stats(omm_nb* nb, const double* pos, const double* box,
+                             int64_t* num_pairs, uint64_t* pair_hash);
+
+/* Mixing function of the pair hash (splitmix64 finaliser of (i << 32) | j). */
+uint64_t omm_pair_key(int32_t i, int32_t j);
+
+/* Frees everything owned by the handle. NULL is accepted. */
+omm_status omm_nb_destroy(omm_nb* nb);
+
+/* ------------------------------------------------------------------------ */
+/* MD context: the step around the operator for the flexible-water workloads.
+ * Stands in for Context::step (context.rs:795-928) with a LangevinMiddle
+ * integrator (integrators/langevin.rs:103-142), HarmonicBondForce
+ * (harmonic_bond.rs:93-148), HarmonicAngleForce (harmonic_angle.rs:99-186) and
+ * the NonbondedForce above, all resident on the device.  Random numbers come
+ * from a counter-based Philox4x32-10 stream keyed by (seed, step, atom), so the
+ * trajectory is reproducible and resumable but not the reference's ChaCha12
+ * stream (the reference notes random paths differ across implementations). */
+
+typedef struct omm_md_desc {
+    int32_t num_particles;
+    const double* masses;              /* [N] Da; 0 = fixed particle */
+    const omm_nb_desc* nonbonded;
+    int32_t num_bonds;
+    const int32_t* bond_particle1;
+    const int32_t* bond_particle2;
+    const double* bond_length;         /* nm */
+    const double* bond_k;              /* kJ/mol/nm^2 */
+    int32_t num_angles;
+    const int32_t* angle_particle1;
+    const int32_t* angle_particle2;    /* vertex */
+    const int32_t* angle_particle3;
+    const double* angle_theta0;        /* rad */
+    const double* angle_k;             /* kJ/mol/rad^2 */
+    double box[9];
+    double temperature;                /* K */
+    double friction;                   /* 1/ps */
+    double step_size;                  /* ps */
+    uint64_t seed;
+} omm_md_desc;
+
+typedef struct omm_ctx omm_ctx;
+
+/* Context::new (context.rs:83-161) for the description above: initialises the
+ * nonbonded operator (freezing alpha/grid) and uploads the topology. */
+omm_status omm_ctx_create(const omm_md_desc* desc, int32_t precision, omm_ctx** out);
+
+/* Context::set_positions / set_velocities (host N x 3). */
+omm_status omm_ctx_set_positions(omm_ctx* ctx, const double* pos);
+omm_status omm_ctx_set_velocities(omm_ctx* ctx, const double* vel);
+
+/* Context::set_velocities_to_temperature (context.rs:182-223): Maxwell-Boltzmann
+ * velocities from `seed`, minus F/m * dt/2 (the LangevinMiddle velocity time
+ * offset, langevin.rs:204-206). */
+omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperature, uint64_t seed);
+
+/* Context::step (context.rs:795): nsteps LangevinMiddle steps. The step is
+ * captured once into a HIP graph (mixed precision) and replayed. */
+omm_status omm_ctx_step(omm_ctx* ctx, int32_t nsteps);
+
+/* Context::get_state (context.rs:930-1105) with positions, velocities, forces
+ * and potential energy; any output pointer may be NULL. */
+omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* forces,
+                             double* potential_energy, int64_t* step_count);
+
+/* Kinetic energy (context.rs:1107-1163), kJ/mol, of the current velocities. */
+omm_status omm_ctx_kinetic_energy(omm_ctx* ctx, double* kinetic_energy);
+
+omm_status omm_ctx_destroy(omm_ctx* ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Seeded synthetic flexible 3-site water box (host only, no device use).
+ * num_molecules waters: O on a jittered cubic lattice (m = ceil(M^(1/3)) sites
+ * per axis, spacing L/m, jitter U(+-0.02 nm)), H at r(OH)=0.09572 nm and
+ * HOH = 104.52 deg, uniformly random orientation (Shoemake quaternion).
+ * Random stream: splitmix64-seeded xoshiro256**.  Atom order O, H1, H2 per
+ * molecule.  positions: [3*num_molecules*3] nm.  Charges/LJ/masses are the
+ * fixed water parameters (O: -0.834 e, sigma 0.315061 nm, eps 0.636386 kJ/mol,
+ * 15.999 Da; H: +0.417 e, sigma 1.0, eps 0, 1.008 Da). */
+omm_status omm_water_box(int32_t num_molecules, double box_edge, uint64_t seed, double* positions);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OMMHIP_H */

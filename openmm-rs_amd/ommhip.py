@@ -1,0 +1,535 @@
+"""Python host binding of the ommhip C ABI (include/ommhip.h).
+
+Mirrors the reference's Python surface for this path — the pyo3 classes
+``NonbondedForce`` (crates/openmm-py/src/lib.rs:222-530) and the Context step /
+state calls (lib.rs:1999-2403) — on top of ``libommhip.so`` through ctypes.
+Builder calls keep the reference's names, argument meaning and assertion
+behaviour (nonbonded.rs:165-659); evaluation goes to the gfx950 library.  There
+is no CPU fallback: evaluation raises if the HIP library or a device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libommhip.so")
+
+NO_CUTOFF, CUTOFF_NON_PERIODIC, CUTOFF_PERIODIC, EWALD, PME, LJPME = range(6)
+METHODS = {
+    "NoCutoff": NO_CUTOFF,
+    "CutoffNonPeriodic": CUTOFF_NON_PERIODIC,
+    "CutoffPeriodic": CUTOFF_PERIODIC,
+    "Ewald": EWALD,
+    "PME": PME,
+    "LJPME": LJPME,
+}
+PRECISION_MIXED, PRECISION_DOUBLE = 0, 1
+STATUS = {
+    0: "OK",
+    1: "INVALID",
+    2: "CUTOFF_BOX",
+    3: "NONFINITE",
+    4: "OVERFLOW",
+    5: "DEVICE",
+    6: "STATE",
+}
+
+# Water force field of the workloads (charges/LJ of a 3-site model; flexible
+# O-H bonds and H-O-H angle; masses in Da).
+WATER = dict(
+    q_o=-0.834, q_h=0.417, sigma_o=0.315061, eps_o=0.636386, sigma_h=1.0, eps_h=0.0,
+    mass_o=15.999, mass_h=1.008, r_oh=0.09572, k_bond=462750.0,
+    theta0=104.52 * math.pi / 180.0, k_angle=836.8,
+)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class NbDesc(C.Structure):
+    """ctypes image of ``omm_nb_desc``."""
+
+    _fields_ = [
+        ("num_particles", C.c_int32),
+        ("charge", _dp), ("sigma", _dp), ("epsilon", _dp),
+        ("num_exceptions", C.c_int32),
+        ("exception_particle1", _ip), ("exception_particle2", _ip),
+        ("exception_charge_prod", _dp), ("exception_sigma", _dp), ("exception_epsilon", _dp),
+        ("method", C.c_int32),
+        ("cutoff", C.c_double),
+        ("reaction_field_dielectric", C.c_double),
+        ("ewald_error_tolerance", C.c_double),
+        ("pme_alpha", C.c_double),
+        ("pme_grid", C.c_int32 * 3),
+        ("use_dispersion_correction", C.c_int32),
+        ("exceptions_use_pbc", C.c_int32),
+        ("use_switching_function", C.c_int32),
+        ("switching_distance", C.c_double),
+        ("force_group", C.c_int32),
+        ("reciprocal_force_group", C.c_int32),
+        ("include_direct_space", C.c_int32),
+    ]
+
+
+class MdDesc(C.Structure):
+    """ctypes image of ``omm_md_desc``."""
+
+    _fields_ = [
+        ("num_particles", C.c_int32),
+        ("masses", _dp),
+        ("nonbonded", C.POINTER(NbDesc)),
+        ("num_bonds", C.c_int32),
+        ("bond_particle1", _ip), ("bond_particle2", _ip),
+        ("bond_length", _dp), ("bond_k", _dp),
+        ("num_angles", C.c_int32),
+        ("angle_particle1", _ip), ("angle_particle2", _ip), ("angle_particle3", _ip),
+        ("angle_theta0", _dp), ("angle_k", _dp),
+        ("box", C.c_double * 9),
+        ("temperature", C.c_double),
+        ("friction", C.c_double),
+        ("step_size", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None and a.size else None
+
+
+_lib = None
+
+
+def library():
+    """Load libommhip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        lib = C.CDLL(LIB_PATH)
+        lib.omm_last_error.restype = C.c_char_p
+        lib.omm_abi_version.restype = C.c_int32
+        lib.omm_device_count.restype = C.c_int32
+        lib.omm_pair_key.restype = C.c_uint64
+        lib.omm_pair_key.argtypes = [C.c_int32, C.c_int32]
+        lib.omm_nb_desc_defaults.restype = None
+        lib.omm_water_box.argtypes = [C.c_int32, C.c_double, C.c_uint64, _dp]
+        lib.omm_ctx_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
+        _lib = lib
+    return _lib
+
+
+class OmmError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def _check(status):
+    if status != 0:
+        raise OmmError(status, library().omm_last_error().decode())
+
+
+def device_count() -> int:
+    return int(library().omm_device_count())
+
+
+def pair_key(i: int, j: int) -> int:
+    return int(library().omm_pair_key(i, j))
+
+
+def water_box(num_molecules: int, box_edge: float, seed: int) -> np.ndarray:
+    """Seeded flexible-water coordinates, shape (3*num_molecules, 3)."""
+    out = np.zeros((3 * num_molecules, 3))
+    _check(library().omm_water_box(num_molecules, box_edge, seed, out.ctypes.data_as(_dp)))
+    return out
+
+
+def cubic_box(edge: float) -> np.ndarray:
+    return np.diag([edge, edge, edge]).astype(np.float64)
+
+
+@dataclass
+class NonbondedForce:
+    """Builder with the reference's NonbondedForce surface (nonbonded.rs:110-659)."""
+
+    particles: list = field(default_factory=list)
+    exceptions: list = field(default_factory=list)
+    method: int = NO_CUTOFF
+    cutoff: float = 1.0
+    reaction_field_dielectric: float = 78.3
+    ewald_error_tolerance: float = 5e-4
+    pme: tuple = (0.0, (0, 0, 0))
+    use_dispersion_correction: bool = True
+    exceptions_use_pbc: bool = False
+    use_switching_function: bool = False
+    switching_distance: float = -1.0
+    force_group: int = 0
+    reciprocal_force_group: int = -1
+    include_direct_space: bool = True
+
+    @staticmethod
+    def _validate(charge, sigma, epsilon):
+        # nonbonded.rs:1252-1261
+        if not (math.isfinite(charge) and math.isfinite(sigma) and math.isfinite(epsilon)
+                and sigma >= 0.0 and epsilon >= 0.0):
+            raise ValueError("nonbonded parameters must be finite with nonnegative sigma and epsilon")
+
+    def add_particle(self, charge, sigma, epsilon):
+        self._validate(charge, sigma, epsilon)
+        self.particles.append((float(charge), float(sigma), float(epsilon)))
+        return len(self.particles) - 1
+
+    def add_exception(self, p1, p2, charge_prod, sigma, epsilon):
+        self._validate(charge_prod, sigma, epsilon)
+        self.exceptions.append((int(p1), int(p2), float(charge_prod), float(sigma), float(epsilon)))
+        return len(self.exceptions) - 1
+
+    def num_particles(self):
+        return len(self.particles)
+
+    def num_exceptions(self):
+        return len(self.exceptions)
+
+    def set_nonbonded_method(self, method):
+        if isinstance(method, str):
+            if method not in METHODS:
+                raise ValueError("unknown method; expected NoCutoff, CutoffNonPeriodic, "
+                                 "CutoffPeriodic, Ewald, PME, or LJPME")
+            method = METHODS[method]
+        self.method = int(method)
+
+    def set_cutoff_distance(self, cutoff):
+        if not (math.isfinite(cutoff) and cutoff > 0.0):
+            raise ValueError("cutoff must be positive and finite")
+        self.cutoff = float(cutoff)
+
+    def set_reaction_field_dielectric(self, eps):
+        if math.isnan(eps) or eps < 1.0:
+            raise ValueError("reaction field dielectric must be at least one")
+        self.reaction_field_dielectric = float(eps)
+
+    def set_ewald_error_tolerance(self, tol):
+        if not (math.isfinite(tol) and 0.0 < tol < 0.5):
+            raise ValueError("Ewald tolerance must be between zero and one half")
+        self.ewald_error_tolerance = float(tol)
+
+    def set_pme_parameters(self, alpha, nx, ny, nz):
+        if not (math.isfinite(alpha) and alpha >= 0.0):
+            raise ValueError("PME alpha must be nonnegative and finite")
+        if alpha != 0.0 and min(nx, ny, nz) < 5:
+            raise ValueError("PME grid dimensions must be at least five")
+        self.pme = (float(alpha), (int(nx), int(ny), int(nz))) if alpha != 0.0 else (0.0, (0, 0, 0))
+
+    def set_use_dispersion_correction(self, enabled):
+        self.use_dispersion_correction = bool(enabled)
+
+    def set_exceptions_use_periodic_boundary_conditions(self, periodic):
+        self.exceptions_use_pbc = bool(periodic)
+
+    def set_use_switching_function(self, enabled):
+        self.use_switching_function = bool(enabled)
+
+    def set_switching_distance(self, distance):
+        if not (math.isfinite(distance) and distance >= 0.0):
+            raise ValueError("switching distance must be nonnegative and finite")
+        self.switching_distance = float(distance)
+
+    def set_force_group(self, group):
+        if not 0 <= group < 32:
+            raise ValueError("force group must be between zero and 31")
+        self.force_group = int(group)
+
+    def set_reciprocal_space_force_group(self, group):
+        if not -1 <= group < 32:
+            raise ValueError("reciprocal force group must be between -1 and 31")
+        self.reciprocal_force_group = int(group)
+
+    def set_include_direct_space(self, enabled):
+        self.include_direct_space = bool(enabled)
+
+    def uses_periodic_boundary_conditions(self):
+        return self.method in (CUTOFF_PERIODIC, EWALD, PME, LJPME)
+
+    def create_exceptions_from_bonds(self, bonds, coulomb14scale, lj14scale):
+        """nonbonded.rs:434-500 (1-2 and 1-3 exclusions, scaled 1-4 pairs)."""
+        n = len(self.particles)
+        nbr = [[] for _ in range(n)]
+        for a, b in bonds:
+            nbr[a].append(b)
+            nbr[b].append(a)
+        excl, p14 = set(), set()
+        for i in range(n):
+            for j in nbr[i]:
+                if i < j:
+                    excl.add((i, j))
+            for j in nbr[i]:
+                for k in nbr[j]:
+                    if k != i:
+                        excl.add((min(i, k), max(i, k)))
+            for j in nbr[i]:
+                for k in nbr[j]:
+                    if k == i:
+                        continue
+                    for l in nbr[k]:
+                        if l in (j, i):
+                            continue
+                        key = (min(i, l), max(i, l))
+                        if key not in excl:
+                            p14.add(key)
+        for i, j in sorted(excl):
+            s = 0.5 * (self.particles[i][1] + self.particles[j][1])
+            self.add_exception(i, j, 0.0, s, 0.0)
+        for i, j in sorted(p14 - excl):
+            qi, si, ei = self.particles[i]
+            qj, sj, ej = self.particles[j]
+            self.add_exception(i, j, coulomb14scale * qi * qj, 0.5 * (si + sj), lj14scale * math.sqrt(ei * ej))
+
+    def desc(self):
+        """Build an ``NbDesc`` (keeps the numpy arrays alive on the returned object)."""
+        p = np.array(self.particles, dtype=np.float64).reshape(-1, 3)
+        e = self.exceptions
+        d = NbDesc()
+        keep = {}
+        keep["q"] = _f64(p[:, 0])
+        keep["s"] = _f64(p[:, 1])
+        keep["e"] = _f64(p[:, 2])
+        keep["p1"] = _i32([x[0] for x in e])
+        keep["p2"] = _i32([x[1] for x in e])
+        keep["eq"] = _f64([x[2] for x in e])
+        keep["es"] = _f64([x[3] for x in e])
+        keep["ee"] = _f64([x[4] for x in e])
+        d.num_particles = len(self.particles)
+        d.charge, d.sigma, d.epsilon = (_ptr(keep[k], C.c_double) for k in ("q", "s", "e"))
+        d.num_exceptions = len(e)
+        d.exception_particle1 = _ptr(keep["p1"], C.c_int32)
+        d.exception_particle2 = _ptr(keep["p2"], C.c_int32)
+        d.exception_charge_prod = _ptr(keep["eq"], C.c_double)
+        d.exception_sigma = _ptr(keep["es"], C.c_double)
+        d.exception_epsilon = _ptr(keep["ee"], C.c_double)
+        d.method = self.method
+        d.cutoff = self.cutoff
+        d.reaction_field_dielectric = self.reaction_field_dielectric
+        d.ewald_error_tolerance = self.ewald_error_tolerance
+        d.pme_alpha = self.pme[0]
+        for a in range(3):
+            d.pme_grid[a] = self.pme[1][a]
+        d.use_dispersion_correction = int(self.use_dispersion_correction)
+        d.exceptions_use_pbc = int(self.exceptions_use_pbc)
+        d.use_switching_function = int(self.use_switching_function)
+        d.switching_distance = self.switching_distance
+        d.force_group = self.force_group
+        d.reciprocal_force_group = self.reciprocal_force_group
+        d.include_direct_space = int(self.include_direct_space)
+        d._keep = keep
+        return d
+
+
+def _box_ptr(box):
+    if box is None:
+        return None, None
+    b = _f64(box).reshape(9)
+    return b, b.ctypes.data_as(_dp)
+
+
+class GpuNonbondedForce:
+    """The device operator: ``impl Force`` for one NonbondedForce (nonbonded.rs:1053-1155)."""
+
+    def __init__(self, force: NonbondedForce, precision: int = PRECISION_MIXED):
+        lib = library()
+        self._desc = force.desc()
+        self._n = force.num_particles()
+        h = C.c_void_p()
+        _check(lib.omm_nb_create(C.byref(self._desc), precision, C.byref(h)))
+        self._h = h
+
+    def initialize(self, box=None):
+        b, bp = _box_ptr(box)
+        _check(library().omm_nb_initialize(self._h, self._n, bp))
+
+    def pme_parameters(self):
+        alpha = C.c_double()
+        grid = (C.c_int32 * 3)()
+        _check(library().omm_nb_get_pme_parameters(self._h, C.byref(alpha), grid))
+        return alpha.value, tuple(grid)
+
+    def calc_forces_and_energy(self, pos, box=None):
+        p = _f64(pos).reshape(-1, 3)
+        b, bp = _box_ptr(box)
+        f = np.zeros_like(p)
+        e = C.c_double()
+        _check(library().omm_nb_calc_forces_and_energy(self._h, p.ctypes.data_as(_dp), bp,
+                                                       f.ctypes.data_as(_dp), C.byref(e)))
+        return f, e.value
+
+    def calc_forces_and_energy_groups(self, pos, box, groups):
+        p = _f64(pos).reshape(-1, 3)
+        b, bp = _box_ptr(box)
+        f = np.zeros_like(p)
+        e = C.c_double()
+        ev = C.c_int32()
+        _check(library().omm_nb_calc_forces_and_energy_groups(
+            self._h, p.ctypes.data_as(_dp), bp, C.c_uint32(groups), f.ctypes.data_as(_dp),
+            C.byref(e), C.byref(ev)))
+        return (f, e.value) if ev.value else None
+
+    def energy_parts(self, pos, box=None):
+        p = _f64(pos).reshape(-1, 3)
+        b, bp = _box_ptr(box)
+        parts = np.zeros(5)
+        _check(library().omm_nb_energy_parts(self._h, p.ctypes.data_as(_dp), bp, parts.ctypes.data_as(_dp)))
+        return parts
+
+    def pair_stats(self, pos, box=None):
+        p = _f64(pos).reshape(-1, 3)
+        b, bp = _box_ptr(box)
+        n = C.c_int64()
+        h = C.c_uint64()
+        _check(library().omm_nb_pair_stats(self._h, p.ctypes.data_as(_dp), bp, C.byref(n), C.byref(h)))
+        return n.value, h.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().omm_nb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class WaterSystem:
+    """A flexible-water system: NonbondedForce (PME) + bonds + angles + masses."""
+
+    nonbonded: NonbondedForce
+    positions: np.ndarray
+    box: np.ndarray
+    masses: np.ndarray
+    bonds: np.ndarray      # (nb, 2) int32
+    bond_params: np.ndarray  # (nb, 2) length, k
+    angles: np.ndarray     # (na, 3) int32
+    angle_params: np.ndarray  # (na, 2) theta0, k
+
+
+def water_system(num_molecules: int, box_edge: float, seed: int, cutoff: float = 0.9,
+                 tolerance: float = 5e-4, method: str = "PME") -> WaterSystem:
+    """Flexible 3-site water box with the workloads' force field."""
+    w = WATER
+    pos = water_box(num_molecules, box_edge, seed)
+    nb = NonbondedForce()
+    nb.set_nonbonded_method(method)
+    nb.set_cutoff_distance(cutoff)
+    nb.set_ewald_error_tolerance(tolerance)
+    for _ in range(num_molecules):
+        nb.add_particle(w["q_o"], w["sigma_o"], w["eps_o"])
+        nb.add_particle(w["q_h"], w["sigma_h"], w["eps_h"])
+        nb.add_particle(w["q_h"], w["sigma_h"], w["eps_h"])
+    so, sh = w["sigma_o"], w["sigma_h"]
+    for m in range(num_molecules):
+        o = 3 * m
+        # create_exceptions_from_bonds for O-H1, O-H2 bonds: two 1-2 and one 1-3 exclusion.
+        nb.add_exception(o, o + 1, 0.0, 0.5 * (so + sh), 0.0)
+        nb.add_exception(o, o + 2, 0.0, 0.5 * (so + sh), 0.0)
+        nb.add_exception(o + 1, o + 2, 0.0, 0.5 * (sh + sh), 0.0)
+    masses = np.tile([w["mass_o"], w["mass_h"], w["mass_h"]], num_molecules).astype(np.float64)
+    o = 3 * np.arange(num_molecules, dtype=np.int32)
+    bonds = np.stack([np.stack([o, o + 1], 1), np.stack([o, o + 2], 1)], 1).reshape(-1, 2).astype(np.int32)
+    bond_params = np.tile([w["r_oh"], w["k_bond"]], (len(bonds), 1))
+    angles = np.stack([o + 1, o, o + 2], 1).astype(np.int32)
+    angle_params = np.tile([w["theta0"], w["k_angle"]], (num_molecules, 1))
+    return WaterSystem(nb, pos, cubic_box(box_edge), masses, bonds, bond_params, angles, angle_params)
+
+
+class Context:
+    """Device-resident MD context: LangevinMiddle + bonds/angles + NonbondedForce.
+
+    Stands in for ``Context`` with a ``LangevinMiddleIntegrator`` (context.rs:795-928).
+    """
+
+    def __init__(self, system: WaterSystem, temperature=300.0, friction=1.0, step_size=0.001,
+                 seed=1, precision=PRECISION_MIXED):
+        lib = library()
+        s = system
+        self._nb = s.nonbonded.desc()
+        self.n = len(s.masses)
+        keep = dict(
+            m=_f64(s.masses), b1=_i32(s.bonds[:, 0]), b2=_i32(s.bonds[:, 1]),
+            bl=_f64(s.bond_params[:, 0]), bk=_f64(s.bond_params[:, 1]),
+            a1=_i32(s.angles[:, 0]), a2=_i32(s.angles[:, 1]), a3=_i32(s.angles[:, 2]),
+            at=_f64(s.angle_params[:, 0]), ak=_f64(s.angle_params[:, 1]),
+        )
+        d = MdDesc()
+        d.num_particles = self.n
+        d.masses = _ptr(keep["m"], C.c_double)
+        d.nonbonded = C.pointer(self._nb)
+        d.num_bonds = len(s.bonds)
+        d.bond_particle1, d.bond_particle2 = _ptr(keep["b1"], C.c_int32), _ptr(keep["b2"], C.c_int32)
+        d.bond_length, d.bond_k = _ptr(keep["bl"], C.c_double), _ptr(keep["bk"], C.c_double)
+        d.num_angles = len(s.angles)
+        d.angle_particle1 = _ptr(keep["a1"], C.c_int32)
+        d.angle_particle2 = _ptr(keep["a2"], C.c_int32)
+        d.angle_particle3 = _ptr(keep["a3"], C.c_int32)
+        d.angle_theta0, d.angle_k = _ptr(keep["at"], C.c_double), _ptr(keep["ak"], C.c_double)
+        for i, v in enumerate(_f64(s.box).reshape(9)):
+            d.box[i] = v
+        d.temperature, d.friction, d.step_size, d.seed = temperature, friction, step_size, seed
+        self._keep, self._d = keep, d
+        h = C.c_void_p()
+        _check(lib.omm_ctx_create(C.byref(d), precision, C.byref(h)))
+        self._h = h
+        self.set_positions(s.positions)
+
+    def set_positions(self, pos):
+        p = _f64(pos).reshape(-1, 3)
+        _check(library().omm_ctx_set_positions(self._h, p.ctypes.data_as(_dp)))
+
+    def set_velocities(self, vel):
+        v = _f64(vel).reshape(-1, 3)
+        _check(library().omm_ctx_set_velocities(self._h, v.ctypes.data_as(_dp)))
+
+    def set_velocities_to_temperature(self, temperature, seed):
+        _check(library().omm_ctx_set_velocities_to_temperature(self._h, temperature, seed))
+
+    def step(self, steps):
+        _check(library().omm_ctx_step(self._h, int(steps)))
+
+    def get_state(self):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        frc = np.zeros((self.n, 3))
+        e = C.c_double()
+        k = C.c_int64()
+        _check(library().omm_ctx_get_state(self._h, pos.ctypes.data_as(_dp), vel.ctypes.data_as(_dp),
+                                           frc.ctypes.data_as(_dp), C.byref(e), C.byref(k)))
+        return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value)
+
+    def kinetic_energy(self):
+        e = C.c_double()
+        _check(library().omm_ctx_kinetic_energy(self._h, C.byref(e)))
+        return e.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().omm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

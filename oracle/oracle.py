@@ -1,0 +1,276 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker, never as the product path.
+The functions restate crates/openmm-core/src/forces/{nonbonded,pme,neighbors,
+ewald,harmonic_bond,harmonic_angle}.rs and integrators/langevin.rs; see
+oracle.h for the per-function citations and the unpinned last-ulp notes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_nb_evaluate.restype = C.c_int
+        L.orc_nb_evaluate.argtypes = [C.c_void_p, C.c_double, _ip, _dp, _dp, C.c_int, C.c_int, C.c_int,
+                                      _dp, _dp, _dp]
+        L.orc_nb_pairs.restype = C.c_int64
+        L.orc_nb_pairs.argtypes = [C.c_void_p, _dp, _dp, _ip, _ip, C.c_int64, C.POINTER(C.c_uint64)]
+        L.orc_cell_candidates.restype = C.c_int64
+        L.orc_cell_candidates.argtypes = [_dp, C.c_int32, C.c_double, _dp, _ip, _ip, C.c_int64]
+        L.orc_pair_key.restype = C.c_uint64
+        L.orc_pair_key.argtypes = [C.c_int32, C.c_int32]
+        L.orc_effective_pme_parameters.argtypes = [C.c_double, C.c_double, _dp, _dp, _ip]
+        L.orc_effective_ljpme_parameters.argtypes = [C.c_double, C.c_double, _dp, _dp, _ip]
+        L.orc_ewald_parameters.argtypes = [_dp, C.c_double, C.c_double, _dp, _ip]
+        L.orc_spline.argtypes = [C.c_double, _dp, _dp]
+        L.orc_reciprocal_box.restype = C.c_double
+        L.orc_reciprocal_box.argtypes = [_dp, _dp]
+        L.orc_fft.argtypes = [_dp, C.c_int32, C.c_int32]
+        L.orc_bspline_moduli.argtypes = [C.c_int32, _dp]
+        L.orc_pme_mesh.restype = C.c_double
+        L.orc_pme_mesh.argtypes = [_dp, _dp, C.c_int32, _dp, C.c_double, _ip, C.c_int32, _dp]
+        L.orc_pme_charge_derivatives.argtypes = [_dp, _dp, C.c_int32, _dp, C.c_double, _ip, _dp]
+        L.orc_ewald_reciprocal.restype = C.c_double
+        L.orc_ewald_reciprocal.argtypes = [_dp, _dp, C.c_int32, _dp, C.c_double, _ip, _dp]
+        L.orc_dispersion_correction.restype = C.c_double
+        L.orc_dispersion_correction.argtypes = [C.c_void_p, C.c_double]
+        L.orc_minimum_image.argtypes = [_dp, _dp]
+        L.orc_harmonic_bonds.restype = C.c_double
+        L.orc_harmonic_bonds.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int32, _dp]
+        L.orc_harmonic_angles.restype = C.c_double
+        L.orc_harmonic_angles.argtypes = [C.c_int32, _ip, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int32, _dp]
+        L.orc_lfmiddle_step.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                        C.c_double, _dp]
+        L.orc_boltzmann.restype = C.c_double
+        L.orc_molecules.restype = C.c_int32
+        L.orc_molecules.argtypes = [C.c_int32, C.c_int32, _ip, _ip, _ip]
+        L.orc_scale_positions.argtypes = [C.c_int32, _dp, C.c_double, C.c_int32, _ip, C.c_int32]
+        L.orc_barostat_work.restype = C.c_double
+        L.orc_barostat_work.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                        C.c_double]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def nb_evaluate(desc, pos, box, include_direct=True, include_reciprocal=True, use_cells=True,
+                ljpme=None):
+    """NonbondedForce::evaluate -> (forces (N,3), energy, parts[5])."""
+    pos = _f(pos).reshape(-1, 3)
+    b = None if box is None else _f(box).reshape(9)
+    forces = np.zeros_like(pos)
+    energy = C.c_double()
+    parts = np.zeros(5)
+    la, lg = (0.0, None) if ljpme is None else (ljpme[0], _i(ljpme[1]))
+    rc = lib().orc_nb_evaluate(C.byref(desc), la, _p(lg, C.c_int32), _p(pos), _p(b), int(include_direct),
+                               int(include_reciprocal), int(use_cells), _p(forces), C.byref(energy), _p(parts))
+    if rc != 0:
+        raise AssertionError(f"oracle evaluate failed a reference assertion (code {rc})")
+    return forces, energy.value, parts
+
+
+def nb_pairs(desc, pos, box, with_list=False):
+    pos = _f(pos).reshape(-1, 3)
+    b = None if box is None else _f(box).reshape(9)
+    h = C.c_uint64()
+    count = lib().orc_nb_pairs(C.byref(desc), _p(pos), _p(b), None, None, 0, C.byref(h))
+    if not with_list:
+        return count, h.value
+    oi = np.zeros(count, np.int32)
+    oj = np.zeros(count, np.int32)
+    lib().orc_nb_pairs(C.byref(desc), _p(pos), _p(b), _p(oi, C.c_int32), _p(oj, C.c_int32), count,
+                       C.byref(h))
+    return count, h.value, oi, oj
+
+
+def cell_candidates(pos, cutoff, box):
+    pos = _f(pos).reshape(-1, 3)
+    b = None if box is None else _f(box).reshape(9)
+    n = lib().orc_cell_candidates(_p(pos), len(pos), cutoff, _p(b), None, None, 0)
+    oi = np.zeros(n, np.int32)
+    oj = np.zeros(n, np.int32)
+    lib().orc_cell_candidates(_p(pos), len(pos), cutoff, _p(b), _p(oi, C.c_int32), _p(oj, C.c_int32), n)
+    return oi, oj
+
+
+def pair_key(i, j):
+    return int(lib().orc_pair_key(i, j))
+
+
+def effective_pme_parameters(tol, cutoff, box):
+    a = C.c_double()
+    g = np.zeros(3, np.int32)
+    lib().orc_effective_pme_parameters(tol, cutoff, _p(_f(box).reshape(9)), C.byref(a), _p(g, C.c_int32))
+    return a.value, tuple(int(x) for x in g)
+
+
+def effective_ljpme_parameters(tol, cutoff, box):
+    a = C.c_double()
+    g = np.zeros(3, np.int32)
+    lib().orc_effective_ljpme_parameters(tol, cutoff, _p(_f(box).reshape(9)), C.byref(a), _p(g, C.c_int32))
+    return a.value, tuple(int(x) for x in g)
+
+
+def ewald_parameters(widths, cutoff, tol):
+    a = C.c_double()
+    g = np.zeros(3, np.int32)
+    lib().orc_ewald_parameters(_p(_f(widths)), cutoff, tol, C.byref(a), _p(g, C.c_int32))
+    return a.value, tuple(int(x) for x in g)
+
+
+def spline(fraction):
+    w = np.zeros(5)
+    d = np.zeros(5)
+    lib().orc_spline(fraction, _p(w), _p(d))
+    return w, d
+
+
+def reciprocal_box(box):
+    r = np.zeros(9)
+    v = lib().orc_reciprocal_box(_p(_f(box).reshape(9)), _p(r))
+    return r.reshape(3, 3), v
+
+
+def fft(x, inverse=False):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128).copy())
+    lib().orc_fft(a.view(np.float64).ctypes.data_as(_dp), len(a), int(inverse))
+    return a
+
+
+def bspline_moduli(n):
+    out = np.zeros(n)
+    lib().orc_bspline_moduli(n, _p(out))
+    return out
+
+
+def pme_mesh(pos, amplitudes, box, alpha, grid, dispersion=False, with_forces=True):
+    pos = _f(pos).reshape(-1, 3)
+    f = np.zeros_like(pos) if with_forces else None
+    g = _i(grid)
+    e = lib().orc_pme_mesh(_p(pos), _p(_f(amplitudes)), len(pos), _p(_f(box).reshape(9)), alpha,
+                           _p(g, C.c_int32), int(dispersion), _p(f))
+    return (f, e) if with_forces else e
+
+
+def pme_charge_derivatives(pos, charges, box, alpha, grid):
+    pos = _f(pos).reshape(-1, 3)
+    out = np.zeros(len(pos))
+    lib().orc_pme_charge_derivatives(_p(pos), _p(_f(charges)), len(pos), _p(_f(box).reshape(9)), alpha,
+                                     _p(_i(grid), C.c_int32), _p(out))
+    return out
+
+
+def ewald_reciprocal(pos, charges, widths, alpha, limits):
+    pos = _f(pos).reshape(-1, 3)
+    f = np.zeros_like(pos)
+    e = lib().orc_ewald_reciprocal(_p(pos), _p(_f(charges)), len(pos), _p(_f(widths)), alpha,
+                                   _p(_i(limits), C.c_int32), _p(f))
+    return f, e
+
+
+def dispersion_correction(desc, volume):
+    return lib().orc_dispersion_correction(C.byref(desc), volume)
+
+
+def minimum_image(delta, box):
+    d = _f(delta).copy()
+    lib().orc_minimum_image(_p(d), _p(_f(box).reshape(9)))
+    return d
+
+
+def harmonic_bonds(bonds, params, pos, box=None, use_pbc=False):
+    pos = _f(pos).reshape(-1, 3)
+    f = np.zeros_like(pos)
+    b = _i(bonds)
+    p = _f(params)
+    bx = None if box is None else _f(box).reshape(9)
+    e = lib().orc_harmonic_bonds(len(b), _p(_i(b[:, 0]), C.c_int32), _p(_i(b[:, 1]), C.c_int32),
+                                 _p(_f(p[:, 0])), _p(_f(p[:, 1])), _p(pos), _p(bx), int(use_pbc), _p(f))
+    return f, e
+
+
+def harmonic_angles(angles, params, pos, box=None, use_pbc=False):
+    pos = _f(pos).reshape(-1, 3)
+    f = np.zeros_like(pos)
+    a = _i(angles)
+    p = _f(params)
+    bx = None if box is None else _f(box).reshape(9)
+    e = lib().orc_harmonic_angles(len(a), _p(_i(a[:, 0]), C.c_int32), _p(_i(a[:, 1]), C.c_int32),
+                                  _p(_i(a[:, 2]), C.c_int32), _p(_f(p[:, 0])), _p(_f(p[:, 1])), _p(pos),
+                                  _p(bx), int(use_pbc), _p(f))
+    return f, e
+
+
+def lfmiddle_step(pos, vel, forces, masses, dt, temperature, friction, normals=None):
+    pos = _f(pos).copy()
+    vel = _f(vel).copy()
+    lib().orc_lfmiddle_step(len(masses), _p(pos), _p(vel), _p(_f(forces)), _p(_f(masses)), dt, temperature,
+                            friction, _p(None if normals is None else _f(normals)))
+    return pos, vel
+
+
+def boltzmann():
+    return lib().orc_boltzmann()
+
+
+def molecules(n, bonds):
+    b = _i(bonds).reshape(-1, 2)
+    out = np.zeros(n, np.int32)
+    k = lib().orc_molecules(n, len(b), _p(_i(b[:, 0]), C.c_int32), _p(_i(b[:, 1]), C.c_int32),
+                            _p(out, C.c_int32))
+    return out, k
+
+
+def scale_positions(pos, scale, rigid, molecule_of, nmol):
+    p = _f(pos).copy()
+    lib().orc_scale_positions(len(p), _p(p), scale, int(rigid), _p(_i(molecule_of), C.c_int32), nmol)
+    return p
+
+
+def barostat_work(de, pressure, v_old, v_new, scaled, temperature):
+    return lib().orc_barostat_work(de, pressure, v_old, v_new, scaled, temperature)
+
+
+def water_energy_forces(system, ljpme=None, include_bonded=True):
+    """Total FP64 forces/energy of a WaterSystem (nonbonded with its frozen PME
+    parameters + harmonic bonds + angles): the reference Context's force sum."""
+    desc = system.nonbonded.desc()
+    f, e, parts = nb_evaluate(desc, system.positions, system.box)
+    if include_bonded:
+        fb, eb = harmonic_bonds(system.bonds, system.bond_params, system.positions)
+        fa, ea = harmonic_angles(system.angles, system.angle_params, system.positions)
+        f = f + fb + fa
+        e = e + eb + ea
+    return f, e
