@@ -134,7 +134,10 @@ omm_status omm_nb_get_pme_parameters(const omm_nb* nb, double* alpha, int32_t gr
 
 /* Force::calc_forces_and_energy (nonbonded.rs:1125-1131): direct space if
  * include_direct_space, reciprocal always.  pos: host N x 3; box: host 9 or
- * NULL; forces_out: host N x 3 (overwritten); energy_out: kJ/mol. */
+ * NULL; forces_out: host N x 3 (overwritten, may be NULL); energy_out: kJ/mol.
+ * With energy_out == NULL only forces are evaluated, on the MD step's path
+ * (MIXED: FP32 pair and mesh arithmetic); with an energy requested, MIXED
+ * evaluates each pair energy and the whole mesh in FP64 as well. */
 omm_status omm_nb_calc_forces_and_energy(omm_nb* nb, const double* pos, const double* box,
                                          double* forces_out, double* energy_out);
 
@@ -227,6 +230,16 @@ omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* for
 
 /* Kinetic energy (context.rs:1107-1163), kJ/mol, of the current velocities. */
 omm_status omm_ctx_kinetic_energy(omm_ctx* ctx, double* kinetic_energy);
+
+/* Measurement hook (no counterpart in the reference, which publishes no
+ * per-phase timing): average device time in ms of each phase of the step over
+ * `reps` launches, timed with HIP events on the context's stream:
+ *   [0] neighbour-list rebuild, [1] direct-space pairs, [2] exceptions,
+ *   [3] PME spread, [4] PME FFT passes + convolution, [5] PME gather,
+ *   [6] bonds + angles, [7] LangevinMiddle update.
+ * *num_pairs (nullable) receives the in-cutoff pair count of the current
+ * positions.  Positions and velocities are left unchanged. */
+omm_status omm_ctx_profile(omm_ctx* ctx, int32_t reps, double phase_ms[8], int64_t* num_pairs);
 
 omm_status omm_ctx_destroy(omm_ctx* ctx);
 

@@ -1,0 +1,211 @@
+// extern "C" entry points of include/ommhip.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+
+#include "../../include/ommhip.h"
+#include "md.h"
+
+namespace ommhip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& message) { g_last_error = message; }
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+}  // namespace ommhip
+
+using ommhip::Error;
+using ommhip::guarded;
+
+struct omm_nb {
+    ommhip::NbOperator op;
+    double parts[5] = {0, 0, 0, 0, 0};
+    omm_nb(const omm_nb_desc* d, int precision) : op(d, precision) {}
+};
+
+struct omm_ctx {
+    ommhip::MdContext ctx;
+    omm_ctx(const omm_md_desc* d, int precision) : ctx(d, precision) {}
+};
+
+extern "C" {
+
+const char* omm_last_error(void) { return ommhip::last_error_cstr(); }
+
+int32_t omm_abi_version(void) { return OMMHIP_ABI_VERSION; }
+
+int32_t omm_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    int usable = 0;
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+            ++usable;
+    }
+    return usable;
+}
+
+uint64_t omm_pair_key(int32_t i, int32_t j) { return ommhip::pair_key(i, j); }
+
+void omm_nb_desc_defaults(omm_nb_desc* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->method = OMM_NB_NO_CUTOFF;
+    d->cutoff = 1.0;
+    d->reaction_field_dielectric = 78.3;
+    d->ewald_error_tolerance = 5e-4;
+    d->pme_alpha = 0.0;
+    d->use_dispersion_correction = 1;
+    d->exceptions_use_pbc = 0;
+    d->use_switching_function = 0;
+    d->switching_distance = -1.0;
+    d->force_group = 0;
+    d->reciprocal_force_group = -1;
+    d->include_direct_space = 1;
+}
+
+omm_status omm_nb_create(const omm_nb_desc* desc, int32_t precision, omm_nb** out) {
+    return guarded([&]() -> omm_status {
+        if (!out) throw Error(OMM_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        *out = new omm_nb(desc, precision);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_initialize(omm_nb* nb, int32_t num_particles, const double* box) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        nb->op.initialize(num_particles, box);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_get_pme_parameters(const omm_nb* nb, double* alpha, int32_t grid[3]) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        if (!nb->op.initialized()) throw Error(OMM_ERR_STATE, "NonbondedForce not initialized");
+        if (alpha) *alpha = nb->op.alpha();
+        if (grid)
+            for (int a = 0; a < 3; ++a) grid[a] = nb->op.grid()[a];
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_calc_forces_and_energy(omm_nb* nb, const double* pos, const double* box, double* forces_out,
+                                         double* energy_out) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        nb->op.compute(pos, box, nb->op.settings().include_direct, true, forces_out, energy_out, nb->parts, nullptr,
+                       nullptr);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_calc_forces_and_energy_groups(omm_nb* nb, const double* pos, const double* box, uint32_t groups,
+                                                double* forces_out, double* energy_out, int32_t* evaluated) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        const ommhip::NbSettings& c = nb->op.settings();
+        // nonbonded.rs:1139-1149
+        bool direct = c.include_direct && (groups & (1u << c.force_group)) != 0;
+        uint32_t rg = c.reciprocal_group < 0 ? (uint32_t)c.force_group : (uint32_t)c.reciprocal_group;
+        bool recip = (c.method == OMM_NB_EWALD || c.method == OMM_NB_PME || c.method == OMM_NB_LJPME) &&
+                     (groups & (1u << rg)) != 0;
+        if (evaluated) *evaluated = (direct || recip) ? 1 : 0;
+        if (!(direct || recip)) return OMM_OK;
+        nb->op.compute(pos, box, direct, recip, forces_out, energy_out, nb->parts, nullptr, nullptr);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_energy_parts(omm_nb* nb, const double* pos, const double* box, double parts[5]) {
+    return guarded([&]() -> omm_status {
+        if (!nb || !parts) throw Error(OMM_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        nb->op.compute(pos, box, true, true, nullptr, nullptr, parts, nullptr, nullptr);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_pair_stats(omm_nb* nb, const double* pos, const double* box, int64_t* num_pairs,
+                             uint64_t* pair_hash) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        int64_t np = 0;
+        uint64_t h = 0;
+        nb->op.compute(pos, box, true, false, nullptr, nullptr, nullptr, &np, &h);
+        if (num_pairs) *num_pairs = np;
+        if (pair_hash) *pair_hash = h;
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_destroy(omm_nb* nb) {
+    return guarded([&]() -> omm_status {
+        delete nb;
+        return OMM_OK;
+    });
+}
+
+omm_status omm_ctx_create(const omm_md_desc* desc, int32_t precision, omm_ctx** out) {
+    return guarded([&]() -> omm_status {
+        if (!out) throw Error(OMM_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        *out = new omm_ctx(desc, precision);
+        return OMM_OK;
+    });
+}
+
+#define CTX_CALL(body)                                                      \
+    return guarded([&]() -> omm_status {                                    \
+        if (!ctx) throw Error(OMM_ERR_INVALID, "null context");             \
+        std::lock_guard<std::mutex> lock(ctx->ctx.mutex);                   \
+        body;                                                               \
+        return OMM_OK;                                                      \
+    })
+
+omm_status omm_ctx_set_positions(omm_ctx* ctx, const double* pos) {
+    CTX_CALL(if (!pos) throw Error(OMM_ERR_INVALID, "null positions"); ctx->ctx.set_positions(pos));
+}
+
+omm_status omm_ctx_set_velocities(omm_ctx* ctx, const double* vel) {
+    CTX_CALL(if (!vel) throw Error(OMM_ERR_INVALID, "null velocities"); ctx->ctx.set_velocities(vel));
+}
+
+omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperature, uint64_t seed) {
+    CTX_CALL(ctx->ctx.set_velocities_to_temperature(temperature, seed));
+}
+
+omm_status omm_ctx_step(omm_ctx* ctx, int32_t nsteps) { CTX_CALL(ctx->ctx.step(nsteps)); }
+
+omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* forces, double* potential_energy,
+                             int64_t* step_count) {
+    CTX_CALL(ctx->ctx.get_state(pos, vel, forces, potential_energy, step_count));
+}
+
+omm_status omm_ctx_kinetic_energy(omm_ctx* ctx, double* kinetic_energy) {
+    CTX_CALL(if (!kinetic_energy) throw Error(OMM_ERR_INVALID, "null output"); *kinetic_energy = ctx->ctx.kinetic_energy());
+}
+
+omm_status omm_ctx_profile(omm_ctx* ctx, int32_t reps, double phase_ms[8], int64_t* num_pairs) {
+    CTX_CALL(if (!phase_ms) throw Error(OMM_ERR_INVALID, "null output"); ctx->ctx.profile(reps, phase_ms, num_pairs));
+}
+
+omm_status omm_ctx_destroy(omm_ctx* ctx) {
+    return guarded([&]() -> omm_status {
+        delete ctx;
+        return OMM_OK;
+    });
+}
+
+}  // extern "C"

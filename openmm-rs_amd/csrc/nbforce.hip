@@ -1,0 +1,528 @@
+// NonbondedForce operator: validation, parameter freezing and the evaluation
+// sequence (nonbonded.rs:771-1155) on one device.
+#include <cmath>
+#include <cstring>
+#include <set>
+
+#include "nbforce.h"
+
+namespace ommhip {
+
+namespace {
+
+__global__ void k_fixed_to_double(const long long* __restrict__ fx, double* __restrict__ out, size_t n3) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n3) out[i] = (double)fx[i] * (1.0 / FORCE_SCALE);
+}
+
+void validate_particle(double q, double s, double e) {
+    // nonbonded.rs:1252-1261
+    if (!(std::isfinite(q) && std::isfinite(s) && std::isfinite(e) && s >= 0.0 && e >= 0.0))
+        throw Error(OMM_ERR_INVALID, "nonbonded parameters must be finite with nonnegative sigma and epsilon");
+}
+
+// system.rs:16-39
+void validate_box(const double* v) {
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(v[i])) throw Error(OMM_ERR_NONFINITE, "periodic box vectors must be finite");
+    if (!(v[1] == 0.0 && v[2] == 0.0)) throw Error(OMM_ERR_INVALID, "first periodic box vector must be parallel to x");
+    if (!(v[5] == 0.0)) throw Error(OMM_ERR_INVALID, "second periodic box vector must be in the x-y plane");
+    if (!(v[0] > 0.0 && v[4] > 0.0 && v[8] > 0.0 && v[0] >= 2.0 * std::fabs(v[3]) && v[0] >= 2.0 * std::fabs(v[6]) &&
+          v[4] >= 2.0 * std::fabs(v[7])))
+        throw Error(OMM_ERR_INVALID, "periodic box vectors must be in reduced form");
+}
+
+// nonbonded.rs:1290-1336 — adaptive Simpson for the switched tail integral.
+template <class F>
+double simpson(const F& f, double a, double b, double fa, double fm, double fb, double est, double tol, int depth) {
+    double m = 0.5 * (a + b);
+    double fl = f(0.5 * (a + m)), fr = f(0.5 * (m + b));
+    double left = (m - a) / 6.0 * (fa + 4.0 * fl + fm);
+    double right = (b - m) / 6.0 * (fm + 4.0 * fr + fb);
+    double diff = left + right - est;
+    if (depth == 0 || std::fabs(diff) <= 15.0 * tol) return left + right + diff / 15.0;
+    return simpson(f, a, m, fa, fl, fm, left, 0.5 * tol, depth - 1) +
+           simpson(f, m, b, fm, fr, fb, right, 0.5 * tol, depth - 1);
+}
+
+}  // namespace
+
+Box make_box(const double* v) {
+    Box b{};
+    if (!v) {
+        b.periodic = 0;
+        return b;
+    }
+    for (int i = 0; i < 9; ++i) b.v[i] = v[i];
+    const double* a = v;
+    const double* bb = v + 3;
+    const double* c = v + 6;
+    double bc[3] = {bb[1] * c[2] - bb[2] * c[1], bb[2] * c[0] - bb[0] * c[2], bb[0] * c[1] - bb[1] * c[0]};
+    double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    double ab[3] = {a[1] * bb[2] - a[2] * bb[1], a[2] * bb[0] - a[0] * bb[2], a[0] * bb[1] - a[1] * bb[0]};
+    double vol = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    if (!(std::isfinite(vol) && vol > 0.0)) throw Error(OMM_ERR_INVALID, "periodic box volume must be positive and finite");
+    for (int i = 0; i < 3; ++i) {
+        b.recip[i] = bc[i] / vol;
+        b.recip[3 + i] = ca[i] / vol;
+        b.recip[6 + i] = ab[i] / vol;
+    }
+    b.volume = vol;
+    b.periodic = 1;
+    b.triclinic = (v[3] != 0.0 || v[6] != 0.0 || v[7] != 0.0) ? 1 : 0;
+    return b;
+}
+
+void launch_fixed_to_double(const long long* fx, double* out, size_t n3, hipStream_t s) {
+    if (!n3) return;
+    hipLaunchKernelGGL(k_fixed_to_double, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, fx, out, n3);
+    OMM_HIP(hipGetLastError());
+}
+
+NbSettings settings_from_desc(const omm_nb_desc* d) {
+    if (!d) throw Error(OMM_ERR_INVALID, "null NonbondedForce descriptor");
+    NbSettings c;
+    if (d->num_particles < 0 || d->num_exceptions < 0) throw Error(OMM_ERR_INVALID, "negative particle/exception count");
+    if (d->num_particles > 0 && (!d->charge || !d->sigma || !d->epsilon))
+        throw Error(OMM_ERR_INVALID, "particle arrays must not be null");
+    if (d->num_exceptions > 0 && (!d->exception_particle1 || !d->exception_particle2 || !d->exception_charge_prod ||
+                                  !d->exception_sigma || !d->exception_epsilon))
+        throw Error(OMM_ERR_INVALID, "exception arrays must not be null");
+    for (int i = 0; i < d->num_particles; ++i) {
+        validate_particle(d->charge[i], d->sigma[i], d->epsilon[i]);
+        c.charge.push_back(d->charge[i]);
+        c.sigma.push_back(d->sigma[i]);
+        c.epsilon.push_back(d->epsilon[i]);
+    }
+    for (int i = 0; i < d->num_exceptions; ++i) {
+        validate_particle(d->exception_charge_prod[i], d->exception_sigma[i], d->exception_epsilon[i]);
+        c.exc1.push_back(d->exception_particle1[i]);
+        c.exc2.push_back(d->exception_particle2[i]);
+        c.exc_qq.push_back(d->exception_charge_prod[i]);
+        c.exc_sig.push_back(d->exception_sigma[i]);
+        c.exc_eps.push_back(d->exception_epsilon[i]);
+    }
+    if (d->method < OMM_NB_NO_CUTOFF || d->method > OMM_NB_LJPME) throw Error(OMM_ERR_INVALID, "unknown nonbonded method");
+    c.method = d->method;
+    if (!(std::isfinite(d->cutoff) && d->cutoff > 0.0)) throw Error(OMM_ERR_INVALID, "cutoff must be positive and finite");
+    c.cutoff = d->cutoff;
+    if (std::isnan(d->reaction_field_dielectric) || d->reaction_field_dielectric < 1.0)
+        throw Error(OMM_ERR_INVALID, "reaction field dielectric must be at least one");
+    c.rf_dielectric = d->reaction_field_dielectric;
+    if (!(std::isfinite(d->ewald_error_tolerance) && d->ewald_error_tolerance > 0.0 && d->ewald_error_tolerance < 0.5))
+        throw Error(OMM_ERR_INVALID, "Ewald tolerance must be between zero and one half");
+    c.tolerance = d->ewald_error_tolerance;
+    if (!(std::isfinite(d->pme_alpha) && d->pme_alpha >= 0.0))
+        throw Error(OMM_ERR_INVALID, "PME alpha must be nonnegative and finite");
+    if (d->pme_alpha > 0.0)
+        for (int a = 0; a < 3; ++a)
+            if (d->pme_grid[a] < 5) throw Error(OMM_ERR_INVALID, "PME grid dimensions must be at least five");
+    c.pme_alpha = d->pme_alpha;
+    for (int a = 0; a < 3; ++a) c.pme_grid[a] = d->pme_grid[a];
+    c.dispersion_correction = d->use_dispersion_correction != 0;
+    c.exceptions_pbc = d->exceptions_use_pbc != 0;
+    c.switching = d->use_switching_function != 0;
+    c.switching_distance = d->switching_distance;
+    if (c.switching && !(std::isfinite(c.switching_distance) && c.switching_distance >= 0.0))
+        throw Error(OMM_ERR_INVALID, "switching distance must be nonnegative and finite");
+    if (d->force_group < 0 || d->force_group > 31) throw Error(OMM_ERR_INVALID, "force group must be between zero and 31");
+    if (d->reciprocal_force_group < -1 || d->reciprocal_force_group > 31)
+        throw Error(OMM_ERR_INVALID, "reciprocal force group must be between -1 and 31");
+    c.force_group = d->force_group;
+    c.reciprocal_group = d->reciprocal_force_group;
+    c.include_direct = d->include_direct_space != 0;
+    return c;
+}
+
+NbOperator::NbOperator(const omm_nb_desc* desc, int precision) {
+    cfg_ = settings_from_desc(desc);
+    if (precision != OMM_PRECISION_MIXED && precision != OMM_PRECISION_DOUBLE)
+        throw Error(OMM_ERR_INVALID, "precision must be OMM_PRECISION_MIXED or OMM_PRECISION_DOUBLE");
+    mixed_ = precision == OMM_PRECISION_MIXED;
+    n_ = (int)cfg_.charge.size();
+}
+
+NbOperator::~NbOperator() {
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void NbOperator::check_box(const double* box) const {
+    if (!cfg_.periodic()) return;
+    if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
+    validate_box(box);
+    // nonbonded.rs:806-812
+    for (int a = 0; a < 3; ++a)
+        if (!(2.0 * cfg_.cutoff <= box[a * 4] + 1e-12))
+            throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
+}
+
+void NbOperator::initialize(int n, const double* box) {
+    // nonbonded.rs:1055-1107
+    if (n != (int)cfg_.charge.size())
+        throw Error(OMM_ERR_INVALID, "NonbondedForce must have one parameter record per particle");
+    std::set<std::pair<int, int>> pairs;
+    for (size_t e = 0; e < cfg_.exc1.size(); ++e) {
+        int a = cfg_.exc1[e], b = cfg_.exc2[e];
+        if (a < 0 || b < 0 || a >= n || b >= n) throw Error(OMM_ERR_INVALID, "exception particle index out of range");
+        if (!pairs.insert({std::min(a, b), std::max(a, b)}).second) throw Error(OMM_ERR_INVALID, "duplicate exception pair");
+    }
+    if (cfg_.switching && !(cfg_.switching_distance >= 0.0 && cfg_.switching_distance < cfg_.cutoff))
+        throw Error(OMM_ERR_INVALID, "switching distance must be between zero and cutoff");
+    if (cfg_.method != OMM_NB_PME && cfg_.method != OMM_NB_CUTOFF_PERIODIC)
+        throw Error(OMM_ERR_INVALID,
+                    "the device operator implements the PME and CutoffPeriodic methods; "
+                    "NoCutoff, CutoffNonPeriodic, Ewald and LJPME are not available on the device yet");
+    if (cfg_.switching) throw Error(OMM_ERR_INVALID, "the switching function is not available on the device yet");
+    if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
+    validate_box(box);
+    for (int a = 0; a < 3; ++a)
+        if (!(cfg_.cutoff <= 0.5 * box[a * 4]))
+            throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
+    if (cfg_.method == OMM_NB_PME) {
+        if (cfg_.pme_alpha > 0.0) {
+            alpha_ = cfg_.pme_alpha;
+            for (int a = 0; a < 3; ++a) grid_[a] = cfg_.pme_grid[a];
+        } else {
+            // nonbonded.rs:647-658
+            alpha_ = std::sqrt(-std::log(2.0 * cfg_.tolerance)) / cfg_.cutoff;
+            for (int a = 0; a < 3; ++a) {
+                double v = std::ceil(2.0 * alpha_ * box[a * 4] / (3.0 * std::pow(cfg_.tolerance, 0.2)));
+                grid_[a] = (int)std::max(v, 6.0);
+            }
+        }
+    }
+    for (int i = 0; i < 9; ++i) box0_[i] = box[i];
+    sum_q2_ = 0.0;
+    sum_q_ = 0.0;
+    for (double q : cfg_.charge) sum_q2_ += q * q;
+    for (double q : cfg_.charge) sum_q_ += q;
+    n_ = n;
+    if (n == 0) {
+        initialized_ = true;
+        return;
+    }
+
+    int count = 0;
+    OMM_HIP(hipGetDeviceCount(&count));
+    if (count <= 0) throw Error(OMM_ERR_DEVICE, "no HIP device available");
+    if (!stream_) OMM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    hipStream_t s = stream_;
+    std::vector<float> qk(n), sh(n), e2(n);
+    std::vector<double> qkd(n), shd(n), e2d(n);
+    const double sk = std::sqrt(ONE_4PI_EPS0);
+    for (int i = 0; i < n; ++i) {
+        qkd[i] = cfg_.charge[i] * sk;
+        shd[i] = 0.5 * cfg_.sigma[i];
+        e2d[i] = 2.0 * std::sqrt(cfg_.epsilon[i]);
+        qk[i] = (float)qkd[i];
+        sh[i] = (float)shd[i];
+        e2[i] = (float)e2d[i];
+    }
+    part_.charge.upload(cfg_.charge, s);
+    if (mixed_) {
+        part_.qk.upload(qk, s);
+        part_.sh.upload(sh, s);
+        part_.e2.upload(e2, s);
+    }
+    part_.qk_d.upload(qkd, s);
+    part_.sh_d.upload(shd, s);
+    part_.e2_d.upload(e2d, s);
+    exc_.count = (int)cfg_.exc1.size();
+    exc_.p1.upload(cfg_.exc1, s);
+    exc_.p2.upload(cfg_.exc2, s);
+    exc_.qq.upload(cfg_.exc_qq, s);
+    exc_.sig.upload(cfg_.exc_sig, s);
+    exc_.eps.upload(cfg_.exc_eps, s);
+
+    Box b = make_box(box);
+    nlist_setup(nl_, n, cfg_.exc1, cfg_.exc2, cfg_.cutoff, 0.1, b, s);
+    if (cfg_.method == OMM_NB_PME) pme_setup(pme_, alpha_, grid_, mixed_, s);
+
+    pp_ = PairParams{};
+    pp_.n = n;
+    pp_.nblocks = nl_.nblocks;
+    pp_.cap = nl_.cap;
+    pp_.periodic = 1;
+    pp_.use_cutoff = 1;
+    pp_.cutoff = cfg_.cutoff;
+    pp_.alpha = alpha_;
+    if (cfg_.method == OMM_NB_PME) {
+        pp_.elec = ELEC_EWALD;
+    } else {
+        pp_.elec = ELEC_REACTION_FIELD;
+        double rc = cfg_.cutoff, eps = cfg_.rf_dielectric;
+        if (std::isinf(eps)) {
+            pp_.krf = 0.5 / (rc * rc * rc);
+            pp_.crf = 1.5 / rc;
+        } else {
+            pp_.krf = (eps - 1.0) / (2.0 * eps + 1.0) / (rc * rc * rc);
+            pp_.crf = (3.0 * eps) / (2.0 * eps + 1.0) / rc;
+        }
+    }
+    pair_grid_ = std::max(64, std::min(4096, nl_.nblocks * 4));
+
+    pos_dev_.alloc(3 * (size_t)n);
+    box_dev_.alloc(1);
+    if (mixed_) {
+        force_fx_.alloc(3 * (size_t)n);
+        energy_fx_.alloc(E_SLOTS);
+    } else {
+        force_d_.alloc(3 * (size_t)n);
+        energy_d_.alloc(E_SLOTS);
+    }
+    status_.alloc(1);
+    stats_.alloc(2);
+    out_forces_.alloc(3 * (size_t)n);
+    OMM_HIP(hipStreamSynchronize(s));
+    have_box_ = false;
+    initialized_ = true;
+}
+
+double NbOperator::dispersion_coefficient() const {
+    // nonbonded.rs:660-719
+    if (disp_cached_) return disp_coeff_;
+    double moments[13] = {0}, diagonal[13] = {0};
+    for (size_t i = 0; i < cfg_.sigma.size(); ++i) {
+        double power = 1.0;
+        for (int k = 0; k <= 12; ++k) {
+            moments[k] += std::sqrt(cfg_.epsilon[i]) * power;
+            diagonal[k] += cfg_.epsilon[i] * power;
+            power *= cfg_.sigma[i];
+        }
+    }
+    auto pair_moment = [&](int p) {
+        double coef = 1.0, sum = 0.0;
+        for (int k = 0; k <= p; ++k) {
+            sum += coef * moments[k] * moments[p - k];
+            coef *= (double)(p - k) / (double)(k + 1);
+        }
+        return sum / std::pow(2.0, p) + diagonal[p];
+    };
+    auto integral = [&](int p) {
+        double tail = std::pow(cfg_.cutoff, 3 - p) / (double)(p - 3);
+        if (!cfg_.switching) return tail;
+        double sw = cfg_.switching_distance, rc = cfg_.cutoff;
+        auto f = [&](double r) {
+            double t = (r - sw) / (rc - sw);
+            return std::pow(t, 3) * (10.0 + t * (-15.0 + 6.0 * t)) * std::pow(r, 2 - p);
+        };
+        double v0 = f(sw), v1 = f(0.5 * (sw + rc)), v2 = f(rc);
+        double est = (rc - sw) / 6.0 * (v0 + 4.0 * v1 + v2);
+        return tail + simpson(f, sw, rc, v0, v1, v2, est, 1e-11 * std::max(std::fabs(est), 1e-12), 20);
+    };
+    double n = (double)cfg_.sigma.size();
+    disp_coeff_ = 8.0 * PI * n / (n + 1.0) * (pair_moment(12) * integral(12) - pair_moment(6) * integral(6));
+    disp_cached_ = true;
+    return disp_coeff_;
+}
+
+HostTerms NbOperator::host_terms(double volume, bool direct, bool recip) const {
+    HostTerms t;
+    if (recip && cfg_.method == OMM_NB_PME)
+        t.self = -ONE_4PI_EPS0 * (alpha_ / std::sqrt(PI) * sum_q2_ + PI * sum_q_ * sum_q_ / (2.0 * alpha_ * alpha_ * volume));
+    if (direct && cfg_.periodic() && cfg_.dispersion_correction && cfg_.method != OMM_NB_LJPME)
+        t.dispersion = dispersion_coefficient() / volume;
+    return t;
+}
+
+void NbOperator::grow_list() {
+    nl_.cap = std::min(nl_.npad + JTILE, nl_.cap * 2);
+    nl_.cap = (nl_.cap + JTILE - 1) / JTILE * JTILE;
+    nl_.jlist.alloc((size_t)nl_.nblocks * nl_.cap);
+    pp_.cap = nl_.cap;
+}
+
+bool NbOperator::calibrate(const double* pos_dev, const Box* box_dev, unsigned* status, hipStream_t s) {
+    if (n_ == 0 || !cfg_.include_direct) return false;
+    bool changed = false;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        fill_async<unsigned>(status, 1, 0u, s);
+        nlist_check(nl_, pos_dev, box_dev, 1, s);
+        nlist_build(nl_, pos_dev, box_dev, status, s);
+        std::vector<int> counts(nl_.nblocks);
+        unsigned st = 0;
+        nl_.jcount.download(counts.data(), counts.size(), s);
+        OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        OMM_HIP(hipStreamSynchronize(s));
+        int mx = 0;
+        for (int c : counts) mx = std::max(mx, c);
+        long long want = std::min<long long>((long long)nl_.npad + JTILE, 2LL * mx + 2 * JTILE);
+        want = (want + JTILE - 1) / JTILE * JTILE;
+        if (!(st & ST_LIST_OVERFLOW) && want <= nl_.cap) break;
+        nl_.cap = (int)std::max<long long>(want, 2LL * nl_.cap);
+        nl_.cap = std::min(nl_.cap, (nl_.npad + 2 * JTILE) / JTILE * JTILE);
+        nl_.jlist.alloc((size_t)nl_.nblocks * nl_.cap);
+        pp_.cap = nl_.cap;
+        changed = true;
+    }
+    fill_async<unsigned>(status, 1, 0u, s);
+    OMM_HIP(hipStreamSynchronize(s));
+    return changed;
+}
+
+void NbOperator::force_rebuild(hipStream_t s) {
+    std::vector<int> one{1};
+    nl_.flag.upload(one, s);
+}
+
+int NbOperator::mesh_mode(bool energy) const {
+    if (!mixed_) return MESH_F64;
+    return energy ? MESH_F64_FIXED : MESH_F32;
+}
+
+void NbOperator::enqueue(const double* pos_dev, const Box* box_dev, bool direct, bool recip, bool energy,
+                         long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
+                         unsigned* status, hipStream_t s) {
+    if (direct) {
+        nlist_check(nl_, pos_dev, box_dev, 0, s);
+        nlist_build(nl_, pos_dev, box_dev, status, s);
+        launch_pairs(mixed_, energy, nl_, pp_, part_, pos_dev, box_dev, force_fx, force_d, energy_fx, energy_d, status,
+                     nullptr, pair_grid_, s);
+        launch_exceptions(mixed_, exc_, part_, pos_dev, box_dev, cfg_.exceptions_pbc ? 1 : 0,
+                          cfg_.method == OMM_NB_PME ? 1 : 0, alpha_, force_fx, force_d, energy_fx, energy_d, status, s);
+    }
+    if (recip && cfg_.method == OMM_NB_PME)
+        pme_run(pme_, mesh_mode(energy), n_, pos_dev, part_.charge.p, box_dev, force_fx, force_d, energy_fx, energy_d,
+                status, s);
+}
+
+void NbOperator::profile(const double* pos_dev, const Box* box_dev, int reps, long long* force_fx, double* force_d,
+                         long long* energy_fx, double* energy_d, unsigned* status, double ms[6], int64_t* num_pairs,
+                         hipStream_t s) {
+    hipEvent_t t0, t1;
+    OMM_HIP(hipEventCreate(&t0));
+    OMM_HIP(hipEventCreate(&t1));
+    auto timed = [&](auto&& body) {
+        body();  // warm-up
+        OMM_HIP(hipEventRecord(t0, s));
+        for (int r = 0; r < reps; ++r) body();
+        OMM_HIP(hipEventRecord(t1, s));
+        OMM_HIP(hipEventSynchronize(t1));
+        float v = 0.f;
+        OMM_HIP(hipEventElapsedTime(&v, t0, t1));
+        return (double)v / reps;
+    };
+    const bool pme = cfg_.method == OMM_NB_PME;
+    ms[0] = timed([&] {
+        nlist_check(nl_, pos_dev, box_dev, 1, s);
+        nlist_build(nl_, pos_dev, box_dev, status, s);
+    });
+    ms[1] = timed([&] {
+        launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, box_dev, force_fx, force_d, energy_fx, energy_d, status,
+                     nullptr, pair_grid_, s);
+    });
+    ms[2] = timed([&] {
+        launch_exceptions(mixed_, exc_, part_, pos_dev, box_dev, cfg_.exceptions_pbc ? 1 : 0, pme ? 1 : 0, alpha_,
+                          force_fx, force_d, energy_fx, energy_d, status, s);
+    });
+    const int phases[3] = {PME_SPREAD, PME_FFT, PME_GATHER};
+    for (int k = 0; k < 3; ++k)
+        ms[3 + k] = pme ? timed([&] {
+            pme_run(pme_, mesh_mode(false), n_, pos_dev, part_.charge.p, box_dev, force_fx, force_d, energy_fx,
+                    energy_d, status, s, phases[k]);
+        })
+                        : 0.0;
+    if (num_pairs) {
+        fill_async<unsigned long long>(stats_.p, 2, 0ull, s);
+        launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, box_dev, force_fx, force_d, energy_fx, energy_d, status,
+                     stats_.p, pair_grid_, s);
+        unsigned long long st[2];
+        stats_.download(st, 2, s);
+        OMM_HIP(hipStreamSynchronize(s));
+        *num_pairs = (int64_t)st[0];
+    }
+    OMM_HIP(hipEventDestroy(t0));
+    OMM_HIP(hipEventDestroy(t1));
+}
+
+void NbOperator::compute(const double* pos, const double* box, bool direct, bool recip, double* forces,
+                         double* energy, double parts[5], int64_t* num_pairs, uint64_t* pair_hash) {
+    if (!initialized_) throw Error(OMM_ERR_STATE, "NonbondedForce used before initialize");
+    if (!pos) throw Error(OMM_ERR_INVALID, "positions must not be null");
+    check_box(box);
+    for (int i = 0; i < 3 * n_; ++i)
+        if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
+    hipStream_t s = stream_;
+    Box b = make_box(box);
+    if (n_ == 0) {
+        if (energy) *energy = 0.0;
+        if (parts)
+            for (int k = 0; k < 5; ++k) parts[k] = 0.0;
+        if (num_pairs) *num_pairs = 0;
+        if (pair_hash) *pair_hash = 0;
+        return;
+    }
+    bool box_changed = !have_box_ || std::memcmp(b.v, last_box_.v, sizeof(b.v)) != 0;
+    if (box_changed) {
+        box_dev_.upload(&b, 1, s);
+        last_box_ = b;
+        have_box_ = true;
+    }
+    pos_dev_.upload(pos, 3 * (size_t)n_, s);
+    const bool want_stats = num_pairs || pair_hash;
+    // Pair statistics are gathered by a force-only pass; energies by the
+    // energy pass (FP64 pair energies and FP64 mesh in the mixed form).
+    const bool want_energy = !want_stats && (energy || parts);
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        fill_async<unsigned>(status_.p, 1, 0u, s);
+        if (want_stats) fill_async<unsigned long long>(stats_.p, 2, 0ull, s);
+        if (mixed_) {
+            fill_async<long long>(force_fx_.p, 3 * (size_t)n_, 0ll, s);
+            fill_async<long long>(energy_fx_.p, E_SLOTS, 0ll, s);
+        } else {
+            fill_async<double>(force_d_.p, 3 * (size_t)n_, 0.0, s);
+            fill_async<double>(energy_d_.p, E_SLOTS, 0.0, s);
+        }
+        if (direct) {
+            nlist_check(nl_, pos_dev_.p, box_dev_.p, (box_changed || attempt > 0) ? 1 : 0, s);
+            nlist_build(nl_, pos_dev_.p, box_dev_.p, status_.p, s);
+            launch_pairs(mixed_, want_energy, nl_, pp_, part_, pos_dev_.p, box_dev_.p, force_fx_.p, force_d_.p,
+                         energy_fx_.p, energy_d_.p, status_.p, want_stats ? stats_.p : nullptr, pair_grid_, s);
+            launch_exceptions(mixed_, exc_, part_, pos_dev_.p, box_dev_.p, cfg_.exceptions_pbc ? 1 : 0,
+                              cfg_.method == OMM_NB_PME ? 1 : 0, alpha_, force_fx_.p, force_d_.p, energy_fx_.p,
+                              energy_d_.p, status_.p, s);
+        }
+        if (recip && cfg_.method == OMM_NB_PME)
+            pme_run(pme_, mesh_mode(want_energy), n_, pos_dev_.p, part_.charge.p, box_dev_.p, force_fx_.p, force_d_.p,
+                    energy_fx_.p, energy_d_.p, status_.p, s);
+        unsigned st = 0;
+        status_.download(&st, 1, s);
+        OMM_HIP(hipStreamSynchronize(s));
+        if (st & ST_LIST_OVERFLOW) {
+            grow_list();
+            continue;
+        }
+        if (st & ST_NONFINITE) throw Error(OMM_ERR_NONFINITE, "non-finite force or energy on the device");
+        if (st & ST_OVERFLOW) throw Error(OMM_ERR_OVERFLOW, "fixed-point force/energy accumulator overflow");
+        double e[E_SLOTS];
+        if (mixed_) {
+            long long efx[E_SLOTS];
+            energy_fx_.download(efx, E_SLOTS, s);
+            launch_fixed_to_double(force_fx_.p, out_forces_.p, 3 * (size_t)n_, s);
+            if (forces) out_forces_.download(forces, 3 * (size_t)n_, s);
+            OMM_HIP(hipStreamSynchronize(s));
+            for (int k = 0; k < E_SLOTS; ++k) e[k] = (double)efx[k] * (1.0 / ENERGY_SCALE);
+        } else {
+            energy_d_.download(e, E_SLOTS, s);
+            if (forces) force_d_.download(forces, 3 * (size_t)n_, s);
+            OMM_HIP(hipStreamSynchronize(s));
+        }
+        if (want_stats) {
+            unsigned long long st2[2];
+            stats_.download(st2, 2, s);
+            OMM_HIP(hipStreamSynchronize(s));
+            if (num_pairs) *num_pairs = (int64_t)st2[0];
+            if (pair_hash) *pair_hash = st2[1];
+        }
+        HostTerms ht = host_terms(b.volume, direct, recip);
+        double p[5] = {e[E_DIRECT], e[E_EXCEPT], e[E_RECIP], ht.self, ht.dispersion};
+        if (parts)
+            for (int k = 0; k < 5; ++k) parts[k] = p[k];
+        // Order of nonbonded.rs:942-1036: direct, reciprocal, self, dispersion, exceptions.
+        if (energy) *energy = p[0] + p[2] + p[3] + p[4] + p[1];
+        return;
+    }
+    throw Error(OMM_ERR_OVERFLOW, "neighbour list capacity could not be grown enough");
+}
+
+}  // namespace ommhip

@@ -1,0 +1,51 @@
+// Direct-space pair kernel and exception kernel.
+#pragma once
+
+#include "nlist.h"
+
+namespace ommhip {
+
+enum ElecKind : int { ELEC_EWALD = 0, ELEC_REACTION_FIELD = 1, ELEC_COULOMB = 2 };
+
+struct PairParams {
+    int n = 0, nblocks = 0, cap = 0;
+    int elec = ELEC_EWALD;
+    int use_cutoff = 1;
+    int periodic = 1;
+    double alpha = 0.0;
+    double cutoff = 0.0;
+    double krf = 0.0, crf = 0.0;  // reaction field (nonbonded.rs:1233-1250)
+};
+
+// Per-particle parameters in the form the kernels use.
+struct ParticleDev {
+    DevBuf<double> charge;  // q (FP64: exceptions, PME)
+    DevBuf<float> qk;       // q * sqrt(ONE_4PI_EPS0)
+    DevBuf<float> sh;       // sigma / 2
+    DevBuf<float> e2;       // 2 sqrt(epsilon)
+    DevBuf<double> qk_d, sh_d, e2_d;  // FP64 copies for the plain form
+};
+
+struct ExceptionDev {
+    int count = 0;
+    DevBuf<int> p1, p2;
+    DevBuf<double> qq, sig, eps;
+};
+
+// Direct-space pairs over the tile list.  mixed: FP32 math with int64
+// fixed-point force/energy accumulation; otherwise FP64 math with FP64 atomics
+// into force_d/energy_d.  energy_on: also accumulate the energy (in the mixed
+// form each pair energy is evaluated in FP64).  stats (nullable): [0] pair
+// count, [1] pair hash (no energy is accumulated then).
+void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
+                  const double* pos, const Box* box_dev, long long* force_fx, double* force_d,
+                  long long* energy_fx, double* energy_d, unsigned* status, unsigned long long* stats,
+                  int grid, hipStream_t s);
+
+// Exceptions (nonbonded.rs:995-1036), FP64 math.  has_pme selects the erf
+// reciprocal correction with particle charges.
+void launch_exceptions(bool mixed, const ExceptionDev& ex, const ParticleDev& part, const double* pos,
+                       const Box* box_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
+                       double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s);
+
+}  // namespace ommhip

@@ -1,0 +1,499 @@
+// Direct-space LJ + Ewald-screened Coulomb over the tile list, and the
+// exception kernel (nonbonded.rs:842-924, 995-1036).
+//
+// One wave64 per task = (i-block of 32 atoms, j-tile of 64 list entries).
+// Lane l owns i-atom l&31 and the j half-tile l>>5; in step k it pairs with
+// slot (l+k)&31 of its half, so every slot is touched by exactly one lane per
+// step.  j-data sit in LDS (read-only); the j-force accumulators travel with
+// their slot through a 32-lane rotation (ds_bpermute), so no lane ever writes
+// LDS inside the loop.  Positions are FP64 in HBM; each task converts them to
+// FP32 offsets from the i-block's FP64 centre, so deltas carry ~1e-7 nm
+// absolute error instead of the ~5e-7 nm of absolute FP32 coordinates.
+// Near-cutoff pairs (|r^2 - rc^2| < 1e-5 rc^2) are re-decided in FP64 with
+// the reference's exact arithmetic, which makes the pair set bit-exact.
+//
+// Energy mode: forces stay on the FP32 path, and each pair's energy is
+// evaluated in FP64 from FP64 offsets.  The Coulomb classes of a water box
+// (O-O, O-H, H-H) are each ~1e6 kJ/mol and cancel to ~1e2-1e3, so any
+// distance-dependent FP32 rounding bias would survive in the sum; FP64 pair
+// energies remove it.  The MD step does not request energies and never pays
+// for this.
+#include <climits>
+
+#include "pairs.h"
+
+namespace ommhip {
+
+namespace {
+
+template <class R>
+struct alignas(4 * sizeof(R)) Q4 {
+    R x, y, z, w;
+};
+template <class R>
+struct alignas(2 * sizeof(R)) Q2 {
+    R x, y;
+};
+
+template <class R>
+struct Math;
+
+// FP32 pair arithmetic.  erfc(x) = erfcx(x) * exp(-x^2) with erfcx fitted
+// (this project's own weighted Chebyshev fit, converted to monomials) as a
+// degree-8 polynomial in t = (x - 2.5)/(x + 2.5) on x in [0, 6]: relative error
+// 1.2e-8 in exact arithmetic, 1.6e-7 evaluated in FP32.  exp(-x^2) reuses the
+// hardware exp2.  Reciprocal and reciprocal square root take one Newton step.
+template <>
+struct Math<float> {
+    static __device__ __forceinline__ float rcp(float x) {
+        float r = __builtin_amdgcn_rcpf(x);
+        float e = fmaf(-x, r, 1.0f);
+        return fmaf(r, e, r);
+    }
+    static __device__ __forceinline__ float rsqrt(float x) {
+        float y = __builtin_amdgcn_rsqf(x);
+        float t = x * y;
+        float e = fmaf(-t, y, 1.0f);
+        return fmaf(0.5f * y, e, y);
+    }
+    static __device__ __forceinline__ void erfc_exp(float x, float& erfc_v, float& ex) {
+        float t = (x - 2.5f) * rcp(x + 2.5f);
+        t = fminf(t, 0.41176470f);
+        float p = 0.0003302688419353217f;
+        p = fmaf(p, t, 0.0008612216333858669f);
+        p = fmaf(p, t, -0.0026842965744435787f);
+        p = fmaf(p, t, -0.003998667001724243f);
+        p = fmaf(p, t, 0.039928220212459564f);
+        p = fmaf(p, t, -0.12503232061862946f);
+        p = fmaf(p, t, 0.2517129182815552f);
+        p = fmaf(p, t, -0.3717367649078369f);
+        p = fmaf(p, t, 0.21080636978149414f);
+        ex = __builtin_amdgcn_exp2f(-(x * x) * 1.4426950408889634f);
+        erfc_v = p * ex;
+    }
+    static __device__ __forceinline__ float round(float x) { return rintf(x); }
+};
+
+template <>
+struct Math<double> {
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
+    static __device__ __forceinline__ void erfc_exp(double x, double& erfc_v, double& ex) {
+        erfc_v = erfc(x);
+        ex = exp(-x * x);
+    }
+    static __device__ __forceinline__ double round(double x) { return rint(x); }
+};
+
+// The reference's cutoff decision for the pair (a, b): delta = x_j - x_i with
+// i < j, reduced-triclinic minimum image, r^2 = dx dx + dy dy + dz dz, no FMA.
+__device__ __noinline__ bool within_cutoff_exact(const double* __restrict__ pos, int a, int b, const Box& bx,
+                                                 int periodic, double cutoff2) {
+#pragma clang fp contract(off)
+    int i = min(a, b), j = max(a, b);
+    double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+    if (periodic) min_image_exact(d, bx.v);
+    double r2 = d[0] * d[0];
+    r2 = r2 + d[1] * d[1];
+    r2 = r2 + d[2] * d[2];
+    return r2 < cutoff2;
+}
+
+// FP64 energy of one pair (nonbonded.rs:1203-1214, 1219-1273) from offsets.
+template <int ELEC>
+__device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double dz, double sig, double eps4,
+                                                   double qq, const PairParams& P) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    double e = 0.0;
+    if (eps4 != 0.0) {
+        const double s2 = sig * sig / r2;
+        const double s6 = s2 * s2 * s2;
+        e = eps4 * (s6 * s6 - s6);
+    }
+    if (qq != 0.0) {
+        if (ELEC == ELEC_EWALD) e += qq * erfc(P.alpha * r) / r;
+        else if (ELEC == ELEC_REACTION_FIELD) e += qq * (1.0 / r + P.krf * r2 - P.crf);
+        else e += qq / r;
+    }
+    return e;
+}
+
+template <class R, class A, int ELEC, bool STATS, bool ENERGY>
+__global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, const R* __restrict__ qk,
+                                              const R* __restrict__ sh, const R* __restrict__ e2,
+                                              const double* __restrict__ qk_d, const double* __restrict__ sh_d,
+                                              const double* __restrict__ e2_d, const int* __restrict__ sorted_atom,
+                                              const int* __restrict__ excl_sorted, const double* __restrict__ center,
+                                              const int* __restrict__ minimg, const int* __restrict__ jlist,
+                                              const int* __restrict__ jcount, const int* __restrict__ task_start,
+                                              const Box* __restrict__ box, PairParams P, A* force, A* energy,
+                                              unsigned* status, unsigned long long* stats) {
+    using M = Math<R>;
+    constexpr bool E64 = ENERGY && sizeof(R) == 4;  // separate FP64 energy path
+    __shared__ Q4<R> s_pq[JTILE];
+    __shared__ Q2<R> s_se[JTILE];
+    __shared__ int s_js[JTILE];
+    __shared__ int s_ja[JTILE];
+    __shared__ Q4<double> s_pd[E64 ? JTILE : 1];
+    __shared__ Q2<double> s_sd[E64 ? JTILE : 1];
+
+    const int lane = threadIdx.x;
+    const int half = lane >> 5;
+    const int il = lane & 31;
+    const int src = (lane & 32) | ((lane + 1) & 31);
+    const Box& bx = *box;
+    const R bv[9] = {(R)bx.v[0], (R)bx.v[1], (R)bx.v[2], (R)bx.v[3], (R)bx.v[4],
+                     (R)bx.v[5], (R)bx.v[6], (R)bx.v[7], (R)bx.v[8]};
+    const R ib0 = (R)(1.0 / bx.v[0]), ib4 = (R)(1.0 / bx.v[4]), ib8 = (R)(1.0 / bx.v[8]);
+    const double cutoff2 = P.cutoff * P.cutoff;
+    const double band = sizeof(R) == 4 ? 1e-5 : 1e-12;
+    const R rc2_lo = (R)(cutoff2 * (1.0 - band));
+    const R rc2_hi = (R)(cutoff2 * (1.0 + band));
+    const R alpha = (R)P.alpha;
+    const R two_a_sqrtpi = (R)(2.0 * P.alpha / sqrt(PI));
+    const R krf = (R)P.krf, crf = (R)P.crf;
+    const int total = task_start[P.nblocks];
+
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = P.nblocks;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (task_start[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int I = lo;
+        const int tile = t - task_start[I];
+        const int cnt = min(JTILE, jcount[I] - tile * JTILE);
+        const double c0 = center[3 * I], c1 = center[3 * I + 1], c2 = center[3 * I + 2];
+
+        // Stage the j-tile in LDS.
+        {
+            int js = INT_MAX, ja = -1;
+            Q4<R> pq = {0, 0, 0, 0};
+            Q2<R> se = {0, 0};
+            Q4<double> pd = {0, 0, 0, 0};
+            Q2<double> sd = {0, 0};
+            if (lane < cnt) {
+                js = jlist[(size_t)I * P.cap + tile * JTILE + lane];
+                ja = sorted_atom[js];
+                double d[3] = {pos[3 * ja] - c0, pos[3 * ja + 1] - c1, pos[3 * ja + 2] - c2};
+                if (P.periodic) wrap_nearest(d, bx);
+                pq = {(R)d[0], (R)d[1], (R)d[2], qk[ja]};
+                se = {sh[ja], e2[ja]};
+                if (E64) {
+                    pd = {d[0], d[1], d[2], qk_d[ja]};
+                    sd = {sh_d[ja], e2_d[ja]};
+                }
+            }
+            s_pq[lane] = pq;
+            s_se[lane] = se;
+            s_js[lane] = js;
+            s_ja[lane] = ja;
+            if (E64) {
+                s_pd[lane] = pd;
+                s_sd[lane] = sd;
+            }
+        }
+        const int is = I * IBLOCK + il;
+        const int ia = is < P.n ? sorted_atom[is] : -1;
+        R xi = 0, yi = 0, zi = 0, qi = 0, si = 0, ei = 0;
+        double xid = 0, yid = 0, zid = 0, qid = 0, sid = 0, eid = 0;
+        if (ia >= 0) {
+            double d[3] = {pos[3 * ia] - c0, pos[3 * ia + 1] - c1, pos[3 * ia + 2] - c2};
+            if (P.periodic) wrap_nearest(d, bx);
+            xi = (R)d[0];
+            yi = (R)d[1];
+            zi = (R)d[2];
+            qi = qk[ia];
+            si = sh[ia];
+            ei = e2[ia];
+            if (E64) {
+                xid = d[0];
+                yid = d[1];
+                zid = d[2];
+                qid = qk_d[ia];
+                sid = sh_d[ia];
+                eid = e2_d[ia];
+            }
+        }
+        __syncthreads();
+
+        // Valid slots of this lane's half: j_sorted > i_sorted, present, not excluded.
+        unsigned vmask = 0;
+        if (ia >= 0) {
+            const int* hjs = s_js + half * 32;
+            int a = 0, b = 32;
+            while (a < b) {
+                int m = (a + b) >> 1;
+                if (hjs[m] > is) b = m;
+                else a = m + 1;
+            }
+            vmask = a >= 32 ? 0u : (0xffffffffu << a);
+            int nvalid = min(max(cnt - half * 32, 0), 32);
+            if (nvalid < 32) vmask &= (1u << nvalid) - 1u;
+            const int* ex = excl_sorted + (size_t)is * MAXEX;
+            for (int m = 0; m < MAXEX; ++m) {
+                int e = ex[m];
+                if (e == INT_MAX) break;
+                if (e <= is) continue;
+                int lo2 = 0, hi2 = 32;
+                while (lo2 < hi2) {
+                    int mid = (lo2 + hi2) >> 1;
+                    if (hjs[mid] < e) lo2 = mid + 1;
+                    else hi2 = mid;
+                }
+                if (lo2 < 32 && hjs[lo2] == e) vmask &= ~(1u << lo2);
+            }
+        }
+        const bool mi = P.periodic && minimg[I];
+
+        R fx = 0, fy = 0, fz = 0, gx = 0, gy = 0, gz = 0;
+        double eacc = 0.0;
+        unsigned long long npair = 0, hsum = 0;
+        for (int k = 0; k < 32; ++k) {
+            const int kk = (il + k) & 31;
+            const int slot = half * 32 + kk;
+            if ((vmask >> kk) & 1u) {
+                const Q4<R> pj = s_pq[slot];
+                R dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+                if (mi) {
+                    R s = M::round(dz * ib8);
+                    dx -= s * bv[6];
+                    dy -= s * bv[7];
+                    dz -= s * bv[8];
+                    s = M::round(dy * ib4);
+                    dx -= s * bv[3];
+                    dy -= s * bv[4];
+                    s = M::round(dx * ib0);
+                    dx -= s * bv[0];
+                }
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                bool in = !P.use_cutoff || r2 < rc2_lo;
+                if (!in && r2 < rc2_hi) in = within_cutoff_exact(pos, ia, s_ja[slot], bx, P.periodic, cutoff2);
+                if (in) {
+                    const Q2<R> sj = s_se[slot];
+                    const R inv_r2 = M::rcp(r2);
+                    const R inv_r = M::rsqrt(r2);
+                    // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
+                    const R sig = si + sj.x;
+                    const R eps4 = ei * sj.y;
+                    const R s2 = sig * sig * inv_r2;
+                    const R s6 = s2 * s2 * s2;
+                    const R s12 = s6 * s6;
+                    R e = eps4 * (s12 - s6);
+                    R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
+                    const R qq = qi * pj.w;
+                    if (ELEC == ELEC_EWALD) {
+                        // screened_coulomb (nonbonded.rs:1263-1273)
+                        const R r = r2 * inv_r;
+                        R erfc_v, ex;
+                        M::erfc_exp(alpha * r, erfc_v, ex);
+                        const R ec = qq * erfc_v * inv_r;
+                        e += ec;
+                        f -= (ec + qq * two_a_sqrtpi * ex) * inv_r2;
+                    } else if (ELEC == ELEC_REACTION_FIELD) {
+                        // reaction_field (nonbonded.rs:1233-1250)
+                        e += qq * (inv_r + krf * r2 - crf);
+                        f += qq * ((R)2 * krf - inv_r * inv_r2);
+                    } else {
+                        // coulomb_pair (nonbonded.rs:1219-1226)
+                        const R ec = qq * inv_r;
+                        e += ec;
+                        f -= ec * inv_r2;
+                    }
+                    fx += f * dx;
+                    fy += f * dy;
+                    fz += f * dz;
+                    gx -= f * dx;
+                    gy -= f * dy;
+                    gz -= f * dz;
+                    if (E64) {
+                        const Q4<double> pjd = s_pd[slot];
+                        const Q2<double> sjd = s_sd[slot];
+                        double ddx = pjd.x - xid, ddy = pjd.y - yid, ddz = pjd.z - zid;
+                        if (mi) {
+                            double dd[3] = {ddx, ddy, ddz};
+                            wrap_nearest(dd, bx);
+                            ddx = dd[0];
+                            ddy = dd[1];
+                            ddz = dd[2];
+                        }
+                        eacc += pair_energy_fp64<ELEC>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
+                    } else if (ENERGY) {
+                        eacc += (double)e;
+                    }
+                    if (STATS) {
+                        const int ja = s_ja[slot];
+                        ++npair;
+                        hsum += pair_key(min(ia, ja), max(ia, ja));
+                    }
+                }
+            }
+            gx = __shfl(gx, src, 64);
+            gy = __shfl(gy, src, 64);
+            gz = __shfl(gz, src, 64);
+        }
+        fx += __shfl_xor(fx, 32, 64);
+        fy += __shfl_xor(fy, 32, 64);
+        fz += __shfl_xor(fz, 32, 64);
+        if (half == 0 && ia >= 0) {
+            acc_add(force + 3 * ia, (double)fx, FORCE_SCALE, status);
+            acc_add(force + 3 * ia + 1, (double)fy, FORCE_SCALE, status);
+            acc_add(force + 3 * ia + 2, (double)fz, FORCE_SCALE, status);
+        }
+        const int jmine = s_ja[lane];
+        if (jmine >= 0) {
+            acc_add(force + 3 * jmine, (double)gx, FORCE_SCALE, status);
+            acc_add(force + 3 * jmine + 1, (double)gy, FORCE_SCALE, status);
+            acc_add(force + 3 * jmine + 2, (double)gz, FORCE_SCALE, status);
+        }
+        if (ENERGY) {
+            double etot = wave_sum(eacc);
+            if (lane == 0) acc_add(energy + E_DIRECT, etot, ENERGY_SCALE, status);
+        }
+        if (STATS) {
+            npair = wave_sum_u64(npair);
+            hsum = wave_sum_u64(hsum);
+            if (lane == 0) {
+                atomicAdd(stats, npair);
+                atomicAdd(stats + 1, hsum);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class A>
+__global__ void k_exceptions(int count, const int* __restrict__ p1, const int* __restrict__ p2,
+                             const double* __restrict__ eqq, const double* __restrict__ esig,
+                             const double* __restrict__ eeps, const double* __restrict__ charge,
+                             const double* __restrict__ pos, const Box* __restrict__ box, int use_pbc, int has_pme,
+                             double alpha, A* force, A* energy, unsigned* status) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (k < count) {
+        int i = p1[k], j = p2[k];
+        double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+        if (use_pbc) min_image_exact(d, box->v);
+        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        double r = sqrt(r2);
+        double f = 0.0;
+        // lj_coulomb_exception (nonbonded.rs:1194-1198)
+        if (eeps[k] != 0.0) {
+            double sr2 = esig[k] * esig[k] / r2;
+            double sr6 = sr2 * sr2 * sr2;
+            double sr12 = sr6 * sr6;
+            e += 4.0 * eeps[k] * (sr12 - sr6);
+            f += 4.0 * eeps[k] * (-12.0 * sr12 + 6.0 * sr6) / r2;
+        }
+        if (eqq[k] != 0.0) {
+            e += ONE_4PI_EPS0 * eqq[k] / r;
+            f -= ONE_4PI_EPS0 * eqq[k] / (r2 * r);
+        }
+        if (has_pme) {
+            // reciprocal_pair (nonbonded.rs:1275-1288)
+            double c = ONE_4PI_EPS0 * charge[i] * charge[j];
+            double corr, deriv;
+            if (alpha * r < 1e-6) {
+                corr = c * 2.0 * alpha / sqrt(PI) * (1.0 - alpha * alpha * r2 / 3.0);
+                deriv = -c * 4.0 * alpha * alpha * alpha / (3.0 * sqrt(PI));
+            } else {
+                double erf_v = erf(alpha * r);
+                corr = c * erf_v / r;
+                deriv = c * (2.0 * alpha / sqrt(PI) * exp(-alpha * alpha * r2) - erf_v / r) / r2;
+            }
+            e -= corr;
+            f -= deriv;
+        }
+        for (int a = 0; a < 3; ++a) {
+            acc_add(force + 3 * i + a, f * d[a], FORCE_SCALE, status);
+            acc_add(force + 3 * j + a, -f * d[a], FORCE_SCALE, status);
+        }
+    }
+    double et = wave_sum(e);
+    if ((threadIdx.x & 63) == 0) acc_add(energy + E_EXCEPT, et, ENERGY_SCALE, status);
+}
+
+template <class R, class A, int ELEC, bool STATS, bool ENERGY>
+void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* sh, const R* e2,
+                const ParticleDev& part, const double* pos, const Box* box_dev, A* force, A* energy, unsigned* status,
+                unsigned long long* stats, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
+                       part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.excl_sorted.p, nl.block_center.p,
+                       nl.block_minimg.p, nl.jlist.p, nl.jcount.p, nl.task_start.p, box_dev, p, force, energy, status,
+                       stats);
+}
+
+template <class R, class A, int ELEC>
+void launch_pairs_t(bool stats_on, bool energy_on, const NlistDev& nl, const PairParams& p, const R* qk, const R* sh,
+                    const R* e2, const ParticleDev& part, const double* pos, const Box* box_dev, A* force, A* energy,
+                    unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
+    if (stats_on)
+        launch_one<R, A, ELEC, true, false>(nl, p, qk, sh, e2, part, pos, box_dev, force, energy, status, stats, grid, s);
+    else if (energy_on)
+        launch_one<R, A, ELEC, false, true>(nl, p, qk, sh, e2, part, pos, box_dev, force, energy, status, stats, grid, s);
+    else
+        launch_one<R, A, ELEC, false, false>(nl, p, qk, sh, e2, part, pos, box_dev, force, energy, status, stats, grid,
+                                             s);
+}
+
+}  // namespace
+
+void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
+                  const double* pos, const Box* box_dev, long long* force_fx, double* force_d, long long* energy_fx,
+                  double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
+    bool st = stats != nullptr;
+    if (mixed) {
+        switch (p.elec) {
+            case ELEC_EWALD:
+                launch_pairs_t<float, long long, ELEC_EWALD>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part,
+                                                             pos, box_dev, force_fx, energy_fx, status, stats, grid, s);
+                break;
+            case ELEC_REACTION_FIELD:
+                launch_pairs_t<float, long long, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk.p, part.sh.p,
+                                                                      part.e2.p, part, pos, box_dev, force_fx,
+                                                                      energy_fx, status, stats, grid, s);
+                break;
+            default:
+                launch_pairs_t<float, long long, ELEC_COULOMB>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p,
+                                                               part, pos, box_dev, force_fx, energy_fx, status, stats,
+                                                               grid, s);
+        }
+    } else {
+        switch (p.elec) {
+            case ELEC_EWALD:
+                launch_pairs_t<double, double, ELEC_EWALD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p, part.e2_d.p,
+                                                           part, pos, box_dev, force_d, energy_d, status, stats, grid, s);
+                break;
+            case ELEC_REACTION_FIELD:
+                launch_pairs_t<double, double, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
+                                                                    part.e2_d.p, part, pos, box_dev, force_d, energy_d,
+                                                                    status, stats, grid, s);
+                break;
+            default:
+                launch_pairs_t<double, double, ELEC_COULOMB>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
+                                                             part.e2_d.p, part, pos, box_dev, force_d, energy_d, status,
+                                                             stats, grid, s);
+        }
+    }
+    OMM_HIP(hipGetLastError());
+}
+
+void launch_exceptions(bool mixed, const ExceptionDev& ex, const ParticleDev& part, const double* pos,
+                       const Box* box_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
+                       double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s) {
+    if (ex.count == 0) return;
+    int blocks = (ex.count + 255) / 256;
+    if (mixed)
+        hipLaunchKernelGGL(k_exceptions<long long>, dim3(blocks), dim3(256), 0, s, ex.count, ex.p1.p, ex.p2.p, ex.qq.p,
+                           ex.sig.p, ex.eps.p, part.charge.p, pos, box_dev, use_pbc, has_pme, alpha, force_fx,
+                           energy_fx, status);
+    else
+        hipLaunchKernelGGL(k_exceptions<double>, dim3(blocks), dim3(256), 0, s, ex.count, ex.p1.p, ex.p2.p, ex.qq.p,
+                           ex.sig.p, ex.eps.p, part.charge.p, pos, box_dev, use_pbc, has_pme, alpha, force_d, energy_d,
+                           status);
+    OMM_HIP(hipGetLastError());
+}
+
+}  // namespace ommhip

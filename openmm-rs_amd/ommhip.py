@@ -374,6 +374,15 @@ class GpuNonbondedForce:
                                                        f.ctypes.data_as(_dp), C.byref(e)))
         return f, e.value
 
+    def calc_forces(self, pos, box=None):
+        """Forces only (energy_out = NULL): the MD step's FP32 path."""
+        p = _f64(pos).reshape(-1, 3)
+        b, bp = _box_ptr(box)
+        f = np.zeros_like(p)
+        _check(library().omm_nb_calc_forces_and_energy(self._h, p.ctypes.data_as(_dp), bp,
+                                                       f.ctypes.data_as(_dp), None))
+        return f
+
     def calc_forces_and_energy_groups(self, pos, box, groups):
         p = _f64(pos).reshape(-1, 3)
         b, bp = _box_ptr(box)
@@ -522,6 +531,16 @@ class Context:
         e = C.c_double()
         _check(library().omm_ctx_kinetic_energy(self._h, C.byref(e)))
         return e.value
+
+    PHASES = ("nlist_build", "direct_pairs", "exceptions", "pme_spread", "pme_fft_conv", "pme_gather",
+              "bonded", "integrator")
+
+    def profile(self, reps=20):
+        """Per-phase average device ms (HIP events on the context stream) and the pair count."""
+        ms = np.zeros(8)
+        npairs = C.c_int64()
+        _check(library().omm_ctx_profile(self._h, int(reps), ms.ctypes.data_as(_dp), C.byref(npairs)))
+        return dict(zip(self.PHASES, ms.tolist())), npairs.value
 
     def close(self):
         if getattr(self, "_h", None):
