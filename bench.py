@@ -11,8 +11,12 @@ is replayed from a HIP graph.
 Prints one JSON line.  Besides the contract fields it carries the roofline of
 the dominant kernel (direct-space pairs, 75 FLOP per in-cutoff pair against
 the FP32 peak), the per-phase device times, the plain FP64 form on the same
-workload and the CPU baseline (the FP64 C++ restatement of the reference,
-single-threaded as the reference is).
+workload, the CPU baseline (the FP64 C++ restatement of the reference,
+single-threaded as the reference is) and the batched workload: 64
+independent water boxes of 500-6,000 molecules (log-uniform from a seed, box
+edge for 33.4 molecules/nm^3, each with its own PME grid) advanced by one set
+of launches per phase, in replica-steps/s, beside the same replicas run as 64
+separate plain FP64 contexts (per-system launches).
 """
 import argparse
 import json
@@ -42,15 +46,88 @@ def ns_per_day(steps, seconds):
     return steps / seconds * DT_PS * 86400.0 / 1000.0
 
 
-def run_gpu(precision, steps, warmup, seed=2024):
-    s = workload(seed)
-    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seed, precision=precision)
-    ctx.set_velocities_to_temperature(300.0, seed + 1)
+def equilibrate_then(ctx, equilibrate, warmup):
+    """The seeded lattice with random orientations starts ~3e5 kJ/mol above
+    liquid water; relaxing it heats the box to ~1000 K.  `equilibrate` steps
+    at friction 50/ps remove that heat before the `warmup` steps at the
+    workload's friction 1/ps; timed steps all run at 300 K, 1/ps."""
+    if equilibrate:
+        ctx.set_integrator(300.0, 50.0, DT_PS)
+        ctx.step(equilibrate)
+        ctx.set_integrator(300.0, 1.0, DT_PS)
     ctx.step(warmup)
+
+
+def timed_steps(ctx, steps):
     t0 = time.perf_counter()
     ctx.step(steps)  # returns after the device finished (stream synchronised)
-    t1 = time.perf_counter()
-    return ctx, t1 - t0
+    return time.perf_counter() - t0
+
+
+def run_single(steps, warmup, equilibrate, plain_steps, seed=2024):
+    """23,550-atom step: mixed form timed after equilibration; the plain FP64
+    form is then timed from the same equilibrated state."""
+    s = workload(seed)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seed,
+                         precision=ommhip.PRECISION_MIXED)
+    ctx.set_velocities_to_temperature(300.0, seed + 1)
+    equilibrate_then(ctx, equilibrate, warmup)
+    secs = timed_steps(ctx, steps)
+    st = ctx.get_state()
+    plain = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seed,
+                           precision=ommhip.PRECISION_DOUBLE)
+    plain.set_positions(st["positions"])
+    plain.set_velocities(st["velocities"])
+    plain.step(20)
+    psecs = timed_steps(plain, plain_steps)
+    plain.close()
+    return ctx, secs, psecs
+
+
+def batch_workload(count=64, seed=4096, lo=500, hi=6000, density=33.4):
+    """Replica r: M_r = round(exp(U(ln lo, ln hi))) molecules, edge (M_r/density)^(1/3) nm."""
+    rng = np.random.default_rng(seed)
+    mols = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=count))).astype(int)
+    return [ommhip.water_system(int(m), float((m / density) ** (1.0 / 3.0)), seed + 1 + r, cutoff=0.9,
+                                tolerance=5e-4) for r, m in enumerate(mols)]
+
+
+def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
+    """Mixed batched form vs the same replicas as separate plain FP64 contexts."""
+    systems = batch_workload(count, seed)
+    seeds = [seed + 100 + r for r in range(count)]
+    batch = ommhip.Batch(systems, temperature=300.0, friction=1.0, step_size=DT_PS, seeds=seeds,
+                         precision=ommhip.PRECISION_MIXED)
+    batch.set_velocities_to_temperature(300.0, seed + 1)
+    equilibrate_then(batch, equilibrate, warmup)
+    secs = timed_steps(batch, steps)
+    phases, npairs = batch.profile(10)
+    plain_secs = 0.0
+    for r, s in enumerate(systems):
+        st = batch.get_state(r)
+        c = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seeds[r],
+                           precision=ommhip.PRECISION_DOUBLE)
+        c.set_positions(st["positions"])
+        c.set_velocities(st["velocities"])
+        c.step(2)
+        plain_secs += timed_steps(c, plain_steps)
+        c.close()
+    atoms = int(sum(len(s.masses) for s in systems))
+    value = count * steps / secs
+    plain = count * plain_steps / plain_secs
+    return {
+        "replicas": count,
+        "atoms": atoms,
+        "molecules_min_max": [int(min(len(s.masses) for s in systems) // 3),
+                              int(max(len(s.masses) for s in systems) // 3)],
+        "replica_steps_per_s": round(value, 2),
+        "ms_per_batch_step": round(secs / steps * 1e3, 4),
+        "steps": steps,
+        "pairs_per_step": npairs,
+        "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "plain_fp64_per_system": {"replica_steps_per_s": round(plain, 2), "steps_per_replica": plain_steps},
+        "speedup_mixed_batched_over_plain": round(value / plain, 2),
+    }
 
 
 def cpu_baseline(sample_steps):
@@ -84,10 +161,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--equilibrate", type=int, default=2000)
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-reps", type=int, default=50)
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--batch-equilibrate", type=int, default=1000)
+    ap.add_argument("--batch-plain-steps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -98,13 +180,13 @@ def main():
         print("no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
 
-    ctx, secs = run_gpu(ommhip.PRECISION_MIXED, args.steps, args.warmup)
+    ctx, secs, psecs = run_single(args.steps, args.warmup, args.equilibrate, args.plain_steps)
     value = ns_per_day(args.steps, secs)
     phases, npairs = ctx.profile(args.profile_reps)
+    tiles, entries = ctx.list_stats()
     t_pairs = phases["direct_pairs"] * 1e-3
     achieved = FLOP_PER_PAIR * npairs / t_pairs / 1e12
 
-    _, psecs = run_gpu(ommhip.PRECISION_DOUBLE, args.plain_steps, 20)
     plain = ns_per_day(args.plain_steps, psecs)
 
     line = {
@@ -136,11 +218,15 @@ def main():
             "kernel": "k_pairs (direct-space LJ + erfc-Ewald), FP32 VALU work: 75 FLOP x in-cutoff pairs per launch",
             "pairs_per_launch": npairs,
             "ms_per_launch": round(phases["direct_pairs"], 5),
+            "tiles_per_launch": tiles,
+            "pair_slot_efficiency": round(npairs / max(1, tiles * 2048), 4),
         },
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "plain_fp64": {"value": round(plain, 3), "unit": "ns/day", "steps": args.plain_steps,
                        "speedup_mixed_over_plain": round(value / plain, 2)},
     }
+    if not args.no_batch:
+        line["batched"] = run_batch(args.batch_steps, 100, args.batch_equilibrate, args.batch_plain_steps)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -219,9 +219,21 @@ omm_status omm_ctx_set_velocities(omm_ctx* ctx, const double* vel);
  * offset, langevin.rs:204-206). */
 omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperature, uint64_t seed);
 
-/* Context::step (context.rs:795): nsteps LangevinMiddle steps. The step is
- * captured once into a HIP graph (mixed precision) and replayed. */
+/* LangevinMiddleIntegrator::set_temperature / set_friction / set_step_size
+ * (integrators/langevin.rs:82-101) on the context's integrator; same checks
+ * (temperature and friction nonnegative and finite, step size positive). */
+omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double friction, double step_size);
+
+/* Context::step (context.rs:795): nsteps LangevinMiddle steps.  The mixed
+ * form replays a captured HIP graph of 10 steps; each step's integrator
+ * flags the neighbour list when an atom moved more than half the list skin,
+ * and the next step's build kernels rebuild it on the device. */
 omm_status omm_ctx_step(omm_ctx* ctx, int32_t nsteps);
+
+/* Shape of the current neighbour list (measurement hook; no counterpart in the
+ * reference): number of 32 x 64 tiles the pair kernel evaluates and the total
+ * number of j-list entries.  Pair-slot efficiency = pairs / (tiles * 2048). */
+omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries);
 
 /* Context::get_state (context.rs:930-1105) with positions, velocities, forces
  * and potential energy; any output pointer may be NULL. */
@@ -253,6 +265,58 @@ omm_status omm_ctx_destroy(omm_ctx* ctx);
  * fixed water parameters (O: -0.834 e, sigma 0.315061 nm, eps 0.636386 kJ/mol,
  * 15.999 Da; H: +0.417 e, sigma 1.0, eps 0, 1.008 Da). */
 omm_status omm_water_box(int32_t num_molecules, double box_edge, uint64_t seed, double* positions);
+
+/* ---------------------------------------------------------------------------
+ * Batched replicas.  The reference runs one Context per independent system;
+ * a batch holds many such systems (each with its own particles, topology,
+ * box, PME grid and seed) and advances all of them with one set of launches
+ * per phase inside one captured graph.  Replica r of a batch follows the same
+ * trajectory bit for bit as an omm_ctx created from the same descriptor
+ * (forces are fixed-point sums; the RNG is keyed by the replica's seed).
+ * The replicas must share the nonbonded method, cutoff, tolerance and
+ * options (so alpha is shared) and the integrator settings.
+ * ------------------------------------------------------------------------ */
+
+typedef struct omm_batch omm_batch;
+
+/* One Context::new (context.rs:83-161) per descriptor of replicas[0..count). */
+omm_status omm_batch_create(const omm_md_desc* replicas, int32_t count, int32_t precision, omm_batch** out);
+omm_status omm_batch_size(omm_batch* b, int32_t* count);
+
+/* Context::set_positions / set_velocities of replica r (host N_r x 3). */
+omm_status omm_batch_set_positions(omm_batch* b, int32_t replica, const double* pos);
+omm_status omm_batch_set_velocities(omm_batch* b, int32_t replica, const double* vel);
+
+/* Context::set_velocities_to_temperature of every replica; replica r draws
+ * from seed + r. */
+omm_status omm_batch_set_velocities_to_temperature(omm_batch* b, double temperature, uint64_t seed);
+
+/* LangevinMiddleIntegrator settings shared by every replica (as
+ * omm_ctx_set_integrator). */
+omm_status omm_batch_set_integrator(omm_batch* b, double temperature, double friction, double step_size);
+
+/* Context::step of every replica (nsteps each). */
+omm_status omm_batch_step(omm_batch* b, int32_t nsteps);
+
+/* Context::get_state of replica r; any output pointer may be NULL. */
+omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
+                               double* potential_energy, int64_t* step_count);
+
+/* Potential energy of every replica (count values), one device evaluation. */
+omm_status omm_batch_energies(omm_batch* b, double* potential_energies);
+
+/* Kinetic energy of replica r, kJ/mol. */
+omm_status omm_batch_kinetic_energy(omm_batch* b, int32_t replica, double* kinetic_energy);
+
+/* Current box vectors of replica r (row-major a, b, c). */
+omm_status omm_batch_get_box(omm_batch* b, int32_t replica, double box[9]);
+
+/* Measurement hooks (no counterpart in the reference): list shape and the
+ * per-phase device times of omm_ctx_profile, over the whole batch. */
+omm_status omm_batch_list_stats(omm_batch* b, int64_t* tiles, int64_t* entries);
+omm_status omm_batch_profile(omm_batch* b, int32_t reps, double phase_ms[8], int64_t* num_pairs);
+
+omm_status omm_batch_destroy(omm_batch* b);
 
 #ifdef __cplusplus
 }

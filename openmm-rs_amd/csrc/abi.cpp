@@ -24,12 +24,17 @@ using ommhip::guarded;
 struct omm_nb {
     ommhip::NbOperator op;
     double parts[5] = {0, 0, 0, 0, 0};
-    omm_nb(const omm_nb_desc* d, int precision) : op(d, precision) {}
+    omm_nb(const omm_nb_desc* d, int precision) : op(&d, 1, precision) {}
 };
 
 struct omm_ctx {
     ommhip::MdContext ctx;
-    omm_ctx(const omm_md_desc* d, int precision) : ctx(d, precision) {}
+    omm_ctx(const omm_md_desc* d, int precision) : ctx(d, 1, precision) {}
+};
+
+struct omm_batch {
+    ommhip::MdContext ctx;
+    omm_batch(const omm_md_desc* d, int count, int precision) : ctx(d, count, precision) {}
 };
 
 extern "C" {
@@ -175,11 +180,11 @@ omm_status omm_ctx_create(const omm_md_desc* desc, int32_t precision, omm_ctx** 
     })
 
 omm_status omm_ctx_set_positions(omm_ctx* ctx, const double* pos) {
-    CTX_CALL(if (!pos) throw Error(OMM_ERR_INVALID, "null positions"); ctx->ctx.set_positions(pos));
+    CTX_CALL(if (!pos) throw Error(OMM_ERR_INVALID, "null positions"); ctx->ctx.set_positions(0, pos));
 }
 
 omm_status omm_ctx_set_velocities(omm_ctx* ctx, const double* vel) {
-    CTX_CALL(if (!vel) throw Error(OMM_ERR_INVALID, "null velocities"); ctx->ctx.set_velocities(vel));
+    CTX_CALL(if (!vel) throw Error(OMM_ERR_INVALID, "null velocities"); ctx->ctx.set_velocities(0, vel));
 }
 
 omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperature, uint64_t seed) {
@@ -188,13 +193,21 @@ omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperatur
 
 omm_status omm_ctx_step(omm_ctx* ctx, int32_t nsteps) { CTX_CALL(ctx->ctx.step(nsteps)); }
 
+omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double friction, double step_size) {
+    CTX_CALL(ctx->ctx.set_integrator(temperature, friction, step_size));
+}
+
+omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries) {
+    CTX_CALL(ctx->ctx.list_stats(tiles, entries));
+}
+
 omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* forces, double* potential_energy,
                              int64_t* step_count) {
-    CTX_CALL(ctx->ctx.get_state(pos, vel, forces, potential_energy, step_count));
+    CTX_CALL(ctx->ctx.get_state(0, pos, vel, forces, potential_energy, step_count));
 }
 
 omm_status omm_ctx_kinetic_energy(omm_ctx* ctx, double* kinetic_energy) {
-    CTX_CALL(if (!kinetic_energy) throw Error(OMM_ERR_INVALID, "null output"); *kinetic_energy = ctx->ctx.kinetic_energy());
+    CTX_CALL(if (!kinetic_energy) throw Error(OMM_ERR_INVALID, "null output"); *kinetic_energy = ctx->ctx.kinetic_energy(0));
 }
 
 omm_status omm_ctx_profile(omm_ctx* ctx, int32_t reps, double phase_ms[8], int64_t* num_pairs) {
@@ -204,6 +217,80 @@ omm_status omm_ctx_profile(omm_ctx* ctx, int32_t reps, double phase_ms[8], int64
 omm_status omm_ctx_destroy(omm_ctx* ctx) {
     return guarded([&]() -> omm_status {
         delete ctx;
+        return OMM_OK;
+    });
+}
+
+omm_status omm_batch_create(const omm_md_desc* replicas, int32_t count, int32_t precision, omm_batch** out) {
+    return guarded([&]() -> omm_status {
+        if (!out) throw Error(OMM_ERR_INVALID, "null output handle");
+        *out = nullptr;
+        if (!replicas || count <= 0) throw Error(OMM_ERR_INVALID, "a batch needs at least one replica");
+        *out = new omm_batch(replicas, count, precision);
+        return OMM_OK;
+    });
+}
+
+#define BATCH_CALL(body)                                                    \
+    return guarded([&]() -> omm_status {                                    \
+        if (!b) throw Error(OMM_ERR_INVALID, "null batch");                 \
+        std::lock_guard<std::mutex> lock(b->ctx.mutex);                     \
+        body;                                                               \
+        return OMM_OK;                                                      \
+    })
+
+omm_status omm_batch_size(omm_batch* b, int32_t* count) {
+    BATCH_CALL(if (!count) throw Error(OMM_ERR_INVALID, "null output"); *count = b->ctx.nsys());
+}
+
+omm_status omm_batch_set_positions(omm_batch* b, int32_t replica, const double* pos) {
+    BATCH_CALL(if (!pos) throw Error(OMM_ERR_INVALID, "null positions"); b->ctx.set_positions(replica, pos));
+}
+
+omm_status omm_batch_set_velocities(omm_batch* b, int32_t replica, const double* vel) {
+    BATCH_CALL(if (!vel) throw Error(OMM_ERR_INVALID, "null velocities"); b->ctx.set_velocities(replica, vel));
+}
+
+omm_status omm_batch_set_velocities_to_temperature(omm_batch* b, double temperature, uint64_t seed) {
+    BATCH_CALL(b->ctx.set_velocities_to_temperature(temperature, seed));
+}
+
+omm_status omm_batch_set_integrator(omm_batch* b, double temperature, double friction, double step_size) {
+    BATCH_CALL(b->ctx.set_integrator(temperature, friction, step_size));
+}
+
+omm_status omm_batch_step(omm_batch* b, int32_t nsteps) { BATCH_CALL(b->ctx.step(nsteps)); }
+
+omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
+                               double* potential_energy, int64_t* step_count) {
+    BATCH_CALL(b->ctx.get_state(replica, pos, vel, forces, potential_energy, step_count));
+}
+
+omm_status omm_batch_energies(omm_batch* b, double* potential_energies) {
+    BATCH_CALL(if (!potential_energies) throw Error(OMM_ERR_INVALID, "null output");
+               b->ctx.energies(potential_energies));
+}
+
+omm_status omm_batch_kinetic_energy(omm_batch* b, int32_t replica, double* kinetic_energy) {
+    BATCH_CALL(if (!kinetic_energy) throw Error(OMM_ERR_INVALID, "null output");
+               *kinetic_energy = b->ctx.kinetic_energy(replica));
+}
+
+omm_status omm_batch_get_box(omm_batch* b, int32_t replica, double box[9]) {
+    BATCH_CALL(if (!box) throw Error(OMM_ERR_INVALID, "null output"); b->ctx.get_box(replica, box));
+}
+
+omm_status omm_batch_list_stats(omm_batch* b, int64_t* tiles, int64_t* entries) {
+    BATCH_CALL(b->ctx.list_stats(tiles, entries));
+}
+
+omm_status omm_batch_profile(omm_batch* b, int32_t reps, double phase_ms[8], int64_t* num_pairs) {
+    BATCH_CALL(if (!phase_ms) throw Error(OMM_ERR_INVALID, "null output"); b->ctx.profile(reps, phase_ms, num_pairs));
+}
+
+omm_status omm_batch_destroy(omm_batch* b) {
+    return guarded([&]() -> omm_status {
+        delete b;
         return OMM_OK;
     });
 }

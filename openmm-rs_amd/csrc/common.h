@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -58,6 +59,48 @@ struct Box {
 };
 
 Box make_box(const double* box9);
+
+constexpr int MAX_STAGES = 12;  // FFT stages per axis (radices, 0-terminated)
+
+// One system of a batch as the kernels see it.  A batch concatenates the
+// particle arrays of its systems; every index the kernels exchange (atoms,
+// sorted slots, list blocks, cells, grid points) is global, and this record
+// gives the ranges that belong to system s:
+//   atoms   [atom_off, atom_off + n)
+//   slots   [slot_off, slot_off + 32 nblocks)   (slots past n hold -1)
+//   blocks  [block_off, block_off + nblocks)
+//   cells   [cell_off, cell_off + ncell[0] ncell[1] ncell[2])
+//   chunks  [chunk_off, chunk_off + ceil(32 nblocks / 64))  (PME spread/gather)
+//   grid    qgrid_off (real points), cgrid_off (complex half-spectrum points)
+// The box is device state: a barostat rescales it in place.
+struct SysInfo {
+    Box box;
+    int atom_off, n;
+    int slot_off, block_off, nblocks;
+    int cell_off, ncell[3];
+    int chunk_off;
+    int grid[3], nzh;
+    long long qgrid_off, cgrid_off;
+    int roots_off[3], moduli_off[3];
+    int fft_lines[2][3];  // lines per FFT workgroup along each axis, [0] FP32 / [1] FP64 transforms
+    int radix[3][MAX_STAGES];
+    double alpha;
+    // Volume-dependent host terms evaluated on the device:
+    // E_self = self_const + neutral_coeff / V, E_dispersion = disp_coeff / V.
+    double self_const, neutral_coeff, disp_coeff;
+    unsigned long long seed;  // integrator noise stream of this system
+};
+
+// Largest s in [0, count) with off[s] <= x, for x in [off[0], off[count]).
+__device__ __forceinline__ int find_sys(const int* __restrict__ off, int count, int x) {
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int m = (lo + hi) >> 1;
+        if (off[m] <= x) lo = m;
+        else hi = m;
+    }
+    return lo;
+}
 
 template <class T>
 struct DevBuf {
@@ -158,6 +201,39 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+// Raises the rebuild flags of a neighbour list for the lanes with moved set:
+// flag[0] (some system) once per wave and flag[1 + si] per system, with one
+// atomic when every moved lane of the wave belongs to the same system.
+// Called by all 64 lanes.
+__device__ __forceinline__ void wave_raise_flags(int* flag, bool moved, int si) {
+    const unsigned long long m = __ballot(moved);
+    if (!m) return;
+    const int first = __ffsll((long long)m) - 1;
+    const int lead = __shfl(si, first, 64);
+    const bool mixed = __ballot(moved && si != lead) != 0ull;
+    const int lane = threadIdx.x & 63;
+    if (lane == first) {
+        atomicOr(flag, 1);
+        atomicOr(flag + 1 + lead, 1);
+    }
+    if (mixed && moved && si != lead) atomicOr(flag + 1 + si, 1);
+}
+
+// Adds each lane's energy v to slot `slot` of its system si (si < 0: lane
+// idle) in the per-system energy array: one atomic per wave when the active
+// lanes share a system, one per lane otherwise.  Called by all 64 lanes.
+template <class A>
+__device__ __forceinline__ void wave_energy_add(A* energy, int si, int slot, double v, unsigned* status) {
+    const int lead = __shfl(si, 0, 64);  // lane 0 holds the lowest index, so it is active if any lane is
+    const bool uniform = __ballot(si >= 0 && si != lead) == 0ull;
+    if (uniform) {
+        const double t = wave_sum(v);
+        if ((threadIdx.x & 63) == 0 && lead >= 0 && t != 0.0) acc_add(energy + lead * E_SLOTS + slot, t, ENERGY_SCALE, status);
+    } else if (si >= 0 && v != 0.0) {
+        acc_add(energy + si * E_SLOTS + slot, v, ENERGY_SCALE, status);
+    }
 }
 
 // Reduced-triclinic minimum image in the reference's operation order

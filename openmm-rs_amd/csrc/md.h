@@ -1,4 +1,6 @@
-// MD context: the Context/LangevinMiddle step around the nonbonded operator.
+// MD contexts: the Context/LangevinMiddle step around the nonbonded operator
+// for one system or a batch of independent systems (replicas) advanced by the
+// same launches.
 #pragma once
 
 #include <memory>
@@ -9,14 +11,28 @@ namespace ommhip {
 
 class MdContext {
 public:
-    MdContext(const omm_md_desc* d, int precision);
+    // Replicas share the nonbonded method/cutoff/tolerance and the integrator
+    // settings; particles, topology, box and seed are per replica.
+    MdContext(const omm_md_desc* descs, int nsys, int precision);
     ~MdContext();
-    void set_positions(const double* pos);
-    void set_velocities(const double* vel);
+    int nsys() const { return nsys_; }
+    int num_particles(int r) const { return nb_->num_particles(r); }
+    void set_positions(int r, const double* pos);
+    void set_velocities(int r, const double* vel);
+    // Replica r draws from seed + r.
     void set_velocities_to_temperature(double temperature, uint64_t seed);
     void step(int nsteps);
-    void get_state(double* pos, double* vel, double* forces, double* energy, int64_t* step);
-    double kinetic_energy();
+    void set_integrator(double temperature, double friction, double step_size);
+    void get_state(int r, double* pos, double* vel, double* forces, double* energy, int64_t* step);
+    // Potential energy of every replica (nsys values).
+    void energies(double* out);
+    double kinetic_energy(int r);
+    void get_box(int r, double* box);
+    void list_stats(int64_t* tiles, int64_t* entries) {
+        OMM_HIP(hipStreamSynchronize(s_));
+        prepare();
+        nb_->list_stats(tiles, entries);
+    }
     // Per-phase average device times (ms): the six NbOperator::profile
     // phases, [6] bonds + angles, [7] LangevinMiddle update; plus the
     // in-cutoff pair count of the current positions.  State is preserved.
@@ -25,30 +41,37 @@ public:
     std::mutex mutex;
 
 private:
-    void enqueue_forces(hipStream_t s, bool energy);
-    void enqueue_step(hipStream_t s);
-    double evaluate(double* forces, bool energy);
+    void enqueue_forces(hipStream_t s, bool energy, int rebuild);
+    void enqueue_step(hipStream_t s, int rebuild);
+    void launch_bonded(hipStream_t s);
+    void clear_accumulators(hipStream_t s);
+    void capture_graph();
+    // Evaluates forces (all atoms, FP64 out, nullable) and the potential
+    // energy of every replica (nullable).
+    void evaluate(double* forces, double* energy, bool want_energy);
     void check_status();
     void drop_graph();
+    // Sizes the neighbour list for the current positions once they are all set.
+    void prepare();
 
     std::unique_ptr<NbOperator> nb_;
+    int nsys_ = 1;
     bool mixed_ = true;
-    int n_ = 0;
+    int n_ = 0;  // atoms over all replicas
     int nbonds_ = 0, nangles_ = 0;
+    bool positions_dirty_ = true;
     hipStream_t s_ = nullptr;
     std::vector<double> masses_;
-    Box box_h_{};
     double dt_ = 0.001, temperature_ = 0.0, friction_ = 0.0;
-    uint64_t seed_ = 0;
     int64_t step_count_ = 0;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
     DevBuf<int> b1_, b2_, a1_, a2_, a3_;
     DevBuf<double> blen_, bk_, ath_, ak_;
-    DevBuf<Box> box_;
     DevBuf<long long> force_fx_, energy_fx_;
     DevBuf<double> force_d_, energy_d_;
     DevBuf<unsigned> status_;
-    DevBuf<unsigned long long> step_dev_;
+    DevBuf<unsigned long long> step_dev_, snap_step_;
+    DevBuf<double> snap_pos_, snap_vel_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
 };

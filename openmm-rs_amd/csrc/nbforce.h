@@ -1,5 +1,6 @@
 // NonbondedForce operator on one gfx950 device: the host object behind the
-// omm_nb_* entry points and the nonbonded part of the MD context.
+// omm_nb_* entry points (one system) and the nonbonded part of the MD
+// contexts (one or many systems evaluated by the same launches).
 #pragma once
 
 #include <mutex>
@@ -38,70 +39,105 @@ struct HostTerms {
 
 class NbOperator {
 public:
-    NbOperator(const omm_nb_desc* desc, int precision);
+    // A batch of nsys systems.  Method, cutoff, tolerance, dielectric,
+    // dispersion correction, exception PBC, switching, force groups and
+    // direct-space flag must agree across the batch; particle and exception
+    // parameters, sizes and boxes are per system.
+    NbOperator(const omm_nb_desc* const* descs, int nsys, int precision);
     ~NbOperator();
 
-    // Force::initialize (nonbonded.rs:1055-1107).
-    void initialize(int n, const double* box);
+    // Force::initialize (nonbonded.rs:1055-1107) of every system.
+    void initialize(const int* n, const double* const* boxes);
+    void initialize(int n, const double* box) { initialize(&n, &box); }
     bool initialized() const { return initialized_; }
-    double alpha() const { return alpha_; }
-    const int* grid() const { return grid_; }
-    const NbSettings& settings() const { return cfg_; }
+    int nsys() const { return (int)cfg_.size(); }
+    double alpha(int s = 0) const { return alpha_[s]; }
+    const int* grid(int s = 0) const { return grid_[s].data(); }
+    const NbSettings& settings(int s = 0) const { return cfg_[s]; }
     bool mixed() const { return mixed_; }
-    int num_particles() const { return n_; }
+    int num_particles(int s = 0) const { return n_[s]; }
+    int total_particles() const { return ntotal_; }
+    int atom_offset(int s) const { return sys_h_[s].atom_off; }
 
-    // Host-pointer evaluation (calc_forces_and_energy[_groups]).  parts
-    // receives the energy split; stats the pair count and hash.
+    // Host-pointer evaluation of a one-system operator (calc_forces_and_energy
+    // [_groups]).  parts receives the energy split; stats the pair count and
+    // hash.
     void compute(const double* pos, const double* box, bool direct, bool recip, double* forces, double* energy,
                  double parts[5], int64_t* num_pairs, uint64_t* pair_hash);
 
-    // Device-resident enqueue for the MD step: neighbour-list check/rebuild,
-    // direct pairs, exceptions and PME into the caller's accumulators.  The
-    // box lives on the device at box_dev.  Graph-capturable.
-    // energy: also accumulate energies (slower, accurate path in the mixed form).
-    void enqueue(const double* pos_dev, const Box* box_dev, bool direct, bool recip, bool energy, long long* force_fx,
-                 double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s);
+    // Neighbour-list policy of one enqueued evaluation.
+    enum Rebuild : int {
+        REBUILD_CHECK = 0,   // displacement check, then a rebuild only if needed
+        REBUILD_FORCED = 1,  // rebuild unconditionally
+        REBUILD_NONE = 2,    // the caller knows the list is fresh
+        REBUILD_FLAG = 3     // rebuild if the list's flag is set (raised by the integrator)
+    };
+    // Device-resident enqueue for the MD step: neighbour-list rebuild per the
+    // policy, direct pairs, exceptions (unless the caller evaluates them
+    // itself) and PME into the caller's accumulators (forces over all atoms,
+    // energies E_SLOTS per system).  Graph-capturable.  energy: also
+    // accumulate energies (slower, accurate path in the mixed form).
+    void enqueue(const double* pos_dev, bool direct, bool recip, bool energy, int rebuild, bool exceptions,
+                 long long* force_fx, double* force_d, long long* energy_fx, double* energy_d, unsigned* status,
+                 hipStream_t s);
     // PME arithmetic for a force-only or an energy evaluation.
     int mesh_mode(bool energy) const;
+
+    // Device data the MD step's fused kernels use.
+    const ExceptionDev& exceptions() const { return exc_; }
+    const double* charge_dev() const { return part_.charge.p; }
+    const int* atom_sys_dev() const { return part_.atom_sys.p; }
+    bool exceptions_pbc() const { return cfg_[0].exceptions_pbc; }
+    bool has_pme() const { return cfg_[0].method == OMM_NB_PME; }
+    const double* pos_build() const { return nl_.pos_build.p; }
+    int* list_flag() { return nl_.flag.p; }
+    double skin() const { return nl_.skin; }
+    // System table: host copy and device table.  Boxes changed on the host
+    // are pushed with upload_systems().
+    std::vector<SysInfo>& systems() { return sys_h_; }
+    SysInfo* systems_dev() { return sys_dev_.p; }
+    void upload_systems(hipStream_t s);
 
     // Average device time (ms) of each phase over `reps` launches, measured
     // with HIP events on stream s: [0] neighbour-list rebuild (forced),
     // [1] direct pairs, [2] exceptions, [3] PME spread, [4] PME FFT passes +
     // convolution, [5] PME gather.  Accumulates into the given buffers.
-    void profile(const double* pos_dev, const Box* box_dev, int reps, long long* force_fx, double* force_d,
-                 long long* energy_fx, double* energy_d, unsigned* status, double ms[6], int64_t* num_pairs,
-                 hipStream_t s);
+    void profile(const double* pos_dev, int reps, long long* force_fx, double* force_d, long long* energy_fx,
+                 double* energy_d, unsigned* status, double ms[6], int64_t* num_pairs, hipStream_t s);
 
-    HostTerms host_terms(double volume, bool direct, bool recip) const;
-    void set_stream(hipStream_t s) { stream_ = s; }
+    HostTerms host_terms(int s, double volume, bool direct, bool recip) const;
     hipStream_t stream() const { return stream_; }
     // Rebuild capacity after a list overflow.
     void grow_list();
     // Builds the list for these positions and sizes its capacity to twice the
     // longest j-list (host-side, outside graph capture).  Returns true when the
     // list storage was reallocated (captured graphs must then be rebuilt).
-    bool calibrate(const double* pos_dev, const Box* box_dev, unsigned* status, hipStream_t s);
+    bool calibrate(const double* pos_dev, unsigned* status, hipStream_t s);
     void force_rebuild(hipStream_t s);
+    // Tile count and total j-list length of the current list (host read-back).
+    void list_stats(int64_t* tiles, int64_t* entries);
 
     std::mutex mutex;
 
 private:
     void check_box(const double* box) const;
-    double dispersion_coefficient() const;
+    double dispersion_coefficient(int s) const;
 
-    NbSettings cfg_;
+    std::vector<NbSettings> cfg_;
     bool mixed_ = true;
     bool initialized_ = false;
-    int n_ = 0;
-    double alpha_ = 0.0;
-    int grid_[3] = {0, 0, 0};
-    double sum_q2_ = 0.0, sum_q_ = 0.0;
-    mutable double disp_coeff_ = 0.0;
-    mutable bool disp_cached_ = false;
+    std::vector<int> n_;
+    int ntotal_ = 0;
+    std::vector<double> alpha_;
+    std::vector<std::array<int, 3>> grid_;
+    std::vector<double> sum_q2_, sum_q_;
+    mutable std::vector<double> disp_coeff_;
+    mutable std::vector<char> disp_cached_;
     int pair_grid_ = 1;
     hipStream_t stream_ = nullptr;
-    double box0_[9] = {0};
 
+    std::vector<SysInfo> sys_h_;
+    DevBuf<SysInfo> sys_dev_;
     ParticleDev part_;
     ExceptionDev exc_;
     NlistDev nl_;
@@ -110,7 +146,6 @@ private:
 
     // Host-API scratch.
     DevBuf<double> pos_dev_;
-    DevBuf<Box> box_dev_;
     DevBuf<long long> force_fx_, energy_fx_;
     DevBuf<double> force_d_, energy_d_;
     DevBuf<unsigned> status_;

@@ -134,12 +134,31 @@ NbSettings settings_from_desc(const omm_nb_desc* d) {
     return c;
 }
 
-NbOperator::NbOperator(const omm_nb_desc* desc, int precision) {
-    cfg_ = settings_from_desc(desc);
+NbOperator::NbOperator(const omm_nb_desc* const* descs, int nsys, int precision) {
+    if (!descs || nsys <= 0) throw Error(OMM_ERR_INVALID, "a NonbondedForce batch needs at least one descriptor");
     if (precision != OMM_PRECISION_MIXED && precision != OMM_PRECISION_DOUBLE)
         throw Error(OMM_ERR_INVALID, "precision must be OMM_PRECISION_MIXED or OMM_PRECISION_DOUBLE");
     mixed_ = precision == OMM_PRECISION_MIXED;
-    n_ = (int)cfg_.charge.size();
+    for (int s = 0; s < nsys; ++s) {
+        cfg_.push_back(settings_from_desc(descs[s]));
+        const NbSettings& a = cfg_[0];
+        const NbSettings& b = cfg_.back();
+        if (a.method != b.method || a.cutoff != b.cutoff || a.rf_dielectric != b.rf_dielectric ||
+            a.tolerance != b.tolerance || a.pme_alpha != b.pme_alpha ||
+            a.dispersion_correction != b.dispersion_correction || a.exceptions_pbc != b.exceptions_pbc ||
+            a.switching != b.switching || a.switching_distance != b.switching_distance ||
+            a.force_group != b.force_group || a.reciprocal_group != b.reciprocal_group ||
+            a.include_direct != b.include_direct)
+            throw Error(OMM_ERR_INVALID,
+                        "the systems of a batch must share the nonbonded method, cutoff, tolerance and options");
+        n_.push_back((int)b.charge.size());
+    }
+    alpha_.assign(nsys, 0.0);
+    grid_.assign(nsys, {0, 0, 0});
+    sum_q2_.assign(nsys, 0.0);
+    sum_q_.assign(nsys, 0.0);
+    disp_coeff_.assign(nsys, 0.0);
+    disp_cached_.assign(nsys, 0);
 }
 
 NbOperator::~NbOperator() {
@@ -147,78 +166,126 @@ NbOperator::~NbOperator() {
 }
 
 void NbOperator::check_box(const double* box) const {
-    if (!cfg_.periodic()) return;
+    if (!cfg_[0].periodic()) return;
     if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
     validate_box(box);
     // nonbonded.rs:806-812
     for (int a = 0; a < 3; ++a)
-        if (!(2.0 * cfg_.cutoff <= box[a * 4] + 1e-12))
+        if (!(2.0 * cfg_[0].cutoff <= box[a * 4] + 1e-12))
             throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
 }
 
-void NbOperator::initialize(int n, const double* box) {
-    // nonbonded.rs:1055-1107
-    if (n != (int)cfg_.charge.size())
-        throw Error(OMM_ERR_INVALID, "NonbondedForce must have one parameter record per particle");
-    std::set<std::pair<int, int>> pairs;
-    for (size_t e = 0; e < cfg_.exc1.size(); ++e) {
-        int a = cfg_.exc1[e], b = cfg_.exc2[e];
-        if (a < 0 || b < 0 || a >= n || b >= n) throw Error(OMM_ERR_INVALID, "exception particle index out of range");
-        if (!pairs.insert({std::min(a, b), std::max(a, b)}).second) throw Error(OMM_ERR_INVALID, "duplicate exception pair");
-    }
-    if (cfg_.switching && !(cfg_.switching_distance >= 0.0 && cfg_.switching_distance < cfg_.cutoff))
-        throw Error(OMM_ERR_INVALID, "switching distance must be between zero and cutoff");
-    if (cfg_.method != OMM_NB_PME && cfg_.method != OMM_NB_CUTOFF_PERIODIC)
-        throw Error(OMM_ERR_INVALID,
-                    "the device operator implements the PME and CutoffPeriodic methods; "
-                    "NoCutoff, CutoffNonPeriodic, Ewald and LJPME are not available on the device yet");
-    if (cfg_.switching) throw Error(OMM_ERR_INVALID, "the switching function is not available on the device yet");
-    if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
-    validate_box(box);
-    for (int a = 0; a < 3; ++a)
-        if (!(cfg_.cutoff <= 0.5 * box[a * 4]))
-            throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
-    if (cfg_.method == OMM_NB_PME) {
-        if (cfg_.pme_alpha > 0.0) {
-            alpha_ = cfg_.pme_alpha;
-            for (int a = 0; a < 3; ++a) grid_[a] = cfg_.pme_grid[a];
-        } else {
-            // nonbonded.rs:647-658
-            alpha_ = std::sqrt(-std::log(2.0 * cfg_.tolerance)) / cfg_.cutoff;
-            for (int a = 0; a < 3; ++a) {
-                double v = std::ceil(2.0 * alpha_ * box[a * 4] / (3.0 * std::pow(cfg_.tolerance, 0.2)));
-                grid_[a] = (int)std::max(v, 6.0);
+void NbOperator::initialize(const int* nlist, const double* const* boxes) {
+    const int nsys = (int)cfg_.size();
+    ntotal_ = 0;
+    for (int si = 0; si < nsys; ++si) {
+        NbSettings& c = cfg_[si];
+        const int n = nlist[si];
+        const double* box = boxes[si];
+        // nonbonded.rs:1055-1107
+        if (n != (int)c.charge.size())
+            throw Error(OMM_ERR_INVALID, "NonbondedForce must have one parameter record per particle");
+        std::set<std::pair<int, int>> pairs;
+        for (size_t e = 0; e < c.exc1.size(); ++e) {
+            int a = c.exc1[e], b = c.exc2[e];
+            if (a < 0 || b < 0 || a >= n || b >= n)
+                throw Error(OMM_ERR_INVALID, "exception particle index out of range");
+            if (!pairs.insert({std::min(a, b), std::max(a, b)}).second)
+                throw Error(OMM_ERR_INVALID, "duplicate exception pair");
+        }
+        if (c.switching && !(c.switching_distance >= 0.0 && c.switching_distance < c.cutoff))
+            throw Error(OMM_ERR_INVALID, "switching distance must be between zero and cutoff");
+        if (c.method != OMM_NB_PME && c.method != OMM_NB_CUTOFF_PERIODIC)
+            throw Error(OMM_ERR_INVALID,
+                        "the device operator implements the PME and CutoffPeriodic methods; "
+                        "NoCutoff, CutoffNonPeriodic, Ewald and LJPME are not available on the device yet");
+        if (c.switching) throw Error(OMM_ERR_INVALID, "the switching function is not available on the device yet");
+        if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
+        validate_box(box);
+        for (int a = 0; a < 3; ++a)
+            if (!(c.cutoff <= 0.5 * box[a * 4]))
+                throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
+        if (c.method == OMM_NB_PME) {
+            if (c.pme_alpha > 0.0) {
+                alpha_[si] = c.pme_alpha;
+                for (int a = 0; a < 3; ++a) grid_[si][a] = c.pme_grid[a];
+            } else {
+                // nonbonded.rs:647-658
+                alpha_[si] = std::sqrt(-std::log(2.0 * c.tolerance)) / c.cutoff;
+                for (int a = 0; a < 3; ++a) {
+                    double v = std::ceil(2.0 * alpha_[si] * box[a * 4] / (3.0 * std::pow(c.tolerance, 0.2)));
+                    grid_[si][a] = (int)std::max(v, 6.0);
+                }
             }
         }
+        sum_q2_[si] = 0.0;
+        sum_q_[si] = 0.0;
+        for (double q : c.charge) sum_q2_[si] += q * q;
+        for (double q : c.charge) sum_q_[si] += q;
+        n_[si] = n;
+        ntotal_ += n;
     }
-    for (int i = 0; i < 9; ++i) box0_[i] = box[i];
-    sum_q2_ = 0.0;
-    sum_q_ = 0.0;
-    for (double q : cfg_.charge) sum_q2_ += q * q;
-    for (double q : cfg_.charge) sum_q_ += q;
-    n_ = n;
-    if (n == 0) {
+    if (nsys == 1 && n_[0] == 0) {
         initialized_ = true;
         return;
     }
+    for (int si = 0; si < nsys; ++si)
+        if (alpha_[si] != alpha_[0])
+            throw Error(OMM_ERR_INVALID, "the systems of a batch must share the Ewald alpha");
 
     int count = 0;
     OMM_HIP(hipGetDeviceCount(&count));
     if (count <= 0) throw Error(OMM_ERR_DEVICE, "no HIP device available");
     if (!stream_) OMM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     hipStream_t s = stream_;
-    std::vector<float> qk(n), sh(n), e2(n);
-    std::vector<double> qkd(n), shd(n), e2d(n);
+    const int N = ntotal_;
+    std::vector<float> qk(N), sh(N), e2(N);
+    std::vector<double> charge(N), qkd(N), shd(N), e2d(N);
+    std::vector<int> atom_sys(N);
+    std::vector<int> p1, p2;
+    std::vector<double> eqq, esig, eeps;
     const double sk = std::sqrt(ONE_4PI_EPS0);
-    for (int i = 0; i < n; ++i) {
-        qkd[i] = cfg_.charge[i] * sk;
-        shd[i] = 0.5 * cfg_.sigma[i];
-        e2d[i] = 2.0 * std::sqrt(cfg_.epsilon[i]);
-        qk[i] = (float)qkd[i];
-        sh[i] = (float)shd[i];
-        e2[i] = (float)e2d[i];
+    sys_h_.assign(nsys, SysInfo{});
+    std::vector<NlistSystem> nls(nsys);
+    int off = 0;
+    for (int si = 0; si < nsys; ++si) {
+        const NbSettings& c = cfg_[si];
+        for (int i = 0; i < n_[si]; ++i) {
+            const int g = off + i;
+            charge[g] = c.charge[i];
+            qkd[g] = c.charge[i] * sk;
+            shd[g] = 0.5 * c.sigma[i];
+            e2d[g] = 2.0 * std::sqrt(c.epsilon[i]);
+            qk[g] = (float)qkd[g];
+            sh[g] = (float)shd[g];
+            e2[g] = (float)e2d[g];
+            atom_sys[g] = si;
+        }
+        for (size_t e = 0; e < c.exc1.size(); ++e) {
+            p1.push_back(off + c.exc1[e]);
+            p2.push_back(off + c.exc2[e]);
+            eqq.push_back(c.exc_qq[e]);
+            esig.push_back(c.exc_sig[e]);
+            eeps.push_back(c.exc_eps[e]);
+        }
+        SysInfo& S = sys_h_[si];
+        S.box = make_box(boxes[si]);
+        S.alpha = alpha_[si];
+        for (int a = 0; a < 3; ++a) S.grid[a] = grid_[si][a];
+        if (c.method == OMM_NB_PME) {
+            S.self_const = -ONE_4PI_EPS0 * alpha_[si] / std::sqrt(PI) * sum_q2_[si];
+            S.neutral_coeff = -ONE_4PI_EPS0 * PI * sum_q_[si] * sum_q_[si] / (2.0 * alpha_[si] * alpha_[si]);
+        }
+        if (c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
+            S.disp_coeff = dispersion_coefficient(si);
+        nls[si].n = n_[si];
+        nls[si].exc1 = &c.exc1;
+        nls[si].exc2 = &c.exc2;
+        nls[si].box = S.box;
+        off += n_[si];
     }
-    part_.charge.upload(cfg_.charge, s);
+    part_.atom_sys.upload(atom_sys, s);
+    part_.charge.upload(charge, s);
     if (mixed_) {
         part_.qk.upload(qk, s);
         part_.sh.upload(sh, s);
@@ -227,30 +294,30 @@ void NbOperator::initialize(int n, const double* box) {
     part_.qk_d.upload(qkd, s);
     part_.sh_d.upload(shd, s);
     part_.e2_d.upload(e2d, s);
-    exc_.count = (int)cfg_.exc1.size();
-    exc_.p1.upload(cfg_.exc1, s);
-    exc_.p2.upload(cfg_.exc2, s);
-    exc_.qq.upload(cfg_.exc_qq, s);
-    exc_.sig.upload(cfg_.exc_sig, s);
-    exc_.eps.upload(cfg_.exc_eps, s);
+    exc_.count = (int)p1.size();
+    exc_.p1.upload(p1, s);
+    exc_.p2.upload(p2, s);
+    exc_.qq.upload(eqq, s);
+    exc_.sig.upload(esig, s);
+    exc_.eps.upload(eeps, s);
 
-    Box b = make_box(box);
-    nlist_setup(nl_, n, cfg_.exc1, cfg_.exc2, cfg_.cutoff, 0.1, b, s);
-    if (cfg_.method == OMM_NB_PME) pme_setup(pme_, alpha_, grid_, mixed_, s);
+    const NbSettings& c0 = cfg_[0];
+    nlist_setup(nl_, nls, sys_h_, c0.cutoff, 0.1, s);
+    if (c0.method == OMM_NB_PME) pme_setup(pme_, sys_h_, mixed_, s);
+    sys_dev_.upload(sys_h_, s);
 
     pp_ = PairParams{};
-    pp_.n = n;
     pp_.nblocks = nl_.nblocks;
     pp_.cap = nl_.cap;
     pp_.periodic = 1;
     pp_.use_cutoff = 1;
-    pp_.cutoff = cfg_.cutoff;
-    pp_.alpha = alpha_;
-    if (cfg_.method == OMM_NB_PME) {
+    pp_.cutoff = c0.cutoff;
+    pp_.alpha = alpha_[0];
+    if (c0.method == OMM_NB_PME) {
         pp_.elec = ELEC_EWALD;
     } else {
         pp_.elec = ELEC_REACTION_FIELD;
-        double rc = cfg_.cutoff, eps = cfg_.rf_dielectric;
+        double rc = c0.cutoff, eps = c0.rf_dielectric;
         if (std::isinf(eps)) {
             pp_.krf = 0.5 / (rc * rc * rc);
             pp_.crf = 1.5 / rc;
@@ -259,35 +326,39 @@ void NbOperator::initialize(int n, const double* box) {
             pp_.crf = (3.0 * eps) / (2.0 * eps + 1.0) / rc;
         }
     }
-    pair_grid_ = std::max(64, std::min(4096, nl_.nblocks * 4));
+    pair_grid_ = std::max(64, std::min(8192, nl_.nblocks * 4));
 
-    pos_dev_.alloc(3 * (size_t)n);
-    box_dev_.alloc(1);
-    if (mixed_) {
-        force_fx_.alloc(3 * (size_t)n);
-        energy_fx_.alloc(E_SLOTS);
-    } else {
-        force_d_.alloc(3 * (size_t)n);
-        energy_d_.alloc(E_SLOTS);
+    if (nsys == 1) {
+        pos_dev_.alloc(3 * (size_t)N);
+        if (mixed_) {
+            force_fx_.alloc(3 * (size_t)N);
+            energy_fx_.alloc(E_SLOTS);
+        } else {
+            force_d_.alloc(3 * (size_t)N);
+            energy_d_.alloc(E_SLOTS);
+        }
+        status_.alloc(1);
+        stats_.alloc(2);
+        out_forces_.alloc(3 * (size_t)N);
     }
-    status_.alloc(1);
-    stats_.alloc(2);
-    out_forces_.alloc(3 * (size_t)n);
     OMM_HIP(hipStreamSynchronize(s));
     have_box_ = false;
     initialized_ = true;
 }
 
-double NbOperator::dispersion_coefficient() const {
+void NbOperator::upload_systems(hipStream_t s) { sys_dev_.upload(sys_h_, s); }
+
+double NbOperator::dispersion_coefficient(int si) const {
     // nonbonded.rs:660-719
-    if (disp_cached_) return disp_coeff_;
+    if (disp_cached_[si]) return disp_coeff_[si];
+    const NbSettings& c = cfg_[si];
     double moments[13] = {0}, diagonal[13] = {0};
-    for (size_t i = 0; i < cfg_.sigma.size(); ++i) {
+    for (size_t i = 0; i < c.sigma.size(); ++i) {
         double power = 1.0;
         for (int k = 0; k <= 12; ++k) {
-            moments[k] += std::sqrt(cfg_.epsilon[i]) * power;
-            diagonal[k] += cfg_.epsilon[i] * power;
-            power *= cfg_.sigma[i];
+            moments[k] += std::sqrt(c.epsilon[i]) * power;
+            diagonal[k] += c.epsilon[i] * power;
+            power *= c.sigma[i];
         }
     }
     auto pair_moment = [&](int p) {
@@ -299,9 +370,9 @@ double NbOperator::dispersion_coefficient() const {
         return sum / std::pow(2.0, p) + diagonal[p];
     };
     auto integral = [&](int p) {
-        double tail = std::pow(cfg_.cutoff, 3 - p) / (double)(p - 3);
-        if (!cfg_.switching) return tail;
-        double sw = cfg_.switching_distance, rc = cfg_.cutoff;
+        double tail = std::pow(c.cutoff, 3 - p) / (double)(p - 3);
+        if (!c.switching) return tail;
+        double sw = c.switching_distance, rc = c.cutoff;
         auto f = [&](double r) {
             double t = (r - sw) / (rc - sw);
             return std::pow(t, 3) * (10.0 + t * (-15.0 + 6.0 * t)) * std::pow(r, 2 - p);
@@ -310,35 +381,37 @@ double NbOperator::dispersion_coefficient() const {
         double est = (rc - sw) / 6.0 * (v0 + 4.0 * v1 + v2);
         return tail + simpson(f, sw, rc, v0, v1, v2, est, 1e-11 * std::max(std::fabs(est), 1e-12), 20);
     };
-    double n = (double)cfg_.sigma.size();
-    disp_coeff_ = 8.0 * PI * n / (n + 1.0) * (pair_moment(12) * integral(12) - pair_moment(6) * integral(6));
-    disp_cached_ = true;
-    return disp_coeff_;
+    double n = (double)c.sigma.size();
+    disp_coeff_[si] = 8.0 * PI * n / (n + 1.0) * (pair_moment(12) * integral(12) - pair_moment(6) * integral(6));
+    disp_cached_[si] = 1;
+    return disp_coeff_[si];
 }
 
-HostTerms NbOperator::host_terms(double volume, bool direct, bool recip) const {
+HostTerms NbOperator::host_terms(int si, double volume, bool direct, bool recip) const {
     HostTerms t;
-    if (recip && cfg_.method == OMM_NB_PME)
-        t.self = -ONE_4PI_EPS0 * (alpha_ / std::sqrt(PI) * sum_q2_ + PI * sum_q_ * sum_q_ / (2.0 * alpha_ * alpha_ * volume));
-    if (direct && cfg_.periodic() && cfg_.dispersion_correction && cfg_.method != OMM_NB_LJPME)
-        t.dispersion = dispersion_coefficient() / volume;
+    const NbSettings& c = cfg_[si];
+    const double a = alpha_[si];
+    if (recip && c.method == OMM_NB_PME)
+        t.self = -ONE_4PI_EPS0 * (a / std::sqrt(PI) * sum_q2_[si] + PI * sum_q_[si] * sum_q_[si] / (2.0 * a * a * volume));
+    if (direct && c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
+        t.dispersion = dispersion_coefficient(si) / volume;
     return t;
 }
 
 void NbOperator::grow_list() {
-    nl_.cap = std::min(nl_.npad + JTILE, nl_.cap * 2);
-    nl_.cap = (nl_.cap + JTILE - 1) / JTILE * JTILE;
-    nl_.jlist.alloc((size_t)nl_.nblocks * nl_.cap);
+    int cap = std::min(nl_.nslots + JTILE, nl_.cap * 2);
+    nlist_resize(nl_, (cap + JTILE - 1) / JTILE * JTILE);
     pp_.cap = nl_.cap;
 }
 
-bool NbOperator::calibrate(const double* pos_dev, const Box* box_dev, unsigned* status, hipStream_t s) {
-    if (n_ == 0 || !cfg_.include_direct) return false;
+bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t s) {
+    if (ntotal_ == 0 || !cfg_[0].include_direct) return false;
+    int max_npad = 0;
+    for (const SysInfo& S : sys_h_) max_npad = std::max(max_npad, S.nblocks * IBLOCK);
     bool changed = false;
     for (int attempt = 0; attempt < 8; ++attempt) {
         fill_async<unsigned>(status, 1, 0u, s);
-        nlist_check(nl_, pos_dev, box_dev, 1, s);
-        nlist_build(nl_, pos_dev, box_dev, status, s);
+        nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
         std::vector<int> counts(nl_.nblocks);
         unsigned st = 0;
         nl_.jcount.download(counts.data(), counts.size(), s);
@@ -346,12 +419,11 @@ bool NbOperator::calibrate(const double* pos_dev, const Box* box_dev, unsigned* 
         OMM_HIP(hipStreamSynchronize(s));
         int mx = 0;
         for (int c : counts) mx = std::max(mx, c);
-        long long want = std::min<long long>((long long)nl_.npad + JTILE, 2LL * mx + 2 * JTILE);
+        long long want = std::min<long long>((long long)max_npad + JTILE, 2LL * mx + 2 * JTILE);
         want = (want + JTILE - 1) / JTILE * JTILE;
         if (!(st & ST_LIST_OVERFLOW) && want <= nl_.cap) break;
-        nl_.cap = (int)std::max<long long>(want, 2LL * nl_.cap);
-        nl_.cap = std::min(nl_.cap, (nl_.npad + 2 * JTILE) / JTILE * JTILE);
-        nl_.jlist.alloc((size_t)nl_.nblocks * nl_.cap);
+        int cap = (int)std::max<long long>(want, 2LL * nl_.cap);
+        nlist_resize(nl_, std::min(cap, (max_npad + 2 * JTILE) / JTILE * JTILE));
         pp_.cap = nl_.cap;
         changed = true;
     }
@@ -360,35 +432,57 @@ bool NbOperator::calibrate(const double* pos_dev, const Box* box_dev, unsigned* 
     return changed;
 }
 
-void NbOperator::force_rebuild(hipStream_t s) {
-    std::vector<int> one{1};
-    nl_.flag.upload(one, s);
+void NbOperator::list_stats(int64_t* tiles, int64_t* entries) {
+    if (ntotal_ == 0) {
+        if (tiles) *tiles = 0;
+        if (entries) *entries = 0;
+        return;
+    }
+    std::vector<int> counts(nl_.nblocks);
+    int total = 0;
+    nl_.jcount.download(counts.data(), counts.size(), stream_);
+    OMM_HIP(hipMemcpyAsync(&total, nl_.task_start.p + nl_.nblocks, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    OMM_HIP(hipStreamSynchronize(stream_));
+    int64_t e = 0;
+    for (int c : counts) e += c;
+    if (tiles) *tiles = total;
+    if (entries) *entries = e;
 }
+
+void NbOperator::force_rebuild(hipStream_t s) { fill_async<int>(nl_.flag.p, (size_t)nl_.nsys + 1, 1, s); }
 
 int NbOperator::mesh_mode(bool energy) const {
     if (!mixed_) return MESH_F64;
     return energy ? MESH_F64_FIXED : MESH_F32;
 }
 
-void NbOperator::enqueue(const double* pos_dev, const Box* box_dev, bool direct, bool recip, bool energy,
+void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool energy, int rebuild, bool exceptions,
                          long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
                          unsigned* status, hipStream_t s) {
+    const NbSettings& c = cfg_[0];
     if (direct) {
-        nlist_check(nl_, pos_dev, box_dev, 0, s);
-        nlist_build(nl_, pos_dev, box_dev, status, s);
-        launch_pairs(mixed_, energy, nl_, pp_, part_, pos_dev, box_dev, force_fx, force_d, energy_fx, energy_d, status,
-                     nullptr, pair_grid_, s);
-        launch_exceptions(mixed_, exc_, part_, pos_dev, box_dev, cfg_.exceptions_pbc ? 1 : 0,
-                          cfg_.method == OMM_NB_PME ? 1 : 0, alpha_, force_fx, force_d, energy_fx, energy_d, status, s);
+        if (rebuild == REBUILD_CHECK) {
+            nlist_check(nl_, pos_dev, 0, s);
+            nlist_build(nl_, pos_dev, sys_dev_.p, status, s, false);
+        } else if (rebuild == REBUILD_FLAG) {
+            nlist_build(nl_, pos_dev, sys_dev_.p, status, s, false);
+        } else if (rebuild == REBUILD_FORCED) {
+            nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
+        }
+        launch_pairs(mixed_, energy, nl_, pp_, part_, pos_dev, sys_dev_.p, force_fx, force_d, energy_fx, energy_d,
+                     status, nullptr, pair_grid_, s);
+        if (exceptions)
+            launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
+                              c.method == OMM_NB_PME ? 1 : 0, alpha_[0], force_fx, force_d, energy_fx, energy_d,
+                              status, s);
     }
-    if (recip && cfg_.method == OMM_NB_PME)
-        pme_run(pme_, mesh_mode(energy), n_, pos_dev, part_.charge.p, box_dev, force_fx, force_d, energy_fx, energy_d,
-                status, s);
+    if (recip && c.method == OMM_NB_PME)
+        pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
+                energy_d, status, s);
 }
 
-void NbOperator::profile(const double* pos_dev, const Box* box_dev, int reps, long long* force_fx, double* force_d,
-                         long long* energy_fx, double* energy_d, unsigned* status, double ms[6], int64_t* num_pairs,
-                         hipStream_t s) {
+void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, double* force_d, long long* energy_fx,
+                         double* energy_d, unsigned* status, double ms[6], int64_t* num_pairs, hipStream_t s) {
     hipEvent_t t0, t1;
     OMM_HIP(hipEventCreate(&t0));
     OMM_HIP(hipEventCreate(&t1));
@@ -402,34 +496,36 @@ void NbOperator::profile(const double* pos_dev, const Box* box_dev, int reps, lo
         OMM_HIP(hipEventElapsedTime(&v, t0, t1));
         return (double)v / reps;
     };
-    const bool pme = cfg_.method == OMM_NB_PME;
-    ms[0] = timed([&] {
-        nlist_check(nl_, pos_dev, box_dev, 1, s);
-        nlist_build(nl_, pos_dev, box_dev, status, s);
-    });
+    const NbSettings& c = cfg_[0];
+    const bool pme = c.method == OMM_NB_PME;
+    ms[0] = timed([&] { nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true); });
     ms[1] = timed([&] {
-        launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, box_dev, force_fx, force_d, energy_fx, energy_d, status,
-                     nullptr, pair_grid_, s);
+        launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, sys_dev_.p, force_fx, force_d, energy_fx, energy_d,
+                     status, nullptr, pair_grid_, s);
     });
     ms[2] = timed([&] {
-        launch_exceptions(mixed_, exc_, part_, pos_dev, box_dev, cfg_.exceptions_pbc ? 1 : 0, pme ? 1 : 0, alpha_,
+        launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0, pme ? 1 : 0, alpha_[0],
                           force_fx, force_d, energy_fx, energy_d, status, s);
     });
     const int phases[3] = {PME_SPREAD, PME_FFT, PME_GATHER};
     for (int k = 0; k < 3; ++k)
         ms[3 + k] = pme ? timed([&] {
-            pme_run(pme_, mesh_mode(false), n_, pos_dev, part_.charge.p, box_dev, force_fx, force_d, energy_fx,
+            pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
                     energy_d, status, s, phases[k]);
         })
                         : 0.0;
     if (num_pairs) {
-        fill_async<unsigned long long>(stats_.p, 2, 0ull, s);
-        launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, box_dev, force_fx, force_d, energy_fx, energy_d, status,
-                     stats_.p, pair_grid_, s);
-        unsigned long long st[2];
-        stats_.download(st, 2, s);
+        DevBuf<unsigned long long> st_dev;
+        st_dev.alloc(2 * sys_h_.size());
+        fill_async<unsigned long long>(st_dev.p, 2 * sys_h_.size(), 0ull, s);
+        launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, sys_dev_.p, force_fx, force_d, energy_fx, energy_d,
+                     status, st_dev.p, pair_grid_, s);
+        std::vector<unsigned long long> st(2 * sys_h_.size());
+        st_dev.download(st.data(), st.size(), s);
         OMM_HIP(hipStreamSynchronize(s));
-        *num_pairs = (int64_t)st[0];
+        int64_t total = 0;
+        for (size_t k = 0; k < sys_h_.size(); ++k) total += (int64_t)st[2 * k];
+        *num_pairs = total;
     }
     OMM_HIP(hipEventDestroy(t0));
     OMM_HIP(hipEventDestroy(t1));
@@ -438,13 +534,15 @@ void NbOperator::profile(const double* pos_dev, const Box* box_dev, int reps, lo
 void NbOperator::compute(const double* pos, const double* box, bool direct, bool recip, double* forces,
                          double* energy, double parts[5], int64_t* num_pairs, uint64_t* pair_hash) {
     if (!initialized_) throw Error(OMM_ERR_STATE, "NonbondedForce used before initialize");
+    if (cfg_.size() != 1) throw Error(OMM_ERR_STATE, "host-pointer evaluation needs a one-system operator");
     if (!pos) throw Error(OMM_ERR_INVALID, "positions must not be null");
     check_box(box);
-    for (int i = 0; i < 3 * n_; ++i)
+    const int n = n_[0];
+    for (int i = 0; i < 3 * n; ++i)
         if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
     hipStream_t s = stream_;
     Box b = make_box(box);
-    if (n_ == 0) {
+    if (n == 0) {
         if (energy) *energy = 0.0;
         if (parts)
             for (int k = 0; k < 5; ++k) parts[k] = 0.0;
@@ -454,37 +552,39 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
     }
     bool box_changed = !have_box_ || std::memcmp(b.v, last_box_.v, sizeof(b.v)) != 0;
     if (box_changed) {
-        box_dev_.upload(&b, 1, s);
+        sys_h_[0].box = b;
+        upload_systems(s);
         last_box_ = b;
         have_box_ = true;
     }
-    pos_dev_.upload(pos, 3 * (size_t)n_, s);
+    pos_dev_.upload(pos, 3 * (size_t)n, s);
     const bool want_stats = num_pairs || pair_hash;
     // Pair statistics are gathered by a force-only pass; energies by the
     // energy pass (FP64 pair energies and FP64 mesh in the mixed form).
     const bool want_energy = !want_stats && (energy || parts);
+    const NbSettings& c = cfg_[0];
     for (int attempt = 0; attempt < 6; ++attempt) {
         fill_async<unsigned>(status_.p, 1, 0u, s);
         if (want_stats) fill_async<unsigned long long>(stats_.p, 2, 0ull, s);
         if (mixed_) {
-            fill_async<long long>(force_fx_.p, 3 * (size_t)n_, 0ll, s);
+            fill_async<long long>(force_fx_.p, 3 * (size_t)n, 0ll, s);
             fill_async<long long>(energy_fx_.p, E_SLOTS, 0ll, s);
         } else {
-            fill_async<double>(force_d_.p, 3 * (size_t)n_, 0.0, s);
+            fill_async<double>(force_d_.p, 3 * (size_t)n, 0.0, s);
             fill_async<double>(energy_d_.p, E_SLOTS, 0.0, s);
         }
         if (direct) {
-            nlist_check(nl_, pos_dev_.p, box_dev_.p, (box_changed || attempt > 0) ? 1 : 0, s);
-            nlist_build(nl_, pos_dev_.p, box_dev_.p, status_.p, s);
-            launch_pairs(mixed_, want_energy, nl_, pp_, part_, pos_dev_.p, box_dev_.p, force_fx_.p, force_d_.p,
+            nlist_check(nl_, pos_dev_.p, (box_changed || attempt > 0) ? 1 : 0, s);
+            nlist_build(nl_, pos_dev_.p, sys_dev_.p, status_.p, s);
+            launch_pairs(mixed_, want_energy, nl_, pp_, part_, pos_dev_.p, sys_dev_.p, force_fx_.p, force_d_.p,
                          energy_fx_.p, energy_d_.p, status_.p, want_stats ? stats_.p : nullptr, pair_grid_, s);
-            launch_exceptions(mixed_, exc_, part_, pos_dev_.p, box_dev_.p, cfg_.exceptions_pbc ? 1 : 0,
-                              cfg_.method == OMM_NB_PME ? 1 : 0, alpha_, force_fx_.p, force_d_.p, energy_fx_.p,
+            launch_exceptions(mixed_, exc_, part_, pos_dev_.p, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
+                              c.method == OMM_NB_PME ? 1 : 0, alpha_[0], force_fx_.p, force_d_.p, energy_fx_.p,
                               energy_d_.p, status_.p, s);
         }
-        if (recip && cfg_.method == OMM_NB_PME)
-            pme_run(pme_, mesh_mode(want_energy), n_, pos_dev_.p, part_.charge.p, box_dev_.p, force_fx_.p, force_d_.p,
-                    energy_fx_.p, energy_d_.p, status_.p, s);
+        if (recip && c.method == OMM_NB_PME)
+            pme_run(pme_, mesh_mode(want_energy), nl_, pos_dev_.p, part_.charge.p, sys_dev_.p, force_fx_.p,
+                    force_d_.p, energy_fx_.p, energy_d_.p, status_.p, s);
         unsigned st = 0;
         status_.download(&st, 1, s);
         OMM_HIP(hipStreamSynchronize(s));
@@ -498,13 +598,13 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
         if (mixed_) {
             long long efx[E_SLOTS];
             energy_fx_.download(efx, E_SLOTS, s);
-            launch_fixed_to_double(force_fx_.p, out_forces_.p, 3 * (size_t)n_, s);
-            if (forces) out_forces_.download(forces, 3 * (size_t)n_, s);
+            launch_fixed_to_double(force_fx_.p, out_forces_.p, 3 * (size_t)n, s);
+            if (forces) out_forces_.download(forces, 3 * (size_t)n, s);
             OMM_HIP(hipStreamSynchronize(s));
             for (int k = 0; k < E_SLOTS; ++k) e[k] = (double)efx[k] * (1.0 / ENERGY_SCALE);
         } else {
             energy_d_.download(e, E_SLOTS, s);
-            if (forces) force_d_.download(forces, 3 * (size_t)n_, s);
+            if (forces) force_d_.download(forces, 3 * (size_t)n, s);
             OMM_HIP(hipStreamSynchronize(s));
         }
         if (want_stats) {
@@ -514,7 +614,7 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
             if (num_pairs) *num_pairs = (int64_t)st2[0];
             if (pair_hash) *pair_hash = st2[1];
         }
-        HostTerms ht = host_terms(b.volume, direct, recip);
+        HostTerms ht = host_terms(0, b.volume, direct, recip);
         double p[5] = {e[E_DIRECT], e[E_EXCEPT], e[E_RECIP], ht.self, ht.dispersion};
         if (parts)
             for (int k = 0; k < 5; ++k) parts[k] = p[k];

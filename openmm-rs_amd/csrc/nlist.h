@@ -1,15 +1,18 @@
-// Tile neighbour list of the direct-space pair kernel.
+// Tile neighbour list of the direct-space pair kernel, for a batch of
+// independent systems built by one set of launches.
 //
 // Atoms are grouped into sort units (a small exception-connected molecule, or
 // a single atom), binned by the fractional coordinates of the unit's first
-// atom on a cell grid of ~0.35 nm, and counting-sorted in Morton order of the
-// cells; ties inside a cell are broken by unit index so the order (and hence
-// every FP32 partial sum downstream) is deterministic.  Consecutive runs of
-// IBLOCK sorted atoms form blocks with an FP64 centre and FP32 half-extents.
-// For every block I the list holds, in increasing order, the sorted indices
-// of the atoms of blocks J >= I within cutoff + skin of I's bounding box
-// (64-lane ballot + mbcnt compaction).  Pairs inside a list are kept with
-// j_sorted > i_sorted, so each unordered pair is visited once.
+// atom on a per-system cell grid of ~0.35 nm, and counting-sorted by
+// (system, Morton rank of the cell); ties inside a cell are broken by unit
+// index so the order (and hence every FP32 partial sum downstream) is
+// deterministic and the same as for the system built alone.  Consecutive runs
+// of IBLOCK sorted atoms of one system form blocks with an FP64 centre and
+// FP32 half-extents.  For every block I the list holds, in increasing order,
+// the sorted slots of the atoms of blocks J >= I of the same system within
+// cutoff + skin of I's bounding box (64-lane ballot + popcount compaction).
+// Pairs inside a list are kept with j_slot > i_slot, so each unordered pair is
+// visited once.
 #pragma once
 
 #include "common.h"
@@ -19,31 +22,55 @@ namespace ommhip {
 constexpr int IBLOCK = 32;  // i-atoms per tile (two j half-tiles per wave64)
 constexpr int JTILE = 64;   // j-slots per tile
 constexpr int MAXEX = 8;    // exclusion partners per atom handled by the pair kernel
+constexpr int PME_CHUNK = 64;  // sorted slots per PME spread/gather workgroup
+
+// Host description of one system for the list set-up (local atom indices).
+struct NlistSystem {
+    int n = 0;
+    const std::vector<int>* exc1 = nullptr;
+    const std::vector<int>* exc2 = nullptr;
+    Box box{};
+};
 
 struct NlistDev {
-    int n = 0, npad = 0, nblocks = 0, cap = 0, nunits = 0, ncells = 0;
-    int ncell[3] = {1, 1, 1};
+    int nsys = 0;
+    int natoms = 0, nslots = 0, nblocks = 0, nchunks = 0, cap = 0, nunits = 0, ncells = 0;
+    int uniform = 0;  // atoms per unit when all units are the same size, else 0
+    int closed = 0;   // every exception pair lies inside one unit
     double cutoff = 0, skin = 0;
-    DevBuf<int> unit_ptr, unit_atoms, unit_key, cell_rank, cell_count, cell_start, cell_cursor;
+    DevBuf<int> unit_ptr, unit_atoms, unit_pos, unit_sys, unit_key, cell_rank, cell_count, cell_start, cell_cursor;
     DevBuf<int> sorted_units, unit_size_sorted, unit_offset;
-    DevBuf<int> sorted_atom, atom_slot;
+    DevBuf<int> sorted_atom, atom_slot, block_sys;
+    DevBuf<int> block_off, chunk_off;  // per-system prefix tables (nsys + 1)
     DevBuf<int> excl_ptr, excl_idx, excl_sorted;
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
-    DevBuf<int> jlist, jcount, tiles, task_start;
-    DevBuf<int> flag;  // [0] rebuild requested
+    DevBuf<int> jlist, jcount, task_start, task_block;
+    DevBuf<int> atom_sys;
+    // Rebuild requests: [0] some system, [1 + s] system s.  Systems that are
+    // not flagged keep their order, bounds and lists through a build, so a
+    // replica rebuilds exactly when it would alone.
+    DevBuf<int> flag;
+    DevBuf<int> one;  // all ones: the flags of forced builds
 };
 
-// Host-side set-up: units from the exception graph, Morton cell ranks, sizes.
-void nlist_setup(NlistDev& nl, int n, const std::vector<int>& exc1, const std::vector<int>& exc2,
-                 double cutoff, double skin, const Box& box, hipStream_t s);
+// Host-side set-up: units from each system's exception graph, Morton cell
+// ranks, slot/block/cell ranges (written into sys[s]), the identity slot
+// layout (so PME may run before the first build), sizes.
+void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vector<SysInfo>& sys, double cutoff,
+                 double skin, hipStream_t s);
+
+// New per-block capacity (reallocates the list and the task table).
+void nlist_resize(NlistDev& nl, int cap);
 
 // Requests a rebuild when an atom moved more than skin/2 since the last build
 // (or always when force != 0).  Device-side: usable inside a captured graph.
-void nlist_check(NlistDev& nl, const double* pos, const Box* box_dev, int force, hipStream_t s);
+void nlist_check(NlistDev& nl, const double* pos, int force, hipStream_t s);
 
-// Rebuilds the list if requested (every kernel exits early otherwise).
-void nlist_build(NlistDev& nl, const double* pos, const Box* box_dev, unsigned* status, hipStream_t s);
+// Rebuilds the list if requested (every kernel exits early otherwise), or
+// unconditionally when forced.  Clears the request.
+void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsigned* status, hipStream_t s,
+                 bool forced = false);
 
 }  // namespace ommhip

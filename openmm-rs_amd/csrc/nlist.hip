@@ -44,60 +44,81 @@ __device__ __forceinline__ double box_gap2(const double dc_in[3], const double h
     return best;
 }
 
-__global__ void k_check(const double* __restrict__ pos, const double* __restrict__ pos_build, int n,
-                        double lim2, int force, int* flag) {
+// flag[0]: some system needs a rebuild; flag[1 + s]: system s does.  force:
+// flag every system.
+__global__ void k_check(const double* __restrict__ pos, const double* __restrict__ pos_build,
+                        const int* __restrict__ atom_sys, int n, int nsys, double lim2, int force, int* flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
+    int si = 0;
     if (i < n) {
         double dx = pos[3 * i] - pos_build[3 * i];
         double dy = pos[3 * i + 1] - pos_build[3 * i + 1];
         double dz = pos[3 * i + 2] - pos_build[3 * i + 2];
         double d2 = dx * dx + dy * dy + dz * dz;
         moved = !(d2 <= lim2);  // NaN also requests a rebuild
+        si = atom_sys[i];
     }
-    if (force && i == 0) moved = true;
-    if (__ballot(moved) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
-}
-
-__global__ void k_zero_i32(int* p, int n, const int* flag) {
-    if (!flag[0]) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+    wave_raise_flags(flag, moved, si);
+    if (force && i < nsys) {
+        atomicOr(flag + 1 + i, 1);
+        atomicOr(flag, 1);
+    }
 }
 
 __global__ void k_unit_keys(const double* __restrict__ pos, const int* __restrict__ unit_ptr,
-                            const int* __restrict__ unit_atoms, int nunits, const Box* __restrict__ box,
-                            int cx, int cy, int cz, const int* __restrict__ cell_rank, int* unit_key,
-                            int* cell_count, const int* flag) {
+                            const int* __restrict__ unit_atoms, const int* __restrict__ unit_sys, int nunits,
+                            const SysInfo* __restrict__ sys, const int* __restrict__ cell_rank, int* unit_key,
+                            int* cell_count, const int* __restrict__ flag) {
     if (!flag[0]) return;
     int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
+    const int si = unit_sys[u];
+    if (!flag[1 + si]) {
+        // Not rebuilt: the previous key reproduces the previous order.
+        atomicAdd(&cell_count[unit_key[u]], 1);
+        return;
+    }
     int a = unit_atoms[unit_ptr[u]];
-    const Box& b = *box;
+    const SysInfo& S = sys[si];
+    const Box& b = S.box;
     int c[3];
-    int nc[3] = {cx, cy, cz};
     for (int ax = 0; ax < 3; ++ax) {
         double f = pos[3 * a] * b.recip[ax * 3] + pos[3 * a + 1] * b.recip[ax * 3 + 1] +
                    pos[3 * a + 2] * b.recip[ax * 3 + 2];
         f -= floor(f);
-        int v = (int)(f * nc[ax]);
-        c[ax] = v < 0 ? 0 : (v >= nc[ax] ? nc[ax] - 1 : v);
+        const int nc = S.ncell[ax];
+        int v = (int)(f * nc);
+        c[ax] = v < 0 ? 0 : (v >= nc ? nc - 1 : v);
     }
-    int key = cell_rank[(c[0] * cy + c[1]) * cz + c[2]];
+    int key = S.cell_off + cell_rank[S.cell_off + (c[0] * S.ncell[1] + c[1]) * S.ncell[2] + c[2]];
     unit_key[u] = key;
     atomicAdd(&cell_count[key], 1);
 }
 
+enum : int { SCAN_ZERO_INPUT = 1, SCAN_TILES = 2 };
+
 // Exclusive scan of in[0..n) into out[0..n], out[n] = total (one workgroup).
-// copy (optional) receives the same values.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ in, int* out, int* copy, int n,
-                                                       const int* flag) {
+//   copy (optional) receives the same values;
+//   SCAN_ZERO_INPUT clears the input after reading it (the cell counts start
+//   the next build at zero without a fill kernel);
+//   SCAN_TILES scans ceil(in/JTILE) and writes task_block[t] = I for every
+//   task t of block I (the pair kernel's task table).
+// clear (optional, nclear entries) is reset to 0 at the end: the rebuild
+// requests are served.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(int* in, int* out, int* copy, int n, int mode,
+                                                       int* task_block, int* clear, int nclear, const int* flag) {
     if (!flag[0]) return;
     __shared__ int s_part[SCAN_THREADS];
     int t = threadIdx.x;
     int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
     int lo = min(n, t * per), hi = min(n, lo + per);
+    auto value = [&](int i) {
+        int v = in[i];
+        return (mode & SCAN_TILES) ? (v + JTILE - 1) / JTILE : v;
+    };
     int sum = 0;
-    for (int i = lo; i < hi; ++i) sum += in[i];
+    for (int i = lo; i < hi; ++i) sum += value(i);
     s_part[t] = sum;
     __syncthreads();
     for (int off = 1; off < SCAN_THREADS; off <<= 1) {
@@ -108,14 +129,21 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int* __restrict__ i
     }
     int run = s_part[t] - sum;
     for (int i = lo; i < hi; ++i) {
+        int v = value(i);
         out[i] = run;
         if (copy) copy[i] = run;
-        run += in[i];
+        if (mode & SCAN_TILES)
+            for (int k = 0; k < v; ++k) task_block[run + k] = i;
+        if (mode & SCAN_ZERO_INPUT) in[i] = 0;
+        run += v;
     }
     if (t == SCAN_THREADS - 1) {
         out[n] = s_part[t];
         if (copy) copy[n] = s_part[t];
     }
+    __syncthreads();
+    if (clear)
+        for (int i = t; i < nclear; i += SCAN_THREADS) clear[i] = 0;
 }
 
 __global__ void k_scatter_units(const int* __restrict__ unit_key, int nunits, int* cursor, int* sorted_units,
@@ -142,34 +170,63 @@ __global__ void k_sort_cells(const int* __restrict__ cell_start, int ncells, int
         }
         sorted_units[j + 1] = v;
     }
-    for (int i = lo; i < hi; ++i) {
-        int u = sorted_units[i];
-        unit_size_sorted[i] = unit_ptr[u + 1] - unit_ptr[u];
-    }
+    if (unit_size_sorted)
+        for (int i = lo; i < hi; ++i) {
+            int u = sorted_units[i];
+            unit_size_sorted[i] = unit_ptr[u + 1] - unit_ptr[u];
+        }
 }
 
-__global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* __restrict__ unit_offset,
-                              const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms, int nunits,
-                              int* sorted_atom, int* atom_slot, const int* flag) {
+// Places the atoms of sorted unit p at their slots.  Units are sorted by
+// (system, cell), so the atoms before unit p are those of the earlier systems
+// plus the earlier units of its own system; uniform > 0: every unit has that
+// many atoms (no size scan).  When every exception partner of an atom lies in
+// its own unit (closed units, e.g. rigid or flexible water), its sorted
+// exclusion list is written here as well.
+__global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* __restrict__ unit_offset, int uniform,
+                              const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
+                              const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
+                              const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
+                              const int* __restrict__ excl_idx, int closed, int nunits, int* sorted_atom,
+                              int* atom_slot, int* excl_sorted, const int* flag) {
     if (!flag[0]) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nunits) return;
-    int u = sorted_units[s];
-    int off = unit_offset[s];
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nunits) return;
+    int u = sorted_units[p];
+    const int si = unit_sys[u];
+    if (!flag[1 + si]) return;  // same keys, same slots: nothing changes
+    const SysInfo& S = sys[si];
+    int off = (uniform > 0 ? p * uniform : unit_offset[p]) - S.atom_off + S.slot_off;
     for (int k = unit_ptr[u]; k < unit_ptr[u + 1]; ++k) {
         int a = unit_atoms[k];
         int slot = off + (k - unit_ptr[u]);
         sorted_atom[slot] = a;
         atom_slot[a] = slot;
+        if (closed) {
+            int v[MAXEX];
+            for (int m = 0; m < MAXEX; ++m) v[m] = INT_MAX;
+            int cnt = 0;
+            for (int e = excl_ptr[a]; e < excl_ptr[a + 1] && cnt < MAXEX; ++e) {
+                int x = off + unit_pos[excl_idx[e]];
+                int j = cnt++;
+                while (j > 0 && v[j - 1] > x) {
+                    v[j] = v[j - 1];
+                    --j;
+                }
+                v[j] = x;
+            }
+            for (int m = 0; m < MAXEX; ++m) excl_sorted[(size_t)slot * MAXEX + m] = v[m];
+        }
     }
 }
 
 __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __restrict__ atom_slot,
-                              const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, int npad,
-                              int* excl_sorted, const int* flag) {
+                              const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx,
+                              const int* __restrict__ block_sys, int npad, int* excl_sorted, const int* flag) {
     if (!flag[0]) return;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= npad) return;
+    if (!flag[1 + block_sys[s / IBLOCK]]) return;
     int v[MAXEX];
     for (int k = 0; k < MAXEX; ++k) v[k] = INT_MAX;
     int a = sorted_atom[s];
@@ -189,19 +246,22 @@ __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __
 }
 
 // Two blocks per wave64: lanes 0-31 block 2*blockIdx.x, lanes 32-63 the next.
+// The first slot of a block always holds an atom (padding only ends a
+// system's last block).
 __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ pos,
-                                                     const int* __restrict__ sorted_atom, int n, int nblocks,
-                                                     const Box* __restrict__ box, double reach,
+                                                     const int* __restrict__ sorted_atom, int nblocks,
+                                                     const int* __restrict__ block_sys,
+                                                     const SysInfo* __restrict__ sys, double reach,
                                                      double* center, float* half, int* minimg,
                                                      double* pos_build, const int* flag) {
     if (!flag[0]) return;
-    const Box& b = *box;
     int lane = threadIdx.x;
     int blk = blockIdx.x * 2 + (lane >> 5);
-    int slot = blk * IBLOCK + (lane & 31);
-    bool live = blk < nblocks && slot < n;
-    int a = live ? sorted_atom[slot] : -1;
-    int ref = blk < nblocks ? sorted_atom[blk * IBLOCK] : 0;
+    const int si = blk < nblocks ? block_sys[blk] : 0;
+    bool valid = blk < nblocks && flag[1 + si];  // unflagged systems keep their bounds
+    int a = valid ? sorted_atom[blk * IBLOCK + (lane & 31)] : -1;
+    int ref = valid ? sorted_atom[blk * IBLOCK] : 0;
+    const Box& b = sys[si].box;
     double mn[3], mx[3];
     for (int k = 0; k < 3; ++k) {
         mn[k] = 1e300;
@@ -218,7 +278,7 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
             mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
             mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
         }
-    if ((lane & 31) == 0 && blk < nblocks) {
+    if ((lane & 31) == 0 && valid) {
         int need = b.triclinic;
         for (int k = 0; k < 3; ++k) {
             double c = pos[3 * ref + k] + 0.5 * (mn[k] + mx[k]);
@@ -232,25 +292,29 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
 }
 
 __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, const int* __restrict__ sorted_atom,
-                                              int n, int nblocks, const double* __restrict__ center,
-                                              const float* __restrict__ half, const Box* __restrict__ box,
+                                              const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
+                                              const double* __restrict__ center, const float* __restrict__ half,
                                               double reach, int cap, int* jlist, int* jcount, unsigned* status,
                                               const int* flag) {
     if (!flag[0]) return;
-    const Box& b = *box;
     const int I = blockIdx.x;
+    const int si = block_sys[I];
+    if (!flag[1 + si]) return;  // unflagged systems keep their lists
+    const SysInfo& S = sys[si];
+    const Box& b = S.box;
+    const int jend = S.block_off + S.nblocks;
     const int lane = threadIdx.x;
     const double reach2 = reach * reach;
     double cI[3] = {center[3 * I], center[3 * I + 1], center[3 * I + 2]};
     double hI[3] = {half[3 * I], half[3 * I + 1], half[3 * I + 2]};
     int* out = jlist + (size_t)I * cap;
-    int own = min(IBLOCK, n - I * IBLOCK);
+    int own = min(IBLOCK, S.slot_off + S.n - I * IBLOCK);
     if (lane < own) out[lane] = I * IBLOCK + lane;
     int count = own;
-    for (int J0 = I + 1; J0 < nblocks; J0 += 64) {
+    for (int J0 = I + 1; J0 < jend; J0 += 64) {
         int J = J0 + lane;
         bool pass = false;
-        if (J < nblocks) {
+        if (J < jend) {
             double dc[3], h[3];
             for (int k = 0; k < 3; ++k) {
                 dc[k] = center[3 * J + k] - cI[k];
@@ -270,8 +334,8 @@ __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, co
             int Jx = (lane < 32) ? J0 + j1 : (j2 >= 0 ? J0 + j2 : -1);
             int slot = Jx >= 0 ? Jx * IBLOCK + (lane & 31) : -1;
             bool ok = false;
-            if (slot >= 0 && slot < n) {
-                int a = sorted_atom[slot];
+            int a = slot >= 0 ? sorted_atom[slot] : -1;
+            if (a >= 0) {
                 double d[3] = {pos[3 * a] - cI[0], pos[3 * a + 1] - cI[1], pos[3 * a + 2] - cI[2]};
                 ok = box_gap2(d, hI, b) < reach2;
             }
@@ -293,14 +357,6 @@ __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, co
     }
 }
 
-__global__ void k_tiles(const int* __restrict__ jcount, int nblocks, int* tiles, const int* flag) {
-    if (!flag[0]) return;
-    int I = blockIdx.x * blockDim.x + threadIdx.x;
-    if (I < nblocks) tiles[I] = (jcount[I] + JTILE - 1) / JTILE;
-}
-
-__global__ void k_clear_flag(int* flag) { flag[0] = 0; }
-
 uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
     auto spread = [](uint32_t v) {
         uint64_t r = 0;
@@ -314,154 +370,218 @@ inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t); }
 
 }  // namespace
 
-void nlist_setup(NlistDev& nl, int n, const std::vector<int>& exc1, const std::vector<int>& exc2, double cutoff,
-                 double skin, const Box& box, hipStream_t s) {
-    nl.n = n;
-    nl.nblocks = (n + IBLOCK - 1) / IBLOCK;
-    nl.npad = nl.nblocks * IBLOCK;
+void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vector<SysInfo>& sys, double cutoff,
+                 double skin, hipStream_t s) {
+    const int nsys = (int)systems.size();
+    nl.nsys = nsys;
     nl.cutoff = cutoff;
     nl.skin = skin;
-    // Exclusion adjacency (both directions) in original indices.
-    std::vector<std::vector<int>> adj(n);
-    for (size_t e = 0; e < exc1.size(); ++e) {
-        adj[exc1[e]].push_back(exc2[e]);
-        adj[exc2[e]].push_back(exc1[e]);
-    }
-    std::vector<int> eptr(n + 1, 0), eidx;
-    for (int i = 0; i < n; ++i) {
-        if ((int)adj[i].size() > MAXEX)
-            throw Error(OMM_ERR_INVALID, "device path supports at most " + std::to_string(MAXEX) +
-                                             " exceptions per particle");
-        eptr[i + 1] = eptr[i] + (int)adj[i].size();
-        eidx.insert(eidx.end(), adj[i].begin(), adj[i].end());
-    }
-    // Sort units: connected components of the exception graph with <= 4
-    // atoms move together; larger components are split into single atoms.
-    std::vector<int> parent(n);
-    std::iota(parent.begin(), parent.end(), 0);
-    auto root = [&](int x) {
-        while (parent[x] != x) x = parent[x] = parent[parent[x]];
-        return x;
-    };
-    for (size_t e = 0; e < exc1.size(); ++e) {
-        int a = root(exc1[e]), b2 = root(exc2[e]);
-        if (a != b2) parent[std::max(a, b2)] = std::min(a, b2);
-    }
-    std::vector<std::vector<int>> comp(n);
-    for (int i = 0; i < n; ++i) comp[root(i)].push_back(i);
-    std::vector<int> uptr{0}, uatoms;
-    for (int i = 0; i < n; ++i) {
-        if (comp[i].empty()) continue;
-        if (comp[i].size() <= 4) {
-            uatoms.insert(uatoms.end(), comp[i].begin(), comp[i].end());
+    std::vector<int> eptr{0}, eidx, uptr{0}, uatoms, upos, usys, rank_all, block_sys, sorted, slot_of, atom_sys;
+    std::vector<int> block_off{0}, chunk_off{0};
+    int natoms = 0, nblocks = 0, nchunks = 0, ncells = 0, max_npad = 0;
+    nl.uniform = -1;
+    nl.closed = 1;
+    for (int si = 0; si < nsys; ++si) {
+        const NlistSystem& sy = systems[si];
+        const int n = sy.n;
+        if (n <= 0) throw Error(OMM_ERR_INVALID, "every system of a batch needs at least one particle");
+        const std::vector<int>& exc1 = *sy.exc1;
+        const std::vector<int>& exc2 = *sy.exc2;
+        // Exclusion adjacency (both directions), global indices.
+        std::vector<std::vector<int>> adj(n);
+        for (size_t e = 0; e < exc1.size(); ++e) {
+            adj[exc1[e]].push_back(exc2[e]);
+            adj[exc2[e]].push_back(exc1[e]);
+        }
+        for (int i = 0; i < n; ++i) {
+            if ((int)adj[i].size() > MAXEX)
+                throw Error(OMM_ERR_INVALID, "device path supports at most " + std::to_string(MAXEX) +
+                                                 " exceptions per particle");
+            for (int j : adj[i]) eidx.push_back(natoms + j);
+            eptr.push_back((int)eidx.size());
+        }
+        // Sort units: connected components of the exception graph with <= 4
+        // atoms move together; larger components are split into single atoms.
+        std::vector<int> parent(n);
+        std::iota(parent.begin(), parent.end(), 0);
+        auto root = [&](int x) {
+            while (parent[x] != x) x = parent[x] = parent[parent[x]];
+            return x;
+        };
+        for (size_t e = 0; e < exc1.size(); ++e) {
+            int a = root(exc1[e]), b2 = root(exc2[e]);
+            if (a != b2) parent[std::max(a, b2)] = std::min(a, b2);
+        }
+        std::vector<std::vector<int>> comp(n);
+        for (int i = 0; i < n; ++i) comp[root(i)].push_back(i);
+        std::vector<int> unit_of(n, 0);
+        upos.resize(natoms + n, 0);
+        auto add_unit = [&](const std::vector<int>& atoms) {
+            const int size = (int)atoms.size();
+            nl.uniform = nl.uniform < 0 ? size : (nl.uniform == size ? size : 0);
+            for (int k = 0; k < size; ++k) {
+                upos[natoms + atoms[k]] = k;
+                unit_of[atoms[k]] = (int)uptr.size() - 1;
+                uatoms.push_back(natoms + atoms[k]);
+            }
             uptr.push_back((int)uatoms.size());
-        } else {
-            for (int a : comp[i]) {
-                uatoms.push_back(a);
-                uptr.push_back((int)uatoms.size());
+            usys.push_back(si);
+        };
+        for (int i = 0; i < n; ++i) {
+            if (comp[i].empty()) continue;
+            if (comp[i].size() <= 4) {
+                add_unit(comp[i]);
+            } else {
+                for (int a : comp[i]) add_unit(std::vector<int>{a});
             }
         }
+        for (size_t e = 0; e < exc1.size(); ++e)
+            if (unit_of[exc1[e]] != unit_of[exc2[e]]) nl.closed = 0;
+        // Cell grid of ~0.35 nm along each reciprocal direction, Morton ranks.
+        SysInfo& S = sys[si];
+        const Box& box = sy.box;
+        for (int a = 0; a < 3; ++a) {
+            double norm = std::sqrt(box.recip[a * 3] * box.recip[a * 3] + box.recip[a * 3 + 1] * box.recip[a * 3 + 1] +
+                                    box.recip[a * 3 + 2] * box.recip[a * 3 + 2]);
+            S.ncell[a] = std::max(1, std::min(1023, (int)std::floor(1.0 / norm / 0.35)));
+        }
+        const int nc = S.ncell[0] * S.ncell[1] * S.ncell[2];
+        std::vector<std::pair<uint32_t, int>> codes(nc);
+        for (int x = 0; x < S.ncell[0]; ++x)
+            for (int y = 0; y < S.ncell[1]; ++y)
+                for (int z = 0; z < S.ncell[2]; ++z) {
+                    int lin = (x * S.ncell[1] + y) * S.ncell[2] + z;
+                    codes[lin] = {morton3(x, y, z), lin};
+                }
+        std::sort(codes.begin(), codes.end());
+        std::vector<int> rank(nc);
+        for (int r = 0; r < nc; ++r) rank[codes[r].second] = r;
+        rank_all.insert(rank_all.end(), rank.begin(), rank.end());
+        S.cell_off = ncells;
+        ncells += nc;
+        // Slots, blocks and PME chunks; the identity layout until the first build.
+        const int nb = (n + IBLOCK - 1) / IBLOCK;
+        S.atom_off = natoms;
+        S.n = n;
+        S.block_off = nblocks;
+        S.nblocks = nb;
+        S.slot_off = nblocks * IBLOCK;
+        S.chunk_off = nchunks;
+        for (int k = 0; k < nb * IBLOCK; ++k) {
+            sorted.push_back(k < n ? natoms + k : -1);
+            if (k < n) slot_of.push_back(S.slot_off + k);
+        }
+        for (int k = 0; k < nb; ++k) block_sys.push_back(si);
+        atom_sys.insert(atom_sys.end(), n, si);
+        natoms += n;
+        nblocks += nb;
+        nchunks += (nb * IBLOCK + PME_CHUNK - 1) / PME_CHUNK;
+        block_off.push_back(nblocks);
+        chunk_off.push_back(nchunks);
+        max_npad = std::max(max_npad, nb * IBLOCK);
     }
+    if (nl.uniform < 0) nl.uniform = 0;
+    nl.natoms = natoms;
+    nl.nblocks = nblocks;
+    nl.nslots = nblocks * IBLOCK;
+    nl.nchunks = nchunks;
     nl.nunits = (int)uptr.size() - 1;
-    // Cell grid of ~0.35 nm along each reciprocal direction.
-    for (int a = 0; a < 3; ++a) {
-        double norm = std::sqrt(box.recip[a * 3] * box.recip[a * 3] + box.recip[a * 3 + 1] * box.recip[a * 3 + 1] +
-                                box.recip[a * 3 + 2] * box.recip[a * 3 + 2]);
-        double width = 1.0 / norm;
-        nl.ncell[a] = std::max(1, std::min(1023, (int)std::floor(width / 0.35)));
-    }
-    nl.ncells = nl.ncell[0] * nl.ncell[1] * nl.ncell[2];
-    std::vector<std::pair<uint32_t, int>> codes(nl.ncells);
-    for (int x = 0; x < nl.ncell[0]; ++x)
-        for (int y = 0; y < nl.ncell[1]; ++y)
-            for (int z = 0; z < nl.ncell[2]; ++z) {
-                int lin = (x * nl.ncell[1] + y) * nl.ncell[2] + z;
-                codes[lin] = {morton3(x, y, z), lin};
-            }
-    std::sort(codes.begin(), codes.end());
-    std::vector<int> rank(nl.ncells);
-    for (int r = 0; r < nl.ncells; ++r) rank[codes[r].second] = r;
+    nl.ncells = ncells;
     // List capacity: the atoms within reach of a compact block at liquid
-    // density, doubled, and never less than all atoms when the box is small.
+    // density, doubled, and never more than the largest system.
     double reach = cutoff + skin;
     double density = 100.0;
     double hb = 0.5 * std::cbrt(IBLOCK / density) * 1.6;
     double vol = std::pow(2 * hb, 3) + 6 * std::pow(2 * hb, 2) * reach + 3 * PI * 2 * hb * reach * reach +
                  4.0 / 3.0 * PI * reach * reach * reach;
     long long cap = (long long)(2.0 * density * vol) + IBLOCK;
-    cap = std::min<long long>(cap, (long long)nl.npad);
+    cap = std::min<long long>(cap, (long long)max_npad);
     cap = std::max<long long>(cap, 256);
     nl.cap = (int)((cap + JTILE - 1) / JTILE * JTILE);
 
     nl.unit_ptr.upload(uptr, s);
     nl.unit_atoms.upload(uatoms, s);
+    nl.unit_pos.upload(upos, s);
+    nl.unit_sys.upload(usys, s);
     nl.unit_key.alloc(nl.nunits);
-    nl.cell_rank.upload(rank, s);
-    nl.cell_count.alloc(nl.ncells);
-    nl.cell_start.alloc(nl.ncells + 1);
-    nl.cell_cursor.alloc(nl.ncells + 1);
+    nl.cell_rank.upload(rank_all, s);
+    std::vector<int> zeros(ncells, 0);
+    nl.cell_count.upload(zeros, s);
+    nl.cell_start.alloc(ncells + 1);
+    nl.cell_cursor.alloc(ncells + 1);
     nl.sorted_units.alloc(nl.nunits);
     nl.unit_size_sorted.alloc(nl.nunits);
     nl.unit_offset.alloc(nl.nunits + 1);
-    std::vector<int> pad(nl.npad, -1);
-    nl.sorted_atom.upload(pad, s);
-    nl.atom_slot.alloc(n);
+    nl.sorted_atom.upload(sorted, s);
+    nl.atom_slot.upload(slot_of, s);
+    nl.block_sys.upload(block_sys, s);
+    nl.block_off.upload(block_off, s);
+    nl.chunk_off.upload(chunk_off, s);
     nl.excl_ptr.upload(eptr, s);
     if (eidx.empty()) eidx.push_back(0);
     nl.excl_idx.upload(eidx, s);
-    nl.excl_sorted.alloc((size_t)nl.npad * MAXEX);
-    nl.block_center.alloc(3 * (size_t)nl.nblocks);
-    nl.block_half.alloc(3 * (size_t)nl.nblocks);
-    nl.block_minimg.alloc(nl.nblocks);
-    nl.pos_build.alloc(3 * (size_t)n);
-    nl.jlist.alloc((size_t)nl.nblocks * nl.cap);
-    nl.jcount.alloc(nl.nblocks);
-    nl.tiles.alloc(nl.nblocks);
-    nl.task_start.alloc(nl.nblocks + 1);
-    std::vector<int> one{1};
+    nl.excl_sorted.alloc((size_t)nl.nslots * MAXEX);
+    nl.block_center.alloc(3 * (size_t)nblocks);
+    nl.block_half.alloc(3 * (size_t)nblocks);
+    nl.block_minimg.alloc(nblocks);
+    nl.pos_build.alloc(3 * (size_t)natoms);
+    nl.jlist.alloc((size_t)nblocks * nl.cap);
+    nl.jcount.alloc(nblocks);
+    nl.task_start.alloc(nblocks + 1);
+    nl.task_block.alloc((size_t)nblocks * (nl.cap / JTILE));
+    std::vector<int> one(nsys + 1, 1);
     nl.flag.upload(one, s);
+    nl.one.upload(one, s);
+    nl.atom_sys.upload(atom_sys, s);
 }
 
-void nlist_check(NlistDev& nl, const double* pos, const Box* box_dev, int force, hipStream_t s) {
-    (void)box_dev;
+void nlist_resize(NlistDev& nl, int cap) {
+    nl.cap = cap;
+    nl.jlist.alloc((size_t)nl.nblocks * nl.cap);
+    nl.task_block.alloc((size_t)nl.nblocks * (nl.cap / JTILE));
+}
+
+void nlist_check(NlistDev& nl, const double* pos, int force, hipStream_t s) {
     double lim = 0.5 * nl.skin;
-    hipLaunchKernelGGL(k_check, dim3(blocks_for(nl.n, 256)), dim3(256), 0, s, pos, nl.pos_build.p, nl.n, lim * lim,
-                       force, nl.flag.p);
+    hipLaunchKernelGGL(k_check, dim3(blocks_for(std::max(nl.natoms, nl.nsys), 256)), dim3(256), 0, s, pos,
+                       nl.pos_build.p, nl.atom_sys.p, nl.natoms, nl.nsys, lim * lim, force, nl.flag.p);
     OMM_HIP(hipGetLastError());
 }
 
-void nlist_build(NlistDev& nl, const double* pos, const Box* box_dev, unsigned* status, hipStream_t s) {
-    const int* flag = nl.flag.p;
-    hipLaunchKernelGGL(k_zero_i32, dim3(std::min(1024, blocks_for(nl.ncells, 256))), dim3(256), 0, s, nl.cell_count.p,
-                       nl.ncells, flag);
+void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsigned* status, hipStream_t s,
+                 bool forced) {
+    // Forced builds (the first step of every captured graph) run
+    // unconditionally; otherwise every kernel exits early unless the
+    // displacement check asked for a rebuild.
+    const int* flag = forced ? nl.one.p : nl.flag.p;
     hipLaunchKernelGGL(k_unit_keys, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, pos, nl.unit_ptr.p,
-                       nl.unit_atoms.p, nl.nunits, box_dev, nl.ncell[0], nl.ncell[1], nl.ncell[2], nl.cell_rank.p,
-                       nl.unit_key.p, nl.cell_count.p, flag);
+                       nl.unit_atoms.p, nl.unit_sys.p, nl.nunits, sys_dev, nl.cell_rank.p, nl.unit_key.p,
+                       nl.cell_count.p, flag);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.cell_count.p, nl.cell_start.p, nl.cell_cursor.p,
-                       nl.ncells, flag);
+                       nl.ncells, (int)SCAN_ZERO_INPUT, (int*)nullptr, (int*)nullptr, 0, flag);
     hipLaunchKernelGGL(k_scatter_units, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.unit_key.p, nl.nunits,
                        nl.cell_cursor.p, nl.sorted_units.p, flag);
     hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p, nl.ncells,
-                       nl.sorted_units.p, nl.unit_ptr.p, nl.unit_size_sorted.p, flag);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.unit_size_sorted.p, nl.unit_offset.p,
-                       (int*)nullptr, nl.nunits, flag);
+                       nl.sorted_units.p, nl.unit_ptr.p, nl.uniform > 0 ? (int*)nullptr : nl.unit_size_sorted.p, flag);
+    if (nl.uniform == 0)
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.unit_size_sorted.p, nl.unit_offset.p,
+                           (int*)nullptr, nl.nunits, 0, (int*)nullptr, (int*)nullptr, 0, flag);
     hipLaunchKernelGGL(k_place_atoms, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.sorted_units.p,
-                       nl.unit_offset.p, nl.unit_ptr.p, nl.unit_atoms.p, nl.nunits, nl.sorted_atom.p,
-                       nl.atom_slot.p, flag);
-    hipLaunchKernelGGL(k_excl_sorted, dim3(blocks_for(nl.npad, 256)), dim3(256), 0, s, nl.sorted_atom.p,
-                       nl.atom_slot.p, nl.excl_ptr.p, nl.excl_idx.p, nl.npad, nl.excl_sorted.p, flag);
+                       nl.unit_offset.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p, nl.unit_sys.p,
+                       sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.nunits, nl.sorted_atom.p, nl.atom_slot.p,
+                       nl.excl_sorted.p, flag);
+    if (!nl.closed)
+        hipLaunchKernelGGL(k_excl_sorted, dim3(blocks_for(nl.nslots, 256)), dim3(256), 0, s, nl.sorted_atom.p,
+                           nl.atom_slot.p, nl.excl_ptr.p, nl.excl_idx.p, nl.block_sys.p, nl.nslots,
+                           nl.excl_sorted.p, flag);
     double reach = nl.cutoff + nl.skin;
-    hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.n,
-                       nl.nblocks, box_dev, reach, nl.block_center.p, nl.block_half.p, nl.block_minimg.p,
-                       nl.pos_build.p, flag);
-    hipLaunchKernelGGL(k_jlist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.n, nl.nblocks,
-                       nl.block_center.p, nl.block_half.p, box_dev, reach, nl.cap, nl.jlist.p, nl.jcount.p, status,
-                       flag);
-    hipLaunchKernelGGL(k_tiles, dim3(blocks_for(nl.nblocks, 256)), dim3(256), 0, s, nl.jcount.p, nl.nblocks,
-                       nl.tiles.p, flag);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.tiles.p, nl.task_start.p, (int*)nullptr,
-                       nl.nblocks, flag);
-    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, nl.flag.p);
+    hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p,
+                       nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
+                       nl.block_minimg.p, nl.pos_build.p, flag);
+    hipLaunchKernelGGL(k_jlist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.block_sys.p, sys_dev,
+                       nl.block_center.p, nl.block_half.p, reach, nl.cap, nl.jlist.p, nl.jcount.p, status, flag);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.jcount.p, nl.task_start.p, (int*)nullptr,
+                       nl.nblocks, (int)SCAN_TILES, nl.task_block.p, nl.flag.p, nl.nsys + 1, flag);
     OMM_HIP(hipGetLastError());
 }
 

@@ -8,7 +8,7 @@ namespace ommhip {
 enum ElecKind : int { ELEC_EWALD = 0, ELEC_REACTION_FIELD = 1, ELEC_COULOMB = 2 };
 
 struct PairParams {
-    int n = 0, nblocks = 0, cap = 0;
+    int nblocks = 0, cap = 0;
     int elec = ELEC_EWALD;
     int use_cutoff = 1;
     int periodic = 1;
@@ -19,6 +19,7 @@ struct PairParams {
 
 // Per-particle parameters in the form the kernels use.
 struct ParticleDev {
+    DevBuf<int> atom_sys;   // system of each atom
     DevBuf<double> charge;  // q (FP64: exceptions, PME)
     DevBuf<float> qk;       // q * sqrt(ONE_4PI_EPS0)
     DevBuf<float> sh;       // sigma / 2
@@ -36,16 +37,17 @@ struct ExceptionDev {
 // fixed-point force/energy accumulation; otherwise FP64 math with FP64 atomics
 // into force_d/energy_d.  energy_on: also accumulate the energy (in the mixed
 // form each pair energy is evaluated in FP64).  stats (nullable): [0] pair
-// count, [1] pair hash (no energy is accumulated then).
+// count, [1] pair hash per system (no energy is accumulated then).  Energies
+// go to slot E_DIRECT of each system's E_SLOTS block.
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
-                  const double* pos, const Box* box_dev, long long* force_fx, double* force_d,
+                  const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d,
                   long long* energy_fx, double* energy_d, unsigned* status, unsigned long long* stats,
                   int grid, hipStream_t s);
 
 // Exceptions (nonbonded.rs:995-1036), FP64 math.  has_pme selects the erf
 // reciprocal correction with particle charges.
 void launch_exceptions(bool mixed, const ExceptionDev& ex, const ParticleDev& part, const double* pos,
-                       const Box* box_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
+                       const SysInfo* sys_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
                        double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s);
 
 }  // namespace ommhip

@@ -21,6 +21,7 @@
 #include <climits>
 
 #include "pairs.h"
+#include "terms.h"
 
 namespace ommhip {
 
@@ -127,8 +128,10 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                                               const int* __restrict__ excl_sorted, const double* __restrict__ center,
                                               const int* __restrict__ minimg, const int* __restrict__ jlist,
                                               const int* __restrict__ jcount, const int* __restrict__ task_start,
-                                              const Box* __restrict__ box, PairParams P, A* force, A* energy,
-                                              unsigned* status, unsigned long long* stats) {
+                                              const int* __restrict__ task_block,
+                                              const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
+                                              PairParams P, A* force, A* energy, unsigned* status,
+                                              unsigned long long* stats) {
     using M = Math<R>;
     constexpr bool E64 = ENERGY && sizeof(R) == 4;  // separate FP64 energy path
     __shared__ Q4<R> s_pq[JTILE];
@@ -142,10 +145,6 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
     const int half = lane >> 5;
     const int il = lane & 31;
     const int src = (lane & 32) | ((lane + 1) & 31);
-    const Box& bx = *box;
-    const R bv[9] = {(R)bx.v[0], (R)bx.v[1], (R)bx.v[2], (R)bx.v[3], (R)bx.v[4],
-                     (R)bx.v[5], (R)bx.v[6], (R)bx.v[7], (R)bx.v[8]};
-    const R ib0 = (R)(1.0 / bx.v[0]), ib4 = (R)(1.0 / bx.v[4]), ib8 = (R)(1.0 / bx.v[8]);
     const double cutoff2 = P.cutoff * P.cutoff;
     const double band = sizeof(R) == 4 ? 1e-5 : 1e-12;
     const R rc2_lo = (R)(cutoff2 * (1.0 - band));
@@ -156,13 +155,9 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
     const int total = task_start[P.nblocks];
 
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        int lo = 0, hi = P.nblocks;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (task_start[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        const int I = lo;
+        const int I = task_block[t];
+        const int sidx = block_sys[I];
+        const Box& bx = sys[sidx].box;
         const int tile = t - task_start[I];
         const int cnt = min(JTILE, jcount[I] - tile * JTILE);
         const double c0 = center[3 * I], c1 = center[3 * I + 1], c2 = center[3 * I + 2];
@@ -196,7 +191,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             }
         }
         const int is = I * IBLOCK + il;
-        const int ia = is < P.n ? sorted_atom[is] : -1;
+        const int ia = sorted_atom[is];  // -1 in the padding of a system's last block
         R xi = 0, yi = 0, zi = 0, qi = 0, si = 0, ei = 0;
         double xid = 0, yid = 0, zid = 0, qid = 0, sid = 0, eid = 0;
         if (ia >= 0) {
@@ -232,7 +227,15 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             vmask = a >= 32 ? 0u : (0xffffffffu << a);
             int nvalid = min(max(cnt - half * 32, 0), 32);
             if (nvalid < 32) vmask &= (1u << nvalid) - 1u;
-            const int* ex = excl_sorted + (size_t)is * MAXEX;
+            int ex[MAXEX];
+            const int4* ex4 = reinterpret_cast<const int4*>(excl_sorted + (size_t)is * MAXEX);
+            for (int q = 0; q < MAXEX / 4; ++q) {
+                const int4 v = ex4[q];
+                ex[4 * q] = v.x;
+                ex[4 * q + 1] = v.y;
+                ex[4 * q + 2] = v.z;
+                ex[4 * q + 3] = v.w;
+            }
             for (int m = 0; m < MAXEX; ++m) {
                 int e = ex[m];
                 if (e == INT_MAX) break;
@@ -247,6 +250,13 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             }
         }
         const bool mi = P.periodic && minimg[I];
+        R bv[9], ib0 = 0, ib4 = 0, ib8 = 0;
+        if (mi) {
+            for (int k = 0; k < 9; ++k) bv[k] = (R)bx.v[k];
+            ib0 = (R)(1.0 / bx.v[0]);
+            ib4 = (R)(1.0 / bx.v[4]);
+            ib8 = (R)(1.0 / bx.v[8]);
+        }
 
         R fx = 0, fy = 0, fz = 0, gx = 0, gy = 0, gz = 0;
         double eacc = 0.0;
@@ -350,14 +360,14 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
         }
         if (ENERGY) {
             double etot = wave_sum(eacc);
-            if (lane == 0) acc_add(energy + E_DIRECT, etot, ENERGY_SCALE, status);
+            if (lane == 0) acc_add(energy + sidx * E_SLOTS + E_DIRECT, etot, ENERGY_SCALE, status);
         }
         if (STATS) {
             npair = wave_sum_u64(npair);
             hsum = wave_sum_u64(hsum);
             if (lane == 0) {
-                atomicAdd(stats, npair);
-                atomicAdd(stats + 1, hsum);
+                atomicAdd(stats + 2 * sidx, npair);
+                atomicAdd(stats + 2 * sidx + 1, hsum);
             }
         }
         __syncthreads();
@@ -368,112 +378,80 @@ template <class A>
 __global__ void k_exceptions(int count, const int* __restrict__ p1, const int* __restrict__ p2,
                              const double* __restrict__ eqq, const double* __restrict__ esig,
                              const double* __restrict__ eeps, const double* __restrict__ charge,
-                             const double* __restrict__ pos, const Box* __restrict__ box, int use_pbc, int has_pme,
-                             double alpha, A* force, A* energy, unsigned* status) {
+                             const double* __restrict__ pos, const int* __restrict__ atom_sys,
+                             const SysInfo* __restrict__ sys, int use_pbc, int has_pme, double alpha, A* force,
+                             A* energy, unsigned* status) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
+    int si = -1;
     if (k < count) {
         int i = p1[k], j = p2[k];
-        double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
-        if (use_pbc) min_image_exact(d, box->v);
-        double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        double r = sqrt(r2);
-        double f = 0.0;
-        // lj_coulomb_exception (nonbonded.rs:1194-1198)
-        if (eeps[k] != 0.0) {
-            double sr2 = esig[k] * esig[k] / r2;
-            double sr6 = sr2 * sr2 * sr2;
-            double sr12 = sr6 * sr6;
-            e += 4.0 * eeps[k] * (sr12 - sr6);
-            f += 4.0 * eeps[k] * (-12.0 * sr12 + 6.0 * sr6) / r2;
-        }
-        if (eqq[k] != 0.0) {
-            e += ONE_4PI_EPS0 * eqq[k] / r;
-            f -= ONE_4PI_EPS0 * eqq[k] / (r2 * r);
-        }
-        if (has_pme) {
-            // reciprocal_pair (nonbonded.rs:1275-1288)
-            double c = ONE_4PI_EPS0 * charge[i] * charge[j];
-            double corr, deriv;
-            if (alpha * r < 1e-6) {
-                corr = c * 2.0 * alpha / sqrt(PI) * (1.0 - alpha * alpha * r2 / 3.0);
-                deriv = -c * 4.0 * alpha * alpha * alpha / (3.0 * sqrt(PI));
-            } else {
-                double erf_v = erf(alpha * r);
-                corr = c * erf_v / r;
-                deriv = c * (2.0 * alpha / sqrt(PI) * exp(-alpha * alpha * r2) - erf_v / r) / r2;
-            }
-            e -= corr;
-            f -= deriv;
-        }
-        for (int a = 0; a < 3; ++a) {
-            acc_add(force + 3 * i + a, f * d[a], FORCE_SCALE, status);
-            acc_add(force + 3 * j + a, -f * d[a], FORCE_SCALE, status);
-        }
+        si = atom_sys[i];
+        e = exception_term(i, j, eqq[k], esig[k], eeps[k], charge[i], charge[j], pos, sys[si].box, use_pbc, has_pme,
+                           alpha, force, status);
     }
-    double et = wave_sum(e);
-    if ((threadIdx.x & 63) == 0) acc_add(energy + E_EXCEPT, et, ENERGY_SCALE, status);
+    wave_energy_add(energy, si, E_EXCEPT, e, status);
 }
 
 template <class R, class A, int ELEC, bool STATS, bool ENERGY>
 void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* sh, const R* e2,
-                const ParticleDev& part, const double* pos, const Box* box_dev, A* force, A* energy, unsigned* status,
-                unsigned long long* stats, int grid, hipStream_t s) {
+                const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
+                unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
                        part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.excl_sorted.p, nl.block_center.p,
-                       nl.block_minimg.p, nl.jlist.p, nl.jcount.p, nl.task_start.p, box_dev, p, force, energy, status,
-                       stats);
+                       nl.block_minimg.p, nl.jlist.p, nl.jcount.p, nl.task_start.p, nl.task_block.p, nl.block_sys.p,
+                       sys_dev, p, force, energy, status, stats);
 }
 
 template <class R, class A, int ELEC>
 void launch_pairs_t(bool stats_on, bool energy_on, const NlistDev& nl, const PairParams& p, const R* qk, const R* sh,
-                    const R* e2, const ParticleDev& part, const double* pos, const Box* box_dev, A* force, A* energy,
+                    const R* e2, const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
                     unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
     if (stats_on)
-        launch_one<R, A, ELEC, true, false>(nl, p, qk, sh, e2, part, pos, box_dev, force, energy, status, stats, grid, s);
+        launch_one<R, A, ELEC, true, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid, s);
     else if (energy_on)
-        launch_one<R, A, ELEC, false, true>(nl, p, qk, sh, e2, part, pos, box_dev, force, energy, status, stats, grid, s);
+        launch_one<R, A, ELEC, false, true>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid, s);
     else
-        launch_one<R, A, ELEC, false, false>(nl, p, qk, sh, e2, part, pos, box_dev, force, energy, status, stats, grid,
+        launch_one<R, A, ELEC, false, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
                                              s);
 }
 
 }  // namespace
 
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
-                  const double* pos, const Box* box_dev, long long* force_fx, double* force_d, long long* energy_fx,
+                  const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx,
                   double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
     bool st = stats != nullptr;
     if (mixed) {
         switch (p.elec) {
             case ELEC_EWALD:
                 launch_pairs_t<float, long long, ELEC_EWALD>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part,
-                                                             pos, box_dev, force_fx, energy_fx, status, stats, grid, s);
+                                                             pos, sys_dev, force_fx, energy_fx, status, stats, grid, s);
                 break;
             case ELEC_REACTION_FIELD:
                 launch_pairs_t<float, long long, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk.p, part.sh.p,
-                                                                      part.e2.p, part, pos, box_dev, force_fx,
+                                                                      part.e2.p, part, pos, sys_dev, force_fx,
                                                                       energy_fx, status, stats, grid, s);
                 break;
             default:
                 launch_pairs_t<float, long long, ELEC_COULOMB>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p,
-                                                               part, pos, box_dev, force_fx, energy_fx, status, stats,
+                                                               part, pos, sys_dev, force_fx, energy_fx, status, stats,
                                                                grid, s);
         }
     } else {
         switch (p.elec) {
             case ELEC_EWALD:
                 launch_pairs_t<double, double, ELEC_EWALD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p, part.e2_d.p,
-                                                           part, pos, box_dev, force_d, energy_d, status, stats, grid, s);
+                                                           part, pos, sys_dev, force_d, energy_d, status, stats, grid, s);
                 break;
             case ELEC_REACTION_FIELD:
                 launch_pairs_t<double, double, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
-                                                                    part.e2_d.p, part, pos, box_dev, force_d, energy_d,
+                                                                    part.e2_d.p, part, pos, sys_dev, force_d, energy_d,
                                                                     status, stats, grid, s);
                 break;
             default:
                 launch_pairs_t<double, double, ELEC_COULOMB>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
-                                                             part.e2_d.p, part, pos, box_dev, force_d, energy_d, status,
+                                                             part.e2_d.p, part, pos, sys_dev, force_d, energy_d, status,
                                                              stats, grid, s);
         }
     }
@@ -481,18 +459,18 @@ void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairPara
 }
 
 void launch_exceptions(bool mixed, const ExceptionDev& ex, const ParticleDev& part, const double* pos,
-                       const Box* box_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
+                       const SysInfo* sys_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
                        double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s) {
     if (ex.count == 0) return;
     int blocks = (ex.count + 255) / 256;
     if (mixed)
         hipLaunchKernelGGL(k_exceptions<long long>, dim3(blocks), dim3(256), 0, s, ex.count, ex.p1.p, ex.p2.p, ex.qq.p,
-                           ex.sig.p, ex.eps.p, part.charge.p, pos, box_dev, use_pbc, has_pme, alpha, force_fx,
-                           energy_fx, status);
+                           ex.sig.p, ex.eps.p, part.charge.p, pos, part.atom_sys.p, sys_dev, use_pbc, has_pme, alpha,
+                           force_fx, energy_fx, status);
     else
         hipLaunchKernelGGL(k_exceptions<double>, dim3(blocks), dim3(256), 0, s, ex.count, ex.p1.p, ex.p2.p, ex.qq.p,
-                           ex.sig.p, ex.eps.p, part.charge.p, pos, box_dev, use_pbc, has_pme, alpha, force_d, energy_d,
-                           status);
+                           ex.sig.p, ex.eps.p, part.charge.p, pos, part.atom_sys.p, sys_dev, use_pbc, has_pme, alpha,
+                           force_d, energy_d, status);
     OMM_HIP(hipGetLastError());
 }
 

@@ -1,4 +1,7 @@
-// Smooth PME reciprocal space (order-5 B-splines), pme.rs:151-425.
+// Smooth PME reciprocal space (order-5 B-splines), pme.rs:151-425, for a
+// batch of systems with their own grids: one launch per phase serves every
+// system (spread and gather by 64-slot chunks of the sorted order, each FFT
+// pass by groups of lines), the per-system layout coming from SysInfo.
 #pragma once
 
 #include "pairs.h"
@@ -6,27 +9,33 @@
 namespace ommhip {
 
 struct PmeDev {
-    int nx = 0, ny = 0, nz = 0, nzh = 0;
-    double alpha = 0.0;
-    std::vector<int> radix[3];
-    DevBuf<long long> qgrid_fx;  // mixed: fixed-point charge grid [nx][ny][nz]
-    DevBuf<double> qgrid_d;      // plain: FP64 charge grid
-    DevBuf<float> cgrid_f;       // complex half-spectrum [nx][ny][nzh] (interleaved re, im)
+    int nsys = 0;
+    long long qpoints = 0, cpoints = 0;  // real / half-spectrum points over all systems
+    DevBuf<long long> qgrid_fx;  // mixed: fixed-point charge grids
+    DevBuf<double> qgrid_d;      // plain: FP64 charge grids
+    DevBuf<float> cgrid_f;       // complex half-spectra [nx][ny][nzh] (interleaved re, im)
     DevBuf<double> cgrid_d;
-    DevBuf<float> phi_f;         // real-space potential [nx][ny][nz]
+    DevBuf<float> phi_f;         // real-space potentials [nx][ny][nz]
     DevBuf<double> phi_d;
-    DevBuf<float> roots_f;       // per axis n roots e^{-2 pi i m/n}, interleaved (x, then y, then z)
+    DevBuf<float> roots_f;       // per system and axis n roots e^{-2 pi i m/n}, interleaved
     DevBuf<double> roots_d;
-    DevBuf<float> moduli_f;      // B-spline moduli, x then y then z
+    DevBuf<float> moduli_f;      // per system B-spline moduli, x then y then z
     DevBuf<double> moduli_d;
-    DevBuf<int> radix_dev;       // [3][8] radices per axis, 0-terminated
+    // Workgroup prefix tables of the five FFT passes, per precision:
+    // fft_off[prec][pass] has nsys + 1 entries (prec 0: FP32, 1: FP64).
+    DevBuf<int> fft_off[2][5];
+    int fft_blocks[2][5] = {};
+    size_t fft_shmem[2][3] = {};  // dynamic LDS per axis
 };
 
 // Grid dimensions must each be >= 5 (pme.rs:222-225) and at most 1024.
-void pme_setup(PmeDev& pm, double alpha, const int dims[3], bool mixed, hipStream_t s);
+// Reads sys[s].grid and sys[s].alpha; writes the grid offsets, radices,
+// root/moduli offsets and FFT line counts into sys[s].
+void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, bool mixed, hipStream_t s);
 
-// Spread, FFT, convolution (+energy into slot E_RECIP), inverse FFT, gather
-// (forces added).  Atoms with zero charge are skipped as in the reference.
+// Spread, FFT, convolution (+energy into slot E_RECIP of each system), inverse
+// FFT, gather (forces added).  Atoms with zero charge are skipped as in the
+// reference.
 enum : int { PME_SPREAD = 1, PME_FFT = 2, PME_GATHER = 4, PME_ALL = 7 };
 // Arithmetic of one PME evaluation.
 //   MESH_F32      : FP32 splines/FFT/gather, fixed-point grid and forces (mixed
@@ -37,9 +46,11 @@ enum : int { PME_SPREAD = 1, PME_FFT = 2, PME_GATHER = 4, PME_ALL = 7 };
 //                   relative error);
 //   MESH_F64      : FP64 throughout with FP64 atomics (plain form).
 enum MeshMode : int { MESH_F32 = 0, MESH_F64_FIXED = 1, MESH_F64 = 2 };
-void pme_run(PmeDev& pm, int mode, int n, const double* pos, const double* charge, const Box* box_dev,
-             long long* force_fx, double* force_d, long long* energy_fx, double* energy_d, unsigned* status,
-             hipStream_t s, int phases = PME_ALL);
+// The atoms are visited in the list's sorted slot order (nl.sorted_atom),
+// which only sets locality: any permutation gives the same fixed-point sums.
+void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
+             const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
+             unsigned* status, hipStream_t s, int phases = PME_ALL);
 
 // B-spline moduli |sum_i M5(i+1) e^{2 pi i k i/n}|^2 with near-zero values
 // replaced by the mean of their neighbours (pme.rs:239-259).

@@ -10,6 +10,7 @@
 // kz <= nz/2 with weights 1 (kz = 0, nz/2) and 2 (otherwise) carries the same
 // sum.  Any grid size is supported: radices are the prime factors of n (8 and
 // 4 preferred), each stage a direct DFT of its radix.
+#include <climits>
 #include <cmath>
 #include <complex>
 
@@ -20,7 +21,6 @@ namespace ommhip {
 namespace {
 
 constexpr int FFT_THREADS = 256;
-constexpr int MAX_STAGES = 12;
 
 template <class R>
 struct Cx {
@@ -64,86 +64,243 @@ __device__ __forceinline__ void stencil(const double* p, const Box& b, const int
     }
 }
 
+// Spread with an LDS-privatised brick.  A workgroup takes SPREAD_ATOMS
+// consecutive atoms of the spatially sorted order (any permutation is correct;
+// the sort only gives locality).  Their stencils are unwrapped relative to the
+// first atom's base point; if the covering brick fits in LDS, the 125
+// contributions per atom go to int64 LDS atomics and the brick is flushed with
+// one global atomic per non-zero point (about a quarter of the per-atom
+// atomics for water at PME spacings); otherwise the workgroup falls back to
+// global atomics.  All sums are fixed point, so the result is identical.
+constexpr int SPREAD_ATOMS = 64;
+constexpr int SPREAD_THREADS = 256;
+constexpr int SPREAD_BRICK = 4096;  // int64 points in LDS (32 KB)
+
+__device__ __forceinline__ void grid_add_fixed(long long* dst, long long v, unsigned*) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
+}
+__device__ __forceinline__ void grid_add_fixed(double*, long long, unsigned*) {}
+
 template <class R, class G>
-__global__ void k_spread(int n, const double* __restrict__ pos, const double* __restrict__ charge,
-                         const Box* __restrict__ box, int nx, int ny, int nz, G* grid, unsigned* status) {
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    int atom = gid / 5, ix = gid - atom * 5;
-    if (atom >= n) return;
-    double q = charge[atom];
-    if (q == 0.0) return;
+__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_lds(const int* __restrict__ chunk_off, int nsys,
+                                                               const int* __restrict__ order,
+                                                               const double* __restrict__ pos,
+                                                               const double* __restrict__ charge,
+                                                               const SysInfo* __restrict__ sys, G* grid,
+                                                               unsigned* status) {
+    static_assert(SPREAD_ATOMS == 64 && SPREAD_ATOMS == PME_CHUNK, "the extent reduction runs in one wave64");
+    __shared__ unsigned long long s_brick[SPREAD_BRICK];
+    __shared__ R s_w[SPREAD_ATOMS][3][5];
+    __shared__ int s_rel[SPREAD_ATOMS][3];
+    __shared__ double s_q[SPREAD_ATOMS];
+    __shared__ int s_base[3], s_ext[3], s_fits;
+    constexpr bool FIXED = sizeof(G) == sizeof(long long) && G(0.5) == G(0);  // int64 grid
+    const int si = find_sys(chunk_off, nsys, blockIdx.x);
+    const SysInfo& S = sys[si];
+    const int nx = S.grid[0], ny = S.grid[1], nz = S.grid[2];
     const int dims[3] = {nx, ny, nz};
-    int idx[3];
-    R w[3][5], dw[3][5];
-    stencil<R>(pos + 3 * atom, *box, dims, idx, w, dw);
-    const R qw = (R)q * w[0][ix];
-    const int gx = (idx[0] + ix) % nx;
-    for (int iy = 0; iy < 5; ++iy) {
-        const int gy = (idx[1] + iy) % ny;
-        const R qwy = qw * w[1][iy];
-        G* row = grid + ((size_t)gx * ny + gy) * nz;
-        for (int iz = 0; iz < 5; ++iz) {
-            const int gz = (idx[2] + iz) % nz;
-            acc_add(row + gz, (double)(qwy * w[2][iz]), GRID_SCALE, status);
+    grid += S.qgrid_off;
+    const int t = threadIdx.x;
+    if (t < SPREAD_ATOMS) {
+        // Wave 0: one atom per lane; stencil, then the brick covering all of
+        // them, unwrapped around the first charged atom's base point.
+        const int slot = S.slot_off + (blockIdx.x - S.chunk_off) * SPREAD_ATOMS + t;
+        const int a = slot < S.slot_off + S.nblocks * IBLOCK ? order[slot] : -1;
+        const double q = a >= 0 ? charge[a] : 0.0;
+        int idx[3] = {0, 0, 0};
+        if (q != 0.0) {
+            R dw[3][5];
+            R w[3][5];
+            stencil<R>(pos + 3 * a, S.box, dims, idx, w, dw);
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 5; ++k) s_w[t][d][k] = w[d][k];
+        }
+        s_q[t] = q;
+        const unsigned long long live = __ballot(q != 0.0);
+        if (!live) {
+            if (t == 0) s_fits = -1;
+        } else {
+            const int ref = __ffsll((long long)live) - 1;
+            int lo[3], hi[3];
+            for (int d = 0; d < 3; ++d) {
+                const int r0 = __shfl(idx[d], ref, 64);
+                int r = idx[d] - r0;  // in (-n, n)
+                const int h = dims[d] >> 1;
+                if (r < -h) r += dims[d];
+                if (r >= dims[d] - h) r -= dims[d];
+                lo[d] = q != 0.0 ? r : (1 << 30);
+                hi[d] = q != 0.0 ? r + 4 : -(1 << 30);
+                for (int off = 32; off >= 1; off >>= 1) {
+                    lo[d] = min(lo[d], __shfl_xor(lo[d], off, 64));
+                    hi[d] = max(hi[d], __shfl_xor(hi[d], off, 64));
+                }
+                if (t == 0) {
+                    s_base[d] = r0 + lo[d];
+                    s_ext[d] = hi[d] - lo[d] + 1;
+                }
+                s_rel[t][d] = r - lo[d];
+            }
+            if (t == 0) {
+                const long long vol = (long long)s_ext[0] * s_ext[1] * s_ext[2];
+                s_fits = (FIXED && vol <= SPREAD_BRICK) ? 1 : 0;
+            }
+        }
+    }
+    __syncthreads();
+    const int fits = s_fits;
+    if (fits < 0) return;  // no charged atom in this chunk
+    const int ex = s_ext[0], ey = s_ext[1], ez = s_ext[2];
+    if (fits)
+        for (int i = t; i < ex * ey * ez; i += SPREAD_THREADS) s_brick[i] = 0ull;
+    __syncthreads();
+    for (int item = t; item < SPREAD_ATOMS * 25; item += SPREAD_THREADS) {
+        const int k = item / 25, ixy = item - k * 25;
+        const double q = s_q[k];
+        if (q == 0.0) continue;
+        const int ix = ixy / 5, iy = ixy - ix * 5;
+        const R wxy = s_w[k][0][ix] * s_w[k][1][iy];
+        const int rx = s_rel[k][0] + ix, ry = s_rel[k][1] + iy, rz = s_rel[k][2];
+        if (fits) {
+            unsigned long long* row = s_brick + ((size_t)rx * ey + ry) * ez + rz;
+            for (int iz = 0; iz < 5; ++iz) {
+                const long long f = to_fixed(q * (double)(wxy * s_w[k][2][iz]), GRID_SCALE, status);
+                if (f) atomicAdd(row + iz, (unsigned long long)f);
+            }
+        } else {
+            int gx = s_base[0] + rx, gy = s_base[1] + ry;
+            while (gx < 0) gx += nx;
+            while (gx >= nx) gx -= nx;
+            while (gy < 0) gy += ny;
+            while (gy >= ny) gy -= ny;
+            G* row = grid + ((size_t)gx * ny + gy) * nz;
+            int gz = s_base[2] + rz;
+            while (gz < 0) gz += nz;
+            while (gz >= nz) gz -= nz;
+            for (int iz = 0; iz < 5; ++iz) {
+                acc_add(row + gz, q * (double)(wxy * s_w[k][2][iz]), GRID_SCALE, status);
+                if (++gz == nz) gz = 0;
+            }
+        }
+    }
+    if (!fits) return;
+    __syncthreads();
+    // Flush one (x, y) row of the brick per thread.
+    for (int rowi = t; rowi < ex * ey; rowi += SPREAD_THREADS) {
+        const int lx = rowi / ey, ly = rowi - lx * ey;
+        int gx = s_base[0] + lx, gy = s_base[1] + ly, gz = s_base[2];
+        while (gx < 0) gx += nx;
+        while (gx >= nx) gx -= nx;
+        while (gy < 0) gy += ny;
+        while (gy >= ny) gy -= ny;
+        while (gz < 0) gz += nz;
+        while (gz >= nz) gz -= nz;
+        G* dst = grid + ((size_t)gx * ny + gy) * nz;
+        const unsigned long long* src = s_brick + (size_t)rowi * ez;
+        for (int lz = 0; lz < ez; ++lz) {
+            const unsigned long long v = src[lz];
+            if (v) grid_add_fixed(dst + gz, (long long)v, status);
+            if (++gz == nz) gz = 0;
         }
     }
 }
 
+// Gather with five threads per atom (one per x plane of the stencil), atoms
+// in the spatially sorted order; the five partial gradients are combined in
+// LDS.
+constexpr int GATHER_ATOMS = PME_CHUNK;
+
 template <class R, class A>
-__global__ void k_gather(int n, const double* __restrict__ pos, const double* __restrict__ charge,
-                         const Box* __restrict__ box, int nx, int ny, int nz, const R* __restrict__ phi, A* force,
-                         unsigned* status) {
-    int atom = blockIdx.x * blockDim.x + threadIdx.x;
-    if (atom >= n) return;
-    double q = charge[atom];
-    if (q == 0.0) return;
-    const Box& b = *box;
-    const int dims[3] = {nx, ny, nz};
-    int idx[3];
-    R w[3][5], dw[3][5];
-    stencil<R>(pos + 3 * atom, b, dims, idx, w, dw);
+__global__ __launch_bounds__(GATHER_ATOMS * 5) void k_gather5(const int* __restrict__ chunk_off, int nsys,
+                                                              const int* __restrict__ order,
+                                                              const double* __restrict__ pos,
+                                                              const double* __restrict__ charge,
+                                                              const SysInfo* __restrict__ sys,
+                                                              const R* __restrict__ phi, A* force, unsigned* status) {
+    __shared__ R s_g[GATHER_ATOMS][5][3];
+    const int si = find_sys(chunk_off, nsys, blockIdx.x);
+    const SysInfo& S = sys[si];
+    const int nx = S.grid[0], ny = S.grid[1], nz = S.grid[2];
+    phi += S.qgrid_off;
+    const int t = threadIdx.x;
+    const int k = t / 5, ix = t - k * 5;
+    const int slot = S.slot_off + (blockIdx.x - S.chunk_off) * GATHER_ATOMS + k;
+    const int a = slot < S.slot_off + S.nblocks * IBLOCK ? order[slot] : -1;
+    const double q = a >= 0 ? charge[a] : 0.0;
+    const Box& b = S.box;
     R g0 = 0, g1 = 0, g2 = 0;
-    for (int ix = 0; ix < 5; ++ix) {
-        const int gx = (idx[0] + ix) % nx;
+    if (q != 0.0) {
+        const int dims[3] = {nx, ny, nz};
+        int idx[3];
+        R w[3][5], dw[3][5];
+        stencil<R>(pos + 3 * a, b, dims, idx, w, dw);
+        int gx = idx[0] + ix;
+        if (gx >= nx) gx -= nx;
+        int zz[5];
+        for (int iz = 0; iz < 5; ++iz) {
+            zz[iz] = idx[2] + iz;
+            if (zz[iz] >= nz) zz[iz] -= nz;
+        }
         for (int iy = 0; iy < 5; ++iy) {
-            const int gy = (idx[1] + iy) % ny;
+            int gy = idx[1] + iy;
+            if (gy >= ny) gy -= ny;
             const R wxy = w[0][ix] * w[1][iy];
             const R d0 = dw[0][ix] * w[1][iy];
             const R d1 = w[0][ix] * dw[1][iy];
             const R* row = phi + ((size_t)gx * ny + gy) * nz;
             for (int iz = 0; iz < 5; ++iz) {
-                const R p = row[(idx[2] + iz) % nz];
+                const R p = row[zz[iz]];
                 g0 += p * d0 * w[2][iz];
                 g1 += p * d1 * w[2][iz];
                 g2 += p * wxy * dw[2][iz];
             }
         }
     }
-    // pme.rs:417-422
-    for (int a = 0; a < 3; ++a) {
-        double s = (double)g0 * nx * b.recip[0 * 3 + a] + (double)g1 * ny * b.recip[1 * 3 + a] +
-                   (double)g2 * nz * b.recip[2 * 3 + a];
-        acc_add(force + 3 * atom + a, -q * s, FORCE_SCALE, status);
+    s_g[k][ix][0] = g0;
+    s_g[k][ix][1] = g1;
+    s_g[k][ix][2] = g2;
+    __syncthreads();
+    if (ix == 0 && q != 0.0) {
+        R G0 = 0, G1 = 0, G2 = 0;
+        for (int m = 0; m < 5; ++m) {
+            G0 += s_g[k][m][0];
+            G1 += s_g[k][m][1];
+            G2 += s_g[k][m][2];
+        }
+        // pme.rs:417-422
+        for (int c = 0; c < 3; ++c) {
+            double s = (double)G0 * nx * b.recip[0 * 3 + c] + (double)G1 * ny * b.recip[1 * 3 + c] +
+                       (double)G2 * nz * b.recip[2 * 3 + c];
+            acc_add(force + 3 * a + c, -q * s, FORCE_SCALE, status);
+        }
     }
 }
 
 enum : int { MODE_Z_R2C = 0, MODE_Y_FWD = 1, MODE_X_CONV = 2, MODE_Y_INV = 3, MODE_Z_C2R = 4 };
 
-struct FftConst {
-    int nx, ny, nz, nzh;
-    int lines_per_block;  // power of two dividing FFT_THREADS
-    double alpha;
-};
 
 __device__ __forceinline__ double grid_real(long long v) { return (double)v * (1.0 / GRID_SCALE); }
 __device__ __forceinline__ double grid_real(double v) { return v; }
 
-// Batched Stockham FFT of `lines` lines of length n held line-major in a/b.
-// Each stage of radix R is evaluated as a direct DFT per output:
-// out[(j/Ns)*Ns*R + k + q*Ns] = sum_r in[j + r*m] W_n^{r (k n/(Ns R) + q m)}.
+// x / d for 0 <= x < 2^20 and 1 <= d: a single-precision estimate corrected
+// by one step either way.
+__device__ __forceinline__ int div_small(int x, int d, float inv_d) {
+    int q = (int)((float)x * inv_d);
+    if (q * d > x) --q;
+    else if ((q + 1) * d <= x) ++q;
+    return q;
+}
+
+// Batched Stockham FFT of `lines` lines of length n held line-major in a/b,
+// T threads per line.  A stage of radix R is a direct DFT per output:
+// out[(j - k) R + k + q Ns] = sum_r in[j + r m] W_n^{r (k n/(Ns R) + q m)},
+// m = n/R, output index o = q m + j, k = j mod Ns.  The exponent base
+// k n/(Ns R) + q m stays below n, so the root index wraps with one conditional
+// subtraction.  Each output is computed by one thread in a fixed order, so
+// the result does not depend on T or on how lines are grouped.
 template <class R>
 __device__ Cx<R>* fft_lds(Cx<R>* a, Cx<R>* b, int n, const int* radices, const Cx<R>* roots, bool inverse, int T) {
-    const int li = threadIdx.x / T, t = threadIdx.x % T;
+    const int li = threadIdx.x / T, t = threadIdx.x - li * T;
+    const R sgn = inverse ? (R)-1 : (R)1;
     int Ns = 1;
     for (int s = 0; s < MAX_STAGES && radices[s]; ++s) {
         const int Rr = radices[s];
@@ -151,22 +308,25 @@ __device__ Cx<R>* fft_lds(Cx<R>* a, Cx<R>* b, int n, const int* radices, const C
         const int step = n / (Ns * Rr);
         const Cx<R>* in = a + li * n;
         Cx<R>* out = b + li * n;
+        const float inv_ns = 1.0f / (float)Ns;
+        const float inv_m = 1.0f / (float)m;
         for (int o = t; o < n; o += T) {
-            const int q = o / m, j = o - q * m;
-            const int k = j % Ns;
-            const int base = (k * step + q * m) % n;
+            const int q = div_small(o, m, inv_m);
+            const int j = o - q * m;
+            const int k = j - div_small(j, Ns, inv_ns) * Ns;
+            const int base = k * step + q * m;
             R ax = 0, ay = 0;
             int idx = 0;
             for (int r = 0; r < Rr; ++r) {
-                Cx<R> w = roots[idx];
-                if (inverse) w.y = -w.y;
+                const Cx<R> w = roots[idx];
+                const R wy = sgn * w.y;
                 const Cx<R> v = in[j + r * m];
-                ax += v.x * w.x - v.y * w.y;
-                ay += v.x * w.y + v.y * w.x;
+                ax += v.x * w.x - v.y * wy;
+                ay += v.x * wy + v.y * w.x;
                 idx += base;
                 if (idx >= n) idx -= n;
             }
-            out[(j / Ns) * Ns * Rr + k + q * Ns] = Cx<R>{ax, ay};
+            out[(j - k) * Rr + k + q * Ns] = Cx<R>{ax, ay};
         }
         __syncthreads();
         Cx<R>* tmp = a;
@@ -178,27 +338,32 @@ __device__ Cx<R>* fft_lds(Cx<R>* a, Cx<R>* b, int n, const int* radices, const C
 }
 
 template <class R, class G, class A, int MODE>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft(FftConst C, const G* __restrict__ qgrid, Cx<R>* cgrid, R* phi,
-                                                     const Cx<R>* __restrict__ roots_all,
-                                                     const R* __restrict__ moduli_all, const int* __restrict__ radix_all,
-                                                     const Box* __restrict__ box, A* energy, unsigned* status) {
+__global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk_off, int nsys,
+                                                     const SysInfo* __restrict__ sys, G* __restrict__ qgrid,
+                                                     Cx<R>* cgrid, R* phi, const Cx<R>* __restrict__ roots_all,
+                                                     const R* __restrict__ moduli_all, A* energy, unsigned* status) {
     extern __shared__ unsigned char smem_raw[];
     const bool zmode = MODE == MODE_Z_R2C || MODE == MODE_Z_C2R;
     const bool ymode = MODE == MODE_Y_FWD || MODE == MODE_Y_INV;
     const int axis = zmode ? 2 : (ymode ? 1 : 0);
-    const int n = zmode ? C.nz : (ymode ? C.ny : C.nx);
-    const int LINES = C.lines_per_block;
+    const int si = find_sys(blk_off, nsys, blockIdx.x);
+    const SysInfo& S = sys[si];
+    const int nx = S.grid[0], ny = S.grid[1], nz = S.grid[2], nzh = S.nzh;
+    const int n = S.grid[axis];
+    const int LINES = S.fft_lines[sizeof(R) == 8 ? 1 : 0][axis];
     const int T = FFT_THREADS / LINES;
-    const int nlines = zmode ? C.nx * C.ny : (ymode ? C.nx * C.nzh : C.ny * C.nzh);
-    const int line0 = blockIdx.x * LINES;
-    const int root_off = axis == 0 ? 0 : (axis == 1 ? C.nx : C.nx + C.ny);
+    const int nlines = zmode ? nx * ny : (ymode ? nx * nzh : ny * nzh);
+    const int line0 = (blockIdx.x - blk_off[si]) * LINES;
+    qgrid += S.qgrid_off;
+    phi += S.qgrid_off;
+    cgrid += S.cgrid_off;
 
     Cx<R>* bufA = reinterpret_cast<Cx<R>*>(smem_raw);
     Cx<R>* bufB = bufA + LINES * n;
     Cx<R>* roots = bufB + LINES * n;
     __shared__ int s_radix[MAX_STAGES];
-    if (threadIdx.x < MAX_STAGES) s_radix[threadIdx.x] = radix_all[axis * MAX_STAGES + threadIdx.x];
-    for (int i = threadIdx.x; i < n; i += FFT_THREADS) roots[i] = roots_all[root_off + i];
+    if (threadIdx.x < MAX_STAGES) s_radix[threadIdx.x] = S.radix[axis][threadIdx.x];
+    for (int i = threadIdx.x; i < n; i += FFT_THREADS) roots[i] = roots_all[S.roots_off[axis] + i];
 
     // Load.
     for (int e = threadIdx.x; e < LINES * n; e += FFT_THREADS) {
@@ -214,20 +379,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(FftConst C, const G* __rest
         Cx<R> v = {0, 0};
         if (L < nlines) {
             if (MODE == MODE_Z_R2C) {
-                v.x = (R)grid_real(qgrid[(size_t)L * C.nz + el]);
+                // The charge grid is consumed here and left zeroed for the
+                // next spread (no separate clearing kernel).
+                const size_t gi = (size_t)L * nz + el;
+                v.x = (R)grid_real(qgrid[gi]);
+                qgrid[gi] = (G)0;
             } else if (MODE == MODE_Z_C2R) {
-                if (el < C.nzh) {
-                    v = cgrid[(size_t)L * C.nzh + el];
+                if (el < nzh) {
+                    v = cgrid[(size_t)L * nzh + el];
                 } else {
-                    v = cgrid[(size_t)L * C.nzh + (C.nz - el)];
+                    v = cgrid[(size_t)L * nzh + (nz - el)];
                     v.y = -v.y;
                 }
             } else if (ymode) {
-                const int x = L / C.nzh, kz = L - x * C.nzh;
-                v = cgrid[((size_t)x * C.ny + el) * C.nzh + kz];
+                const int x = L / nzh, kz = L - x * nzh;
+                v = cgrid[((size_t)x * ny + el) * nzh + kz];
             } else {
-                const int y = L / C.nzh, kz = L - y * C.nzh;
-                v = cgrid[((size_t)el * C.ny + y) * C.nzh + kz];
+                const int y = L / nzh, kz = L - y * nzh;
+                v = cgrid[((size_t)el * ny + y) * nzh + kz];
             }
         }
         bufA[li * n + el] = v;
@@ -239,26 +408,26 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(FftConst C, const G* __rest
 
     if (MODE == MODE_X_CONV) {
         // Reciprocal convolution and energy (pme.rs:332-372) on the half spectrum.
-        const Box& b = *box;
-        const R* mod_x = moduli_all;
-        const R* mod_y = moduli_all + C.nx;
-        const R* mod_z = moduli_all + C.nx + C.ny;
-        const double pi2_a2 = PI * PI / (C.alpha * C.alpha);
+        const Box& b = S.box;
+        const R* mod_x = moduli_all + S.moduli_off[0];
+        const R* mod_y = moduli_all + S.moduli_off[1];
+        const R* mod_z = moduli_all + S.moduli_off[2];
+        const double pi2_a2 = PI * PI / (S.alpha * S.alpha);
         const double pref = ONE_4PI_EPS0 / (PI * b.volume);
         double eacc = 0.0;
         for (int e = threadIdx.x; e < LINES * n; e += FFT_THREADS) {
             const int li = e % LINES, kx = e / LINES;
             const int L = line0 + li;
             if (L >= nlines) continue;
-            const int y = L / C.nzh, kz = L - y * C.nzh;
+            const int y = L / nzh, kz = L - y * nzh;
             Cx<R>& S = res[li * n + kx];
             if (kx == 0 && y == 0 && kz == 0) {
                 S = Cx<R>{0, 0};
                 continue;
             }
-            const double mx = kx < (C.nx + 1) / 2 ? kx : kx - C.nx;
-            const double my = y < (C.ny + 1) / 2 ? y : y - C.ny;
-            const double mz = kz < (C.nz + 1) / 2 ? kz : kz - C.nz;
+            const double mx = kx < (nx + 1) / 2 ? kx : kx - nx;
+            const double my = y < (ny + 1) / 2 ? y : y - ny;
+            const double mz = kz < (nz + 1) / 2 ? kz : kz - nz;
             double k2 = 0.0;
             for (int a = 0; a < 3; ++a) {
                 double kv = mx * b.recip[0 * 3 + a] + my * b.recip[1 * 3 + a] + mz * b.recip[2 * 3 + a];
@@ -266,14 +435,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(FftConst C, const G* __rest
             }
             const double mod = (double)mod_x[kx] * (double)mod_y[y] * (double)mod_z[kz];
             const double infl = pref * exp(-pi2_a2 * k2) / (k2 * mod);
-            const double wz = (kz == 0 || 2 * kz == C.nz) ? 1.0 : 2.0;
+            const double wz = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
             const double s2 = (double)S.x * S.x + (double)S.y * S.y;
             eacc += wz * 0.5 * infl * s2;
             S.x = (R)(S.x * infl);
             S.y = (R)(S.y * infl);
         }
         double et = wave_sum(eacc);
-        if ((threadIdx.x & 63) == 0) acc_add(energy + E_RECIP, et, ENERGY_SCALE, status);
+        if ((threadIdx.x & 63) == 0) acc_add(energy + si * E_SLOTS + E_RECIP, et, ENERGY_SCALE, status);
         __syncthreads();
         Cx<R>* other = res == bufA ? bufB : bufA;
         res = fft_lds<R>(res, other, n, rad, roots, true, T);
@@ -293,15 +462,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(FftConst C, const G* __rest
         if (L >= nlines) continue;
         const Cx<R> v = res[li * n + el];
         if (MODE == MODE_Z_R2C) {
-            if (el < C.nzh) cgrid[(size_t)L * C.nzh + el] = v;
+            if (el < nzh) cgrid[(size_t)L * nzh + el] = v;
         } else if (MODE == MODE_Z_C2R) {
-            phi[(size_t)L * C.nz + el] = v.x;
+            phi[(size_t)L * nz + el] = v.x;
         } else if (ymode) {
-            const int x = L / C.nzh, kz = L - x * C.nzh;
-            cgrid[((size_t)x * C.ny + el) * C.nzh + kz] = v;
+            const int x = L / nzh, kz = L - x * nzh;
+            cgrid[((size_t)x * ny + el) * nzh + kz] = v;
         } else {
-            const int y = L / C.nzh, kz = L - y * C.nzh;
-            cgrid[((size_t)el * C.ny + y) * C.nzh + kz] = v;
+            const int y = L / nzh, kz = L - y * nzh;
+            cgrid[((size_t)el * ny + y) * nzh + kz] = v;
         }
     }
 }
@@ -329,47 +498,45 @@ std::vector<int> factorize(int n) {
     return r;
 }
 
-int lines_for(int n, size_t csize) {
+// Most lines per workgroup whose two LDS buffers fit in 60 KB (at most 16).
+int max_lines(int n, size_t csize) {
     int lines = 16;
     while (lines > 1 && (2 * (size_t)lines * n + n) * csize > 60000) lines >>= 1;
     return lines;
 }
 
 template <class R, class G, class A, int MODE>
-void launch_fft(const PmeDev& pm, FftConst C, const G* qgrid, Cx<R>* cgrid, R* phi, const R* roots, const R* moduli,
-                const Box* box, A* energy, unsigned* status, hipStream_t s) {
+void launch_fft(const PmeDev& pm, const SysInfo* sys, G* qgrid, Cx<R>* cgrid, R* phi, const R* roots, const R* moduli,
+                A* energy, unsigned* status, hipStream_t s) {
+    const int prec = sizeof(R) == 8 ? 1 : 0;
     const bool zmode = MODE == MODE_Z_R2C || MODE == MODE_Z_C2R;
     const bool ymode = MODE == MODE_Y_FWD || MODE == MODE_Y_INV;
-    const int n = zmode ? C.nz : (ymode ? C.ny : C.nx);
-    const int nlines = zmode ? C.nx * C.ny : (ymode ? C.nx * C.nzh : C.ny * C.nzh);
-    C.lines_per_block = lines_for(n, sizeof(Cx<R>));
-    const int blocks = (nlines + C.lines_per_block - 1) / C.lines_per_block;
-    const size_t shmem = (2 * (size_t)C.lines_per_block * n + n) * sizeof(Cx<R>);
-    hipLaunchKernelGGL((k_fft<R, G, A, MODE>), dim3(blocks), dim3(FFT_THREADS), shmem, s, C, qgrid, cgrid, phi,
-                       reinterpret_cast<const Cx<R>*>(roots), moduli, pm.radix_dev.p, box, energy, status);
+    const int axis = zmode ? 2 : (ymode ? 1 : 0);
+    hipLaunchKernelGGL((k_fft<R, G, A, MODE>), dim3(pm.fft_blocks[prec][MODE]), dim3(FFT_THREADS),
+                       pm.fft_shmem[prec][axis], s, pm.fft_off[prec][MODE].p, pm.nsys, sys, qgrid, cgrid, phi,
+                       reinterpret_cast<const Cx<R>*>(roots), moduli, energy, status);
 }
 
 template <class R, class G, class A>
-void run_t(PmeDev& pm, int n, const double* pos, const double* charge, const Box* box, G* qgrid, R* cgrid, R* phi,
-           const R* roots, const R* moduli, A* force, A* energy, unsigned* status, int phases, hipStream_t s) {
-    const size_t G3 = (size_t)pm.nx * pm.ny * pm.nz;
+void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* charge, const SysInfo* sys, G* qgrid,
+           R* cgrid, R* phi, const R* roots, const R* moduli, A* force, A* energy, unsigned* status, int phases,
+           hipStream_t s) {
     if (phases & PME_SPREAD) {
-        fill_async<G>(qgrid, G3, (G)0, s);
-        hipLaunchKernelGGL((k_spread<R, G>), dim3((5 * n + 255) / 256), dim3(256), 0, s, n, pos, charge, box, pm.nx,
-                           pm.ny, pm.nz, qgrid, status);
+        // qgrid is zero here: the z pass of the previous evaluation cleared it.
+        hipLaunchKernelGGL((k_spread_lds<R, G>), dim3(nl.nchunks), dim3(SPREAD_THREADS), 0, s, nl.chunk_off.p,
+                           pm.nsys, nl.sorted_atom.p, pos, charge, sys, qgrid, status);
     }
     if (phases & PME_FFT) {
-        FftConst C{pm.nx, pm.ny, pm.nz, pm.nzh, 16, pm.alpha};
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);
-        launch_fft<R, G, A, MODE_Z_R2C>(pm, C, qgrid, cg, phi, roots, moduli, box, energy, status, s);
-        launch_fft<R, G, A, MODE_Y_FWD>(pm, C, qgrid, cg, phi, roots, moduli, box, energy, status, s);
-        launch_fft<R, G, A, MODE_X_CONV>(pm, C, qgrid, cg, phi, roots, moduli, box, energy, status, s);
-        launch_fft<R, G, A, MODE_Y_INV>(pm, C, qgrid, cg, phi, roots, moduli, box, energy, status, s);
-        launch_fft<R, G, A, MODE_Z_C2R>(pm, C, qgrid, cg, phi, roots, moduli, box, energy, status, s);
+        launch_fft<R, G, A, MODE_Z_R2C>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        launch_fft<R, G, A, MODE_Y_FWD>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        launch_fft<R, G, A, MODE_X_CONV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        launch_fft<R, G, A, MODE_Y_INV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        launch_fft<R, G, A, MODE_Z_C2R>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
     }
     if (phases & PME_GATHER)
-        hipLaunchKernelGGL((k_gather<R, A>), dim3((n + 127) / 128), dim3(128), 0, s, n, pos, charge, box, pm.nx,
-                           pm.ny, pm.nz, phi, force, status);
+        hipLaunchKernelGGL((k_gather5<R, A>), dim3(nl.nchunks), dim3(GATHER_ATOMS * 5), 0, s, nl.chunk_off.p, pm.nsys,
+                           nl.sorted_atom.p, pos, charge, sys, phi, force, status);
     OMM_HIP(hipGetLastError());
 }
 
@@ -394,61 +561,115 @@ std::vector<double> bspline_moduli(int n) {
     return values;
 }
 
-void pme_setup(PmeDev& pm, double alpha, const int dims[3], bool mixed, hipStream_t s) {
-    for (int a = 0; a < 3; ++a)
-        if (dims[a] < 5 || dims[a] > 1024) throw Error(OMM_ERR_INVALID, "PME grid dimensions must be in [5, 1024]");
-    pm.nx = dims[0];
-    pm.ny = dims[1];
-    pm.nz = dims[2];
-    pm.nzh = pm.nz / 2 + 1;
-    pm.alpha = alpha;
-    std::vector<int> rad(3 * MAX_STAGES, 0);
-    for (int a = 0; a < 3; ++a) {
-        pm.radix[a] = factorize(dims[a]);
-        if ((int)pm.radix[a].size() >= MAX_STAGES) throw Error(OMM_ERR_INVALID, "PME grid dimension has too many factors");
-        for (size_t k = 0; k < pm.radix[a].size(); ++k) rad[a * MAX_STAGES + k] = pm.radix[a][k];
-    }
-    pm.radix_dev.upload(rad, s);
+void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, bool mixed, hipStream_t s) {
+    const int nsys = (int)sys.size();
+    pm.nsys = nsys;
     std::vector<double> roots, moduli;
-    for (int a = 0; a < 3; ++a) {
-        int n = dims[a];
-        for (int m = 0; m < n; ++m) {
-            double t = -2.0 * PI * (double)m / (double)n;
-            roots.push_back(std::cos(t));
-            roots.push_back(std::sin(t));
+    std::vector<int> off[2][5];
+    for (int p = 0; p < 2; ++p)
+        for (int k = 0; k < 5; ++k) {
+            off[p][k].assign(1, 0);
+            pm.fft_shmem[p][k % 3] = 0;
         }
-        auto mod = bspline_moduli(n);
-        moduli.insert(moduli.end(), mod.begin(), mod.end());
+    long long qpts = 0, cpts = 0;
+    for (int si = 0; si < nsys; ++si) {
+        SysInfo& S = sys[si];
+        for (int a = 0; a < 3; ++a)
+            if (S.grid[a] < 5 || S.grid[a] > 1024)
+                throw Error(OMM_ERR_INVALID, "PME grid dimensions must be in [5, 1024]");
+        S.nzh = S.grid[2] / 2 + 1;
+        S.qgrid_off = qpts;
+        S.cgrid_off = cpts;
+        qpts += (long long)S.grid[0] * S.grid[1] * S.grid[2];
+        cpts += (long long)S.grid[0] * S.grid[1] * S.nzh;
+        for (int a = 0; a < 3; ++a) {
+            const int n = S.grid[a];
+            std::vector<int> f = factorize(n);
+            if ((int)f.size() >= MAX_STAGES) throw Error(OMM_ERR_INVALID, "PME grid dimension has too many factors");
+            for (int k = 0; k < MAX_STAGES; ++k) S.radix[a][k] = k < (int)f.size() ? f[k] : 0;
+            S.roots_off[a] = (int)(roots.size() / 2);
+            for (int m = 0; m < n; ++m) {
+                double t = -2.0 * PI * (double)m / (double)n;
+                roots.push_back(std::cos(t));
+                roots.push_back(std::sin(t));
+            }
+            S.moduli_off[a] = (int)moduli.size();
+            auto mod = bspline_moduli(n);
+            moduli.insert(moduli.end(), mod.begin(), mod.end());
+        }
     }
-    const size_t G3 = (size_t)pm.nx * pm.ny * pm.nz;
-    const size_t GH = (size_t)pm.nx * pm.ny * pm.nzh;
+    // Lines per workgroup: the most that fit in LDS, lowered for the whole
+    // batch together until every pass has ~2048 workgroups (8 per CU) or two
+    // lines per workgroup.  The transforms do not depend on this grouping.
+    const int pass_axis[5] = {2, 1, 0, 1, 2};  // MODE_Z_R2C, Y_FWD, X_CONV, Y_INV, Z_C2R
+    auto nlines = [&](const SysInfo& S, int a) {
+        return a == 0 ? S.grid[1] * S.nzh : (a == 1 ? S.grid[0] * S.nzh : S.grid[0] * S.grid[1]);
+    };
+    for (int p = 0; p < 2; ++p) {
+        const size_t csize = p ? 2 * sizeof(double) : 2 * sizeof(float);
+        int cap = 16;
+        for (;; cap >>= 1) {
+            long long fewest = LLONG_MAX;
+            for (int k = 0; k < 5; ++k) {
+                long long blocks = 0;
+                for (const SysInfo& S : sys) {
+                    const int a = pass_axis[k];
+                    const int l = std::min(cap, max_lines(S.grid[a], csize));
+                    blocks += (nlines(S, a) + l - 1) / l;
+                }
+                fewest = std::min(fewest, blocks);
+            }
+            if (fewest >= 2048 || cap <= 2) break;
+        }
+        for (SysInfo& S : sys) {
+            for (int a = 0; a < 3; ++a) {
+                S.fft_lines[p][a] = std::min(cap, max_lines(S.grid[a], csize));
+                pm.fft_shmem[p][a] =
+                    std::max(pm.fft_shmem[p][a], (2 * (size_t)S.fft_lines[p][a] * S.grid[a] + S.grid[a]) * csize);
+            }
+            for (int k = 0; k < 5; ++k) {
+                const int a = pass_axis[k];
+                off[p][k].push_back(off[p][k].back() + (nlines(S, a) + S.fft_lines[p][a] - 1) / S.fft_lines[p][a]);
+            }
+        }
+    }
+    pm.qpoints = qpts;
+    pm.cpoints = cpts;
+    for (int p = 0; p < 2; ++p)
+        for (int k = 0; k < 5; ++k) {
+            pm.fft_off[p][k].upload(off[p][k], s);
+            pm.fft_blocks[p][k] = off[p][k].back();
+        }
     pm.roots_d.upload(roots, s);
     pm.moduli_d.upload(moduli, s);
-    pm.cgrid_d.alloc(2 * GH);
-    pm.phi_d.alloc(G3);
+    pm.cgrid_d.alloc(2 * (size_t)cpts);
+    pm.phi_d.alloc((size_t)qpts);
     if (mixed) {
         std::vector<float> rf(roots.begin(), roots.end()), mf(moduli.begin(), moduli.end());
         pm.roots_f.upload(rf, s);
         pm.moduli_f.upload(mf, s);
-        pm.qgrid_fx.alloc(G3);
-        pm.cgrid_f.alloc(2 * GH);
-        pm.phi_f.alloc(G3);
+        pm.qgrid_fx.alloc((size_t)qpts);
+        fill_async<long long>(pm.qgrid_fx.p, (size_t)qpts, 0ll, s);
+        pm.cgrid_f.alloc(2 * (size_t)cpts);
+        pm.phi_f.alloc((size_t)qpts);
     } else {
-        pm.qgrid_d.alloc(G3);
+        pm.qgrid_d.alloc((size_t)qpts);
+        fill_async<double>(pm.qgrid_d.p, (size_t)qpts, 0.0, s);
     }
+    OMM_HIP(hipStreamSynchronize(s));
 }
 
-void pme_run(PmeDev& pm, int mode, int n, const double* pos, const double* charge, const Box* box_dev,
-             long long* force_fx, double* force_d, long long* energy_fx, double* energy_d, unsigned* status,
-             hipStream_t s, int phases) {
+void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
+             const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
+             unsigned* status, hipStream_t s, int phases) {
     if (mode == MESH_F32)
-        run_t<float, long long, long long>(pm, n, pos, charge, box_dev, pm.qgrid_fx.p, pm.cgrid_f.p, pm.phi_f.p,
+        run_t<float, long long, long long>(pm, nl, pos, charge, sys_dev, pm.qgrid_fx.p, pm.cgrid_f.p, pm.phi_f.p,
                                            pm.roots_f.p, pm.moduli_f.p, force_fx, energy_fx, status, phases, s);
     else if (mode == MESH_F64_FIXED)
-        run_t<double, long long, long long>(pm, n, pos, charge, box_dev, pm.qgrid_fx.p, pm.cgrid_d.p, pm.phi_d.p,
+        run_t<double, long long, long long>(pm, nl, pos, charge, sys_dev, pm.qgrid_fx.p, pm.cgrid_d.p, pm.phi_d.p,
                                             pm.roots_d.p, pm.moduli_d.p, force_fx, energy_fx, status, phases, s);
     else
-        run_t<double, double, double>(pm, n, pos, charge, box_dev, pm.qgrid_d.p, pm.cgrid_d.p, pm.phi_d.p,
+        run_t<double, double, double>(pm, nl, pos, charge, sys_dev, pm.qgrid_d.p, pm.cgrid_d.p, pm.phi_d.p,
                                       pm.roots_d.p, pm.moduli_d.p, force_d, energy_d, status, phases, s);
 }
 

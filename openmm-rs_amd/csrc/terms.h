@@ -1,0 +1,96 @@
+// FP64 per-term device functions shared by the exception kernel and the MD
+// step's fused bonded kernel.  Each returns the term's energy and adds its
+// forces to the accumulator.
+#pragma once
+
+#include "common.h"
+
+namespace ommhip {
+
+// Exception pair (nonbonded.rs:995-1036): LJ and bare Coulomb with the pair's
+// own parameters (lj_coulomb_exception, 1194-1198), minus the reciprocal-space
+// erf correction with the particle charges (reciprocal_pair, 1275-1288) when
+// a reciprocal method is active.
+template <class A>
+__device__ __forceinline__ double exception_term(int i, int j, double qq, double sig, double eps, double qi, double qj,
+                                                 const double* __restrict__ pos, const Box& box, int use_pbc,
+                                                 int has_pme, double alpha, A* force, unsigned* status) {
+    double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+    if (use_pbc) min_image_exact(d, box.v);
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double r = sqrt(r2);
+    double e = 0.0, f = 0.0;
+    if (eps != 0.0) {
+        double sr2 = sig * sig / r2;
+        double sr6 = sr2 * sr2 * sr2;
+        double sr12 = sr6 * sr6;
+        e += 4.0 * eps * (sr12 - sr6);
+        f += 4.0 * eps * (-12.0 * sr12 + 6.0 * sr6) / r2;
+    }
+    if (qq != 0.0) {
+        e += ONE_4PI_EPS0 * qq / r;
+        f -= ONE_4PI_EPS0 * qq / (r2 * r);
+    }
+    if (has_pme) {
+        double c = ONE_4PI_EPS0 * qi * qj;
+        double corr, deriv;
+        if (alpha * r < 1e-6) {
+            corr = c * 2.0 * alpha / sqrt(PI) * (1.0 - alpha * alpha * r2 / 3.0);
+            deriv = -c * 4.0 * alpha * alpha * alpha / (3.0 * sqrt(PI));
+        } else {
+            double erf_v = erf(alpha * r);
+            corr = c * erf_v / r;
+            deriv = c * (2.0 * alpha / sqrt(PI) * exp(-alpha * alpha * r2) - erf_v / r) / r2;
+        }
+        e -= corr;
+        f -= deriv;
+    }
+    for (int a = 0; a < 3; ++a) {
+        acc_add(force + 3 * i + a, f * d[a], FORCE_SCALE, status);
+        acc_add(force + 3 * j + a, -f * d[a], FORCE_SCALE, status);
+    }
+    return e;
+}
+
+// HarmonicBondForce term without PBC (harmonic_bond.rs:113-140).
+template <class A>
+__device__ __forceinline__ double bond_term(int i, int j, double length, double k, const double* __restrict__ pos,
+                                            A* force, unsigned* status) {
+    double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double dr = r - length;
+    double f = k * dr / r;
+    for (int a = 0; a < 3; ++a) {
+        acc_add(force + 3 * i + a, f * d[a], FORCE_SCALE, status);
+        acc_add(force + 3 * j + a, -f * d[a], FORCE_SCALE, status);
+    }
+    return 0.5 * k * dr * dr;
+}
+
+// HarmonicAngleForce term without PBC (harmonic_angle.rs:118-167); j is the vertex.
+template <class A>
+__device__ __forceinline__ double angle_term(int i, int j, int k, double theta0, double ka,
+                                             const double* __restrict__ pos, A* force, unsigned* status) {
+    double rji[3], rjk[3];
+    for (int a = 0; a < 3; ++a) {
+        rji[a] = pos[3 * i + a] - pos[3 * j + a];
+        rjk[a] = pos[3 * k + a] - pos[3 * j + a];
+    }
+    double lji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
+    double ljk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
+    double c = (rji[0] * rjk[0] + rji[1] * rjk[1] + rji[2] * rjk[2]) / (lji * ljk);
+    c = fmin(fmax(c, -1.0), 1.0);
+    double dth = acos(c) - theta0;
+    double s = fmax(sqrt(1.0 - c * c), 1e-8);
+    double de = ka * dth;
+    for (int a = 0; a < 3; ++a) {
+        double fi = de * (c * rji[a] / lji - rjk[a] / ljk) / (lji * s);
+        double fk = de * (c * rjk[a] / ljk - rji[a] / lji) / (ljk * s);
+        acc_add(force + 3 * i + a, -fi, FORCE_SCALE, status);
+        acc_add(force + 3 * k + a, -fk, FORCE_SCALE, status);
+        acc_add(force + 3 * j + a, fi + fk, FORCE_SCALE, status);
+    }
+    return 0.5 * ka * dth * dth;
+}
+
+}  // namespace ommhip

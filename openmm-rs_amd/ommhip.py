@@ -127,6 +127,9 @@ def library():
         lib.omm_nb_desc_defaults.restype = None
         lib.omm_water_box.argtypes = [C.c_int32, C.c_double, C.c_uint64, _dp]
         lib.omm_ctx_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
+        lib.omm_ctx_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        lib.omm_batch_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
+        lib.omm_batch_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         _lib = lib
     return _lib
 
@@ -464,6 +467,38 @@ def water_system(num_molecules: int, box_edge: float, seed: int, cutoff: float =
     return WaterSystem(nb, pos, cubic_box(box_edge), masses, bonds, bond_params, angles, angle_params)
 
 
+def _md_desc(system: WaterSystem, temperature, friction, step_size, seed):
+    """An ``omm_md_desc`` for the system plus the arrays it points into."""
+    s = system
+    nbd = s.nonbonded.desc()
+    keep = dict(
+        nb=nbd, m=_f64(s.masses), b1=_i32(s.bonds[:, 0]), b2=_i32(s.bonds[:, 1]),
+        bl=_f64(s.bond_params[:, 0]), bk=_f64(s.bond_params[:, 1]),
+        a1=_i32(s.angles[:, 0]), a2=_i32(s.angles[:, 1]), a3=_i32(s.angles[:, 2]),
+        at=_f64(s.angle_params[:, 0]), ak=_f64(s.angle_params[:, 1]),
+    )
+    d = MdDesc()
+    d.num_particles = len(s.masses)
+    d.masses = _ptr(keep["m"], C.c_double)
+    d.nonbonded = C.pointer(nbd)
+    d.num_bonds = len(s.bonds)
+    d.bond_particle1, d.bond_particle2 = _ptr(keep["b1"], C.c_int32), _ptr(keep["b2"], C.c_int32)
+    d.bond_length, d.bond_k = _ptr(keep["bl"], C.c_double), _ptr(keep["bk"], C.c_double)
+    d.num_angles = len(s.angles)
+    d.angle_particle1 = _ptr(keep["a1"], C.c_int32)
+    d.angle_particle2 = _ptr(keep["a2"], C.c_int32)
+    d.angle_particle3 = _ptr(keep["a3"], C.c_int32)
+    d.angle_theta0, d.angle_k = _ptr(keep["at"], C.c_double), _ptr(keep["ak"], C.c_double)
+    for i, v in enumerate(_f64(s.box).reshape(9)):
+        d.box[i] = v
+    d.temperature, d.friction, d.step_size, d.seed = temperature, friction, step_size, seed
+    return d, keep
+
+
+PHASES = ("nlist_build", "direct_pairs", "exceptions", "pme_spread", "pme_fft_conv", "pme_gather",
+          "bonded", "integrator")
+
+
 class Context:
     """Device-resident MD context: LangevinMiddle + bonds/angles + NonbondedForce.
 
@@ -473,35 +508,12 @@ class Context:
     def __init__(self, system: WaterSystem, temperature=300.0, friction=1.0, step_size=0.001,
                  seed=1, precision=PRECISION_MIXED):
         lib = library()
-        s = system
-        self._nb = s.nonbonded.desc()
-        self.n = len(s.masses)
-        keep = dict(
-            m=_f64(s.masses), b1=_i32(s.bonds[:, 0]), b2=_i32(s.bonds[:, 1]),
-            bl=_f64(s.bond_params[:, 0]), bk=_f64(s.bond_params[:, 1]),
-            a1=_i32(s.angles[:, 0]), a2=_i32(s.angles[:, 1]), a3=_i32(s.angles[:, 2]),
-            at=_f64(s.angle_params[:, 0]), ak=_f64(s.angle_params[:, 1]),
-        )
-        d = MdDesc()
-        d.num_particles = self.n
-        d.masses = _ptr(keep["m"], C.c_double)
-        d.nonbonded = C.pointer(self._nb)
-        d.num_bonds = len(s.bonds)
-        d.bond_particle1, d.bond_particle2 = _ptr(keep["b1"], C.c_int32), _ptr(keep["b2"], C.c_int32)
-        d.bond_length, d.bond_k = _ptr(keep["bl"], C.c_double), _ptr(keep["bk"], C.c_double)
-        d.num_angles = len(s.angles)
-        d.angle_particle1 = _ptr(keep["a1"], C.c_int32)
-        d.angle_particle2 = _ptr(keep["a2"], C.c_int32)
-        d.angle_particle3 = _ptr(keep["a3"], C.c_int32)
-        d.angle_theta0, d.angle_k = _ptr(keep["at"], C.c_double), _ptr(keep["ak"], C.c_double)
-        for i, v in enumerate(_f64(s.box).reshape(9)):
-            d.box[i] = v
-        d.temperature, d.friction, d.step_size, d.seed = temperature, friction, step_size, seed
-        self._keep, self._d = keep, d
+        self.n = len(system.masses)
+        self._d, self._keep = _md_desc(system, temperature, friction, step_size, seed)
         h = C.c_void_p()
-        _check(lib.omm_ctx_create(C.byref(d), precision, C.byref(h)))
+        _check(lib.omm_ctx_create(C.byref(self._d), precision, C.byref(h)))
         self._h = h
-        self.set_positions(s.positions)
+        self.set_positions(system.positions)
 
     def set_positions(self, pos):
         p = _f64(pos).reshape(-1, 3)
@@ -516,6 +528,16 @@ class Context:
 
     def step(self, steps):
         _check(library().omm_ctx_step(self._h, int(steps)))
+
+    def set_integrator(self, temperature, friction, step_size):
+        """LangevinMiddleIntegrator set_temperature / set_friction / set_step_size."""
+        _check(library().omm_ctx_set_integrator(self._h, temperature, friction, step_size))
+
+    def list_stats(self):
+        """(tiles, j-list entries) of the current neighbour list."""
+        t, e = C.c_int64(), C.c_int64()
+        _check(library().omm_ctx_list_stats(self._h, C.byref(t), C.byref(e)))
+        return t.value, e.value
 
     def get_state(self):
         pos = np.zeros((self.n, 3))
@@ -532,19 +554,113 @@ class Context:
         _check(library().omm_ctx_kinetic_energy(self._h, C.byref(e)))
         return e.value
 
-    PHASES = ("nlist_build", "direct_pairs", "exceptions", "pme_spread", "pme_fft_conv", "pme_gather",
-              "bonded", "integrator")
+    PHASES = PHASES
 
     def profile(self, reps=20):
         """Per-phase average device ms (HIP events on the context stream) and the pair count."""
         ms = np.zeros(8)
         npairs = C.c_int64()
         _check(library().omm_ctx_profile(self._h, int(reps), ms.ctypes.data_as(_dp), C.byref(npairs)))
-        return dict(zip(self.PHASES, ms.tolist())), npairs.value
+        return dict(zip(PHASES, ms.tolist())), npairs.value
 
     def close(self):
         if getattr(self, "_h", None):
             library().omm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent MD systems (replicas) advanced by the same launches.
+
+    The reference would hold one ``Context`` per system; replica r here follows
+    the trajectory of ``Context(systems[r], seed=seeds[r])`` bit for bit.
+    """
+
+    def __init__(self, systems, temperature=300.0, friction=1.0, step_size=0.001, seeds=None,
+                 precision=PRECISION_MIXED):
+        lib = library()
+        systems = list(systems)
+        if seeds is None:
+            seeds = list(range(1, len(systems) + 1))
+        self.sizes = [len(s.masses) for s in systems]
+        descs = (MdDesc * len(systems))()
+        self._keep = []
+        for r, s in enumerate(systems):
+            d, keep = _md_desc(s, temperature, friction, step_size, int(seeds[r]))
+            descs[r] = d
+            self._keep.append(keep)
+        self._descs = descs
+        h = C.c_void_p()
+        _check(lib.omm_batch_create(descs, len(systems), precision, C.byref(h)))
+        self._h = h
+        for r, s in enumerate(systems):
+            self.set_positions(r, s.positions)
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def set_positions(self, replica, pos):
+        p = _f64(pos).reshape(-1, 3)
+        _check(library().omm_batch_set_positions(self._h, int(replica), p.ctypes.data_as(_dp)))
+
+    def set_velocities(self, replica, vel):
+        v = _f64(vel).reshape(-1, 3)
+        _check(library().omm_batch_set_velocities(self._h, int(replica), v.ctypes.data_as(_dp)))
+
+    def set_velocities_to_temperature(self, temperature, seed):
+        _check(library().omm_batch_set_velocities_to_temperature(self._h, temperature, seed))
+
+    def step(self, steps):
+        _check(library().omm_batch_step(self._h, int(steps)))
+
+    def set_integrator(self, temperature, friction, step_size):
+        _check(library().omm_batch_set_integrator(self._h, temperature, friction, step_size))
+
+    def get_state(self, replica):
+        n = self.sizes[replica]
+        pos, vel, frc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        e = C.c_double()
+        k = C.c_int64()
+        _check(library().omm_batch_get_state(self._h, int(replica), pos.ctypes.data_as(_dp),
+                                             vel.ctypes.data_as(_dp), frc.ctypes.data_as(_dp), C.byref(e),
+                                             C.byref(k)))
+        return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value)
+
+    def energies(self):
+        out = np.zeros(len(self.sizes))
+        _check(library().omm_batch_energies(self._h, out.ctypes.data_as(_dp)))
+        return out
+
+    def kinetic_energy(self, replica):
+        e = C.c_double()
+        _check(library().omm_batch_kinetic_energy(self._h, int(replica), C.byref(e)))
+        return e.value
+
+    def box(self, replica):
+        b = np.zeros(9)
+        _check(library().omm_batch_get_box(self._h, int(replica), b.ctypes.data_as(_dp)))
+        return b.reshape(3, 3)
+
+    def list_stats(self):
+        t, e = C.c_int64(), C.c_int64()
+        _check(library().omm_batch_list_stats(self._h, C.byref(t), C.byref(e)))
+        return t.value, e.value
+
+    def profile(self, reps=20):
+        ms = np.zeros(8)
+        npairs = C.c_int64()
+        _check(library().omm_batch_profile(self._h, int(reps), ms.ctypes.data_as(_dp), C.byref(npairs)))
+        return dict(zip(PHASES, ms.tolist())), npairs.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().omm_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -35,6 +35,9 @@ def test_zero_temperature_step_is_leapfrog():
 
 
 def test_graph_steps_match_eager_steps():
+    # Graph replays and eager launches run the same kernels with the same
+    # list-rebuild decisions (the integrator's skin flag), so the trajectories
+    # are identical bit for bit.
     s = water(500, 2.5, seed=23)
     a = ommhip.Context(s, temperature=300.0, friction=1.0, seed=5)
     b = ommhip.Context(s, temperature=300.0, friction=1.0, seed=5)
@@ -46,6 +49,19 @@ def test_graph_steps_match_eager_steps():
     sa, sb = a.get_state(), b.get_state()
     assert np.array_equal(sa["positions"], sb["positions"])
     assert np.array_equal(sa["velocities"], sb["velocities"])
+
+
+def test_graph_trajectory_is_bit_reproducible():
+    s = water(500, 2.5, seed=25)
+    runs = []
+    for _ in range(2):
+        c = ommhip.Context(s, temperature=300.0, friction=1.0, seed=6)
+        c.set_velocities_to_temperature(300.0, 10)
+        c.step(30)
+        runs.append(c.get_state())
+    assert np.array_equal(runs[0]["positions"], runs[1]["positions"])
+    assert np.array_equal(runs[0]["velocities"], runs[1]["velocities"])
+    assert runs[0]["potential_energy"] == runs[1]["potential_energy"]
 
 
 def test_thermostat_holds_temperature():
