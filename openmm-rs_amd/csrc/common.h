@@ -36,6 +36,7 @@ enum : unsigned {
     ST_NONFINITE = 1u,
     ST_OVERFLOW = 2u,
     ST_LIST_OVERFLOW = 4u,
+    ST_BIN_OVERFLOW = 8u,  // a PME spread tile received more atoms than its bin holds
 };
 
 // Energy slots of the device energy accumulator (fixed point).
@@ -72,6 +73,7 @@ constexpr int MAX_STAGES = 12;  // FFT stages per axis (radices, 0-terminated)
 //   cells   [cell_off, cell_off + ncell[0] ncell[1] ncell[2])
 //   chunks  [chunk_off, chunk_off + ceil(32 nblocks / 64))  (PME spread/gather)
 //   grid    qgrid_off (real points), cgrid_off (complex half-spectrum points)
+//   tiles   [tile_off, tile_off + tiles[0] tiles[1] tiles[2])  (PME spread)
 // The box is device state: a barostat rescales it in place.
 struct SysInfo {
     Box box;
@@ -81,6 +83,7 @@ struct SysInfo {
     int chunk_off;
     int grid[3], nzh;
     long long qgrid_off, cgrid_off;
+    int tiles[3], tile_off;  // PME spread tiles per axis, first tile
     int roots_off[3], moduli_off[3];
     int fft_lines[2][3];  // lines per FFT workgroup along each axis, [0] FP32 / [1] FP64 transforms
     int radix[3][MAX_STAGES];

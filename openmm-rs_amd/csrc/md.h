@@ -9,6 +9,17 @@
 
 namespace ommhip {
 
+// Bonded terms of the MD step on the device: exceptions, harmonic bonds and
+// harmonic angles, grouped by small molecule (see MdContext::build_bonded).
+struct BondedDev {
+    int ngroups = 0, ne = 0, nb = 0, na = 0;  // groups; all exceptions / bonds / angles
+    int ge = 0, gb = 0, ga = 0;               // of which grouped (stored first)
+    DevBuf<int> grp_atoms;                    // [ngroups][4], -1 padded
+    DevBuf<int> grp_e, grp_b, grp_a;          // [ngroups + 1] term offsets per group
+    DevBuf<int> e1, e2, b1, b2, a1, a2, a3;
+    DevBuf<double> eqq, esig, eeps, blen, bk, ath, ak;
+};
+
 class MdContext {
 public:
     // Replicas share the nonbonded method/cutoff/tolerance and the integrator
@@ -44,12 +55,19 @@ private:
     void enqueue_forces(hipStream_t s, bool energy, int rebuild);
     void enqueue_step(hipStream_t s, int rebuild);
     void launch_bonded(hipStream_t s);
+    void build_bonded(const std::vector<int>& e1, const std::vector<int>& e2, const std::vector<double>& eqq,
+                      const std::vector<double>& esig, const std::vector<double>& eeps, const std::vector<int>& b1,
+                      const std::vector<int>& b2, const std::vector<double>& blen, const std::vector<double>& bk,
+                      const std::vector<int>& a1, const std::vector<int>& a2, const std::vector<int>& a3,
+                      const std::vector<double>& ath, const std::vector<double>& ak);
     void clear_accumulators(hipStream_t s);
     void capture_graph();
     // Evaluates forces (all atoms, FP64 out, nullable) and the potential
     // energy of every replica (nullable).
     void evaluate(double* forces, double* energy, bool want_energy);
     void check_status();
+    unsigned read_status();
+    void restore_snapshot();
     void drop_graph();
     // Sizes the neighbour list for the current positions once they are all set.
     void prepare();
@@ -58,15 +76,13 @@ private:
     int nsys_ = 1;
     bool mixed_ = true;
     int n_ = 0;  // atoms over all replicas
-    int nbonds_ = 0, nangles_ = 0;
     bool positions_dirty_ = true;
     hipStream_t s_ = nullptr;
     std::vector<double> masses_;
     double dt_ = 0.001, temperature_ = 0.0, friction_ = 0.0;
     int64_t step_count_ = 0;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
-    DevBuf<int> b1_, b2_, a1_, a2_, a3_;
-    DevBuf<double> blen_, bk_, ath_, ak_;
+    BondedDev bonded_;
     DevBuf<long long> force_fx_, energy_fx_;
     DevBuf<double> force_d_, energy_d_;
     DevBuf<unsigned> status_;

@@ -1,8 +1,11 @@
 // Device-resident MD step for the flexible-water workloads: NonbondedForce
 // (PME) + HarmonicBondForce + HarmonicAngleForce + LangevinMiddle, captured
 // into a HIP graph (Context::step, context.rs:795-928).
+#include <climits>
 #include <cmath>
+#include <functional>
 #include <random>
+#include <type_traits>
 
 #include "md.h"
 #include "terms.h"
@@ -15,35 +18,91 @@ namespace {
 const double BOLTZMANN = 1.380649e-23 * 6.02214076e23 / 1000.0;
 constexpr int GRAPH_STEPS = 10;
 
-// Exceptions, harmonic bonds and harmonic angles in one launch: threads
-// [0, ne) take exceptions, [ne, ne+nb) bonds, the rest angles.  Indices are
-// global over the batch; energies go to the term's system.
+// Bonded terms in one launch.  Threads [0, ngroups) take one small molecule
+// each (at most four atoms joined by bonds, angles or exceptions, e.g. a
+// water) and sum its exceptions, bonds and angles in registers before one
+// fixed-point add per atom and component; the remaining threads take the
+// terms outside such molecules one each.  Indices are global over the batch;
+// energies go to the term's system.
+__device__ __forceinline__ int group_slot(const int at[4], int i) {
+    return i == at[0] ? 0 : (i == at[1] ? 1 : (i == at[2] ? 2 : 3));
+}
+
 template <class A>
-__global__ void k_bonded_all(int ne, const int* __restrict__ e1, const int* __restrict__ e2,
-                             const double* __restrict__ eqq, const double* __restrict__ esig,
-                             const double* __restrict__ eeps, const double* __restrict__ charge, int use_pbc,
-                             int has_pme, double alpha, int nb, const int* __restrict__ b1, const int* __restrict__ b2,
-                             const double* __restrict__ blen, const double* __restrict__ bk, int na,
-                             const int* __restrict__ a1, const int* __restrict__ a2, const int* __restrict__ a3,
-                             const double* __restrict__ ath, const double* __restrict__ ak,
-                             const double* __restrict__ pos, const int* __restrict__ atom_sys,
-                             const SysInfo* __restrict__ sys, A* force, A* energy, unsigned* status) {
+__global__ void k_bonded(int ngroups, const int* __restrict__ grp_atoms, const int* __restrict__ grp_e,
+                         const int* __restrict__ grp_b, const int* __restrict__ grp_a, int ne, int ge,
+                         const int* __restrict__ e1, const int* __restrict__ e2, const double* __restrict__ eqq,
+                         const double* __restrict__ esig, const double* __restrict__ eeps, int nb, int gb,
+                         const int* __restrict__ b1, const int* __restrict__ b2, const double* __restrict__ blen,
+                         const double* __restrict__ bk, int na, int ga, const int* __restrict__ a1,
+                         const int* __restrict__ a2, const int* __restrict__ a3, const double* __restrict__ ath,
+                         const double* __restrict__ ak, const double* __restrict__ charge, int use_pbc, int has_pme,
+                         double alpha, const double* __restrict__ pos, const int* __restrict__ atom_sys,
+                         const SysInfo* __restrict__ sys, A* force, A* energy, unsigned* status) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double ee = 0.0, eb = 0.0, ea = 0.0;
     int se = -1, sb = -1, sa = -1;
-    if (t < ne) {
-        int i = e1[t], j = e2[t];
-        se = atom_sys[i];
-        ee = exception_term(i, j, eqq[t], esig[t], eeps[t], charge[i], charge[j], pos, sys[se].box, use_pbc, has_pme,
-                            alpha, force, status);
-    } else if (t < ne + nb) {
-        int b = t - ne;
-        sb = atom_sys[b1[b]];
-        eb = bond_term(b1[b], b2[b], blen[b], bk[b], pos, force, status);
-    } else if (t < ne + nb + na) {
-        int a = t - ne - nb;
-        sa = atom_sys[a1[a]];
-        ea = angle_term(a1[a], a2[a], a3[a], ath[a], ak[a], pos, force, status);
+    if (t < ngroups) {
+        int at[4];
+        for (int k = 0; k < 4; ++k) at[k] = grp_atoms[4 * t + k];
+        const int si = atom_sys[at[0]];
+        const Box& box = sys[si].box;
+        double f[4][3] = {};
+        for (int e = grp_e[t]; e < grp_e[t + 1]; ++e) {
+            const int i = e1[e], j = e2[e];
+            double fi[3];
+            ee += exception_pair(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, box, use_pbc, has_pme,
+                                 alpha, fi);
+            const int si_ = group_slot(at, i), sj = group_slot(at, j);
+            for (int c = 0; c < 3; ++c) {
+                f[si_][c] += fi[c];
+                f[sj][c] -= fi[c];
+            }
+        }
+        for (int b = grp_b[t]; b < grp_b[t + 1]; ++b) {
+            const int i = b1[b], j = b2[b];
+            double fi[3];
+            eb += bond_pair(i, j, blen[b], bk[b], pos, fi);
+            const int si_ = group_slot(at, i), sj = group_slot(at, j);
+            for (int c = 0; c < 3; ++c) {
+                f[si_][c] += fi[c];
+                f[sj][c] -= fi[c];
+            }
+        }
+        for (int a = grp_a[t]; a < grp_a[t + 1]; ++a) {
+            const int i = a1[a], j = a2[a], k = a3[a];
+            double fi[3], fk[3];
+            ea += angle_triple(i, j, k, ath[a], ak[a], pos, fi, fk);
+            const int si_ = group_slot(at, i), sj = group_slot(at, j), sk = group_slot(at, k);
+            for (int c = 0; c < 3; ++c) {
+                f[si_][c] += fi[c];
+                f[sk][c] += fk[c];
+                f[sj][c] -= fi[c] + fk[c];
+            }
+        }
+        for (int k = 0; k < 4; ++k)
+            if (at[k] >= 0)
+                for (int c = 0; c < 3; ++c) acc_add(force + 3 * at[k] + c, f[k][c], FORCE_SCALE, status);
+        se = grp_e[t] < grp_e[t + 1] ? si : -1;
+        sb = grp_b[t] < grp_b[t + 1] ? si : -1;
+        sa = grp_a[t] < grp_a[t + 1] ? si : -1;
+    } else {
+        int u = t - ngroups;
+        const int le = ne - ge, lb = nb - gb, la = na - ga;
+        if (u < le) {
+            const int e = ge + u, i = e1[e], j = e2[e];
+            se = atom_sys[i];
+            ee = exception_term(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, sys[se].box, use_pbc,
+                                has_pme, alpha, force, status);
+        } else if ((u -= le) < lb) {
+            const int b = gb + u;
+            sb = atom_sys[b1[b]];
+            eb = bond_term(b1[b], b2[b], blen[b], bk[b], pos, force, status);
+        } else if ((u -= lb) < la) {
+            const int a = ga + u;
+            sa = atom_sys[a1[a]];
+            ea = angle_term(a1[a], a2[a], a3[a], ath[a], ak[a], pos, force, status);
+        }
     }
     wave_energy_add(energy, se, E_EXCEPT, ee, status);
     wave_energy_add(energy, sb, E_BOND, eb, status);
@@ -208,21 +267,22 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     }
     nb_->upload_systems(s_);
     inv_mass_.upload(inv, s_);
-    nbonds_ = (int)b1.size();
-    nangles_ = (int)a1.size();
-    if (nbonds_ > 0) {
-        b1_.upload(b1, s_);
-        b2_.upload(b2, s_);
-        blen_.upload(blen, s_);
-        bk_.upload(bk, s_);
-    }
-    if (nangles_ > 0) {
-        a1_.upload(a1, s_);
-        a2_.upload(a2, s_);
-        a3_.upload(a3, s_);
-        ath_.upload(ath, s_);
-        ak_.upload(ak, s_);
-    }
+    // Exceptions in global indices (the bonded kernel evaluates them).
+    std::vector<int> e1, e2;
+    std::vector<double> eqq, esig, eeps;
+    if (nb_->settings().include_direct)
+        for (int r = 0; r < nsys; ++r) {
+            const NbSettings& c = nb_->settings(r);
+            const int off = nb_->atom_offset(r);
+            for (size_t k = 0; k < c.exc1.size(); ++k) {
+                e1.push_back(off + c.exc1[k]);
+                e2.push_back(off + c.exc2[k]);
+                eqq.push_back(c.exc_qq[k]);
+                esig.push_back(c.exc_sig[k]);
+                eeps.push_back(c.exc_eps[k]);
+            }
+        }
+    build_bonded(e1, e2, eqq, esig, eeps, b1, b2, blen, bk, a1, a2, a3, ath, ak);
     dt_ = descs[0].step_size;
     temperature_ = descs[0].temperature;
     friction_ = descs[0].friction;
@@ -289,24 +349,148 @@ void MdContext::set_velocities(int r, const double* vel) {
     OMM_HIP(hipStreamSynchronize(s_));
 }
 
+// Groups the bonded terms by small molecule (connected components of at most
+// four atoms over bonds, angles and exceptions); terms of larger components
+// stay single.  Groups are ordered by their lowest atom, terms keep their
+// order within a group, so the layout is deterministic.
+void MdContext::build_bonded(const std::vector<int>& e1, const std::vector<int>& e2, const std::vector<double>& eqq,
+                             const std::vector<double>& esig, const std::vector<double>& eeps,
+                             const std::vector<int>& b1, const std::vector<int>& b2, const std::vector<double>& blen,
+                             const std::vector<double>& bk, const std::vector<int>& a1, const std::vector<int>& a2,
+                             const std::vector<int>& a3, const std::vector<double>& ath,
+                             const std::vector<double>& ak) {
+    std::vector<int> parent(n_), size(n_, 1);
+    for (int i = 0; i < n_; ++i) parent[i] = i;
+    std::function<int(int)> root = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    auto join = [&](int a, int b) {
+        a = root(a);
+        b = root(b);
+        if (a == b) return;
+        if (a > b) std::swap(a, b);
+        parent[b] = a;
+        size[a] += size[b];
+    };
+    for (size_t k = 0; k < e1.size(); ++k) join(e1[k], e2[k]);
+    for (size_t k = 0; k < b1.size(); ++k) join(b1[k], b2[k]);
+    for (size_t k = 0; k < a1.size(); ++k) {
+        join(a1[k], a2[k]);
+        join(a2[k], a3[k]);
+    }
+    // Group index of every small component that carries terms.
+    std::vector<int> group_of(n_, -1), roots;
+    auto small = [&](int a) { return size[root(a)] <= 4; };
+    auto mark = [&](int a) {
+        const int r = root(a);
+        if (group_of[r] < 0) {
+            group_of[r] = 0;
+            roots.push_back(r);
+        }
+    };
+    for (size_t k = 0; k < e1.size(); ++k)
+        if (small(e1[k])) mark(e1[k]);
+    for (size_t k = 0; k < b1.size(); ++k)
+        if (small(b1[k])) mark(b1[k]);
+    for (size_t k = 0; k < a1.size(); ++k)
+        if (small(a1[k])) mark(a1[k]);
+    std::sort(roots.begin(), roots.end());  // roots are the lowest atoms of their components
+    const int ng = (int)roots.size();
+    for (int g = 0; g < ng; ++g) group_of[roots[g]] = g;
+    std::vector<int> atoms(4 * (size_t)ng, -1), fill(ng, 0);
+    for (int i = 0; i < n_; ++i) {
+        const int r = root(i);
+        if (size[r] <= 4 && group_of[r] >= 0) {
+            const int g = group_of[r];
+            atoms[4 * g + fill[g]++] = i;
+        }
+    }
+    auto gid = [&](int a) { return small(a) ? group_of[root(a)] : -1; };
+    // Stable order: grouped terms by group, then the others.
+    auto order = [&](size_t count, auto&& group_fn, std::vector<int>& offsets) {
+        std::vector<int> idx(count);
+        for (size_t k = 0; k < count; ++k) idx[k] = (int)k;
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            int gx = group_fn(x), gy = group_fn(y);
+            gx = gx < 0 ? INT_MAX : gx;
+            gy = gy < 0 ? INT_MAX : gy;
+            return gx < gy;
+        });
+        offsets.assign(ng + 1, 0);
+        int grouped = 0;
+        for (size_t k = 0; k < count; ++k) {
+            const int g = group_fn((int)k);
+            if (g >= 0) {
+                ++offsets[g + 1];
+                ++grouped;
+            }
+        }
+        for (int g = 0; g < ng; ++g) offsets[g + 1] += offsets[g];
+        return std::make_pair(idx, grouped);
+    };
+    std::vector<int> oe, ob, oa;
+    auto [ie, ge] = order(e1.size(), [&](int k) { return gid(e1[k]); }, oe);
+    auto [ib, gb] = order(b1.size(), [&](int k) { return gid(b1[k]); }, ob);
+    auto [ia, ga] = order(a1.size(), [&](int k) { return gid(a1[k]); }, oa);
+    auto permute = [](const auto& v, const std::vector<int>& idx) {
+        std::remove_const_t<std::remove_reference_t<decltype(v)>> out(idx.size());
+        for (size_t k = 0; k < idx.size(); ++k) out[k] = v[idx[k]];
+        return out;
+    };
+    BondedDev& B = bonded_;
+    B.ngroups = ng;
+    B.ne = (int)e1.size();
+    B.nb = (int)b1.size();
+    B.na = (int)a1.size();
+    B.ge = ge;
+    B.gb = gb;
+    B.ga = ga;
+    B.grp_atoms.upload(atoms.empty() ? std::vector<int>{-1, -1, -1, -1} : atoms, s_);
+    B.grp_e.upload(oe, s_);
+    B.grp_b.upload(ob, s_);
+    B.grp_a.upload(oa, s_);
+    auto up_i = [&](DevBuf<int>& d, const std::vector<int>& v, const std::vector<int>& idx) {
+        std::vector<int> p = permute(v, idx);
+        if (p.empty()) p.push_back(0);
+        d.upload(p, s_);
+    };
+    auto up_d = [&](DevBuf<double>& d, const std::vector<double>& v, const std::vector<int>& idx) {
+        std::vector<double> p = permute(v, idx);
+        if (p.empty()) p.push_back(0.0);
+        d.upload(p, s_);
+    };
+    up_i(B.e1, e1, ie);
+    up_i(B.e2, e2, ie);
+    up_d(B.eqq, eqq, ie);
+    up_d(B.esig, esig, ie);
+    up_d(B.eeps, eeps, ie);
+    up_i(B.b1, b1, ib);
+    up_i(B.b2, b2, ib);
+    up_d(B.blen, blen, ib);
+    up_d(B.bk, bk, ib);
+    up_i(B.a1, a1, ia);
+    up_i(B.a2, a2, ia);
+    up_i(B.a3, a3, ia);
+    up_d(B.ath, ath, ia);
+    up_d(B.ak, ak, ia);
+}
+
 void MdContext::launch_bonded(hipStream_t s) {
-    const NbSettings& c = nb_->settings();
-    const ExceptionDev& ex = nb_->exceptions();
-    const int ne = c.include_direct ? ex.count : 0;
-    const int total = ne + nbonds_ + nangles_;
+    const BondedDev& B = bonded_;
+    const int total = B.ngroups + (B.ne - B.ge) + (B.nb - B.gb) + (B.na - B.ga);
     if (total == 0) return;
     const int blocks = (total + 255) / 256;
-    if (mixed_)
-        hipLaunchKernelGGL(k_bonded_all<long long>, dim3(blocks), dim3(256), 0, s, ne, ex.p1.p, ex.p2.p, ex.qq.p,
-                           ex.sig.p, ex.eps.p, nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->has_pme() ? 1 : 0,
-                           nb_->alpha(), nbonds_, b1_.p, b2_.p, blen_.p, bk_.p, nangles_, a1_.p, a2_.p, a3_.p, ath_.p,
-                           ak_.p, pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force_fx_.p, energy_fx_.p,
-                           status_.p);
-    else
-        hipLaunchKernelGGL(k_bonded_all<double>, dim3(blocks), dim3(256), 0, s, ne, ex.p1.p, ex.p2.p, ex.qq.p,
-                           ex.sig.p, ex.eps.p, nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->has_pme() ? 1 : 0,
-                           nb_->alpha(), nbonds_, b1_.p, b2_.p, blen_.p, bk_.p, nangles_, a1_.p, a2_.p, a3_.p, ath_.p,
-                           ak_.p, pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force_d_.p, energy_d_.p, status_.p);
+    auto go = [&](auto* force, auto* energy) {
+        using A = std::remove_pointer_t<decltype(force)>;
+        hipLaunchKernelGGL(k_bonded<A>, dim3(blocks), dim3(256), 0, s, B.ngroups, B.grp_atoms.p, B.grp_e.p,
+                           B.grp_b.p, B.grp_a.p, B.ne, B.ge, B.e1.p, B.e2.p, B.eqq.p, B.esig.p, B.eeps.p, B.nb, B.gb,
+                           B.b1.p, B.b2.p, B.blen.p, B.bk.p, B.na, B.ga, B.a1.p, B.a2.p, B.a3.p, B.ath.p, B.ak.p,
+                           nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->has_pme() ? 1 : 0, nb_->alpha(),
+                           pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force, energy, status_.p);
+    };
+    if (mixed_) go(force_fx_.p, energy_fx_.p);
+    else go(force_d_.p, energy_d_.p);
     OMM_HIP(hipGetLastError());
 }
 
@@ -355,23 +539,39 @@ void MdContext::enqueue_step(hipStream_t s, int rebuild) {
     OMM_HIP(hipGetLastError());
 }
 
-void MdContext::check_status() {
+unsigned MdContext::read_status() {
     unsigned st = 0;
     status_.download(&st, 1, s_);
     OMM_HIP(hipStreamSynchronize(s_));
-    if (st & ST_LIST_OVERFLOW) {
+    return st;
+}
+
+void MdContext::check_status() {
+    const unsigned st = read_status();
+    if (st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW)) {
         fill_async<unsigned>(status_.p, 1, 0u, s_);
-        throw Error(OMM_ERR_OVERFLOW, "neighbour list capacity exceeded during the step");
+        throw Error(OMM_ERR_OVERFLOW, "neighbour list or PME bin capacity exceeded during the step");
     }
     if (st & ST_NONFINITE) throw Error(OMM_ERR_NONFINITE, "non-finite force or energy on the device");
     if (st & ST_OVERFLOW) throw Error(OMM_ERR_OVERFLOW, "fixed-point force/energy accumulator overflow");
 }
 
+void MdContext::restore_snapshot() {
+    const size_t bytes = 3 * (size_t)n_ * sizeof(double);
+    OMM_HIP(hipMemcpyAsync(pos_.p, snap_pos_.p, bytes, hipMemcpyDeviceToDevice, s_));
+    OMM_HIP(hipMemcpyAsync(vel_.p, snap_vel_.p, bytes, hipMemcpyDeviceToDevice, s_));
+    OMM_HIP(hipMemcpyAsync(step_dev_.p, snap_step_.p, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s_));
+    clear_accumulators(s_);
+    fill_async<unsigned>(status_.p, 1, 0u, s_);
+    nb_->force_rebuild(s_);
+}
+
 // The captured graph holds GRAPH_STEPS steps and a snapshot of the state
 // taken before them.  Each step rebuilds the list only when the previous
 // integrator flagged it (the build kernels exit at once otherwise).  A list
-// overflow inside the graph restores the snapshot, grows the list and replays
-// the steps eagerly (identical random numbers: the RNG is counter-based).
+// or PME-bin overflow inside the graph restores the snapshot, grows the
+// storage and replays the steps eagerly (identical random numbers: the RNG is
+// counter-based).
 void MdContext::capture_graph() {
     OMM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     try {
@@ -411,24 +611,16 @@ void MdContext::step(int nsteps) {
         if (!graph_exec_) capture_graph();
         while (nsteps - done >= GRAPH_STEPS) {
             OMM_HIP(hipGraphLaunch(graph_exec_, s_));
-            unsigned st = 0;
-            OMM_HIP(hipMemcpyAsync(&st, status_.p, sizeof(unsigned), hipMemcpyDeviceToHost, s_));
-            OMM_HIP(hipStreamSynchronize(s_));
-            if (st & ST_LIST_OVERFLOW) {
-                // Restore the snapshot, grow the list and replay eagerly.
-                const size_t bytes = 3 * (size_t)n_ * sizeof(double);
-                OMM_HIP(hipMemcpyAsync(pos_.p, snap_pos_.p, bytes, hipMemcpyDeviceToDevice, s_));
-                OMM_HIP(hipMemcpyAsync(vel_.p, snap_vel_.p, bytes, hipMemcpyDeviceToDevice, s_));
-                OMM_HIP(hipMemcpyAsync(step_dev_.p, snap_step_.p, sizeof(unsigned long long),
-                                       hipMemcpyDeviceToDevice, s_));
-                clear_accumulators(s_);
-                fill_async<unsigned>(status_.p, 1, 0u, s_);
+            unsigned st = read_status();
+            for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
+                if (attempt == 4) check_status();  // throws
+                // Restore the snapshot, grow the storage and replay eagerly.
                 drop_graph();
-                nb_->grow_list();
-                nb_->force_rebuild(s_);
+                restore_snapshot();
+                nb_->grow(st);
                 for (int k = 0; k < GRAPH_STEPS; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
-                check_status();
-                capture_graph();
+                st = read_status();
+                if (!(st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW))) capture_graph();
             }
             done += GRAPH_STEPS;
         }
@@ -441,8 +633,18 @@ void MdContext::step(int nsteps) {
 
 void MdContext::evaluate(double* forces, double* energy, bool want_energy) {
     prepare();
-    clear_accumulators(s_);
-    enqueue_forces(s_, want_energy, NbOperator::REBUILD_FLAG);
+    for (int attempt = 0;; ++attempt) {
+        clear_accumulators(s_);
+        enqueue_forces(s_, want_energy, NbOperator::REBUILD_FLAG);
+        const unsigned st = read_status();
+        if (!(st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW)) || attempt == 4) break;
+        if (attempt == 3)
+            throw Error(OMM_ERR_OVERFLOW, "evaluation kept overflowing (status " + std::to_string(st) + ")");
+        fill_async<unsigned>(status_.p, 1, 0u, s_);
+        nb_->grow(st);
+        nb_->force_rebuild(s_);
+        drop_graph();
+    }
     const size_t eslots = (size_t)E_SLOTS * nsys_;
     std::vector<double> e(eslots);
     if (mixed_) {

@@ -107,8 +107,9 @@ public:
 
     HostTerms host_terms(int s, double volume, bool direct, bool recip) const;
     hipStream_t stream() const { return stream_; }
-    // Rebuild capacity after a list overflow.
-    void grow_list();
+    // Grows the neighbour-list capacity (ST_LIST_OVERFLOW) and/or the PME
+    // spread bins (ST_BIN_OVERFLOW) named by the status bits.
+    void grow(unsigned status);
     // Builds the list for these positions and sizes its capacity to twice the
     // longest j-list (host-side, outside graph capture).  Returns true when the
     // list storage was reallocated (captured graphs must then be rebuilt).

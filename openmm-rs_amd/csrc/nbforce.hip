@@ -303,7 +303,7 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
 
     const NbSettings& c0 = cfg_[0];
     nlist_setup(nl_, nls, sys_h_, c0.cutoff, 0.1, s);
-    if (c0.method == OMM_NB_PME) pme_setup(pme_, sys_h_, mixed_, s);
+    if (c0.method == OMM_NB_PME) pme_setup(pme_, sys_h_, N, part_.atom_sys.p, mixed_, s);
     sys_dev_.upload(sys_h_, s);
 
     pp_ = PairParams{};
@@ -398,10 +398,14 @@ HostTerms NbOperator::host_terms(int si, double volume, bool direct, bool recip)
     return t;
 }
 
-void NbOperator::grow_list() {
-    int cap = std::min(nl_.nslots + JTILE, nl_.cap * 2);
-    nlist_resize(nl_, (cap + JTILE - 1) / JTILE * JTILE);
-    pp_.cap = nl_.cap;
+void NbOperator::grow(unsigned status) {
+    if (status & ST_LIST_OVERFLOW) {
+        const int limit = std::min(255 * JTILE, (nl_.nslots + 2 * JTILE) / JTILE * JTILE);
+        if (nl_.cap >= limit) throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per block");
+        nlist_resize(nl_, std::min(limit, 2 * nl_.cap));
+        pp_.cap = nl_.cap;
+    }
+    if ((status & ST_BIN_OVERFLOW) && cfg_[0].method == OMM_NB_PME) pme_grow_bins(pme_);
 }
 
 bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t s) {
@@ -409,6 +413,7 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
     int max_npad = 0;
     for (const SysInfo& S : sys_h_) max_npad = std::max(max_npad, S.nblocks * IBLOCK);
     bool changed = false;
+    std::string history;
     for (int attempt = 0; attempt < 8; ++attempt) {
         fill_async<unsigned>(status, 1, 0u, s);
         nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
@@ -422,8 +427,12 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
         long long want = std::min<long long>((long long)max_npad + JTILE, 2LL * mx + 2 * JTILE);
         want = (want + JTILE - 1) / JTILE * JTILE;
         if (!(st & ST_LIST_OVERFLOW) && want <= nl_.cap) break;
-        int cap = (int)std::max<long long>(want, 2LL * nl_.cap);
-        nlist_resize(nl_, std::min(cap, (max_npad + 2 * JTILE) / JTILE * JTILE));
+        history += " [longest list " + std::to_string(mx) + ", capacity " + std::to_string(nl_.cap) + ", status " +
+                   std::to_string(st) + "]";
+        const long long limit = std::min<long long>(255 * JTILE, (max_npad + 2 * JTILE) / JTILE * JTILE);
+        if (want > limit)
+            throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per block:" + history);
+        nlist_resize(nl_, (int)std::min<long long>(limit, std::max<long long>(want, 2LL * nl_.cap)));
         pp_.cap = nl_.cap;
         changed = true;
     }
@@ -439,12 +448,13 @@ void NbOperator::list_stats(int64_t* tiles, int64_t* entries) {
         return;
     }
     std::vector<int> counts(nl_.nblocks);
-    int total = 0;
     nl_.jcount.download(counts.data(), counts.size(), stream_);
-    OMM_HIP(hipMemcpyAsync(&total, nl_.task_start.p + nl_.nblocks, sizeof(int), hipMemcpyDeviceToHost, stream_));
     OMM_HIP(hipStreamSynchronize(stream_));
-    int64_t e = 0;
-    for (int c : counts) e += c;
+    int64_t e = 0, total = 0;
+    for (int c : counts) {
+        e += c;
+        total += (c + JTILE - 1) / JTILE;
+    }
     if (tiles) *tiles = total;
     if (entries) *entries = e;
 }
@@ -588,8 +598,8 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
         unsigned st = 0;
         status_.download(&st, 1, s);
         OMM_HIP(hipStreamSynchronize(s));
-        if (st & ST_LIST_OVERFLOW) {
-            grow_list();
+        if (st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW)) {
+            grow(st);
             continue;
         }
         if (st & ST_NONFINITE) throw Error(OMM_ERR_NONFINITE, "non-finite force or energy on the device");

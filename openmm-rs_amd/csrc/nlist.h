@@ -3,10 +3,11 @@
 //
 // Atoms are grouped into sort units (a small exception-connected molecule, or
 // a single atom), binned by the fractional coordinates of the unit's first
-// atom on a per-system cell grid of ~0.35 nm, and counting-sorted by
-// (system, Morton rank of the cell); ties inside a cell are broken by unit
-// index so the order (and hence every FP32 partial sum downstream) is
-// deterministic and the same as for the system built alone.  Consecutive runs
+// atom on a per-system cell grid (power-of-two cells per axis, 0.3-0.6 nm),
+// and counting-sorted by (system, Hilbert rank of the cell); ties inside a
+// cell are broken by unit index so the order (and hence every FP32 partial
+// sum downstream) is deterministic and the same as for the system built
+// alone.  Consecutive runs
 // of IBLOCK sorted atoms of one system form blocks with an FP64 centre and
 // FP32 half-extents.  For every block I the list holds, in increasing order,
 // the sorted slots of the atoms of blocks J >= I of the same system within
@@ -46,7 +47,9 @@ struct NlistDev {
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
-    DevBuf<int> jlist, jcount, task_start, task_block;
+    DevBuf<int> jlist, jcount;
+    DevBuf<int> tasks, ntasks;  // pair-kernel tasks (block << 8 | tile) and their count
+    DevBuf<int> scan_part;  // chunk sums of the multi-workgroup scans
     DevBuf<int> atom_sys;
     // Rebuild requests: [0] some system, [1 + s] system s.  Systems that are
     // not flagged keep their order, bounds and lists through a build, so a
@@ -69,7 +72,8 @@ void nlist_resize(NlistDev& nl, int cap);
 void nlist_check(NlistDev& nl, const double* pos, int force, hipStream_t s);
 
 // Rebuilds the list if requested (every kernel exits early otherwise), or
-// unconditionally when forced.  Clears the request.
+// unconditionally when forced.  The pair kernel, the next kernel to run on
+// the list, clears the requests.
 void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsigned* status, hipStream_t s,
                  bool forced = false);
 

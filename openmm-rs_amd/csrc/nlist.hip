@@ -11,7 +11,6 @@ namespace ommhip {
 
 namespace {
 
-constexpr int SCAN_THREADS = 1024;
 
 __device__ __forceinline__ void wrap_fast(double d[3], const Box& b) { wrap_nearest(d, b); }
 
@@ -96,54 +95,93 @@ __global__ void k_unit_keys(const double* __restrict__ pos, const int* __restric
     atomicAdd(&cell_count[key], 1);
 }
 
-enum : int { SCAN_ZERO_INPUT = 1, SCAN_TILES = 2 };
+// Exclusive scan of in[0..n) into out[0..n], out[n] = total, in three
+// launches: per-chunk sums, a one-workgroup scan of the chunk sums, and the
+// per-chunk scans with their offsets.  copy (optional) receives the same
+// values; zero_input clears the input after reading it (the cell counts start
+// the next build at zero without a fill kernel).
+constexpr int SCAN_WG = 256;
+constexpr int SCAN_PER = 8;
+constexpr int SCAN_CHUNK = SCAN_WG * SCAN_PER;
 
-// Exclusive scan of in[0..n) into out[0..n], out[n] = total (one workgroup).
-//   copy (optional) receives the same values;
-//   SCAN_ZERO_INPUT clears the input after reading it (the cell counts start
-//   the next build at zero without a fill kernel);
-//   SCAN_TILES scans ceil(in/JTILE) and writes task_block[t] = I for every
-//   task t of block I (the pair kernel's task table).
-// clear (optional, nclear entries) is reset to 0 at the end: the rebuild
-// requests are served.
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(int* in, int* out, int* copy, int n, int mode,
-                                                       int* task_block, int* clear, int nclear, const int* flag) {
+// Inclusive scan of v over the workgroup (SCAN_WG threads); returns it and
+// leaves the workgroup total in *total.
+__device__ __forceinline__ int block_scan_inclusive(int v, int* s_wave, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) s_wave[wave] = v;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < SCAN_WG / 64; ++w) {
+        const int x = s_wave[w];
+        if (w < wave) before += x;
+        all += x;
+    }
+    *total = all;
+    return v + before;
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_reduce(const int* __restrict__ in, int n, int* partial,
+                                                         const int* flag) {
     if (!flag[0]) return;
-    __shared__ int s_part[SCAN_THREADS];
-    int t = threadIdx.x;
-    int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-    int lo = min(n, t * per), hi = min(n, lo + per);
-    auto value = [&](int i) {
-        int v = in[i];
-        return (mode & SCAN_TILES) ? (v + JTILE - 1) / JTILE : v;
-    };
+    __shared__ int s_wave[SCAN_WG / 64];
+    const int base = blockIdx.x * SCAN_CHUNK;
     int sum = 0;
-    for (int i = lo; i < hi; ++i) sum += value(i);
-    s_part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        int v = t >= off ? s_part[t - off] : 0;
+    for (int k = threadIdx.x; k < SCAN_CHUNK; k += SCAN_WG) {
+        const int i = base + k;
+        if (i < n) sum += in[i];
+    }
+    int total;
+    block_scan_inclusive(sum, s_wave, &total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive scan of the chunk sums in place, total to *out_total.
+__global__ __launch_bounds__(SCAN_WG) void k_scan_top(int* partial, int nparts, int* out_total, int* copy_total,
+                                                      const int* flag) {
+    if (!flag[0]) return;
+    __shared__ int s_wave[SCAN_WG / 64];
+    int carry = 0;
+    for (int base = 0; base < nparts; base += SCAN_WG) {
+        const int i = base + threadIdx.x;
+        const int v = i < nparts ? partial[i] : 0;
+        int total;
+        const int inc = block_scan_inclusive(v, s_wave, &total);
+        if (i < nparts) partial[i] = carry + inc - v;
+        carry += total;
         __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
     }
-    int run = s_part[t] - sum;
-    for (int i = lo; i < hi; ++i) {
-        int v = value(i);
-        out[i] = run;
-        if (copy) copy[i] = run;
-        if (mode & SCAN_TILES)
-            for (int k = 0; k < v; ++k) task_block[run + k] = i;
-        if (mode & SCAN_ZERO_INPUT) in[i] = 0;
-        run += v;
+    if (threadIdx.x == 0) {
+        *out_total = carry;
+        if (copy_total) *copy_total = carry;
     }
-    if (t == SCAN_THREADS - 1) {
-        out[n] = s_part[t];
-        if (copy) copy[n] = s_part[t];
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_down(int* in, int* out, int* copy, int n, int zero_input,
+                                                       const int* __restrict__ partial, const int* flag) {
+    if (!flag[0]) return;
+    __shared__ int s_wave[SCAN_WG / 64];
+    const int lo = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER;
+    int v[SCAN_PER];
+    int sum = 0;
+    for (int k = 0; k < SCAN_PER; ++k) {
+        v[k] = lo + k < n ? in[lo + k] : 0;
+        sum += v[k];
     }
-    __syncthreads();
-    if (clear)
-        for (int i = t; i < nclear; i += SCAN_THREADS) clear[i] = 0;
+    int total;
+    int run = partial[blockIdx.x] + block_scan_inclusive(sum, s_wave, &total) - sum;
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int i = lo + k;
+        if (i < n) {
+            out[i] = run;
+            if (copy) copy[i] = run;
+            if (zero_input) in[i] = 0;
+        }
+        run += v[k];
+    }
 }
 
 __global__ void k_scatter_units(const int* __restrict__ unit_key, int nunits, int* cursor, int* sorted_units,
@@ -245,6 +283,18 @@ __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __
     for (int k = 0; k < MAXEX; ++k) excl_sorted[(size_t)s * MAXEX + k] = v[k];
 }
 
+// Appends the pair kernel's tasks of block I: (I, tile) for every JTILE-slot
+// tile of its list, packed as I * 256 + tile.  The order of the tasks in the
+// table is whatever the atomics give; forces are fixed-point sums, so the
+// result does not depend on it.  Called by the 64 lanes of one wave.
+__device__ __forceinline__ void append_tasks(int I, int count, int* tasks, int* ntasks, int lane) {
+    const int tiles = (count + JTILE - 1) / JTILE;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(ntasks, tiles);
+    base = __shfl(base, 0, 64);
+    for (int k = lane; k < tiles; k += 64) tasks[base + k] = (I << 8) | k;
+}
+
 // Two blocks per wave64: lanes 0-31 block 2*blockIdx.x, lanes 32-63 the next.
 // The first slot of a block always holds an atom (padding only ends a
 // system's last block).
@@ -253,8 +303,9 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
                                                      const int* __restrict__ block_sys,
                                                      const SysInfo* __restrict__ sys, double reach,
                                                      double* center, float* half, int* minimg,
-                                                     double* pos_build, const int* flag) {
+                                                     double* pos_build, int* ntasks, const int* flag) {
     if (!flag[0]) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ntasks[0] = 0;  // k_jlist appends the tasks anew
     int lane = threadIdx.x;
     int blk = blockIdx.x * 2 + (lane >> 5);
     const int si = blk < nblocks ? block_sys[blk] : 0;
@@ -294,16 +345,20 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
 __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, const int* __restrict__ sorted_atom,
                                               const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
                                               const double* __restrict__ center, const float* __restrict__ half,
-                                              double reach, int cap, int* jlist, int* jcount, unsigned* status,
-                                              const int* flag) {
+                                              double reach, int cap, int* jlist, int* jcount, int* tasks,
+                                              int* ntasks, unsigned* status, const int* flag) {
     if (!flag[0]) return;
     const int I = blockIdx.x;
     const int si = block_sys[I];
-    if (!flag[1 + si]) return;  // unflagged systems keep their lists
+    const int lane = threadIdx.x;
+    if (!flag[1 + si]) {
+        // Unflagged systems keep their lists; only their tasks are appended.
+        append_tasks(I, jcount[I], tasks, ntasks, lane);
+        return;
+    }
     const SysInfo& S = sys[si];
     const Box& b = S.box;
     const int jend = S.block_off + S.nblocks;
-    const int lane = threadIdx.x;
     const double reach2 = reach * reach;
     double cI[3] = {center[3 * I], center[3 * I + 1], center[3 * I + 2]};
     double hI[3] = {half[3 * I], half[3 * I + 1], half[3 * I + 2]};
@@ -348,25 +403,54 @@ __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, co
             count += __popcll(mm);
         }
     }
-    if (lane == 0) {
-        if (count > cap) {
-            atomicOr(status, ST_LIST_OVERFLOW);
-            count = cap;
-        }
-        jcount[I] = count;
+    if (count > cap) {
+        if (lane == 0) atomicOr(status, ST_LIST_OVERFLOW);
+        count = cap;
     }
+    if (lane == 0) jcount[I] = count;
+    append_tasks(I, count, tasks, ntasks, lane);
 }
 
-uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-    auto spread = [](uint32_t v) {
-        uint64_t r = 0;
-        for (int i = 0; i < 10; ++i) r |= (uint64_t)((v >> i) & 1u) << (3 * i);
-        return (uint32_t)r;
-    };
-    return (spread(x) << 2) | (spread(y) << 1) | spread(z);
+// Position of cell (x, y, z) along a 3-D Hilbert curve over a 2^bits cube
+// (J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004:
+// axes-to-transpose, then the transposed bits interleaved).  Consecutive
+// cells on the curve share a face, so runs of sorted atoms stay compact.
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+    uint32_t X[3] = {x, y, z};
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint64_t h = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+    return h;
 }
 
 inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t); }
+
+void scan_async(NlistDev& nl, int* in, int* out, int* copy, int n, int zero_input, const int* flag, hipStream_t s) {
+    const int parts = blocks_for(n, SCAN_CHUNK);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(parts), dim3(SCAN_WG), 0, s, in, n, nl.scan_part.p, flag);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_WG), 0, s, nl.scan_part.p, parts, out + n,
+                       copy ? copy + n : (int*)nullptr, flag);
+    hipLaunchKernelGGL(k_scan_down, dim3(parts), dim3(SCAN_WG), 0, s, in, out, copy, n, zero_input, nl.scan_part.p,
+                       flag);
+}
 
 }  // namespace
 
@@ -437,21 +521,28 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
         }
         for (size_t e = 0; e < exc1.size(); ++e)
             if (unit_of[exc1[e]] != unit_of[exc2[e]]) nl.closed = 0;
-        // Cell grid of ~0.35 nm along each reciprocal direction, Morton ranks.
+        // Cell grid along each reciprocal direction, Hilbert ranks.
         SysInfo& S = sys[si];
         const Box& box = sy.box;
         for (int a = 0; a < 3; ++a) {
             double norm = std::sqrt(box.recip[a * 3] * box.recip[a * 3] + box.recip[a * 3 + 1] * box.recip[a * 3 + 1] +
                                     box.recip[a * 3 + 2] * box.recip[a * 3 + 2]);
-            S.ncell[a] = std::max(1, std::min(1023, (int)std::floor(1.0 / norm / 0.35)));
+            // A power of two per axis (cell edge 0.3-0.6 nm), where the
+            // Hilbert curve never jumps; the cells only order the sort.
+            const int want = std::max(1, std::min(512, (int)std::floor(1.0 / norm / 0.3)));
+            int c = 1;
+            while (2 * c <= want) c *= 2;
+            S.ncell[a] = c;
         }
         const int nc = S.ncell[0] * S.ncell[1] * S.ncell[2];
-        std::vector<std::pair<uint32_t, int>> codes(nc);
+        int bits = 1;
+        while ((1 << bits) < std::max(S.ncell[0], std::max(S.ncell[1], S.ncell[2]))) ++bits;
+        std::vector<std::pair<uint64_t, int>> codes(nc);
         for (int x = 0; x < S.ncell[0]; ++x)
             for (int y = 0; y < S.ncell[1]; ++y)
                 for (int z = 0; z < S.ncell[2]; ++z) {
                     int lin = (x * S.ncell[1] + y) * S.ncell[2] + z;
-                    codes[lin] = {morton3(x, y, z), lin};
+                    codes[lin] = {hilbert3(x, y, z, bits), lin};
                 }
         std::sort(codes.begin(), codes.end());
         std::vector<int> rank(nc);
@@ -525,10 +616,12 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.block_half.alloc(3 * (size_t)nblocks);
     nl.block_minimg.alloc(nblocks);
     nl.pos_build.alloc(3 * (size_t)natoms);
+    nl.cap = std::min(nl.cap, 255 * JTILE);
     nl.jlist.alloc((size_t)nblocks * nl.cap);
+    nl.tasks.alloc((size_t)nblocks * (nl.cap / JTILE));
+    nl.ntasks.alloc(1);
     nl.jcount.alloc(nblocks);
-    nl.task_start.alloc(nblocks + 1);
-    nl.task_block.alloc((size_t)nblocks * (nl.cap / JTILE));
+    nl.scan_part.alloc((size_t)std::max(1, blocks_for(std::max(ncells, nl.nunits), SCAN_CHUNK)));
     std::vector<int> one(nsys + 1, 1);
     nl.flag.upload(one, s);
     nl.one.upload(one, s);
@@ -536,9 +629,10 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
 }
 
 void nlist_resize(NlistDev& nl, int cap) {
+    if (cap / JTILE > 255) throw Error(OMM_ERR_INVALID, "neighbour list needs more than 255 tiles per block");
     nl.cap = cap;
     nl.jlist.alloc((size_t)nl.nblocks * nl.cap);
-    nl.task_block.alloc((size_t)nl.nblocks * (nl.cap / JTILE));
+    nl.tasks.alloc((size_t)nl.nblocks * (nl.cap / JTILE));
 }
 
 void nlist_check(NlistDev& nl, const double* pos, int force, hipStream_t s) {
@@ -557,15 +651,13 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     hipLaunchKernelGGL(k_unit_keys, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, pos, nl.unit_ptr.p,
                        nl.unit_atoms.p, nl.unit_sys.p, nl.nunits, sys_dev, nl.cell_rank.p, nl.unit_key.p,
                        nl.cell_count.p, flag);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.cell_count.p, nl.cell_start.p, nl.cell_cursor.p,
-                       nl.ncells, (int)SCAN_ZERO_INPUT, (int*)nullptr, (int*)nullptr, 0, flag);
+    scan_async(nl, nl.cell_count.p, nl.cell_start.p, nl.cell_cursor.p, nl.ncells, 1, flag, s);
     hipLaunchKernelGGL(k_scatter_units, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.unit_key.p, nl.nunits,
                        nl.cell_cursor.p, nl.sorted_units.p, flag);
     hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p, nl.ncells,
                        nl.sorted_units.p, nl.unit_ptr.p, nl.uniform > 0 ? (int*)nullptr : nl.unit_size_sorted.p, flag);
     if (nl.uniform == 0)
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.unit_size_sorted.p, nl.unit_offset.p,
-                           (int*)nullptr, nl.nunits, 0, (int*)nullptr, (int*)nullptr, 0, flag);
+        scan_async(nl, nl.unit_size_sorted.p, nl.unit_offset.p, nullptr, nl.nunits, 0, flag, s);
     hipLaunchKernelGGL(k_place_atoms, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.sorted_units.p,
                        nl.unit_offset.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p, nl.unit_sys.p,
                        sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.nunits, nl.sorted_atom.p, nl.atom_slot.p,
@@ -577,11 +669,10 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     double reach = nl.cutoff + nl.skin;
     hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p,
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
-                       nl.block_minimg.p, nl.pos_build.p, flag);
+                       nl.block_minimg.p, nl.pos_build.p, nl.ntasks.p, flag);
     hipLaunchKernelGGL(k_jlist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.block_sys.p, sys_dev,
-                       nl.block_center.p, nl.block_half.p, reach, nl.cap, nl.jlist.p, nl.jcount.p, status, flag);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, s, nl.jcount.p, nl.task_start.p, (int*)nullptr,
-                       nl.nblocks, (int)SCAN_TILES, nl.task_block.p, nl.flag.p, nl.nsys + 1, flag);
+                       nl.block_center.p, nl.block_half.p, reach, nl.cap, nl.jlist.p, nl.jcount.p, nl.tasks.p,
+                       nl.ntasks.p, status, flag);
     OMM_HIP(hipGetLastError());
 }
 

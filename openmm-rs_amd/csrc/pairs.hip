@@ -127,11 +127,11 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                                               const double* __restrict__ e2_d, const int* __restrict__ sorted_atom,
                                               const int* __restrict__ excl_sorted, const double* __restrict__ center,
                                               const int* __restrict__ minimg, const int* __restrict__ jlist,
-                                              const int* __restrict__ jcount, const int* __restrict__ task_start,
-                                              const int* __restrict__ task_block,
-                                              const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
-                                              PairParams P, A* force, A* energy, unsigned* status,
-                                              unsigned long long* stats) {
+                                              const int* __restrict__ jcount, const int* __restrict__ tasks,
+                                              const int* __restrict__ ntasks, const int* __restrict__ block_sys,
+                                              const SysInfo* __restrict__ sys, PairParams P, A* force, A* energy,
+                                              unsigned* status, unsigned long long* stats, int* list_flag,
+                                              int nflags) {
     using M = Math<R>;
     constexpr bool E64 = ENERGY && sizeof(R) == 4;  // separate FP64 energy path
     __shared__ Q4<R> s_pq[JTILE];
@@ -152,14 +152,19 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
     const R alpha = (R)P.alpha;
     const R two_a_sqrtpi = (R)(2.0 * P.alpha / sqrt(PI));
     const R krf = (R)P.krf, crf = (R)P.crf;
-    const int total = task_start[P.nblocks];
+    // This kernel runs after every list build and before the next integrator,
+    // so it serves the rebuild requests.
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < nflags; i += blockDim.x) list_flag[i] = 0;
 
+    const int total = ntasks[0];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int I = task_block[t];
+        const int task = tasks[t];
+        const int I = task >> 8;
+        const int tile = task & 255;
+        const int cnt = min(JTILE, jcount[I] - tile * JTILE);
         const int sidx = block_sys[I];
         const Box& bx = sys[sidx].box;
-        const int tile = t - task_start[I];
-        const int cnt = min(JTILE, jcount[I] - tile * JTILE);
         const double c0 = center[3 * I], c1 = center[3 * I + 1], c2 = center[3 * I + 2];
 
         // Stage the j-tile in LDS.
@@ -399,8 +404,8 @@ void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* s
                 unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
                        part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.excl_sorted.p, nl.block_center.p,
-                       nl.block_minimg.p, nl.jlist.p, nl.jcount.p, nl.task_start.p, nl.task_block.p, nl.block_sys.p,
-                       sys_dev, p, force, energy, status, stats);
+                       nl.block_minimg.p, nl.jlist.p, nl.jcount.p, nl.tasks.p, nl.ntasks.p, nl.block_sys.p, sys_dev, p,
+                       force, energy, status, stats, nl.flag.p, nl.nsys + 1);
 }
 
 template <class R, class A, int ELEC>

@@ -26,12 +26,23 @@ struct PmeDev {
     DevBuf<int> fft_off[2][5];
     int fft_blocks[2][5] = {};
     size_t fft_shmem[2][3] = {};  // dynamic LDS per axis
+    // Spread: per-atom stencils (base point, weights) and per-tile atom bins.
+    int natoms = 0, ntiles = 0, bin_cap = 0;
+    const int* atom_sys = nullptr;  // system of each atom (owned by the caller)
+    DevBuf<int> tile_off;           // nsys + 1 prefix of tiles per system
+    DevBuf<int> bin_count, bins;    // [ntiles], [ntiles][bin_cap]
+    DevBuf<int> st_idx;             // [natoms][4] stencil base point
+    DevBuf<float> st_wf;            // [natoms][16] weights, FP32 mesh
+    DevBuf<double> st_wd;           // [natoms][16] weights, FP64 mesh
 };
 
 // Grid dimensions must each be >= 5 (pme.rs:222-225) and at most 1024.
 // Reads sys[s].grid and sys[s].alpha; writes the grid offsets, radices,
 // root/moduli offsets and FFT line counts into sys[s].
-void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, bool mixed, hipStream_t s);
+void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* atom_sys, bool mixed, hipStream_t s);
+
+// Doubles the per-tile bin capacity after ST_BIN_OVERFLOW.
+void pme_grow_bins(PmeDev& pm);
 
 // Spread, FFT, convolution (+energy into slot E_RECIP of each system), inverse
 // FFT, gather (forces added).  Atoms with zero charge are skipped as in the

@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <type_traits>
 
 #include "pme.h"
 
@@ -64,143 +65,153 @@ __device__ __forceinline__ void stencil(const double* p, const Box& b, const int
     }
 }
 
-// Spread with an LDS-privatised brick.  A workgroup takes SPREAD_ATOMS
-// consecutive atoms of the spatially sorted order (any permutation is correct;
-// the sort only gives locality).  Their stencils are unwrapped relative to the
-// first atom's base point; if the covering brick fits in LDS, the 125
-// contributions per atom go to int64 LDS atomics and the brick is flushed with
-// one global atomic per non-zero point (about a quarter of the per-atom
-// atomics for water at PME spacings); otherwise the workgroup falls back to
-// global atomics.  All sums are fixed point, so the result is identical.
-constexpr int SPREAD_ATOMS = 64;
+// Spread without atomics on the grid.  The grid of every system is cut into tiles
+// of 8 points per axis (the last tile of an axis takes the remainder, so tile
+// widths lie in [4, 11]).  Each step, k_pme_bin computes every charged atom's
+// stencil (base point and weights, stored per atom) and appends the atom to
+// the bin of the tile holding its base point.  An atom's 5-point stencil then
+// only reaches its own tile and the next one along each axis, so the points
+// of tile T receive contributions from the bins of T and of the tiles before
+// it: at most 8 bins.  k_spread_tiles runs one workgroup per tile, sums those
+// contributions in LDS (fixed point: the order of the adds does not matter)
+// and writes every point of the tile with plain stores.  Each point has one
+// owner, and the whole grid is overwritten.
+constexpr int TILE = 8;
+constexpr int TILE_MAX = 11;
 constexpr int SPREAD_THREADS = 256;
-constexpr int SPREAD_BRICK = 4096;  // int64 points in LDS (32 KB)
 
-__device__ __forceinline__ void grid_add_fixed(long long* dst, long long v, unsigned*) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)v);
+__device__ __forceinline__ int tile_of(int i, int tiles) { return min(i / TILE, tiles - 1); }
+
+// Atoms are visited in the neighbour list's sorted order, so the lanes of a
+// wave mostly fall into one or two tiles; each distinct tile of a wave takes
+// one atomic for all of its lanes (contended device-scope atomics serialise).
+template <class R>
+__global__ void k_pme_bin(int nslots, const int* __restrict__ order, const double* __restrict__ pos,
+                          const double* __restrict__ charge, const int* __restrict__ atom_sys,
+                          const SysInfo* __restrict__ sys, int* st_idx, R* st_w, int* bin_count, int* bins,
+                          int bin_cap, unsigned* status) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int a = slot < nslots ? order[slot] : -1;
+    const double q = a >= 0 ? charge[a] : 0.0;
+    int tile = -1;
+    if (q != 0.0) {  // uncharged atoms do not spread (pme.rs:315-330)
+        const SysInfo& S = sys[atom_sys[a]];
+        int idx[3];
+        R w[3][5], dw[3][5];
+        stencil<R>(pos + 3 * a, S.box, S.grid, idx, w, dw);
+        for (int d = 0; d < 3; ++d) st_idx[4 * a + d] = idx[d];
+        for (int d = 0; d < 3; ++d)
+            for (int k = 0; k < 5; ++k) st_w[16 * a + 5 * d + k] = w[d][k];
+        tile = S.tile_off + (tile_of(idx[0], S.tiles[0]) * S.tiles[1] + tile_of(idx[1], S.tiles[1])) * S.tiles[2] +
+               tile_of(idx[2], S.tiles[2]);
+    }
+    unsigned long long pending = __ballot(tile >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int t = __shfl(tile, leader, 64);
+        const unsigned long long group = __ballot(tile == t);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&bin_count[t], __popcll(group));
+        base = __shfl(base, leader, 64);
+        if (tile == t) {
+            const int p = base + __popcll(group & ((1ull << lane) - 1ull));
+            if (p < bin_cap) bins[(size_t)t * bin_cap + p] = a;
+            else atomicOr(status, ST_BIN_OVERFLOW);
+        }
+        pending &= ~group;
+    }
 }
-__device__ __forceinline__ void grid_add_fixed(double*, long long, unsigned*) {}
+
+__device__ __forceinline__ void lds_add(unsigned long long* p, long long v) {
+    if (v) atomicAdd(p, (unsigned long long)v);
+}
+__device__ __forceinline__ void lds_add(double* p, double v) {
+    if (v != 0.0) atomicAdd(p, v);
+}
 
 template <class R, class G>
-__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_lds(const int* __restrict__ chunk_off, int nsys,
-                                                               const int* __restrict__ order,
-                                                               const double* __restrict__ pos,
-                                                               const double* __restrict__ charge,
-                                                               const SysInfo* __restrict__ sys, G* grid,
-                                                               unsigned* status) {
-    static_assert(SPREAD_ATOMS == 64 && SPREAD_ATOMS == PME_CHUNK, "the extent reduction runs in one wave64");
-    __shared__ unsigned long long s_brick[SPREAD_BRICK];
-    __shared__ R s_w[SPREAD_ATOMS][3][5];
-    __shared__ int s_rel[SPREAD_ATOMS][3];
-    __shared__ double s_q[SPREAD_ATOMS];
-    __shared__ int s_base[3], s_ext[3], s_fits;
+__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __restrict__ tile_off, int nsys,
+                                                                 const SysInfo* __restrict__ sys,
+                                                                 const int* __restrict__ st_idx,
+                                                                 const R* __restrict__ st_w,
+                                                                 const double* __restrict__ charge,
+                                                                 const int* __restrict__ bin_count,
+                                                                 const int* __restrict__ bins, int bin_cap, G* grid,
+                                                                 unsigned* status) {
     constexpr bool FIXED = sizeof(G) == sizeof(long long) && G(0.5) == G(0);  // int64 grid
-    const int si = find_sys(chunk_off, nsys, blockIdx.x);
+    using Acc = typename std::conditional<FIXED, unsigned long long, double>::type;
+    __shared__ Acc s_acc[TILE_MAX * TILE_MAX * TILE_MAX];
+    __shared__ int s_src[8], s_pre[9];
+    const int si = find_sys(tile_off, nsys, blockIdx.x);
     const SysInfo& S = sys[si];
-    const int nx = S.grid[0], ny = S.grid[1], nz = S.grid[2];
-    const int dims[3] = {nx, ny, nz};
-    grid += S.qgrid_off;
-    const int t = threadIdx.x;
-    if (t < SPREAD_ATOMS) {
-        // Wave 0: one atom per lane; stencil, then the brick covering all of
-        // them, unwrapped around the first charged atom's base point.
-        const int slot = S.slot_off + (blockIdx.x - S.chunk_off) * SPREAD_ATOMS + t;
-        const int a = slot < S.slot_off + S.nblocks * IBLOCK ? order[slot] : -1;
-        const double q = a >= 0 ? charge[a] : 0.0;
-        int idx[3] = {0, 0, 0};
-        if (q != 0.0) {
-            R dw[3][5];
-            R w[3][5];
-            stencil<R>(pos + 3 * a, S.box, dims, idx, w, dw);
-            for (int d = 0; d < 3; ++d)
-                for (int k = 0; k < 5; ++k) s_w[t][d][k] = w[d][k];
-        }
-        s_q[t] = q;
-        const unsigned long long live = __ballot(q != 0.0);
-        if (!live) {
-            if (t == 0) s_fits = -1;
-        } else {
-            const int ref = __ffsll((long long)live) - 1;
-            int lo[3], hi[3];
-            for (int d = 0; d < 3; ++d) {
-                const int r0 = __shfl(idx[d], ref, 64);
-                int r = idx[d] - r0;  // in (-n, n)
-                const int h = dims[d] >> 1;
-                if (r < -h) r += dims[d];
-                if (r >= dims[d] - h) r -= dims[d];
-                lo[d] = q != 0.0 ? r : (1 << 30);
-                hi[d] = q != 0.0 ? r + 4 : -(1 << 30);
-                for (int off = 32; off >= 1; off >>= 1) {
-                    lo[d] = min(lo[d], __shfl_xor(lo[d], off, 64));
-                    hi[d] = max(hi[d], __shfl_xor(hi[d], off, 64));
+    const int T0 = S.tiles[0], T1 = S.tiles[1], T2 = S.tiles[2];
+    const int local = blockIdx.x - S.tile_off;
+    const int tz = local % T2, txy = local / T2;
+    const int ty = txy % T1, tx = txy / T1;
+    const int tcoord[3] = {tx, ty, tz}, tiles[3] = {T0, T1, T2};
+    int lo[3], wid[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = tcoord[d] * TILE;
+        wid[d] = (tcoord[d] == tiles[d] - 1 ? S.grid[d] : lo[d] + TILE) - lo[d];
+    }
+    const int npts = wid[0] * wid[1] * wid[2];
+    for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) s_acc[i] = (Acc)0;
+    if (threadIdx.x == 0) {
+        // Source bins: this tile and the previous one along each axis
+        // (periodic), without repeats when an axis has one or two tiles.
+        int nsrc = 0;
+        for (int dx = -1; dx <= 0; ++dx)
+            for (int dy = -1; dy <= 0; ++dy)
+                for (int dz = -1; dz <= 0; ++dz) {
+                    if ((dx && T0 == 1) || (dy && T1 == 1) || (dz && T2 == 1)) continue;
+                    const int sx = (tx + dx + T0) % T0, sy = (ty + dy + T1) % T1, sz = (tz + dz + T2) % T2;
+                    s_src[nsrc++] = S.tile_off + (sx * T1 + sy) * T2 + sz;
                 }
-                if (t == 0) {
-                    s_base[d] = r0 + lo[d];
-                    s_ext[d] = hi[d] - lo[d] + 1;
-                }
-                s_rel[t][d] = r - lo[d];
+        s_pre[0] = 0;
+        for (int b = 0; b < nsrc; ++b) s_pre[b + 1] = s_pre[b] + min(bin_count[s_src[b]], bin_cap);
+        for (int b = nsrc; b < 8; ++b) s_pre[b + 1] = s_pre[b];
+    }
+    __syncthreads();
+    const int total = s_pre[8];
+    for (int k = threadIdx.x; k < total; k += SPREAD_THREADS) {
+        int b = 0;
+        while (s_pre[b + 1] <= k) ++b;
+        const int a = bins[(size_t)s_src[b] * bin_cap + (k - s_pre[b])];
+        const double q = charge[a];
+        int loc[3][5];
+        for (int d = 0; d < 3; ++d) {
+            const int n = S.grid[d];
+            int p = st_idx[4 * a + d];
+            for (int j = 0; j < 5; ++j) {
+                const int l = p - lo[d];
+                loc[d][j] = (l >= 0 && l < wid[d]) ? l : -1;
+                if (++p == n) p = 0;
             }
-            if (t == 0) {
-                const long long vol = (long long)s_ext[0] * s_ext[1] * s_ext[2];
-                s_fits = (FIXED && vol <= SPREAD_BRICK) ? 1 : 0;
+        }
+        const R* w = st_w + 16 * (size_t)a;
+        for (int ix = 0; ix < 5; ++ix) {
+            if (loc[0][ix] < 0) continue;
+            for (int iy = 0; iy < 5; ++iy) {
+                if (loc[1][iy] < 0) continue;
+                const R wxy = w[ix] * w[5 + iy];
+                Acc* row = s_acc + (loc[0][ix] * wid[1] + loc[1][iy]) * wid[2];
+                for (int iz = 0; iz < 5; ++iz) {
+                    if (loc[2][iz] < 0) continue;
+                    const double c = q * (double)(wxy * w[10 + iz]);
+                    if (FIXED) lds_add(row + loc[2][iz], (long long)to_fixed(c, GRID_SCALE, status));
+                    else lds_add(row + loc[2][iz], c);
+                }
             }
         }
     }
     __syncthreads();
-    const int fits = s_fits;
-    if (fits < 0) return;  // no charged atom in this chunk
-    const int ex = s_ext[0], ey = s_ext[1], ez = s_ext[2];
-    if (fits)
-        for (int i = t; i < ex * ey * ez; i += SPREAD_THREADS) s_brick[i] = 0ull;
-    __syncthreads();
-    for (int item = t; item < SPREAD_ATOMS * 25; item += SPREAD_THREADS) {
-        const int k = item / 25, ixy = item - k * 25;
-        const double q = s_q[k];
-        if (q == 0.0) continue;
-        const int ix = ixy / 5, iy = ixy - ix * 5;
-        const R wxy = s_w[k][0][ix] * s_w[k][1][iy];
-        const int rx = s_rel[k][0] + ix, ry = s_rel[k][1] + iy, rz = s_rel[k][2];
-        if (fits) {
-            unsigned long long* row = s_brick + ((size_t)rx * ey + ry) * ez + rz;
-            for (int iz = 0; iz < 5; ++iz) {
-                const long long f = to_fixed(q * (double)(wxy * s_w[k][2][iz]), GRID_SCALE, status);
-                if (f) atomicAdd(row + iz, (unsigned long long)f);
-            }
-        } else {
-            int gx = s_base[0] + rx, gy = s_base[1] + ry;
-            while (gx < 0) gx += nx;
-            while (gx >= nx) gx -= nx;
-            while (gy < 0) gy += ny;
-            while (gy >= ny) gy -= ny;
-            G* row = grid + ((size_t)gx * ny + gy) * nz;
-            int gz = s_base[2] + rz;
-            while (gz < 0) gz += nz;
-            while (gz >= nz) gz -= nz;
-            for (int iz = 0; iz < 5; ++iz) {
-                acc_add(row + gz, q * (double)(wxy * s_w[k][2][iz]), GRID_SCALE, status);
-                if (++gz == nz) gz = 0;
-            }
-        }
-    }
-    if (!fits) return;
-    __syncthreads();
-    // Flush one (x, y) row of the brick per thread.
-    for (int rowi = t; rowi < ex * ey; rowi += SPREAD_THREADS) {
-        const int lx = rowi / ey, ly = rowi - lx * ey;
-        int gx = s_base[0] + lx, gy = s_base[1] + ly, gz = s_base[2];
-        while (gx < 0) gx += nx;
-        while (gx >= nx) gx -= nx;
-        while (gy < 0) gy += ny;
-        while (gy >= ny) gy -= ny;
-        while (gz < 0) gz += nz;
-        while (gz >= nz) gz -= nz;
-        G* dst = grid + ((size_t)gx * ny + gy) * nz;
-        const unsigned long long* src = s_brick + (size_t)rowi * ez;
-        for (int lz = 0; lz < ez; ++lz) {
-            const unsigned long long v = src[lz];
-            if (v) grid_add_fixed(dst + gz, (long long)v, status);
-            if (++gz == nz) gz = 0;
-        }
+    G* g = grid + S.qgrid_off;
+    const int ny = S.grid[1], nz = S.grid[2];
+    for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) {
+        const int lz = i % wid[2], r = i / wid[2];
+        const int ly = r % wid[1], lx = r / wid[1];
+        g[((size_t)(lo[0] + lx) * ny + (lo[1] + ly)) * nz + lo[2] + lz] = (G)s_acc[i];
     }
 }
 
@@ -379,11 +390,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         Cx<R> v = {0, 0};
         if (L < nlines) {
             if (MODE == MODE_Z_R2C) {
-                // The charge grid is consumed here and left zeroed for the
-                // next spread (no separate clearing kernel).
-                const size_t gi = (size_t)L * nz + el;
-                v.x = (R)grid_real(qgrid[gi]);
-                qgrid[gi] = (G)0;
+                v.x = (R)grid_real(qgrid[(size_t)L * nz + el]);
             } else if (MODE == MODE_Z_C2R) {
                 if (el < nzh) {
                     v = cgrid[(size_t)L * nzh + el];
@@ -517,14 +524,30 @@ void launch_fft(const PmeDev& pm, const SysInfo* sys, G* qgrid, Cx<R>* cgrid, R*
                        reinterpret_cast<const Cx<R>*>(roots), moduli, energy, status);
 }
 
+template <class R>
+R* pme_weights(PmeDev& pm);
+template <>
+float* pme_weights<float>(PmeDev& pm) {
+    return pm.st_wf.p;
+}
+template <>
+double* pme_weights<double>(PmeDev& pm) {
+    return pm.st_wd.p;
+}
+
 template <class R, class G, class A>
 void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* charge, const SysInfo* sys, G* qgrid,
            R* cgrid, R* phi, const R* roots, const R* moduli, A* force, A* energy, unsigned* status, int phases,
            hipStream_t s) {
     if (phases & PME_SPREAD) {
-        // qgrid is zero here: the z pass of the previous evaluation cleared it.
-        hipLaunchKernelGGL((k_spread_lds<R, G>), dim3(nl.nchunks), dim3(SPREAD_THREADS), 0, s, nl.chunk_off.p,
-                           pm.nsys, nl.sorted_atom.p, pos, charge, sys, qgrid, status);
+        R* st_w = pme_weights<R>(pm);
+        fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
+        hipLaunchKernelGGL((k_pme_bin<R>), dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.nslots,
+                           nl.sorted_atom.p, pos, charge, pm.atom_sys, sys, pm.st_idx.p, st_w, pm.bin_count.p,
+                           pm.bins.p, pm.bin_cap, status);
+        hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
+                           pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap, qgrid,
+                           status);
     }
     if (phases & PME_FFT) {
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);
@@ -561,9 +584,18 @@ std::vector<double> bspline_moduli(int n) {
     return values;
 }
 
-void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, bool mixed, hipStream_t s) {
+void pm_set_bin_cap(PmeDev& pm, int cap) {
+    pm.bin_cap = cap;
+    pm.bins.alloc((size_t)pm.ntiles * cap);
+}
+
+void pme_grow_bins(PmeDev& pm) { pm_set_bin_cap(pm, 2 * pm.bin_cap); }
+
+void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* atom_sys, bool mixed, hipStream_t s) {
     const int nsys = (int)sys.size();
     pm.nsys = nsys;
+    pm.natoms = natoms;
+    pm.atom_sys = atom_sys;
     std::vector<double> roots, moduli;
     std::vector<int> off[2][5];
     for (int p = 0; p < 2; ++p)
@@ -635,6 +667,23 @@ void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, bool mixed, hipStream_t s)
     }
     pm.qpoints = qpts;
     pm.cpoints = cpts;
+    // Spread tiles and bins.
+    std::vector<int> toff{0};
+    int bin_cap = 64;
+    for (SysInfo& S : sys) {
+        for (int a = 0; a < 3; ++a) S.tiles[a] = std::max(1, (S.grid[a] + TILE / 2) / TILE);
+        S.tile_off = toff.back();
+        const int nt = S.tiles[0] * S.tiles[1] * S.tiles[2];
+        toff.push_back(toff.back() + nt);
+        bin_cap = std::max(bin_cap, 2 * ((S.n + nt - 1) / nt) + 64);
+    }
+    pm.ntiles = toff.back();
+    pm.tile_off.upload(toff, s);
+    pm.bin_count.alloc((size_t)pm.ntiles);
+    pm_set_bin_cap(pm, bin_cap);
+    pm.st_idx.alloc(4 * (size_t)pm.natoms);
+    if (mixed) pm.st_wf.alloc(16 * (size_t)pm.natoms);
+    pm.st_wd.alloc(16 * (size_t)pm.natoms);
     for (int p = 0; p < 2; ++p)
         for (int k = 0; k < 5; ++k) {
             pm.fft_off[p][k].upload(off[p][k], s);

@@ -1,6 +1,6 @@
 // FP64 per-term device functions shared by the exception kernel and the MD
-// step's fused bonded kernel.  Each returns the term's energy and adds its
-// forces to the accumulator.
+// step's fused bonded kernel.  Each returns the term's energy and writes the
+// forces on its particles; the caller accumulates them.
 #pragma once
 
 #include "common.h"
@@ -10,11 +10,10 @@ namespace ommhip {
 // Exception pair (nonbonded.rs:995-1036): LJ and bare Coulomb with the pair's
 // own parameters (lj_coulomb_exception, 1194-1198), minus the reciprocal-space
 // erf correction with the particle charges (reciprocal_pair, 1275-1288) when
-// a reciprocal method is active.
-template <class A>
-__device__ __forceinline__ double exception_term(int i, int j, double qq, double sig, double eps, double qi, double qj,
+// a reciprocal method is active.  Force on i is fi, on j is -fi.
+__device__ __forceinline__ double exception_pair(int i, int j, double qq, double sig, double eps, double qi, double qj,
                                                  const double* __restrict__ pos, const Box& box, int use_pbc,
-                                                 int has_pme, double alpha, A* force, unsigned* status) {
+                                                 int has_pme, double alpha, double fi[3]) {
     double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
     if (use_pbc) min_image_exact(d, box.v);
     double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -45,32 +44,26 @@ __device__ __forceinline__ double exception_term(int i, int j, double qq, double
         e -= corr;
         f -= deriv;
     }
-    for (int a = 0; a < 3; ++a) {
-        acc_add(force + 3 * i + a, f * d[a], FORCE_SCALE, status);
-        acc_add(force + 3 * j + a, -f * d[a], FORCE_SCALE, status);
-    }
+    for (int a = 0; a < 3; ++a) fi[a] = f * d[a];
     return e;
 }
 
-// HarmonicBondForce term without PBC (harmonic_bond.rs:113-140).
-template <class A>
-__device__ __forceinline__ double bond_term(int i, int j, double length, double k, const double* __restrict__ pos,
-                                            A* force, unsigned* status) {
+// HarmonicBondForce term without PBC (harmonic_bond.rs:113-140).  Force on i
+// is fi, on j is -fi.
+__device__ __forceinline__ double bond_pair(int i, int j, double length, double k, const double* __restrict__ pos,
+                                            double fi[3]) {
     double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
     double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     double dr = r - length;
     double f = k * dr / r;
-    for (int a = 0; a < 3; ++a) {
-        acc_add(force + 3 * i + a, f * d[a], FORCE_SCALE, status);
-        acc_add(force + 3 * j + a, -f * d[a], FORCE_SCALE, status);
-    }
+    for (int a = 0; a < 3; ++a) fi[a] = f * d[a];
     return 0.5 * k * dr * dr;
 }
 
-// HarmonicAngleForce term without PBC (harmonic_angle.rs:118-167); j is the vertex.
-template <class A>
-__device__ __forceinline__ double angle_term(int i, int j, int k, double theta0, double ka,
-                                             const double* __restrict__ pos, A* force, unsigned* status) {
+// HarmonicAngleForce term without PBC (harmonic_angle.rs:118-167); j is the
+// vertex.  Forces on i, j, k are fi, -(fi + fk), fk.
+__device__ __forceinline__ double angle_triple(int i, int j, int k, double theta0, double ka,
+                                               const double* __restrict__ pos, double fi[3], double fk[3]) {
     double rji[3], rjk[3];
     for (int a = 0; a < 3; ++a) {
         rji[a] = pos[3 * i + a] - pos[3 * j + a];
@@ -84,13 +77,49 @@ __device__ __forceinline__ double angle_term(int i, int j, int k, double theta0,
     double s = fmax(sqrt(1.0 - c * c), 1e-8);
     double de = ka * dth;
     for (int a = 0; a < 3; ++a) {
-        double fi = de * (c * rji[a] / lji - rjk[a] / ljk) / (lji * s);
-        double fk = de * (c * rjk[a] / ljk - rji[a] / lji) / (ljk * s);
-        acc_add(force + 3 * i + a, -fi, FORCE_SCALE, status);
-        acc_add(force + 3 * k + a, -fk, FORCE_SCALE, status);
-        acc_add(force + 3 * j + a, fi + fk, FORCE_SCALE, status);
+        fi[a] = -de * (c * rji[a] / lji - rjk[a] / ljk) / (lji * s);
+        fk[a] = -de * (c * rjk[a] / ljk - rji[a] / lji) / (ljk * s);
     }
     return 0.5 * ka * dth * dth;
+}
+
+// Accumulating forms: add the forces to the force accumulator.
+template <class A>
+__device__ __forceinline__ double exception_term(int i, int j, double qq, double sig, double eps, double qi, double qj,
+                                                 const double* __restrict__ pos, const Box& box, int use_pbc,
+                                                 int has_pme, double alpha, A* force, unsigned* status) {
+    double fi[3];
+    const double e = exception_pair(i, j, qq, sig, eps, qi, qj, pos, box, use_pbc, has_pme, alpha, fi);
+    for (int a = 0; a < 3; ++a) {
+        acc_add(force + 3 * i + a, fi[a], FORCE_SCALE, status);
+        acc_add(force + 3 * j + a, -fi[a], FORCE_SCALE, status);
+    }
+    return e;
+}
+
+template <class A>
+__device__ __forceinline__ double bond_term(int i, int j, double length, double k, const double* __restrict__ pos,
+                                            A* force, unsigned* status) {
+    double fi[3];
+    const double e = bond_pair(i, j, length, k, pos, fi);
+    for (int a = 0; a < 3; ++a) {
+        acc_add(force + 3 * i + a, fi[a], FORCE_SCALE, status);
+        acc_add(force + 3 * j + a, -fi[a], FORCE_SCALE, status);
+    }
+    return e;
+}
+
+template <class A>
+__device__ __forceinline__ double angle_term(int i, int j, int k, double theta0, double ka,
+                                             const double* __restrict__ pos, A* force, unsigned* status) {
+    double fi[3], fk[3];
+    const double e = angle_triple(i, j, k, theta0, ka, pos, fi, fk);
+    for (int a = 0; a < 3; ++a) {
+        acc_add(force + 3 * i + a, fi[a], FORCE_SCALE, status);
+        acc_add(force + 3 * k + a, fk[a], FORCE_SCALE, status);
+        acc_add(force + 3 * j + a, -(fi[a] + fk[a]), FORCE_SCALE, status);
+    }
+    return e;
 }
 
 }  // namespace ommhip
