@@ -64,12 +64,16 @@ def timed_steps(ctx, steps):
     return time.perf_counter() - t0
 
 
+GRAPH_STEPS = 10  # steps per captured graph; 0 with --eager
+
+
 def run_single(steps, warmup, equilibrate, plain_steps, seed=2024):
     """23,550-atom step: mixed form timed after equilibration; the plain FP64
     form is then timed from the same equilibrated state."""
     s = workload(seed)
     ctx = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seed,
                          precision=ommhip.PRECISION_MIXED)
+    ctx.set_graph_steps(GRAPH_STEPS)
     ctx.set_velocities_to_temperature(300.0, seed + 1)
     equilibrate_then(ctx, equilibrate, warmup)
     secs = timed_steps(ctx, steps)
@@ -98,6 +102,7 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
     seeds = [seed + 100 + r for r in range(count)]
     batch = ommhip.Batch(systems, temperature=300.0, friction=1.0, step_size=DT_PS, seeds=seeds,
                          precision=ommhip.PRECISION_MIXED)
+    batch.set_graph_steps(GRAPH_STEPS)
     batch.set_velocities_to_temperature(300.0, seed + 1)
     equilibrate_then(batch, equilibrate, warmup)
     secs = timed_steps(batch, steps)
@@ -170,8 +175,15 @@ def main():
     ap.add_argument("--batch-equilibrate", type=int, default=1000)
     ap.add_argument("--batch-plain-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the 64-replica workload and print its record (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step eagerly instead of replaying captured graphs (profiling)")
     args = ap.parse_args()
+    global GRAPH_STEPS
+    if args.eager:
+        GRAPH_STEPS = 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas would shard "
               "across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
@@ -180,6 +192,10 @@ def main():
         print("no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.batch_only:
+        rec = run_batch(args.batch_steps, 100, args.batch_equilibrate, args.batch_plain_steps)
+        print(json.dumps(rec), flush=True)
+        return
     ctx, secs, psecs = run_single(args.steps, args.warmup, args.equilibrate, args.plain_steps)
     value = ns_per_day(args.steps, secs)
     phases, npairs = ctx.profile(args.profile_reps)

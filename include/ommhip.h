@@ -230,6 +230,11 @@ omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double frict
  * and the next step's build kernels rebuild it on the device. */
 omm_status omm_ctx_step(omm_ctx* ctx, int32_t nsteps);
 
+/* Steps per captured HIP graph of the mixed form (default 10; 0 launches
+ * every step eagerly, with the same kernels and results).  Launch policy, no
+ * counterpart in the reference. */
+omm_status omm_ctx_set_graph_steps(omm_ctx* ctx, int32_t steps);
+
 /* Shape of the current neighbour list (measurement hook; no counterpart in the
  * reference): number of 32 x 64 tiles the pair kernel evaluates and the total
  * number of j-list entries.  Pair-slot efficiency = pairs / (tiles * 2048). */
@@ -297,6 +302,9 @@ omm_status omm_batch_set_integrator(omm_batch* b, double temperature, double fri
 
 /* Context::step of every replica (nsteps each). */
 omm_status omm_batch_step(omm_batch* b, int32_t nsteps);
+
+/* As omm_ctx_set_graph_steps, for the whole batch. */
+omm_status omm_batch_set_graph_steps(omm_batch* b, int32_t steps);
 
 /* Context::get_state of replica r; any output pointer may be NULL. */
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,

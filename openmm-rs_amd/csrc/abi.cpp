@@ -197,6 +197,8 @@ omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double frict
     CTX_CALL(ctx->ctx.set_integrator(temperature, friction, step_size));
 }
 
+omm_status omm_ctx_set_graph_steps(omm_ctx* ctx, int32_t steps) { CTX_CALL(ctx->ctx.set_graph_steps(steps)); }
+
 omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries) {
     CTX_CALL(ctx->ctx.list_stats(tiles, entries));
 }
@@ -260,6 +262,8 @@ omm_status omm_batch_set_integrator(omm_batch* b, double temperature, double fri
 }
 
 omm_status omm_batch_step(omm_batch* b, int32_t nsteps) { BATCH_CALL(b->ctx.step(nsteps)); }
+
+omm_status omm_batch_set_graph_steps(omm_batch* b, int32_t steps) { BATCH_CALL(b->ctx.set_graph_steps(steps)); }
 
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
                                double* potential_energy, int64_t* step_count) {

@@ -34,6 +34,9 @@ public:
     void set_velocities_to_temperature(double temperature, uint64_t seed);
     void step(int nsteps);
     void set_integrator(double temperature, double friction, double step_size);
+    // Steps per captured HIP graph in the mixed form (0: launch every step
+    // eagerly; the same kernels run either way).
+    void set_graph_steps(int steps);
     void get_state(int r, double* pos, double* vel, double* forces, double* energy, int64_t* step);
     // Potential energy of every replica (nsys values).
     void energies(double* out);
@@ -81,6 +84,7 @@ private:
     std::vector<double> masses_;
     double dt_ = 0.001, temperature_ = 0.0, friction_ = 0.0;
     int64_t step_count_ = 0;
+    int graph_steps_ = 10;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
     BondedDev bonded_;
     DevBuf<long long> force_fx_, energy_fx_;

@@ -16,7 +16,6 @@ namespace {
 
 // BOLTZMANN in kJ/mol/K (langevin.rs:12).
 const double BOLTZMANN = 1.380649e-23 * 6.02214076e23 / 1000.0;
-constexpr int GRAPH_STEPS = 10;
 
 // Bonded terms in one launch.  Threads [0, ngroups) take one small molecule
 // each (at most four atoms joined by bonds, angles or exceptions, e.g. a
@@ -566,7 +565,7 @@ void MdContext::restore_snapshot() {
     nb_->force_rebuild(s_);
 }
 
-// The captured graph holds GRAPH_STEPS steps and a snapshot of the state
+// The captured graph holds graph_steps_ steps and a snapshot of the state
 // taken before them.  Each step rebuilds the list only when the previous
 // integrator flagged it (the build kernels exit at once otherwise).  A list
 // or PME-bin overflow inside the graph restores the snapshot, grows the
@@ -578,7 +577,7 @@ void MdContext::capture_graph() {
         const size_t n3 = 3 * (size_t)n_;
         hipLaunchKernelGGL(k_copy2, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s_, pos_.p, snap_pos_.p, vel_.p,
                            snap_vel_.p, n3, step_dev_.p, snap_step_.p);
-        for (int k = 0; k < GRAPH_STEPS; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
+        for (int k = 0; k < graph_steps_; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s_, &g);
@@ -587,6 +586,12 @@ void MdContext::capture_graph() {
     }
     OMM_HIP(hipStreamEndCapture(s_, &graph_));
     OMM_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+}
+
+void MdContext::set_graph_steps(int steps) {
+    if (steps < 0 || steps > 1000) throw Error(OMM_ERR_INVALID, "graph step count must be in [0, 1000]");
+    if (steps != graph_steps_) drop_graph();
+    graph_steps_ = steps;
 }
 
 void MdContext::set_integrator(double temperature, double friction, double step_size) {
@@ -607,9 +612,9 @@ void MdContext::step(int nsteps) {
     if (nsteps < 0) throw Error(OMM_ERR_INVALID, "step count must be nonnegative");
     prepare();
     int done = 0;
-    if (mixed_ && nsteps >= GRAPH_STEPS) {
+    if (mixed_ && graph_steps_ > 0 && nsteps >= graph_steps_) {
         if (!graph_exec_) capture_graph();
-        while (nsteps - done >= GRAPH_STEPS) {
+        while (nsteps - done >= graph_steps_) {
             OMM_HIP(hipGraphLaunch(graph_exec_, s_));
             unsigned st = read_status();
             for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
@@ -618,11 +623,11 @@ void MdContext::step(int nsteps) {
                 drop_graph();
                 restore_snapshot();
                 nb_->grow(st);
-                for (int k = 0; k < GRAPH_STEPS; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
+                for (int k = 0; k < graph_steps_; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
                 st = read_status();
                 if (!(st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW))) capture_graph();
             }
-            done += GRAPH_STEPS;
+            done += graph_steps_;
         }
     }
     // Plain FP64 form, and the remainder: one launch sequence per step.

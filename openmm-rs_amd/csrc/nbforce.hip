@@ -423,7 +423,18 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
         OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         OMM_HIP(hipStreamSynchronize(s));
         int mx = 0;
-        for (int c : counts) mx = std::max(mx, c);
+        long long tasks = 0;
+        for (int c : counts) {
+            mx = std::max(mx, c);
+            tasks += (c + JTILE - 1) / JTILE;
+        }
+        // One wave per task of the calibrated list (grid-stride beyond it):
+        // enough resident waves to hide the latency of the pair loop.
+        const int grid = (int)std::max<long long>(64, std::min<long long>(65535, tasks));
+        if (grid != pair_grid_) {
+            pair_grid_ = grid;
+            changed = true;  // captured graphs hold the old launch shape
+        }
         long long want = std::min<long long>((long long)max_npad + JTILE, 2LL * mx + 2 * JTILE);
         want = (want + JTILE - 1) / JTILE * JTILE;
         if (!(st & ST_LIST_OVERFLOW) && want <= nl_.cap) break;

@@ -143,6 +143,9 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
     using Acc = typename std::conditional<FIXED, unsigned long long, double>::type;
     __shared__ Acc s_acc[TILE_MAX * TILE_MAX * TILE_MAX];
     __shared__ int s_src[8], s_pre[9];
+    __shared__ int s_idx[3][SPREAD_THREADS];
+    __shared__ R s_w[15][SPREAD_THREADS];
+    __shared__ double s_q[SPREAD_THREADS];
     const int si = find_sys(tile_off, nsys, blockIdx.x);
     const SysInfo& S = sys[si];
     const int T0 = S.tiles[0], T1 = S.tiles[1], T2 = S.tiles[2];
@@ -173,39 +176,55 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
         for (int b = nsrc; b < 8; ++b) s_pre[b + 1] = s_pre[b];
     }
     __syncthreads();
-    const int total = s_pre[8];
-    for (int k = threadIdx.x; k < total; k += SPREAD_THREADS) {
-        int b = 0;
-        while (s_pre[b + 1] <= k) ++b;
-        const int a = bins[(size_t)s_src[b] * bin_cap + (k - s_pre[b])];
-        const double q = charge[a];
-        int loc[3][5];
-        for (int d = 0; d < 3; ++d) {
-            const int n = S.grid[d];
-            int p = st_idx[4 * a + d];
-            for (int j = 0; j < 5; ++j) {
-                const int l = p - lo[d];
-                loc[d][j] = (l >= 0 && l < wid[d]) ? l : -1;
-                if (++p == n) p = 0;
-            }
+    // Source atoms are staged in LDS, SPREAD_THREADS at a time, with one
+    // independent load chain per thread; the work items are then (staged
+    // atom, x plane of its stencil), five per atom, so atoms that reach only
+    // a corner of the tile do not hold up the others.
+    const int natoms_src = s_pre[8];
+    for (int base = 0; base < natoms_src; base += SPREAD_THREADS) {
+        const int k = base + threadIdx.x;
+        if (k < natoms_src) {
+            int b = 0;
+            while (s_pre[b + 1] <= k) ++b;
+            const int a = bins[(size_t)s_src[b] * bin_cap + (k - s_pre[b])];
+            for (int d = 0; d < 3; ++d) s_idx[d][threadIdx.x] = st_idx[4 * a + d];
+            for (int j = 0; j < 15; ++j) s_w[j][threadIdx.x] = st_w[16 * (size_t)a + j];
+            s_q[threadIdx.x] = charge[a];
         }
-        const R* w = st_w + 16 * (size_t)a;
-        for (int ix = 0; ix < 5; ++ix) {
-            if (loc[0][ix] < 0) continue;
+        __syncthreads();
+        const int cnt = min(SPREAD_THREADS, natoms_src - base);
+        for (int item = threadIdx.x; item < 5 * cnt; item += SPREAD_THREADS) {
+            const int m = item / 5, ix = item - 5 * m;
+            int px = s_idx[0][m] + ix;
+            if (px >= S.grid[0]) px -= S.grid[0];
+            const int lx = px - lo[0];
+            if (lx < 0 || lx >= wid[0]) continue;
+            const double q = s_q[m];
+            int loc[2][5];
+            for (int d = 0; d < 2; ++d) {
+                const int n = S.grid[d + 1];
+                int p = s_idx[d + 1][m];
+                for (int j = 0; j < 5; ++j) {
+                    const int l = p - lo[d + 1];
+                    loc[d][j] = (l >= 0 && l < wid[d + 1]) ? l : -1;
+                    if (++p == n) p = 0;
+                }
+            }
+            const R wx = s_w[ix][m];
             for (int iy = 0; iy < 5; ++iy) {
-                if (loc[1][iy] < 0) continue;
-                const R wxy = w[ix] * w[5 + iy];
-                Acc* row = s_acc + (loc[0][ix] * wid[1] + loc[1][iy]) * wid[2];
+                if (loc[0][iy] < 0) continue;
+                const R wxy = wx * s_w[5 + iy][m];
+                Acc* row = s_acc + (lx * wid[1] + loc[0][iy]) * wid[2];
                 for (int iz = 0; iz < 5; ++iz) {
-                    if (loc[2][iz] < 0) continue;
-                    const double c = q * (double)(wxy * w[10 + iz]);
-                    if (FIXED) lds_add(row + loc[2][iz], (long long)to_fixed(c, GRID_SCALE, status));
-                    else lds_add(row + loc[2][iz], c);
+                    if (loc[1][iz] < 0) continue;
+                    const double c = q * (double)(wxy * s_w[10 + iz][m]);
+                    if (FIXED) lds_add(row + loc[1][iz], (long long)to_fixed(c, GRID_SCALE, status));
+                    else lds_add(row + loc[1][iz], c);
                 }
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
     G* g = grid + S.qgrid_off;
     const int ny = S.grid[1], nz = S.grid[2];
     for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) {

@@ -533,6 +533,10 @@ class Context:
         """LangevinMiddleIntegrator set_temperature / set_friction / set_step_size."""
         _check(library().omm_ctx_set_integrator(self._h, temperature, friction, step_size))
 
+    def set_graph_steps(self, steps):
+        """Steps per captured HIP graph (0: eager launches)."""
+        _check(library().omm_ctx_set_graph_steps(self._h, int(steps)))
+
     def list_stats(self):
         """(tiles, j-list entries) of the current neighbour list."""
         t, e = C.c_int64(), C.c_int64()
@@ -621,6 +625,9 @@ class Batch:
 
     def set_integrator(self, temperature, friction, step_size):
         _check(library().omm_batch_set_integrator(self._h, temperature, friction, step_size))
+
+    def set_graph_steps(self, steps):
+        _check(library().omm_batch_set_graph_steps(self._h, int(steps)))
 
     def get_state(self, replica):
         n = self.sizes[replica]
