@@ -235,6 +235,40 @@ omm_status omm_ctx_step(omm_ctx* ctx, int32_t nsteps);
  * counterpart in the reference. */
 omm_status omm_ctx_set_graph_steps(omm_ctx* ctx, int32_t steps);
 
+/* MonteCarloBarostat (forces/barostat.rs:59-218) added to the context:
+ * isotropic volume moves every `frequency` steps at `pressure` bar and
+ * `temperature` K, molecules (bond-connected components, system.rs:100-141)
+ * scaled as rigid bodies; frequency 0 removes it.  The attempt runs at the
+ * start of the step that completes a period (context/program.rs:5-45), with
+ * energies from the FP64 energy path.  The random stream (Philox keyed by
+ * seed) differs from the reference's ChaCha12, as the reference allows for
+ * seeded streams across implementations.  alpha and the PME grid stay as
+ * initialised (nonbonded.rs:1055-1107). */
+omm_status omm_ctx_set_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
+                                uint64_t seed);
+
+/* Barostat bookkeeping of one system (no counterpart in the reference, which
+ * keeps it private): the adaptive step and counters (barostat.rs:180-195)
+ * and the record of the last attempt. */
+typedef struct omm_barostat_info {
+    double volume_scale;
+    int64_t attempted, accepted;              /* since the last step-size change */
+    int64_t total_attempted, total_accepted;
+    double volume_old, volume_new;            /* nm^3 */
+    double energy_old, energy_new, work;      /* kJ/mol */
+    int32_t accepted_last, skipped;
+} omm_barostat_info;
+
+omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
+
+/* Current box vectors (row-major a, b, c); they change under a barostat
+ * (Context::get_state box vectors, context.rs:930-1105). */
+omm_status omm_ctx_get_box(omm_ctx* ctx, double box[9]);
+
+/* One barostat attempt now, outside the step sequence (test hook; no
+ * counterpart in the reference). */
+omm_status omm_ctx_barostat_attempt(omm_ctx* ctx);
+
 /* Shape of the current neighbour list (measurement hook; no counterpart in the
  * reference): number of 32 x 64 tiles the pair kernel evaluates and the total
  * number of j-list entries.  Pair-slot efficiency = pairs / (tiles * 2048). */
@@ -305,6 +339,12 @@ omm_status omm_batch_step(omm_batch* b, int32_t nsteps);
 
 /* As omm_ctx_set_graph_steps, for the whole batch. */
 omm_status omm_batch_set_graph_steps(omm_batch* b, int32_t steps);
+
+/* As omm_ctx_set_barostat for every replica; replica r draws from seed + r. */
+omm_status omm_batch_set_barostat(omm_batch* b, double pressure, double temperature, int32_t frequency,
+                                  uint64_t seed);
+omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info);
+omm_status omm_batch_barostat_attempt(omm_batch* b);
 
 /* Context::get_state of replica r; any output pointer may be NULL. */
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,

@@ -199,6 +199,36 @@ omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double frict
 
 omm_status omm_ctx_set_graph_steps(omm_ctx* ctx, int32_t steps) { CTX_CALL(ctx->ctx.set_graph_steps(steps)); }
 
+static void fill_info(const ommhip::BaroState& st, omm_barostat_info* info) {
+    info->volume_scale = st.volume_scale;
+    info->attempted = st.attempted;
+    info->accepted = st.accepted;
+    info->total_attempted = st.total_attempted;
+    info->total_accepted = st.total_accepted;
+    info->volume_old = st.v_old;
+    info->volume_new = st.v_new;
+    info->energy_old = st.e_old;
+    info->energy_new = st.e_new;
+    info->work = st.work;
+    info->accepted_last = st.accepted_last;
+    info->skipped = st.skipped;
+}
+
+omm_status omm_ctx_set_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
+                                uint64_t seed) {
+    CTX_CALL(ctx->ctx.set_barostat(pressure, temperature, frequency, seed));
+}
+
+omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info) {
+    CTX_CALL(if (!info) throw Error(OMM_ERR_INVALID, "null output"); fill_info(ctx->ctx.barostat_state(0), info));
+}
+
+omm_status omm_ctx_barostat_attempt(omm_ctx* ctx) { CTX_CALL(ctx->ctx.barostat_attempt()); }
+
+omm_status omm_ctx_get_box(omm_ctx* ctx, double box[9]) {
+    CTX_CALL(if (!box) throw Error(OMM_ERR_INVALID, "null output"); ctx->ctx.get_box(0, box));
+}
+
 omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries) {
     CTX_CALL(ctx->ctx.list_stats(tiles, entries));
 }
@@ -264,6 +294,18 @@ omm_status omm_batch_set_integrator(omm_batch* b, double temperature, double fri
 omm_status omm_batch_step(omm_batch* b, int32_t nsteps) { BATCH_CALL(b->ctx.step(nsteps)); }
 
 omm_status omm_batch_set_graph_steps(omm_batch* b, int32_t steps) { BATCH_CALL(b->ctx.set_graph_steps(steps)); }
+
+omm_status omm_batch_set_barostat(omm_batch* b, double pressure, double temperature, int32_t frequency,
+                                  uint64_t seed) {
+    BATCH_CALL(b->ctx.set_barostat(pressure, temperature, frequency, seed));
+}
+
+omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info) {
+    BATCH_CALL(if (!info) throw Error(OMM_ERR_INVALID, "null output");
+               fill_info(b->ctx.barostat_state(replica), info));
+}
+
+omm_status omm_batch_barostat_attempt(omm_batch* b) { BATCH_CALL(b->ctx.barostat_attempt()); }
 
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
                                double* potential_energy, int64_t* step_count) {

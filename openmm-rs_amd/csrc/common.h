@@ -37,6 +37,7 @@ enum : unsigned {
     ST_OVERFLOW = 2u,
     ST_LIST_OVERFLOW = 4u,
     ST_BIN_OVERFLOW = 8u,  // a PME spread tile received more atoms than its bin holds
+    ST_CUTOFF_BOX = 16u,   // a box edge fell below twice the cutoff
 };
 
 // Energy slots of the device energy accumulator (fixed point).
@@ -60,6 +61,26 @@ struct Box {
 };
 
 Box make_box(const double* box9);
+
+// Recomputes the reciprocal rows, volume and triclinic flag of a box from its
+// vectors (pme.rs:176-199); used where a barostat rescales a box on the device.
+__host__ __device__ inline void box_refresh(Box& b) {
+    const double* a = b.v;
+    const double* bb = b.v + 3;
+    const double* c = b.v + 6;
+    const double bc[3] = {bb[1] * c[2] - bb[2] * c[1], bb[2] * c[0] - bb[0] * c[2], bb[0] * c[1] - bb[1] * c[0]};
+    const double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    const double ab[3] = {a[1] * bb[2] - a[2] * bb[1], a[2] * bb[0] - a[0] * bb[2], a[0] * bb[1] - a[1] * bb[0]};
+    const double vol = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
+    for (int i = 0; i < 3; ++i) {
+        b.recip[i] = bc[i] / vol;
+        b.recip[3 + i] = ca[i] / vol;
+        b.recip[6 + i] = ab[i] / vol;
+    }
+    b.volume = vol;
+    b.periodic = 1;
+    b.triclinic = (b.v[3] != 0.0 || b.v[6] != 0.0 || b.v[7] != 0.0) ? 1 : 0;
+}
 
 constexpr int MAX_STAGES = 12;  // FFT stages per axis (radices, 0-terminated)
 

@@ -20,6 +20,32 @@ struct BondedDev {
     DevBuf<double> eqq, esig, eeps, blen, bk, ath, ak;
 };
 
+// Monte Carlo barostat state of one system (barostat.rs:42-57, 129-195) plus
+// the record of its last attempt.
+struct BaroState {
+    double volume_scale;                      // maximum |dV| of a proposal
+    long long attempted, accepted;            // since the last step-size adaptation
+    long long total_attempted, total_accepted;
+    double u_volume, u_accept;                // the attempt's two uniforms
+    double v_old, v_new, e_old, e_new, work;  // last attempt
+    int accepted_last, skipped;
+};
+
+struct BaroDev {
+    bool enabled = false;
+    int frequency = 0, phase = 0;  // phase: steps since the last attempt slot
+    double pressure = 0.0, temperature = 0.0;
+    uint64_t seed = 0;
+    int nmol = 0;
+    DevBuf<int> mol_start, mol_atoms, mol_sys, nmol_sys;
+    DevBuf<BaroState> st, snap_st;
+    DevBuf<Box> saved_box;
+    DevBuf<int> rejected;
+    DevBuf<double> scale, pos_saved;
+    DevBuf<long long> force_fx, e_old_fx, e_new_fx;
+    DevBuf<double> force_d, e_old_d, e_new_d;
+};
+
 class MdContext {
 public:
     // Replicas share the nonbonded method/cutoff/tolerance and the integrator
@@ -37,6 +63,14 @@ public:
     // Steps per captured HIP graph in the mixed form (0: launch every step
     // eagerly; the same kernels run either way).
     void set_graph_steps(int steps);
+    // MonteCarloBarostat (barostat.rs:59-218) on every replica: isotropic
+    // moves every `frequency` steps (0 removes it), molecules scaled as rigid
+    // bodies, replica r drawing from seed + r.  alpha and the PME grid stay
+    // as initialised, as in the reference.
+    void set_barostat(double pressure, double temperature, int frequency, uint64_t seed);
+    // One barostat attempt now, outside the step sequence (test hook).
+    void barostat_attempt();
+    BaroState barostat_state(int r);
     void get_state(int r, double* pos, double* vel, double* forces, double* energy, int64_t* step);
     // Potential energy of every replica (nsys values).
     void energies(double* out);
@@ -56,8 +90,12 @@ public:
 
 private:
     void enqueue_forces(hipStream_t s, bool energy, int rebuild);
-    void enqueue_step(hipStream_t s, int rebuild);
-    void launch_bonded(hipStream_t s);
+    void enqueue_forces_into(hipStream_t s, bool energy, int rebuild, long long* force_fx, double* force_d,
+                             long long* energy_fx, double* energy_d);
+    void enqueue_step(hipStream_t s, int rebuild, bool barostat);
+    void enqueue_barostat(hipStream_t s);
+    void launch_bonded(hipStream_t s, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d);
+    int graph_length() const { return baro_.enabled ? baro_.frequency : graph_steps_; }
     void build_bonded(const std::vector<int>& e1, const std::vector<int>& e2, const std::vector<double>& eqq,
                       const std::vector<double>& esig, const std::vector<double>& eeps, const std::vector<int>& b1,
                       const std::vector<int>& b2, const std::vector<double>& blen, const std::vector<double>& bk,
@@ -87,6 +125,9 @@ private:
     int graph_steps_ = 10;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
     BondedDev bonded_;
+    BaroDev baro_;
+    std::vector<int> bond1_h_, bond2_h_;  // bonds in global indices (molecules)
+    DevBuf<SysInfo> snap_sys_;
     DevBuf<long long> force_fx_, energy_fx_;
     DevBuf<double> force_d_, energy_d_;
     DevBuf<unsigned> status_;

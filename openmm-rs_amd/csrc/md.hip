@@ -200,6 +200,146 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
     wave_raise_flags(flag, moved, si);
 }
 
+// Monte Carlo barostat (barostat.rs:129-195, context/program.rs:5-45,
+// context.rs:976-1029), one thread per system or molecule.
+//
+// Proposal: V = a_x b_y c_z, dV = volume_scale * 2 (u - 1/2), V' = V + dV; an
+// impossible V' counts as a rejected attempt without a move.  The box is
+// scaled by s = (V'/V)^(1/3) and every molecule is translated by (s - 1) times
+// its unweighted centroid.  The two uniforms come from Philox keyed by the
+// barostat seed + replica index, counter = attempt number, so a replica draws
+// the same numbers alone or in a batch.
+__device__ __forceinline__ void baro_record(BaroState& B, bool accepted, double previous_volume) {
+    // barostat.rs:180-195
+    B.attempted += 1;
+    B.accepted += accepted ? 1 : 0;
+    if (B.attempted >= 10) {
+        const double fraction = (double)B.accepted / (double)B.attempted;
+        if (fraction < 0.25) {
+            B.volume_scale /= 1.1;
+        } else if (fraction > 0.75) {
+            B.volume_scale = fmin(B.volume_scale * 1.1, previous_volume * 0.3);
+        } else {
+            return;
+        }
+        B.attempted = 0;
+        B.accepted = 0;
+    }
+}
+
+__global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved_box, double* scale, uint64_t seed,
+                               double cutoff, unsigned* status) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nsys) return;
+    BaroState& B = st[r];
+    Box& box = sys[r].box;
+    saved_box[r] = box;
+    const uint64_t key = seed + (uint64_t)r;
+    uint32_t c[4] = {(uint32_t)B.total_attempted, (uint32_t)((uint64_t)B.total_attempted >> 32), 0u, 0x42415230u};
+    philox4x32_10(c, (uint32_t)key, (uint32_t)(key >> 32));
+    B.u_volume = (double)c[0] * (1.0 / 4294967296.0);
+    B.u_accept = (double)c[1] * (1.0 / 4294967296.0);
+    B.total_attempted += 1;
+    const double v_old = box.v[0] * box.v[4] * box.v[8];
+    const double v_new = v_old + B.volume_scale * 2.0 * (B.u_volume - 0.5);
+    B.v_old = v_old;
+    B.v_new = v_new;
+    B.accepted_last = 0;
+    if (!(isfinite(v_new) && v_new > 0.0)) {
+        B.skipped = 1;
+        scale[r] = 1.0;
+        baro_record(B, false, v_old);
+        return;
+    }
+    B.skipped = 0;
+    const double sc = cbrt(v_new / v_old);
+    // The reference asserts 2 rc <= box edge on every evaluation
+    // (nonbonded.rs:806-812); a move that would break it is an error.
+    if (!(2.0 * cutoff <= box.v[0] * sc && 2.0 * cutoff <= box.v[4] * sc && 2.0 * cutoff <= box.v[8] * sc)) {
+        atomicOr(status, ST_CUTOFF_BOX);
+        B.skipped = 1;
+        scale[r] = 1.0;
+        return;
+    }
+    scale[r] = sc;
+    for (int k = 0; k < 9; ++k) box.v[k] *= sc;
+    box_refresh(box);
+}
+
+__global__ void k_baro_scale(int nmol, const int* __restrict__ mol_start, const int* __restrict__ mol_atoms,
+                             const int* __restrict__ mol_sys, const double* __restrict__ scale, double* pos,
+                             double* pos_saved) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const int lo = mol_start[m], hi = mol_start[m + 1];
+    double center[3] = {0.0, 0.0, 0.0};
+    for (int k = lo; k < hi; ++k) {
+        const int a = mol_atoms[k];
+        for (int d = 0; d < 3; ++d) {
+            pos_saved[3 * a + d] = pos[3 * a + d];
+            center[d] += pos[3 * a + d];
+        }
+    }
+    const double sc = scale[mol_sys[m]];
+    if (sc == 1.0) return;
+    const double inv = 1.0 / (double)(hi - lo);
+    double offset[3];
+    for (int d = 0; d < 3; ++d) offset[d] = (sc - 1.0) * (center[d] * inv);
+    for (int k = lo; k < hi; ++k) {
+        const int a = mol_atoms[k];
+        for (int d = 0; d < 3; ++d) pos[3 * a + d] += offset[d];
+    }
+}
+
+inline __device__ double energy_value(long long e) { return (double)e * (1.0 / ENERGY_SCALE); }
+inline __device__ double energy_value(double e) { return e; }
+
+// Potential energy of one system from its energy slots plus the volume terms.
+template <class A>
+__device__ double system_energy(const A* e, const SysInfo& S, double volume) {
+    return energy_value(e[E_DIRECT]) + energy_value(e[E_RECIP]) + S.self_const +
+           (S.neutral_coeff + S.disp_coeff) / volume + energy_value(e[E_EXCEPT]) + energy_value(e[E_BOND]) +
+           energy_value(e[E_ANGLE]);
+}
+
+template <class A>
+__global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* __restrict__ saved_box, int* rejected,
+                              const A* __restrict__ e_old, const A* __restrict__ e_new,
+                              const int* __restrict__ nmol_sys, double pressure, double kt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nsys) return;
+    BaroState& B = st[r];
+    if (B.skipped) {
+        rejected[r] = 0;  // nothing moved
+        return;
+    }
+    SysInfo& S = sys[r];
+    const double eo = system_energy(e_old + (size_t)r * E_SLOTS, S, saved_box[r].volume);
+    const double en = system_energy(e_new + (size_t)r * E_SLOTS, S, S.box.volume);
+    // barostat.rs:160-178 (BAR_NM3_TO_KJ_PER_MOL = 0.0602214076)
+    const double work = (en - eo) + pressure * 0.0602214076 * (B.v_new - B.v_old) -
+                        (double)nmol_sys[r] * kt * log(B.v_new / B.v_old);
+    bool accepted;
+    if (isnan(work)) accepted = false;
+    else if (work <= 0.0) accepted = true;
+    else accepted = kt > 0.0 && B.u_accept <= exp(-work / kt);
+    B.e_old = eo;
+    B.e_new = en;
+    B.work = work;
+    B.accepted_last = accepted ? 1 : 0;
+    B.total_accepted += accepted ? 1 : 0;
+    baro_record(B, accepted, B.v_old);
+    if (!accepted) S.box = saved_box[r];
+    rejected[r] = accepted ? 0 : 1;
+}
+
+__global__ void k_baro_restore(int n, const int* __restrict__ atom_sys, const int* __restrict__ rejected, double* pos,
+                               const double* __restrict__ pos_saved) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !rejected[atom_sys[i]]) return;
+    for (int d = 0; d < 3; ++d) pos[3 * i + d] = pos_saved[3 * i + d];
+}
+
 }  // namespace
 
 MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
@@ -282,6 +422,8 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
             }
         }
     build_bonded(e1, e2, eqq, esig, eeps, b1, b2, blen, bk, a1, a2, a3, ath, ak);
+    bond1_h_ = b1;
+    bond2_h_ = b2;
     dt_ = descs[0].step_size;
     temperature_ = descs[0].temperature;
     friction_ = descs[0].friction;
@@ -305,6 +447,7 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     snap_pos_.alloc(3 * (size_t)n_);
     snap_vel_.alloc(3 * (size_t)n_);
     snap_step_.alloc(1);
+    snap_sys_.alloc(nsys);
     clear_accumulators(s_);
     OMM_HIP(hipStreamSynchronize(s_));
 }
@@ -475,7 +618,8 @@ void MdContext::build_bonded(const std::vector<int>& e1, const std::vector<int>&
     up_d(B.ak, ak, ia);
 }
 
-void MdContext::launch_bonded(hipStream_t s) {
+void MdContext::launch_bonded(hipStream_t s, long long* force_fx, double* force_d, long long* energy_fx,
+                              double* energy_d) {
     const BondedDev& B = bonded_;
     const int total = B.ngroups + (B.ne - B.ge) + (B.nb - B.gb) + (B.na - B.ga);
     if (total == 0) return;
@@ -488,16 +632,145 @@ void MdContext::launch_bonded(hipStream_t s) {
                            nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->has_pme() ? 1 : 0, nb_->alpha(),
                            pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force, energy, status_.p);
     };
-    if (mixed_) go(force_fx_.p, energy_fx_.p);
-    else go(force_d_.p, energy_d_.p);
+    if (mixed_) go(force_fx, energy_fx);
+    else go(force_d, energy_d);
     OMM_HIP(hipGetLastError());
 }
 
 void MdContext::enqueue_forces(hipStream_t s, bool energy, int rebuild) {
+    enqueue_forces_into(s, energy, rebuild, force_fx_.p, force_d_.p, energy_fx_.p, energy_d_.p);
+}
+
+void MdContext::enqueue_forces_into(hipStream_t s, bool energy, int rebuild, long long* force_fx, double* force_d,
+                                    long long* energy_fx, double* energy_d) {
     const NbSettings& c = nb_->settings();
-    nb_->enqueue(pos_.p, c.include_direct, true, energy, rebuild, false, force_fx_.p, force_d_.p, energy_fx_.p,
-                 energy_d_.p, status_.p, s);
-    launch_bonded(s);
+    nb_->enqueue(pos_.p, c.include_direct, true, energy, rebuild, false, force_fx, force_d, energy_fx, energy_d,
+                 status_.p, s);
+    launch_bonded(s, force_fx, force_d, energy_fx, energy_d);
+}
+
+// One barostat attempt for every replica (context/program.rs:14-41): the
+// energy before the move, the proposal and rigid scaling, the energy after
+// (FP64 energy path, list rebuilt for the new boxes), the Metropolis test,
+// and the restore of rejected replicas.  The list is rebuilt afterwards.
+void MdContext::enqueue_barostat(hipStream_t s) {
+    BaroDev& B = baro_;
+    const size_t n3 = 3 * (size_t)n_, es = (size_t)E_SLOTS * nsys_;
+    const int sb = (nsys_ + 63) / 64;
+    if (mixed_) {
+        fill_async<long long>(B.force_fx.p, n3, 0ll, s);
+        fill_async<long long>(B.e_old_fx.p, es, 0ll, s);
+        fill_async<long long>(B.e_new_fx.p, es, 0ll, s);
+    } else {
+        fill_async<double>(B.force_d.p, n3, 0.0, s);
+        fill_async<double>(B.e_old_d.p, es, 0.0, s);
+        fill_async<double>(B.e_new_d.p, es, 0.0, s);
+    }
+    enqueue_forces_into(s, true, NbOperator::REBUILD_FLAG, B.force_fx.p, B.force_d.p, B.e_old_fx.p, B.e_old_d.p);
+    hipLaunchKernelGGL(k_baro_propose, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p, B.saved_box.p,
+                       B.scale.p, B.seed, nb_->settings().cutoff, status_.p);
+    hipLaunchKernelGGL(k_baro_scale, dim3((B.nmol + 255) / 256), dim3(256), 0, s, B.nmol, B.mol_start.p,
+                       B.mol_atoms.p, B.mol_sys.p, B.scale.p, pos_.p, B.pos_saved.p);
+    nb_->force_rebuild(s);
+    enqueue_forces_into(s, true, NbOperator::REBUILD_FLAG, B.force_fx.p, B.force_d.p, B.e_new_fx.p, B.e_new_d.p);
+    const double kt = BOLTZMANN * B.temperature;
+    if (mixed_)
+        hipLaunchKernelGGL(k_baro_accept<long long>, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p,
+                           B.saved_box.p, B.rejected.p, B.e_old_fx.p, B.e_new_fx.p, B.nmol_sys.p, B.pressure, kt);
+    else
+        hipLaunchKernelGGL(k_baro_accept<double>, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p,
+                           B.saved_box.p, B.rejected.p, B.e_old_d.p, B.e_new_d.p, B.nmol_sys.p, B.pressure, kt);
+    hipLaunchKernelGGL(k_baro_restore, dim3((n_ + 255) / 256), dim3(256), 0, s, n_, nb_->atom_sys_dev(),
+                       B.rejected.p, pos_.p, B.pos_saved.p);
+    nb_->force_rebuild(s);
+    OMM_HIP(hipGetLastError());
+}
+
+void MdContext::set_barostat(double pressure, double temperature, int frequency, uint64_t seed) {
+    // barostat.rs:65-86, 205-218
+    if (!std::isfinite(pressure)) throw Error(OMM_ERR_INVALID, "pressure must be finite");
+    if (!(std::isfinite(temperature) && temperature >= 0.0))
+        throw Error(OMM_ERR_INVALID, "temperature must be nonnegative and finite");
+    if (frequency < 0) throw Error(OMM_ERR_INVALID, "barostat frequency must be nonnegative");
+    OMM_HIP(hipStreamSynchronize(s_));
+    drop_graph();
+    BaroDev& B = baro_;
+    B.enabled = frequency > 0;
+    B.frequency = frequency;
+    B.phase = 0;
+    B.pressure = pressure;
+    B.temperature = temperature;
+    B.seed = seed;
+    if (!B.enabled) return;
+    // Molecules: connected components over the bonds (system.rs:100-141).
+    std::vector<int> parent(n_);
+    for (int i = 0; i < n_; ++i) parent[i] = i;
+    auto root = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (size_t k = 0; k < bond1_h_.size(); ++k) {
+        const int a = root(bond1_h_[k]), b = root(bond2_h_[k]);
+        parent[std::max(a, b)] = std::min(a, b);
+    }
+    std::vector<std::vector<int>> groups(n_);
+    for (int i = 0; i < n_; ++i) groups[root(i)].push_back(i);
+    std::vector<int> start{0}, atoms, msys, per_sys(nsys_, 0);
+    std::vector<SysInfo> sys(nsys_);
+    OMM_HIP(hipMemcpyAsync(sys.data(), nb_->systems_dev(), nsys_ * sizeof(SysInfo), hipMemcpyDeviceToHost, s_));
+    OMM_HIP(hipStreamSynchronize(s_));
+    std::vector<int> atom_sys(n_);
+    for (int r = 0; r < nsys_; ++r)
+        for (int i = 0; i < sys[r].n; ++i) atom_sys[sys[r].atom_off + i] = r;
+    for (int i = 0; i < n_; ++i) {
+        if (groups[i].empty()) continue;
+        atoms.insert(atoms.end(), groups[i].begin(), groups[i].end());
+        start.push_back((int)atoms.size());
+        msys.push_back(atom_sys[i]);
+        ++per_sys[atom_sys[i]];
+    }
+    B.nmol = (int)msys.size();
+    B.mol_start.upload(start, s_);
+    B.mol_atoms.upload(atoms, s_);
+    B.mol_sys.upload(msys, s_);
+    B.nmol_sys.upload(per_sys, s_);
+    std::vector<BaroState> st(nsys_);
+    for (int r = 0; r < nsys_; ++r) {
+        st[r] = BaroState{};
+        st[r].volume_scale = 0.01 * sys[r].box.v[0] * sys[r].box.v[4] * sys[r].box.v[8];
+    }
+    B.st.upload(st, s_);
+    B.snap_st.alloc(nsys_);
+    B.saved_box.alloc(nsys_);
+    B.rejected.alloc(nsys_);
+    B.scale.alloc(nsys_);
+    B.pos_saved.alloc(3 * (size_t)n_);
+    const size_t es = (size_t)E_SLOTS * nsys_;
+    if (mixed_) {
+        B.force_fx.alloc(3 * (size_t)n_);
+        B.e_old_fx.alloc(es);
+        B.e_new_fx.alloc(es);
+    } else {
+        B.force_d.alloc(3 * (size_t)n_);
+        B.e_old_d.alloc(es);
+        B.e_new_d.alloc(es);
+    }
+}
+
+void MdContext::barostat_attempt() {
+    if (!baro_.enabled) throw Error(OMM_ERR_STATE, "no barostat is set");
+    prepare();
+    enqueue_barostat(s_);
+    check_status();
+}
+
+BaroState MdContext::barostat_state(int r) {
+    if (r < 0 || r >= nsys_) throw Error(OMM_ERR_INVALID, "replica index out of range");
+    if (!baro_.enabled) throw Error(OMM_ERR_STATE, "no barostat is set");
+    BaroState st;
+    OMM_HIP(hipMemcpyAsync(&st, baro_.st.p + r, sizeof(BaroState), hipMemcpyDeviceToHost, s_));
+    OMM_HIP(hipStreamSynchronize(s_));
+    return st;
 }
 
 void MdContext::clear_accumulators(hipStream_t s) {
@@ -514,7 +787,8 @@ void MdContext::clear_accumulators(hipStream_t s) {
 // One step.  Invariant: the force accumulator is zero on entry (the previous
 // integrator cleared it).  The integrator raises the list's rebuild flag when
 // an atom left the skin, and the next step's build kernels act on it.
-void MdContext::enqueue_step(hipStream_t s, int rebuild) {
+void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat) {
+    if (barostat) enqueue_barostat(s);
     const int eslots = E_SLOTS * nsys_;
     const int eb = std::min(64, (eslots + 255) / 256);
     if (mixed_)
@@ -553,10 +827,15 @@ void MdContext::check_status() {
     }
     if (st & ST_NONFINITE) throw Error(OMM_ERR_NONFINITE, "non-finite force or energy on the device");
     if (st & ST_OVERFLOW) throw Error(OMM_ERR_OVERFLOW, "fixed-point force/energy accumulator overflow");
+    if (st & ST_CUTOFF_BOX)
+        throw Error(OMM_ERR_CUTOFF_BOX, "a barostat move would make the box smaller than twice the cutoff");
 }
 
 void MdContext::restore_snapshot() {
     const size_t bytes = 3 * (size_t)n_ * sizeof(double);
+    OMM_HIP(hipMemcpyAsync(nb_->systems_dev(), snap_sys_.p, nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice, s_));
+    if (baro_.enabled)
+        OMM_HIP(hipMemcpyAsync(baro_.st.p, baro_.snap_st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToDevice, s_));
     OMM_HIP(hipMemcpyAsync(pos_.p, snap_pos_.p, bytes, hipMemcpyDeviceToDevice, s_));
     OMM_HIP(hipMemcpyAsync(vel_.p, snap_vel_.p, bytes, hipMemcpyDeviceToDevice, s_));
     OMM_HIP(hipMemcpyAsync(step_dev_.p, snap_step_.p, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s_));
@@ -577,7 +856,16 @@ void MdContext::capture_graph() {
         const size_t n3 = 3 * (size_t)n_;
         hipLaunchKernelGGL(k_copy2, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s_, pos_.p, snap_pos_.p, vel_.p,
                            snap_vel_.p, n3, step_dev_.p, snap_step_.p);
-        for (int k = 0; k < graph_steps_; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
+        OMM_HIP(hipMemcpyAsync(snap_sys_.p, nb_->systems_dev(), nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice,
+                               s_));
+        if (baro_.enabled)
+            OMM_HIP(hipMemcpyAsync(baro_.snap_st.p, baro_.st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToDevice,
+                                   s_));
+        // With a barostat the graph spans one period, the attempt opening its
+        // last step (the phase is aligned before the graph is replayed).
+        const int len = graph_length();
+        for (int k = 0; k < len; ++k)
+            enqueue_step(s_, NbOperator::REBUILD_FLAG, baro_.enabled && k == len - 1);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s_, &g);
@@ -611,10 +899,19 @@ void MdContext::set_integrator(double temperature, double friction, double step_
 void MdContext::step(int nsteps) {
     if (nsteps < 0) throw Error(OMM_ERR_INVALID, "step count must be nonnegative");
     prepare();
+    BaroDev& B = baro_;
+    // Attempt before the step that completes a period (barostat.rs:129-135).
+    auto due = [&]() { return B.enabled && B.phase == B.frequency - 1; };
+    auto advance = [&](int k) {
+        if (B.enabled) B.phase = (B.phase + k) % B.frequency;
+    };
+    const int len = graph_length();
+    const bool graphs = mixed_ && graph_steps_ > 0 && len > 0;
     int done = 0;
-    if (mixed_ && graph_steps_ > 0 && nsteps >= graph_steps_) {
-        if (!graph_exec_) capture_graph();
-        while (nsteps - done >= graph_steps_) {
+    while (done < nsteps) {
+        const bool aligned = !B.enabled || B.phase == 0;
+        if (graphs && aligned && nsteps - done >= len) {
+            if (!graph_exec_) capture_graph();
             OMM_HIP(hipGraphLaunch(graph_exec_, s_));
             unsigned st = read_status();
             for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
@@ -623,15 +920,19 @@ void MdContext::step(int nsteps) {
                 drop_graph();
                 restore_snapshot();
                 nb_->grow(st);
-                for (int k = 0; k < graph_steps_; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG);
+                for (int k = 0; k < len; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG, B.enabled && k == len - 1);
                 st = read_status();
                 if (!(st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW))) capture_graph();
             }
-            done += graph_steps_;
+            advance(len);
+            done += len;
+        } else {
+            // Plain FP64 form, alignment and remainders: one launch sequence per step.
+            enqueue_step(s_, NbOperator::REBUILD_FLAG, due());
+            advance(1);
+            ++done;
         }
     }
-    // Plain FP64 form, and the remainder: one launch sequence per step.
-    for (; done < nsteps; ++done) enqueue_step(s_, NbOperator::REBUILD_FLAG);
     step_count_ += nsteps;
     check_status();
 }
@@ -746,7 +1047,7 @@ void MdContext::profile(int reps, double ms[8], int64_t* num_pairs) {
         OMM_HIP(hipEventElapsedTime(&v, t0, t1));
         return (double)v / reps;
     };
-    ms[6] = timed([&] { launch_bonded(s_); });
+    ms[6] = timed([&] { launch_bonded(s_, force_fx_.p, force_d_.p, energy_fx_.p, energy_d_.p); });
     // The integrator is timed on scratch copies of the state.
     DevBuf<double> px, vx;
     DevBuf<int> scratch_flag;  // stands in for the list's nsys + 1 rebuild flags

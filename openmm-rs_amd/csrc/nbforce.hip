@@ -54,22 +54,9 @@ Box make_box(const double* v) {
         return b;
     }
     for (int i = 0; i < 9; ++i) b.v[i] = v[i];
-    const double* a = v;
-    const double* bb = v + 3;
-    const double* c = v + 6;
-    double bc[3] = {bb[1] * c[2] - bb[2] * c[1], bb[2] * c[0] - bb[0] * c[2], bb[0] * c[1] - bb[1] * c[0]};
-    double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
-    double ab[3] = {a[1] * bb[2] - a[2] * bb[1], a[2] * bb[0] - a[0] * bb[2], a[0] * bb[1] - a[1] * bb[0]};
-    double vol = a[0] * bc[0] + a[1] * bc[1] + a[2] * bc[2];
-    if (!(std::isfinite(vol) && vol > 0.0)) throw Error(OMM_ERR_INVALID, "periodic box volume must be positive and finite");
-    for (int i = 0; i < 3; ++i) {
-        b.recip[i] = bc[i] / vol;
-        b.recip[3 + i] = ca[i] / vol;
-        b.recip[6 + i] = ab[i] / vol;
-    }
-    b.volume = vol;
-    b.periodic = 1;
-    b.triclinic = (v[3] != 0.0 || v[6] != 0.0 || v[7] != 0.0) ? 1 : 0;
+    box_refresh(b);
+    if (!(std::isfinite(b.volume) && b.volume > 0.0))
+        throw Error(OMM_ERR_INVALID, "periodic box volume must be positive and finite");
     return b;
 }
 

@@ -51,6 +51,22 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+class BarostatInfo(C.Structure):
+    """ctypes image of ``omm_barostat_info``."""
+
+    _fields_ = [
+        ("volume_scale", C.c_double),
+        ("attempted", C.c_int64), ("accepted", C.c_int64),
+        ("total_attempted", C.c_int64), ("total_accepted", C.c_int64),
+        ("volume_old", C.c_double), ("volume_new", C.c_double),
+        ("energy_old", C.c_double), ("energy_new", C.c_double), ("work", C.c_double),
+        ("accepted_last", C.c_int32), ("skipped", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class NbDesc(C.Structure):
     """ctypes image of ``omm_nb_desc``."""
 
@@ -130,6 +146,8 @@ def library():
         lib.omm_ctx_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         lib.omm_batch_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
         lib.omm_batch_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        lib.omm_ctx_set_barostat.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_uint64]
+        lib.omm_batch_set_barostat.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_uint64]
         _lib = lib
     return _lib
 
@@ -537,6 +555,23 @@ class Context:
         """Steps per captured HIP graph (0: eager launches)."""
         _check(library().omm_ctx_set_graph_steps(self._h, int(steps)))
 
+    def set_barostat(self, pressure, temperature, frequency=25, seed=1):
+        """MonteCarloBarostat(pressure bar, temperature K, frequency) with rigid molecules."""
+        _check(library().omm_ctx_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
+
+    def barostat_info(self):
+        info = BarostatInfo()
+        _check(library().omm_ctx_barostat_info(self._h, C.byref(info)))
+        return info.as_dict()
+
+    def barostat_attempt(self):
+        _check(library().omm_ctx_barostat_attempt(self._h))
+
+    def box(self):
+        b = np.zeros(9)
+        _check(library().omm_ctx_get_box(self._h, b.ctypes.data_as(_dp)))
+        return b.reshape(3, 3)
+
     def list_stats(self):
         """(tiles, j-list entries) of the current neighbour list."""
         t, e = C.c_int64(), C.c_int64()
@@ -628,6 +663,17 @@ class Batch:
 
     def set_graph_steps(self, steps):
         _check(library().omm_batch_set_graph_steps(self._h, int(steps)))
+
+    def set_barostat(self, pressure, temperature, frequency=25, seed=1):
+        _check(library().omm_batch_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
+
+    def barostat_info(self, replica):
+        info = BarostatInfo()
+        _check(library().omm_batch_barostat_info(self._h, int(replica), C.byref(info)))
+        return info.as_dict()
+
+    def barostat_attempt(self):
+        _check(library().omm_batch_barostat_attempt(self._h))
 
     def get_state(self, replica):
         n = self.sizes[replica]
