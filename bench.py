@@ -16,7 +16,8 @@ single-threaded as the reference is) and the batched workload: 64
 independent water boxes of 500-6,000 molecules (log-uniform from a seed, box
 edge for 33.4 molecules/nm^3, each with its own PME grid) advanced by one set
 of launches per phase, in replica-steps/s, beside the same replicas run as 64
-separate plain FP64 contexts (per-system launches).
+separate plain FP64 contexts (per-system launches); and 128 replicas of
+300-4,000 molecules under NPT (Monte Carlo barostat at 1 bar every 25 steps).
 """
 import argparse
 import json
@@ -135,6 +136,38 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
     }
 
 
+def run_npt(steps, equilibrate, count=128, seed=8192):
+    """128 uneven replicas (300-4,000 molecules) under NPT: Langevin 1 fs at
+    300 K plus an isotropic Monte Carlo barostat at 1 bar every 25 steps."""
+    systems = batch_workload(count, seed, lo=300, hi=4000)
+    seeds = [seed + 100 + r for r in range(count)]
+    batch = ommhip.Batch(systems, temperature=300.0, friction=1.0, step_size=DT_PS, seeds=seeds,
+                         precision=ommhip.PRECISION_MIXED)
+    batch.set_graph_steps(GRAPH_STEPS)
+    batch.set_velocities_to_temperature(300.0, seed + 1)
+    equilibrate_then(batch, equilibrate, 50)  # NVT relaxation of the lattice start
+    batch.set_barostat(1.0, 300.0, 25, seed=seed + 7)
+    batch.step(100)
+    a0 = sum(batch.barostat_info(r)["total_attempted"] for r in range(count))
+    secs = timed_steps(batch, steps)
+    infos = [batch.barostat_info(r) for r in range(count)]
+    attempted = sum(i["total_attempted"] for i in infos) - a0
+    accepted = sum(i["total_accepted"] for i in infos)
+    vol = [float(np.prod(np.diag(batch.box(r)))) for r in range(count)]
+    vol0 = [float(np.prod(np.diag(s.box))) for s in systems]
+    return {
+        "replicas": count,
+        "atoms": int(sum(len(s.masses) for s in systems)),
+        "replica_steps_per_s": round(count * steps / secs, 2),
+        "ms_per_batch_step": round(secs / steps * 1e3, 4),
+        "steps": steps,
+        "barostat": {"pressure_bar": 1.0, "frequency": 25, "attempts_timed": attempted,
+                     "accepted_total": accepted, "attempted_total": sum(i["total_attempted"] for i in infos),
+                     "mean_volume_ratio": round(float(np.mean(np.array(vol) / np.array(vol0))), 5)},
+        "alpha_grid": "fixed at initialisation (reference behaviour)",
+    }
+
+
 def cpu_baseline(sample_steps):
     import oracle as orc
 
@@ -175,6 +208,9 @@ def main():
     ap.add_argument("--batch-equilibrate", type=int, default=1000)
     ap.add_argument("--batch-plain-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--npt-steps", type=int, default=250)
+    ap.add_argument("--npt-equilibrate", type=int, default=500)
+    ap.add_argument("--no-npt", action="store_true")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 64-replica workload and print its record (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -243,6 +279,8 @@ def main():
     }
     if not args.no_batch:
         line["batched"] = run_batch(args.batch_steps, 100, args.batch_equilibrate, args.batch_plain_steps)
+    if not args.no_npt:
+        line["batched_npt"] = run_npt(args.npt_steps, args.npt_equilibrate)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line), flush=True)

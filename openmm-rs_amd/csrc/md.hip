@@ -645,8 +645,9 @@ void MdContext::enqueue_forces_into(hipStream_t s, bool energy, int rebuild, lon
                                     long long* energy_fx, double* energy_d) {
     const NbSettings& c = nb_->settings();
     nb_->enqueue(pos_.p, c.include_direct, true, energy, rebuild, false, force_fx, force_d, energy_fx, energy_d,
-                 status_.p, s);
-    launch_bonded(s, force_fx, force_d, energy_fx, energy_d);
+                 status_.p, s, false);
+    launch_bonded(s, force_fx, force_d, energy_fx, energy_d);  // concurrent with the mesh
+    nb_->join(s);
 }
 
 // One barostat attempt for every replica (context/program.rs:14-41): the

@@ -77,9 +77,12 @@ public:
     // itself) and PME into the caller's accumulators (forces over all atoms,
     // energies E_SLOTS per system).  Graph-capturable.  energy: also
     // accumulate energies (slower, accurate path in the mixed form).
+    // The PME mesh runs on a side stream concurrently with the pairs; with
+    // join_now false the caller enqueues more work on s and then calls join(s).
     void enqueue(const double* pos_dev, bool direct, bool recip, bool energy, int rebuild, bool exceptions,
                  long long* force_fx, double* force_d, long long* energy_fx, double* energy_d, unsigned* status,
-                 hipStream_t s);
+                 hipStream_t s, bool join_now = true);
+    void join(hipStream_t s);
     // PME arithmetic for a force-only or an energy evaluation.
     int mesh_mode(bool energy) const;
 
@@ -136,6 +139,9 @@ private:
     mutable std::vector<char> disp_cached_;
     int pair_grid_ = 1;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;  // PME branch (default priority, like stream_)
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    bool joined_ = true;
 
     std::vector<SysInfo> sys_h_;
     DevBuf<SysInfo> sys_dev_;

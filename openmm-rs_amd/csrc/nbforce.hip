@@ -149,6 +149,9 @@ NbOperator::NbOperator(const omm_nb_desc* const* descs, int nsys, int precision)
 }
 
 NbOperator::~NbOperator() {
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -224,6 +227,9 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     OMM_HIP(hipGetDeviceCount(&count));
     if (count <= 0) throw Error(OMM_ERR_DEVICE, "no HIP device available");
     if (!stream_) OMM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!side_) OMM_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    if (!ev_fork_) OMM_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    if (!ev_join_) OMM_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     hipStream_t s = stream_;
     const int N = ntotal_;
     std::vector<float> qk(N), sh(N), e2(N);
@@ -466,7 +472,7 @@ int NbOperator::mesh_mode(bool energy) const {
 
 void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool energy, int rebuild, bool exceptions,
                          long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
-                         unsigned* status, hipStream_t s) {
+                         unsigned* status, hipStream_t s, bool join_now) {
     const NbSettings& c = cfg_[0];
     if (direct) {
         if (rebuild == REBUILD_CHECK) {
@@ -477,6 +483,20 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
         } else if (rebuild == REBUILD_FORCED) {
             nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
         }
+    }
+    // The mesh runs on the side stream, forked after the list build (the
+    // spread visits atoms in the list's sorted order) and concurrent with the
+    // direct-space pairs; join() makes s wait for it.
+    const bool mesh = recip && c.method == OMM_NB_PME;
+    if (mesh) {
+        OMM_HIP(hipEventRecord(ev_fork_, s));
+        OMM_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
+        pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
+                energy_d, status, side_);
+        OMM_HIP(hipEventRecord(ev_join_, side_));
+        joined_ = false;
+    }
+    if (direct) {
         launch_pairs(mixed_, energy, nl_, pp_, part_, pos_dev, sys_dev_.p, force_fx, force_d, energy_fx, energy_d,
                      status, nullptr, pair_grid_, s);
         if (exceptions)
@@ -484,9 +504,13 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
                               c.method == OMM_NB_PME ? 1 : 0, alpha_[0], force_fx, force_d, energy_fx, energy_d,
                               status, s);
     }
-    if (recip && c.method == OMM_NB_PME)
-        pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
-                energy_d, status, s);
+    if (join_now) join(s);
+}
+
+void NbOperator::join(hipStream_t s) {
+    if (joined_) return;
+    OMM_HIP(hipStreamWaitEvent(s, ev_join_, 0));
+    joined_ = true;
 }
 
 void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, double* force_d, long long* energy_fx,
