@@ -36,6 +36,10 @@ METRIC = "ns/day (23.5k-atom PME water) + batched replica-steps/s, mixed vs plai
 DT_PS = 0.001
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_PAIR = 75
+# HBM bytes per k_pairs launch on the 23,550-atom workload: rocprofv3 FETCH_SIZE
+# and WRITE_SIZE passes, each scaled by the same counter's reading for a plain
+# copy kernel of known size (profiles/k_pairs_md_step_water_23550_atoms.md).
+K_PAIRS_TRAFFIC_BYTES = 58.47e6
 
 
 def workload(seed=2024):
@@ -266,7 +270,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": K_PAIRS_TRAFFIC_BYTES,
             "kernel": "k_pairs (direct-space LJ + erfc-Ewald), FP32 VALU work: 75 FLOP x in-cutoff pairs per launch",
             "pairs_per_launch": npairs,
             "ms_per_launch": round(phases["direct_pairs"], 5),

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define OMMHIP_ABI_VERSION 1
+#define OMMHIP_ABI_VERSION 2
 
 typedef enum omm_status {
     OMM_OK = 0,
@@ -99,6 +99,22 @@ typedef struct omm_nb_desc {
     int32_t force_group;                 /* 0..31, default 0 */
     int32_t reciprocal_force_group;      /* -1..31, default -1 (= force_group) */
     int32_t include_direct_space;        /* default 1 */
+    /* Global parameters and parameter offsets (nonbonded.rs:247-430).  The
+     * parameters used are base + value(parameter) * scale summed over the
+     * offsets of each particle / exception, a later offset of the same
+     * (parameter, target) replacing an earlier one (nonbonded.rs:395-430).
+     * All counts default to 0. */
+    int32_t num_global_parameters;
+    const char* const* global_parameter_names;   /* non-empty, distinct */
+    const double* global_parameter_defaults;     /* finite */
+    int32_t num_particle_offsets;
+    const int32_t* particle_offset_parameter;    /* index into the global parameters */
+    const int32_t* particle_offset_index;        /* particle */
+    const double* particle_offset_scales;        /* [n][3]: charge, sigma, epsilon */
+    int32_t num_exception_offsets;
+    const int32_t* exception_offset_parameter;
+    const int32_t* exception_offset_index;       /* exception */
+    const double* exception_offset_scales;       /* [n][3]: charge product, sigma, epsilon */
 } omm_nb_desc;
 
 /* Fill *desc with the reference defaults of NonbondedForce::new (nonbonded.rs:129-162);
@@ -170,6 +186,21 @@ omm_status omm_nb_pair_stats(omm_nb* nb, const double* pos, const double* box,
 uint64_t omm_pair_key(int32_t i, int32_t j);
 
 /* Frees everything owned by the handle. NULL is accepted. */
+/* Force::set_global_parameter (nonbonded.rs:1113-1123): *recognised = 1 when
+ * the force has the parameter (then its value is set and the effective
+ * parameters follow), 0 otherwise. */
+omm_status omm_nb_set_global_parameter(omm_nb* nb, const char* name, double value, int32_t* recognised);
+
+/* Current value of a global parameter (*recognised as above). */
+omm_status omm_nb_get_global_parameter(omm_nb* nb, const char* name, double* value, int32_t* recognised);
+
+/* NonbondedForce::update_parameters_from (nonbonded.rs:721-767): new particle
+ * and exception parameters, offsets and global parameter defaults from desc,
+ * under the reference's rules (same counts, same exception pairs, same
+ * offset targets, excluded exceptions stay excluded).  Current global
+ * parameter values are kept. */
+omm_status omm_nb_update_parameters(omm_nb* nb, const omm_nb_desc* desc);
+
 omm_status omm_nb_destroy(omm_nb* nb);
 
 /* ------------------------------------------------------------------------ */
@@ -223,6 +254,10 @@ omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperatur
  * (integrators/langevin.rs:82-101) on the context's integrator; same checks
  * (temperature and friction nonnegative and finite, step size positive). */
 omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double friction, double step_size);
+
+/* Context::set_parameter (context.rs:251-261) for the NonbondedForce's global
+ * parameters: *recognised = 1 when the force has the parameter. */
+omm_status omm_ctx_set_parameter(omm_ctx* ctx, const char* name, double value, int32_t* recognised);
 
 /* Context::step (context.rs:795): nsteps LangevinMiddle steps.  The mixed
  * form replays a captured HIP graph of 10 steps; each step's integrator

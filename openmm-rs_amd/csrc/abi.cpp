@@ -155,6 +155,35 @@ omm_status omm_nb_pair_stats(omm_nb* nb, const double* pos, const double* box, i
     });
 }
 
+omm_status omm_nb_set_global_parameter(omm_nb* nb, const char* name, double value, int32_t* recognised) {
+    return guarded([&]() -> omm_status {
+        if (!nb || !name) throw Error(OMM_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        const bool found = nb->op.set_global_parameter(name, value);
+        if (recognised) *recognised = found ? 1 : 0;
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_get_global_parameter(omm_nb* nb, const char* name, double* value, int32_t* recognised) {
+    return guarded([&]() -> omm_status {
+        if (!nb || !name) throw Error(OMM_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        const bool found = nb->op.get_global_parameter(name, value);
+        if (recognised) *recognised = found ? 1 : 0;
+        return OMM_OK;
+    });
+}
+
+omm_status omm_nb_update_parameters(omm_nb* nb, const omm_nb_desc* desc) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        nb->op.update_parameters(desc);
+        return OMM_OK;
+    });
+}
+
 omm_status omm_nb_destroy(omm_nb* nb) {
     return guarded([&]() -> omm_status {
         delete nb;
@@ -198,6 +227,11 @@ omm_status omm_ctx_set_integrator(omm_ctx* ctx, double temperature, double frict
 }
 
 omm_status omm_ctx_set_graph_steps(omm_ctx* ctx, int32_t steps) { CTX_CALL(ctx->ctx.set_graph_steps(steps)); }
+
+omm_status omm_ctx_set_parameter(omm_ctx* ctx, const char* name, double value, int32_t* recognised) {
+    CTX_CALL(if (!name) throw Error(OMM_ERR_INVALID, "null parameter name");
+             const bool found = ctx->ctx.set_parameter(name, value); if (recognised) *recognised = found ? 1 : 0);
+}
 
 static void fill_info(const ommhip::BaroState& st, omm_barostat_info* info) {
     info->volume_scale = st.volume_scale;

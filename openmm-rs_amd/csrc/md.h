@@ -63,6 +63,9 @@ public:
     // Steps per captured HIP graph in the mixed form (0: launch every step
     // eagerly; the same kernels run either way).
     void set_graph_steps(int steps);
+    // Context::set_parameter (context.rs:251-261) for the NonbondedForce's
+    // global parameters; returns whether the force has the parameter.
+    bool set_parameter(const std::string& name, double value);
     // MonteCarloBarostat (barostat.rs:59-218) on every replica: isotropic
     // moves every `frequency` steps (0 removes it), molecules scaled as rigid
     // bodies, replica r drawing from seed + r.  alpha and the PME grid stay
@@ -125,6 +128,7 @@ private:
     int graph_steps_ = 10;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
     BondedDev bonded_;
+    std::vector<int> bonded_exc_order_;  // exception k of the bonded arrays = global exception order[k]
     BaroDev baro_;
     std::vector<int> bond1_h_, bond2_h_;  // bonds in global indices (molecules)
     DevBuf<SysInfo> snap_sys_;

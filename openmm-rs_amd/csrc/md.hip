@@ -602,6 +602,7 @@ void MdContext::build_bonded(const std::vector<int>& e1, const std::vector<int>&
         if (p.empty()) p.push_back(0.0);
         d.upload(p, s_);
     };
+    bonded_exc_order_ = ie;
     up_i(B.e1, e1, ie);
     up_i(B.e2, e2, ie);
     up_d(B.eqq, eqq, ie);
@@ -756,6 +757,32 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
         B.e_old_d.alloc(es);
         B.e_new_d.alloc(es);
     }
+}
+
+bool MdContext::set_parameter(const std::string& name, double value) {
+    OMM_HIP(hipStreamSynchronize(s_));
+    bool changed = false;
+    const bool found = nb_->set_global_parameter(name, value, &changed);
+    if (changed && nb_->settings().include_direct) {
+        // The bonded kernel keeps its own (grouped) copy of the exception parameters.
+        std::vector<double> qq, sig, eps;
+        for (int r = 0; r < nsys_; ++r) {
+            const NbSettings& c = nb_->settings(r);
+            qq.insert(qq.end(), c.exc_qq.begin(), c.exc_qq.end());
+            sig.insert(sig.end(), c.exc_sig.begin(), c.exc_sig.end());
+            eps.insert(eps.end(), c.exc_eps.begin(), c.exc_eps.end());
+        }
+        auto permuted = [&](const std::vector<double>& v) {
+            std::vector<double> out(bonded_exc_order_.size());
+            for (size_t k = 0; k < out.size(); ++k) out[k] = v[bonded_exc_order_[k]];
+            if (out.empty()) out.push_back(0.0);
+            return out;
+        };
+        bonded_.eqq.upload(permuted(qq), s_);
+        bonded_.esig.upload(permuted(sig), s_);
+        bonded_.eeps.upload(permuted(eps), s_);
+    }
+    return found;
 }
 
 void MdContext::barostat_attempt() {

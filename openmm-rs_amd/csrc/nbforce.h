@@ -9,10 +9,23 @@
 
 namespace ommhip {
 
+struct ParamOffset {
+    int parameter = 0, index = 0;
+    double scale[3] = {0, 0, 0};  // charge (product), sigma, epsilon
+};
+
 struct NbSettings {
+    // Effective parameters (base + offsets at the current global values).
     std::vector<double> charge, sigma, epsilon;
     std::vector<int> exc1, exc2;
     std::vector<double> exc_qq, exc_sig, exc_eps;
+    // Base parameters, global parameters and offsets (nonbonded.rs:247-430).
+    std::vector<double> base_charge, base_sigma, base_epsilon, base_qq, base_sig, base_eps;
+    std::vector<std::string> gp_names;
+    std::vector<double> gp_defaults, gp_values;
+    std::vector<ParamOffset> particle_offsets, exception_offsets;
+    // Recomputes the effective parameters (nonbonded.rs:395-430).
+    void apply_offsets();
     int method = OMM_NB_NO_CUTOFF;
     double cutoff = 1.0, rf_dielectric = 78.3, tolerance = 5e-4;
     double pme_alpha = 0.0;
@@ -45,6 +58,15 @@ public:
     // parameters, sizes and boxes are per system.
     NbOperator(const omm_nb_desc* const* descs, int nsys, int precision);
     ~NbOperator();
+
+    // Force::set_global_parameter (nonbonded.rs:1113-1123) on every system
+    // that has the parameter; returns whether any has it.  Changed values
+    // refresh the device parameters (and *changed is set).
+    bool set_global_parameter(const std::string& name, double value, bool* changed = nullptr);
+    bool get_global_parameter(const std::string& name, double* value) const;
+    // NonbondedForce::update_parameters_from (nonbonded.rs:721-767) of a
+    // one-system operator.
+    void update_parameters(const omm_nb_desc* desc);
 
     // Force::initialize (nonbonded.rs:1055-1107) of every system.
     void initialize(const int* n, const double* const* boxes);
@@ -124,6 +146,10 @@ public:
     std::mutex mutex;
 
 private:
+    // Uploads the effective particle and exception parameters and the
+    // parameter-dependent energy constants of every system.
+    void refresh_parameters();
+    void upload_parameters(hipStream_t s);
     void check_box(const double* box) const;
     double dispersion_coefficient(int s) const;
 

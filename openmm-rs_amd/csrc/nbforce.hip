@@ -118,7 +118,80 @@ NbSettings settings_from_desc(const omm_nb_desc* d) {
     c.force_group = d->force_group;
     c.reciprocal_group = d->reciprocal_force_group;
     c.include_direct = d->include_direct_space != 0;
+    // Global parameters and offsets (nonbonded.rs:247-303).
+    if (d->num_global_parameters < 0 || d->num_particle_offsets < 0 || d->num_exception_offsets < 0)
+        throw Error(OMM_ERR_INVALID, "negative parameter or offset count");
+    if (d->num_global_parameters > 0 && (!d->global_parameter_names || !d->global_parameter_defaults))
+        throw Error(OMM_ERR_INVALID, "global parameter arrays must not be null");
+    for (int k = 0; k < d->num_global_parameters; ++k) {
+        const char* name = d->global_parameter_names[k];
+        const double v = d->global_parameter_defaults[k];
+        if (!name || !*name || !std::isfinite(v))
+            throw Error(OMM_ERR_INVALID, "global parameters require a name and a finite default value");
+        for (const std::string& other : c.gp_names)
+            if (other == name) throw Error(OMM_ERR_INVALID, "duplicate global parameter name");
+        c.gp_names.emplace_back(name);
+        c.gp_defaults.push_back(v);
+        c.gp_values.push_back(v);
+    }
+    auto offsets = [&](int count, const int32_t* param, const int32_t* index, const double* scales, int targets,
+                       std::vector<ParamOffset>& out) {
+        if (count > 0 && (!param || !index || !scales)) throw Error(OMM_ERR_INVALID, "offset arrays must not be null");
+        for (int k = 0; k < count; ++k) {
+            ParamOffset o;
+            o.parameter = param[k];
+            o.index = index[k];
+            if (o.parameter < 0 || o.parameter >= (int)c.gp_names.size())
+                throw Error(OMM_ERR_INVALID, "unknown offset parameter");
+            if (o.index < 0 || o.index >= targets) throw Error(OMM_ERR_INVALID, "offset index out of range");
+            for (int a = 0; a < 3; ++a) {
+                o.scale[a] = scales[3 * k + a];
+                if (!std::isfinite(o.scale[a])) throw Error(OMM_ERR_INVALID, "offset scales must be finite");
+            }
+            out.push_back(o);
+        }
+    };
+    offsets(d->num_particle_offsets, d->particle_offset_parameter, d->particle_offset_index,
+            d->particle_offset_scales, d->num_particles, c.particle_offsets);
+    offsets(d->num_exception_offsets, d->exception_offset_parameter, d->exception_offset_index,
+            d->exception_offset_scales, d->num_exceptions, c.exception_offsets);
+    c.base_charge = c.charge;
+    c.base_sigma = c.sigma;
+    c.base_epsilon = c.epsilon;
+    c.base_qq = c.exc_qq;
+    c.base_sig = c.exc_sig;
+    c.base_eps = c.exc_eps;
+    c.apply_offsets();
     return c;
+}
+
+void NbSettings::apply_offsets() {
+    // nonbonded.rs:395-430: offsets applied last-first, the first seen of each
+    // (parameter, target) wins, i.e. a later offset replaces an earlier one.
+    charge = base_charge;
+    sigma = base_sigma;
+    epsilon = base_epsilon;
+    exc_qq = base_qq;
+    exc_sig = base_sig;
+    exc_eps = base_eps;
+    std::set<std::pair<int, int>> seen;
+    for (size_t k = particle_offsets.size(); k-- > 0;) {
+        const ParamOffset& o = particle_offsets[k];
+        if (!seen.insert({o.parameter, o.index}).second) continue;
+        const double v = gp_values[o.parameter];
+        charge[o.index] += v * o.scale[0];
+        sigma[o.index] += v * o.scale[1];
+        epsilon[o.index] += v * o.scale[2];
+    }
+    seen.clear();
+    for (size_t k = exception_offsets.size(); k-- > 0;) {
+        const ParamOffset& o = exception_offsets[k];
+        if (!seen.insert({o.parameter, o.index}).second) continue;
+        const double v = gp_values[o.parameter];
+        exc_qq[o.index] += v * o.scale[0];
+        exc_sig[o.index] += v * o.scale[1];
+        exc_eps[o.index] += v * o.scale[2];
+    }
 }
 
 NbOperator::NbOperator(const omm_nb_desc* const* descs, int nsys, int precision) {
@@ -208,10 +281,6 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
                 }
             }
         }
-        sum_q2_[si] = 0.0;
-        sum_q_[si] = 0.0;
-        for (double q : c.charge) sum_q2_[si] += q * q;
-        for (double q : c.charge) sum_q_[si] += q;
         n_[si] = n;
         ntotal_ += n;
     }
@@ -232,45 +301,22 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     if (!ev_join_) OMM_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     hipStream_t s = stream_;
     const int N = ntotal_;
-    std::vector<float> qk(N), sh(N), e2(N);
-    std::vector<double> charge(N), qkd(N), shd(N), e2d(N);
     std::vector<int> atom_sys(N);
     std::vector<int> p1, p2;
-    std::vector<double> eqq, esig, eeps;
-    const double sk = std::sqrt(ONE_4PI_EPS0);
     sys_h_.assign(nsys, SysInfo{});
     std::vector<NlistSystem> nls(nsys);
     int off = 0;
     for (int si = 0; si < nsys; ++si) {
         const NbSettings& c = cfg_[si];
-        for (int i = 0; i < n_[si]; ++i) {
-            const int g = off + i;
-            charge[g] = c.charge[i];
-            qkd[g] = c.charge[i] * sk;
-            shd[g] = 0.5 * c.sigma[i];
-            e2d[g] = 2.0 * std::sqrt(c.epsilon[i]);
-            qk[g] = (float)qkd[g];
-            sh[g] = (float)shd[g];
-            e2[g] = (float)e2d[g];
-            atom_sys[g] = si;
-        }
+        for (int i = 0; i < n_[si]; ++i) atom_sys[off + i] = si;
         for (size_t e = 0; e < c.exc1.size(); ++e) {
             p1.push_back(off + c.exc1[e]);
             p2.push_back(off + c.exc2[e]);
-            eqq.push_back(c.exc_qq[e]);
-            esig.push_back(c.exc_sig[e]);
-            eeps.push_back(c.exc_eps[e]);
         }
         SysInfo& S = sys_h_[si];
         S.box = make_box(boxes[si]);
         S.alpha = alpha_[si];
         for (int a = 0; a < 3; ++a) S.grid[a] = grid_[si][a];
-        if (c.method == OMM_NB_PME) {
-            S.self_const = -ONE_4PI_EPS0 * alpha_[si] / std::sqrt(PI) * sum_q2_[si];
-            S.neutral_coeff = -ONE_4PI_EPS0 * PI * sum_q_[si] * sum_q_[si] / (2.0 * alpha_[si] * alpha_[si]);
-        }
-        if (c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
-            S.disp_coeff = dispersion_coefficient(si);
         nls[si].n = n_[si];
         nls[si].exc1 = &c.exc1;
         nls[si].exc2 = &c.exc2;
@@ -278,21 +324,10 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
         off += n_[si];
     }
     part_.atom_sys.upload(atom_sys, s);
-    part_.charge.upload(charge, s);
-    if (mixed_) {
-        part_.qk.upload(qk, s);
-        part_.sh.upload(sh, s);
-        part_.e2.upload(e2, s);
-    }
-    part_.qk_d.upload(qkd, s);
-    part_.sh_d.upload(shd, s);
-    part_.e2_d.upload(e2d, s);
     exc_.count = (int)p1.size();
     exc_.p1.upload(p1, s);
     exc_.p2.upload(p2, s);
-    exc_.qq.upload(eqq, s);
-    exc_.sig.upload(esig, s);
-    exc_.eps.upload(eeps, s);
+    upload_parameters(s);
 
     const NbSettings& c0 = cfg_[0];
     nlist_setup(nl_, nls, sys_h_, c0.cutoff, 0.1, s);
@@ -340,6 +375,157 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
 }
 
 void NbOperator::upload_systems(hipStream_t s) { sys_dev_.upload(sys_h_, s); }
+
+// Effective per-particle and exception parameters of every system in the
+// kernels' form, the charge sums and the energy constants of sys_h_.
+void NbOperator::upload_parameters(hipStream_t s) {
+    const int nsys = (int)cfg_.size();
+    const int N = ntotal_;
+    std::vector<float> qk(N), sh(N), e2(N);
+    std::vector<double> charge(N), qkd(N), shd(N), e2d(N);
+    std::vector<double> eqq, esig, eeps;
+    const double sk = std::sqrt(ONE_4PI_EPS0);
+    int off = 0;
+    for (int si = 0; si < nsys; ++si) {
+        const NbSettings& c = cfg_[si];
+        sum_q2_[si] = 0.0;
+        sum_q_[si] = 0.0;
+        for (int i = 0; i < n_[si]; ++i) {
+            if (!(std::isfinite(c.charge[i]) && c.sigma[i] >= 0.0 && c.epsilon[i] >= 0.0 && std::isfinite(c.sigma[i]) &&
+                  std::isfinite(c.epsilon[i])))
+                throw Error(OMM_ERR_INVALID, "effective particle parameters must be finite with nonnegative sigma "
+                                             "and epsilon");
+            const int g = off + i;
+            charge[g] = c.charge[i];
+            qkd[g] = c.charge[i] * sk;
+            shd[g] = 0.5 * c.sigma[i];
+            e2d[g] = 2.0 * std::sqrt(c.epsilon[i]);
+            qk[g] = (float)qkd[g];
+            sh[g] = (float)shd[g];
+            e2[g] = (float)e2d[g];
+            sum_q2_[si] += c.charge[i] * c.charge[i];
+            sum_q_[si] += c.charge[i];
+        }
+        for (size_t e = 0; e < c.exc1.size(); ++e) {
+            eqq.push_back(c.exc_qq[e]);
+            esig.push_back(c.exc_sig[e]);
+            eeps.push_back(c.exc_eps[e]);
+        }
+        disp_cached_[si] = 0;
+        SysInfo& S = sys_h_[si];
+        S.self_const = S.neutral_coeff = S.disp_coeff = 0.0;
+        if (c.method == OMM_NB_PME) {
+            S.self_const = -ONE_4PI_EPS0 * alpha_[si] / std::sqrt(PI) * sum_q2_[si];
+            S.neutral_coeff = -ONE_4PI_EPS0 * PI * sum_q_[si] * sum_q_[si] / (2.0 * alpha_[si] * alpha_[si]);
+        }
+        if (c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
+            S.disp_coeff = dispersion_coefficient(si);
+        off += n_[si];
+    }
+    part_.charge.upload(charge, s);
+    if (mixed_) {
+        part_.qk.upload(qk, s);
+        part_.sh.upload(sh, s);
+        part_.e2.upload(e2, s);
+    }
+    part_.qk_d.upload(qkd, s);
+    part_.sh_d.upload(shd, s);
+    part_.e2_d.upload(e2d, s);
+    exc_.qq.upload(eqq, s);
+    exc_.sig.upload(esig, s);
+    exc_.eps.upload(eeps, s);
+}
+
+void NbOperator::refresh_parameters() {
+    hipStream_t s = stream_;
+    upload_parameters(s);
+    // Patch the energy constants into the device table, whose boxes may have
+    // moved on the device (barostat).
+    std::vector<SysInfo> dev(sys_h_.size());
+    OMM_HIP(hipMemcpyAsync(dev.data(), sys_dev_.p, dev.size() * sizeof(SysInfo), hipMemcpyDeviceToHost, s));
+    OMM_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < dev.size(); ++k) {
+        dev[k].self_const = sys_h_[k].self_const;
+        dev[k].neutral_coeff = sys_h_[k].neutral_coeff;
+        dev[k].disp_coeff = sys_h_[k].disp_coeff;
+    }
+    sys_dev_.upload(dev, s);
+}
+
+bool NbOperator::set_global_parameter(const std::string& name, double value, bool* changed) {
+    if (!std::isfinite(value)) throw Error(OMM_ERR_INVALID, "global parameter values must be finite");
+    bool found = false, any = false;
+    for (NbSettings& c : cfg_)
+        for (size_t k = 0; k < c.gp_names.size(); ++k)
+            if (c.gp_names[k] == name) {
+                found = true;
+                if (c.gp_values[k] != value) {
+                    c.gp_values[k] = value;
+                    c.apply_offsets();
+                    any = true;
+                }
+            }
+    if (any && initialized_ && ntotal_ > 0) refresh_parameters();
+    if (changed) *changed = any;
+    return found;
+}
+
+bool NbOperator::get_global_parameter(const std::string& name, double* value) const {
+    for (const NbSettings& c : cfg_)
+        for (size_t k = 0; k < c.gp_names.size(); ++k)
+            if (c.gp_names[k] == name) {
+                if (value) *value = c.gp_values[k];
+                return true;
+            }
+    return false;
+}
+
+void NbOperator::update_parameters(const omm_nb_desc* desc) {
+    // nonbonded.rs:721-767
+    if (cfg_.size() != 1) throw Error(OMM_ERR_STATE, "parameter updates need a one-system operator");
+    NbSettings n = settings_from_desc(desc);
+    NbSettings& c = cfg_[0];
+    if (n.base_charge.size() != c.base_charge.size()) throw Error(OMM_ERR_INVALID, "particle count must match");
+    if (n.exc1.size() != c.exc1.size()) throw Error(OMM_ERR_INVALID, "exception count must match");
+    if (n.gp_names.size() != c.gp_names.size())
+        throw Error(OMM_ERR_INVALID, "global parameter names must not change");
+    std::vector<int> remap(n.gp_names.size());
+    for (size_t k = 0; k < n.gp_names.size(); ++k) {
+        auto it = std::find(c.gp_names.begin(), c.gp_names.end(), n.gp_names[k]);
+        if (it == c.gp_names.end()) throw Error(OMM_ERR_INVALID, "global parameter names must not change");
+        remap[k] = (int)(it - c.gp_names.begin());
+    }
+    auto same_targets = [&](std::vector<ParamOffset>& fresh, const std::vector<ParamOffset>& old) {
+        if (fresh.size() != old.size()) throw Error(OMM_ERR_INVALID, "parameter offset count must match");
+        for (ParamOffset& o : fresh) {
+            o.parameter = remap[o.parameter];
+            bool hit = false;
+            for (const ParamOffset& p : old) hit = hit || (p.parameter == o.parameter && p.index == o.index);
+            if (!hit) throw Error(OMM_ERR_INVALID, "parameter offset names and target indices must not change");
+        }
+    };
+    same_targets(n.particle_offsets, c.particle_offsets);
+    same_targets(n.exception_offsets, c.exception_offsets);
+    for (size_t e = 0; e < c.exc1.size(); ++e) {
+        if (n.exc1[e] != c.exc1[e] || n.exc2[e] != c.exc2[e])
+            throw Error(OMM_ERR_INVALID, "exception particle indices must not change");
+        bool has_offset = false;
+        for (const ParamOffset& o : c.exception_offsets) has_offset = has_offset || o.index == (int)e;
+        if (!(c.base_qq[e] != 0.0 || c.base_eps[e] != 0.0 || has_offset || (n.base_qq[e] == 0.0 && n.base_eps[e] == 0.0)))
+            throw Error(OMM_ERR_INVALID, "excluded exceptions cannot become non-excluded without reinitialization");
+    }
+    c.base_charge = n.base_charge;
+    c.base_sigma = n.base_sigma;
+    c.base_epsilon = n.base_epsilon;
+    c.base_qq = n.base_qq;
+    c.base_sig = n.base_sig;
+    c.base_eps = n.base_eps;
+    c.particle_offsets = n.particle_offsets;
+    c.exception_offsets = n.exception_offsets;
+    for (size_t k = 0; k < n.gp_names.size(); ++k) c.gp_defaults[remap[k]] = n.gp_defaults[k];
+    c.apply_offsets();
+    if (initialized_ && ntotal_ > 0) refresh_parameters();
+}
 
 double NbOperator::dispersion_coefficient(int si) const {
     // nonbonded.rs:660-719

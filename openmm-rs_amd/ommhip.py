@@ -89,6 +89,13 @@ class NbDesc(C.Structure):
         ("force_group", C.c_int32),
         ("reciprocal_force_group", C.c_int32),
         ("include_direct_space", C.c_int32),
+        ("num_global_parameters", C.c_int32),
+        ("global_parameter_names", C.POINTER(C.c_char_p)),
+        ("global_parameter_defaults", _dp),
+        ("num_particle_offsets", C.c_int32),
+        ("particle_offset_parameter", _ip), ("particle_offset_index", _ip), ("particle_offset_scales", _dp),
+        ("num_exception_offsets", C.c_int32),
+        ("exception_offset_parameter", _ip), ("exception_offset_index", _ip), ("exception_offset_scales", _dp),
     ]
 
 
@@ -147,6 +154,8 @@ def library():
         lib.omm_batch_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
         lib.omm_batch_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         lib.omm_ctx_set_barostat.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_uint64]
+        lib.omm_nb_set_global_parameter.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.POINTER(C.c_int32)]
+        lib.omm_ctx_set_parameter.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.POINTER(C.c_int32)]
         lib.omm_batch_set_barostat.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_uint64]
         _lib = lib
     return _lib
@@ -200,6 +209,9 @@ class NonbondedForce:
     force_group: int = 0
     reciprocal_force_group: int = -1
     include_direct_space: bool = True
+    global_parameters: list = field(default_factory=list)   # (name, default)
+    particle_offsets: list = field(default_factory=list)    # (parameter, index, charge, sigma, epsilon)
+    exception_offsets: list = field(default_factory=list)
 
     @staticmethod
     def _validate(charge, sigma, epsilon):
@@ -220,6 +232,34 @@ class NonbondedForce:
 
     def num_particles(self):
         return len(self.particles)
+
+    def add_global_parameter(self, name, default):
+        """nonbonded.rs:247-256."""
+        if not name or not math.isfinite(default):
+            raise ValueError("global parameters require a name and a finite default value")
+        self.global_parameters.append((str(name), float(default)))
+        return len(self.global_parameters) - 1
+
+    def _offset(self, parameter, index, scales, count, what):
+        if parameter not in [n for n, _ in self.global_parameters]:
+            raise ValueError(f"unknown offset parameter '{parameter}'")
+        if not 0 <= index < count:
+            raise ValueError(f"{what} offset index out of range")
+        if not all(math.isfinite(x) for x in scales):
+            raise ValueError("offset scales must be finite")
+        return (str(parameter), int(index)) + tuple(float(x) for x in scales)
+
+    def add_particle_parameter_offset(self, parameter, index, charge, sigma, epsilon):
+        """nonbonded.rs:305-321."""
+        self.particle_offsets.append(self._offset(parameter, index, (charge, sigma, epsilon),
+                                                  len(self.particles), "particle"))
+        return len(self.particle_offsets) - 1
+
+    def add_exception_parameter_offset(self, parameter, index, charge, sigma, epsilon):
+        """nonbonded.rs:323-339."""
+        self.exception_offsets.append(self._offset(parameter, index, (charge, sigma, epsilon),
+                                                   len(self.exceptions), "exception"))
+        return len(self.exception_offsets) - 1
 
     def num_exceptions(self):
         return len(self.exceptions)
@@ -354,6 +394,24 @@ class NonbondedForce:
         d.force_group = self.force_group
         d.reciprocal_force_group = self.reciprocal_force_group
         d.include_direct_space = int(self.include_direct_space)
+        names = [n for n, _ in self.global_parameters]
+        keep["gn"] = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        keep["gd"] = _f64([v for _, v in self.global_parameters])
+        d.num_global_parameters = len(names)
+        d.global_parameter_names = C.cast(keep["gn"], C.POINTER(C.c_char_p))
+        d.global_parameter_defaults = _ptr(keep["gd"], C.c_double)
+        for tag, offs in (("po", self.particle_offsets), ("eo", self.exception_offsets)):
+            keep[tag + "p"] = _i32([names.index(o[0]) for o in offs])
+            keep[tag + "i"] = _i32([o[1] for o in offs])
+            keep[tag + "s"] = _f64([list(o[2:5]) for o in offs]).reshape(-1)
+        d.num_particle_offsets = len(self.particle_offsets)
+        d.particle_offset_parameter = _ptr(keep["pop"], C.c_int32)
+        d.particle_offset_index = _ptr(keep["poi"], C.c_int32)
+        d.particle_offset_scales = _ptr(keep["pos"], C.c_double)
+        d.num_exception_offsets = len(self.exception_offsets)
+        d.exception_offset_parameter = _ptr(keep["eop"], C.c_int32)
+        d.exception_offset_index = _ptr(keep["eoi"], C.c_int32)
+        d.exception_offset_scales = _ptr(keep["eos"], C.c_double)
         d._keep = keep
         return d
 
@@ -429,6 +487,22 @@ class GpuNonbondedForce:
         h = C.c_uint64()
         _check(library().omm_nb_pair_stats(self._h, p.ctypes.data_as(_dp), bp, C.byref(n), C.byref(h)))
         return n.value, h.value
+
+    def set_global_parameter(self, name, value):
+        """Force::set_global_parameter: True when the force has the parameter."""
+        found = C.c_int32()
+        _check(library().omm_nb_set_global_parameter(self._h, name.encode(), float(value), C.byref(found)))
+        return bool(found.value)
+
+    def get_global_parameter(self, name):
+        v, found = C.c_double(), C.c_int32()
+        _check(library().omm_nb_get_global_parameter(self._h, name.encode(), C.byref(v), C.byref(found)))
+        return v.value if found.value else None
+
+    def update_parameters_from(self, force: NonbondedForce):
+        """NonbondedForce::update_parameters_from (nonbonded.rs:721-767)."""
+        d = force.desc()
+        _check(library().omm_nb_update_parameters(self._h, C.byref(d)))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -554,6 +628,12 @@ class Context:
     def set_graph_steps(self, steps):
         """Steps per captured HIP graph (0: eager launches)."""
         _check(library().omm_ctx_set_graph_steps(self._h, int(steps)))
+
+    def set_parameter(self, name, value):
+        """Context::set_parameter for the NonbondedForce's global parameters."""
+        found = C.c_int32()
+        _check(library().omm_ctx_set_parameter(self._h, name.encode(), float(value), C.byref(found)))
+        return bool(found.value)
 
     def set_barostat(self, pressure, temperature, frequency=25, seed=1):
         """MonteCarloBarostat(pressure bar, temperature K, frequency) with rigid molecules."""

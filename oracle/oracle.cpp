@@ -15,6 +15,7 @@
 #include <cstring>
 #include <unordered_map>
 #include <unordered_set>
+#include <set>
 #include <vector>
 
 namespace {
@@ -460,6 +461,32 @@ struct Desc {
         exceptions_pbc = d->exceptions_use_pbc != 0;
         switching = d->use_switching_function != 0;
         switching_distance = d->switching_distance;
+        apply_offsets(d);
+    }
+    // nonbonded.rs:395-430: parameter offsets at the global parameters'
+    // values (here their defaults), walked last-first with the first
+    // (parameter, target) seen winning.
+    void apply_offsets(const omm_nb_desc* d) {
+        std::set<std::pair<int, int>> seen;
+        for (int k = d->num_particle_offsets - 1; k >= 0; --k) {
+            const int prm = d->particle_offset_parameter[k], idx = d->particle_offset_index[k];
+            if (!seen.insert({prm, idx}).second) continue;
+            const double v = d->global_parameter_defaults[prm];
+            const double* sc = d->particle_offset_scales + 3 * k;
+            particles[idx].charge += v * sc[0];
+            particles[idx].sigma += v * sc[1];
+            particles[idx].epsilon += v * sc[2];
+        }
+        seen.clear();
+        for (int k = d->num_exception_offsets - 1; k >= 0; --k) {
+            const int prm = d->exception_offset_parameter[k], idx = d->exception_offset_index[k];
+            if (!seen.insert({prm, idx}).second) continue;
+            const double v = d->global_parameter_defaults[prm];
+            const double* sc = d->exception_offset_scales + 3 * k;
+            exceptions[idx].charge_prod += v * sc[0];
+            exceptions[idx].sigma += v * sc[1];
+            exceptions[idx].epsilon += v * sc[2];
+        }
     }
     bool periodic() const {
         return method == OMM_NB_CUTOFF_PERIODIC || method == OMM_NB_EWALD || method == OMM_NB_PME ||

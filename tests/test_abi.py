@@ -30,7 +30,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_pair_key():
     lib = ommhip.library()
-    assert lib.omm_abi_version() == 1
+    header = open(os.path.join(ROOT, "include", "ommhip.h")).read()
+    declared = int(re.search(r"#define OMMHIP_ABI_VERSION (\d+)", header).group(1))
+    assert lib.omm_abi_version() == declared
     # splitmix64 finaliser of (i << 32 | j)
     z = ((3 << 32) | 7) + 0x9E3779B97F4A7C15
     z &= 2**64 - 1

@@ -144,3 +144,68 @@ def test_cutoff_larger_than_half_box_is_rejected():
     with pytest.raises(ommhip.OmmError) as e:
         op.initialize(s.box)
     assert e.value.status == 2
+
+
+def _offset_water(value, explicit):
+    """300 waters with a global parameter 'lambda' offsetting the charge and
+    sigma of atom 0 and the charge product / sigma of exception 1 (an O-H
+    pair); explicit=True folds the offsets into the base parameters instead."""
+    s = water(300, 2.1, seed=51)
+    f = s.nonbonded
+    q, sg, ep = f.particles[0]
+    p1, p2, eq, es, ee = f.exceptions[1]
+    if explicit:
+        f.particles[0] = (q + value * 0.1, sg + value * 0.01, ep)
+        f.exceptions[1] = (p1, p2, eq + value * 0.05, es + value * 0.02, ee)
+    else:
+        f.add_global_parameter("lambda", 0.0)
+        f.add_particle_parameter_offset("lambda", 0, 0.1, 0.01, 0.0)
+        f.add_exception_parameter_offset("lambda", 1, 0.05, 0.02, 0.0)
+    return s
+
+
+def test_parameter_offsets_on_device():
+    # set_global_parameter moves the device parameters exactly as explicit ones would.
+    s = _offset_water(0.0, explicit=False)
+    op = device_op(s)
+    assert op.set_global_parameter("lambda", 0.0)
+    assert not op.set_global_parameter("unknown", 1.0)
+    for value in (0.0, 0.6, -0.35):
+        assert op.set_global_parameter("lambda", value)
+        assert op.get_global_parameter("lambda") == value
+        F, E = op.calc_forces_and_energy(s.positions, s.box)
+        ref = _offset_water(value, explicit=True)
+        F2, E2 = device_op(ref).calc_forces_and_energy(ref.positions, ref.box)
+        print(f"lambda={value}: E={E:.9f} explicit={E2:.9f}")
+        assert np.array_equal(F, F2)
+        assert E == E2
+        F_ref, E_ref, _ = orc.nb_evaluate(ref.nonbonded.desc(), ref.positions, ref.box)
+        assert force_rms_rel(F, F_ref) <= FORCE_TOL
+        assert energy_rel(E, E_ref) <= ENERGY_TOL
+
+
+def test_update_parameters_from():
+    s = water(300, 2.1, seed=52)
+    op = device_op(s)
+    f = s.nonbonded
+    q, sg, ep = f.particles[3]
+    f.particles[3] = (q * 0.5, sg, ep)
+    op.update_parameters_from(f)
+    F, E = op.calc_forces_and_energy(s.positions, s.box)
+    F2, E2 = device_op(s).calc_forces_and_energy(s.positions, s.box)
+    assert np.array_equal(F, F2) and E == E2
+    bad = water(300, 2.1, seed=52).nonbonded
+    p1, p2, eq, es, ee = bad.exceptions[0]
+    bad.exceptions[0] = (p1, p2, 0.1, es, ee)  # an exclusion cannot become an interaction
+    with pytest.raises(ommhip.OmmError):
+        op.update_parameters_from(bad)
+
+
+def test_context_set_parameter():
+    s = _offset_water(0.0, explicit=False)
+    ctx = ommhip.Context(s, temperature=0.0, friction=0.0)
+    assert ctx.set_parameter("lambda", 0.4)
+    ref = _offset_water(0.4, explicit=True)
+    a, b = ctx.get_state(), ommhip.Context(ref, temperature=0.0, friction=0.0).get_state()
+    assert np.array_equal(a["forces"], b["forces"])
+    assert abs(a["potential_energy"] - b["potential_energy"]) < 1e-7

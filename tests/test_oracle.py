@@ -13,6 +13,7 @@ import math
 import numpy as np
 import pytest
 
+import ommhip
 import oracle as orc
 from ommhip import NonbondedForce
 
@@ -381,3 +382,46 @@ def test_barostat_work():
     kt = orc.boltzmann() * 300.0
     w = orc.barostat_work(1.5, 1.0, 100.0, 101.0, 10, 300.0)
     assert w == pytest.approx(1.5 + 0.0602214076 * 1.0 - 10 * kt * math.log(1.01), rel=1e-14)
+
+
+def _two_particles(q0, s0, e0):
+    f = ommhip.NonbondedForce()
+    f.add_particle(q0, s0, e0)
+    f.add_particle(-0.3, 0.35, 0.5)
+    return f
+
+
+def test_parameter_offsets_equal_explicit_parameters():
+    # nonbonded.rs:1437-1455: an offset at value v equals the explicit parameters.
+    pos = np.array([[0.1, 0.2, 0.3], [0.6, 0.4, 0.2]])
+    for value in (0.25, 0.7, -0.2, 0.25):
+        f = _two_particles(0.2, 0.3, 0.4)
+        f.add_global_parameter("lambda", value)
+        f.add_particle_parameter_offset("lambda", 0, 0.4, 0.02, 0.2)
+        explicit = _two_particles(0.2 + 0.4 * value, 0.3 + 0.02 * value, 0.4 + 0.2 * value)
+        F1, E1, _ = orc.nb_evaluate(f.desc(), pos, None)
+        F2, E2, _ = orc.nb_evaluate(explicit.desc(), pos, None)
+        assert E1 == E2
+        assert np.array_equal(F1, F2)
+
+
+def test_later_offset_of_same_target_replaces_earlier():
+    # nonbonded.rs:401-405: offsets walked last-first, the first seen per (parameter, target) wins.
+    pos = np.array([[0.1, 0.2, 0.3], [0.6, 0.4, 0.2]])
+    f = _two_particles(0.2, 0.3, 0.4)
+    f.add_global_parameter("lambda", 0.5)
+    f.add_particle_parameter_offset("lambda", 0, 10.0, 10.0, 10.0)
+    f.add_particle_parameter_offset("lambda", 0, 0.4, 0.0, 0.0)
+    explicit = _two_particles(0.2 + 0.4 * 0.5, 0.3, 0.4)
+    assert orc.nb_evaluate(f.desc(), pos, None)[1] == orc.nb_evaluate(explicit.desc(), pos, None)[1]
+
+
+def test_exception_offsets_equal_explicit_parameters():
+    pos = np.array([[0.1, 0.2, 0.3], [0.6, 0.4, 0.2]])
+    f = _two_particles(0.2, 0.3, 0.4)
+    f.add_exception(0, 1, -0.1, 0.3, 0.2)
+    f.add_global_parameter("mu", 0.3)
+    f.add_exception_parameter_offset("mu", 0, 0.5, 0.1, 0.4)
+    explicit = _two_particles(0.2, 0.3, 0.4)
+    explicit.add_exception(0, 1, -0.1 + 0.3 * 0.5, 0.3 + 0.3 * 0.1, 0.2 + 0.3 * 0.4)
+    assert orc.nb_evaluate(f.desc(), pos, None)[1] == orc.nb_evaluate(explicit.desc(), pos, None)[1]
