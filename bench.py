@@ -17,7 +17,8 @@ independent water boxes of 500-6,000 molecules (log-uniform from a seed, box
 edge for 33.4 molecules/nm^3, each with its own PME grid) advanced by one set
 of launches per phase, in replica-steps/s, beside the same replicas run as 64
 separate plain FP64 contexts (per-system launches); and 128 replicas of
-300-4,000 molecules under NPT (Monte Carlo barostat at 1 bar every 25 steps).
+300-4,000 molecules under NPT (Monte Carlo barostat at 1 bar every 25 steps);
+and one 300,000-atom box with the HBM rate of every phase.
 """
 import argparse
 import json
@@ -36,6 +37,7 @@ METRIC = "ns/day (23.5k-atom PME water) + batched replica-steps/s, mixed vs plai
 DT_PS = 0.001
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_PAIR = 75
+TILE_SLOTS = 8 * 64  # pair slots of one k_pairs tile (8-atom cluster x 64 list entries)
 # HBM bytes per k_pairs launch on the 23,550-atom workload: rocprofv3 FETCH_SIZE
 # and WRITE_SIZE passes, each scaled by the same counter's reading for a plain
 # copy kernel of known size (profiles/k_pairs_md_step_water_23550_atoms.md).
@@ -172,6 +174,67 @@ def run_npt(steps, equilibrate, count=128, seed=8192):
     }
 
 
+def run_large(steps, equilibrate, seed=30000):
+    """One 300,000-atom system: 100,000 flexible waters in a 14.42 nm box,
+    cutoff 1.0 nm, PME tolerance 5e-4 (alpha 2.628261, grid 116^3), Langevin
+    1 fs at 300 K.  Besides ns/day it reports, per phase, the device time and
+    the HBM rate implied by the phase's algorithmic bytes (the least data the
+    phase must move, listed in ALGORITHMIC_BYTES)."""
+    s = ommhip.water_system(100000, 14.42, seed, cutoff=1.0, tolerance=5e-4)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seed,
+                         precision=ommhip.PRECISION_MIXED)
+    ctx.set_graph_steps(GRAPH_STEPS)
+    ctx.set_velocities_to_temperature(300.0, seed + 1)
+    equilibrate_then(ctx, equilibrate, 100)
+    secs = timed_steps(ctx, steps)
+    phases, npairs = ctx.profile(10)
+    tiles, entries = ctx.list_stats()
+    alpha, grid = pme_rule(1.0, 5e-4, 14.42)
+    n = len(s.masses)
+    g = int(np.prod(grid))
+    nbytes = algorithmic_bytes(n, g, entries)
+    rates = {k: round(nbytes[k] / (phases[k] * 1e-3) / 1e9, 1) for k in nbytes if phases.get(k, 0) > 0}
+    ctx.close()
+    return {
+        "workload": "100000 flexible 3-site waters (300000 atoms), 14.42 nm cubic box, PME cutoff 1.0 nm "
+                    f"tol 5e-4 (alpha {alpha:.6f}, grid {grid[0]}x{grid[1]}x{grid[2]}), LangevinMiddle 1 fs 300 K",
+        "atoms": n,
+        "value": round(ns_per_day(steps, secs), 3),
+        "unit": "ns/day",
+        "ms_per_step": round(secs / steps * 1e3, 4),
+        "steps": steps,
+        "pairs_per_step": npairs,
+        "pair_tflops": round(FLOP_PER_PAIR * npairs / (phases["direct_pairs"] * 1e-3) / 1e12, 3),
+        "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "algorithmic_bytes": nbytes,
+        "achieved_hbm_gbps": rates,
+        "hbm_peak_gbps": 8000.0,
+    }
+
+
+def pme_rule(cutoff, tol, edge):
+    """alpha and grid of a cubic box by the reference's rule (nonbonded.rs:647-658)."""
+    alpha = np.sqrt(-np.log(2.0 * tol)) / cutoff
+    n = max(6, int(np.ceil(2.0 * alpha * edge / (3.0 * tol ** 0.2))))
+    return float(alpha), (n, n, n)
+
+
+def algorithmic_bytes(n, g, entries):
+    """Least HBM bytes each phase moves: FP64 positions/velocities (24 B per
+    atom each), int64 fixed-point forces (24 B), FP32 particle parameters
+    (12 B), j-list entries (4 B), the int64 charge grid (8 B per point) and
+    the mesh's 24 B per grid point over the five FFT passes."""
+    return {
+        "nlist_build": n * (24 + 8) + entries * 4,
+        "direct_pairs": n * (24 + 12 + 24) + entries * 4,
+        "pme_spread": n * (24 + 4) + g * 8,
+        "pme_fft_conv": g * 24,
+        "pme_gather": n * (24 + 4 + 24) + g * 4,
+        "bonded": n * (24 + 24),
+        "integrator": n * 128,
+    }
+
+
 def cpu_baseline(sample_steps):
     import oracle as orc
 
@@ -215,6 +278,11 @@ def main():
     ap.add_argument("--npt-steps", type=int, default=250)
     ap.add_argument("--npt-equilibrate", type=int, default=500)
     ap.add_argument("--no-npt", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=300)
+    ap.add_argument("--large-equilibrate", type=int, default=600)
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the 300,000-atom workload and print its record (for profiling)")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 64-replica workload and print its record (for profiling)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -232,6 +300,9 @@ def main():
         print("no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.large_only:
+        print(json.dumps(run_large(args.large_steps, args.large_equilibrate)), flush=True)
+        return
     if args.batch_only:
         rec = run_batch(args.batch_steps, 100, args.batch_equilibrate, args.batch_plain_steps)
         print(json.dumps(rec), flush=True)
@@ -275,7 +346,7 @@ def main():
             "pairs_per_launch": npairs,
             "ms_per_launch": round(phases["direct_pairs"], 5),
             "tiles_per_launch": tiles,
-            "pair_slot_efficiency": round(npairs / max(1, tiles * 2048), 4),
+            "pair_slot_efficiency": round(npairs / max(1, tiles * TILE_SLOTS), 4),
         },
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "plain_fp64": {"value": round(plain, 3), "unit": "ns/day", "steps": args.plain_steps,
@@ -285,6 +356,8 @@ def main():
         line["batched"] = run_batch(args.batch_steps, 100, args.batch_equilibrate, args.batch_plain_steps)
     if not args.no_npt:
         line["batched_npt"] = run_npt(args.npt_steps, args.npt_equilibrate)
+    if not args.no_large:
+        line["large_system"] = run_large(args.large_steps, args.large_equilibrate)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(line), flush=True)

@@ -305,8 +305,9 @@ omm_status omm_ctx_get_box(omm_ctx* ctx, double box[9]);
 omm_status omm_ctx_barostat_attempt(omm_ctx* ctx);
 
 /* Shape of the current neighbour list (measurement hook; no counterpart in the
- * reference): number of 32 x 64 tiles the pair kernel evaluates and the total
- * number of j-list entries.  Pair-slot efficiency = pairs / (tiles * 2048). */
+ * reference): number of 8 x 64 tiles (8-atom cluster x 64 list entries) the
+ * pair kernel evaluates and the total number of cluster-list entries.
+ * Pair-slot efficiency = pairs / (tiles * 512). */
 omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries);
 
 /* Context::get_state (context.rs:930-1105) with positions, velocities, forces

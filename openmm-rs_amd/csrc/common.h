@@ -200,9 +200,10 @@ inline void fill_async(T* p, size_t n, T v, hipStream_t s) {
 
 // Nearest image in the reduced box (c, then b, then a) with round-to-nearest
 // shifts; used where any consistent image will do (binning, block bounds).
+// recip[4k] is 1 / v[4k] for a reduced (lower-triangular) box.
 __device__ __forceinline__ void wrap_nearest(double d[3], const Box& b) {
     for (int k = 2; k >= 0; --k) {
-        double s = rint(d[k] / b.v[k * 4]);
+        double s = rint(d[k] * b.recip[k * 4]);
         for (int a = 0; a <= k; ++a) d[a] -= s * b.v[k * 3 + a];
     }
 }

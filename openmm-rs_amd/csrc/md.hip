@@ -27,17 +27,25 @@ __device__ __forceinline__ int group_slot(const int at[4], int i) {
     return i == at[0] ? 0 : (i == at[1] ? 1 : (i == at[2] ? 2 : 3));
 }
 
+// f[slot] += v, with the slot chosen by selects (a register array; indexing
+// it with a run-time slot would put it in scratch memory).
+__device__ __forceinline__ void add_slot(double f[4][3], int slot, const double v[3]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[k][c] += slot == k ? v[c] : 0.0;
+}
+
 template <class A>
-__global__ void k_bonded(int ngroups, const int* __restrict__ grp_atoms, const int* __restrict__ grp_e,
-                         const int* __restrict__ grp_b, const int* __restrict__ grp_a, int ne, int ge,
-                         const int* __restrict__ e1, const int* __restrict__ e2, const double* __restrict__ eqq,
-                         const double* __restrict__ esig, const double* __restrict__ eeps, int nb, int gb,
-                         const int* __restrict__ b1, const int* __restrict__ b2, const double* __restrict__ blen,
-                         const double* __restrict__ bk, int na, int ga, const int* __restrict__ a1,
-                         const int* __restrict__ a2, const int* __restrict__ a3, const double* __restrict__ ath,
-                         const double* __restrict__ ak, const double* __restrict__ charge, int use_pbc, int has_pme,
-                         double alpha, const double* __restrict__ pos, const int* __restrict__ atom_sys,
-                         const SysInfo* __restrict__ sys, A* force, A* energy, unsigned* status) {
+__global__ __launch_bounds__(256) void k_bonded(
+    int ngroups, const int* __restrict__ grp_atoms, const int* __restrict__ grp_e, const int* __restrict__ grp_b,
+    const int* __restrict__ grp_a, int ne, int ge, const int* __restrict__ e1, const int* __restrict__ e2,
+    const double* __restrict__ eqq, const double* __restrict__ esig, const double* __restrict__ eeps, int nb, int gb,
+    const int* __restrict__ b1, const int* __restrict__ b2, const double* __restrict__ blen,
+    const double* __restrict__ bk, int na, int ga, const int* __restrict__ a1, const int* __restrict__ a2,
+    const int* __restrict__ a3, const double* __restrict__ ath, const double* __restrict__ ak,
+    const double* __restrict__ charge, int use_pbc, int has_pme, double alpha, const double* __restrict__ pos,
+    const int* __restrict__ atom_sys, const SysInfo* __restrict__ sys, A* force, A* energy, unsigned* status) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double ee = 0.0, eb = 0.0, ea = 0.0;
     int se = -1, sb = -1, sa = -1;
@@ -52,33 +60,28 @@ __global__ void k_bonded(int ngroups, const int* __restrict__ grp_atoms, const i
             double fi[3];
             ee += exception_pair(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, box, use_pbc, has_pme,
                                  alpha, fi);
-            const int si_ = group_slot(at, i), sj = group_slot(at, j);
-            for (int c = 0; c < 3; ++c) {
-                f[si_][c] += fi[c];
-                f[sj][c] -= fi[c];
-            }
+            const double mi[3] = {-fi[0], -fi[1], -fi[2]};
+            add_slot(f, group_slot(at, i), fi);
+            add_slot(f, group_slot(at, j), mi);
         }
         for (int b = grp_b[t]; b < grp_b[t + 1]; ++b) {
             const int i = b1[b], j = b2[b];
             double fi[3];
             eb += bond_pair(i, j, blen[b], bk[b], pos, fi);
-            const int si_ = group_slot(at, i), sj = group_slot(at, j);
-            for (int c = 0; c < 3; ++c) {
-                f[si_][c] += fi[c];
-                f[sj][c] -= fi[c];
-            }
+            const double mi[3] = {-fi[0], -fi[1], -fi[2]};
+            add_slot(f, group_slot(at, i), fi);
+            add_slot(f, group_slot(at, j), mi);
         }
         for (int a = grp_a[t]; a < grp_a[t + 1]; ++a) {
             const int i = a1[a], j = a2[a], k = a3[a];
             double fi[3], fk[3];
             ea += angle_triple(i, j, k, ath[a], ak[a], pos, fi, fk);
-            const int si_ = group_slot(at, i), sj = group_slot(at, j), sk = group_slot(at, k);
-            for (int c = 0; c < 3; ++c) {
-                f[si_][c] += fi[c];
-                f[sk][c] += fk[c];
-                f[sj][c] -= fi[c] + fk[c];
-            }
+            const double fj[3] = {-(fi[0] + fk[0]), -(fi[1] + fk[1]), -(fi[2] + fk[2])};
+            add_slot(f, group_slot(at, i), fi);
+            add_slot(f, group_slot(at, k), fk);
+            add_slot(f, group_slot(at, j), fj);
         }
+#pragma unroll
         for (int k = 0; k < 4; ++k)
             if (at[k] >= 0)
                 for (int c = 0; c < 3; ++c) acc_add(force + 3 * at[k] + c, f[k][c], FORCE_SCALE, status);

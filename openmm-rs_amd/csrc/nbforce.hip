@@ -336,6 +336,7 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
 
     pp_ = PairParams{};
     pp_.nblocks = nl_.nblocks;
+    pp_.nslots = nl_.nslots;
     pp_.cap = nl_.cap;
     pp_.periodic = 1;
     pp_.use_cutoff = 1;
@@ -596,17 +597,16 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
     for (int attempt = 0; attempt < 8; ++attempt) {
         fill_async<unsigned>(status, 1, 0u, s);
         nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
-        std::vector<int> counts(nl_.nblocks);
+        std::vector<int> counts(nl_.nblocks), ccounts(nl_.nclusters);
         unsigned st = 0;
         nl_.jcount.download(counts.data(), counts.size(), s);
+        nl_.ccount.download(ccounts.data(), ccounts.size(), s);
         OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         OMM_HIP(hipStreamSynchronize(s));
         int mx = 0;
         long long tasks = 0;
-        for (int c : counts) {
-            mx = std::max(mx, c);
-            tasks += (c + JTILE - 1) / JTILE;
-        }
+        for (int c : counts) mx = std::max(mx, c);
+        for (int c : ccounts) tasks += ((c + JTILE - 1) / JTILE + TPT - 1) / TPT;
         // One wave per task of the calibrated list (grid-stride beyond it):
         // enough resident waves to hide the latency of the pair loop.
         const int grid = (int)std::max<long long>(64, std::min<long long>(65535, tasks));
@@ -637,8 +637,8 @@ void NbOperator::list_stats(int64_t* tiles, int64_t* entries) {
         if (entries) *entries = 0;
         return;
     }
-    std::vector<int> counts(nl_.nblocks);
-    nl_.jcount.download(counts.data(), counts.size(), stream_);
+    std::vector<int> counts(nl_.nclusters);
+    nl_.ccount.download(counts.data(), counts.size(), stream_);
     OMM_HIP(hipStreamSynchronize(stream_));
     int64_t e = 0, total = 0;
     for (int c : counts) {

@@ -12,16 +12,22 @@
 // FP32 half-extents.  For every block I the list holds, in increasing order,
 // the sorted slots of the atoms of blocks J >= I of the same system within
 // cutoff + skin of I's bounding box (64-lane ballot + popcount compaction).
-// Pairs inside a list are kept with j_slot > i_slot, so each unordered pair is
-// visited once.
+// Each block's list is then split over its four 8-atom clusters: cluster c
+// keeps the entries within cutoff + skin of its own bounding box (its own
+// atoms always), still in increasing slot order.  The pair kernel walks the
+// cluster lists in tiles of 8 i-atoms x 64 entries.  Pairs inside a list are
+// kept with j_slot > i_slot, so each unordered pair is visited once.
 #pragma once
 
 #include "common.h"
 
 namespace ommhip {
 
-constexpr int IBLOCK = 32;  // i-atoms per tile (two j half-tiles per wave64)
+constexpr int IBLOCK = 32;  // atoms per block (the unit of the coarse list)
+constexpr int CLUSTER = 8;  // i-atoms per cluster (the unit of the pair kernel's lists)
+constexpr int CPB = IBLOCK / CLUSTER;  // clusters per block
 constexpr int JTILE = 64;   // j-slots per tile
+constexpr int TPT = 4;      // tiles of one cluster per pair-kernel task
 constexpr int MAXEX = 8;    // exclusion partners per atom handled by the pair kernel
 constexpr int PME_CHUNK = 64;  // sorted slots per PME spread/gather workgroup
 
@@ -35,7 +41,7 @@ struct NlistSystem {
 
 struct NlistDev {
     int nsys = 0;
-    int natoms = 0, nslots = 0, nblocks = 0, nchunks = 0, cap = 0, nunits = 0, ncells = 0;
+    int natoms = 0, nslots = 0, nblocks = 0, nclusters = 0, nchunks = 0, cap = 0, nunits = 0, ncells = 0;
     int uniform = 0;  // atoms per unit when all units are the same size, else 0
     int closed = 0;   // every exception pair lies inside one unit
     double cutoff = 0, skin = 0;
@@ -47,8 +53,15 @@ struct NlistDev {
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
-    DevBuf<int> jlist, jcount;
-    DevBuf<int> tasks, ntasks;  // pair-kernel tasks (block << 8 | tile) and their count
+    DevBuf<int> jlist, jcount;  // block lists [nblocks][cap]
+    DevBuf<int2> clist;         // cluster lists [nclusters][cap] of (sorted slot, atom)
+    DevBuf<int> ccount;
+    DevBuf<int> tasks, ntasks;  // pair-kernel tasks (cluster << 8 | first tile) and their count
+    // Direct-space forces by sorted slot, one component after another
+    // ([3][nslots]): the pair kernel's adds from consecutive list entries fall
+    // on consecutive addresses.  Zero between evaluations.
+    DevBuf<long long> fslot_fx;
+    DevBuf<double> fslot_d;
     DevBuf<int> scan_part;  // chunk sums of the multi-workgroup scans
     DevBuf<int> atom_sys;
     // Rebuild requests: [0] some system, [1 + s] system s.  Systems that are
@@ -58,13 +71,14 @@ struct NlistDev {
     DevBuf<int> one;  // all ones: the flags of forced builds
 };
 
-// Host-side set-up: units from each system's exception graph, Morton cell
+// Host-side set-up: units from each system's exception graph, Hilbert cell
 // ranks, slot/block/cell ranges (written into sys[s]), the identity slot
 // layout (so PME may run before the first build), sizes.
 void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vector<SysInfo>& sys, double cutoff,
                  double skin, hipStream_t s);
 
-// New per-block capacity (reallocates the list and the task table).
+// New per-block (and per-cluster) capacity (reallocates the lists and the
+// task table).
 void nlist_resize(NlistDev& nl, int cap);
 
 // Requests a rebuild when an atom moved more than skin/2 since the last build

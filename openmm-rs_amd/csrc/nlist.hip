@@ -283,16 +283,18 @@ __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __
     for (int k = 0; k < MAXEX; ++k) excl_sorted[(size_t)s * MAXEX + k] = v[k];
 }
 
-// Appends the pair kernel's tasks of block I: (I, tile) for every JTILE-slot
-// tile of its list, packed as I * 256 + tile.  The order of the tasks in the
-// table is whatever the atomics give; forces are fixed-point sums, so the
-// result does not depend on it.  Called by the 64 lanes of one wave.
-__device__ __forceinline__ void append_tasks(int I, int count, int* tasks, int* ntasks, int lane) {
+// Appends the pair kernel's tasks of cluster C: one per run of up to TPT
+// JTILE-entry tiles of its list, packed as C * 256 + first tile.  The order of
+// the tasks in the table is whatever the atomics give; forces are fixed-point
+// sums, so the result does not depend on it.  Called by the 64 lanes of one
+// wave.
+__device__ __forceinline__ void append_tasks(int C, int count, int* tasks, int* ntasks, int lane) {
     const int tiles = (count + JTILE - 1) / JTILE;
+    const int runs = (tiles + TPT - 1) / TPT;
     int base = 0;
-    if (lane == 0) base = atomicAdd(ntasks, tiles);
+    if (lane == 0 && runs > 0) base = atomicAdd(ntasks, runs);
     base = __shfl(base, 0, 64);
-    for (int k = lane; k < tiles; k += 64) tasks[base + k] = (I << 8) | k;
+    for (int k = lane; k < runs; k += 64) tasks[base + k] = (C << 8) | (k * TPT);
 }
 
 // Two blocks per wave64: lanes 0-31 block 2*blockIdx.x, lanes 32-63 the next.
@@ -345,17 +347,13 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
 __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, const int* __restrict__ sorted_atom,
                                               const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
                                               const double* __restrict__ center, const float* __restrict__ half,
-                                              double reach, int cap, int* jlist, int* jcount, int* tasks,
-                                              int* ntasks, unsigned* status, const int* flag) {
+                                              double reach, int cap, int* jlist, int* jcount, unsigned* status,
+                                              const int* flag) {
     if (!flag[0]) return;
     const int I = blockIdx.x;
     const int si = block_sys[I];
     const int lane = threadIdx.x;
-    if (!flag[1 + si]) {
-        // Unflagged systems keep their lists; only their tasks are appended.
-        append_tasks(I, jcount[I], tasks, ntasks, lane);
-        return;
-    }
+    if (!flag[1 + si]) return;  // unflagged systems keep their lists
     const SysInfo& S = sys[si];
     const Box& b = S.box;
     const int jend = S.block_off + S.nblocks;
@@ -408,7 +406,91 @@ __global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, co
         count = cap;
     }
     if (lane == 0) jcount[I] = count;
-    append_tasks(I, count, tasks, ntasks, lane);
+}
+
+// Splits block I's list over its CPB clusters of CLUSTER atoms: cluster q
+// keeps the entries at or after its own first slot that lie within reach of
+// its bounding box (its own atoms always), in the list's order, and appends
+// its pair-kernel tasks.  One wave per block; unflagged systems keep their
+// cluster lists and only append their tasks.
+__global__ __launch_bounds__(64) void k_clist(const double* __restrict__ pos, const int* __restrict__ sorted_atom,
+                                              const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
+                                              double reach, int cap, const int* __restrict__ jlist,
+                                              const int* __restrict__ jcount, int2* clist, int* ccount, int* tasks,
+                                              int* ntasks, const int* flag) {
+    if (!flag[0]) return;
+    const int I = blockIdx.x;
+    const int si = block_sys[I];
+    const int lane = threadIdx.x;
+    if (!flag[1 + si]) {
+        for (int q = 0; q < CPB; ++q) append_tasks(I * CPB + q, ccount[I * CPB + q], tasks, ntasks, lane);
+        return;
+    }
+    const Box& b = sys[si].box;
+    // Cluster bounding boxes relative to the block's first atom (a block
+    // always starts with an atom; padding only ends a system's last block).
+    const int ref = sorted_atom[I * IBLOCK];
+    const double rp[3] = {pos[3 * ref], pos[3 * ref + 1], pos[3 * ref + 2]};
+    const int a = lane < IBLOCK ? sorted_atom[I * IBLOCK + lane] : -1;
+    double mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = 1e300;
+        mx[k] = -1e300;
+    }
+    if (a >= 0) {
+        double d[3] = {pos[3 * a] - rp[0], pos[3 * a + 1] - rp[1], pos[3 * a + 2] - rp[2]};
+        wrap_fast(d, b);
+        for (int k = 0; k < 3; ++k) mn[k] = mx[k] = d[k];
+    }
+    for (int off = 1; off < CLUSTER; off <<= 1)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
+            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
+        }
+    double cq[CPB][3], hq[CPB][3];
+    bool has[CPB];
+    for (int q = 0; q < CPB; ++q) {
+        has[q] = true;
+        for (int k = 0; k < 3; ++k) {
+            const double lo = __shfl(mn[k], q * CLUSTER, 64), hi = __shfl(mx[k], q * CLUSTER, 64);
+            if (lo > hi) has[q] = false;  // a cluster of padding only
+            cq[q][k] = rp[k] + 0.5 * (lo + hi);
+            hq[q][k] = 0.5 * (hi - lo) * (1.0 + 1e-6) + 1e-6;
+        }
+    }
+    const double reach2 = reach * reach;
+    const int jc = jcount[I];
+    const int* in = jlist + (size_t)I * cap;
+    int count[CPB];
+    for (int q = 0; q < CPB; ++q) count[q] = 0;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int base = 0; base < jc; base += 64) {
+        const int e = base + lane;
+        const int slot = e < jc ? in[e] : -1;
+        const int aj = slot >= 0 ? sorted_atom[slot] : -1;
+        double p[3] = {0.0, 0.0, 0.0};
+        if (aj >= 0)
+            for (int k = 0; k < 3; ++k) p[k] = pos[3 * aj + k];
+        for (int q = 0; q < CPB; ++q) {
+            const int first = I * IBLOCK + q * CLUSTER;
+            bool ok = false;
+            if (aj >= 0 && has[q] && slot >= first) {
+                if (slot < first + CLUSTER) {
+                    ok = true;
+                } else {
+                    double d[3] = {p[0] - cq[q][0], p[1] - cq[q][1], p[2] - cq[q][2]};
+                    ok = box_gap2(d, hq[q], b) < reach2;
+                }
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) clist[(size_t)(I * CPB + q) * cap + count[q] + __popcll(m & below)] = make_int2(slot, aj);
+            count[q] += __popcll(m);
+        }
+    }
+    for (int q = 0; q < CPB; ++q) {
+        if (lane == 0) ccount[I * CPB + q] = count[q];
+        append_tasks(I * CPB + q, count[q], tasks, ntasks, lane);
+    }
 }
 
 // Position of cell (x, y, z) along a 3-D Hilbert curve over a 2^bits cube
@@ -574,6 +656,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     if (nl.uniform < 0) nl.uniform = 0;
     nl.natoms = natoms;
     nl.nblocks = nblocks;
+    nl.nclusters = nblocks * CPB;
     nl.nslots = nblocks * IBLOCK;
     nl.nchunks = nchunks;
     nl.nunits = (int)uptr.size() - 1;
@@ -618,9 +701,15 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.pos_build.alloc(3 * (size_t)natoms);
     nl.cap = std::min(nl.cap, 255 * JTILE);
     nl.jlist.alloc((size_t)nblocks * nl.cap);
-    nl.tasks.alloc((size_t)nblocks * (nl.cap / JTILE));
+    nl.clist.alloc((size_t)nl.nclusters * nl.cap);
+    nl.tasks.alloc((size_t)nl.nclusters * ((nl.cap / JTILE + TPT - 1) / TPT));
     nl.ntasks.alloc(1);
     nl.jcount.alloc(nblocks);
+    nl.ccount.alloc(nl.nclusters);
+    nl.fslot_fx.alloc(3 * (size_t)nl.nslots);
+    nl.fslot_d.alloc(3 * (size_t)nl.nslots);
+    fill_async<long long>(nl.fslot_fx.p, 3 * (size_t)nl.nslots, 0ll, s);
+    fill_async<double>(nl.fslot_d.p, 3 * (size_t)nl.nslots, 0.0, s);
     nl.scan_part.alloc((size_t)std::max(1, blocks_for(std::max(ncells, nl.nunits), SCAN_CHUNK)));
     std::vector<int> one(nsys + 1, 1);
     nl.flag.upload(one, s);
@@ -632,7 +721,8 @@ void nlist_resize(NlistDev& nl, int cap) {
     if (cap / JTILE > 255) throw Error(OMM_ERR_INVALID, "neighbour list needs more than 255 tiles per block");
     nl.cap = cap;
     nl.jlist.alloc((size_t)nl.nblocks * nl.cap);
-    nl.tasks.alloc((size_t)nl.nblocks * (nl.cap / JTILE));
+    nl.clist.alloc((size_t)nl.nclusters * nl.cap);
+    nl.tasks.alloc((size_t)nl.nclusters * ((nl.cap / JTILE + TPT - 1) / TPT));
 }
 
 void nlist_check(NlistDev& nl, const double* pos, int force, hipStream_t s) {
@@ -671,8 +761,10 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
                        nl.block_minimg.p, nl.pos_build.p, nl.ntasks.p, flag);
     hipLaunchKernelGGL(k_jlist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.block_sys.p, sys_dev,
-                       nl.block_center.p, nl.block_half.p, reach, nl.cap, nl.jlist.p, nl.jcount.p, nl.tasks.p,
-                       nl.ntasks.p, status, flag);
+                       nl.block_center.p, nl.block_half.p, reach, nl.cap, nl.jlist.p, nl.jcount.p, status, flag);
+    hipLaunchKernelGGL(k_clist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.block_sys.p, sys_dev,
+                       reach, nl.cap, nl.jlist.p, nl.jcount.p, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p,
+                       flag);
     OMM_HIP(hipGetLastError());
 }
 

@@ -9,6 +9,7 @@ enum ElecKind : int { ELEC_EWALD = 0, ELEC_REACTION_FIELD = 1, ELEC_COULOMB = 2 
 
 struct PairParams {
     int nblocks = 0, cap = 0;
+    int nslots = 0;  // stride of the slot-ordered force components
     int elec = ELEC_EWALD;
     int use_cutoff = 1;
     int periodic = 1;
@@ -35,7 +36,9 @@ struct ExceptionDev {
 
 // Direct-space pairs over the tile list.  mixed: FP32 math with int64
 // fixed-point force/energy accumulation; otherwise FP64 math with FP64 atomics
-// into force_d/energy_d.  energy_on: also accumulate the energy (in the mixed
+// into force_d/energy_d.  The pair kernel accumulates into the list's
+// slot-ordered buffer; a second kernel adds that into the caller's forces
+// (atom order) and clears it.  energy_on: also accumulate the energy (in the mixed
 // form each pair energy is evaluated in FP64).  stats (nullable): [0] pair
 // count, [1] pair hash per system (no energy is accumulated then).  Energies
 // go to slot E_DIRECT of each system's E_SLOTS block.

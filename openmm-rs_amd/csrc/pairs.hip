@@ -1,12 +1,14 @@
 // Direct-space LJ + Ewald-screened Coulomb over the tile list, and the
 // exception kernel (nonbonded.rs:842-924, 995-1036).
 //
-// One wave64 per task = (i-block of 32 atoms, j-tile of 64 list entries).
-// Lane l owns i-atom l&31 and the j half-tile l>>5; in step k it pairs with
-// slot (l+k)&31 of its half, so every slot is touched by exactly one lane per
-// step.  j-data sit in LDS (read-only); the j-force accumulators travel with
-// their slot through a 32-lane rotation (ds_bpermute), so no lane ever writes
-// LDS inside the loop.  Positions are FP64 in HBM; each task converts them to
+// One wave64 per task = (cluster of 8 i-atoms, up to TPT consecutive tiles
+// of 64 entries of the cluster's list); the loads of the next tiles are in
+// flight while a tile is evaluated.  Lane l owns i-atom l%8 and the group of entries
+// 8*(l/8) .. 8*(l/8)+7; in step k (0..7) it pairs with entry 8*(l/8) + (l%8 ^ k),
+// so every entry is touched by exactly one lane per step.  j-data sit in LDS
+// (read-only); the j-force accumulators travel with their entry between the
+// lanes of a group through DPP permutations (lane ^ 1, ^ 3, ^ 7), so no lane
+// writes LDS inside the loop and no LDS instruction moves them.  Positions are FP64 in HBM; each task converts them to
 // FP32 offsets from the i-block's FP64 centre, so deltas carry ~1e-7 nm
 // absolute error instead of the ~5e-7 nm of absolute FP32 coordinates.
 // Near-cutoff pairs (|r^2 - rc^2| < 1e-5 rc^2) are re-decided in FP64 with
@@ -120,31 +122,85 @@ __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double 
     return e;
 }
 
+// Lane permutations by DPP inside each row of 16 lanes: quad_perm [1,0,3,2]
+// is lane ^ 1, quad_perm [3,2,1,0] is lane ^ 3, row_half_mirror is lane ^ 7
+// (each group of 8 reversed), row_ror:8 is lane ^ 8.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR3 = 0x1B, DPP_XOR7 = 0x141, DPP_XOR8 = 0x128;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// The j-force accumulators move from step k to step k + 1: lane l takes the
+// accumulator of lane l ^ (k ^ (k + 1)) of its group of 8.
+template <int K, class R>
+__device__ __forceinline__ void rotate_j(R& gx, R& gy, R& gz) {
+    constexpr int m = K ^ (K + 1);
+    constexpr int ctrl = m == 1 ? DPP_XOR1 : (m == 3 ? DPP_XOR3 : DPP_XOR7);
+    gx = dpp<ctrl>(gx);
+    gy = dpp<ctrl>(gy);
+    gz = dpp<ctrl>(gz);
+}
+
+// Raw data of one tile entry as loaded from HBM, before it is staged in LDS.
+template <class R, bool E64>
+struct EntryRaw {
+    double x = 0, y = 0, z = 0;
+    R q = 0, s = 0, e = 0;
+    double qd = 0, sd = 0, ed = 0;  // the energy path's FP64 parameters
+};
+
+template <class R, bool E64>
+__device__ __forceinline__ void load_entry(const int2 en, const double* __restrict__ pos, const R* __restrict__ qk,
+                                           const R* __restrict__ sh, const R* __restrict__ e2,
+                                           const double* __restrict__ qk_d, const double* __restrict__ sh_d,
+                                           const double* __restrict__ e2_d, EntryRaw<R, E64>& r) {
+    if (en.y >= 0) {
+        r.x = pos[3 * en.y];
+        r.y = pos[3 * en.y + 1];
+        r.z = pos[3 * en.y + 2];
+        r.q = qk[en.y];
+        r.s = sh[en.y];
+        r.e = e2[en.y];
+        if (E64) {
+            r.qd = qk_d[en.y];
+            r.sd = sh_d[en.y];
+            r.ed = e2_d[en.y];
+        }
+    }
+}
+
 template <class R, class A, int ELEC, bool STATS, bool ENERGY>
 __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, const R* __restrict__ qk,
                                               const R* __restrict__ sh, const R* __restrict__ e2,
                                               const double* __restrict__ qk_d, const double* __restrict__ sh_d,
                                               const double* __restrict__ e2_d, const int* __restrict__ sorted_atom,
                                               const int* __restrict__ excl_sorted, const double* __restrict__ center,
-                                              const int* __restrict__ minimg, const int* __restrict__ jlist,
-                                              const int* __restrict__ jcount, const int* __restrict__ tasks,
+                                              const int* __restrict__ minimg, const int2* __restrict__ clist,
+                                              const int* __restrict__ ccount, const int* __restrict__ tasks,
                                               const int* __restrict__ ntasks, const int* __restrict__ block_sys,
-                                              const SysInfo* __restrict__ sys, PairParams P, A* force, A* energy,
+                                              const SysInfo* __restrict__ sys, PairParams P, A* fslot, A* energy,
                                               unsigned* status, unsigned long long* stats, int* list_flag,
                                               int nflags) {
     using M = Math<R>;
     constexpr bool E64 = ENERGY && sizeof(R) == 4;  // separate FP64 energy path
     __shared__ Q4<R> s_pq[JTILE];
     __shared__ Q2<R> s_se[JTILE];
-    __shared__ int s_js[JTILE];
+    __shared__ __align__(16) int s_js[JTILE];
     __shared__ int s_ja[JTILE];
     __shared__ Q4<double> s_pd[E64 ? JTILE : 1];
     __shared__ Q2<double> s_sd[E64 ? JTILE : 1];
 
     const int lane = threadIdx.x;
-    const int half = lane >> 5;
-    const int il = lane & 31;
-    const int src = (lane & 32) | ((lane + 1) & 31);
+    const int grp = lane / CLUSTER;  // the lane's group of 8 tile entries
+    const int il = lane % CLUSTER;   // the lane's i-atom in the cluster
     const double cutoff2 = P.cutoff * P.cutoff;
     const double band = sizeof(R) == 4 ? 1e-5 : 1e-12;
     const R rc2_lo = (R)(cutoff2 * (1.0 - band));
@@ -160,45 +216,32 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
     const int total = ntasks[0];
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int task = tasks[t];
-        const int I = task >> 8;
-        const int tile = task & 255;
-        const int cnt = min(JTILE, jcount[I] - tile * JTILE);
+        const int C = task >> 8;
+        const int first = task & 255;
+        const int I = C / CPB;
+        const int count = ccount[C];
+        const int ntile = min(TPT, (count + JTILE - 1) / JTILE - first);
         const int sidx = block_sys[I];
         const Box& bx = sys[sidx].box;
         const double c0 = center[3 * I], c1 = center[3 * I + 1], c2 = center[3 * I + 2];
+        const int2* list = clist + (size_t)C * P.cap + (size_t)first * JTILE;
+        const int nentries = count - first * JTILE;
+        auto entry = [&](int tl) {
+            const int e = tl * JTILE + lane;
+            return (tl < ntile && e < nentries) ? list[e] : make_int2(INT_MAX, -1);
+        };
+        // Software pipeline: the entries of tile k + 2 and the coordinates of
+        // tile k + 1 are in flight while tile k is evaluated.
+        int2 e0 = entry(0), e1 = entry(1);
+        EntryRaw<R, E64> r0;
+        load_entry<R, E64>(e0, pos, qk, sh, e2, qk_d, sh_d, e2_d, r0);
 
-        // Stage the j-tile in LDS.
-        {
-            int js = INT_MAX, ja = -1;
-            Q4<R> pq = {0, 0, 0, 0};
-            Q2<R> se = {0, 0};
-            Q4<double> pd = {0, 0, 0, 0};
-            Q2<double> sd = {0, 0};
-            if (lane < cnt) {
-                js = jlist[(size_t)I * P.cap + tile * JTILE + lane];
-                ja = sorted_atom[js];
-                double d[3] = {pos[3 * ja] - c0, pos[3 * ja + 1] - c1, pos[3 * ja + 2] - c2};
-                if (P.periodic) wrap_nearest(d, bx);
-                pq = {(R)d[0], (R)d[1], (R)d[2], qk[ja]};
-                se = {sh[ja], e2[ja]};
-                if (E64) {
-                    pd = {d[0], d[1], d[2], qk_d[ja]};
-                    sd = {sh_d[ja], e2_d[ja]};
-                }
-            }
-            s_pq[lane] = pq;
-            s_se[lane] = se;
-            s_js[lane] = js;
-            s_ja[lane] = ja;
-            if (E64) {
-                s_pd[lane] = pd;
-                s_sd[lane] = sd;
-            }
-        }
-        const int is = I * IBLOCK + il;
+        const int is = C * CLUSTER + il;
         const int ia = sorted_atom[is];  // -1 in the padding of a system's last block
         R xi = 0, yi = 0, zi = 0, qi = 0, si = 0, ei = 0;
         double xid = 0, yid = 0, zid = 0, qid = 0, sid = 0, eid = 0;
+        int ex[MAXEX];
+        for (int m = 0; m < MAXEX; ++m) ex[m] = INT_MAX;
         if (ia >= 0) {
             double d[3] = {pos[3 * ia] - c0, pos[3 * ia + 1] - c1, pos[3 * ia + 2] - c2};
             if (P.periodic) wrap_nearest(d, bx);
@@ -216,23 +259,6 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 sid = sh_d[ia];
                 eid = e2_d[ia];
             }
-        }
-        __syncthreads();
-
-        // Valid slots of this lane's half: j_sorted > i_sorted, present, not excluded.
-        unsigned vmask = 0;
-        if (ia >= 0) {
-            const int* hjs = s_js + half * 32;
-            int a = 0, b = 32;
-            while (a < b) {
-                int m = (a + b) >> 1;
-                if (hjs[m] > is) b = m;
-                else a = m + 1;
-            }
-            vmask = a >= 32 ? 0u : (0xffffffffu << a);
-            int nvalid = min(max(cnt - half * 32, 0), 32);
-            if (nvalid < 32) vmask &= (1u << nvalid) - 1u;
-            int ex[MAXEX];
             const int4* ex4 = reinterpret_cast<const int4*>(excl_sorted + (size_t)is * MAXEX);
             for (int q = 0; q < MAXEX / 4; ++q) {
                 const int4 v = ex4[q];
@@ -240,18 +266,6 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 ex[4 * q + 1] = v.y;
                 ex[4 * q + 2] = v.z;
                 ex[4 * q + 3] = v.w;
-            }
-            for (int m = 0; m < MAXEX; ++m) {
-                int e = ex[m];
-                if (e == INT_MAX) break;
-                if (e <= is) continue;
-                int lo2 = 0, hi2 = 32;
-                while (lo2 < hi2) {
-                    int mid = (lo2 + hi2) >> 1;
-                    if (hjs[mid] < e) lo2 = mid + 1;
-                    else hi2 = mid;
-                }
-                if (lo2 < 32 && hjs[lo2] == e) vmask &= ~(1u << lo2);
             }
         }
         const bool mi = P.periodic && minimg[I];
@@ -263,105 +277,187 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             ib8 = (R)(1.0 / bx.v[8]);
         }
 
-        R fx = 0, fy = 0, fz = 0, gx = 0, gy = 0, gz = 0;
+        R fx = 0, fy = 0, fz = 0;
         double eacc = 0.0;
         unsigned long long npair = 0, hsum = 0;
-        for (int k = 0; k < 32; ++k) {
-            const int kk = (il + k) & 31;
-            const int slot = half * 32 + kk;
-            if ((vmask >> kk) & 1u) {
-                const Q4<R> pj = s_pq[slot];
-                R dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-                if (mi) {
-                    R s = M::round(dz * ib8);
-                    dx -= s * bv[6];
-                    dy -= s * bv[7];
-                    dz -= s * bv[8];
-                    s = M::round(dy * ib4);
-                    dx -= s * bv[3];
-                    dy -= s * bv[4];
-                    s = M::round(dx * ib0);
-                    dx -= s * bv[0];
-                }
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                bool in = !P.use_cutoff || r2 < rc2_lo;
-                if (!in && r2 < rc2_hi) in = within_cutoff_exact(pos, ia, s_ja[slot], bx, P.periodic, cutoff2);
-                if (in) {
-                    const Q2<R> sj = s_se[slot];
-                    const R inv_r2 = M::rcp(r2);
-                    const R inv_r = M::rsqrt(r2);
-                    // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
-                    const R sig = si + sj.x;
-                    const R eps4 = ei * sj.y;
-                    const R s2 = sig * sig * inv_r2;
-                    const R s6 = s2 * s2 * s2;
-                    const R s12 = s6 * s6;
-                    R e = eps4 * (s12 - s6);
-                    R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
-                    const R qq = qi * pj.w;
-                    if (ELEC == ELEC_EWALD) {
-                        // screened_coulomb (nonbonded.rs:1263-1273)
-                        const R r = r2 * inv_r;
-                        R erfc_v, ex;
-                        M::erfc_exp(alpha * r, erfc_v, ex);
-                        const R ec = qq * erfc_v * inv_r;
-                        e += ec;
-                        f -= (ec + qq * two_a_sqrtpi * ex) * inv_r2;
-                    } else if (ELEC == ELEC_REACTION_FIELD) {
-                        // reaction_field (nonbonded.rs:1233-1250)
-                        e += qq * (inv_r + krf * r2 - crf);
-                        f += qq * ((R)2 * krf - inv_r * inv_r2);
-                    } else {
-                        // coulomb_pair (nonbonded.rs:1219-1226)
-                        const R ec = qq * inv_r;
-                        e += ec;
-                        f -= ec * inv_r2;
-                    }
-                    fx += f * dx;
-                    fy += f * dy;
-                    fz += f * dz;
-                    gx -= f * dx;
-                    gy -= f * dy;
-                    gz -= f * dz;
+        for (int k = 0; k < ntile; ++k) {
+            const int2 e2n = entry(k + 2);
+            EntryRaw<R, E64> r1;
+            load_entry<R, E64>(e1, pos, qk, sh, e2, qk_d, sh_d, e2_d, r1);
+
+            // Stage tile k in LDS as offsets from the block centre.
+            {
+                Q4<R> pq = {0, 0, 0, 0};
+                Q2<R> se = {0, 0};
+                Q4<double> pd = {0, 0, 0, 0};
+                Q2<double> sd = {0, 0};
+                if (e0.y >= 0) {
+                    double d[3] = {r0.x - c0, r0.y - c1, r0.z - c2};
+                    if (P.periodic) wrap_nearest(d, bx);
+                    pq = {(R)d[0], (R)d[1], (R)d[2], r0.q};
+                    se = {r0.s, r0.e};
                     if (E64) {
-                        const Q4<double> pjd = s_pd[slot];
-                        const Q2<double> sjd = s_sd[slot];
-                        double ddx = pjd.x - xid, ddy = pjd.y - yid, ddz = pjd.z - zid;
-                        if (mi) {
-                            double dd[3] = {ddx, ddy, ddz};
-                            wrap_nearest(dd, bx);
-                            ddx = dd[0];
-                            ddy = dd[1];
-                            ddz = dd[2];
-                        }
-                        eacc += pair_energy_fp64<ELEC>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
-                    } else if (ENERGY) {
-                        eacc += (double)e;
+                        pd = {d[0], d[1], d[2], r0.qd};
+                        sd = {r0.sd, r0.ed};
                     }
-                    if (STATS) {
-                        const int ja = s_ja[slot];
-                        ++npair;
-                        hsum += pair_key(min(ia, ja), max(ia, ja));
+                }
+                s_pq[lane] = pq;
+                s_se[lane] = se;
+                s_js[lane] = e0.x;
+                s_ja[lane] = e0.y;
+                if (E64) {
+                    s_pd[lane] = pd;
+                    s_sd[lane] = sd;
+                }
+            }
+            __syncthreads();
+
+            // Valid entries of the lane's group: present, j_sorted > i_sorted, not excluded.
+            unsigned vmask = 0;
+            if (ia >= 0) {
+                const int4* g4 = reinterpret_cast<const int4*>(s_js + grp * CLUSTER);
+                const int4 lo = g4[0], hi = g4[1];
+                const int jv[CLUSTER] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+                for (int s8 = 0; s8 < CLUSTER; ++s8)
+                    if (jv[s8] != INT_MAX && jv[s8] > is) vmask |= 1u << s8;
+                if (vmask) {
+                    for (int m = 0; m < MAXEX; ++m) {
+                        const int e = ex[m];
+                        if (e == INT_MAX) break;
+                        if (e <= is || e < jv[0] || e > jv[CLUSTER - 1]) continue;
+#pragma unroll
+                        for (int s8 = 0; s8 < CLUSTER; ++s8)
+                            if (jv[s8] == e) vmask &= ~(1u << s8);
                     }
                 }
             }
-            gx = __shfl(gx, src, 64);
-            gy = __shfl(gy, src, 64);
-            gz = __shfl(gz, src, 64);
+
+            R gx = 0, gy = 0, gz = 0;
+            // Step k: lane (grp, il) pairs i-atom il with entry grp * 8 + (il ^ k);
+            // each entry meets every i-atom of the cluster once over the 8 steps.
+            auto pair_step = [&](const int k) {
+                const int s8 = il ^ k;
+                const int slot = grp * CLUSTER + s8;
+                if ((vmask >> s8) & 1u) {
+                    const Q4<R> pj = s_pq[slot];
+                    R dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+                    if (mi) {
+                        R s = M::round(dz * ib8);
+                        dx -= s * bv[6];
+                        dy -= s * bv[7];
+                        dz -= s * bv[8];
+                        s = M::round(dy * ib4);
+                        dx -= s * bv[3];
+                        dy -= s * bv[4];
+                        s = M::round(dx * ib0);
+                        dx -= s * bv[0];
+                    }
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    bool in = !P.use_cutoff || r2 < rc2_lo;
+                    if (!in && r2 < rc2_hi) in = within_cutoff_exact(pos, ia, s_ja[slot], bx, P.periodic, cutoff2);
+                    if (in) {
+                        const Q2<R> sj = s_se[slot];
+                        const R inv_r2 = M::rcp(r2);
+                        const R inv_r = M::rsqrt(r2);
+                        // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
+                        const R sig = si + sj.x;
+                        const R eps4 = ei * sj.y;
+                        const R s2 = sig * sig * inv_r2;
+                        const R s6 = s2 * s2 * s2;
+                        const R s12 = s6 * s6;
+                        R e = eps4 * (s12 - s6);
+                        R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
+                        const R qq = qi * pj.w;
+                        if (ELEC == ELEC_EWALD) {
+                            // screened_coulomb (nonbonded.rs:1263-1273)
+                            const R r = r2 * inv_r;
+                            R erfc_v, ex2;
+                            M::erfc_exp(alpha * r, erfc_v, ex2);
+                            const R ec = qq * erfc_v * inv_r;
+                            e += ec;
+                            f -= (ec + qq * two_a_sqrtpi * ex2) * inv_r2;
+                        } else if (ELEC == ELEC_REACTION_FIELD) {
+                            // reaction_field (nonbonded.rs:1233-1250)
+                            e += qq * (inv_r + krf * r2 - crf);
+                            f += qq * ((R)2 * krf - inv_r * inv_r2);
+                        } else {
+                            // coulomb_pair (nonbonded.rs:1219-1226)
+                            const R ec = qq * inv_r;
+                            e += ec;
+                            f -= ec * inv_r2;
+                        }
+                        fx += f * dx;
+                        fy += f * dy;
+                        fz += f * dz;
+                        gx -= f * dx;
+                        gy -= f * dy;
+                        gz -= f * dz;
+                        if (E64) {
+                            const Q4<double> pjd = s_pd[slot];
+                            const Q2<double> sjd = s_sd[slot];
+                            double ddx = pjd.x - xid, ddy = pjd.y - yid, ddz = pjd.z - zid;
+                            if (mi) {
+                                double dd[3] = {ddx, ddy, ddz};
+                                wrap_nearest(dd, bx);
+                                ddx = dd[0];
+                                ddy = dd[1];
+                                ddz = dd[2];
+                            }
+                            eacc += pair_energy_fp64<ELEC>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
+                        } else if (ENERGY) {
+                            eacc += (double)e;
+                        }
+                        if (STATS) {
+                            const int ja = s_ja[slot];
+                            ++npair;
+                            hsum += pair_key(min(ia, ja), max(ia, ja));
+                        }
+                    }
+                }
+            };
+            pair_step(0);
+            rotate_j<0>(gx, gy, gz);
+            pair_step(1);
+            rotate_j<1>(gx, gy, gz);
+            pair_step(2);
+            rotate_j<2>(gx, gy, gz);
+            pair_step(3);
+            rotate_j<3>(gx, gy, gz);
+            pair_step(4);
+            rotate_j<4>(gx, gy, gz);
+            pair_step(5);
+            rotate_j<5>(gx, gy, gz);
+            pair_step(6);
+            rotate_j<6>(gx, gy, gz);
+            pair_step(7);
+            // After step 7 the lane holds the accumulator of entry grp * 8 + (il ^ 7).
+            const int jmine = grp * CLUSTER + (il ^ (CLUSTER - 1));
+            if (s_ja[jmine] >= 0) {
+                const int js = s_js[jmine];
+                acc_add(fslot + js, (double)gx, FORCE_SCALE, status);
+                acc_add(fslot + P.nslots + js, (double)gy, FORCE_SCALE, status);
+                acc_add(fslot + 2 * P.nslots + js, (double)gz, FORCE_SCALE, status);
+            }
+            __syncthreads();
+            e0 = e1;
+            e1 = e2n;
+            r0 = r1;
         }
+        // i-forces: the eight groups' partial sums, in a fixed order.
+        fx += dpp<DPP_XOR8>(fx);
+        fy += dpp<DPP_XOR8>(fy);
+        fz += dpp<DPP_XOR8>(fz);
+        fx += __shfl_xor(fx, 16, 64);
+        fy += __shfl_xor(fy, 16, 64);
+        fz += __shfl_xor(fz, 16, 64);
         fx += __shfl_xor(fx, 32, 64);
         fy += __shfl_xor(fy, 32, 64);
         fz += __shfl_xor(fz, 32, 64);
-        if (half == 0 && ia >= 0) {
-            acc_add(force + 3 * ia, (double)fx, FORCE_SCALE, status);
-            acc_add(force + 3 * ia + 1, (double)fy, FORCE_SCALE, status);
-            acc_add(force + 3 * ia + 2, (double)fz, FORCE_SCALE, status);
-        }
-        const int jmine = s_ja[lane];
-        if (jmine >= 0) {
-            acc_add(force + 3 * jmine, (double)gx, FORCE_SCALE, status);
-            acc_add(force + 3 * jmine + 1, (double)gy, FORCE_SCALE, status);
-            acc_add(force + 3 * jmine + 2, (double)gz, FORCE_SCALE, status);
+        if (grp == 0 && ia >= 0) {
+            acc_add(fslot + is, (double)fx, FORCE_SCALE, status);
+            acc_add(fslot + P.nslots + is, (double)fy, FORCE_SCALE, status);
+            acc_add(fslot + 2 * P.nslots + is, (double)fz, FORCE_SCALE, status);
         }
         if (ENERGY) {
             double etot = wave_sum(eacc);
@@ -375,7 +471,6 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 atomicAdd(stats + 2 * sidx + 1, hsum);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -398,14 +493,49 @@ __global__ void k_exceptions(int count, const int* __restrict__ p1, const int* _
     wave_energy_add(energy, si, E_EXCEPT, e, status);
 }
 
+template <class A>
+A* slot_buffer(const NlistDev& nl);
+template <>
+long long* slot_buffer<long long>(const NlistDev& nl) {
+    return nl.fslot_fx.p;
+}
+template <>
+double* slot_buffer<double>(const NlistDev& nl) {
+    return nl.fslot_d.p;
+}
+
+// Adds the slot-ordered pair forces into the atom-ordered accumulator and
+// clears them.  Atomic, because the mesh may add into the same accumulator
+// from another stream meanwhile.
+__device__ __forceinline__ void add_raw(long long* p, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+__device__ __forceinline__ void add_raw(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+template <class A>
+__global__ void k_slot_forces(const int* __restrict__ sorted_atom, int nslots, A* fslot, A* force) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int a = sorted_atom[s];
+    for (int c = 0; c < 3; ++c) {
+        const A v = fslot[(size_t)c * nslots + s];
+        if (v != (A)0) {
+            if (a >= 0) add_raw(force + 3 * a + c, v);
+            fslot[(size_t)c * nslots + s] = (A)0;
+        }
+    }
+}
+
 template <class R, class A, int ELEC, bool STATS, bool ENERGY>
 void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* sh, const R* e2,
                 const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
                 unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
                        part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.excl_sorted.p, nl.block_center.p,
-                       nl.block_minimg.p, nl.jlist.p, nl.jcount.p, nl.tasks.p, nl.ntasks.p, nl.block_sys.p, sys_dev, p,
-                       force, energy, status, stats, nl.flag.p, nl.nsys + 1);
+                       nl.block_minimg.p, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, nl.block_sys.p, sys_dev, p,
+                       slot_buffer<A>(nl), energy, status, stats, nl.flag.p, nl.nsys + 1);
+    hipLaunchKernelGGL(k_slot_forces<A>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p, nl.nslots,
+                       slot_buffer<A>(nl), force);
 }
 
 template <class R, class A, int ELEC>
