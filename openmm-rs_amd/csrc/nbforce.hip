@@ -581,7 +581,7 @@ HostTerms NbOperator::host_terms(int si, double volume, bool direct, bool recip)
 void NbOperator::grow(unsigned status) {
     if (status & ST_LIST_OVERFLOW) {
         const int limit = std::min(255 * JTILE, (nl_.nslots + 2 * JTILE) / JTILE * JTILE);
-        if (nl_.cap >= limit) throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per block");
+        if (nl_.cap >= limit) throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per cluster");
         nlist_resize(nl_, std::min(limit, 2 * nl_.cap));
         pp_.cap = nl_.cap;
     }
@@ -597,16 +597,17 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
     for (int attempt = 0; attempt < 8; ++attempt) {
         fill_async<unsigned>(status, 1, 0u, s);
         nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
-        std::vector<int> counts(nl_.nblocks), ccounts(nl_.nclusters);
+        std::vector<int> ccounts(nl_.nclusters);
         unsigned st = 0;
-        nl_.jcount.download(counts.data(), counts.size(), s);
         nl_.ccount.download(ccounts.data(), ccounts.size(), s);
         OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
         OMM_HIP(hipStreamSynchronize(s));
         int mx = 0;
         long long tasks = 0;
-        for (int c : counts) mx = std::max(mx, c);
-        for (int c : ccounts) tasks += ((c + JTILE - 1) / JTILE + TPT - 1) / TPT;
+        for (int c : ccounts) {
+            mx = std::max(mx, c);
+            tasks += ((c + JTILE - 1) / JTILE + TPT - 1) / TPT;
+        }
         // One wave per task of the calibrated list (grid-stride beyond it):
         // enough resident waves to hide the latency of the pair loop.
         const int grid = (int)std::max<long long>(64, std::min<long long>(65535, tasks));
@@ -621,7 +622,7 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
                    std::to_string(st) + "]";
         const long long limit = std::min<long long>(255 * JTILE, (max_npad + 2 * JTILE) / JTILE * JTILE);
         if (want > limit)
-            throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per block:" + history);
+            throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per cluster:" + history);
         nlist_resize(nl_, (int)std::min<long long>(limit, std::max<long long>(want, 2LL * nl_.cap)));
         pp_.cap = nl_.cap;
         changed = true;

@@ -9,14 +9,13 @@
 // sum downstream) is deterministic and the same as for the system built
 // alone.  Consecutive runs
 // of IBLOCK sorted atoms of one system form blocks with an FP64 centre and
-// FP32 half-extents.  For every block I the list holds, in increasing order,
-// the sorted slots of the atoms of blocks J >= I of the same system within
-// cutoff + skin of I's bounding box (64-lane ballot + popcount compaction).
-// Each block's list is then split over its four 8-atom clusters: cluster c
-// keeps the entries within cutoff + skin of its own bounding box (its own
-// atoms always), still in increasing slot order.  The pair kernel walks the
-// cluster lists in tiles of 8 i-atoms x 64 entries.  Pairs inside a list are
-// kept with j_slot > i_slot, so each unordered pair is visited once.
+// FP32 half-extents; each block holds four 8-atom clusters.  The list of a
+// cluster holds, in increasing order, the (sorted slot, atom) of its own
+// atoms and of the later atoms of the same system within cutoff + skin of
+// its bounding box; blocks whose boxes are out of reach are skipped whole
+// (64-lane ballot + popcount compaction).  The pair kernel walks the cluster
+// lists in tiles of 8 i-atoms x 64 entries.  Pairs inside a list are kept
+// with j_slot > i_slot, so each unordered pair is visited once.
 #pragma once
 
 #include "common.h"
@@ -53,7 +52,6 @@ struct NlistDev {
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
-    DevBuf<int> jlist, jcount;  // block lists [nblocks][cap]
     DevBuf<int2> clist;         // cluster lists [nclusters][cap] of (sorted slot, atom)
     DevBuf<int> ccount;
     DevBuf<int> tasks, ntasks;  // pair-kernel tasks (cluster << 8 | first tile) and their count
@@ -77,8 +75,7 @@ struct NlistDev {
 void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vector<SysInfo>& sys, double cutoff,
                  double skin, hipStream_t s);
 
-// New per-block (and per-cluster) capacity (reallocates the lists and the
-// task table).
+// New per-cluster list capacity (reallocates the lists and the task table).
 void nlist_resize(NlistDev& nl, int cap);
 
 // Requests a rebuild when an atom moved more than skin/2 since the last build

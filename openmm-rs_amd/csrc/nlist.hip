@@ -283,20 +283,6 @@ __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __
     for (int k = 0; k < MAXEX; ++k) excl_sorted[(size_t)s * MAXEX + k] = v[k];
 }
 
-// Appends the pair kernel's tasks of cluster C: one per run of up to TPT
-// JTILE-entry tiles of its list, packed as C * 256 + first tile.  The order of
-// the tasks in the table is whatever the atomics give; forces are fixed-point
-// sums, so the result does not depend on it.  Called by the 64 lanes of one
-// wave.
-__device__ __forceinline__ void append_tasks(int C, int count, int* tasks, int* ntasks, int lane) {
-    const int tiles = (count + JTILE - 1) / JTILE;
-    const int runs = (tiles + TPT - 1) / TPT;
-    int base = 0;
-    if (lane == 0 && runs > 0) base = atomicAdd(ntasks, runs);
-    base = __shfl(base, 0, 64);
-    for (int k = lane; k < runs; k += 64) tasks[base + k] = (C << 8) | (k * TPT);
-}
-
 // Two blocks per wave64: lanes 0-31 block 2*blockIdx.x, lanes 32-63 the next.
 // The first slot of a block always holds an atom (padding only ends a
 // system's last block).
@@ -307,7 +293,7 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
                                                      double* center, float* half, int* minimg,
                                                      double* pos_build, int* ntasks, const int* flag) {
     if (!flag[0]) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) ntasks[0] = 0;  // k_jlist appends the tasks anew
+    if (blockIdx.x == 0 && threadIdx.x == 0) ntasks[0] = 0;  // k_clist appends the tasks anew
     int lane = threadIdx.x;
     int blk = blockIdx.x * 2 + (lane >> 5);
     const int si = blk < nblocks ? block_sys[blk] : 0;
@@ -344,153 +330,161 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(64) void k_jlist(const double* __restrict__ pos, const int* __restrict__ sorted_atom,
-                                              const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
-                                              const double* __restrict__ center, const float* __restrict__ half,
-                                              double reach, int cap, int* jlist, int* jcount, unsigned* status,
-                                              const int* flag) {
-    if (!flag[0]) return;
-    const int I = blockIdx.x;
-    const int si = block_sys[I];
-    const int lane = threadIdx.x;
-    if (!flag[1 + si]) return;  // unflagged systems keep their lists
-    const SysInfo& S = sys[si];
-    const Box& b = S.box;
-    const int jend = S.block_off + S.nblocks;
-    const double reach2 = reach * reach;
-    double cI[3] = {center[3 * I], center[3 * I + 1], center[3 * I + 2]};
-    double hI[3] = {half[3 * I], half[3 * I + 1], half[3 * I + 2]};
-    int* out = jlist + (size_t)I * cap;
-    int own = min(IBLOCK, S.slot_off + S.n - I * IBLOCK);
-    if (lane < own) out[lane] = I * IBLOCK + lane;
-    int count = own;
-    for (int J0 = I + 1; J0 < jend; J0 += 64) {
-        int J = J0 + lane;
-        bool pass = false;
-        if (J < jend) {
-            double dc[3], h[3];
-            for (int k = 0; k < 3; ++k) {
-                dc[k] = center[3 * J + k] - cI[k];
-                h[k] = hI[k] + (double)half[3 * J + k];
-            }
-            pass = box_gap2(dc, h, b) < reach2;
-        }
-        unsigned long long m = __ballot(pass);
-        while (m) {
-            int j1 = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            int j2 = -1;
-            if (m) {
-                j2 = __ffsll((long long)m) - 1;
-                m &= m - 1;
-            }
-            int Jx = (lane < 32) ? J0 + j1 : (j2 >= 0 ? J0 + j2 : -1);
-            int slot = Jx >= 0 ? Jx * IBLOCK + (lane & 31) : -1;
-            bool ok = false;
-            int a = slot >= 0 ? sorted_atom[slot] : -1;
-            if (a >= 0) {
-                double d[3] = {pos[3 * a] - cI[0], pos[3 * a + 1] - cI[1], pos[3 * a + 2] - cI[2]};
-                ok = box_gap2(d, hI, b) < reach2;
-            }
-            unsigned long long mm = __ballot(ok);
-            int off = __popcll(mm & ((1ull << lane) - 1ull));
-            if (ok) {
-                int w = count + off;
-                if (w < cap) out[w] = slot;
-            }
-            count += __popcll(mm);
+// Cluster lists, one wave per cluster C of block I = C / CPB (CLIST_WAVES
+// clusters per workgroup).  Candidate blocks J > I of the same system pass
+// when their bounding boxes come within reach of the cluster's box; the atoms
+// of the passing blocks and the later atoms of I itself are then tested
+// against the cluster's box.  The list keeps, in increasing slot order, the
+// (sorted slot, atom) of the cluster's own atoms and of every later atom
+// within reach, and the cluster appends its pair-kernel tasks: one run of up
+// to TPT tiles of JTILE entries per task, packed as cluster * 256 + first
+// tile (their order in the table does not matter, as forces are fixed-point
+// sums; one atomic per workgroup reserves the runs of its clusters).  The
+// atoms of four passing blocks are loaded together, so their loads overlap.
+// Unflagged systems keep their lists and only append their tasks.
+__device__ __forceinline__ void append_entry(int slot, int atom, const double p[3], int first, const double cq[3],
+                                             const double hq[3], const Box& b, double reach2, int cap,
+                                             int2* out, int& count, unsigned long long below) {
+    bool ok = false;
+    if (atom >= 0 && slot >= first) {
+        if (slot < first + CLUSTER) {
+            ok = true;
+        } else {
+            double d[3] = {p[0] - cq[0], p[1] - cq[1], p[2] - cq[2]};
+            ok = box_gap2(d, hq, b) < reach2;
         }
     }
-    if (count > cap) {
-        if (lane == 0) atomicOr(status, ST_LIST_OVERFLOW);
-        count = cap;
+    const unsigned long long m = __ballot(ok);
+    if (ok) {
+        const int w = count + __popcll(m & below);
+        if (w < cap) out[w] = make_int2(slot, atom);
     }
-    if (lane == 0) jcount[I] = count;
+    count += __popcll(m);
 }
 
-// Splits block I's list over its CPB clusters of CLUSTER atoms: cluster q
-// keeps the entries at or after its own first slot that lie within reach of
-// its bounding box (its own atoms always), in the list's order, and appends
-// its pair-kernel tasks.  One wave per block; unflagged systems keep their
-// cluster lists and only append their tasks.
-__global__ __launch_bounds__(64) void k_clist(const double* __restrict__ pos, const int* __restrict__ sorted_atom,
-                                              const int* __restrict__ block_sys, const SysInfo* __restrict__ sys,
-                                              double reach, int cap, const int* __restrict__ jlist,
-                                              const int* __restrict__ jcount, int2* clist, int* ccount, int* tasks,
-                                              int* ntasks, const int* flag) {
+// Position of the k-th (from 0) set bit of m; m has more than k set bits.
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int k) {
+    int pos = 0;
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll(m & ((1ull << w) - 1ull));
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+constexpr int CLIST_WAVES = 16;  // clusters (one per wave) per workgroup of k_clist
+
+__global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
+    const double* __restrict__ pos, const int* __restrict__ sorted_atom, const int* __restrict__ block_sys,
+    const SysInfo* __restrict__ sys, const double* __restrict__ center, const float* __restrict__ half,
+    double reach, int cap, int nclusters, int2* clist, int* ccount, int* tasks, int* ntasks, unsigned* status,
+    const int* flag) {
+    __shared__ int s_runs[CLIST_WAVES];
+    __shared__ int s_base;
     if (!flag[0]) return;
-    const int I = blockIdx.x;
-    const int si = block_sys[I];
-    const int lane = threadIdx.x;
-    if (!flag[1 + si]) {
-        for (int q = 0; q < CPB; ++q) append_tasks(I * CPB + q, ccount[I * CPB + q], tasks, ntasks, lane);
-        return;
-    }
-    const Box& b = sys[si].box;
-    // Cluster bounding boxes relative to the block's first atom (a block
-    // always starts with an atom; padding only ends a system's last block).
-    const int ref = sorted_atom[I * IBLOCK];
-    const double rp[3] = {pos[3 * ref], pos[3 * ref + 1], pos[3 * ref + 2]};
-    const int a = lane < IBLOCK ? sorted_atom[I * IBLOCK + lane] : -1;
-    double mn[3], mx[3];
-    for (int k = 0; k < 3; ++k) {
-        mn[k] = 1e300;
-        mx[k] = -1e300;
-    }
-    if (a >= 0) {
-        double d[3] = {pos[3 * a] - rp[0], pos[3 * a + 1] - rp[1], pos[3 * a + 2] - rp[2]};
-        wrap_fast(d, b);
-        for (int k = 0; k < 3; ++k) mn[k] = mx[k] = d[k];
-    }
-    for (int off = 1; off < CLUSTER; off <<= 1)
+    const int wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int C = blockIdx.x * CLIST_WAVES + wave;
+    const int I = C / CPB;
+    const int si = C < nclusters ? block_sys[I] : 0;
+    const int first = C * CLUSTER;  // the cluster's first slot
+    int count = 0;
+    if (C < nclusters && !flag[1 + si]) {
+        count = ccount[C];  // unflagged systems keep their lists
+    } else if (C < nclusters && sorted_atom[first] >= 0) {
+        const SysInfo& S = sys[si];
+        const Box& b = S.box;
+        const int jend = S.block_off + S.nblocks;
+        const double reach2 = reach * reach;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        int2* out = clist + (size_t)C * cap;
+
+        // The rest of block I from the cluster on: lanes 0 .. 31 - 8 q.
+        const int rest = (I + 1) * IBLOCK - first;
+        const int own_slot = lane < rest ? first + lane : -1;
+        const int own_atom = own_slot >= 0 ? sorted_atom[own_slot] : -1;
+        double own_p[3] = {0.0, 0.0, 0.0};
+        if (own_atom >= 0)
+            for (int k = 0; k < 3; ++k) own_p[k] = pos[3 * own_atom + k];
+        // Bounding box of the cluster (lanes 0-7), relative to its first atom.
+        const double rp[3] = {__shfl(own_p[0], 0, 64), __shfl(own_p[1], 0, 64), __shfl(own_p[2], 0, 64)};
+        double mn[3], mx[3];
         for (int k = 0; k < 3; ++k) {
-            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
-            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
+            mn[k] = 1e300;
+            mx[k] = -1e300;
         }
-    double cq[CPB][3], hq[CPB][3];
-    bool has[CPB];
-    for (int q = 0; q < CPB; ++q) {
-        has[q] = true;
-        for (int k = 0; k < 3; ++k) {
-            const double lo = __shfl(mn[k], q * CLUSTER, 64), hi = __shfl(mx[k], q * CLUSTER, 64);
-            if (lo > hi) has[q] = false;  // a cluster of padding only
-            cq[q][k] = rp[k] + 0.5 * (lo + hi);
-            hq[q][k] = 0.5 * (hi - lo) * (1.0 + 1e-6) + 1e-6;
+        if (lane < CLUSTER && own_atom >= 0) {
+            double d[3] = {own_p[0] - rp[0], own_p[1] - rp[1], own_p[2] - rp[2]};
+            wrap_fast(d, b);
+            for (int k = 0; k < 3; ++k) mn[k] = mx[k] = d[k];
         }
-    }
-    const double reach2 = reach * reach;
-    const int jc = jcount[I];
-    const int* in = jlist + (size_t)I * cap;
-    int count[CPB];
-    for (int q = 0; q < CPB; ++q) count[q] = 0;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (int base = 0; base < jc; base += 64) {
-        const int e = base + lane;
-        const int slot = e < jc ? in[e] : -1;
-        const int aj = slot >= 0 ? sorted_atom[slot] : -1;
-        double p[3] = {0.0, 0.0, 0.0};
-        if (aj >= 0)
-            for (int k = 0; k < 3; ++k) p[k] = pos[3 * aj + k];
-        for (int q = 0; q < CPB; ++q) {
-            const int first = I * IBLOCK + q * CLUSTER;
-            bool ok = false;
-            if (aj >= 0 && has[q] && slot >= first) {
-                if (slot < first + CLUSTER) {
-                    ok = true;
-                } else {
-                    double d[3] = {p[0] - cq[q][0], p[1] - cq[q][1], p[2] - cq[q][2]};
-                    ok = box_gap2(d, hq[q], b) < reach2;
-                }
+        for (int off = 1; off < CLUSTER; off <<= 1)
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
+                mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
             }
-            const unsigned long long m = __ballot(ok);
-            if (ok) clist[(size_t)(I * CPB + q) * cap + count[q] + __popcll(m & below)] = make_int2(slot, aj);
-            count[q] += __popcll(m);
+        double cq[3], hq[3];
+        for (int k = 0; k < 3; ++k) {
+            const double lo = __shfl(mn[k], 0, 64), hi = __shfl(mx[k], 0, 64);
+            cq[k] = rp[k] + 0.5 * (lo + hi);
+            hq[k] = 0.5 * (hi - lo) * (1.0 + 1e-6) + 1e-6;
         }
+        append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, count, below);
+
+        for (int J0 = I + 1; J0 < jend; J0 += 64) {
+            const int J = J0 + lane;
+            bool pass = false;
+            if (J < jend) {
+                double dc[3], h[3];
+                for (int k = 0; k < 3; ++k) {
+                    dc[k] = center[3 * J + k] - cq[k];
+                    h[k] = hq[k] + (double)half[3 * J + k];
+                }
+                pass = box_gap2(dc, h, b) < reach2;
+            }
+            const unsigned long long m = __ballot(pass);
+            const int npass = __popcll(m);
+            // Four passing blocks per round: lanes 0-31 take the first of
+            // each pair, lanes 32-63 the second.
+            for (int r = 0; r < npass; r += 4) {
+                const int ka = r + (lane >> 5), kb = r + 2 + (lane >> 5);
+                const int slot_a = ka < npass ? (J0 + nth_set_bit(m, ka)) * IBLOCK + (lane & 31) : -1;
+                const int slot_b = kb < npass ? (J0 + nth_set_bit(m, kb)) * IBLOCK + (lane & 31) : -1;
+                const int atom_a = slot_a >= 0 ? sorted_atom[slot_a] : -1;
+                const int atom_b = slot_b >= 0 ? sorted_atom[slot_b] : -1;
+                double pa[3] = {0.0, 0.0, 0.0}, pb[3] = {0.0, 0.0, 0.0};
+                if (atom_a >= 0)
+                    for (int k = 0; k < 3; ++k) pa[k] = pos[3 * atom_a + k];
+                if (atom_b >= 0)
+                    for (int k = 0; k < 3; ++k) pb[k] = pos[3 * atom_b + k];
+                append_entry(slot_a, atom_a, pa, first, cq, hq, b, reach2, cap, out, count, below);
+                append_entry(slot_b, atom_b, pb, first, cq, hq, b, reach2, cap, out, count, below);
+            }
+        }
+        if (count > cap) {
+            if (lane == 0) atomicOr(status, ST_LIST_OVERFLOW);
+            count = cap;
+        }
+        if (lane == 0) ccount[C] = count;
+    } else if (C < nclusters) {
+        if (lane == 0) ccount[C] = 0;  // a cluster of padding only
     }
-    for (int q = 0; q < CPB; ++q) {
-        if (lane == 0) ccount[I * CPB + q] = count[q];
-        append_tasks(I * CPB + q, count[q], tasks, ntasks, lane);
+    const int runs = ((count + JTILE - 1) / JTILE + TPT - 1) / TPT;
+    if (lane == 0) s_runs[wave] = runs;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int all = 0;
+        for (int w = 0; w < CLIST_WAVES; ++w) all += s_runs[w];
+        s_base = all > 0 ? atomicAdd(ntasks, all) : 0;
     }
+    __syncthreads();
+    int base = s_base;
+    for (int w = 0; w < wave; ++w) base += s_runs[w];
+    for (int k = lane; k < runs; k += 64) tasks[base + k] = (C << 8) | (k * TPT);
 }
 
 // Position of cell (x, y, z) along a 3-D Hilbert curve over a 2^bits cube
@@ -661,14 +655,15 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.nchunks = nchunks;
     nl.nunits = (int)uptr.size() - 1;
     nl.ncells = ncells;
-    // List capacity: the atoms within reach of a compact block at liquid
-    // density, doubled, and never more than the largest system.
+    // List capacity: the atoms within reach of a compact cluster at liquid
+    // density (a cluster's list holds about half of them), and never more
+    // than the largest system; calibration and overflows grow it.
     double reach = cutoff + skin;
     double density = 100.0;
-    double hb = 0.5 * std::cbrt(IBLOCK / density) * 1.6;
+    double hb = 0.5 * std::cbrt(CLUSTER / density) * 1.6;
     double vol = std::pow(2 * hb, 3) + 6 * std::pow(2 * hb, 2) * reach + 3 * PI * 2 * hb * reach * reach +
                  4.0 / 3.0 * PI * reach * reach * reach;
-    long long cap = (long long)(2.0 * density * vol) + IBLOCK;
+    long long cap = (long long)(density * vol) + IBLOCK;
     cap = std::min<long long>(cap, (long long)max_npad);
     cap = std::max<long long>(cap, 256);
     nl.cap = (int)((cap + JTILE - 1) / JTILE * JTILE);
@@ -700,11 +695,9 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.block_minimg.alloc(nblocks);
     nl.pos_build.alloc(3 * (size_t)natoms);
     nl.cap = std::min(nl.cap, 255 * JTILE);
-    nl.jlist.alloc((size_t)nblocks * nl.cap);
     nl.clist.alloc((size_t)nl.nclusters * nl.cap);
     nl.tasks.alloc((size_t)nl.nclusters * ((nl.cap / JTILE + TPT - 1) / TPT));
     nl.ntasks.alloc(1);
-    nl.jcount.alloc(nblocks);
     nl.ccount.alloc(nl.nclusters);
     nl.fslot_fx.alloc(3 * (size_t)nl.nslots);
     nl.fslot_d.alloc(3 * (size_t)nl.nslots);
@@ -718,9 +711,8 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
 }
 
 void nlist_resize(NlistDev& nl, int cap) {
-    if (cap / JTILE > 255) throw Error(OMM_ERR_INVALID, "neighbour list needs more than 255 tiles per block");
+    if (cap / JTILE > 255) throw Error(OMM_ERR_INVALID, "neighbour list needs more than 255 tiles per cluster");
     nl.cap = cap;
-    nl.jlist.alloc((size_t)nl.nblocks * nl.cap);
     nl.clist.alloc((size_t)nl.nclusters * nl.cap);
     nl.tasks.alloc((size_t)nl.nclusters * ((nl.cap / JTILE + TPT - 1) / TPT));
 }
@@ -760,11 +752,9 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p,
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
                        nl.block_minimg.p, nl.pos_build.p, nl.ntasks.p, flag);
-    hipLaunchKernelGGL(k_jlist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.block_sys.p, sys_dev,
-                       nl.block_center.p, nl.block_half.p, reach, nl.cap, nl.jlist.p, nl.jcount.p, status, flag);
-    hipLaunchKernelGGL(k_clist, dim3(nl.nblocks), dim3(64), 0, s, pos, nl.sorted_atom.p, nl.block_sys.p, sys_dev,
-                       reach, nl.cap, nl.jlist.p, nl.jcount.p, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p,
-                       flag);
+    hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
+                       nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p, reach, nl.cap,
+                       nl.nclusters, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
     OMM_HIP(hipGetLastError());
 }
 
