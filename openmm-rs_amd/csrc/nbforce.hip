@@ -424,6 +424,8 @@ void NbOperator::upload_parameters(hipStream_t s) {
         off += n_[si];
     }
     part_.charge.upload(charge, s);
+    pme_.max_charge = 0.0;
+    for (double q : charge) pme_.max_charge = std::max(pme_.max_charge, std::fabs(q));
     if (mixed_) {
         part_.qk.upload(qk, s);
         part_.sh.upload(sh, s);

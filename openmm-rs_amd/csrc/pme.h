@@ -28,6 +28,7 @@ struct PmeDev {
     size_t fft_shmem[2][3] = {};  // dynamic LDS per axis
     // Spread: per-atom stencils (base point, weights) and per-tile atom bins.
     int natoms = 0, ntiles = 0, bin_cap = 0;
+    double max_charge = 0.0;        // largest |q| (selects the spread's int32 rounds)
     const int* atom_sys = nullptr;  // system of each atom (owned by the caller)
     DevBuf<int> tile_off;           // nsys + 1 prefix of tiles per system
     DevBuf<int> bin_count, bins;    // [ntiles], [ntiles][bin_cap]
@@ -50,7 +51,7 @@ void pme_grow_bins(PmeDev& pm);
 enum : int { PME_SPREAD = 1, PME_FFT = 2, PME_GATHER = 4, PME_ALL = 7 };
 // Arithmetic of one PME evaluation.
 //   MESH_F32      : FP32 splines/FFT/gather, fixed-point grid and forces (mixed
-//                   form, force-only MD steps);
+//                   form, force-only MD steps; no mesh energy is summed);
 //   MESH_F64_FIXED: FP64 splines/FFT/gather on the same fixed-point grid with
 //                   fixed-point forces (mixed form when the energy is wanted:
 //                   the mesh energy is ~1e4-1e5 kJ/mol and must carry ~1e-9

@@ -130,6 +130,15 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
     if (v != 0.0) atomicAdd(p, v);
 }
 
+// FP32 spread into the fixed-point grid: the contributions of one staged
+// round of SPREAD_THREADS atoms are summed as int32 at 2^-24 and added to the
+// int64 tile sums at the end of the round.  A contribution q w is at most
+// |q| 0.2153 (the largest 3-D weight of the order-5 spline is (115/192)^3),
+// so a round's sum stays below 2^31 / 2^24 = 128 whenever |q| <= 2 for every
+// atom (SPREAD32_MAX_CHARGE); otherwise the int64 path is used throughout.
+constexpr double SPREAD32_SCALE = 16777216.0;  // 2^24
+constexpr double SPREAD32_MAX_CHARGE = 2.0;
+
 template <class R, class G>
 __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __restrict__ tile_off, int nsys,
                                                                  const SysInfo* __restrict__ sys,
@@ -138,10 +147,13 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
                                                                  const double* __restrict__ charge,
                                                                  const int* __restrict__ bin_count,
                                                                  const int* __restrict__ bins, int bin_cap, G* grid,
-                                                                 unsigned* status) {
+                                                                 int small_charges, unsigned* status) {
     constexpr bool FIXED = sizeof(G) == sizeof(long long) && G(0.5) == G(0);  // int64 grid
     using Acc = typename std::conditional<FIXED, unsigned long long, double>::type;
+    constexpr bool I32 = FIXED && sizeof(R) == 4;
+    const bool use32 = I32 && small_charges;
     __shared__ Acc s_acc[TILE_MAX * TILE_MAX * TILE_MAX];
+    __shared__ int s_acc32[I32 ? TILE_MAX * TILE_MAX * TILE_MAX : 1];
     __shared__ int s_src[8], s_pre[9];
     __shared__ int s_idx[3][SPREAD_THREADS];
     __shared__ R s_w[15][SPREAD_THREADS];
@@ -159,7 +171,10 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
         wid[d] = (tcoord[d] == tiles[d] - 1 ? S.grid[d] : lo[d] + TILE) - lo[d];
     }
     const int npts = wid[0] * wid[1] * wid[2];
-    for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) s_acc[i] = (Acc)0;
+    for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) {
+        s_acc[i] = (Acc)0;
+        if (I32) s_acc32[i] = 0;
+    }
     if (threadIdx.x == 0) {
         // Source bins: this tile and the previous one along each axis
         // (periodic), without repeats when an axis has one or two tiles.
@@ -211,6 +226,20 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
                 }
             }
             const R wx = s_w[ix][m];
+            if (I32 && use32) {
+                const float qx = (float)(q * SPREAD32_SCALE) * (float)wx;
+                for (int iy = 0; iy < 5; ++iy) {
+                    if (loc[0][iy] < 0) continue;
+                    const float qxy = qx * (float)s_w[5 + iy][m];
+                    int* row = s_acc32 + (lx * wid[1] + loc[0][iy]) * wid[2];
+                    for (int iz = 0; iz < 5; ++iz) {
+                        if (loc[1][iz] < 0) continue;
+                        const int v = __float2int_rn(qxy * (float)s_w[10 + iz][m]);
+                        if (v) atomicAdd(row + loc[1][iz], v);
+                    }
+                }
+                continue;
+            }
             for (int iy = 0; iy < 5; ++iy) {
                 if (loc[0][iy] < 0) continue;
                 const R wxy = wx * s_w[5 + iy][m];
@@ -224,6 +253,14 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
             }
         }
         __syncthreads();
+        if (I32 && use32) {
+            // Fold the round's int32 sums into the int64 tile sums (2^-24 -> 2^-40).
+            for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) {
+                s_acc[i] += (Acc)((long long)s_acc32[i] * 65536ll);
+                s_acc32[i] = 0;
+            }
+            __syncthreads();
+        }
     }
     G* g = grid + S.qgrid_off;
     const int ny = S.grid[1], nz = S.grid[2];
@@ -440,6 +477,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         const R* mod_z = moduli_all + S.moduli_off[2];
         const double pi2_a2 = PI * PI / (S.alpha * S.alpha);
         const double pref = ONE_4PI_EPS0 / (PI * b.volume);
+        // The FP32 mesh serves force-only evaluations: the influence function
+        // in FP32 (exp2 of the hardware), no energy.  The FP64 meshes evaluate
+        // it in FP64 and sum the energy.
+        constexpr bool F32 = sizeof(R) == 4;
+        float rf[9];
+        for (int k = 0; k < 9; ++k) rf[k] = (float)b.recip[k];
+        const float pi2_a2_log2e = (float)(pi2_a2 * 1.4426950408889634);
+        const float pref_f = (float)pref;
         double eacc = 0.0;
         for (int e = threadIdx.x; e < LINES * n; e += FFT_THREADS) {
             const int li = e % LINES, kx = e / LINES;
@@ -449,6 +494,21 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
             Cx<R>& S = res[li * n + kx];
             if (kx == 0 && y == 0 && kz == 0) {
                 S = Cx<R>{0, 0};
+                continue;
+            }
+            if (F32) {
+                const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
+                const float my = (float)(y < (ny + 1) / 2 ? y : y - ny);
+                const float mz = (float)(kz < (nz + 1) / 2 ? kz : kz - nz);
+                float k2 = 0.0f;
+                for (int a = 0; a < 3; ++a) {
+                    const float kv = mx * rf[a] + my * rf[3 + a] + mz * rf[6 + a];
+                    k2 += kv * kv;
+                }
+                const float mod = (float)mod_x[kx] * (float)mod_y[y] * (float)mod_z[kz];
+                const float infl = pref_f * __builtin_amdgcn_exp2f(-pi2_a2_log2e * k2) / (k2 * mod);
+                S.x = (R)((float)S.x * infl);
+                S.y = (R)((float)S.y * infl);
                 continue;
             }
             const double mx = kx < (nx + 1) / 2 ? kx : kx - nx;
@@ -467,8 +527,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
             S.x = (R)(S.x * infl);
             S.y = (R)(S.y * infl);
         }
-        double et = wave_sum(eacc);
-        if ((threadIdx.x & 63) == 0) acc_add(energy + si * E_SLOTS + E_RECIP, et, ENERGY_SCALE, status);
+        if (!F32) {
+            double et = wave_sum(eacc);
+            if ((threadIdx.x & 63) == 0) acc_add(energy + si * E_SLOTS + E_RECIP, et, ENERGY_SCALE, status);
+        }
         __syncthreads();
         Cx<R>* other = res == bufA ? bufB : bufA;
         res = fft_lds<R>(res, other, n, rad, roots, true, T);
@@ -566,7 +628,7 @@ void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* char
                            pm.bins.p, pm.bin_cap, status);
         hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
                            pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap, qgrid,
-                           status);
+                           pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0, status);
     }
     if (phases & PME_FFT) {
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);
