@@ -45,20 +45,12 @@ struct Math;
 // (this project's own weighted Chebyshev fit, converted to monomials) as a
 // degree-8 polynomial in t = (x - 2.5)/(x + 2.5) on x in [0, 6]: relative error
 // 1.2e-8 in exact arithmetic, 1.6e-7 evaluated in FP32.  exp(-x^2) reuses the
-// hardware exp2.  Reciprocal and reciprocal square root take one Newton step.
+// hardware exp2; reciprocal and reciprocal square root are the hardware's
+// (about 1 ulp), well inside the FP32 rounding of the force sums.
 template <>
 struct Math<float> {
-    static __device__ __forceinline__ float rcp(float x) {
-        float r = __builtin_amdgcn_rcpf(x);
-        float e = fmaf(-x, r, 1.0f);
-        return fmaf(r, e, r);
-    }
-    static __device__ __forceinline__ float rsqrt(float x) {
-        float y = __builtin_amdgcn_rsqf(x);
-        float t = x * y;
-        float e = fmaf(-t, y, 1.0f);
-        return fmaf(0.5f * y, e, y);
-    }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
     static __device__ __forceinline__ void erfc_exp(float x, float& erfc_v, float& ex) {
         float t = (x - 2.5f) * rcp(x + 2.5f);
         t = fminf(t, 0.41176470f);
@@ -334,85 +326,101 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             }
 
             R gx = 0, gy = 0, gz = 0;
+            // Offset of entry `slot` from the lane's i-atom (FP32 minimum image
+            // when the block is near half the box).
+            auto delta = [&](const int slot, R& dx, R& dy, R& dz) {
+                const Q4<R> pj = s_pq[slot];
+                dx = pj.x - xi;
+                dy = pj.y - yi;
+                dz = pj.z - zi;
+                if (mi) {
+                    R s = M::round(dz * ib8);
+                    dx -= s * bv[6];
+                    dy -= s * bv[7];
+                    dz -= s * bv[8];
+                    s = M::round(dy * ib4);
+                    dx -= s * bv[3];
+                    dy -= s * bv[4];
+                    s = M::round(dx * ib0);
+                    dx -= s * bv[0];
+                }
+            };
+            // F/r of the pair (and its energy) for an accepted pair.
+            auto pair_force = [&](const int slot, const R dx, const R dy, const R dz, const R r2) {
+                const Q2<R> sj = s_se[slot];
+                const R inv_r2 = M::rcp(r2);
+                const R inv_r = M::rsqrt(r2);
+                // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
+                const R sig = si + sj.x;
+                const R eps4 = ei * sj.y;
+                const R s2 = sig * sig * inv_r2;
+                const R s6 = s2 * s2 * s2;
+                const R s12 = s6 * s6;
+                R e = eps4 * (s12 - s6);
+                R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
+                const R qq = qi * s_pq[slot].w;
+                if (ELEC == ELEC_EWALD) {
+                    // screened_coulomb (nonbonded.rs:1263-1273)
+                    const R r = r2 * inv_r;
+                    R erfc_v, ex2;
+                    M::erfc_exp(alpha * r, erfc_v, ex2);
+                    const R ec = qq * erfc_v * inv_r;
+                    e += ec;
+                    f -= (ec + qq * two_a_sqrtpi * ex2) * inv_r2;
+                } else if (ELEC == ELEC_REACTION_FIELD) {
+                    // reaction_field (nonbonded.rs:1233-1250)
+                    e += qq * (inv_r + krf * r2 - crf);
+                    f += qq * ((R)2 * krf - inv_r * inv_r2);
+                } else {
+                    // coulomb_pair (nonbonded.rs:1219-1226)
+                    const R ec = qq * inv_r;
+                    e += ec;
+                    f -= ec * inv_r2;
+                }
+                if (E64) {
+                    const Q4<double> pjd = s_pd[slot];
+                    const Q2<double> sjd = s_sd[slot];
+                    double ddx = pjd.x - xid, ddy = pjd.y - yid, ddz = pjd.z - zid;
+                    if (mi) {
+                        double dd[3] = {ddx, ddy, ddz};
+                        wrap_nearest(dd, bx);
+                        ddx = dd[0];
+                        ddy = dd[1];
+                        ddz = dd[2];
+                    }
+                    eacc += pair_energy_fp64<ELEC>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
+                } else if (ENERGY) {
+                    eacc += (double)e;
+                }
+                if (STATS) {
+                    const int ja = s_ja[slot];
+                    ++npair;
+                    hsum += pair_key(min(ia, ja), max(ia, ja));
+                }
+                return f;
+            };
             // Step k: lane (grp, il) pairs i-atom il with entry grp * 8 + (il ^ k);
             // each entry meets every i-atom of the cluster once over the 8 steps.
+            // Pairs whose FP32 r^2 falls in the band around the cutoff are only
+            // marked here and decided in FP64 after the steps.
+            unsigned near = 0;
             auto pair_step = [&](const int k) {
                 const int s8 = il ^ k;
                 const int slot = grp * CLUSTER + s8;
                 if ((vmask >> s8) & 1u) {
-                    const Q4<R> pj = s_pq[slot];
-                    R dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-                    if (mi) {
-                        R s = M::round(dz * ib8);
-                        dx -= s * bv[6];
-                        dy -= s * bv[7];
-                        dz -= s * bv[8];
-                        s = M::round(dy * ib4);
-                        dx -= s * bv[3];
-                        dy -= s * bv[4];
-                        s = M::round(dx * ib0);
-                        dx -= s * bv[0];
-                    }
+                    R dx, dy, dz;
+                    delta(slot, dx, dy, dz);
                     const R r2 = dx * dx + dy * dy + dz * dz;
-                    bool in = !P.use_cutoff || r2 < rc2_lo;
-                    if (!in && r2 < rc2_hi) in = within_cutoff_exact(pos, ia, s_ja[slot], bx, P.periodic, cutoff2);
+                    const bool in = !P.use_cutoff || r2 < rc2_lo;
+                    if (!in && r2 < rc2_hi) near |= 1u << s8;
                     if (in) {
-                        const Q2<R> sj = s_se[slot];
-                        const R inv_r2 = M::rcp(r2);
-                        const R inv_r = M::rsqrt(r2);
-                        // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
-                        const R sig = si + sj.x;
-                        const R eps4 = ei * sj.y;
-                        const R s2 = sig * sig * inv_r2;
-                        const R s6 = s2 * s2 * s2;
-                        const R s12 = s6 * s6;
-                        R e = eps4 * (s12 - s6);
-                        R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
-                        const R qq = qi * pj.w;
-                        if (ELEC == ELEC_EWALD) {
-                            // screened_coulomb (nonbonded.rs:1263-1273)
-                            const R r = r2 * inv_r;
-                            R erfc_v, ex2;
-                            M::erfc_exp(alpha * r, erfc_v, ex2);
-                            const R ec = qq * erfc_v * inv_r;
-                            e += ec;
-                            f -= (ec + qq * two_a_sqrtpi * ex2) * inv_r2;
-                        } else if (ELEC == ELEC_REACTION_FIELD) {
-                            // reaction_field (nonbonded.rs:1233-1250)
-                            e += qq * (inv_r + krf * r2 - crf);
-                            f += qq * ((R)2 * krf - inv_r * inv_r2);
-                        } else {
-                            // coulomb_pair (nonbonded.rs:1219-1226)
-                            const R ec = qq * inv_r;
-                            e += ec;
-                            f -= ec * inv_r2;
-                        }
+                        const R f = pair_force(slot, dx, dy, dz, r2);
                         fx += f * dx;
                         fy += f * dy;
                         fz += f * dz;
                         gx -= f * dx;
                         gy -= f * dy;
                         gz -= f * dz;
-                        if (E64) {
-                            const Q4<double> pjd = s_pd[slot];
-                            const Q2<double> sjd = s_sd[slot];
-                            double ddx = pjd.x - xid, ddy = pjd.y - yid, ddz = pjd.z - zid;
-                            if (mi) {
-                                double dd[3] = {ddx, ddy, ddz};
-                                wrap_nearest(dd, bx);
-                                ddx = dd[0];
-                                ddy = dd[1];
-                                ddz = dd[2];
-                            }
-                            eacc += pair_energy_fp64<ELEC>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
-                        } else if (ENERGY) {
-                            eacc += (double)e;
-                        }
-                        if (STATS) {
-                            const int ja = s_ja[slot];
-                            ++npair;
-                            hsum += pair_key(min(ia, ja), max(ia, ja));
-                        }
                     }
                 }
             };
@@ -431,6 +439,27 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             pair_step(6);
             rotate_j<6>(gx, gy, gz);
             pair_step(7);
+            // The cutoff band: the reference's FP64 decision (rare pairs; their
+            // j-forces go straight to the slot-ordered accumulator).
+            while (__ballot(near != 0u)) {
+                if (near) {
+                    const int s8 = __ffs(near) - 1;
+                    near &= near - 1u;
+                    const int slot = grp * CLUSTER + s8;
+                    if (within_cutoff_exact(pos, ia, s_ja[slot], bx, P.periodic, cutoff2)) {
+                        R dx, dy, dz;
+                        delta(slot, dx, dy, dz);
+                        const R f = pair_force(slot, dx, dy, dz, dx * dx + dy * dy + dz * dz);
+                        fx += f * dx;
+                        fy += f * dy;
+                        fz += f * dz;
+                        const int js = s_js[slot];
+                        acc_add(fslot + js, (double)(-f * dx), FORCE_SCALE, status);
+                        acc_add(fslot + P.nslots + js, (double)(-f * dy), FORCE_SCALE, status);
+                        acc_add(fslot + 2 * P.nslots + js, (double)(-f * dz), FORCE_SCALE, status);
+                    }
+                }
+            }
             // After step 7 the lane holds the accumulator of entry grp * 8 + (il ^ 7).
             const int jmine = grp * CLUSTER + (il ^ (CLUSTER - 1));
             if (s_ja[jmine] >= 0) {
