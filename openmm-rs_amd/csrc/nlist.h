@@ -25,6 +25,8 @@ namespace ommhip {
 constexpr int IBLOCK = 32;  // atoms per block (the unit of the coarse list)
 constexpr int CLUSTER = 8;  // i-atoms per cluster (the unit of the pair kernel's lists)
 constexpr int CPB = IBLOCK / CLUSTER;  // clusters per block
+constexpr int SUPER = 64;   // blocks per superblock (the coarse level of the candidate search)
+constexpr int SUPER_MIN = 256;  // systems with more blocks search superblocks first
 constexpr int JTILE = 64;   // j-slots per tile
 constexpr int TPT = 4;      // tiles of one cluster per pair-kernel task
 constexpr int MAXEX = 8;    // exclusion partners per atom handled by the pair kernel
@@ -52,6 +54,13 @@ struct NlistDev {
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
+    // Superblocks of SUPER consecutive blocks (systems with more than
+    // SUPER_MIN blocks): bounding boxes, per-system offsets and owners.
+    int nsuper = 0;
+    bool use_super = false;
+    DevBuf<int> super_off, super_sys;
+    DevBuf<double> super_center;
+    DevBuf<float> super_half;
     DevBuf<int2> clist;         // cluster lists [nclusters][cap] of (sorted slot, atom)
     DevBuf<int> ccount;
     DevBuf<int> tasks, ntasks;  // pair-kernel tasks (cluster << 8 | first tile) and their count

@@ -330,6 +330,48 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
     }
 }
 
+// Bounding box of each superblock (SUPER consecutive blocks of a system)
+// from its blocks' boxes, for flagged systems.  One wave per superblock.
+__global__ __launch_bounds__(64) void k_super_bounds(const int* __restrict__ super_off,
+                                                     const int* __restrict__ super_sys,
+                                                     const SysInfo* __restrict__ sys,
+                                                     const double* __restrict__ center,
+                                                     const float* __restrict__ half, double* super_center,
+                                                     float* super_half, const int* flag) {
+    if (!flag[0]) return;
+    const int sb = blockIdx.x;
+    const int si = super_sys[sb];
+    if (!flag[1 + si]) return;
+    const SysInfo& S = sys[si];
+    const int lane = threadIdx.x;
+    const int first = S.block_off + (sb - super_off[si]) * SUPER;
+    const int J = first + lane;
+    double mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = 1e300;
+        mx[k] = -1e300;
+    }
+    if (J < S.block_off + S.nblocks) {
+        double d[3] = {center[3 * J] - center[3 * first], center[3 * J + 1] - center[3 * first + 1],
+                       center[3 * J + 2] - center[3 * first + 2]};
+        wrap_fast(d, S.box);
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = d[k] - (double)half[3 * J + k];
+            mx[k] = d[k] + (double)half[3 * J + k];
+        }
+    }
+    for (int off = 1; off < 64; off <<= 1)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
+            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
+        }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+            super_center[3 * sb + k] = center[3 * first + k] + 0.5 * (mn[k] + mx[k]);
+            super_half[3 * sb + k] = (float)(0.5 * (mx[k] - mn[k]) * (1.0 + 1e-6) + 1e-6);
+        }
+}
+
 // Cluster lists, one wave per cluster C of block I = C / CPB (CLIST_WAVES
 // clusters per workgroup).  Candidate blocks J > I of the same system pass
 // when their bounding boxes come within reach of the cluster's box; the atoms
@@ -381,8 +423,9 @@ constexpr int CLIST_WAVES = 16;  // clusters (one per wave) per workgroup of k_c
 __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     const double* __restrict__ pos, const int* __restrict__ sorted_atom, const int* __restrict__ block_sys,
     const SysInfo* __restrict__ sys, const double* __restrict__ center, const float* __restrict__ half,
-    double reach, int cap, int nclusters, int2* clist, int* ccount, int* tasks, int* ntasks, unsigned* status,
-    const int* flag) {
+    const int* __restrict__ super_off, const double* __restrict__ super_center,
+    const float* __restrict__ super_half, double reach, int cap, int nclusters, int2* clist, int* ccount,
+    int* tasks, int* ntasks, unsigned* status, const int* flag) {
     __shared__ int s_runs[CLIST_WAVES];
     __shared__ int s_base;
     if (!flag[0]) return;
@@ -435,10 +478,11 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
         }
         append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, count, below);
 
-        for (int J0 = I + 1; J0 < jend; J0 += 64) {
+        // Blocks [J0, min(J0 + 64, jlim)) whose boxes come within reach.
+        auto scan_blocks = [&](const int J0, const int jlim) {
             const int J = J0 + lane;
             bool pass = false;
-            if (J < jend) {
+            if (J < jlim) {
                 double dc[3], h[3];
                 for (int k = 0; k < 3; ++k) {
                     dc[k] = center[3 * J + k] - cq[k];
@@ -464,6 +508,31 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
                 append_entry(slot_a, atom_a, pa, first, cq, hq, b, reach2, cap, out, count, below);
                 append_entry(slot_b, atom_b, pb, first, cq, hq, b, reach2, cap, out, count, below);
             }
+        };
+        if (S.nblocks > SUPER_MIN) {
+            // Superblocks from I's own on, then the blocks of those in reach.
+            const int sb0 = super_off[si], nsb = super_off[si + 1] - sb0;
+            for (int B0 = (I - S.block_off) / SUPER; B0 < nsb; B0 += 64) {
+                const int B = B0 + lane;
+                bool pass = false;
+                if (B < nsb) {
+                    double dc[3], h[3];
+                    for (int k = 0; k < 3; ++k) {
+                        dc[k] = super_center[3 * (sb0 + B) + k] - cq[k];
+                        h[k] = hq[k] + (double)super_half[3 * (sb0 + B) + k];
+                    }
+                    pass = box_gap2(dc, h, b) < reach2;
+                }
+                unsigned long long m = __ballot(pass);
+                while (m) {
+                    const int Bk = B0 + __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    const int lo = S.block_off + Bk * SUPER;
+                    scan_blocks(max(lo, I + 1), min(lo + SUPER, jend));
+                }
+            }
+        } else {
+            for (int J0 = I + 1; J0 < jend; J0 += 64) scan_blocks(J0, jend);
         }
         if (count > cap) {
             if (lane == 0) atomicOr(status, ST_LIST_OVERFLOW);
@@ -537,7 +606,8 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.cutoff = cutoff;
     nl.skin = skin;
     std::vector<int> eptr{0}, eidx, uptr{0}, uatoms, upos, usys, rank_all, block_sys, sorted, slot_of, atom_sys;
-    std::vector<int> block_off{0}, chunk_off{0};
+    std::vector<int> block_off{0}, chunk_off{0}, super_off{0}, super_sys;
+    nl.use_super = false;
     int natoms = 0, nblocks = 0, nchunks = 0, ncells = 0, max_npad = 0;
     nl.uniform = -1;
     nl.closed = 1;
@@ -639,6 +709,10 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
             if (k < n) slot_of.push_back(S.slot_off + k);
         }
         for (int k = 0; k < nb; ++k) block_sys.push_back(si);
+        const int nsb = (nb + SUPER - 1) / SUPER;
+        for (int k = 0; k < nsb; ++k) super_sys.push_back(si);
+        super_off.push_back(super_off.back() + nsb);
+        if (nb > SUPER_MIN) nl.use_super = true;
         atom_sys.insert(atom_sys.end(), n, si);
         natoms += n;
         nblocks += nb;
@@ -693,6 +767,11 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.block_center.alloc(3 * (size_t)nblocks);
     nl.block_half.alloc(3 * (size_t)nblocks);
     nl.block_minimg.alloc(nblocks);
+    nl.nsuper = super_off.back();
+    nl.super_off.upload(super_off, s);
+    nl.super_sys.upload(super_sys, s);
+    nl.super_center.alloc(3 * (size_t)nl.nsuper);
+    nl.super_half.alloc(3 * (size_t)nl.nsuper);
     nl.pos_build.alloc(3 * (size_t)natoms);
     nl.cap = std::min(nl.cap, 255 * JTILE);
     nl.clist.alloc((size_t)nl.nclusters * nl.cap);
@@ -752,9 +831,13 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p,
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
                        nl.block_minimg.p, nl.pos_build.p, nl.ntasks.p, flag);
+    if (nl.use_super)
+        hipLaunchKernelGGL(k_super_bounds, dim3(nl.nsuper), dim3(64), 0, s, nl.super_off.p, nl.super_sys.p, sys_dev,
+                           nl.block_center.p, nl.block_half.p, nl.super_center.p, nl.super_half.p, flag);
     hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
-                       nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p, reach, nl.cap,
-                       nl.nclusters, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
+                       nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p,
+                       nl.super_off.p, nl.super_center.p, nl.super_half.p, reach, nl.cap, nl.nclusters, nl.clist.p,
+                       nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
     OMM_HIP(hipGetLastError());
 }
 
