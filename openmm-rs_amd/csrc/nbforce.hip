@@ -591,7 +591,22 @@ void NbOperator::grow(unsigned status) {
 }
 
 bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t s) {
-    if (ntotal_ == 0 || !cfg_[0].include_direct) return false;
+    if (ntotal_ == 0) return false;
+    // PME spread bins: grow until the current positions fit.
+    if (cfg_[0].method == OMM_NB_PME) {
+        for (int attempt = 0; attempt < 8; ++attempt) {
+            fill_async<unsigned>(status, 1, 0u, s);
+            pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, nullptr, nullptr, nullptr,
+                    nullptr, status, s, PME_SPREAD);
+            unsigned st = 0;
+            OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            OMM_HIP(hipStreamSynchronize(s));
+            if (!(st & ST_BIN_OVERFLOW)) break;
+            pme_grow_bins(pme_);
+        }
+        fill_async<unsigned>(status, 1, 0u, s);
+    }
+    if (!cfg_[0].include_direct) return false;
     int max_npad = 0;
     for (const SysInfo& S : sys_h_) max_npad = std::max(max_npad, S.nblocks * IBLOCK);
     bool changed = false;

@@ -348,8 +348,8 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             // F/r of the pair (and its energy) for an accepted pair.
             auto pair_force = [&](const int slot, const R dx, const R dy, const R dz, const R r2) {
                 const Q2<R> sj = s_se[slot];
-                const R inv_r2 = M::rcp(r2);
                 const R inv_r = M::rsqrt(r2);
+                const R inv_r2 = inv_r * inv_r;
                 // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
                 const R sig = si + sj.x;
                 const R eps4 = ei * sj.y;

@@ -66,8 +66,9 @@ __device__ __forceinline__ void stencil(const double* p, const Box& b, const int
 }
 
 // Spread without atomics on the grid.  The grid of every system is cut into tiles
-// of 8 points per axis (the last tile of an axis takes the remainder, so tile
-// widths lie in [4, 11]).  Each step, k_pme_bin computes every charged atom's
+// of ptile = 8 or 6 points per axis (the last tile of an axis takes the
+// remainder, so tile widths lie in [4, 11]; smaller tiles give small grids
+// more workgroups).  Each step, k_pme_bin computes every charged atom's
 // stencil (base point and weights, stored per atom) and appends the atom to
 // the bin of the tile holding its base point.  An atom's 5-point stencil then
 // only reaches its own tile and the next one along each axis, so the points
@@ -76,11 +77,14 @@ __device__ __forceinline__ void stencil(const double* p, const Box& b, const int
 // contributions in LDS (fixed point: the order of the adds does not matter)
 // and writes every point of the tile with plain stores.  Each point has one
 // owner, and the whole grid is overwritten.
-constexpr int TILE = 8;
 constexpr int TILE_MAX = 11;
 constexpr int SPREAD_THREADS = 256;
+constexpr int SPREAD_MIN_TILES = 1024;  // smaller tile edges until a batch has this many tiles
 
-__device__ __forceinline__ int tile_of(int i, int tiles) { return min(i / TILE, tiles - 1); }
+__device__ __forceinline__ int tile_of(int i, int tiles, int edge) { return min(i / edge, tiles - 1); }
+
+// Tiles along an axis of n points for edge e: widths e ... (last) in [4, 11].
+inline int tiles_for(int n, int e) { return e == 8 ? std::max(1, (n + 4) / 8) : std::max(1, n / e); }
 
 // Atoms are visited in the neighbour list's sorted order, so the lanes of a
 // wave mostly fall into one or two tiles; each distinct tile of a wave takes
@@ -103,8 +107,10 @@ __global__ void k_pme_bin(int nslots, const int* __restrict__ order, const doubl
         for (int d = 0; d < 3; ++d) st_idx[4 * a + d] = idx[d];
         for (int d = 0; d < 3; ++d)
             for (int k = 0; k < 5; ++k) st_w[16 * a + 5 * d + k] = w[d][k];
-        tile = S.tile_off + (tile_of(idx[0], S.tiles[0]) * S.tiles[1] + tile_of(idx[1], S.tiles[1])) * S.tiles[2] +
-               tile_of(idx[2], S.tiles[2]);
+        tile = S.tile_off +
+               (tile_of(idx[0], S.tiles[0], S.ptile) * S.tiles[1] + tile_of(idx[1], S.tiles[1], S.ptile)) *
+                   S.tiles[2] +
+               tile_of(idx[2], S.tiles[2], S.ptile);
     }
     unsigned long long pending = __ballot(tile >= 0);
     while (pending) {
@@ -167,8 +173,8 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
     const int tcoord[3] = {tx, ty, tz}, tiles[3] = {T0, T1, T2};
     int lo[3], wid[3];
     for (int d = 0; d < 3; ++d) {
-        lo[d] = tcoord[d] * TILE;
-        wid[d] = (tcoord[d] == tiles[d] - 1 ? S.grid[d] : lo[d] + TILE) - lo[d];
+        lo[d] = tcoord[d] * S.ptile;
+        wid[d] = (tcoord[d] == tiles[d] - 1 ? S.grid[d] : lo[d] + S.ptile) - lo[d];
     }
     const int npts = wid[0] * wid[1] * wid[2];
     for (int i = threadIdx.x; i < npts; i += SPREAD_THREADS) {
@@ -751,12 +757,28 @@ void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* ato
     // Spread tiles and bins.
     std::vector<int> toff{0};
     int bin_cap = 64;
+    int edge = 8;
+    for (int e : {8, 6}) {
+        long long total = 0;
+        for (const SysInfo& S : sys)
+            total += (long long)tiles_for(S.grid[0], e) * tiles_for(S.grid[1], e) * tiles_for(S.grid[2], e);
+        edge = e;
+        if (total >= SPREAD_MIN_TILES) break;
+    }
     for (SysInfo& S : sys) {
-        for (int a = 0; a < 3; ++a) S.tiles[a] = std::max(1, (S.grid[a] + TILE / 2) / TILE);
+        S.ptile = edge;
+        for (int a = 0; a < 3; ++a) S.tiles[a] = tiles_for(S.grid[a], edge);
         S.tile_off = toff.back();
         const int nt = S.tiles[0] * S.tiles[1] * S.tiles[2];
         toff.push_back(toff.back() + nt);
-        bin_cap = std::max(bin_cap, 2 * ((S.n + nt - 1) / nt) + 64);
+        // Twice the atoms of the largest tile at the system's mean density
+        // (the last tile of an axis is up to twice as wide as the others).
+        double largest = 1.0;
+        for (int a = 0; a < 3; ++a) {
+            const int last = S.grid[a] - (S.tiles[a] - 1) * edge;
+            largest *= (double)std::max(last, std::min(edge, S.grid[a])) / (double)S.grid[a];
+        }
+        bin_cap = std::max(bin_cap, (int)(2.0 * largest * S.n) + 64);
     }
     pm.ntiles = toff.back();
     pm.tile_off.upload(toff, s);
