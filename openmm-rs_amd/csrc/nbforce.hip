@@ -735,10 +735,12 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
     const NbSettings& c = cfg_[0];
     const bool pme = c.method == OMM_NB_PME;
     ms[0] = timed([&] { nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true); });
+    // The pair kernel alone; its slot-ordered sums are folded in once after.
     ms[1] = timed([&] {
         launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, sys_dev_.p, force_fx, force_d, energy_fx, energy_d,
-                     status, nullptr, pair_grid_, s);
+                     status, nullptr, pair_grid_, s, false);
     });
+    launch_slot_forces(mixed_, nl_, force_fx, force_d, s);
     ms[2] = timed([&] {
         launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0, pme ? 1 : 0, alpha_[0],
                           force_fx, force_d, energy_fx, energy_d, status, s);

@@ -45,7 +45,11 @@ struct ExceptionDev {
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
                   const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d,
                   long long* energy_fx, double* energy_d, unsigned* status, unsigned long long* stats,
-                  int grid, hipStream_t s);
+                  int grid, hipStream_t s, bool combine = true);
+
+// The second kernel alone: adds the slot-ordered pair forces into force_fx
+// (mixed) or force_d and clears them (after launch_pairs with combine = false).
+void launch_slot_forces(bool mixed, const NlistDev& nl, long long* force_fx, double* force_d, hipStream_t s);
 
 // Exceptions (nonbonded.rs:995-1036), FP64 math.  has_pme selects the erf
 // reciprocal correction with particle charges.

@@ -558,67 +558,82 @@ __global__ void k_slot_forces(const int* __restrict__ sorted_atom, int nslots, A
 template <class R, class A, int ELEC, bool STATS, bool ENERGY>
 void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* sh, const R* e2,
                 const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
-                unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
+                unsigned* status, unsigned long long* stats, int grid, bool combine, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
                        part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.excl_sorted.p, nl.block_center.p,
                        nl.block_minimg.p, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, nl.block_sys.p, sys_dev, p,
                        slot_buffer<A>(nl), energy, status, stats, nl.flag.p, nl.nsys + 1);
-    hipLaunchKernelGGL(k_slot_forces<A>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p, nl.nslots,
-                       slot_buffer<A>(nl), force);
+    if (combine)
+        hipLaunchKernelGGL(k_slot_forces<A>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p,
+                           nl.nslots, slot_buffer<A>(nl), force);
 }
 
 template <class R, class A, int ELEC>
 void launch_pairs_t(bool stats_on, bool energy_on, const NlistDev& nl, const PairParams& p, const R* qk, const R* sh,
                     const R* e2, const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
-                    unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
+                    unsigned* status, unsigned long long* stats, int grid, bool combine, hipStream_t s) {
     if (stats_on)
-        launch_one<R, A, ELEC, true, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid, s);
+        launch_one<R, A, ELEC, true, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
+                                            combine, s);
     else if (energy_on)
-        launch_one<R, A, ELEC, false, true>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid, s);
+        launch_one<R, A, ELEC, false, true>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
+                                            combine, s);
     else
         launch_one<R, A, ELEC, false, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
-                                             s);
+                                             combine, s);
 }
 
 }  // namespace
 
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
                   const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx,
-                  double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s) {
+                  double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s,
+                  bool combine) {
     bool st = stats != nullptr;
     if (mixed) {
         switch (p.elec) {
             case ELEC_EWALD:
                 launch_pairs_t<float, long long, ELEC_EWALD>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part,
-                                                             pos, sys_dev, force_fx, energy_fx, status, stats, grid, s);
+                                                             pos, sys_dev, force_fx, energy_fx, status, stats, grid,
+                                                             combine, s);
                 break;
             case ELEC_REACTION_FIELD:
                 launch_pairs_t<float, long long, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk.p, part.sh.p,
                                                                       part.e2.p, part, pos, sys_dev, force_fx,
-                                                                      energy_fx, status, stats, grid, s);
+                                                                      energy_fx, status, stats, grid, combine, s);
                 break;
             default:
                 launch_pairs_t<float, long long, ELEC_COULOMB>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p,
                                                                part, pos, sys_dev, force_fx, energy_fx, status, stats,
-                                                               grid, s);
+                                                               grid, combine, s);
         }
     } else {
         switch (p.elec) {
             case ELEC_EWALD:
                 launch_pairs_t<double, double, ELEC_EWALD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p, part.e2_d.p,
-                                                           part, pos, sys_dev, force_d, energy_d, status, stats, grid, s);
+                                                           part, pos, sys_dev, force_d, energy_d, status, stats, grid, combine, s);
                 break;
             case ELEC_REACTION_FIELD:
                 launch_pairs_t<double, double, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
                                                                     part.e2_d.p, part, pos, sys_dev, force_d, energy_d,
-                                                                    status, stats, grid, s);
+                                                                    status, stats, grid, combine, s);
                 break;
             default:
                 launch_pairs_t<double, double, ELEC_COULOMB>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
                                                              part.e2_d.p, part, pos, sys_dev, force_d, energy_d, status,
-                                                             stats, grid, s);
+                                                             stats, grid, combine, s);
         }
     }
+    OMM_HIP(hipGetLastError());
+}
+
+void launch_slot_forces(bool mixed, const NlistDev& nl, long long* force_fx, double* force_d, hipStream_t s) {
+    if (mixed)
+        hipLaunchKernelGGL(k_slot_forces<long long>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p,
+                           nl.nslots, nl.fslot_fx.p, force_fx);
+    else
+        hipLaunchKernelGGL(k_slot_forces<double>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p,
+                           nl.nslots, nl.fslot_d.p, force_d);
     OMM_HIP(hipGetLastError());
 }
 
