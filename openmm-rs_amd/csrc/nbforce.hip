@@ -262,7 +262,6 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
             throw Error(OMM_ERR_INVALID,
                         "the device operator implements the PME and CutoffPeriodic methods; "
                         "NoCutoff, CutoffNonPeriodic, Ewald and LJPME are not available on the device yet");
-        if (c.switching) throw Error(OMM_ERR_INVALID, "the switching function is not available on the device yet");
         if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
         validate_box(box);
         for (int a = 0; a < 3; ++a)
@@ -342,6 +341,8 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     pp_.use_cutoff = 1;
     pp_.cutoff = c0.cutoff;
     pp_.alpha = alpha_[0];
+    pp_.switching = c0.switching ? 1 : 0;
+    pp_.switch_distance = c0.switching_distance;
     if (c0.method == OMM_NB_PME) {
         pp_.elec = ELEC_EWALD;
     } else {

@@ -16,6 +16,8 @@ struct PairParams {
     double alpha = 0.0;
     double cutoff = 0.0;
     double krf = 0.0, crf = 0.0;  // reaction field (nonbonded.rs:1233-1250)
+    int switching = 0;            // LJ switching function (nonbonded.rs:887-899)
+    double switch_distance = 0.0;
 };
 
 // Per-particle parameters in the form the kernels use.

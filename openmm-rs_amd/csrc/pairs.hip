@@ -94,6 +94,16 @@ __device__ __noinline__ bool within_cutoff_exact(const double* __restrict__ pos,
     return r2 < cutoff2;
 }
 
+// The reference's LJ switch (nonbonded.rs:887-899): S(t) = 1 + t^3 (-10 + t (15 - 6 t))
+// and dS/dr for t = (r - r_switch) / (r_cut - r_switch), applied past r_switch.
+template <class R>
+__device__ __forceinline__ void lj_switch(R r, const PairParams& P, R& s, R& ds) {
+    const R width = (R)(P.cutoff - P.switch_distance);
+    const R t = (r - (R)P.switch_distance) / width;
+    s = (R)1 + t * t * t * ((R)-10 + t * ((R)15 - (R)6 * t));
+    ds = (R)-30 * t * t * ((R)1 - t) * ((R)1 - t) / width;
+}
+
 // FP64 energy of one pair (nonbonded.rs:1203-1214, 1219-1273) from offsets.
 template <int ELEC>
 __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double dz, double sig, double eps4,
@@ -105,6 +115,11 @@ __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double 
         const double s2 = sig * sig / r2;
         const double s6 = s2 * s2 * s2;
         e = eps4 * (s6 * s6 - s6);
+        if (P.switching && r > P.switch_distance) {
+            double sw, dsw;
+            lj_switch<double>(r, P, sw, dsw);
+            e *= sw;
+        }
     }
     if (qq != 0.0) {
         if (ELEC == ELEC_EWALD) e += qq * erfc(P.alpha * r) / r;
@@ -358,6 +373,15 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 const R s12 = s6 * s6;
                 R e = eps4 * (s12 - s6);
                 R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
+                if (P.switching) {
+                    const R r = r2 * inv_r;
+                    if (r > (R)P.switch_distance) {
+                        R sw, dsw;
+                        lj_switch<R>(r, P, sw, dsw);
+                        f = sw * f + e * dsw * inv_r;
+                        e *= sw;
+                    }
+                }
                 const R qq = qi * s_pq[slot].w;
                 if (ELEC == ELEC_EWALD) {
                     // screened_coulomb (nonbonded.rs:1263-1273)

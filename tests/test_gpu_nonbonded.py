@@ -98,6 +98,16 @@ def test_reaction_field_parity():
     check_against_oracle(s)
 
 
+@pytest.mark.parametrize("method", ["PME", "CutoffPeriodic"])
+def test_switching_function_parity(method):
+    """LJ switching from 0.75 nm to the 0.9 nm cutoff (nonbonded.rs:887-899),
+    including the switched dispersion correction."""
+    s = water(1000, 3.1, seed=13, method=method)
+    s.nonbonded.set_use_switching_function(True)
+    s.nonbonded.set_switching_distance(0.75)
+    check_against_oracle(s)
+
+
 def test_groups_split_and_none():
     s = water(500, 2.5, seed=2)
     s.nonbonded.set_force_group(1)
