@@ -137,6 +137,7 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
         "steps": steps,
         "pairs_per_step": npairs,
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "launch_tuning": batch.launch_tuning(),
         "plain_fp64_per_system": {"replica_steps_per_s": round(plain, 2), "steps_per_replica": plain_steps},
         "speedup_mixed_batched_over_plain": round(value / plain, 2),
     }
@@ -349,6 +350,7 @@ def main():
             "pair_slot_efficiency": round(npairs / max(1, tiles * TILE_SLOTS), 4),
         },
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "launch_tuning": ctx.launch_tuning(),
         "plain_fp64": {"value": round(plain, 3), "unit": "ns/day", "steps": args.plain_steps,
                        "speedup_mixed_over_plain": round(value / plain, 2)},
     }

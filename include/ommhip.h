@@ -310,6 +310,20 @@ omm_status omm_ctx_barostat_attempt(omm_ctx* ctx);
  * Pair-slot efficiency = pairs / (tiles * 512). */
 omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries);
 
+/* Launch shapes picked by the autotuner when the context is first prepared
+ * (no counterpart in the reference): the spread tile edge (grid points), the
+ * lines per FFT workgroup and the pair kernel's grid, each the fastest of its
+ * candidates on the context's own positions, with that candidate's time.
+ * None of them changes a result bit. */
+typedef struct omm_launch_tuning {
+    int32_t spread_tile_edge;
+    int32_t fft_lines_per_workgroup;
+    int32_t pair_grid;
+    double spread_ms, fft_ms, pair_ms;
+} omm_launch_tuning;
+
+omm_status omm_ctx_launch_tuning(omm_ctx* ctx, omm_launch_tuning* out);
+
 /* Context::get_state (context.rs:930-1105) with positions, velocities, forces
  * and potential energy; any output pointer may be NULL. */
 omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* forces,
@@ -399,6 +413,7 @@ omm_status omm_batch_get_box(omm_batch* b, int32_t replica, double box[9]);
  * per-phase device times of omm_ctx_profile, over the whole batch. */
 omm_status omm_batch_list_stats(omm_batch* b, int64_t* tiles, int64_t* entries);
 omm_status omm_batch_profile(omm_batch* b, int32_t reps, double phase_ms[8], int64_t* num_pairs);
+omm_status omm_batch_launch_tuning(omm_batch* b, omm_launch_tuning* out);
 
 omm_status omm_batch_destroy(omm_batch* b);
 

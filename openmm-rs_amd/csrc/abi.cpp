@@ -267,6 +267,22 @@ omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries) {
     CTX_CALL(ctx->ctx.list_stats(tiles, entries));
 }
 
+namespace {
+void fill_tuning(const ommhip::NbOperator::Tuning& t, omm_launch_tuning* out) {
+    if (!out) throw ommhip::Error(OMM_ERR_INVALID, "null output");
+    out->spread_tile_edge = t.spread_edge;
+    out->fft_lines_per_workgroup = t.fft_cap;
+    out->pair_grid = t.pair_grid;
+    out->spread_ms = t.spread_ms;
+    out->fft_ms = t.fft_ms;
+    out->pair_ms = t.pair_ms;
+}
+}  // namespace
+
+omm_status omm_ctx_launch_tuning(omm_ctx* ctx, omm_launch_tuning* out) {
+    CTX_CALL(fill_tuning(ctx->ctx.tuning(), out));
+}
+
 omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* forces, double* potential_energy,
                              int64_t* step_count) {
     CTX_CALL(ctx->ctx.get_state(0, pos, vel, forces, potential_energy, step_count));
@@ -362,6 +378,10 @@ omm_status omm_batch_get_box(omm_batch* b, int32_t replica, double box[9]) {
 
 omm_status omm_batch_list_stats(omm_batch* b, int64_t* tiles, int64_t* entries) {
     BATCH_CALL(b->ctx.list_stats(tiles, entries));
+}
+
+omm_status omm_batch_launch_tuning(omm_batch* b, omm_launch_tuning* out) {
+    BATCH_CALL(fill_tuning(b->ctx.tuning(), out));
 }
 
 omm_status omm_batch_profile(omm_batch* b, int32_t reps, double phase_ms[8], int64_t* num_pairs) {

@@ -84,6 +84,11 @@ public:
         prepare();
         nb_->list_stats(tiles, entries);
     }
+    // The autotuner's choices (made by the first preparation).
+    NbOperator::Tuning tuning() {
+        prepare();
+        return nb_->tuning();
+    }
     // Per-phase average device times (ms): the six NbOperator::profile
     // phases, [6] bonds + angles, [7] LangevinMiddle update; plus the
     // in-cutoff pair count of the current positions.  State is preserved.

@@ -139,6 +139,17 @@ public:
     // longest j-list (host-side, outside graph capture).  Returns true when the
     // list storage was reallocated (captured graphs must then be rebuilt).
     bool calibrate(const double* pos_dev, unsigned* status, hipStream_t s);
+    // Launch-shape autotuner, run once by the first calibration: the spread
+    // tile edge, the FFT lines per workgroup and the pair kernel's grid, each
+    // picked by timing the candidates on the current positions.  None of
+    // them changes a result bit.
+    void autotune(const double* pos_dev, unsigned* status, hipStream_t s);
+    struct Tuning {
+        int spread_edge = 0, fft_cap = 0, pair_grid = 0;
+        int pair_scale = 1;  // pair grid per calibrated task count: 1, 2 (doubled) or -2 (halved)
+        double spread_ms = 0, fft_ms = 0, pair_ms = 0;
+    };
+    const Tuning& tuning() const { return tuning_; }
     void force_rebuild(hipStream_t s);
     // Tile count and total j-list length of the current list (host read-back).
     void list_stats(int64_t* tiles, int64_t* entries);
@@ -149,6 +160,11 @@ private:
     // Uploads the effective particle and exception parameters and the
     // parameter-dependent energy constants of every system.
     void refresh_parameters();
+    // Copies the launch layout fields of sys_h_ (spread tiles, FFT lines)
+    // into the device table, keeping its boxes.
+    void patch_layout(hipStream_t s);
+    // Grows the spread bins until the positions fit.
+    void fit_bins(const double* pos_dev, unsigned* status, hipStream_t s);
     void upload_parameters(hipStream_t s);
     void check_box(const double* box) const;
     double dispersion_coefficient(int s) const;
@@ -164,6 +180,8 @@ private:
     mutable std::vector<double> disp_coeff_;
     mutable std::vector<char> disp_cached_;
     int pair_grid_ = 1;
+    bool tuned_ = false;
+    Tuning tuning_;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;  // PME branch (default priority, like stream_)
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;

@@ -591,22 +591,134 @@ void NbOperator::grow(unsigned status) {
     if ((status & ST_BIN_OVERFLOW) && cfg_[0].method == OMM_NB_PME) pme_grow_bins(pme_);
 }
 
+void NbOperator::fit_bins(const double* pos_dev, unsigned* status, hipStream_t s) {
+    if (cfg_[0].method != OMM_NB_PME) return;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        fill_async<unsigned>(status, 1, 0u, s);
+        pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, nullptr, nullptr, nullptr, nullptr,
+                status, s, PME_SPREAD);
+        unsigned st = 0;
+        OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        OMM_HIP(hipStreamSynchronize(s));
+        if (!(st & ST_BIN_OVERFLOW)) break;
+        pme_grow_bins(pme_);
+    }
+    fill_async<unsigned>(status, 1, 0u, s);
+}
+
+void NbOperator::patch_layout(hipStream_t s) {
+    std::vector<SysInfo> dev(sys_h_.size());
+    OMM_HIP(hipMemcpyAsync(dev.data(), sys_dev_.p, dev.size() * sizeof(SysInfo), hipMemcpyDeviceToHost, s));
+    OMM_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < dev.size(); ++k) {
+        const SysInfo& h = sys_h_[k];
+        SysInfo& d = dev[k];
+        d.ptile = h.ptile;
+        d.tile_off = h.tile_off;
+        for (int a = 0; a < 3; ++a) {
+            d.tiles[a] = h.tiles[a];
+            for (int p = 0; p < 2; ++p) d.fft_lines[p][a] = h.fft_lines[p][a];
+        }
+    }
+    sys_dev_.upload(dev, s);
+}
+
+void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s) {
+    tuned_ = true;
+    hipEvent_t t0, t1;
+    OMM_HIP(hipEventCreate(&t0));
+    OMM_HIP(hipEventCreate(&t1));
+    auto timed = [&](auto&& body) {
+        body();  // warm-up
+        OMM_HIP(hipEventRecord(t0, s));
+        for (int r = 0; r < 5; ++r) body();
+        OMM_HIP(hipEventRecord(t1, s));
+        OMM_HIP(hipEventSynchronize(t1));
+        float v = 0.f;
+        OMM_HIP(hipEventElapsedTime(&v, t0, t1));
+        return (double)v / 5;
+    };
+    const NbSettings& c = cfg_[0];
+    // Scratch accumulators: the FP64 mesh's convolution always sums the
+    // energy, and the pair timings add forces.
+    const size_t eslots = (size_t)E_SLOTS * sys_h_.size();
+    DevBuf<long long> f_fx, e_fx;
+    DevBuf<double> f_d, e_d;
+    if (mixed_) {
+        f_fx.alloc(3 * (size_t)ntotal_);
+        e_fx.alloc(eslots);
+    } else {
+        f_d.alloc(3 * (size_t)ntotal_);
+        e_d.alloc(eslots);
+    }
+    if (c.method == OMM_NB_PME) {
+        const int mode = mesh_mode(false);
+        const int prec = mode == MESH_F32 ? 0 : 1;
+        auto phase = [&](int ph) {
+            return [&, ph] {
+                pme_run(pme_, mode, nl_, pos_dev, part_.charge.p, sys_dev_.p, f_fx.p, f_d.p, e_fx.p, e_d.p, status,
+                        s, ph);
+            };
+        };
+        // Spread tile edge.
+        double best = 1e30;
+        for (int e : {8, 6}) {
+            pme_set_spread_edge(pme_, sys_h_, e, s);
+            patch_layout(s);
+            fit_bins(pos_dev, status, s);
+            const double t = timed(phase(PME_SPREAD));
+            if (t < best) {
+                best = t;
+                tuning_.spread_edge = e;
+            }
+        }
+        tuning_.spread_ms = best;
+        pme_set_spread_edge(pme_, sys_h_, tuning_.spread_edge, s);
+        patch_layout(s);
+        fit_bins(pos_dev, status, s);
+        // FFT lines per workgroup (the transforms are the same for any grouping).
+        best = 1e30;
+        for (int cap : {16, 8, 4, 2}) {
+            pme_set_fft_lines(pme_, sys_h_, prec, cap, s);
+            patch_layout(s);
+            const double t = timed(phase(PME_FFT));
+            if (t < best) {
+                best = t;
+                tuning_.fft_cap = cap;
+            }
+        }
+        tuning_.fft_ms = best;
+        pme_set_fft_lines(pme_, sys_h_, prec, tuning_.fft_cap, s);
+        patch_layout(s);
+    }
+    if (c.include_direct) {
+        // Pair kernel grid: waves per task of the calibrated list, halved or doubled.
+        const int base = pair_grid_;
+        double best = 1e30;
+        for (int scale : {1, 2, -2}) {
+            const int g = scale == 1 ? base : (scale == 2 ? std::min(65535, 2 * base) : std::max(64, base / 2));
+            const double t = timed([&] {
+                launch_pairs(mixed_, false, nl_, pp_, part_, pos_dev, sys_dev_.p, f_fx.p, f_d.p, e_fx.p, e_d.p, status,
+                             nullptr, g, s);
+            });
+            if (t < best) {
+                best = t;
+                tuning_.pair_grid = g;
+                tuning_.pair_scale = scale;
+            }
+        }
+        tuning_.pair_ms = best;
+        pair_grid_ = tuning_.pair_grid;
+    }
+    fill_async<unsigned>(status, 1, 0u, s);
+    OMM_HIP(hipStreamSynchronize(s));
+    OMM_HIP(hipEventDestroy(t0));
+    OMM_HIP(hipEventDestroy(t1));
+}
+
 bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t s) {
     if (ntotal_ == 0) return false;
-    // PME spread bins: grow until the current positions fit.
-    if (cfg_[0].method == OMM_NB_PME) {
-        for (int attempt = 0; attempt < 8; ++attempt) {
-            fill_async<unsigned>(status, 1, 0u, s);
-            pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, nullptr, nullptr, nullptr,
-                    nullptr, status, s, PME_SPREAD);
-            unsigned st = 0;
-            OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-            OMM_HIP(hipStreamSynchronize(s));
-            if (!(st & ST_BIN_OVERFLOW)) break;
-            pme_grow_bins(pme_);
-        }
-        fill_async<unsigned>(status, 1, 0u, s);
-    }
+    fit_bins(pos_dev, status, s);
     if (!cfg_[0].include_direct) return false;
     int max_npad = 0;
     for (const SysInfo& S : sys_h_) max_npad = std::max(max_npad, S.nblocks * IBLOCK);
@@ -628,7 +740,9 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
         }
         // One wave per task of the calibrated list (grid-stride beyond it):
         // enough resident waves to hide the latency of the pair loop.
-        const int grid = (int)std::max<long long>(64, std::min<long long>(65535, tasks));
+        const long long scaled = tuning_.pair_scale > 1 ? tasks * tuning_.pair_scale
+                                                         : (tuning_.pair_scale < 0 ? tasks / 2 : tasks);
+        const int grid = (int)std::max<long long>(64, std::min<long long>(65535, scaled));
         if (grid != pair_grid_) {
             pair_grid_ = grid;
             changed = true;  // captured graphs hold the old launch shape
@@ -643,6 +757,10 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
             throw Error(OMM_ERR_OVERFLOW, "neighbour list needs more than 255 tiles per cluster:" + history);
         nlist_resize(nl_, (int)std::min<long long>(limit, std::max<long long>(want, 2LL * nl_.cap)));
         pp_.cap = nl_.cap;
+        changed = true;
+    }
+    if (!tuned_) {
+        autotune(pos_dev, status, s);
         changed = true;
     }
     fill_async<unsigned>(status, 1, 0u, s);

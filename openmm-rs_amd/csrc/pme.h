@@ -25,6 +25,8 @@ struct PmeDev {
     // fft_off[prec][pass] has nsys + 1 entries (prec 0: FP32, 1: FP64).
     DevBuf<int> fft_off[2][5];
     int fft_blocks[2][5] = {};
+    int fft_cap[2] = {};  // lines per FFT workgroup (at most), per precision
+    int spread_edge = 0;  // spread tile edge
     size_t fft_shmem[2][3] = {};  // dynamic LDS per axis
     // Spread: per-atom stencils (base point, weights) and per-tile atom bins.
     int natoms = 0, ntiles = 0, bin_cap = 0;
@@ -41,6 +43,14 @@ struct PmeDev {
 // Reads sys[s].grid and sys[s].alpha; writes the grid offsets, radices,
 // root/moduli offsets and FFT line counts into sys[s].
 void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* atom_sys, bool mixed, hipStream_t s);
+
+// FFT workgroup grouping of precision prec (0: FP32, 1: FP64): at most cap
+// lines per workgroup (cap <= 0: the default rule); writes sys[s].fft_lines,
+// which the caller uploads.  Returns the cap.  Results do not depend on it.
+int pme_set_fft_lines(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int cap, hipStream_t s);
+// Spread tile edge (6 or 8; <= 0: the default rule), with tiles and bins to
+// match; writes sys[s].ptile/tiles/tile_off.  Results do not depend on it.
+int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStream_t s);
 
 // Doubles the per-tile bin capacity after ST_BIN_OVERFLOW.
 void pme_grow_bins(PmeDev& pm);

@@ -678,18 +678,97 @@ void pm_set_bin_cap(PmeDev& pm, int cap) {
 
 void pme_grow_bins(PmeDev& pm) { pm_set_bin_cap(pm, 2 * pm.bin_cap); }
 
+namespace {
+const int PASS_AXIS[5] = {2, 1, 0, 1, 2};  // MODE_Z_R2C, Y_FWD, X_CONV, Y_INV, Z_C2R
+
+int fft_nlines(const SysInfo& S, int a) {
+    return a == 0 ? S.grid[1] * S.nzh : (a == 1 ? S.grid[0] * S.nzh : S.grid[0] * S.grid[1]);
+}
+}  // namespace
+
+int pme_set_fft_lines(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int cap, hipStream_t s) {
+    const size_t csize = prec ? 2 * sizeof(double) : 2 * sizeof(float);
+    if (cap <= 0) {
+        // The most lines that fit in LDS, lowered for the whole batch until
+        // every pass has ~2048 workgroups (8 per CU) or two lines each.
+        for (cap = 16;; cap >>= 1) {
+            long long fewest = LLONG_MAX;
+            for (int k = 0; k < 5; ++k) {
+                long long blocks = 0;
+                for (const SysInfo& S : sys) {
+                    const int a = PASS_AXIS[k];
+                    const int l = std::min(cap, max_lines(S.grid[a], csize));
+                    blocks += (fft_nlines(S, a) + l - 1) / l;
+                }
+                fewest = std::min(fewest, blocks);
+            }
+            if (fewest >= 2048 || cap <= 2) break;
+        }
+    }
+    std::vector<int> off[5];
+    for (int k = 0; k < 5; ++k) off[k].assign(1, 0);
+    for (int a = 0; a < 3; ++a) pm.fft_shmem[prec][a] = 0;
+    for (SysInfo& S : sys) {
+        for (int a = 0; a < 3; ++a) {
+            S.fft_lines[prec][a] = std::min(cap, max_lines(S.grid[a], csize));
+            pm.fft_shmem[prec][a] =
+                std::max(pm.fft_shmem[prec][a], (2 * (size_t)S.fft_lines[prec][a] * S.grid[a] + S.grid[a]) * csize);
+        }
+        for (int k = 0; k < 5; ++k) {
+            const int a = PASS_AXIS[k];
+            off[k].push_back(off[k].back() + (fft_nlines(S, a) + S.fft_lines[prec][a] - 1) / S.fft_lines[prec][a]);
+        }
+    }
+    for (int k = 0; k < 5; ++k) {
+        pm.fft_off[prec][k].upload(off[k], s);
+        pm.fft_blocks[prec][k] = off[k].back();
+    }
+    pm.fft_cap[prec] = cap;
+    return cap;
+}
+
+int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStream_t s) {
+    if (edge <= 0) {
+        edge = 8;
+        for (int e : {8, 6}) {
+            long long total = 0;
+            for (const SysInfo& S : sys)
+                total += (long long)tiles_for(S.grid[0], e) * tiles_for(S.grid[1], e) * tiles_for(S.grid[2], e);
+            edge = e;
+            if (total >= SPREAD_MIN_TILES) break;
+        }
+    }
+    std::vector<int> toff{0};
+    int bin_cap = 64;
+    for (SysInfo& S : sys) {
+        S.ptile = edge;
+        for (int a = 0; a < 3; ++a) S.tiles[a] = tiles_for(S.grid[a], edge);
+        S.tile_off = toff.back();
+        const int nt = S.tiles[0] * S.tiles[1] * S.tiles[2];
+        toff.push_back(toff.back() + nt);
+        // Twice the atoms of the largest tile at the system's mean density
+        // (the last tile of an axis is up to twice as wide as the others).
+        double largest = 1.0;
+        for (int a = 0; a < 3; ++a) {
+            const int last = S.grid[a] - (S.tiles[a] - 1) * edge;
+            largest *= (double)std::max(last, std::min(edge, S.grid[a])) / (double)S.grid[a];
+        }
+        bin_cap = std::max(bin_cap, (int)(2.0 * largest * S.n) + 64);
+    }
+    pm.ntiles = toff.back();
+    pm.tile_off.upload(toff, s);
+    pm.bin_count.alloc((size_t)pm.ntiles);
+    pm_set_bin_cap(pm, bin_cap);
+    pm.spread_edge = edge;
+    return edge;
+}
+
 void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* atom_sys, bool mixed, hipStream_t s) {
     const int nsys = (int)sys.size();
     pm.nsys = nsys;
     pm.natoms = natoms;
     pm.atom_sys = atom_sys;
     std::vector<double> roots, moduli;
-    std::vector<int> off[2][5];
-    for (int p = 0; p < 2; ++p)
-        for (int k = 0; k < 5; ++k) {
-            off[p][k].assign(1, 0);
-            pm.fft_shmem[p][k % 3] = 0;
-        }
     long long qpts = 0, cpts = 0;
     for (int si = 0; si < nsys; ++si) {
         SysInfo& S = sys[si];
@@ -717,81 +796,16 @@ void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* ato
             moduli.insert(moduli.end(), mod.begin(), mod.end());
         }
     }
-    // Lines per workgroup: the most that fit in LDS, lowered for the whole
-    // batch together until every pass has ~2048 workgroups (8 per CU) or two
-    // lines per workgroup.  The transforms do not depend on this grouping.
-    const int pass_axis[5] = {2, 1, 0, 1, 2};  // MODE_Z_R2C, Y_FWD, X_CONV, Y_INV, Z_C2R
-    auto nlines = [&](const SysInfo& S, int a) {
-        return a == 0 ? S.grid[1] * S.nzh : (a == 1 ? S.grid[0] * S.nzh : S.grid[0] * S.grid[1]);
-    };
-    for (int p = 0; p < 2; ++p) {
-        const size_t csize = p ? 2 * sizeof(double) : 2 * sizeof(float);
-        int cap = 16;
-        for (;; cap >>= 1) {
-            long long fewest = LLONG_MAX;
-            for (int k = 0; k < 5; ++k) {
-                long long blocks = 0;
-                for (const SysInfo& S : sys) {
-                    const int a = pass_axis[k];
-                    const int l = std::min(cap, max_lines(S.grid[a], csize));
-                    blocks += (nlines(S, a) + l - 1) / l;
-                }
-                fewest = std::min(fewest, blocks);
-            }
-            if (fewest >= 2048 || cap <= 2) break;
-        }
-        for (SysInfo& S : sys) {
-            for (int a = 0; a < 3; ++a) {
-                S.fft_lines[p][a] = std::min(cap, max_lines(S.grid[a], csize));
-                pm.fft_shmem[p][a] =
-                    std::max(pm.fft_shmem[p][a], (2 * (size_t)S.fft_lines[p][a] * S.grid[a] + S.grid[a]) * csize);
-            }
-            for (int k = 0; k < 5; ++k) {
-                const int a = pass_axis[k];
-                off[p][k].push_back(off[p][k].back() + (nlines(S, a) + S.fft_lines[p][a] - 1) / S.fft_lines[p][a]);
-            }
-        }
-    }
     pm.qpoints = qpts;
     pm.cpoints = cpts;
-    // Spread tiles and bins.
-    std::vector<int> toff{0};
-    int bin_cap = 64;
-    int edge = 8;
-    for (int e : {8, 6}) {
-        long long total = 0;
-        for (const SysInfo& S : sys)
-            total += (long long)tiles_for(S.grid[0], e) * tiles_for(S.grid[1], e) * tiles_for(S.grid[2], e);
-        edge = e;
-        if (total >= SPREAD_MIN_TILES) break;
-    }
-    for (SysInfo& S : sys) {
-        S.ptile = edge;
-        for (int a = 0; a < 3; ++a) S.tiles[a] = tiles_for(S.grid[a], edge);
-        S.tile_off = toff.back();
-        const int nt = S.tiles[0] * S.tiles[1] * S.tiles[2];
-        toff.push_back(toff.back() + nt);
-        // Twice the atoms of the largest tile at the system's mean density
-        // (the last tile of an axis is up to twice as wide as the others).
-        double largest = 1.0;
-        for (int a = 0; a < 3; ++a) {
-            const int last = S.grid[a] - (S.tiles[a] - 1) * edge;
-            largest *= (double)std::max(last, std::min(edge, S.grid[a])) / (double)S.grid[a];
-        }
-        bin_cap = std::max(bin_cap, (int)(2.0 * largest * S.n) + 64);
-    }
-    pm.ntiles = toff.back();
-    pm.tile_off.upload(toff, s);
-    pm.bin_count.alloc((size_t)pm.ntiles);
-    pm_set_bin_cap(pm, bin_cap);
+    // Default grouping: the largest line cap with ~2048 workgroups per FFT
+    // pass, and the largest spread tile edge with SPREAD_MIN_TILES tiles.
+    pme_set_fft_lines(pm, sys, 0, 0, s);
+    pme_set_fft_lines(pm, sys, 1, 0, s);
+    pme_set_spread_edge(pm, sys, 0, s);
     pm.st_idx.alloc(4 * (size_t)pm.natoms);
     if (mixed) pm.st_wf.alloc(16 * (size_t)pm.natoms);
     pm.st_wd.alloc(16 * (size_t)pm.natoms);
-    for (int p = 0; p < 2; ++p)
-        for (int k = 0; k < 5; ++k) {
-            pm.fft_off[p][k].upload(off[p][k], s);
-            pm.fft_blocks[p][k] = off[p][k].back();
-        }
     pm.roots_d.upload(roots, s);
     pm.moduli_d.upload(moduli, s);
     pm.cgrid_d.alloc(2 * (size_t)cpts);

@@ -51,6 +51,14 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
+class LaunchTuning(C.Structure):
+    _fields_ = [("spread_tile_edge", C.c_int32), ("fft_lines_per_workgroup", C.c_int32), ("pair_grid", C.c_int32),
+                ("spread_ms", C.c_double), ("fft_ms", C.c_double), ("pair_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BarostatInfo(C.Structure):
     """ctypes image of ``omm_barostat_info``."""
 
@@ -658,6 +666,12 @@ class Context:
         _check(library().omm_ctx_list_stats(self._h, C.byref(t), C.byref(e)))
         return t.value, e.value
 
+    def launch_tuning(self):
+        """The autotuner's launch shapes (dict)."""
+        t = LaunchTuning()
+        _check(library().omm_ctx_launch_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
     def get_state(self):
         pos = np.zeros((self.n, 3))
         vel = np.zeros((self.n, 3))
@@ -784,6 +798,11 @@ class Batch:
         t, e = C.c_int64(), C.c_int64()
         _check(library().omm_batch_list_stats(self._h, C.byref(t), C.byref(e)))
         return t.value, e.value
+
+    def launch_tuning(self):
+        t = LaunchTuning()
+        _check(library().omm_batch_launch_tuning(self._h, C.byref(t)))
+        return t.as_dict()
 
     def profile(self, reps=20):
         ms = np.zeros(8)

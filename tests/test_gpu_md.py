@@ -51,6 +51,30 @@ def test_graph_steps_match_eager_steps():
     assert np.array_equal(sa["velocities"], sb["velocities"])
 
 
+def test_autotuned_launch_shapes():
+    # The autotuner picks among exact regroupings: its choices are reported
+    # and a context tuned alone matches a batch (tuned for the whole batch)
+    # bit for bit.
+    s = water(500, 2.5, seed=27)
+    c = ommhip.Context(s, temperature=300.0, friction=1.0, seed=5)
+    t = c.launch_tuning()
+    assert t["spread_tile_edge"] in (6, 8)
+    assert t["fft_lines_per_workgroup"] in (2, 4, 8, 16)
+    assert t["pair_grid"] >= 64
+    assert t["spread_ms"] > 0 and t["fft_ms"] > 0 and t["pair_ms"] > 0
+    other = water(1500, 3.6, seed=28)
+    b = ommhip.Batch([s, other], temperature=300.0, friction=1.0, seeds=[5, 6])
+    print("single", t, "batch", b.launch_tuning())
+    c.set_velocities_to_temperature(300.0, 11)
+    st = c.get_state()
+    b.set_velocities(0, st["velocities"])
+    c.step(20)
+    b.step(20)
+    sa, sb = c.get_state(), b.get_state(0)
+    assert np.array_equal(sa["positions"], sb["positions"])
+    assert np.array_equal(sa["velocities"], sb["velocities"])
+
+
 def test_graph_trajectory_is_bit_reproducible():
     s = water(500, 2.5, seed=25)
     runs = []
