@@ -54,8 +54,11 @@ __global__ __launch_bounds__(256) void k_bonded(
         for (int k = 0; k < 4; ++k) at[k] = grp_atoms[4 * t + k];
         const int si = atom_sys[at[0]];
         const Box& box = sys[si].box;
+        const int e0 = grp_e[t], e_end = grp_e[t + 1];
+        const int b0 = grp_b[t], b_end = grp_b[t + 1];
+        const int a0 = grp_a[t], a_end = grp_a[t + 1];
         double f[4][3] = {};
-        for (int e = grp_e[t]; e < grp_e[t + 1]; ++e) {
+        for (int e = e0; e < e_end; ++e) {
             const int i = e1[e], j = e2[e];
             double fi[3];
             ee += exception_pair(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, box, use_pbc, has_pme,
@@ -64,7 +67,7 @@ __global__ __launch_bounds__(256) void k_bonded(
             add_slot(f, group_slot(at, i), fi);
             add_slot(f, group_slot(at, j), mi);
         }
-        for (int b = grp_b[t]; b < grp_b[t + 1]; ++b) {
+        for (int b = b0; b < b_end; ++b) {
             const int i = b1[b], j = b2[b];
             double fi[3];
             eb += bond_pair(i, j, blen[b], bk[b], pos, fi);
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(256) void k_bonded(
             add_slot(f, group_slot(at, i), fi);
             add_slot(f, group_slot(at, j), mi);
         }
-        for (int a = grp_a[t]; a < grp_a[t + 1]; ++a) {
+        for (int a = a0; a < a_end; ++a) {
             const int i = a1[a], j = a2[a], k = a3[a];
             double fi[3], fk[3];
             ea += angle_triple(i, j, k, ath[a], ak[a], pos, fi, fk);
@@ -85,9 +88,9 @@ __global__ __launch_bounds__(256) void k_bonded(
         for (int k = 0; k < 4; ++k)
             if (at[k] >= 0)
                 for (int c = 0; c < 3; ++c) acc_add(force + 3 * at[k] + c, f[k][c], FORCE_SCALE, status);
-        se = grp_e[t] < grp_e[t + 1] ? si : -1;
-        sb = grp_b[t] < grp_b[t + 1] ? si : -1;
-        sa = grp_a[t] < grp_a[t + 1] ? si : -1;
+        se = e0 < e_end ? si : -1;
+        sb = b0 < b_end ? si : -1;
+        sa = a0 < a_end ? si : -1;
     } else {
         int u = t - ngroups;
         const int le = ne - ge, lb = nb - gb, la = na - ga;

@@ -343,6 +343,7 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     pp_.alpha = alpha_[0];
     pp_.switching = c0.switching ? 1 : 0;
     pp_.switch_distance = c0.switching_distance;
+    pp_.inv_switch_width = c0.switching ? 1.0 / (c0.cutoff - c0.switching_distance) : 0.0;
     if (c0.method == OMM_NB_PME) {
         pp_.elec = ELEC_EWALD;
     } else {

@@ -184,6 +184,43 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_down(int* in, int* out, int* c
     }
 }
 
+// The same scan in one workgroup (chunks of SCAN_CHUNK with a running carry),
+// for inputs of at most SCAN_SINGLE_MAX elements: one launch instead of three.
+constexpr int SCAN_SINGLE_MAX = 8 * SCAN_CHUNK;
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_single(int* in, int* out, int* copy, int n, int zero_input,
+                                                         const int* flag) {
+    if (!flag[0]) return;
+    __shared__ int s_wave[SCAN_WG / 64];
+    int carry = 0;
+    for (int base = 0; base < n; base += SCAN_CHUNK) {
+        const int lo = base + threadIdx.x * SCAN_PER;
+        int v[SCAN_PER];
+        int sum = 0;
+        for (int k = 0; k < SCAN_PER; ++k) {
+            v[k] = lo + k < n ? in[lo + k] : 0;
+            sum += v[k];
+        }
+        int total;
+        int run = carry + block_scan_inclusive(sum, s_wave, &total) - sum;
+        for (int k = 0; k < SCAN_PER; ++k) {
+            const int i = lo + k;
+            if (i < n) {
+                out[i] = run;
+                if (copy) copy[i] = run;
+                if (zero_input) in[i] = 0;
+            }
+            run += v[k];
+        }
+        carry += total;
+        __syncthreads();  // s_wave is reused by the next chunk
+    }
+    if (threadIdx.x == 0) {
+        out[n] = carry;
+        if (copy) copy[n] = carry;
+    }
+}
+
 __global__ void k_scatter_units(const int* __restrict__ unit_key, int nunits, int* cursor, int* sorted_units,
                                 const int* flag) {
     if (!flag[0]) return;
@@ -589,6 +626,10 @@ uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
 inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t); }
 
 void scan_async(NlistDev& nl, int* in, int* out, int* copy, int n, int zero_input, const int* flag, hipStream_t s) {
+    if (n <= SCAN_SINGLE_MAX) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(SCAN_WG), 0, s, in, out, copy, n, zero_input, flag);
+        return;
+    }
     const int parts = blocks_for(n, SCAN_CHUNK);
     hipLaunchKernelGGL(k_scan_reduce, dim3(parts), dim3(SCAN_WG), 0, s, in, n, nl.scan_part.p, flag);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_WG), 0, s, nl.scan_part.p, parts, out + n,

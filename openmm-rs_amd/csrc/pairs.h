@@ -6,6 +6,7 @@
 namespace ommhip {
 
 enum ElecKind : int { ELEC_EWALD = 0, ELEC_REACTION_FIELD = 1, ELEC_COULOMB = 2 };
+constexpr int PAIR_SWITCH = 8;  // kernel variant flag: LJ switching function on
 
 struct PairParams {
     int nblocks = 0, cap = 0;
@@ -18,6 +19,7 @@ struct PairParams {
     double krf = 0.0, crf = 0.0;  // reaction field (nonbonded.rs:1233-1250)
     int switching = 0;            // LJ switching function (nonbonded.rs:887-899)
     double switch_distance = 0.0;
+    double inv_switch_width = 0.0;  // 1 / (cutoff - switch_distance)
 };
 
 // Per-particle parameters in the form the kernels use.

@@ -98,16 +98,17 @@ __device__ __noinline__ bool within_cutoff_exact(const double* __restrict__ pos,
 // and dS/dr for t = (r - r_switch) / (r_cut - r_switch), applied past r_switch.
 template <class R>
 __device__ __forceinline__ void lj_switch(R r, const PairParams& P, R& s, R& ds) {
-    const R width = (R)(P.cutoff - P.switch_distance);
-    const R t = (r - (R)P.switch_distance) / width;
+    const R inv_width = (R)P.inv_switch_width;
+    const R t = (r - (R)P.switch_distance) * inv_width;
     s = (R)1 + t * t * t * ((R)-10 + t * ((R)15 - (R)6 * t));
-    ds = (R)-30 * t * t * ((R)1 - t) * ((R)1 - t) / width;
+    ds = (R)-30 * t * t * ((R)1 - t) * ((R)1 - t) * inv_width;
 }
 
 // FP64 energy of one pair (nonbonded.rs:1203-1214, 1219-1273) from offsets.
-template <int ELEC>
+template <int KIND>
 __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double dz, double sig, double eps4,
                                                    double qq, const PairParams& P) {
+    constexpr int ELEC = KIND & ~PAIR_SWITCH;
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double r = sqrt(r2);
     double e = 0.0;
@@ -115,7 +116,7 @@ __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double 
         const double s2 = sig * sig / r2;
         const double s6 = s2 * s2 * s2;
         e = eps4 * (s6 * s6 - s6);
-        if (P.switching && r > P.switch_distance) {
+        if ((KIND & PAIR_SWITCH) && r > P.switch_distance) {
             double sw, dsw;
             lj_switch<double>(r, P, sw, dsw);
             e *= sw;
@@ -184,7 +185,7 @@ __device__ __forceinline__ void load_entry(const int2 en, const double* __restri
     }
 }
 
-template <class R, class A, int ELEC, bool STATS, bool ENERGY>
+template <class R, class A, int KIND, bool STATS, bool ENERGY>
 __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, const R* __restrict__ qk,
                                               const R* __restrict__ sh, const R* __restrict__ e2,
                                               const double* __restrict__ qk_d, const double* __restrict__ sh_d,
@@ -197,6 +198,8 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                                               unsigned* status, unsigned long long* stats, int* list_flag,
                                               int nflags) {
     using M = Math<R>;
+    constexpr int ELEC = KIND & ~PAIR_SWITCH;
+    constexpr bool SWITCH = (KIND & PAIR_SWITCH) != 0;
     constexpr bool E64 = ENERGY && sizeof(R) == 4;  // separate FP64 energy path
     __shared__ Q4<R> s_pq[JTILE];
     __shared__ Q2<R> s_se[JTILE];
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 const R s12 = s6 * s6;
                 R e = eps4 * (s12 - s6);
                 R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
-                if (P.switching) {
+                if (SWITCH) {
                     const R r = r2 * inv_r;
                     if (r > (R)P.switch_distance) {
                         R sw, dsw;
@@ -412,7 +415,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                         ddy = dd[1];
                         ddz = dd[2];
                     }
-                    eacc += pair_energy_fp64<ELEC>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
+                    eacc += pair_energy_fp64<KIND>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
                 } else if (ENERGY) {
                     eacc += (double)e;
                 }
@@ -609,45 +612,55 @@ void launch_pairs_t(bool stats_on, bool energy_on, const NlistDev& nl, const Pai
 
 }  // namespace
 
+namespace {
+// Pair kernel variants: the electrostatics kind, plus PAIR_SWITCH when the LJ
+// switching function is on (a separate kernel, so the common case carries
+// none of its arithmetic).
+template <class R, class A>
+void dispatch_kind(int kind, bool st, bool energy_on, const NlistDev& nl, const PairParams& p, const R* qk,
+                   const R* sh, const R* e2, const ParticleDev& part, const double* pos, const SysInfo* sys_dev,
+                   A* force, A* energy, unsigned* status, unsigned long long* stats, int grid, bool combine,
+                   hipStream_t s) {
+    switch (kind) {
+        case ELEC_EWALD:
+            launch_pairs_t<R, A, ELEC_EWALD>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status,
+                                             stats, grid, combine, s);
+            break;
+        case ELEC_REACTION_FIELD:
+            launch_pairs_t<R, A, ELEC_REACTION_FIELD>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force,
+                                                      energy, status, stats, grid, combine, s);
+            break;
+        case ELEC_COULOMB:
+            launch_pairs_t<R, A, ELEC_COULOMB>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force, energy,
+                                               status, stats, grid, combine, s);
+            break;
+        case ELEC_EWALD | PAIR_SWITCH:
+            launch_pairs_t<R, A, ELEC_EWALD | PAIR_SWITCH>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force,
+                                                           energy, status, stats, grid, combine, s);
+            break;
+        case ELEC_REACTION_FIELD | PAIR_SWITCH:
+            launch_pairs_t<R, A, ELEC_REACTION_FIELD | PAIR_SWITCH>(st, energy_on, nl, p, qk, sh, e2, part, pos,
+                                                                    sys_dev, force, energy, status, stats, grid,
+                                                                    combine, s);
+            break;
+        default:
+            throw Error(OMM_ERR_INVALID, "no pair kernel for this electrostatics / switching combination");
+    }
+}
+}  // namespace
+
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
                   const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx,
                   double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s,
                   bool combine) {
-    bool st = stats != nullptr;
-    if (mixed) {
-        switch (p.elec) {
-            case ELEC_EWALD:
-                launch_pairs_t<float, long long, ELEC_EWALD>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part,
-                                                             pos, sys_dev, force_fx, energy_fx, status, stats, grid,
-                                                             combine, s);
-                break;
-            case ELEC_REACTION_FIELD:
-                launch_pairs_t<float, long long, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk.p, part.sh.p,
-                                                                      part.e2.p, part, pos, sys_dev, force_fx,
-                                                                      energy_fx, status, stats, grid, combine, s);
-                break;
-            default:
-                launch_pairs_t<float, long long, ELEC_COULOMB>(st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p,
-                                                               part, pos, sys_dev, force_fx, energy_fx, status, stats,
-                                                               grid, combine, s);
-        }
-    } else {
-        switch (p.elec) {
-            case ELEC_EWALD:
-                launch_pairs_t<double, double, ELEC_EWALD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p, part.e2_d.p,
-                                                           part, pos, sys_dev, force_d, energy_d, status, stats, grid, combine, s);
-                break;
-            case ELEC_REACTION_FIELD:
-                launch_pairs_t<double, double, ELEC_REACTION_FIELD>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
-                                                                    part.e2_d.p, part, pos, sys_dev, force_d, energy_d,
-                                                                    status, stats, grid, combine, s);
-                break;
-            default:
-                launch_pairs_t<double, double, ELEC_COULOMB>(st, energy_on, nl, p, part.qk_d.p, part.sh_d.p,
-                                                             part.e2_d.p, part, pos, sys_dev, force_d, energy_d, status,
-                                                             stats, grid, combine, s);
-        }
-    }
+    const bool st = stats != nullptr;
+    const int kind = p.elec | (p.switching ? PAIR_SWITCH : 0);
+    if (mixed)
+        dispatch_kind<float, long long>(kind, st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part, pos,
+                                        sys_dev, force_fx, energy_fx, status, stats, grid, combine, s);
+    else
+        dispatch_kind<double, double>(kind, st, energy_on, nl, p, part.qk_d.p, part.sh_d.p, part.e2_d.p, part, pos,
+                                      sys_dev, force_d, energy_d, status, stats, grid, combine, s);
     OMM_HIP(hipGetLastError());
 }
 

@@ -11,10 +11,10 @@ namespace ommhip {
 // own parameters (lj_coulomb_exception, 1194-1198), minus the reciprocal-space
 // erf correction with the particle charges (reciprocal_pair, 1275-1288) when
 // a reciprocal method is active.  Force on i is fi, on j is -fi.
-__device__ __forceinline__ double exception_pair(int i, int j, double qq, double sig, double eps, double qi, double qj,
-                                                 const double* __restrict__ pos, const Box& box, int use_pbc,
-                                                 int has_pme, double alpha, double fi[3]) {
-    double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+__device__ __forceinline__ double exception_xyz(const double pi[3], const double pj[3], double qq, double sig,
+                                                double eps, double qi, double qj, const Box& box, int use_pbc,
+                                                int has_pme, double alpha, double fi[3]) {
+    double d[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
     if (use_pbc) min_image_exact(d, box.v);
     double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     double r = sqrt(r2);
@@ -48,11 +48,19 @@ __device__ __forceinline__ double exception_pair(int i, int j, double qq, double
     return e;
 }
 
+__device__ __forceinline__ double exception_pair(int i, int j, double qq, double sig, double eps, double qi, double qj,
+                                                 const double* __restrict__ pos, const Box& box, int use_pbc,
+                                                 int has_pme, double alpha, double fi[3]) {
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double pj[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    return exception_xyz(pi, pj, qq, sig, eps, qi, qj, box, use_pbc, has_pme, alpha, fi);
+}
+
 // HarmonicBondForce term without PBC (harmonic_bond.rs:113-140).  Force on i
 // is fi, on j is -fi.
-__device__ __forceinline__ double bond_pair(int i, int j, double length, double k, const double* __restrict__ pos,
-                                            double fi[3]) {
-    double d[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1], pos[3 * j + 2] - pos[3 * i + 2]};
+__device__ __forceinline__ double bond_xyz(const double pi[3], const double pj[3], double length, double k,
+                                           double fi[3]) {
+    double d[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
     double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     double dr = r - length;
     double f = k * dr / r;
@@ -60,14 +68,21 @@ __device__ __forceinline__ double bond_pair(int i, int j, double length, double 
     return 0.5 * k * dr * dr;
 }
 
+__device__ __forceinline__ double bond_pair(int i, int j, double length, double k, const double* __restrict__ pos,
+                                            double fi[3]) {
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double pj[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    return bond_xyz(pi, pj, length, k, fi);
+}
+
 // HarmonicAngleForce term without PBC (harmonic_angle.rs:118-167); j is the
 // vertex.  Forces on i, j, k are fi, -(fi + fk), fk.
-__device__ __forceinline__ double angle_triple(int i, int j, int k, double theta0, double ka,
-                                               const double* __restrict__ pos, double fi[3], double fk[3]) {
+__device__ __forceinline__ double angle_xyz(const double pi[3], const double pj[3], const double pk[3], double theta0,
+                                            double ka, double fi[3], double fk[3]) {
     double rji[3], rjk[3];
     for (int a = 0; a < 3; ++a) {
-        rji[a] = pos[3 * i + a] - pos[3 * j + a];
-        rjk[a] = pos[3 * k + a] - pos[3 * j + a];
+        rji[a] = pi[a] - pj[a];
+        rjk[a] = pk[a] - pj[a];
     }
     double lji = sqrt(rji[0] * rji[0] + rji[1] * rji[1] + rji[2] * rji[2]);
     double ljk = sqrt(rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2]);
@@ -81,6 +96,14 @@ __device__ __forceinline__ double angle_triple(int i, int j, int k, double theta
         fk[a] = -de * (c * rjk[a] / ljk - rji[a] / lji) / (ljk * s);
     }
     return 0.5 * ka * dth * dth;
+}
+
+__device__ __forceinline__ double angle_triple(int i, int j, int k, double theta0, double ka,
+                                               const double* __restrict__ pos, double fi[3], double fk[3]) {
+    const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    const double pj[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
+    const double pk[3] = {pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+    return angle_xyz(pi, pj, pk, theta0, ka, fi, fk);
 }
 
 // Accumulating forms: add the forces to the force accumulator.
