@@ -240,11 +240,14 @@ __device__ __forceinline__ void wave_raise_flags(int* flag, bool moved, int si) 
     const int lead = __shfl(si, first, 64);
     const bool mixed = __ballot(moved && si != lead) != 0ull;
     const int lane = threadIdx.x & 63;
+    // The flags only go from 0 to 1 here: a plain read skips the atomic when
+    // another wave has already raised it (contended device-scope atomics are
+    // slow, and a whole system's atoms may cross the skin in the same step).
     if (lane == first) {
-        atomicOr(flag, 1);
-        atomicOr(flag + 1 + lead, 1);
+        if (!__atomic_load_n(flag, __ATOMIC_RELAXED)) atomicOr(flag, 1);
+        if (!__atomic_load_n(flag + 1 + lead, __ATOMIC_RELAXED)) atomicOr(flag + 1 + lead, 1);
     }
-    if (mixed && moved && si != lead) atomicOr(flag + 1 + si, 1);
+    if (mixed && moved && si != lead && !__atomic_load_n(flag + 1 + si, __ATOMIC_RELAXED)) atomicOr(flag + 1 + si, 1);
 }
 
 // Adds each lane's energy v to slot `slot` of its system si (si < 0: lane

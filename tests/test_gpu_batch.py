@@ -38,6 +38,17 @@ def test_batch_forces_match_single_contexts_bitwise():
         assert abs(a["potential_energy"] - b["potential_energy"]) < 1e-7
 
 
+def test_many_replicas_multi_workgroup_sort():
+    # 33 boxes of 1,000 waters: 33 x 512 list cells, past what the sort's
+    # one-workgroup scan handles, so the three-launch scan runs; replicas
+    # still match lone contexts bit for bit.
+    sys_ = [water(1000, 3.1, seed=300 + r) for r in range(33)]
+    batch = ommhip.Batch(sys_, temperature=0.0, friction=0.0, seeds=list(range(300, 333)))
+    for r in (0, 17, 32):
+        single = ommhip.Context(sys_[r], temperature=0.0, friction=0.0, seed=300 + r)
+        assert np.array_equal(batch.get_state(r)["forces"], single.get_state()["forces"])
+
+
 def test_batch_replicas_match_oracle():
     sys_ = systems()
     batch = ommhip.Batch(sys_, temperature=0.0, friction=0.0)

@@ -101,6 +101,30 @@ def test_thermostat_holds_temperature():
     assert 150.0 < T < 600.0
 
 
+def test_langevin_noise_is_standard_normal():
+    # One step from rest at 300 K and at 0 K with the same seed: the velocity
+    # difference is the noise term sqrt(kT (1 - a^2) / m) xi alone
+    # (langevin.rs:103-142), so xi can be read back and its moments checked.
+    s = water(1000, 3.1, seed=29)
+    gamma, dt, temp = 1.0, 0.001, 300.0
+    hot = ommhip.Context(s, temperature=temp, friction=gamma, step_size=dt, seed=41)
+    cold = ommhip.Context(s, temperature=0.0, friction=gamma, step_size=dt, seed=41)
+    hot.step(1)
+    cold.step(1)
+    dv = hot.get_state()["velocities"] - cold.get_state()["velocities"]
+    kt = orc.boltzmann() * temp
+    scale = np.sqrt(kt * -np.expm1(-2.0 * gamma * dt) / np.asarray(s.masses))[:, None]
+    xi = (dv / scale).ravel()
+    n = xi.size
+    assert abs(xi.mean()) < 4.0 / np.sqrt(n)
+    assert abs(xi.var() - 1.0) < 4.0 * np.sqrt(2.0 / n)
+    kurt = np.mean((xi - xi.mean()) ** 4) / xi.var() ** 2
+    assert abs(kurt - 3.0) < 4.0 * np.sqrt(24.0 / n)
+    # Components are uncorrelated.
+    x = xi.reshape(-1, 3)
+    assert abs(np.corrcoef(x[:, 0], x[:, 1])[0, 1]) < 4.0 / np.sqrt(len(x))
+
+
 def test_larger_box_multi_chunk_paths():
     # 3,200 waters in 4.6 nm: 13^3 = 2,197 list cells (the sort's scan spans
     # several workgroup chunks) and ~300 list blocks; forces against the
