@@ -381,6 +381,9 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     }
     mixed_ = precision == OMM_PRECISION_MIXED;
     nb_.reset(new NbOperator(nbd.data(), nsys, precision));
+    if (!nb_->settings().periodic())
+        throw Error(OMM_ERR_INVALID, "the device MD step runs the periodic methods (PME, CutoffPeriodic); "
+                                     "NoCutoff and CutoffNonPeriodic are evaluated by the NonbondedForce operator");
     nb_->initialize(ns.data(), boxes.data());
     s_ = nb_->stream();
     n_ = nb_->total_particles();

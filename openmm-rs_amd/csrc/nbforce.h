@@ -39,6 +39,10 @@ struct NbSettings {
         return method == OMM_NB_CUTOFF_PERIODIC || method == OMM_NB_EWALD || method == OMM_NB_PME ||
                method == OMM_NB_LJPME;
     }
+    bool use_cutoff() const { return method != OMM_NB_NO_CUTOFF; }
+    bool reaction_field() const {
+        return method == OMM_NB_CUTOFF_PERIODIC || method == OMM_NB_CUTOFF_NON_PERIODIC;
+    }
 };
 
 // Validates a descriptor the way the reference's builders do and copies it.
@@ -167,11 +171,23 @@ private:
     void fit_bins(const double* pos_dev, unsigned* status, hipStream_t s);
     void upload_parameters(hipStream_t s);
     void check_box(const double* box) const;
+    // Device set-up of every system for these (validated) boxes: list,
+    // mesh, kernel parameters and buffers.
+    void setup_device(const std::vector<Box>& boxes);
+    // Non-periodic methods (NoCutoff, CutoffNonPeriodic) run the periodic
+    // machinery on a virtual rectangular box: one large enough that the
+    // minimum image never changes a displacement the list or the kernels
+    // look at, and never brings a far pair within reach.  Returns the box
+    // for these positions, keeping the current one while it still fits.
+    Box virtual_box_for(const double* pos, bool* changed);
     double dispersion_coefficient(int s) const;
 
     std::vector<NbSettings> cfg_;
     bool mixed_ = true;
     bool initialized_ = false;
+    bool device_ready_ = false;  // setup_device has run (deferred for non-periodic methods)
+    bool virtual_ = false;       // non-periodic method on a virtual box
+    Box vbox_{};
     std::vector<int> n_;
     int ntotal_ = 0;
     std::vector<double> alpha_;

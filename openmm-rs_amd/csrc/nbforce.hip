@@ -229,7 +229,7 @@ NbOperator::~NbOperator() {
 }
 
 void NbOperator::check_box(const double* box) const {
-    if (!cfg_[0].periodic()) return;
+    if (!cfg_[0].periodic()) return;  // box vectors are ignored (nonbonded.rs:802-806)
     if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
     validate_box(box);
     // nonbonded.rs:806-812
@@ -256,12 +256,22 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
             if (!pairs.insert({std::min(a, b), std::max(a, b)}).second)
                 throw Error(OMM_ERR_INVALID, "duplicate exception pair");
         }
-        if (c.switching && !(c.switching_distance >= 0.0 && c.switching_distance < c.cutoff))
+        if (c.use_cutoff() && c.switching && !(c.switching_distance >= 0.0 && c.switching_distance < c.cutoff))
             throw Error(OMM_ERR_INVALID, "switching distance must be between zero and cutoff");
-        if (c.method != OMM_NB_PME && c.method != OMM_NB_CUTOFF_PERIODIC)
+        if (c.method == OMM_NB_EWALD || c.method == OMM_NB_LJPME)
             throw Error(OMM_ERR_INVALID,
-                        "the device operator implements the PME and CutoffPeriodic methods; "
-                        "NoCutoff, CutoffNonPeriodic, Ewald and LJPME are not available on the device yet");
+                        "the device operator implements the PME, CutoffPeriodic, CutoffNonPeriodic and NoCutoff "
+                        "methods; Ewald and LJPME are not available on the device yet");
+        if (!c.periodic()) {
+            // Non-periodic: exceptions never use the box (nonbonded.rs:1006), and
+            // the virtual box is chosen from the first positions.
+            if (nsys != 1)
+                throw Error(OMM_ERR_INVALID, "non-periodic methods run on a one-system operator");
+            c.exceptions_pbc = false;
+            n_[si] = n;
+            ntotal_ += n;
+            continue;
+        }
         if (!box) throw Error(OMM_ERR_INVALID, "periodic nonbonded forces require box vectors");
         validate_box(box);
         for (int a = 0; a < 3; ++a)
@@ -283,6 +293,8 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
         n_[si] = n;
         ntotal_ += n;
     }
+    device_ready_ = false;
+    virtual_ = !cfg_[0].periodic();
     if (nsys == 1 && n_[0] == 0) {
         initialized_ = true;
         return;
@@ -290,7 +302,17 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     for (int si = 0; si < nsys; ++si)
         if (alpha_[si] != alpha_[0])
             throw Error(OMM_ERR_INVALID, "the systems of a batch must share the Ewald alpha");
+    if (virtual_) {
+        initialized_ = true;  // set up at the first evaluation, when positions are known
+        return;
+    }
+    std::vector<Box> bx(nsys);
+    for (int si = 0; si < nsys; ++si) bx[si] = make_box(boxes[si]);
+    setup_device(bx);
+}
 
+void NbOperator::setup_device(const std::vector<Box>& boxes) {
+    const int nsys = (int)cfg_.size();
     int count = 0;
     OMM_HIP(hipGetDeviceCount(&count));
     if (count <= 0) throw Error(OMM_ERR_DEVICE, "no HIP device available");
@@ -313,7 +335,7 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
             p2.push_back(off + c.exc2[e]);
         }
         SysInfo& S = sys_h_[si];
-        S.box = make_box(boxes[si]);
+        S.box = boxes[si];
         S.alpha = alpha_[si];
         for (int a = 0; a < 3; ++a) S.grid[a] = grid_[si][a];
         nls[si].n = n_[si];
@@ -329,7 +351,8 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     upload_parameters(s);
 
     const NbSettings& c0 = cfg_[0];
-    nlist_setup(nl_, nls, sys_h_, c0.cutoff, 0.1, s);
+    // NoCutoff lists hold every later atom of the system.
+    nlist_setup(nl_, nls, sys_h_, c0.use_cutoff() ? c0.cutoff : INFINITY, 0.1, s);
     if (c0.method == OMM_NB_PME) pme_setup(pme_, sys_h_, N, part_.atom_sys.p, mixed_, s);
     sys_dev_.upload(sys_h_, s);
 
@@ -337,15 +360,23 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     pp_.nblocks = nl_.nblocks;
     pp_.nslots = nl_.nslots;
     pp_.cap = nl_.cap;
-    pp_.periodic = 1;
-    pp_.use_cutoff = 1;
+    pp_.periodic = c0.periodic() ? 1 : 0;
+    pp_.use_cutoff = c0.use_cutoff() ? 1 : 0;
     pp_.cutoff = c0.cutoff;
     pp_.alpha = alpha_[0];
-    pp_.switching = c0.switching ? 1 : 0;
+    // nonbonded.rs:887-890: switching applies with a cutoff only.
+    const bool switching = c0.switching && c0.use_cutoff();
+    pp_.switching = switching ? 1 : 0;
     pp_.switch_distance = c0.switching_distance;
-    pp_.inv_switch_width = c0.switching ? 1.0 / (c0.cutoff - c0.switching_distance) : 0.0;
+    pp_.inv_switch_width = switching ? 1.0 / (c0.cutoff - c0.switching_distance) : 0.0;
     if (c0.method == OMM_NB_PME) {
         pp_.elec = ELEC_EWALD;
+    } else if (c0.method == OMM_NB_NO_CUTOFF) {
+        // Bare Coulomb (nonbonded.rs:1219-1228) is the reaction-field form
+        // with k_rf = c_rf = 0.
+        pp_.elec = ELEC_REACTION_FIELD;
+        pp_.krf = 0.0;
+        pp_.crf = 0.0;
     } else {
         pp_.elec = ELEC_REACTION_FIELD;
         double rc = c0.cutoff, eps = c0.rf_dielectric;
@@ -375,6 +406,40 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
     OMM_HIP(hipStreamSynchronize(s));
     have_box_ = false;
     initialized_ = true;
+    device_ready_ = true;
+}
+
+Box NbOperator::virtual_box_for(const double* pos, bool* changed) {
+    const int n = n_[0];
+    const NbSettings& c = cfg_[0];
+    double lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]};
+    for (int i = 1; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pos[3 * i + a]);
+            hi[a] = std::max(hi[a], pos[3 * i + a]);
+        }
+    // Every true displacement d has |d_a| <= extent_a.  With a cutoff the box
+    // needs L_a > 2 reach (pairs in reach keep their displacement) and
+    // L_a >= extent_a + reach (an imaged far pair stays out of reach); with
+    // no cutoff every pair is in the list, so L_a > 2 extent_a.  The new box
+    // leaves room for the atoms to spread before it must change.
+    const double reach = c.cutoff + 0.1;
+    double need[3], want[3];
+    for (int a = 0; a < 3; ++a) {
+        const double e = hi[a] - lo[a];
+        need[a] = c.use_cutoff() ? std::max(2.0 * reach, e + reach) * (1.0 + 1e-9) + 1e-9 : 2.0 * e * (1.0 + 1e-9) + 1e-9;
+        want[a] = c.use_cutoff() ? 1.5 * e + 2.0 * reach + 1.0 : 3.0 * e + 1.0;
+    }
+    bool fits = device_ready_;
+    for (int a = 0; a < 3 && fits; ++a) fits = vbox_.v[a * 4] > need[a];
+    if (fits) {
+        *changed = false;
+        return vbox_;
+    }
+    double v[9] = {want[0], 0, 0, 0, want[1], 0, 0, 0, want[2]};
+    vbox_ = make_box(v);
+    *changed = true;
+    return vbox_;
 }
 
 void NbOperator::upload_systems(hipStream_t s) { sys_dev_.upload(sys_h_, s); }
@@ -470,7 +535,7 @@ bool NbOperator::set_global_parameter(const std::string& name, double value, boo
                     any = true;
                 }
             }
-    if (any && initialized_ && ntotal_ > 0) refresh_parameters();
+    if (any && device_ready_ && ntotal_ > 0) refresh_parameters();
     if (changed) *changed = any;
     return found;
 }
@@ -529,7 +594,7 @@ void NbOperator::update_parameters(const omm_nb_desc* desc) {
     c.exception_offsets = n.exception_offsets;
     for (size_t k = 0; k < n.gp_names.size(); ++k) c.gp_defaults[remap[k]] = n.gp_defaults[k];
     c.apply_offsets();
-    if (initialized_ && ntotal_ > 0) refresh_parameters();
+    if (device_ready_ && ntotal_ > 0) refresh_parameters();
 }
 
 double NbOperator::dispersion_coefficient(int si) const {
@@ -898,8 +963,15 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
     const int n = n_[0];
     for (int i = 0; i < 3 * n; ++i)
         if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
+    Box b{};
+    bool vchanged = false;
+    if (n > 0 && virtual_) {
+        b = virtual_box_for(pos, &vchanged);
+        if (!device_ready_) setup_device(std::vector<Box>{b});
+    } else {
+        b = make_box(box);
+    }
     hipStream_t s = stream_;
-    Box b = make_box(box);
     if (n == 0) {
         if (energy) *energy = 0.0;
         if (parts)
@@ -908,7 +980,7 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
         if (pair_hash) *pair_hash = 0;
         return;
     }
-    bool box_changed = !have_box_ || std::memcmp(b.v, last_box_.v, sizeof(b.v)) != 0;
+    bool box_changed = vchanged || !have_box_ || std::memcmp(b.v, last_box_.v, sizeof(b.v)) != 0;
     if (box_changed) {
         sys_h_[0].box = b;
         upload_systems(s);

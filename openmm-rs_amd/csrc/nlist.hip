@@ -778,7 +778,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     double hb = 0.5 * std::cbrt(CLUSTER / density) * 1.6;
     double vol = std::pow(2 * hb, 3) + 6 * std::pow(2 * hb, 2) * reach + 3 * PI * 2 * hb * reach * reach +
                  4.0 / 3.0 * PI * reach * reach * reach;
-    long long cap = (long long)(density * vol) + IBLOCK;
+    long long cap = std::isfinite(vol) ? (long long)std::min(density * vol, 1e9) + IBLOCK : (long long)max_npad;
     cap = std::min<long long>(cap, (long long)max_npad);
     cap = std::max<long long>(cap, 256);
     nl.cap = (int)((cap + JTILE - 1) / JTILE * JTILE);

@@ -1,0 +1,209 @@
+"""The NonbondedForce methods beyond PME on the device, against the FP64 oracle.
+
+NoCutoff and CutoffNonPeriodic run on a virtual rectangular box chosen from
+the positions (large enough that the minimum image never alters a displacement
+in reach), so their results must equal the reference's non-periodic sums.
+
+The method matrix is the reference's acceptance grid: the 41 four-particle
+cases of tests/reference/nonbonded.cpp:13-86 (inputs only; that file needs an
+external OpenMM build for its values, so each case is checked against the
+oracle here).  Cases of methods the device does not run yet (Ewald, LJPME)
+must be rejected at initialisation with OMM_ERR_INVALID.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+import ommhip
+import oracle as orc
+from helpers import energy_rel, force_rms_rel, water
+
+FORCE_TOL = 2e-6
+ENERGY_TOL = 1e-6
+
+CASES = [
+    "nocutoff", "cutoff", "periodic", "periodic_triclinic", "periodic_exception", "periodic_exception_wrapped",
+    "periodic_dispersion", "periodic_switch_dispersion", "pme", "pme_triclinic", "pme_exception",
+    "pme_exception_wrapped", "pme_charged", "pme_switch_dispersion",
+    "ewald", "ewald_exception", "ewald_exception_wrapped", "ewald_charged_switch_dispersion",
+    "ljpme", "ljpme_triclinic", "ljpme_exception", "ljpme_exception_wrapped", "ljpme_charged",
+    "ljpme_explicit_triclinic", "ljpme_neutral",
+    "nocutoff_exception_offset", "nocutoff_exception_offset_runtime", "nocutoff_exception_offset_runtime_update",
+    "periodic_exception_offset_dispersion", "periodic_exception_offset_runtime_dispersion",
+    "periodic_exception_offset_runtime_update_dispersion",
+    "pme_exception_offset", "pme_exception_offset_runtime", "pme_exception_offset_runtime_update_dispersion",
+    "ljpme_exception_offset", "ljpme_exception_offset_runtime", "ljpme_exception_offset_runtime_update",
+    "ewald_resize", "pme_triclinic_resize", "ljpme_triclinic_resize", "ljpme_switch",
+]
+assert len(CASES) == 41
+
+POSITIONS = np.array([[0.1, 0.2, 0.3], [0.6, 0.4, 0.2], [2.8, 0.7, 0.4], [1.3, 1.1, 0.9]])
+
+
+def case_force(name):
+    """The force and box of one case, as tests/reference/nonbonded.cpp:13-47 builds them."""
+    pme, ljpme, ewald = name.startswith("pme"), name.startswith("ljpme"), name.startswith("ewald")
+    periodic = pme or ljpme or ewald or name.startswith("periodic")
+    tric = "triclinic" in name
+    f = ommhip.NonbondedForce()
+    method = ("LJPME" if ljpme else "Ewald" if ewald else "PME" if pme else "CutoffPeriodic" if periodic
+              else "CutoffNonPeriodic" if name == "cutoff" else "NoCutoff")
+    f.set_nonbonded_method(method)
+    f.set_cutoff_distance(1.2)
+    f.set_ewald_error_tolerance(1e-5)
+    f.set_use_dispersion_correction("dispersion" in name)
+    f.set_use_switching_function("switch" in name)
+    f.set_switching_distance(0.8)
+    f.set_exceptions_use_periodic_boundary_conditions("wrapped" in name)
+    if "explicit" in name:
+        f.set_pme_parameters(2.7, 24, 25, 28)
+    charges = [0.4, -0.3, 0.1, 0.2 if "charged" in name else -0.2]
+    for i in range(4):
+        f.add_particle(0.0 if "neutral" in name else charges[i], 0.3 + 0.02 * i, 0.5 + 0.1 * i)
+    if "exception" in name:
+        f.add_exception(0, 2, -0.02, 0.32, 0.4)
+        f.add_exception(0, 1, 0.0, 1.0, 0.0)
+    if "offset" in name:
+        f.add_global_parameter("lambda", 0.25)
+        f.add_global_parameter("mu", 0.1)
+        f.add_particle_parameter_offset("lambda", 0, 0.2, 0.04, 0.3)
+        f.add_particle_parameter_offset("mu", 0, -0.1, 0.01, -0.2)
+        f.add_particle_parameter_offset("lambda", 2, -0.15, -0.02, 0.2)
+        f.add_exception_parameter_offset("lambda", 0, 0.04, 0.05, 0.1)
+        f.add_exception_parameter_offset("mu", 1, 0.03, -0.2, 0.15)
+    box = np.array([[3.0, 0, 0], [0.4 if tric else 0, 3.0, 0], [-0.2 if tric else 0, 0.3 if tric else 0, 3.0]])
+    return f, (box if periodic else None)
+
+
+def run_case(name, precision):
+    """Device forces and energy of one case and the oracle's on the same
+    effective inputs (runtime parameter values, updated parameters, frozen
+    PME parameters after a resize)."""
+    f, box = case_force(name)
+    op = ommhip.GpuNonbondedForce(f, precision)
+    op.initialize(box)
+    values = dict(f.global_parameters)
+    if "runtime" in name:
+        for k, v in (("lambda", 0.7), ("mu", 0.2)):
+            assert op.set_global_parameter(k, v)
+            values[k] = v
+    if "update" in name:
+        op.calc_forces_and_energy(POSITIONS, box)
+        f.particle_offsets[0] = ("lambda", 0, 0.3, 0.06, -0.1)
+        f.exception_offsets[0] = ("lambda", 0, -0.02, 0.01, 0.2)
+        f.particles[3] = (0.1, 0.38, 0.7)
+        f.exceptions[0] = (0, 2, 0.03, 0.36, 0.2)
+        op.update_parameters_from(f)
+    if "resize" in name:
+        box = np.array([[3.4, 0, 0], [box[1, 0], 3.2, 0], [box[2, 0], box[2, 1], 3.3]])
+    F, E = op.calc_forces_and_energy(POSITIONS, box)
+    ref = copy.deepcopy(f)
+    ref.global_parameters = [(k, values[k]) for k, _ in f.global_parameters]
+    if f.method == ommhip.METHODS["PME"]:
+        alpha, grid = op.pme_parameters()  # frozen at initialize (nonbonded.rs:1055-1107)
+        ref.set_pme_parameters(alpha, *grid)
+    F_ref, E_ref, _ = orc.nb_evaluate(ref.desc(), POSITIONS, box)
+    return F, E, F_ref, E_ref
+
+
+DEVICE_CASES = [c for c in CASES if not c.startswith(("ewald", "ljpme"))]
+HOST_ONLY_CASES = [c for c in CASES if c.startswith(("ewald", "ljpme"))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", DEVICE_CASES)
+def test_method_matrix_plain_fp64(name):
+    F, E, F_ref, E_ref = run_case(name, ommhip.PRECISION_DOUBLE)
+    print(f"{name}: E={E:.12f} E_ref={E_ref:.12f} dF={np.abs(F - F_ref).max():.3e}")
+    assert np.abs(F - F_ref).max() <= 1e-9 * max(1.0, np.abs(F_ref).max())
+    assert abs(E - E_ref) <= 1e-9 * max(1.0, abs(E_ref))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", DEVICE_CASES)
+def test_method_matrix_mixed(name):
+    F, E, F_ref, E_ref = run_case(name, ommhip.PRECISION_MIXED)
+    frel, erel = force_rms_rel(F, F_ref), energy_rel(E, E_ref)
+    print(f"{name}: force_rms_rel={frel:.3e} energy_rel={erel:.3e}")
+    assert frel <= FORCE_TOL
+    assert erel <= ENERGY_TOL
+
+
+# Host-side validation: runs without a GPU.
+@pytest.mark.parametrize("name", HOST_ONLY_CASES)
+def test_method_matrix_rejects_methods_not_on_device(name):
+    f, box = case_force(name)
+    op = ommhip.GpuNonbondedForce(f)
+    with pytest.raises(ommhip.OmmError) as e:
+        op.initialize(box)
+    assert e.value.status == 1
+
+
+def droplet(method, molecules=300, seed=61):
+    """Waters of a 2.1 nm lattice cell with no periodic box."""
+    s = water(molecules, 2.1, seed=seed)
+    s.nonbonded.set_nonbonded_method(method)
+    return s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,switching", [("NoCutoff", False), ("CutoffNonPeriodic", False),
+                                              ("CutoffNonPeriodic", True)])
+def test_nonperiodic_droplet_parity(method, switching):
+    s = droplet(method)
+    if switching:
+        s.nonbonded.set_use_switching_function(True)
+        s.nonbonded.set_switching_distance(0.75)
+    desc = s.nonbonded.desc()
+    op = ommhip.GpuNonbondedForce(s.nonbonded)
+    op.initialize(None)
+    F, E = op.calc_forces_and_energy(s.positions, None)
+    F_ref, E_ref, _ = orc.nb_evaluate(desc, s.positions, None)
+    frel, erel = force_rms_rel(F, F_ref), energy_rel(E, E_ref)
+    print(f"{method} switching={switching}: force_rms_rel={frel:.3e} energy_rel={erel:.3e}")
+    assert frel <= FORCE_TOL
+    assert erel <= ENERGY_TOL
+    count, h = op.pair_stats(s.positions, None)
+    count_ref, h_ref = orc.nb_pairs(desc, s.positions, None)
+    assert (count, h) == (count_ref, h_ref)
+    if method == "NoCutoff":
+        n = len(s.positions)
+        assert count == n * (n - 1) // 2 - len(s.nonbonded.exceptions)
+    plain = ommhip.GpuNonbondedForce(s.nonbonded, ommhip.PRECISION_DOUBLE)
+    plain.initialize(None)
+    F64, E64 = plain.calc_forces_and_energy(s.positions, None)
+    assert np.abs(F64 - F_ref).max() <= 1e-9 * np.abs(F_ref).max()
+    assert abs(E64 - E_ref) <= 1e-9 * abs(E_ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["NoCutoff", "CutoffNonPeriodic"])
+def test_nonperiodic_virtual_box_follows_the_atoms(method):
+    # The droplet is expanded about a far-off centre between evaluations: the
+    # virtual box must be re-chosen (and the list rebuilt) so that results
+    # keep matching the oracle.
+    s = droplet(method, molecules=200, seed=62)
+    op = ommhip.GpuNonbondedForce(s.nonbonded)
+    op.initialize(None)
+    desc = s.nonbonded.desc()
+    centre = s.positions.mean(0)
+    for scale, shift in ((1.0, 0.0), (1.6, 0.0), (1.6, -37.5), (0.8, 12.25)):
+        mol = s.positions.reshape(-1, 3, 3)
+        com = mol.mean(1, keepdims=True)
+        pos = ((com - centre) * scale + centre + shift + (mol - com)).reshape(-1, 3)
+        F, E = op.calc_forces_and_energy(pos, None)
+        F_ref, E_ref, _ = orc.nb_evaluate(desc, pos, None)
+        frel, erel = force_rms_rel(F, F_ref), energy_rel(E, E_ref)
+        print(f"scale={scale} shift={shift}: force_rms_rel={frel:.3e} energy_rel={erel:.3e}")
+        assert frel <= FORCE_TOL
+        assert erel <= ENERGY_TOL
+        assert op.pair_stats(pos, None) == orc.nb_pairs(desc, pos, None)
+
+
+# Host-side validation: runs without a GPU.
+def test_md_context_rejects_nonperiodic_methods():
+    s = water(100, 1.9, seed=63)
+    s.nonbonded.set_nonbonded_method("CutoffNonPeriodic")
+    with pytest.raises(ommhip.OmmError):
+        ommhip.Context(s, temperature=300.0, friction=1.0)
