@@ -645,7 +645,7 @@ void MdContext::launch_bonded(hipStream_t s, long long* force_fx, double* force_
         hipLaunchKernelGGL(k_bonded<A>, dim3(blocks), dim3(256), 0, s, B.ngroups, B.grp_atoms.p, B.grp_e.p,
                            B.grp_b.p, B.grp_a.p, B.ne, B.ge, B.e1.p, B.e2.p, B.eqq.p, B.esig.p, B.eeps.p, B.nb, B.gb,
                            B.b1.p, B.b2.p, B.blen.p, B.bk.p, B.na, B.ga, B.a1.p, B.a2.p, B.a3.p, B.ath.p, B.ak.p,
-                           nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->has_pme() ? 1 : 0, nb_->alpha(),
+                           nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->screened() ? 1 : 0, nb_->alpha(),
                            pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force, energy, status_.p);
     };
     if (mixed_) go(force_fx, energy_fx);

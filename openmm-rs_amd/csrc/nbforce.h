@@ -5,6 +5,7 @@
 
 #include <mutex>
 
+#include "ewald.h"
 #include "pme.h"
 
 namespace ommhip {
@@ -40,6 +41,8 @@ struct NbSettings {
                method == OMM_NB_LJPME;
     }
     bool use_cutoff() const { return method != OMM_NB_NO_CUTOFF; }
+    // Ewald-screened direct space with a reciprocal part (PME or plain Ewald).
+    bool screened() const { return method == OMM_NB_PME || method == OMM_NB_EWALD; }
     bool reaction_field() const {
         return method == OMM_NB_CUTOFF_PERIODIC || method == OMM_NB_CUTOFF_NON_PERIODIC;
     }
@@ -117,7 +120,8 @@ public:
     const double* charge_dev() const { return part_.charge.p; }
     const int* atom_sys_dev() const { return part_.atom_sys.p; }
     bool exceptions_pbc() const { return cfg_[0].exceptions_pbc; }
-    bool has_pme() const { return cfg_[0].method == OMM_NB_PME; }
+    // Exceptions carry the reciprocal-space correction (PME or Ewald).
+    bool screened() const { return cfg_[0].screened(); }
     const double* pos_build() const { return nl_.pos_build.p; }
     int* list_flag() { return nl_.flag.p; }
     double skin() const { return nl_.skin; }
@@ -209,6 +213,7 @@ private:
     ExceptionDev exc_;
     NlistDev nl_;
     PmeDev pme_;
+    EwaldDev ewald_;
     PairParams pp_;
 
     // Host-API scratch.

@@ -258,10 +258,10 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
         }
         if (c.use_cutoff() && c.switching && !(c.switching_distance >= 0.0 && c.switching_distance < c.cutoff))
             throw Error(OMM_ERR_INVALID, "switching distance must be between zero and cutoff");
-        if (c.method == OMM_NB_EWALD || c.method == OMM_NB_LJPME)
+        if (c.method == OMM_NB_LJPME)
             throw Error(OMM_ERR_INVALID,
-                        "the device operator implements the PME, CutoffPeriodic, CutoffNonPeriodic and NoCutoff "
-                        "methods; Ewald and LJPME are not available on the device yet");
+                        "the device operator implements the PME, Ewald, CutoffPeriodic, CutoffNonPeriodic and "
+                        "NoCutoff methods; LJPME is not available on the device yet");
         if (!c.periodic()) {
             // Non-periodic: exceptions never use the box (nonbonded.rs:1006), and
             // the virtual box is chosen from the first positions.
@@ -277,6 +277,15 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
         for (int a = 0; a < 3; ++a)
             if (!(c.cutoff <= 0.5 * box[a * 4]))
                 throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
+        if (c.method == OMM_NB_EWALD) {
+            // nonbonded.rs:1087-1097: rectangular boxes; explicit PME
+            // parameters do not apply.
+            if (!(box[3] == 0.0 && box[6] == 0.0 && box[7] == 0.0))
+                throw Error(OMM_ERR_INVALID, "Ewald requires a rectangular periodic box; use PME for triclinic boxes");
+            if (nsys != 1) throw Error(OMM_ERR_INVALID, "Ewald runs on a one-system operator");
+            const double widths[3] = {box[0], box[4], box[8]};
+            ewald_parameters(widths, c.cutoff, c.tolerance, &alpha_[si], grid_[si].data());
+        }
         if (c.method == OMM_NB_PME) {
             if (c.pme_alpha > 0.0) {
                 alpha_[si] = c.pme_alpha;
@@ -354,6 +363,7 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     // NoCutoff lists hold every later atom of the system.
     nlist_setup(nl_, nls, sys_h_, c0.use_cutoff() ? c0.cutoff : INFINITY, 0.1, s);
     if (c0.method == OMM_NB_PME) pme_setup(pme_, sys_h_, N, part_.atom_sys.p, mixed_, s);
+    if (c0.method == OMM_NB_EWALD) ewald_setup(ewald_, N, grid_[0].data(), s);
     sys_dev_.upload(sys_h_, s);
 
     pp_ = PairParams{};
@@ -369,7 +379,7 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     pp_.switching = switching ? 1 : 0;
     pp_.switch_distance = c0.switching_distance;
     pp_.inv_switch_width = switching ? 1.0 / (c0.cutoff - c0.switching_distance) : 0.0;
-    if (c0.method == OMM_NB_PME) {
+    if (c0.screened()) {
         pp_.elec = ELEC_EWALD;
     } else if (c0.method == OMM_NB_NO_CUTOFF) {
         // Bare Coulomb (nonbonded.rs:1219-1228) is the reaction-field form
@@ -482,7 +492,7 @@ void NbOperator::upload_parameters(hipStream_t s) {
         disp_cached_[si] = 0;
         SysInfo& S = sys_h_[si];
         S.self_const = S.neutral_coeff = S.disp_coeff = 0.0;
-        if (c.method == OMM_NB_PME) {
+        if (c.screened()) {
             S.self_const = -ONE_4PI_EPS0 * alpha_[si] / std::sqrt(PI) * sum_q2_[si];
             S.neutral_coeff = -ONE_4PI_EPS0 * PI * sum_q_[si] * sum_q_[si] / (2.0 * alpha_[si] * alpha_[si]);
         }
@@ -640,7 +650,7 @@ HostTerms NbOperator::host_terms(int si, double volume, bool direct, bool recip)
     HostTerms t;
     const NbSettings& c = cfg_[si];
     const double a = alpha_[si];
-    if (recip && c.method == OMM_NB_PME)
+    if (recip && c.screened())
         t.self = -ONE_4PI_EPS0 * (a / std::sqrt(PI) * sum_q2_[si] + PI * sum_q_[si] * sum_q_[si] / (2.0 * a * a * volume));
     if (direct && c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
         t.dispersion = dispersion_coefficient(si) / volume;
@@ -876,12 +886,16 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
     // The mesh runs on the side stream, forked after the list build (the
     // spread visits atoms in the list's sorted order) and concurrent with the
     // direct-space pairs; join() makes s wait for it.
-    const bool mesh = recip && c.method == OMM_NB_PME;
+    const bool mesh = recip && c.screened();
     if (mesh) {
         OMM_HIP(hipEventRecord(ev_fork_, s));
         OMM_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
-        pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
-                energy_d, status, side_);
+        if (c.method == OMM_NB_PME)
+            pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
+                    energy_d, status, side_);
+        else
+            ewald_run(ewald_, pos_dev, part_.charge.p, sys_dev_.p, alpha_[0], force_fx, force_d,
+                      energy ? energy_fx : nullptr, energy ? energy_d : nullptr, status, mixed_, side_);
         OMM_HIP(hipEventRecord(ev_join_, side_));
         joined_ = false;
     }
@@ -890,7 +904,7 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
                      status, nullptr, pair_grid_, s);
         if (exceptions)
             launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
-                              c.method == OMM_NB_PME ? 1 : 0, alpha_[0], force_fx, force_d, energy_fx, energy_d,
+                              c.screened() ? 1 : 0, alpha_[0], force_fx, force_d, energy_fx, energy_d,
                               status, s);
     }
     if (join_now) join(s);
@@ -919,6 +933,7 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
     };
     const NbSettings& c = cfg_[0];
     const bool pme = c.method == OMM_NB_PME;
+    const bool screened = c.screened();
     ms[0] = timed([&] { nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true); });
     // The pair kernel alone; its slot-ordered sums are folded in once after.
     ms[1] = timed([&] {
@@ -927,8 +942,8 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
     });
     launch_slot_forces(mixed_, nl_, force_fx, force_d, s);
     ms[2] = timed([&] {
-        launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0, pme ? 1 : 0, alpha_[0],
-                          force_fx, force_d, energy_fx, energy_d, status, s);
+        launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0, screened ? 1 : 0,
+                          alpha_[0], force_fx, force_d, energy_fx, energy_d, status, s);
     });
     const int phases[3] = {PME_SPREAD, PME_FFT, PME_GATHER};
     for (int k = 0; k < 3; ++k)
@@ -937,6 +952,11 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
                     energy_d, status, s, phases[k]);
         })
                         : 0.0;
+    if (c.method == OMM_NB_EWALD)
+        ms[4] = timed([&] {
+            ewald_run(ewald_, pos_dev, part_.charge.p, sys_dev_.p, alpha_[0], force_fx, force_d, nullptr, nullptr,
+                      status, mixed_, s);
+        });
     if (num_pairs) {
         DevBuf<unsigned long long> st_dev;
         st_dev.alloc(2 * sys_h_.size());
@@ -1009,12 +1029,15 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
             launch_pairs(mixed_, want_energy, nl_, pp_, part_, pos_dev_.p, sys_dev_.p, force_fx_.p, force_d_.p,
                          energy_fx_.p, energy_d_.p, status_.p, want_stats ? stats_.p : nullptr, pair_grid_, s);
             launch_exceptions(mixed_, exc_, part_, pos_dev_.p, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
-                              c.method == OMM_NB_PME ? 1 : 0, alpha_[0], force_fx_.p, force_d_.p, energy_fx_.p,
+                              c.screened() ? 1 : 0, alpha_[0], force_fx_.p, force_d_.p, energy_fx_.p,
                               energy_d_.p, status_.p, s);
         }
         if (recip && c.method == OMM_NB_PME)
             pme_run(pme_, mesh_mode(want_energy), nl_, pos_dev_.p, part_.charge.p, sys_dev_.p, force_fx_.p,
                     force_d_.p, energy_fx_.p, energy_d_.p, status_.p, s);
+        if (recip && c.method == OMM_NB_EWALD)
+            ewald_run(ewald_, pos_dev_.p, part_.charge.p, sys_dev_.p, alpha_[0], force_fx_.p, force_d_.p,
+                      energy_fx_.p, energy_d_.p, status_.p, mixed_, s);
         unsigned st = 0;
         status_.download(&st, 1, s);
         OMM_HIP(hipStreamSynchronize(s));

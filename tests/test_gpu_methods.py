@@ -7,8 +7,8 @@ in reach), so their results must equal the reference's non-periodic sums.
 The method matrix is the reference's acceptance grid: the 41 four-particle
 cases of tests/reference/nonbonded.cpp:13-86 (inputs only; that file needs an
 external OpenMM build for its values, so each case is checked against the
-oracle here).  Cases of methods the device does not run yet (Ewald, LJPME)
-must be rejected at initialisation with OMM_ERR_INVALID.
+oracle here).  Cases of the method the device does not run yet (LJPME) must
+be rejected at initialisation with OMM_ERR_INVALID.
 """
 import copy
 
@@ -100,15 +100,17 @@ def run_case(name, precision):
     F, E = op.calc_forces_and_energy(POSITIONS, box)
     ref = copy.deepcopy(f)
     ref.global_parameters = [(k, values[k]) for k, _ in f.global_parameters]
-    if f.method == ommhip.METHODS["PME"]:
-        alpha, grid = op.pme_parameters()  # frozen at initialize (nonbonded.rs:1055-1107)
+    if f.method in (ommhip.METHODS["PME"], ommhip.METHODS["Ewald"]):
+        # alpha and the grid (PME) or wave-vector limits (Ewald) are frozen at
+        # initialize (nonbonded.rs:1055-1107); the oracle takes them explicitly.
+        alpha, grid = op.pme_parameters()
         ref.set_pme_parameters(alpha, *grid)
     F_ref, E_ref, _ = orc.nb_evaluate(ref.desc(), POSITIONS, box)
     return F, E, F_ref, E_ref
 
 
-DEVICE_CASES = [c for c in CASES if not c.startswith(("ewald", "ljpme"))]
-HOST_ONLY_CASES = [c for c in CASES if c.startswith(("ewald", "ljpme"))]
+DEVICE_CASES = [c for c in CASES if not c.startswith("ljpme")]
+HOST_ONLY_CASES = [c for c in CASES if c.startswith("ljpme")]
 
 
 @pytest.mark.gpu
@@ -207,3 +209,50 @@ def test_md_context_rejects_nonperiodic_methods():
     s.nonbonded.set_nonbonded_method("CutoffNonPeriodic")
     with pytest.raises(ommhip.OmmError):
         ommhip.Context(s, temperature=300.0, friction=1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,ftol,etol", [(ommhip.PRECISION_MIXED, FORCE_TOL, ENERGY_TOL),
+                                                 (ommhip.PRECISION_DOUBLE, 1e-9, 1e-9)])
+def test_ewald_water_parity(precision, ftol, etol):
+    """Plain Ewald (ewald.rs:22-122) on 300 waters: forces, energy and pair set."""
+    s = water(300, 2.1, seed=64, method="Ewald")
+    desc = s.nonbonded.desc()
+    op = ommhip.GpuNonbondedForce(s.nonbonded, precision)
+    op.initialize(s.box)
+    alpha, limits = op.pme_parameters()
+    ref_alpha, ref_limits = orc.ewald_parameters(np.diag(s.box), s.nonbonded.cutoff,
+                                                 s.nonbonded.ewald_error_tolerance)
+    assert alpha == ref_alpha and tuple(limits) == tuple(ref_limits)
+    F, E = op.calc_forces_and_energy(s.positions, s.box)
+    F_ref, E_ref, parts_ref = orc.nb_evaluate(desc, s.positions, s.box)
+    parts = op.energy_parts(s.positions, s.box)
+    frel, erel = force_rms_rel(F, F_ref), energy_rel(E, E_ref)
+    print(f"limits={limits} force_rms_rel={frel:.3e} energy_rel={erel:.3e} parts={parts} ref={parts_ref}")
+    assert frel <= ftol
+    assert erel <= etol
+    assert np.abs(parts - parts_ref).max() <= etol * np.abs(parts_ref).max()
+    assert op.pair_stats(s.positions, s.box) == orc.nb_pairs(desc, s.positions, s.box)
+
+
+@pytest.mark.gpu
+def test_ewald_md_context():
+    """The MD step with Ewald: state forces against the oracle, then steps."""
+    s = water(300, 2.1, seed=65, method="Ewald")
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, seed=3)
+    st = ctx.get_state()
+    F_ref, E_ref = orc.water_energy_forces(s)
+    assert force_rms_rel(st["forces"], F_ref) <= FORCE_TOL
+    assert energy_rel(st["potential_energy"], E_ref) <= ENERGY_TOL
+    ctx.set_velocities_to_temperature(300.0, 5)
+    ctx.step(20)
+    assert np.isfinite(ctx.get_state()["potential_energy"])
+
+
+# Host-side validation: runs without a GPU.
+def test_ewald_rejects_triclinic_box():
+    s = water(300, 2.1, seed=66, method="Ewald")
+    op = ommhip.GpuNonbondedForce(s.nonbonded)
+    with pytest.raises(ommhip.OmmError) as e:
+        op.initialize(np.array([[2.1, 0, 0], [0.3, 2.1, 0], [0, 0, 2.1]]))
+    assert e.value.status == 1
