@@ -115,6 +115,10 @@ typedef struct omm_nb_desc {
     const int32_t* exception_offset_parameter;
     const int32_t* exception_offset_index;       /* exception */
     const double* exception_offset_scales;       /* [n][3]: charge product, sigma, epsilon */
+    /* LJPME dispersion mesh (nonbonded.rs:615-645): 0 = derive alpha and the
+     * grid from the tolerance (half the Coulomb rule's grid). */
+    double ljpme_alpha;
+    int32_t ljpme_grid[3];
 } omm_nb_desc;
 
 /* Fill *desc with the reference defaults of NonbondedForce::new (nonbonded.rs:129-162);
@@ -147,6 +151,9 @@ omm_status omm_nb_initialize(omm_nb* nb, int32_t num_particles, const double* bo
 /* The frozen (alpha, grid) — NonbondedForce::pme_parameters after initialize
  * (initialized_reciprocal, nonbonded.rs:1086-1102). */
 omm_status omm_nb_get_pme_parameters(const omm_nb* nb, double* alpha, int32_t grid[3]);
+/* NonbondedForce LJPME parameters frozen by initialize (nonbonded.rs:1098-1100,
+ * 635-645): alpha and grid of the dispersion mesh; zeros for other methods. */
+omm_status omm_nb_get_ljpme_parameters(const omm_nb* nb, double* alpha, int32_t grid[3]);
 
 /* Force::calc_forces_and_energy (nonbonded.rs:1125-1131): direct space if
  * include_direct_space, reciprocal always.  pos: host N x 3; box: host 9 or

@@ -103,6 +103,17 @@ omm_status omm_nb_get_pme_parameters(const omm_nb* nb, double* alpha, int32_t gr
     });
 }
 
+omm_status omm_nb_get_ljpme_parameters(const omm_nb* nb, double* alpha, int32_t grid[3]) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        if (!nb->op.initialized()) throw Error(OMM_ERR_STATE, "NonbondedForce not initialized");
+        if (alpha) *alpha = nb->op.lj_alpha();
+        if (grid)
+            for (int a = 0; a < 3; ++a) grid[a] = nb->op.lj_grid()[a];
+        return OMM_OK;
+    });
+}
+
 omm_status omm_nb_calc_forces_and_energy(omm_nb* nb, const double* pos, const double* box, double* forces_out,
                                          double* energy_out) {
     return guarded([&]() -> omm_status {

@@ -106,6 +106,7 @@ struct SysInfo {
     long long qgrid_off, cgrid_off;
     int tiles[3], tile_off;  // PME spread tiles per axis, first tile
     int ptile;               // PME spread tile edge (6 or 8 grid points)
+    int dispersion;          // this table serves the LJPME dispersion mesh (its influence function)
     int roots_off[3], moduli_off[3];
     int fft_lines[2][3];  // lines per FFT workgroup along each axis, [0] FP32 / [1] FP64 transforms
     int radix[3][MAX_STAGES];

@@ -18,6 +18,7 @@ struct BondedDev {
     DevBuf<int> grp_e, grp_b, grp_a;          // [ngroups + 1] term offsets per group
     DevBuf<int> e1, e2, b1, b2, a1, a2, a3;
     DevBuf<double> eqq, esig, eeps, blen, bk, ath, ak;
+    DevBuf<double> ec6;  // LJPME: exception particles' C6
 };
 
 // Monte Carlo barostat state of one system (barostat.rs:42-57, 129-195) plus

@@ -44,7 +44,8 @@ __global__ __launch_bounds__(256) void k_bonded(
     const int* __restrict__ b1, const int* __restrict__ b2, const double* __restrict__ blen,
     const double* __restrict__ bk, int na, int ga, const int* __restrict__ a1, const int* __restrict__ a2,
     const int* __restrict__ a3, const double* __restrict__ ath, const double* __restrict__ ak,
-    const double* __restrict__ charge, int use_pbc, int has_pme, double alpha, const double* __restrict__ pos,
+    const double* __restrict__ charge, int use_pbc, int has_pme, double alpha, const double* __restrict__ ec6,
+    double lj_alpha, const double* __restrict__ pos,
     const int* __restrict__ atom_sys, const SysInfo* __restrict__ sys, A* force, A* energy, unsigned* status) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     double ee = 0.0, eb = 0.0, ea = 0.0;
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(256) void k_bonded(
             const int i = e1[e], j = e2[e];
             double fi[3];
             ee += exception_pair(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, box, use_pbc, has_pme,
-                                 alpha, fi);
+                                 alpha, ec6 ? ec6[e] : 0.0, lj_alpha, fi);
             const double mi[3] = {-fi[0], -fi[1], -fi[2]};
             add_slot(f, group_slot(at, i), fi);
             add_slot(f, group_slot(at, j), mi);
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(256) void k_bonded(
             const int e = ge + u, i = e1[e], j = e2[e];
             se = atom_sys[i];
             ee = exception_term(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, sys[se].box, use_pbc,
-                                has_pme, alpha, force, status);
+                                has_pme, alpha, ec6 ? ec6[e] : 0.0, lj_alpha, force, status);
         } else if ((u -= le) < lb) {
             const int b = gb + u;
             sb = atom_sys[b1[b]];
@@ -423,11 +424,13 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     inv_mass_.upload(inv, s_);
     // Exceptions in global indices (the bonded kernel evaluates them).
     std::vector<int> e1, e2;
-    std::vector<double> eqq, esig, eeps;
+    std::vector<double> eqq, esig, eeps, ec6;
     if (nb_->settings().include_direct)
         for (int r = 0; r < nsys; ++r) {
             const NbSettings& c = nb_->settings(r);
             const int off = nb_->atom_offset(r);
+            const std::vector<double> c6 = exception_c6(c);
+            ec6.insert(ec6.end(), c6.begin(), c6.end());
             for (size_t k = 0; k < c.exc1.size(); ++k) {
                 e1.push_back(off + c.exc1[k]);
                 e2.push_back(off + c.exc2[k]);
@@ -437,6 +440,12 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
             }
         }
     build_bonded(e1, e2, eqq, esig, eeps, b1, b2, blen, bk, a1, a2, a3, ath, ak);
+    if (nb_->ljpme()) {
+        std::vector<double> p(bonded_exc_order_.size());
+        for (size_t k = 0; k < p.size(); ++k) p[k] = ec6[bonded_exc_order_[k]];
+        if (p.empty()) p.push_back(0.0);
+        bonded_.ec6.upload(p, s_);
+    }
     bond1_h_ = b1;
     bond2_h_ = b2;
     dt_ = descs[0].step_size;
@@ -646,7 +655,7 @@ void MdContext::launch_bonded(hipStream_t s, long long* force_fx, double* force_
                            B.grp_b.p, B.grp_a.p, B.ne, B.ge, B.e1.p, B.e2.p, B.eqq.p, B.esig.p, B.eeps.p, B.nb, B.gb,
                            B.b1.p, B.b2.p, B.blen.p, B.bk.p, B.na, B.ga, B.a1.p, B.a2.p, B.a3.p, B.ath.p, B.ak.p,
                            nb_->charge_dev(), nb_->exceptions_pbc() ? 1 : 0, nb_->screened() ? 1 : 0, nb_->alpha(),
-                           pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force, energy, status_.p);
+                           nb_->ljpme() ? B.ec6.p : nullptr, nb_->ljpme() ? nb_->lj_alpha() : 0.0, pos_.p, nb_->atom_sys_dev(), nb_->systems_dev(), force, energy, status_.p);
     };
     if (mixed_) go(force_fx, energy_fx);
     else go(force_d, energy_d);
@@ -780,9 +789,11 @@ bool MdContext::set_parameter(const std::string& name, double value) {
     const bool found = nb_->set_global_parameter(name, value, &changed);
     if (changed && nb_->settings().include_direct) {
         // The bonded kernel keeps its own (grouped) copy of the exception parameters.
-        std::vector<double> qq, sig, eps;
+        std::vector<double> qq, sig, eps, c6;
         for (int r = 0; r < nsys_; ++r) {
             const NbSettings& c = nb_->settings(r);
+            const std::vector<double> v = exception_c6(c);
+            c6.insert(c6.end(), v.begin(), v.end());
             qq.insert(qq.end(), c.exc_qq.begin(), c.exc_qq.end());
             sig.insert(sig.end(), c.exc_sig.begin(), c.exc_sig.end());
             eps.insert(eps.end(), c.exc_eps.begin(), c.exc_eps.end());
@@ -796,6 +807,7 @@ bool MdContext::set_parameter(const std::string& name, double value) {
         bonded_.eqq.upload(permuted(qq), s_);
         bonded_.esig.upload(permuted(sig), s_);
         bonded_.eeps.upload(permuted(eps), s_);
+        if (nb_->ljpme()) bonded_.ec6.upload(permuted(c6), s_);
     }
     return found;
 }

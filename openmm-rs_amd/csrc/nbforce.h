@@ -31,6 +31,8 @@ struct NbSettings {
     double cutoff = 1.0, rf_dielectric = 78.3, tolerance = 5e-4;
     double pme_alpha = 0.0;
     int pme_grid[3] = {0, 0, 0};
+    double ljpme_alpha = 0.0;  // 0: derived from the tolerance
+    int ljpme_grid[3] = {0, 0, 0};
     bool dispersion_correction = true, exceptions_pbc = false, switching = false;
     double switching_distance = -1.0;
     int force_group = 0, reciprocal_group = -1;
@@ -41,12 +43,18 @@ struct NbSettings {
                method == OMM_NB_LJPME;
     }
     bool use_cutoff() const { return method != OMM_NB_NO_CUTOFF; }
-    // Ewald-screened direct space with a reciprocal part (PME or plain Ewald).
-    bool screened() const { return method == OMM_NB_PME || method == OMM_NB_EWALD; }
+    // Ewald-screened direct space with a reciprocal part (PME, LJPME or plain Ewald).
+    bool screened() const { return method == OMM_NB_PME || method == OMM_NB_EWALD || method == OMM_NB_LJPME; }
+    // A smooth-PME Coulomb mesh (PME, LJPME).
+    bool pme_mesh() const { return method == OMM_NB_PME || method == OMM_NB_LJPME; }
     bool reaction_field() const {
         return method == OMM_NB_CUTOFF_PERIODIC || method == OMM_NB_CUTOFF_NON_PERIODIC;
     }
 };
+
+// C6 = 4 sqrt(eps_i eps_j) (sigma_i sigma_j)^3 of every exception's particles
+// (nonbonded.rs:1022-1027) under LJPME, zeros otherwise.
+std::vector<double> exception_c6(const NbSettings& c);
 
 // Validates a descriptor the way the reference's builders do and copies it.
 NbSettings settings_from_desc(const omm_nb_desc* d);
@@ -82,6 +90,9 @@ public:
     int nsys() const { return (int)cfg_.size(); }
     double alpha(int s = 0) const { return alpha_[s]; }
     const int* grid(int s = 0) const { return grid_[s].data(); }
+    double lj_alpha(int s = 0) const { return lj_alpha_[s]; }
+    const int* lj_grid(int s = 0) const { return lj_grid_[s].data(); }
+    bool ljpme() const { return cfg_[0].method == OMM_NB_LJPME; }
     const NbSettings& settings(int s = 0) const { return cfg_[s]; }
     bool mixed() const { return mixed_; }
     int num_particles(int s = 0) const { return n_[s]; }
@@ -185,6 +196,8 @@ private:
     // for these positions, keeping the current one while it still fits.
     Box virtual_box_for(const double* pos, bool* changed);
     double dispersion_coefficient(int s) const;
+    // LJPME self term alpha^6 / 3 sum_i eps_i sigma_i^6 (nonbonded.rs:984-988).
+    double lj_self_energy(int s) const;
 
     std::vector<NbSettings> cfg_;
     bool mixed_ = true;
@@ -196,6 +209,8 @@ private:
     int ntotal_ = 0;
     std::vector<double> alpha_;
     std::vector<std::array<int, 3>> grid_;
+    std::vector<double> lj_alpha_;  // LJPME dispersion mesh
+    std::vector<std::array<int, 3>> lj_grid_;
     std::vector<double> sum_q2_, sum_q_;
     mutable std::vector<double> disp_coeff_;
     mutable std::vector<char> disp_cached_;
@@ -214,6 +229,14 @@ private:
     NlistDev nl_;
     PmeDev pme_;
     EwaldDev ewald_;
+    // LJPME: the dispersion mesh and its system table (the Coulomb table's
+    // layout with the dispersion grid, alpha and influence function; boxes
+    // copied from the Coulomb table before each use).
+    PmeDev pme_lj_;
+    std::vector<SysInfo> sys_lj_h_;
+    DevBuf<SysInfo> sys_lj_dev_;
+    void lj_mesh(int mode, const double* pos_dev, long long* force_fx, double* force_d, long long* energy_fx,
+                 double* energy_d, unsigned* status, hipStream_t s, int phases = PME_ALL);
     PairParams pp_;
 
     // Host-API scratch.

@@ -106,6 +106,14 @@ NbSettings settings_from_desc(const omm_nb_desc* d) {
             if (d->pme_grid[a] < 5) throw Error(OMM_ERR_INVALID, "PME grid dimensions must be at least five");
     c.pme_alpha = d->pme_alpha;
     for (int a = 0; a < 3; ++a) c.pme_grid[a] = d->pme_grid[a];
+    // nonbonded.rs:615-630
+    if (!(std::isfinite(d->ljpme_alpha) && d->ljpme_alpha >= 0.0))
+        throw Error(OMM_ERR_INVALID, "LJPME alpha must be nonnegative and finite");
+    if (d->ljpme_alpha > 0.0)
+        for (int a = 0; a < 3; ++a)
+            if (d->ljpme_grid[a] < 5) throw Error(OMM_ERR_INVALID, "LJPME grid dimensions must be at least five");
+    c.ljpme_alpha = d->ljpme_alpha;
+    for (int a = 0; a < 3; ++a) c.ljpme_grid[a] = d->ljpme_alpha > 0.0 ? d->ljpme_grid[a] : 0;
     c.dispersion_correction = d->use_dispersion_correction != 0;
     c.exceptions_pbc = d->exceptions_use_pbc != 0;
     c.switching = d->use_switching_function != 0;
@@ -204,7 +212,7 @@ NbOperator::NbOperator(const omm_nb_desc* const* descs, int nsys, int precision)
         const NbSettings& a = cfg_[0];
         const NbSettings& b = cfg_.back();
         if (a.method != b.method || a.cutoff != b.cutoff || a.rf_dielectric != b.rf_dielectric ||
-            a.tolerance != b.tolerance || a.pme_alpha != b.pme_alpha ||
+            a.tolerance != b.tolerance || a.pme_alpha != b.pme_alpha || a.ljpme_alpha != b.ljpme_alpha ||
             a.dispersion_correction != b.dispersion_correction || a.exceptions_pbc != b.exceptions_pbc ||
             a.switching != b.switching || a.switching_distance != b.switching_distance ||
             a.force_group != b.force_group || a.reciprocal_group != b.reciprocal_group ||
@@ -215,6 +223,8 @@ NbOperator::NbOperator(const omm_nb_desc* const* descs, int nsys, int precision)
     }
     alpha_.assign(nsys, 0.0);
     grid_.assign(nsys, {0, 0, 0});
+    lj_alpha_.assign(nsys, 0.0);
+    lj_grid_.assign(nsys, {0, 0, 0});
     sum_q2_.assign(nsys, 0.0);
     sum_q_.assign(nsys, 0.0);
     disp_coeff_.assign(nsys, 0.0);
@@ -258,10 +268,6 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
         }
         if (c.use_cutoff() && c.switching && !(c.switching_distance >= 0.0 && c.switching_distance < c.cutoff))
             throw Error(OMM_ERR_INVALID, "switching distance must be between zero and cutoff");
-        if (c.method == OMM_NB_LJPME)
-            throw Error(OMM_ERR_INVALID,
-                        "the device operator implements the PME, Ewald, CutoffPeriodic, CutoffNonPeriodic and "
-                        "NoCutoff methods; LJPME is not available on the device yet");
         if (!c.periodic()) {
             // Non-periodic: exceptions never use the box (nonbonded.rs:1006), and
             // the virtual box is chosen from the first positions.
@@ -286,7 +292,7 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
             const double widths[3] = {box[0], box[4], box[8]};
             ewald_parameters(widths, c.cutoff, c.tolerance, &alpha_[si], grid_[si].data());
         }
-        if (c.method == OMM_NB_PME) {
+        if (c.pme_mesh()) {
             if (c.pme_alpha > 0.0) {
                 alpha_[si] = c.pme_alpha;
                 for (int a = 0; a < 3; ++a) grid_[si][a] = c.pme_grid[a];
@@ -296,6 +302,19 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
                 for (int a = 0; a < 3; ++a) {
                     double v = std::ceil(2.0 * alpha_[si] * box[a * 4] / (3.0 * std::pow(c.tolerance, 0.2)));
                     grid_[si][a] = (int)std::max(v, 6.0);
+                }
+            }
+        }
+        if (c.method == OMM_NB_LJPME) {
+            if (c.ljpme_alpha > 0.0) {
+                lj_alpha_[si] = c.ljpme_alpha;
+                for (int a = 0; a < 3; ++a) lj_grid_[si][a] = c.ljpme_grid[a];
+            } else {
+                // nonbonded.rs:635-645: the Coulomb rule with half the grid.
+                lj_alpha_[si] = std::sqrt(-std::log(2.0 * c.tolerance)) / c.cutoff;
+                for (int a = 0; a < 3; ++a) {
+                    double v = std::ceil(lj_alpha_[si] * box[a * 4] / (3.0 * std::pow(c.tolerance, 0.2)));
+                    lj_grid_[si][a] = (int)std::max(v, 6.0);
                 }
             }
         }
@@ -309,7 +328,7 @@ void NbOperator::initialize(const int* nlist, const double* const* boxes) {
         return;
     }
     for (int si = 0; si < nsys; ++si)
-        if (alpha_[si] != alpha_[0])
+        if (alpha_[si] != alpha_[0] || lj_alpha_[si] != lj_alpha_[0])
             throw Error(OMM_ERR_INVALID, "the systems of a batch must share the Ewald alpha");
     if (virtual_) {
         initialized_ = true;  // set up at the first evaluation, when positions are known
@@ -362,8 +381,20 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     const NbSettings& c0 = cfg_[0];
     // NoCutoff lists hold every later atom of the system.
     nlist_setup(nl_, nls, sys_h_, c0.use_cutoff() ? c0.cutoff : INFINITY, 0.1, s);
-    if (c0.method == OMM_NB_PME) pme_setup(pme_, sys_h_, N, part_.atom_sys.p, mixed_, s);
+    if (c0.pme_mesh()) pme_setup(pme_, sys_h_, N, part_.atom_sys.p, mixed_, s);
     if (c0.method == OMM_NB_EWALD) ewald_setup(ewald_, N, grid_[0].data(), s);
+    if (c0.method == OMM_NB_LJPME) {
+        // The dispersion mesh: the same atoms and sort order, its own grid.
+        sys_lj_h_ = sys_h_;
+        for (int si = 0; si < nsys; ++si) {
+            SysInfo& L = sys_lj_h_[si];
+            L.alpha = lj_alpha_[si];
+            for (int a = 0; a < 3; ++a) L.grid[a] = lj_grid_[si][a];
+            L.dispersion = 1;
+        }
+        pme_setup(pme_lj_, sys_lj_h_, N, part_.atom_sys.p, mixed_, s);
+        sys_lj_dev_.upload(sys_lj_h_, s);
+    }
     sys_dev_.upload(sys_h_, s);
 
     pp_ = PairParams{};
@@ -375,10 +406,12 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     pp_.cutoff = c0.cutoff;
     pp_.alpha = alpha_[0];
     // nonbonded.rs:887-890: switching applies with a cutoff only.
-    const bool switching = c0.switching && c0.use_cutoff();
+    const bool switching = c0.switching && c0.use_cutoff() && c0.method != OMM_NB_LJPME;
     pp_.switching = switching ? 1 : 0;
     pp_.switch_distance = c0.switching_distance;
     pp_.inv_switch_width = switching ? 1.0 / (c0.cutoff - c0.switching_distance) : 0.0;
+    pp_.ljpme = c0.method == OMM_NB_LJPME ? 1 : 0;
+    pp_.lj_alpha = lj_alpha_[0];
     if (c0.screened()) {
         pp_.elec = ELEC_EWALD;
     } else if (c0.method == OMM_NB_NO_CUTOFF) {
@@ -460,8 +493,10 @@ void NbOperator::upload_parameters(hipStream_t s) {
     const int nsys = (int)cfg_.size();
     const int N = ntotal_;
     std::vector<float> qk(N), sh(N), e2(N);
-    std::vector<double> charge(N), qkd(N), shd(N), e2d(N);
-    std::vector<double> eqq, esig, eeps;
+    std::vector<double> charge(N), qkd(N), shd(N), e2d(N), amp;
+    std::vector<double> eqq, esig, eeps, ec6;
+    const bool ljpme = cfg_[0].method == OMM_NB_LJPME;
+    if (ljpme) amp.resize(N);
     const double sk = std::sqrt(ONE_4PI_EPS0);
     int off = 0;
     for (int si = 0; si < nsys; ++si) {
@@ -483,6 +518,11 @@ void NbOperator::upload_parameters(hipStream_t s) {
             e2[g] = (float)e2d[g];
             sum_q2_[si] += c.charge[i] * c.charge[i];
             sum_q_[si] += c.charge[i];
+            if (ljpme) amp[g] = 2.0 * std::sqrt(c.epsilon[i]) * std::pow(c.sigma[i], 3);  // pme.rs:50-51
+        }
+        if (ljpme) {
+            const std::vector<double> v = exception_c6(c);
+            ec6.insert(ec6.end(), v.begin(), v.end());
         }
         for (size_t e = 0; e < c.exc1.size(); ++e) {
             eqq.push_back(c.exc_qq[e]);
@@ -496,6 +536,7 @@ void NbOperator::upload_parameters(hipStream_t s) {
             S.self_const = -ONE_4PI_EPS0 * alpha_[si] / std::sqrt(PI) * sum_q2_[si];
             S.neutral_coeff = -ONE_4PI_EPS0 * PI * sum_q_[si] * sum_q_[si] / (2.0 * alpha_[si] * alpha_[si]);
         }
+        if (c.method == OMM_NB_LJPME) S.self_const += lj_self_energy(si);
         if (c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
             S.disp_coeff = dispersion_coefficient(si);
         off += n_[si];
@@ -503,6 +544,13 @@ void NbOperator::upload_parameters(hipStream_t s) {
     part_.charge.upload(charge, s);
     pme_.max_charge = 0.0;
     for (double q : charge) pme_.max_charge = std::max(pme_.max_charge, std::fabs(q));
+    if (ljpme) {
+        part_.c6amp.upload(amp, s);
+        pme_lj_.max_charge = 0.0;
+        for (double b : amp) pme_lj_.max_charge = std::max(pme_lj_.max_charge, std::fabs(b));
+        if (ec6.empty()) ec6.push_back(0.0);
+        exc_.c6.upload(ec6, s);
+    }
     if (mixed_) {
         part_.qk.upload(qk, s);
         part_.sh.upload(sh, s);
@@ -646,12 +694,30 @@ double NbOperator::dispersion_coefficient(int si) const {
     return disp_coeff_[si];
 }
 
+double NbOperator::lj_self_energy(int si) const {
+    const NbSettings& c = cfg_[si];
+    double sum = 0.0;
+    for (size_t i = 0; i < c.sigma.size(); ++i) sum += c.epsilon[i] * std::pow(c.sigma[i], 6);
+    return sum * std::pow(lj_alpha_[si], 6) / 3.0;
+}
+
+std::vector<double> exception_c6(const NbSettings& c) {
+    std::vector<double> out(c.exc1.size(), 0.0);
+    if (c.method != OMM_NB_LJPME) return out;
+    for (size_t e = 0; e < out.size(); ++e) {
+        const int i = c.exc1[e], j = c.exc2[e];
+        out[e] = 4.0 * std::sqrt(c.epsilon[i] * c.epsilon[j]) * std::pow(c.sigma[i] * c.sigma[j], 3);
+    }
+    return out;
+}
+
 HostTerms NbOperator::host_terms(int si, double volume, bool direct, bool recip) const {
     HostTerms t;
     const NbSettings& c = cfg_[si];
     const double a = alpha_[si];
+    if (recip && c.method == OMM_NB_LJPME) t.self += lj_self_energy(si);
     if (recip && c.screened())
-        t.self = -ONE_4PI_EPS0 * (a / std::sqrt(PI) * sum_q2_[si] + PI * sum_q_[si] * sum_q_[si] / (2.0 * a * a * volume));
+        t.self += -ONE_4PI_EPS0 * (a / std::sqrt(PI) * sum_q2_[si] + PI * sum_q_[si] * sum_q_[si] / (2.0 * a * a * volume));
     if (direct && c.periodic() && c.dispersion_correction && c.method != OMM_NB_LJPME)
         t.dispersion = dispersion_coefficient(si) / volume;
     return t;
@@ -664,22 +730,45 @@ void NbOperator::grow(unsigned status) {
         nlist_resize(nl_, std::min(limit, 2 * nl_.cap));
         pp_.cap = nl_.cap;
     }
-    if ((status & ST_BIN_OVERFLOW) && cfg_[0].method == OMM_NB_PME) pme_grow_bins(pme_);
+    if ((status & ST_BIN_OVERFLOW) && cfg_[0].pme_mesh()) pme_grow_bins(pme_);
+    if ((status & ST_BIN_OVERFLOW) && ljpme()) pme_grow_bins(pme_lj_);
 }
 
 void NbOperator::fit_bins(const double* pos_dev, unsigned* status, hipStream_t s) {
-    if (cfg_[0].method != OMM_NB_PME) return;
-    for (int attempt = 0; attempt < 8; ++attempt) {
-        fill_async<unsigned>(status, 1, 0u, s);
-        pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, nullptr, nullptr, nullptr, nullptr,
-                status, s, PME_SPREAD);
-        unsigned st = 0;
-        OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        OMM_HIP(hipStreamSynchronize(s));
-        if (!(st & ST_BIN_OVERFLOW)) break;
-        pme_grow_bins(pme_);
+    if (!cfg_[0].pme_mesh()) return;
+    for (int mesh = 0; mesh < (ljpme() ? 2 : 1); ++mesh) {
+        for (int attempt = 0; attempt < 8; ++attempt) {
+            fill_async<unsigned>(status, 1, 0u, s);
+            if (mesh == 0)
+                pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, nullptr, nullptr, nullptr,
+                        nullptr, status, s, PME_SPREAD);
+            else
+                lj_mesh(mesh_mode(false), pos_dev, nullptr, nullptr, nullptr, nullptr, status, s, PME_SPREAD);
+            unsigned st = 0;
+            OMM_HIP(hipMemcpyAsync(&st, status, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            OMM_HIP(hipStreamSynchronize(s));
+            if (!(st & ST_BIN_OVERFLOW)) break;
+            pme_grow_bins(mesh == 0 ? pme_ : pme_lj_);
+        }
     }
     fill_async<unsigned>(status, 1, 0u, s);
+}
+
+namespace {
+__global__ void k_copy_boxes(int nsys, const SysInfo* __restrict__ src, SysInfo* dst) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) dst[s].box = src[s].box;
+}
+}  // namespace
+
+void NbOperator::lj_mesh(int mode, const double* pos_dev, long long* force_fx, double* force_d, long long* energy_fx,
+                         double* energy_d, unsigned* status, hipStream_t s, int phases) {
+    // The boxes live in the Coulomb table (a barostat rescales them there).
+    const int nsys = (int)sys_lj_h_.size();
+    hipLaunchKernelGGL(k_copy_boxes, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, sys_dev_.p, sys_lj_dev_.p);
+    OMM_HIP(hipGetLastError());
+    pme_run(pme_lj_, mode, nl_, pos_dev, part_.c6amp.p, sys_lj_dev_.p, force_fx, force_d, energy_fx, energy_d, status,
+            s, phases);
 }
 
 void NbOperator::patch_layout(hipStream_t s) {
@@ -727,7 +816,7 @@ void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s
         f_d.alloc(3 * (size_t)ntotal_);
         e_d.alloc(eslots);
     }
-    if (c.method == OMM_NB_PME) {
+    if (c.pme_mesh()) {
         const int mode = mesh_mode(false);
         const int prec = mode == MESH_F32 ? 0 : 1;
         auto phase = [&](int ph) {
@@ -890,10 +979,11 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
     if (mesh) {
         OMM_HIP(hipEventRecord(ev_fork_, s));
         OMM_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
-        if (c.method == OMM_NB_PME)
+        if (c.pme_mesh()) {
             pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
                     energy_d, status, side_);
-        else
+            if (ljpme()) lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_);
+        } else
             ewald_run(ewald_, pos_dev, part_.charge.p, sys_dev_.p, alpha_[0], force_fx, force_d,
                       energy ? energy_fx : nullptr, energy ? energy_d : nullptr, status, mixed_, side_);
         OMM_HIP(hipEventRecord(ev_join_, side_));
@@ -904,8 +994,8 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
                      status, nullptr, pair_grid_, s);
         if (exceptions)
             launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
-                              c.screened() ? 1 : 0, alpha_[0], force_fx, force_d, energy_fx, energy_d,
-                              status, s);
+                              c.screened() ? 1 : 0, alpha_[0], ljpme() ? lj_alpha_[0] : 0.0, force_fx, force_d,
+                              energy_fx, energy_d, status, s);
     }
     if (join_now) join(s);
 }
@@ -932,8 +1022,9 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
         return (double)v / reps;
     };
     const NbSettings& c = cfg_[0];
-    const bool pme = c.method == OMM_NB_PME;
+    const bool pme = c.pme_mesh();
     const bool screened = c.screened();
+    const double lj_a = ljpme() ? lj_alpha_[0] : 0.0;
     ms[0] = timed([&] { nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true); });
     // The pair kernel alone; its slot-ordered sums are folded in once after.
     ms[1] = timed([&] {
@@ -943,13 +1034,14 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
     launch_slot_forces(mixed_, nl_, force_fx, force_d, s);
     ms[2] = timed([&] {
         launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0, screened ? 1 : 0,
-                          alpha_[0], force_fx, force_d, energy_fx, energy_d, status, s);
+                          alpha_[0], lj_a, force_fx, force_d, energy_fx, energy_d, status, s);
     });
     const int phases[3] = {PME_SPREAD, PME_FFT, PME_GATHER};
     for (int k = 0; k < 3; ++k)
         ms[3 + k] = pme ? timed([&] {
             pme_run(pme_, mesh_mode(false), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
                     energy_d, status, s, phases[k]);
+            if (ljpme()) lj_mesh(mesh_mode(false), pos_dev, force_fx, force_d, energy_fx, energy_d, status, s, phases[k]);
         })
                         : 0.0;
     if (c.method == OMM_NB_EWALD)
@@ -1029,12 +1121,14 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
             launch_pairs(mixed_, want_energy, nl_, pp_, part_, pos_dev_.p, sys_dev_.p, force_fx_.p, force_d_.p,
                          energy_fx_.p, energy_d_.p, status_.p, want_stats ? stats_.p : nullptr, pair_grid_, s);
             launch_exceptions(mixed_, exc_, part_, pos_dev_.p, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
-                              c.screened() ? 1 : 0, alpha_[0], force_fx_.p, force_d_.p, energy_fx_.p,
-                              energy_d_.p, status_.p, s);
+                              c.screened() ? 1 : 0, alpha_[0], ljpme() ? lj_alpha_[0] : 0.0, force_fx_.p,
+                              force_d_.p, energy_fx_.p, energy_d_.p, status_.p, s);
         }
-        if (recip && c.method == OMM_NB_PME)
+        if (recip && c.pme_mesh())
             pme_run(pme_, mesh_mode(want_energy), nl_, pos_dev_.p, part_.charge.p, sys_dev_.p, force_fx_.p,
                     force_d_.p, energy_fx_.p, energy_d_.p, status_.p, s);
+        if (recip && ljpme())
+            lj_mesh(mesh_mode(want_energy), pos_dev_.p, force_fx_.p, force_d_.p, energy_fx_.p, energy_d_.p, status_.p, s);
         if (recip && c.method == OMM_NB_EWALD)
             ewald_run(ewald_, pos_dev_.p, part_.charge.p, sys_dev_.p, alpha_[0], force_fx_.p, force_d_.p,
                       energy_fx_.p, energy_d_.p, status_.p, mixed_, s);

@@ -7,6 +7,8 @@ namespace ommhip {
 
 enum ElecKind : int { ELEC_EWALD = 0, ELEC_REACTION_FIELD = 1, ELEC_COULOMB = 2 };
 constexpr int PAIR_SWITCH = 8;  // kernel variant flag: LJ switching function on
+constexpr int PAIR_LJPME = 16;  // kernel variant flag: LJPME dispersion correction (nonbonded.rs:875-886)
+constexpr int ELEC_MASK = 7;
 
 struct PairParams {
     int nblocks = 0, cap = 0;
@@ -20,6 +22,8 @@ struct PairParams {
     int switching = 0;            // LJ switching function (nonbonded.rs:887-899)
     double switch_distance = 0.0;
     double inv_switch_width = 0.0;  // 1 / (cutoff - switch_distance)
+    int ljpme = 0;                  // LJPME direct-space dispersion term
+    double lj_alpha = 0.0;
 };
 
 // Per-particle parameters in the form the kernels use.
@@ -30,12 +34,14 @@ struct ParticleDev {
     DevBuf<float> sh;       // sigma / 2
     DevBuf<float> e2;       // 2 sqrt(epsilon)
     DevBuf<double> qk_d, sh_d, e2_d;  // FP64 copies for the plain form
+    DevBuf<double> c6amp;             // LJPME mesh amplitude 2 sqrt(epsilon) sigma^3 (pme.rs:50-51)
 };
 
 struct ExceptionDev {
     int count = 0;
     DevBuf<int> p1, p2;
     DevBuf<double> qq, sig, eps;
+    DevBuf<double> c6;  // LJPME: the particles' C6 (the dispersion correction), else empty
 };
 
 // Direct-space pairs over the tile list.  mixed: FP32 math with int64
@@ -57,8 +63,10 @@ void launch_slot_forces(bool mixed, const NlistDev& nl, long long* force_fx, dou
 
 // Exceptions (nonbonded.rs:995-1036), FP64 math.  has_pme selects the erf
 // reciprocal correction with particle charges.
+// lj_alpha > 0 adds the LJPME dispersion correction with ex.c6.
 void launch_exceptions(bool mixed, const ExceptionDev& ex, const ParticleDev& part, const double* pos,
-                       const SysInfo* sys_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
-                       double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s);
+                       const SysInfo* sys_dev, int use_pbc, int has_pme, double alpha, double lj_alpha,
+                       long long* force_fx, double* force_d, long long* energy_fx, double* energy_d, unsigned* status,
+                       hipStream_t s);
 
 }  // namespace ommhip

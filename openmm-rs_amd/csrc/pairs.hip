@@ -22,6 +22,7 @@
 // for this.
 #include <climits>
 
+#include "fmath.h"
 #include "pairs.h"
 #include "terms.h"
 
@@ -36,48 +37,6 @@ struct alignas(4 * sizeof(R)) Q4 {
 template <class R>
 struct alignas(2 * sizeof(R)) Q2 {
     R x, y;
-};
-
-template <class R>
-struct Math;
-
-// FP32 pair arithmetic.  erfc(x) = erfcx(x) * exp(-x^2) with erfcx fitted
-// (this project's own weighted Chebyshev fit, converted to monomials) as a
-// degree-8 polynomial in t = (x - 2.5)/(x + 2.5) on x in [0, 6]: relative error
-// 1.2e-8 in exact arithmetic, 1.6e-7 evaluated in FP32.  exp(-x^2) reuses the
-// hardware exp2; reciprocal and reciprocal square root are the hardware's
-// (about 1 ulp), well inside the FP32 rounding of the force sums.
-template <>
-struct Math<float> {
-    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
-    static __device__ __forceinline__ void erfc_exp(float x, float& erfc_v, float& ex) {
-        float t = (x - 2.5f) * rcp(x + 2.5f);
-        t = fminf(t, 0.41176470f);
-        float p = 0.0003302688419353217f;
-        p = fmaf(p, t, 0.0008612216333858669f);
-        p = fmaf(p, t, -0.0026842965744435787f);
-        p = fmaf(p, t, -0.003998667001724243f);
-        p = fmaf(p, t, 0.039928220212459564f);
-        p = fmaf(p, t, -0.12503232061862946f);
-        p = fmaf(p, t, 0.2517129182815552f);
-        p = fmaf(p, t, -0.3717367649078369f);
-        p = fmaf(p, t, 0.21080636978149414f);
-        ex = __builtin_amdgcn_exp2f(-(x * x) * 1.4426950408889634f);
-        erfc_v = p * ex;
-    }
-    static __device__ __forceinline__ float round(float x) { return rintf(x); }
-};
-
-template <>
-struct Math<double> {
-    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-    static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / sqrt(x); }
-    static __device__ __forceinline__ void erfc_exp(double x, double& erfc_v, double& ex) {
-        erfc_v = erfc(x);
-        ex = exp(-x * x);
-    }
-    static __device__ __forceinline__ double round(double x) { return rint(x); }
 };
 
 // The reference's cutoff decision for the pair (a, b): delta = x_j - x_i with
@@ -107,8 +66,8 @@ __device__ __forceinline__ void lj_switch(R r, const PairParams& P, R& s, R& ds)
 // FP64 energy of one pair (nonbonded.rs:1203-1214, 1219-1273) from offsets.
 template <int KIND>
 __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double dz, double sig, double eps4,
-                                                   double qq, const PairParams& P) {
-    constexpr int ELEC = KIND & ~PAIR_SWITCH;
+                                                   double ss, double qq, const PairParams& P) {
+    constexpr int ELEC = KIND & ELEC_MASK;
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double r = sqrt(r2);
     double e = 0.0;
@@ -116,6 +75,15 @@ __device__ __forceinline__ double pair_energy_fp64(double dx, double dy, double 
         const double s2 = sig * sig / r2;
         const double s6 = s2 * s2 * s2;
         e = eps4 * (s6 * s6 - s6);
+        if (KIND & PAIR_LJPME) {
+            // nonbonded.rs:875-886: + the reciprocal dispersion of the pair,
+            // minus both terms at the cutoff.  C6 = 4 sqrt(eps_i eps_j)
+            // (sigma_i sigma_j)^3 = eps4 (4 ss)^3 with ss = (sigma_i/2)(sigma_j/2).
+            const double c6 = eps4 * 64.0 * ss * ss * ss, rc2 = P.cutoff * P.cutoff, a = P.lj_alpha;
+            double dd;
+            const double sc2 = sig * sig / rc2, sc6 = sc2 * sc2 * sc2;
+            e += disp_recip_pair(c6, r2, a, &dd) - (eps4 * (sc6 * sc6 - sc6) + disp_recip_pair(c6, rc2, a, &dd));
+        }
         if ((KIND & PAIR_SWITCH) && r > P.switch_distance) {
             double sw, dsw;
             lj_switch<double>(r, P, sw, dsw);
@@ -198,8 +166,9 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                                               unsigned* status, unsigned long long* stats, int* list_flag,
                                               int nflags) {
     using M = Math<R>;
-    constexpr int ELEC = KIND & ~PAIR_SWITCH;
+    constexpr int ELEC = KIND & ELEC_MASK;
     constexpr bool SWITCH = (KIND & PAIR_SWITCH) != 0;
+    constexpr bool LJPME = (KIND & PAIR_LJPME) != 0;
     constexpr bool E64 = ENERGY && sizeof(R) == 4;  // separate FP64 energy path
     __shared__ Q4<R> s_pq[JTILE];
     __shared__ Q2<R> s_se[JTILE];
@@ -218,6 +187,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
     const R alpha = (R)P.alpha;
     const R two_a_sqrtpi = (R)(2.0 * P.alpha / sqrt(PI));
     const R krf = (R)P.krf, crf = (R)P.crf;
+    const R lj_a2 = (R)(P.lj_alpha * P.lj_alpha);
     // This kernel runs after every list build and before the next integrator,
     // so it serves the rebuild requests.
     if (blockIdx.x == 0)
@@ -376,6 +346,28 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 const R s12 = s6 * s6;
                 R e = eps4 * (s12 - s6);
                 R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
+                if (LJPME) {
+                    // nonbonded.rs:875-886 with C6 = eps4 (4 (sigma_i/2)(sigma_j/2))^3.
+                    // Closed form (no small-x series: x = alpha^2 r^2 < 0.01 needs
+                    // r < 0.1/alpha, far inside any non-excluded pair's LJ core).
+                    const R ss = si * sj.x;
+                    const R c6 = eps4 * (R)64 * ss * ss * ss;
+                    const R x = lj_a2 * r2;
+                    const R ex = M::exp_neg(x);
+                    const R poly = (R)1 + x * ((R)1 + (R)0.5 * x);
+                    const R c6r6 = c6 * inv_r2 * inv_r2 * inv_r2;
+                    f -= (R)6 * c6r6 * inv_r2 * ((R)1 - ex * (poly + x * x * x * (R)(1.0 / 6.0)));
+                    if (ENERGY && !E64) {
+                        // The plain form's energy (the mixed form's is the FP64 pair energy).
+                        const R rc2 = (R)(P.cutoff * P.cutoff);
+                        const R sc2 = sig * sig / rc2, sc6 = sc2 * sc2 * sc2;
+                        const R xc = lj_a2 * rc2;
+                        const R shift = eps4 * (sc6 * sc6 - sc6) +
+                                        c6 / (rc2 * rc2 * rc2) *
+                                            ((R)1 - M::exp_neg(xc) * ((R)1 + xc * ((R)1 + (R)0.5 * xc)));
+                        e += c6r6 * ((R)1 - ex * poly) - shift;
+                    }
+                }
                 if (SWITCH) {
                     const R r = r2 * inv_r;
                     if (r > (R)P.switch_distance) {
@@ -415,7 +407,8 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                         ddy = dd[1];
                         ddz = dd[2];
                     }
-                    eacc += pair_energy_fp64<KIND>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, qid * pjd.w, P);
+                    eacc += pair_energy_fp64<KIND>(ddx, ddy, ddz, sid + sjd.x, eid * sjd.y, sid * sjd.x, qid * pjd.w,
+                                                   P);
                 } else if (ENERGY) {
                     eacc += (double)e;
                 }
@@ -535,8 +528,8 @@ __global__ void k_exceptions(int count, const int* __restrict__ p1, const int* _
                              const double* __restrict__ eqq, const double* __restrict__ esig,
                              const double* __restrict__ eeps, const double* __restrict__ charge,
                              const double* __restrict__ pos, const int* __restrict__ atom_sys,
-                             const SysInfo* __restrict__ sys, int use_pbc, int has_pme, double alpha, A* force,
-                             A* energy, unsigned* status) {
+                             const SysInfo* __restrict__ sys, int use_pbc, int has_pme, double alpha,
+                             const double* __restrict__ c6, double lj_alpha, A* force, A* energy, unsigned* status) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double e = 0.0;
     int si = -1;
@@ -544,7 +537,7 @@ __global__ void k_exceptions(int count, const int* __restrict__ p1, const int* _
         int i = p1[k], j = p2[k];
         si = atom_sys[i];
         e = exception_term(i, j, eqq[k], esig[k], eeps[k], charge[i], charge[j], pos, sys[si].box, use_pbc, has_pme,
-                           alpha, force, status);
+                           alpha, c6 ? c6[k] : 0.0, lj_alpha, force, status);
     }
     wave_energy_add(energy, si, E_EXCEPT, e, status);
 }
@@ -643,6 +636,10 @@ void dispatch_kind(int kind, bool st, bool energy_on, const NlistDev& nl, const 
                                                                     sys_dev, force, energy, status, stats, grid,
                                                                     combine, s);
             break;
+        case ELEC_EWALD | PAIR_LJPME:
+            launch_pairs_t<R, A, ELEC_EWALD | PAIR_LJPME>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force,
+                                                          energy, status, stats, grid, combine, s);
+            break;
         default:
             throw Error(OMM_ERR_INVALID, "no pair kernel for this electrostatics / switching combination");
     }
@@ -654,7 +651,7 @@ void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairPara
                   double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s,
                   bool combine) {
     const bool st = stats != nullptr;
-    const int kind = p.elec | (p.switching ? PAIR_SWITCH : 0);
+    const int kind = p.elec | (p.switching ? PAIR_SWITCH : 0) | (p.ljpme ? PAIR_LJPME : 0);
     if (mixed)
         dispatch_kind<float, long long>(kind, st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part, pos,
                                         sys_dev, force_fx, energy_fx, status, stats, grid, combine, s);
@@ -675,18 +672,20 @@ void launch_slot_forces(bool mixed, const NlistDev& nl, long long* force_fx, dou
 }
 
 void launch_exceptions(bool mixed, const ExceptionDev& ex, const ParticleDev& part, const double* pos,
-                       const SysInfo* sys_dev, int use_pbc, int has_pme, double alpha, long long* force_fx,
-                       double* force_d, long long* energy_fx, double* energy_d, unsigned* status, hipStream_t s) {
+                       const SysInfo* sys_dev, int use_pbc, int has_pme, double alpha, double lj_alpha,
+                       long long* force_fx, double* force_d, long long* energy_fx, double* energy_d, unsigned* status,
+                       hipStream_t s) {
     if (ex.count == 0) return;
     int blocks = (ex.count + 255) / 256;
+    const double* c6 = lj_alpha > 0.0 ? ex.c6.p : nullptr;
     if (mixed)
         hipLaunchKernelGGL(k_exceptions<long long>, dim3(blocks), dim3(256), 0, s, ex.count, ex.p1.p, ex.p2.p, ex.qq.p,
                            ex.sig.p, ex.eps.p, part.charge.p, pos, part.atom_sys.p, sys_dev, use_pbc, has_pme, alpha,
-                           force_fx, energy_fx, status);
+                           c6, lj_alpha, force_fx, energy_fx, status);
     else
         hipLaunchKernelGGL(k_exceptions<double>, dim3(blocks), dim3(256), 0, s, ex.count, ex.p1.p, ex.p2.p, ex.qq.p,
                            ex.sig.p, ex.eps.p, part.charge.p, pos, part.atom_sys.p, sys_dev, use_pbc, has_pme, alpha,
-                           force_d, energy_d, status);
+                           c6, lj_alpha, force_d, energy_d, status);
     OMM_HIP(hipGetLastError());
 }
 

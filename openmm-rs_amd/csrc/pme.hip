@@ -15,6 +15,7 @@
 #include <complex>
 #include <type_traits>
 
+#include "fmath.h"
 #include "pme.h"
 
 namespace ommhip {
@@ -483,6 +484,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         const R* mod_z = moduli_all + S.moduli_off[2];
         const double pi2_a2 = PI * PI / (S.alpha * S.alpha);
         const double pref = ONE_4PI_EPS0 / (PI * b.volume);
+        // Dispersion mesh (pme.rs:358-362): I(k) = -pi^1.5 / (3 V B(k))
+        // (2 pi^3 sqrt(pi) erfc(b) k^3 + e^{-b^2} (alpha^3 - 2 alpha pi^2 k^2)),
+        // b = pi k / alpha, with the k = 0 term kept.
+        const bool disp = S.dispersion != 0;
+        const double dpref = -PI * sqrt(PI) / (3.0 * b.volume);
+        const double dc1 = 2.0 * PI * PI * PI * sqrt(PI), da3 = S.alpha * S.alpha * S.alpha;
+        const double dc2 = 2.0 * S.alpha * PI * PI, pi_a = PI / S.alpha;
         // The FP32 mesh serves force-only evaluations: the influence function
         // in FP32 (exp2 of the hardware), no energy.  The FP64 meshes evaluate
         // it in FP64 and sum the energy.
@@ -498,8 +506,27 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
             if (L >= nlines) continue;
             const int y = L / nzh, kz = L - y * nzh;
             Cx<R>& S = res[li * n + kx];
-            if (kx == 0 && y == 0 && kz == 0) {
+            if (kx == 0 && y == 0 && kz == 0 && !disp) {
                 S = Cx<R>{0, 0};
+                continue;
+            }
+            if (F32 && disp) {
+                const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
+                const float my = (float)(y < (ny + 1) / 2 ? y : y - ny);
+                const float mz = (float)(kz < (nz + 1) / 2 ? kz : kz - nz);
+                float k2 = 0.0f;
+                for (int a = 0; a < 3; ++a) {
+                    const float kv = mx * rf[a] + my * rf[3 + a] + mz * rf[6 + a];
+                    k2 += kv * kv;
+                }
+                const float kk = __builtin_sqrtf(k2);
+                float ec, ex;
+                Math<float>::erfc_exp((float)pi_a * kk, ec, ex);
+                const float mod = (float)mod_x[kx] * (float)mod_y[y] * (float)mod_z[kz];
+                const float infl =
+                    (float)dpref * ((float)dc1 * ec * kk * k2 + ex * ((float)da3 - (float)dc2 * k2)) / mod;
+                S.x = (R)((float)S.x * infl);
+                S.y = (R)((float)S.y * infl);
                 continue;
             }
             if (F32) {
@@ -526,7 +553,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
                 k2 += kv * kv;
             }
             const double mod = (double)mod_x[kx] * (double)mod_y[y] * (double)mod_z[kz];
-            const double infl = pref * exp(-pi2_a2 * k2) / (k2 * mod);
+            double infl;
+            if (disp) {
+                const double kk = sqrt(k2), bb = pi_a * kk;
+                infl = dpref / mod * (dc1 * erfc(bb) * kk * k2 + exp(-bb * bb) * (da3 - dc2 * k2));
+            } else {
+                infl = pref * exp(-pi2_a2 * k2) / (k2 * mod);
+            }
             const double wz = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
             const double s2 = (double)S.x * S.x + (double)S.y * S.y;
             eacc += wz * 0.5 * infl * s2;

@@ -7,13 +7,42 @@
 
 namespace ommhip {
 
+// LJPME reciprocal-space dispersion of a pair (nonbonded.rs:1158-1191):
+// E = C6 (1 - e^{-x} (1 + x + x^2/2)) / r^6 with x = alpha^2 r^2, and
+// dE/dr / r; below x = 0.01 the power series of (1 - e^{-x}(1 + x + x^2/2))/x^3
+// (terms t_k = t_{k-1} (-x)(k + 2)/((k + 3) k), t_0 = 1/6) avoids the
+// cancellation.
+__device__ __forceinline__ double disp_recip_pair(double c6, double r2, double alpha, double* deriv) {
+    if (c6 == 0.0) {
+        *deriv = 0.0;
+        return 0.0;
+    }
+    const double x = alpha * alpha * r2;
+    if (x < 0.01) {
+        double term = 1.0 / 6.0, value = term, d = 0.0;
+        for (int k = 1; k <= 12; ++k) {
+            term *= -x * (k + 2) / (double)((k + 3) * k);
+            value += term;
+            d += x == 0.0 ? (k == 1 ? -1.0 / 8.0 : 0.0) : k * term / x;
+        }
+        const double a2 = alpha * alpha, a6 = a2 * a2 * a2;
+        *deriv = 2.0 * c6 * a6 * a2 * d;
+        return c6 * a6 * value;
+    }
+    const double ex = exp(-x), inv_r6 = 1.0 / (r2 * r2 * r2);
+    const double poly = 1.0 + x + 0.5 * x * x;
+    *deriv = -6.0 * c6 * inv_r6 / r2 * (1.0 - ex * (poly + x * x * x / 6.0));
+    return c6 * inv_r6 * (1.0 - ex * poly);
+}
+
 // Exception pair (nonbonded.rs:995-1036): LJ and bare Coulomb with the pair's
 // own parameters (lj_coulomb_exception, 1194-1198), minus the reciprocal-space
 // erf correction with the particle charges (reciprocal_pair, 1275-1288) when
 // a reciprocal method is active.  Force on i is fi, on j is -fi.
 __device__ __forceinline__ double exception_xyz(const double pi[3], const double pj[3], double qq, double sig,
                                                 double eps, double qi, double qj, const Box& box, int use_pbc,
-                                                int has_pme, double alpha, double fi[3]) {
+                                                int has_pme, double alpha, double c6, double lj_alpha,
+                                                double fi[3]) {
     double d[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
     if (use_pbc) min_image_exact(d, box.v);
     double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -44,16 +73,23 @@ __device__ __forceinline__ double exception_xyz(const double pi[3], const double
         e -= corr;
         f -= deriv;
     }
+    if (c6 != 0.0) {
+        // LJPME: the particles' reciprocal-space dispersion is added back.
+        double deriv;
+        e += disp_recip_pair(c6, r2, lj_alpha, &deriv);
+        f += deriv;
+    }
     for (int a = 0; a < 3; ++a) fi[a] = f * d[a];
     return e;
 }
 
 __device__ __forceinline__ double exception_pair(int i, int j, double qq, double sig, double eps, double qi, double qj,
                                                  const double* __restrict__ pos, const Box& box, int use_pbc,
-                                                 int has_pme, double alpha, double fi[3]) {
+                                                 int has_pme, double alpha, double c6, double lj_alpha,
+                                                 double fi[3]) {
     const double pi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
     const double pj[3] = {pos[3 * j], pos[3 * j + 1], pos[3 * j + 2]};
-    return exception_xyz(pi, pj, qq, sig, eps, qi, qj, box, use_pbc, has_pme, alpha, fi);
+    return exception_xyz(pi, pj, qq, sig, eps, qi, qj, box, use_pbc, has_pme, alpha, c6, lj_alpha, fi);
 }
 
 // HarmonicBondForce term without PBC (harmonic_bond.rs:113-140).  Force on i
@@ -110,9 +146,10 @@ __device__ __forceinline__ double angle_triple(int i, int j, int k, double theta
 template <class A>
 __device__ __forceinline__ double exception_term(int i, int j, double qq, double sig, double eps, double qi, double qj,
                                                  const double* __restrict__ pos, const Box& box, int use_pbc,
-                                                 int has_pme, double alpha, A* force, unsigned* status) {
+                                                 int has_pme, double alpha, double c6, double lj_alpha, A* force,
+                                                 unsigned* status) {
     double fi[3];
-    const double e = exception_pair(i, j, qq, sig, eps, qi, qj, pos, box, use_pbc, has_pme, alpha, fi);
+    const double e = exception_pair(i, j, qq, sig, eps, qi, qj, pos, box, use_pbc, has_pme, alpha, c6, lj_alpha, fi);
     for (int a = 0; a < 3; ++a) {
         acc_add(force + 3 * i + a, fi[a], FORCE_SCALE, status);
         acc_add(force + 3 * j + a, -fi[a], FORCE_SCALE, status);

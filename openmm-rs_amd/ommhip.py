@@ -104,6 +104,8 @@ class NbDesc(C.Structure):
         ("particle_offset_parameter", _ip), ("particle_offset_index", _ip), ("particle_offset_scales", _dp),
         ("num_exception_offsets", C.c_int32),
         ("exception_offset_parameter", _ip), ("exception_offset_index", _ip), ("exception_offset_scales", _dp),
+        ("ljpme_alpha", C.c_double),
+        ("ljpme_grid", C.c_int32 * 3),
     ]
 
 
@@ -210,6 +212,7 @@ class NonbondedForce:
     reaction_field_dielectric: float = 78.3
     ewald_error_tolerance: float = 5e-4
     pme: tuple = (0.0, (0, 0, 0))
+    ljpme: tuple = (0.0, (0, 0, 0))
     use_dispersion_correction: bool = True
     exceptions_use_pbc: bool = False
     use_switching_function: bool = False
@@ -301,6 +304,14 @@ class NonbondedForce:
         if alpha != 0.0 and min(nx, ny, nz) < 5:
             raise ValueError("PME grid dimensions must be at least five")
         self.pme = (float(alpha), (int(nx), int(ny), int(nz))) if alpha != 0.0 else (0.0, (0, 0, 0))
+
+    def set_ljpme_parameters(self, alpha, nx, ny, nz):
+        """nonbonded.rs:615-630."""
+        if not (math.isfinite(alpha) and alpha >= 0.0):
+            raise ValueError("LJPME alpha must be nonnegative and finite")
+        if alpha != 0.0 and min(nx, ny, nz) < 5:
+            raise ValueError("LJPME grid dimensions must be at least five")
+        self.ljpme = (float(alpha), (int(nx), int(ny), int(nz))) if alpha != 0.0 else (0.0, (0, 0, 0))
 
     def set_use_dispersion_correction(self, enabled):
         self.use_dispersion_correction = bool(enabled)
@@ -420,6 +431,9 @@ class NonbondedForce:
         d.exception_offset_parameter = _ptr(keep["eop"], C.c_int32)
         d.exception_offset_index = _ptr(keep["eoi"], C.c_int32)
         d.exception_offset_scales = _ptr(keep["eos"], C.c_double)
+        d.ljpme_alpha = self.ljpme[0]
+        for a in range(3):
+            d.ljpme_grid[a] = self.ljpme[1][a]
         d._keep = keep
         return d
 
@@ -450,6 +464,12 @@ class GpuNonbondedForce:
         alpha = C.c_double()
         grid = (C.c_int32 * 3)()
         _check(library().omm_nb_get_pme_parameters(self._h, C.byref(alpha), grid))
+        return alpha.value, tuple(grid)
+
+    def ljpme_parameters(self):
+        alpha = C.c_double()
+        grid = (C.c_int32 * 3)()
+        _check(library().omm_nb_get_ljpme_parameters(self._h, C.byref(alpha), grid))
         return alpha.value, tuple(grid)
 
     def calc_forces_and_energy(self, pos, box=None):

@@ -267,7 +267,7 @@ def water_energy_forces(system, ljpme=None, include_bonded=True):
     """Total FP64 forces/energy of a WaterSystem (nonbonded with its frozen PME
     parameters + harmonic bonds + angles): the reference Context's force sum."""
     desc = system.nonbonded.desc()
-    f, e, parts = nb_evaluate(desc, system.positions, system.box)
+    f, e, parts = nb_evaluate(desc, system.positions, system.box, ljpme=ljpme)
     if include_bonded:
         fb, eb = harmonic_bonds(system.bonds, system.bond_params, system.positions)
         fa, ea = harmonic_angles(system.angles, system.angle_params, system.positions)

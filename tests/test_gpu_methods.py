@@ -7,8 +7,7 @@ in reach), so their results must equal the reference's non-periodic sums.
 The method matrix is the reference's acceptance grid: the 41 four-particle
 cases of tests/reference/nonbonded.cpp:13-86 (inputs only; that file needs an
 external OpenMM build for its values, so each case is checked against the
-oracle here).  Cases of the method the device does not run yet (LJPME) must
-be rejected at initialisation with OMM_ERR_INVALID.
+oracle here), every one on the device.
 """
 import copy
 
@@ -58,6 +57,7 @@ def case_force(name):
     f.set_exceptions_use_periodic_boundary_conditions("wrapped" in name)
     if "explicit" in name:
         f.set_pme_parameters(2.7, 24, 25, 28)
+        f.set_ljpme_parameters(2.5, 18, 20, 21)
     charges = [0.4, -0.3, 0.1, 0.2 if "charged" in name else -0.2]
     for i in range(4):
         f.add_particle(0.0 if "neutral" in name else charges[i], 0.3 + 0.02 * i, 0.5 + 0.1 * i)
@@ -100,17 +100,21 @@ def run_case(name, precision):
     F, E = op.calc_forces_and_energy(POSITIONS, box)
     ref = copy.deepcopy(f)
     ref.global_parameters = [(k, values[k]) for k, _ in f.global_parameters]
-    if f.method in (ommhip.METHODS["PME"], ommhip.METHODS["Ewald"]):
+    ljpme = None
+    if f.method in (ommhip.METHODS["PME"], ommhip.METHODS["Ewald"], ommhip.METHODS["LJPME"]):
         # alpha and the grid (PME) or wave-vector limits (Ewald) are frozen at
         # initialize (nonbonded.rs:1055-1107); the oracle takes them explicitly.
         alpha, grid = op.pme_parameters()
         ref.set_pme_parameters(alpha, *grid)
-    F_ref, E_ref, _ = orc.nb_evaluate(ref.desc(), POSITIONS, box)
+    if f.method == ommhip.METHODS["LJPME"]:
+        ljpme = op.ljpme_parameters()
+        if "explicit" in name:
+            assert ljpme == (2.5, (18, 20, 21))
+    F_ref, E_ref, _ = orc.nb_evaluate(ref.desc(), POSITIONS, box, ljpme=ljpme)
     return F, E, F_ref, E_ref
 
 
-DEVICE_CASES = [c for c in CASES if not c.startswith("ljpme")]
-HOST_ONLY_CASES = [c for c in CASES if c.startswith("ljpme")]
+DEVICE_CASES = CASES
 
 
 @pytest.mark.gpu
@@ -130,16 +134,6 @@ def test_method_matrix_mixed(name):
     print(f"{name}: force_rms_rel={frel:.3e} energy_rel={erel:.3e}")
     assert frel <= FORCE_TOL
     assert erel <= ENERGY_TOL
-
-
-# Host-side validation: runs without a GPU.
-@pytest.mark.parametrize("name", HOST_ONLY_CASES)
-def test_method_matrix_rejects_methods_not_on_device(name):
-    f, box = case_force(name)
-    op = ommhip.GpuNonbondedForce(f)
-    with pytest.raises(ommhip.OmmError) as e:
-        op.initialize(box)
-    assert e.value.status == 1
 
 
 def droplet(method, molecules=300, seed=61):
@@ -256,3 +250,59 @@ def test_ewald_rejects_triclinic_box():
     with pytest.raises(ommhip.OmmError) as e:
         op.initialize(np.array([[2.1, 0, 0], [0.3, 2.1, 0], [0, 0, 2.1]]))
     assert e.value.status == 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,ftol,etol", [(ommhip.PRECISION_MIXED, FORCE_TOL, ENERGY_TOL),
+                                                 (ommhip.PRECISION_DOUBLE, 1e-9, 1e-9)])
+def test_ljpme_water_parity(precision, ftol, etol):
+    """LJPME (dispersion mesh, nonbonded.rs:875-886, 974-989) on 1,000 waters."""
+    s = water(1000, 3.1, seed=67, method="LJPME")
+    desc = s.nonbonded.desc()
+    op = ommhip.GpuNonbondedForce(s.nonbonded, precision)
+    op.initialize(s.box)
+    tol, rc = s.nonbonded.ewald_error_tolerance, s.nonbonded.cutoff
+    assert op.ljpme_parameters() == orc.effective_ljpme_parameters(tol, rc, s.box)
+    assert op.pme_parameters() == orc.effective_pme_parameters(tol, rc, s.box)
+    F, E = op.calc_forces_and_energy(s.positions, s.box)
+    F_ref, E_ref, parts_ref = orc.nb_evaluate(desc, s.positions, s.box)
+    parts = op.energy_parts(s.positions, s.box)
+    frel, erel = force_rms_rel(F, F_ref), energy_rel(E, E_ref)
+    print(f"force_rms_rel={frel:.3e} energy_rel={erel:.3e} parts={parts} ref={parts_ref}")
+    assert frel <= ftol
+    assert erel <= etol
+    assert np.abs(parts - parts_ref).max() <= etol * np.abs(parts_ref).max()
+    F_only = op.calc_forces(s.positions, s.box)
+    assert force_rms_rel(F_only, F_ref) <= ftol
+    assert op.pair_stats(s.positions, s.box) == orc.nb_pairs(desc, s.positions, s.box)
+
+
+@pytest.mark.gpu
+def test_ljpme_md_context_with_barostat():
+    """The MD step under LJPME: state against the oracle, then NPT steps; after
+    the barostat has moved the box the forces still match the oracle with the
+    parameters frozen at initialisation (the dispersion mesh follows the box)."""
+    s = water(500, 2.5, seed=68, method="LJPME")
+    tol, rc = s.nonbonded.ewald_error_tolerance, s.nonbonded.cutoff
+    frozen = orc.effective_pme_parameters(tol, rc, s.box)
+    frozen_lj = orc.effective_ljpme_parameters(tol, rc, s.box)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, seed=4)
+    st = ctx.get_state()
+    F_ref, E_ref = orc.water_energy_forces(s)
+    assert force_rms_rel(st["forces"], F_ref) <= FORCE_TOL
+    assert energy_rel(st["potential_energy"], E_ref) <= ENERGY_TOL
+    ctx.set_velocities_to_temperature(300.0, 6)
+    ctx.set_barostat(1.0, 300.0, frequency=2, seed=9)
+    ctx.step(40)
+    info = ctx.barostat_info()
+    box = ctx.box()
+    st = ctx.get_state()
+    moved = copy.deepcopy(s)
+    moved.positions = st["positions"]
+    moved.box = box
+    moved.nonbonded.set_pme_parameters(frozen[0], *frozen[1])
+    F_ref, E_ref = orc.water_energy_forces(moved, ljpme=frozen_lj)
+    print("box", np.diag(box), "info", info)
+    assert info["total_accepted"] > 0
+    assert force_rms_rel(st["forces"], F_ref) <= FORCE_TOL
+    assert energy_rel(st["potential_energy"], E_ref) <= ENERGY_TOL
