@@ -253,9 +253,11 @@ __device__ __forceinline__ void wave_raise_flags(int* flag, bool moved, int si) 
 
 // Adds each lane's energy v to slot `slot` of its system si (si < 0: lane
 // idle) in the per-system energy array: one atomic per wave when the active
-// lanes share a system, one per lane otherwise.  Called by all 64 lanes.
+// lanes share a system, one per lane otherwise.  Called by all 64 lanes; a
+// null energy array (force-only evaluations) adds nothing.
 template <class A>
 __device__ __forceinline__ void wave_energy_add(A* energy, int si, int slot, double v, unsigned* status) {
+    if (!energy) return;  // force-only evaluation
     const int lead = __shfl(si, 0, 64);  // lane 0 holds the lowest index, so it is active if any lane is
     const bool uniform = __ballot(si >= 0 && si != lead) == 0ull;
     if (uniform) {

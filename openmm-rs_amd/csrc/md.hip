@@ -671,7 +671,8 @@ void MdContext::enqueue_forces_into(hipStream_t s, bool energy, int rebuild, lon
     const NbSettings& c = nb_->settings();
     nb_->enqueue(pos_.p, c.include_direct, true, energy, rebuild, false, force_fx, force_d, energy_fx, energy_d,
                  status_.p, s, false);
-    launch_bonded(s, force_fx, force_d, energy_fx, energy_d);  // concurrent with the mesh
+    // Concurrent with the mesh; force-only steps sum no bonded energies.
+    launch_bonded(s, force_fx, force_d, energy ? energy_fx : nullptr, energy ? energy_d : nullptr);
     nb_->join(s);
 }
 
@@ -1102,7 +1103,7 @@ void MdContext::profile(int reps, double ms[8], int64_t* num_pairs) {
         OMM_HIP(hipEventElapsedTime(&v, t0, t1));
         return (double)v / reps;
     };
-    ms[6] = timed([&] { launch_bonded(s_, force_fx_.p, force_d_.p, energy_fx_.p, energy_d_.p); });
+    ms[6] = timed([&] { launch_bonded(s_, force_fx_.p, force_d_.p, nullptr, nullptr); });
     // The integrator is timed on scratch copies of the state.
     DevBuf<double> px, vx;
     DevBuf<int> scratch_flag;  // stands in for the list's nsys + 1 rebuild flags
