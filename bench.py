@@ -71,7 +71,7 @@ def timed_steps(ctx, steps):
     return time.perf_counter() - t0
 
 
-GRAPH_STEPS = 10  # steps per captured graph; 0 with --eager
+GRAPH_STEPS = 50  # steps per captured graph (results do not depend on it); 0 with --eager
 
 
 def run_single(steps, warmup, equilibrate, plain_steps, seed=2024):

@@ -1,4 +1,6 @@
 """The device MD step (LangevinMiddle + bonds + angles + PME) against the oracle."""
+import copy
+
 import numpy as np
 import pytest
 
@@ -144,3 +146,28 @@ def test_larger_box_multi_chunk_paths():
     sa, sb = a.get_state(), b.get_state()
     assert np.array_equal(sa["positions"], sb["positions"])
     assert np.isfinite(sa["potential_energy"])
+
+
+@pytest.mark.parametrize("graph_steps", [10, 0])
+def test_fast_atom_keeps_the_list_exact(graph_steps):
+    # Atoms launched at ~48 nm/ps (0.05 nm per step, half the list skin)
+    # cross the skin/2 rebuild threshold within one or two steps, in graph
+    # replays and in eager steps alike; the list must follow them, so the
+    # forces after the run still match the oracle.
+    s = water(500, 2.5, seed=31)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, seed=7)
+    ctx.set_graph_steps(graph_steps)
+    ctx.set_velocities_to_temperature(300.0, 8)
+    v = ctx.get_state()["velocities"]
+    for k in range(5):
+        v[3 * 40 * k] = [40.0, -25.0, 10.0]
+    ctx.set_velocities(v)
+    ctx.step(30)
+    st = ctx.get_state()
+    moved = copy.deepcopy(s)
+    moved.positions = st["positions"]
+    F_ref, E_ref = orc.water_energy_forces(moved)
+    frel, erel = force_rms_rel(st["forces"], F_ref), energy_rel(st["potential_energy"], E_ref)
+    print(f"graph_steps={graph_steps}: force_rms_rel={frel:.3e} energy_rel={erel:.3e}")
+    assert frel <= 2e-6
+    assert erel <= 1e-6
