@@ -240,6 +240,14 @@ typedef struct omm_md_desc {
     double friction;                   /* 1/ps */
     double step_size;                  /* ps */
     uint64_t seed;
+    /* Distance constraints (System::add_constraint; constraints.rs): SHAKE
+     * positions / RATTLE velocities inside the LangevinMiddle step
+     * (langevin.rs:215-277).  0 constraints = unconstrained (default). */
+    int32_t num_constraints;
+    const int32_t* constraint_particle1;
+    const int32_t* constraint_particle2;
+    const double* constraint_distance;  /* nm */
+    double constraint_tolerance;        /* 0 = the integrator default 1e-5 (langevin.rs:63) */
 } omm_md_desc;
 
 typedef struct omm_ctx omm_ctx;
@@ -251,6 +259,12 @@ omm_status omm_ctx_create(const omm_md_desc* desc, int32_t precision, omm_ctx** 
 /* Context::set_positions / set_velocities (host N x 3). */
 omm_status omm_ctx_set_positions(omm_ctx* ctx, const double* pos);
 omm_status omm_ctx_set_velocities(omm_ctx* ctx, const double* vel);
+
+/* Context::apply_constraints / apply_velocity_constraints (context.rs:767-810):
+ * project the positions onto the constraints (against themselves), or the
+ * velocities (against the positions), with the given tolerance. */
+omm_status omm_ctx_apply_constraints(omm_ctx* ctx, double tolerance);
+omm_status omm_ctx_apply_velocity_constraints(omm_ctx* ctx, double tolerance);
 
 /* Context::set_velocities_to_temperature (context.rs:182-223): Maxwell-Boltzmann
  * velocities from `seed`, minus F/m * dt/2 (the LangevinMiddle velocity time

@@ -227,6 +227,14 @@ omm_status omm_ctx_set_velocities(omm_ctx* ctx, const double* vel) {
     CTX_CALL(if (!vel) throw Error(OMM_ERR_INVALID, "null velocities"); ctx->ctx.set_velocities(0, vel));
 }
 
+omm_status omm_ctx_apply_constraints(omm_ctx* ctx, double tolerance) {
+    CTX_CALL(ctx->ctx.project_constraints(tolerance, false));
+}
+
+omm_status omm_ctx_apply_velocity_constraints(omm_ctx* ctx, double tolerance) {
+    CTX_CALL(ctx->ctx.project_constraints(tolerance, true));
+}
+
 omm_status omm_ctx_set_velocities_to_temperature(omm_ctx* ctx, double temperature, uint64_t seed) {
     CTX_CALL(ctx->ctx.set_velocities_to_temperature(temperature, seed));
 }

@@ -38,6 +38,7 @@ enum : unsigned {
     ST_LIST_OVERFLOW = 4u,
     ST_BIN_OVERFLOW = 8u,  // a PME spread tile received more atoms than its bin holds
     ST_CUTOFF_BOX = 16u,   // a box edge fell below twice the cutoff
+    ST_CONSTRAINTS = 32u,  // a constraint projection was singular or did not converge
 };
 
 // Energy slots of the device energy accumulator (fixed point).

@@ -32,6 +32,21 @@ struct BaroState {
     int accepted_last, skipped;
 };
 
+// Distance constraints (constraints.rs) of massive particles, grouped into
+// connected components: one thread projects one component, sweeping its
+// constraints in the reference's order (constraints only couple atoms of a
+// component, so this is the reference's global Gauss-Seidel loop).
+struct ConstraintDev {
+    int ncomp = 0, ncons = 0;
+    double tolerance = 1e-5;
+    DevBuf<int> atom_off, atoms;  // [ncomp + 1] prefix, atoms of each component (ascending)
+    DevBuf<int> cons_off;         // [ncomp + 1] prefix into the constraint arrays
+    DevBuf<int> c1, c2;           // global particle indices, reference order within a component
+    DevBuf<double> d2, w1, w2;    // distance^2, inverse masses
+    DevBuf<unsigned char> constrained;  // [N]: integrated by the constraint kernel
+    DevBuf<double> prev, proposed;      // [3N] step scratch
+};
+
 struct BaroDev {
     bool enabled = false;
     int frequency = 0, phase = 0;  // phase: steps since the last attempt slot
@@ -97,12 +112,18 @@ public:
 
     std::mutex mutex;
 
+    // Context::apply_constraints / apply_velocity_constraints (context.rs:767-810):
+    // projects the positions (against themselves) or the velocities (against
+    // the positions) onto the constraints with this tolerance.
+    void project_constraints(double tolerance, bool velocities);
+
 private:
     void enqueue_forces(hipStream_t s, bool energy, int rebuild);
     void enqueue_forces_into(hipStream_t s, bool energy, int rebuild, long long* force_fx, double* force_d,
                              long long* energy_fx, double* energy_d);
     void enqueue_step(hipStream_t s, int rebuild, bool barostat);
     void enqueue_barostat(hipStream_t s);
+    void setup_constraints(const omm_md_desc* descs, int nsys, const std::vector<double>& inv);
     void launch_bonded(hipStream_t s, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d);
     int graph_length() const { return baro_.enabled ? baro_.frequency : graph_steps_; }
     void build_bonded(const std::vector<int>& e1, const std::vector<int>& e2, const std::vector<double>& eqq,
@@ -134,6 +155,7 @@ private:
     int graph_steps_ = 10;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
     BondedDev bonded_;
+    ConstraintDev cons_;
     std::vector<int> bonded_exc_order_;  // exception k of the bonded arrays = global exception order[k]
     BaroDev baro_;
     std::vector<int> bond1_h_, bond2_h_;  // bonds in global indices (molecules)

@@ -1,10 +1,14 @@
 // Device-resident MD step for the flexible-water workloads: NonbondedForce
 // (PME) + HarmonicBondForce + HarmonicAngleForce + LangevinMiddle, captured
 // into a HIP graph (Context::step, context.rs:795-928).
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <functional>
 #include <random>
+#include <map>
+#include <numeric>
+#include <set>
 #include <type_traits>
 
 #include "md.h"
@@ -152,6 +156,27 @@ __global__ void k_copy2(const double* __restrict__ a, double* __restrict__ a_out
 inline __device__ double force_value(long long f) { return (double)f * (1.0 / FORCE_SCALE); }
 inline __device__ double force_value(double f) { return f; }
 
+// The three standard normals of atom i (global index) at the current step:
+// Philox keyed by the system's seed, counter (index within the system,
+// step), then Box-Muller in FP32 with the hardware log2 and sin/cos (whose
+// argument is in turns): the normals only scale the noise term, and ~1e-7
+// relative error in them is statistically invisible.
+__device__ __forceinline__ void step_normals(int i, const SysInfo& S, unsigned long long step, double xi[3]) {
+    const uint64_t seed = S.seed;
+    uint32_t c[4] = {(uint32_t)(i - S.atom_off), (uint32_t)step, (uint32_t)(step >> 32), 0x4c464d31u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float u1 = ((float)(c[0] >> 8) + 1.0f) * (1.0f / 16777216.0f);  // (0, 1]
+    const float u2 = (float)(c[1] >> 8) * (1.0f / 16777216.0f);           // [0, 1)
+    const float u3 = ((float)(c[2] >> 8) + 1.0f) * (1.0f / 16777216.0f);
+    const float u4 = (float)(c[3] >> 8) * (1.0f / 16777216.0f);
+    const float ln2 = 0.6931471805599453f;
+    const float r1 = __builtin_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
+    const float r2 = __builtin_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u3));
+    xi[0] = (double)(r1 * __builtin_amdgcn_cosf(u2));
+    xi[1] = (double)(r1 * __builtin_amdgcn_sinf(u2));
+    xi[2] = (double)(r2 * __builtin_amdgcn_cosf(u4));
+}
+
 // LangevinMiddle update (langevin.rs:103-142) with counter-based normals:
 // Philox keyed by the system's seed, counter (index within the system,
 // step), so a replica draws the same numbers alone or in a batch.  The force
@@ -164,11 +189,11 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
                            const double* __restrict__ inv_mass, const int* __restrict__ atom_sys,
                            const SysInfo* __restrict__ sys, double dt, double decay, double variance,
                            const unsigned long long* __restrict__ step, const double* __restrict__ pos_build,
-                           double lim2, int check, int* flag) {
+                           double lim2, int check, int* flag, const unsigned char* __restrict__ constrained) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     int si = 0;
-    if (i < n) {
+    if (i < n && !(constrained && constrained[i])) {  // constrained atoms: k_constrained_step
         double im = inv_mass[i];
         double f[3];
         for (int a = 0; a < 3; ++a) {
@@ -177,22 +202,8 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
         }
         if (im != 0.0) {  // massless particles do not move
             si = atom_sys[i];
-            const SysInfo& S = sys[si];
-            const uint64_t seed = S.seed;
-            uint32_t c[4] = {(uint32_t)(i - S.atom_off), (uint32_t)step[0], (uint32_t)(step[0] >> 32), 0x4c464d31u};
-            philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-            // Box-Muller in FP32 with the hardware log2 and sin/cos (whose
-            // argument is in turns): the normals only scale the noise term,
-            // and ~1e-7 relative error in them is statistically invisible.
-            const float u1 = ((float)(c[0] >> 8) + 1.0f) * (1.0f / 16777216.0f);  // (0, 1]
-            const float u2 = (float)(c[1] >> 8) * (1.0f / 16777216.0f);           // [0, 1)
-            const float u3 = ((float)(c[2] >> 8) + 1.0f) * (1.0f / 16777216.0f);
-            const float u4 = (float)(c[3] >> 8) * (1.0f / 16777216.0f);
-            const float ln2 = 0.6931471805599453f;
-            const float r1 = __builtin_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u1));
-            const float r2 = __builtin_sqrtf(-2.0f * ln2 * __builtin_amdgcn_logf(u3));
-            const double xi[3] = {(double)(r1 * __builtin_amdgcn_cosf(u2)), (double)(r1 * __builtin_amdgcn_sinf(u2)),
-                                  (double)(r2 * __builtin_amdgcn_cosf(u4))};
+            double xi[3];
+            step_normals(i, sys[si], step[0], xi);
             double noise = sqrt(variance * im);
             double half = 0.5 * dt;
             double d2 = 0.0;
@@ -211,6 +222,132 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
         }
     }
     wave_raise_flags(flag, moved, si);
+}
+
+// One SHAKE (velocities = false) or RATTLE (velocities = true) projection of
+// a component's constraints (constraints.rs:80-149): Gauss-Seidel sweeps in
+// the reference's order, directions from the reference positions `ref`,
+// until every constraint is within tolerance (at most 1000 sweeps).  Returns
+// false on the reference's errors (singular or nonfinite projection, no
+// convergence).
+__device__ bool project_component(int k0, int k1, const int* __restrict__ c1, const int* __restrict__ c2,
+                                  const double* __restrict__ d2, const double* __restrict__ w1,
+                                  const double* __restrict__ w2, const double* __restrict__ ref, double* values,
+                                  bool velocities, double tolerance) {
+    for (int sweep = 0; sweep < 1000; ++sweep) {
+        bool converged = true;
+        for (int k = k0; k < k1; ++k) {
+            const int i = c1[k], j = c2[k];
+            const double wi = w1[k], wj = w2[k];
+            double dir[3], delta[3], dot = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                dir[a] = ref[3 * i + a] - ref[3 * j + a];
+                delta[a] = values[3 * i + a] - values[3 * j + a];
+                dot += delta[a] * dir[a];
+            }
+            double mult;
+            if (velocities) {
+                const double norm = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
+                if (norm == 0.0 || !isfinite(norm)) return false;
+                const double correction = -dot / ((wi + wj) * norm);
+                if (fabs(correction) <= tolerance) continue;
+                mult = correction;
+            } else {
+                const double norm = delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2];
+                const double ratio = sqrt(norm / d2[k]);
+                if (isfinite(ratio) && fabs(ratio - 1.0) <= tolerance) continue;
+                const double denominator = 2.0 * (wi + wj) * dot;
+                if (fabs(denominator) < DBL_MIN || !isfinite(denominator)) return false;
+                mult = (d2[k] - norm) / denominator;
+            }
+            if (!isfinite(mult)) return false;
+            converged = false;
+            for (int a = 0; a < 3; ++a) {
+                const double correction = mult * dir[a];
+                values[3 * i + a] += wi * correction;
+                values[3 * j + a] -= wj * correction;
+            }
+        }
+        if (converged) return true;
+    }
+    return false;
+}
+
+// Constrained LangevinMiddle (langevin.rs:215-277), one thread per
+// constraint component: kick, RATTLE the velocities against the current
+// positions, drift / friction + noise (the same normals as k_lfmiddle) /
+// drift, SHAKE the positions against the previous ones, and add the position
+// correction / dt to the velocities.  mode 1 projects only the positions
+// (against themselves), mode 2 only the velocities (against the positions):
+// Context::apply_constraints / apply_velocity_constraints.
+template <class A>
+__global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, const int* __restrict__ atoms,
+                                   const int* __restrict__ cons_off, const int* __restrict__ c1,
+                                   const int* __restrict__ c2, const double* __restrict__ d2,
+                                   const double* __restrict__ w1, const double* __restrict__ w2, double tolerance,
+                                   int mode, double* pos, double* vel, A* __restrict__ force,
+                                   const double* __restrict__ inv_mass, const int* __restrict__ atom_sys,
+                                   const SysInfo* __restrict__ sys, double dt, double decay, double variance,
+                                   const unsigned long long* __restrict__ step,
+                                   const double* __restrict__ pos_build, double lim2, int check, int* flag,
+                                   double* prev, double* proposed, unsigned* status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool moved = false;
+    int si = 0;
+    if (t < ncomp) {
+        const int a0 = atom_off[t], a1 = atom_off[t + 1], k0 = cons_off[t], k1 = cons_off[t + 1];
+        si = atom_sys[atoms[a0]];
+        bool ok = true;
+        if (mode == 2) {
+            ok = project_component(k0, k1, c1, c2, d2, w1, w2, pos, vel, true, tolerance);
+        } else if (mode == 1) {
+            for (int m = a0; m < a1; ++m)
+                for (int a = 0; a < 3; ++a) prev[3 * atoms[m] + a] = pos[3 * atoms[m] + a];
+            ok = project_component(k0, k1, c1, c2, d2, w1, w2, prev, pos, false, tolerance);
+        } else {
+            for (int m = a0; m < a1; ++m) {
+                const int i = atoms[m];
+                const double im = inv_mass[i];
+                for (int a = 0; a < 3; ++a) {
+                    prev[3 * i + a] = pos[3 * i + a];
+                    const double f = force_value(force[3 * i + a]);
+                    force[3 * i + a] = (A)0;
+                    vel[3 * i + a] += f * im * dt;
+                }
+            }
+            ok = project_component(k0, k1, c1, c2, d2, w1, w2, prev, vel, true, tolerance);
+            const double half = 0.5 * dt;
+            for (int m = a0; m < a1; ++m) {
+                const int i = atoms[m];
+                double xi[3];
+                step_normals(i, sys[si], step[0], xi);
+                const double noise = sqrt(variance * inv_mass[i]);
+                for (int a = 0; a < 3; ++a) {
+                    const int k = 3 * i + a;
+                    double v = vel[k];
+                    const double mid = pos[k] + half * v;
+                    v = decay * v + noise * xi[a];
+                    vel[k] = v;
+                    pos[k] = mid + half * v;
+                    proposed[k] = pos[k];
+                }
+            }
+            ok = ok && project_component(k0, k1, c1, c2, d2, w1, w2, prev, pos, false, tolerance);
+            for (int m = a0; m < a1; ++m) {
+                const int i = atoms[m];
+                double dd = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const int k = 3 * i + a;
+                    vel[k] += (pos[k] - proposed[k]) / dt;
+                    const double d = pos[k] - pos_build[k];
+                    dd += d * d;
+                }
+                moved = moved || (check && !(dd <= lim2));
+            }
+        }
+        if (!ok) atomicOr(status, ST_CONSTRAINTS);
+    }
+    if (mode == 0) wave_raise_flags(flag, moved, si);
 }
 
 // Monte Carlo barostat (barostat.rs:129-195, context/program.rs:5-45,
@@ -448,6 +585,7 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     }
     bond1_h_ = b1;
     bond2_h_ = b2;
+    setup_constraints(descs, nsys, inv);
     dt_ = descs[0].step_size;
     temperature_ = descs[0].temperature;
     friction_ = descs[0].friction;
@@ -857,15 +995,160 @@ void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat) {
     const double lim = 0.5 * nb_->skin();
     const int blocks = (n_ + 255) / 256;
     const int check = nb_->settings().include_direct ? 1 : 0;
+    const ConstraintDev& C = cons_;
+    const unsigned char* constrained = C.ncomp ? C.constrained.p : nullptr;
     if (mixed_)
         hipLaunchKernelGGL(k_lfmiddle<long long>, dim3(blocks), dim3(256), 0, s, n_, pos_.p, vel_.p, force_fx_.p,
                            inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance, step_dev_.p,
-                           nb_->pos_build(), lim * lim, check, nb_->list_flag());
+                           nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
     else
         hipLaunchKernelGGL(k_lfmiddle<double>, dim3(blocks), dim3(256), 0, s, n_, pos_.p, vel_.p, force_d_.p,
                            inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance, step_dev_.p,
-                           nb_->pos_build(), lim * lim, check, nb_->list_flag());
+                           nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
+    if (C.ncomp) {
+        const int cb = (C.ncomp + 63) / 64;
+        auto go = [&](auto* force) {
+            using A = std::remove_pointer_t<decltype(force)>;
+            hipLaunchKernelGGL(k_constrained_step<A>, dim3(cb), dim3(64), 0, s, C.ncomp, C.atom_off.p, C.atoms.p,
+                               C.cons_off.p, C.c1.p, C.c2.p, C.d2.p, C.w1.p, C.w2.p, C.tolerance, 0, pos_.p, vel_.p,
+                               force, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
+                               step_dev_.p, nb_->pos_build(), lim * lim, check, nb_->list_flag(), C.prev.p,
+                               C.proposed.p, status_.p);
+        };
+        if (mixed_) go(force_fx_.p);
+        else go(force_d_.p);
+    }
     OMM_HIP(hipGetLastError());
+}
+
+// Constraints of every replica in global indices (System::add_constraint,
+// constraints.rs:35-75 checks), grouped into connected components.
+void MdContext::setup_constraints(const omm_md_desc* descs, int nsys, const std::vector<double>& inv) {
+    std::vector<int> c1, c2;
+    std::vector<double> dist;
+    double tol = 0.0;
+    for (int r = 0; r < nsys; ++r) {
+        const omm_md_desc* d = descs + r;
+        const int off = nb_->atom_offset(r), n = d->num_particles;
+        if (d->num_constraints < 0) throw Error(OMM_ERR_INVALID, "constraint count must be nonnegative");
+        if (d->num_constraints > 0 && (!d->constraint_particle1 || !d->constraint_particle2 || !d->constraint_distance))
+            throw Error(OMM_ERR_INVALID, "constraint arrays must not be null");
+        const double t = d->constraint_tolerance == 0.0 ? 1e-5 : d->constraint_tolerance;
+        if (!(std::isfinite(t) && t > 0.0 && t < 1.0))
+            throw Error(OMM_ERR_INVALID, "constraint tolerance must be between zero and one");
+        if (r > 0 && t != tol) throw Error(OMM_ERR_INVALID, "the replicas of a batch must share the constraint tolerance");
+        tol = t;
+        std::set<std::pair<int, int>> seen;
+        for (int k = 0; k < d->num_constraints; ++k) {
+            const int i = d->constraint_particle1[k], j = d->constraint_particle2[k];
+            const double dd = d->constraint_distance[k];
+            if (i < 0 || j < 0 || i >= n || j >= n || i == j)
+                throw Error(OMM_ERR_INVALID, "constraint particle indices must be distinct and valid");
+            if (!(std::isfinite(dd) && dd > 0.0))
+                throw Error(OMM_ERR_INVALID, "constraint distance must be positive and finite");
+            if (!seen.insert({std::min(i, j), std::max(i, j)}).second)
+                throw Error(OMM_ERR_INVALID, "duplicate constraint between the same particles");
+            if ((d->masses[i] == 0.0) != (d->masses[j] == 0.0))
+                throw Error(OMM_ERR_INVALID, "a constraint cannot connect a massive and massless particle");
+            if (d->masses[i] != 0.0) {  // constraints between massless particles are dropped
+                c1.push_back(off + i);
+                c2.push_back(off + j);
+                dist.push_back(dd);
+            }
+        }
+    }
+    ConstraintDev& C = cons_;
+    C.tolerance = tol;
+    C.ncons = (int)c1.size();
+    if (C.ncons == 0) return;
+    std::vector<int> parent(n_);
+    std::iota(parent.begin(), parent.end(), 0);
+    auto root = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (int k = 0; k < C.ncons; ++k) {
+        const int a = root(c1[k]), b = root(c2[k]);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    }
+    std::map<int, int> comp_of;  // root -> component
+    std::vector<std::vector<int>> cons_of, atoms_of;
+    for (int k = 0; k < C.ncons; ++k) {  // constraints keep the reference's order within a component
+        const int r = root(c1[k]);
+        auto it = comp_of.find(r);
+        if (it == comp_of.end()) {
+            it = comp_of.emplace(r, (int)cons_of.size()).first;
+            cons_of.emplace_back();
+            atoms_of.emplace_back();
+        }
+        cons_of[it->second].push_back(k);
+    }
+    std::vector<unsigned char> constrained(n_, 0);
+    for (int a = 0; a < n_; ++a) {
+        const auto it = comp_of.find(root(a));
+        if (it != comp_of.end()) {
+            atoms_of[it->second].push_back(a);
+            constrained[a] = 1;
+        }
+    }
+    C.ncomp = (int)cons_of.size();
+    std::vector<int> aoff{0}, atoms, coff{0}, p1, p2;
+    std::vector<double> d2, w1, w2;
+    for (int c = 0; c < C.ncomp; ++c) {
+        atoms.insert(atoms.end(), atoms_of[c].begin(), atoms_of[c].end());
+        aoff.push_back((int)atoms.size());
+        for (int k : cons_of[c]) {
+            p1.push_back(c1[k]);
+            p2.push_back(c2[k]);
+            d2.push_back(dist[k] * dist[k]);
+            w1.push_back(inv[c1[k]]);
+            w2.push_back(inv[c2[k]]);
+        }
+        coff.push_back((int)p1.size());
+    }
+    C.atom_off.upload(aoff, s_);
+    C.atoms.upload(atoms, s_);
+    C.cons_off.upload(coff, s_);
+    C.c1.upload(p1, s_);
+    C.c2.upload(p2, s_);
+    C.d2.upload(d2, s_);
+    C.w1.upload(w1, s_);
+    C.w2.upload(w2, s_);
+    C.constrained.upload(constrained, s_);
+    C.prev.alloc(3 * (size_t)n_);
+    C.proposed.alloc(3 * (size_t)n_);
+}
+
+void MdContext::project_constraints(double tolerance, bool velocities) {
+    // constraints.rs:80-92
+    if (!(std::isfinite(tolerance) && tolerance > 0.0 && tolerance < 1.0))
+        throw Error(OMM_ERR_INVALID, "constraint tolerance must be finite and between zero and one");
+    prepare();
+    const ConstraintDev& C = cons_;
+    if (!C.ncomp) return;
+    fill_async<unsigned>(status_.p, 1, 0u, s_);
+    const int cb = (C.ncomp + 63) / 64;
+    auto go = [&](auto* force) {
+        using A = std::remove_pointer_t<decltype(force)>;
+        hipLaunchKernelGGL(k_constrained_step<A>, dim3(cb), dim3(64), 0, s_, C.ncomp, C.atom_off.p, C.atoms.p,
+                           C.cons_off.p, C.c1.p, C.c2.p, C.d2.p, C.w1.p, C.w2.p, tolerance, velocities ? 2 : 1,
+                           pos_.p, vel_.p, force, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, 1.0,
+                           0.0, step_dev_.p, nb_->pos_build(), 0.0, 0, nb_->list_flag(), C.prev.p, C.proposed.p,
+                           status_.p);
+    };
+    if (mixed_) go(force_fx_.p);
+    else go(force_d_.p);
+    OMM_HIP(hipGetLastError());
+    const unsigned st = read_status();
+    if (st & ST_CONSTRAINTS) {
+        fill_async<unsigned>(status_.p, 1, 0u, s_);
+        OMM_HIP(hipStreamSynchronize(s_));
+        throw Error(OMM_ERR_INVALID, "constraint projection is singular or did not converge; check positions");
+    }
+    if (!velocities) {
+        nb_->force_rebuild(s_);  // the positions moved
+        OMM_HIP(hipStreamSynchronize(s_));
+    }
 }
 
 unsigned MdContext::read_status() {
@@ -885,6 +1168,9 @@ void MdContext::check_status() {
     if (st & ST_OVERFLOW) throw Error(OMM_ERR_OVERFLOW, "fixed-point force/energy accumulator overflow");
     if (st & ST_CUTOFF_BOX)
         throw Error(OMM_ERR_CUTOFF_BOX, "a barostat move would make the box smaller than twice the cutoff");
+    if (st & ST_CONSTRAINTS)
+        throw Error(OMM_ERR_INVALID, "constraints failed: a projection was singular or did not converge in 1000 "
+                                     "iterations (langevin.rs:215-277)");
 }
 
 void MdContext::restore_snapshot() {
@@ -1119,11 +1405,11 @@ void MdContext::profile(int reps, double ms[8], int64_t* num_pairs) {
         if (mixed_)
             hipLaunchKernelGGL(k_lfmiddle<long long>, dim3((n_ + 255) / 256), dim3(256), 0, s_, n_, px.p, vx.p,
                                force_fx_.p, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, nb_->pos_build(), lim * lim, 1, scratch_flag.p);
+                               step_dev_.p, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
         else
             hipLaunchKernelGGL(k_lfmiddle<double>, dim3((n_ + 255) / 256), dim3(256), 0, s_, n_, px.p, vx.p,
                                force_d_.p, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, nb_->pos_build(), lim * lim, 1, scratch_flag.p);
+                               step_dev_.p, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
     });
     OMM_HIP(hipStreamSynchronize(s_));
     OMM_HIP(hipEventDestroy(t0));

@@ -127,6 +127,10 @@ class MdDesc(C.Structure):
         ("friction", C.c_double),
         ("step_size", C.c_double),
         ("seed", C.c_uint64),
+        ("num_constraints", C.c_int32),
+        ("constraint_particle1", _ip), ("constraint_particle2", _ip),
+        ("constraint_distance", _dp),
+        ("constraint_tolerance", C.c_double),
     ]
 
 
@@ -160,6 +164,8 @@ def library():
         lib.omm_nb_desc_defaults.restype = None
         lib.omm_water_box.argtypes = [C.c_int32, C.c_double, C.c_uint64, _dp]
         lib.omm_ctx_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
+        lib.omm_ctx_apply_constraints.argtypes = [C.c_void_p, C.c_double]
+        lib.omm_ctx_apply_velocity_constraints.argtypes = [C.c_void_p, C.c_double]
         lib.omm_ctx_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         lib.omm_batch_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
         lib.omm_batch_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
@@ -556,6 +562,8 @@ class WaterSystem:
     bond_params: np.ndarray  # (nb, 2) length, k
     angles: np.ndarray     # (na, 3) int32
     angle_params: np.ndarray  # (na, 2) theta0, k
+    constraints: list = field(default_factory=list)  # (i, j, distance nm), System::add_constraint
+    constraint_tolerance: float = 0.0  # 0: the integrator default 1e-5
 
 
 def water_system(num_molecules: int, box_edge: float, seed: int, cutoff: float = 0.9,
@@ -612,6 +620,15 @@ def _md_desc(system: WaterSystem, temperature, friction, step_size, seed):
     for i, v in enumerate(_f64(s.box).reshape(9)):
         d.box[i] = v
     d.temperature, d.friction, d.step_size, d.seed = temperature, friction, step_size, seed
+    cons = list(getattr(s, "constraints", None) or [])
+    keep["c1"] = _i32([c[0] for c in cons])
+    keep["c2"] = _i32([c[1] for c in cons])
+    keep["cd"] = _f64([c[2] for c in cons])
+    d.num_constraints = len(cons)
+    d.constraint_particle1 = _ptr(keep["c1"], C.c_int32)
+    d.constraint_particle2 = _ptr(keep["c2"], C.c_int32)
+    d.constraint_distance = _ptr(keep["cd"], C.c_double)
+    d.constraint_tolerance = float(getattr(s, "constraint_tolerance", 0.0) or 0.0)
     return d, keep
 
 
@@ -645,6 +662,12 @@ class Context:
 
     def set_velocities_to_temperature(self, temperature, seed):
         _check(library().omm_ctx_set_velocities_to_temperature(self._h, temperature, seed))
+
+    def apply_constraints(self, tolerance=1e-5):
+        _check(library().omm_ctx_apply_constraints(self._h, float(tolerance)))
+
+    def apply_velocity_constraints(self, tolerance=1e-5):
+        _check(library().omm_ctx_apply_velocity_constraints(self._h, float(tolerance)))
 
     def step(self, steps):
         _check(library().omm_ctx_step(self._h, int(steps)))

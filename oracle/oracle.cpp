@@ -8,6 +8,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <array>
 #include <cmath>
 #include <complex>
@@ -1172,6 +1173,112 @@ void orc_lfmiddle_step(int32_t n, double* pos, double* vel, const double* forces
             pos[i] = midpoint + half_dt * vel[i];
         }
     }
+}
+
+// constraints.rs:35-149 — sparse SHAKE (positions) / RATTLE (velocities)
+// projection: Gauss-Seidel sweeps over the constraints in order, directions
+// from the reference positions, until every constraint is within tolerance
+// (at most 1000 sweeps).  Constraints between two massless particles are
+// dropped; returns 0, or 1 on a reference error (invalid input, singular or
+// nonfinite projection, no convergence).
+int32_t orc_constraints_solve(int32_t n, int32_t nc, const int32_t* c1, const int32_t* c2, const double* dist,
+                              const double* masses, const double* reference, double* values, int32_t velocities,
+                              double tolerance) {
+    if (!(std::isfinite(tolerance) && tolerance > 0.0 && tolerance < 1.0)) return 1;
+    struct Prepared {
+        int i, j;
+        double wi, wj, d2;
+    };
+    std::vector<Prepared> cs;
+    for (int k = 0; k < nc; ++k) {
+        const int i = c1[k], j = c2[k];
+        if (i < 0 || j < 0 || i >= n || j >= n || i == j) return 1;
+        if (!(std::isfinite(dist[k]) && dist[k] > 0.0)) return 1;
+        if ((masses[i] == 0.0) != (masses[j] == 0.0)) return 1;
+        if (masses[i] != 0.0) cs.push_back({i, j, 1.0 / masses[i], 1.0 / masses[j], dist[k] * dist[k]});
+    }
+    std::vector<std::array<double, 3>> dir(cs.size());
+    for (size_t k = 0; k < cs.size(); ++k)
+        for (int a = 0; a < 3; ++a) dir[k][a] = reference[3 * cs[k].i + a] - reference[3 * cs[k].j + a];
+    for (int sweep = 0; sweep < 1000; ++sweep) {
+        bool converged = true;
+        for (size_t k = 0; k < cs.size(); ++k) {
+            const Prepared& c = cs[k];
+            double delta[3], dot = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                delta[a] = values[3 * c.i + a] - values[3 * c.j + a];
+                dot += delta[a] * dir[k][a];
+            }
+            double mult;
+            if (velocities) {
+                double norm = 0.0;
+                for (int a = 0; a < 3; ++a) norm += dir[k][a] * dir[k][a];
+                if (norm == 0.0 || !std::isfinite(norm)) return 1;
+                const double correction = -dot / ((c.wi + c.wj) * norm);
+                if (std::fabs(correction) <= tolerance) continue;
+                mult = correction;
+            } else {
+                double norm = 0.0;
+                for (int a = 0; a < 3; ++a) norm += delta[a] * delta[a];
+                const double ratio = std::sqrt(norm / c.d2);
+                if (std::isfinite(ratio) && std::fabs(ratio - 1.0) <= tolerance) continue;
+                const double denominator = 2.0 * (c.wi + c.wj) * dot;
+                if (std::fabs(denominator) < DBL_MIN || !std::isfinite(denominator)) return 1;
+                mult = (c.d2 - norm) / denominator;
+            }
+            if (!std::isfinite(mult)) return 1;
+            converged = false;
+            for (int a = 0; a < 3; ++a) {
+                const double correction = mult * dir[k][a];
+                values[3 * c.i + a] += c.wi * correction;
+                values[3 * c.j + a] -= c.wj * correction;
+            }
+        }
+        if (converged) return 0;
+    }
+    return 1;
+}
+
+// langevin.rs:215-277 (host path) — constrained LFMiddle: kick, RATTLE the
+// velocities against the current positions, drift / friction + noise / drift,
+// SHAKE the positions against the previous ones, and add the position
+// correction / dt to the velocities.  normals as in orc_lfmiddle_step.
+int32_t orc_lfmiddle_step_constrained(int32_t n, double* pos, double* vel, const double* forces,
+                                      const double* masses, double dt, double temperature, double friction,
+                                      const double* normals, int32_t nc, const int32_t* c1, const int32_t* c2,
+                                      const double* dist, double tolerance) {
+    const std::vector<double> previous(pos, pos + 3 * (size_t)n);
+    for (int atom = 0; atom < n; ++atom) {
+        if (masses[atom] == 0.0) continue;
+        const double inverse_mass = 1.0 / masses[atom];
+        for (int axis = 0; axis < 3; ++axis) vel[3 * atom + axis] += forces[3 * atom + axis] * inverse_mass * dt;
+    }
+    if (orc_constraints_solve(n, nc, c1, c2, dist, masses, pos, vel, 1, tolerance)) return 1;
+    const double half_dt = 0.5 * dt;
+    const double decay = std::exp(-friction * dt);
+    const double variance = BOLTZMANN * temperature * -std::expm1(-2.0 * friction * dt);
+    size_t r = 0;
+    for (int atom = 0; atom < n; ++atom) {
+        if (masses[atom] == 0.0) continue;
+        const double noise_scale = std::sqrt(variance / masses[atom]);
+        for (int axis = 0; axis < 3; ++axis) {
+            const size_t i = (size_t)3 * atom + axis;
+            const double midpoint = pos[i] + half_dt * vel[i];
+            vel[i] = decay * vel[i] + noise_scale * (normals ? normals[r] : 0.0);
+            ++r;
+            pos[i] = midpoint + half_dt * vel[i];
+        }
+    }
+    const std::vector<double> proposed(pos, pos + 3 * (size_t)n);
+    if (orc_constraints_solve(n, nc, c1, c2, dist, masses, previous.data(), pos, 0, tolerance)) return 1;
+    for (int atom = 0; atom < n; ++atom) {
+        if (masses[atom] == 0.0) continue;
+        for (int axis = 0; axis < 3; ++axis) {
+            const size_t i = (size_t)3 * atom + axis;
+            vel[i] += (pos[i] - proposed[i]) / dt;
+        }
+    }
+    return 0;
 }
 
 // system.rs:100-141 — union-find over bonded pairs.

@@ -108,6 +108,17 @@ double orc_harmonic_angles(int32_t na, const int32_t* p1, const int32_t* p2, con
 void orc_lfmiddle_step(int32_t n, double* pos, double* vel, const double* forces,
                        const double* masses, double dt, double temperature, double friction,
                        const double* normals);
+
+/* constraints.rs:35-149: SHAKE (velocities = 0) / RATTLE (velocities = 1)
+ * projection of values against reference positions.  0 = converged. */
+int32_t orc_constraints_solve(int32_t n, int32_t nc, const int32_t* c1, const int32_t* c2, const double* dist,
+                              const double* masses, const double* reference, double* values, int32_t velocities,
+                              double tolerance);
+/* langevin.rs:215-277: constrained LFMiddle step (normals nullable = zero). */
+int32_t orc_lfmiddle_step_constrained(int32_t n, double* pos, double* vel, const double* forces,
+                                      const double* masses, double dt, double temperature, double friction,
+                                      const double* normals, int32_t nc, const int32_t* c1, const int32_t* c2,
+                                      const double* dist, double tolerance);
 /* BOLTZMANN, langevin.rs:12. */
 double orc_boltzmann(void);
 

@@ -62,6 +62,12 @@ def lib():
         L.orc_harmonic_angles.argtypes = [C.c_int32, _ip, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int32, _dp]
         L.orc_lfmiddle_step.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                         C.c_double, _dp]
+        L.orc_constraints_solve.restype = C.c_int32
+        L.orc_constraints_solve.argtypes = [C.c_int32, C.c_int32, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int32,
+                                            C.c_double]
+        L.orc_lfmiddle_step_constrained.restype = C.c_int32
+        L.orc_lfmiddle_step_constrained.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                                    C.c_double, _dp, C.c_int32, _ip, _ip, _dp, C.c_double]
         L.orc_boltzmann.restype = C.c_double
         L.orc_molecules.restype = C.c_int32
         L.orc_molecules.argtypes = [C.c_int32, C.c_int32, _ip, _ip, _ip]
@@ -238,6 +244,37 @@ def lfmiddle_step(pos, vel, forces, masses, dt, temperature, friction, normals=N
     vel = _f(vel).copy()
     lib().orc_lfmiddle_step(len(masses), _p(pos), _p(vel), _p(_f(forces)), _p(_f(masses)), dt, temperature,
                             friction, _p(None if normals is None else _f(normals)))
+    return pos, vel
+
+
+def constraints_solve(constraints, masses, reference, values, velocities, tolerance):
+    """ConstraintSolver::solve (constraints.rs:80-149); constraints = [(i, j, d)].
+    Returns the projected values; raises on the reference's errors."""
+    c = list(constraints)
+    c1 = _i([x[0] for x in c])
+    c2 = _i([x[1] for x in c])
+    d = _f([x[2] for x in c])
+    out = _f(values).copy()
+    rc = lib().orc_constraints_solve(len(masses), len(c), _p(c1, C.c_int32), _p(c2, C.c_int32), _p(d),
+                                     _p(_f(masses)), _p(_f(reference)), _p(out), int(velocities), tolerance)
+    if rc:
+        raise AssertionError("constraint solve failed a reference check")
+    return out
+
+
+def lfmiddle_step_constrained(pos, vel, forces, masses, dt, temperature, friction, constraints, tolerance,
+                              normals=None):
+    pos = _f(pos).copy()
+    vel = _f(vel).copy()
+    c = list(constraints)
+    c1 = _i([x[0] for x in c])
+    c2 = _i([x[1] for x in c])
+    d = _f([x[2] for x in c])
+    rc = lib().orc_lfmiddle_step_constrained(len(masses), _p(pos), _p(vel), _p(_f(forces)), _p(_f(masses)), dt,
+                                             temperature, friction, _p(None if normals is None else _f(normals)),
+                                             len(c), _p(c1, C.c_int32), _p(c2, C.c_int32), _p(d), tolerance)
+    if rc:
+        raise AssertionError("constrained step failed a reference check")
     return pos, vel
 
 

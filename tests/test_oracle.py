@@ -425,3 +425,52 @@ def test_exception_offsets_equal_explicit_parameters():
     explicit = _two_particles(0.2, 0.3, 0.4)
     explicit.add_exception(0, 1, -0.1 + 0.3 * 0.5, 0.3 + 0.3 * 0.1, 0.2 + 0.3 * 0.4)
     assert orc.nb_evaluate(f.desc(), pos, None)[1] == orc.nb_evaluate(explicit.desc(), pos, None)[1]
+
+
+# --- constraints (SHAKE / RATTLE) ------------------------------------------
+
+def test_constraint_projection_known_answer():
+    # tests/constraints.rs:15-38: masses 1 and 2 joined at 1 nm, started 1.2 nm apart.
+    masses = [1.0, 2.0]
+    c = [(0, 1, 1.0)]
+    pos = np.array([[0.0, 0, 0], [1.2, 0, 0]])
+    vel = np.array([[1.0, 2.0, 0.0], [-1.0, -1.0, 0.0]])
+    p = orc.constraints_solve(c, masses, pos, pos, False, 1e-6).reshape(-1, 3)
+    v = orc.constraints_solve(c, masses, p, vel, True, 1e-6).reshape(-1, 3)
+    assert p[1, 0] - p[0, 0] == pytest.approx(1.0, abs=2e-6)
+    assert p[0, 0] + 2.0 * p[1, 0] == pytest.approx(2.4, abs=2e-6)
+    assert v[1, 0] - v[0, 0] == pytest.approx(0.0, abs=2e-6)
+    assert v[0, 0] + 2.0 * v[1, 0] == pytest.approx(-1.0, abs=2e-6)
+    assert v[0, 1] == 2.0 and v[1, 1] == -1.0
+
+
+def test_constraint_projection_errors():
+    # tests/constraints.rs:77-110: coincident particles are singular; a
+    # massive-massless constraint is rejected.
+    with pytest.raises(AssertionError):
+        orc.constraints_solve([(0, 1, 1.0)], [1.0, 1.0], np.zeros(6), np.zeros(6), False, 1e-5)
+    with pytest.raises(AssertionError):
+        orc.constraints_solve([(0, 1, 1.0)], [1.0, 0.0], np.arange(6.0), np.arange(6.0), False, 1e-5)
+
+
+def test_constrained_lfmiddle_keeps_water_rigid():
+    # tests/constraints.rs:41-74 with LangevinMiddle: the coupled triangle
+    # keeps its lengths over 100 steps; the projections conserve momentum.
+    masses = np.array([16.0, 1.0, 1.0])
+    c = [(0, 1, 0.1), (0, 2, 0.1), (1, 2, 0.16)]
+    pos = np.array([[0.0, 0, 0], [0.08, 0.06, 0.0], [-0.08, 0.06, 0.0]])
+    vel = np.tile([0.2, 0.1, 0.0], (3, 1))
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        normals = rng.normal(size=9)
+        pos, vel = orc.lfmiddle_step_constrained(pos, vel, np.zeros(9), masses, 0.001, 300.0, 2.0, c, 1e-5,
+                                                 normals)
+        p = pos.reshape(-1, 3)
+        for i, j, d in c:
+            assert np.linalg.norm(p[i] - p[j]) == pytest.approx(d, rel=2e-5)
+    # Zero temperature and friction: the projections move no net momentum.
+    v0 = rng.normal(size=(3, 3))
+    p1, v1 = orc.lfmiddle_step_constrained(p, v0, np.zeros(9), masses, 0.001, 0.0, 0.0, c, 1e-10)
+    mom0 = (masses[:, None] * v0).sum(0)
+    mom1 = (masses[:, None] * v1.reshape(-1, 3)).sum(0)
+    assert np.abs(mom1 - mom0).max() < 1e-12
