@@ -313,9 +313,17 @@ typedef struct omm_barostat_info {
     double volume_old, volume_new;            /* nm^3 */
     double energy_old, energy_new, work;      /* kJ/mol */
     int32_t accepted_last, skipped;
+    int32_t axis;                             /* anisotropic barostat: axis of the last attempt */
 } omm_barostat_info;
 
 omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
+
+/* MonteCarloAnisotropicBarostat (forces/anisotropic_barostat.rs:42-243):
+ * pressures per axis (bar), scale_axes bit d enables axis d; each attempt
+ * scales one enabled axis by V'/V with its own adaptive step.  Replaces any
+ * barostat set before; frequency 0 removes it. */
+omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[3], double temperature,
+                                            int32_t scale_axes, int32_t frequency, uint64_t seed);
 
 /* Current box vectors (row-major a, b, c); they change under a barostat
  * (Context::get_state box vectors, context.rs:930-1105). */
@@ -415,6 +423,8 @@ omm_status omm_batch_set_graph_steps(omm_batch* b, int32_t steps);
 omm_status omm_batch_set_barostat(omm_batch* b, double pressure, double temperature, int32_t frequency,
                                   uint64_t seed);
 omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info);
+omm_status omm_batch_set_anisotropic_barostat(omm_batch* b, const double pressure[3], double temperature,
+                                              int32_t scale_axes, int32_t frequency, uint64_t seed);
 omm_status omm_batch_barostat_attempt(omm_batch* b);
 
 /* Context::get_state of replica r; any output pointer may be NULL. */

@@ -265,11 +265,18 @@ static void fill_info(const ommhip::BaroState& st, omm_barostat_info* info) {
     info->work = st.work;
     info->accepted_last = st.accepted_last;
     info->skipped = st.skipped;
+    info->axis = st.axis;
 }
 
 omm_status omm_ctx_set_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
                                 uint64_t seed) {
     CTX_CALL(ctx->ctx.set_barostat(pressure, temperature, frequency, seed));
+}
+
+omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[3], double temperature,
+                                            int32_t scale_axes, int32_t frequency, uint64_t seed) {
+    CTX_CALL(if (!pressure) throw Error(OMM_ERR_INVALID, "null pressure");
+             ctx->ctx.set_anisotropic_barostat(pressure, temperature, scale_axes, frequency, seed));
 }
 
 omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info) {
@@ -367,6 +374,12 @@ omm_status omm_batch_set_graph_steps(omm_batch* b, int32_t steps) { BATCH_CALL(b
 omm_status omm_batch_set_barostat(omm_batch* b, double pressure, double temperature, int32_t frequency,
                                   uint64_t seed) {
     BATCH_CALL(b->ctx.set_barostat(pressure, temperature, frequency, seed));
+}
+
+omm_status omm_batch_set_anisotropic_barostat(omm_batch* b, const double pressure[3], double temperature,
+                                              int32_t scale_axes, int32_t frequency, uint64_t seed) {
+    BATCH_CALL(if (!pressure) throw Error(OMM_ERR_INVALID, "null pressure");
+               b->ctx.set_anisotropic_barostat(pressure, temperature, scale_axes, frequency, seed));
 }
 
 omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info) {

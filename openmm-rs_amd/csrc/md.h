@@ -30,6 +30,11 @@ struct BaroState {
     double u_volume, u_accept;                // the attempt's two uniforms
     double v_old, v_new, e_old, e_new, work;  // last attempt
     int accepted_last, skipped;
+    // Anisotropic barostat (anisotropic_barostat.rs): the same per axis, and
+    // the axis of the last proposal.
+    double axis_volume_scale[3];
+    long long axis_attempted[3], axis_accepted[3];
+    int axis;
 };
 
 // Distance constraints (constraints.rs) of massive particles, grouped into
@@ -51,6 +56,9 @@ struct BaroDev {
     bool enabled = false;
     int frequency = 0, phase = 0;  // phase: steps since the last attempt slot
     double pressure = 0.0, temperature = 0.0;
+    bool anisotropic = false;      // MonteCarloAnisotropicBarostat: one axis per attempt
+    double axis_pressure[3] = {};
+    int scale_axes = 7;            // bit d: axis d may be scaled
     uint64_t seed = 0;
     int nmol = 0;
     DevBuf<int> mol_start, mol_atoms, mol_sys, nmol_sys;
@@ -87,6 +95,10 @@ public:
     // bodies, replica r drawing from seed + r.  alpha and the PME grid stay
     // as initialised, as in the reference.
     void set_barostat(double pressure, double temperature, int frequency, uint64_t seed);
+    // MonteCarloAnisotropicBarostat (anisotropic_barostat.rs:42-243): per-axis
+    // pressures (bar); each attempt scales one enabled axis (scale_axes bit d).
+    void set_anisotropic_barostat(const double pressure[3], double temperature, int scale_axes, int frequency,
+                                  uint64_t seed);
     // One barostat attempt now, outside the step sequence (test hook).
     void barostat_attempt();
     BaroState barostat_state(int r);
@@ -159,6 +171,7 @@ private:
     std::vector<int> bonded_exc_order_;  // exception k of the bonded arrays = global exception order[k]
     BaroDev baro_;
     std::vector<int> bond1_h_, bond2_h_;  // bonds in global indices (molecules)
+    std::vector<int> cons1_h_, cons2_h_;  // constraints in global indices (molecules)
     DevBuf<SysInfo> snap_sys_;
     DevBuf<long long> force_fx_, energy_fx_;
     DevBuf<double> force_d_, energy_d_;

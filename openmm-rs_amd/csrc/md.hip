@@ -359,26 +359,43 @@ __global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, 
 // its unweighted centroid.  The two uniforms come from Philox keyed by the
 // barostat seed + replica index, counter = attempt number, so a replica draws
 // the same numbers alone or in a batch.
-__device__ __forceinline__ void baro_record(BaroState& B, bool accepted, double previous_volume) {
-    // barostat.rs:180-195
-    B.attempted += 1;
-    B.accepted += accepted ? 1 : 0;
-    if (B.attempted >= 10) {
-        const double fraction = (double)B.accepted / (double)B.attempted;
+//
+// The anisotropic form (anisotropic_barostat.rs:122-211) draws one of the
+// enabled axes (a third uniform, uniform over them), scales that axis by
+// V'/V, and keeps the step size and counters per axis.
+__device__ __forceinline__ void adapt_step(double& volume_scale, long long& attempted, long long& accepted_count,
+                                           bool accepted, double previous_volume) {
+    // barostat.rs:180-195, anisotropic_barostat.rs:190-211
+    attempted += 1;
+    accepted_count += accepted ? 1 : 0;
+    if (attempted >= 10) {
+        const double fraction = (double)accepted_count / (double)attempted;
         if (fraction < 0.25) {
-            B.volume_scale /= 1.1;
+            volume_scale /= 1.1;
         } else if (fraction > 0.75) {
-            B.volume_scale = fmin(B.volume_scale * 1.1, previous_volume * 0.3);
+            volume_scale = fmin(volume_scale * 1.1, previous_volume * 0.3);
         } else {
             return;
         }
-        B.attempted = 0;
-        B.accepted = 0;
+        attempted = 0;
+        accepted_count = 0;
+    }
+}
+
+__device__ __forceinline__ void baro_record(BaroState& B, bool anisotropic, bool accepted, double previous_volume) {
+    if (anisotropic) {
+        const int a = B.axis;
+        adapt_step(B.axis_volume_scale[a], B.axis_attempted[a], B.axis_accepted[a], accepted, previous_volume);
+        B.volume_scale = B.axis_volume_scale[a];
+        B.attempted = B.axis_attempted[a];
+        B.accepted = B.axis_accepted[a];
+    } else {
+        adapt_step(B.volume_scale, B.attempted, B.accepted, accepted, previous_volume);
     }
 }
 
 __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved_box, double* scale, uint64_t seed,
-                               double cutoff, unsigned* status) {
+                               double cutoff, int anisotropic, int scale_axes, unsigned* status) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nsys) return;
     BaroState& B = st[r];
@@ -390,29 +407,53 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
     B.u_volume = (double)c[0] * (1.0 / 4294967296.0);
     B.u_accept = (double)c[1] * (1.0 / 4294967296.0);
     B.total_attempted += 1;
+    double step = B.volume_scale;
+    if (anisotropic) {
+        // The k-th enabled axis, k uniform (anisotropic_barostat.rs:133-139).
+        const int enabled = __popc(scale_axes & 7);
+        int k = (int)((double)c[2] * (1.0 / 4294967296.0) * enabled);
+        if (k >= enabled) k = enabled - 1;
+        int axis = 0;
+        for (int a = 0; a < 3; ++a)
+            if ((scale_axes >> a) & 1) {
+                if (k == 0) {
+                    axis = a;
+                    break;
+                }
+                --k;
+            }
+        B.axis = axis;
+        step = B.axis_volume_scale[axis];
+    }
     const double v_old = box.v[0] * box.v[4] * box.v[8];
-    const double v_new = v_old + B.volume_scale * 2.0 * (B.u_volume - 0.5);
+    const double v_new = v_old + step * 2.0 * (B.u_volume - 0.5);
     B.v_old = v_old;
     B.v_new = v_new;
     B.accepted_last = 0;
+    for (int a = 0; a < 3; ++a) scale[3 * r + a] = 1.0;
     if (!(isfinite(v_new) && v_new > 0.0)) {
         B.skipped = 1;
-        scale[r] = 1.0;
-        baro_record(B, false, v_old);
+        baro_record(B, anisotropic, false, v_old);
         return;
     }
     B.skipped = 0;
-    const double sc = cbrt(v_new / v_old);
+    double sc[3];
+    if (anisotropic) {
+        for (int a = 0; a < 3; ++a) sc[a] = a == B.axis ? v_new / v_old : 1.0;
+    } else {
+        sc[0] = sc[1] = sc[2] = cbrt(v_new / v_old);
+    }
     // The reference asserts 2 rc <= box edge on every evaluation
     // (nonbonded.rs:806-812); a move that would break it is an error.
-    if (!(2.0 * cutoff <= box.v[0] * sc && 2.0 * cutoff <= box.v[4] * sc && 2.0 * cutoff <= box.v[8] * sc)) {
+    if (!(2.0 * cutoff <= box.v[0] * sc[0] && 2.0 * cutoff <= box.v[4] * sc[1] && 2.0 * cutoff <= box.v[8] * sc[2])) {
         atomicOr(status, ST_CUTOFF_BOX);
         B.skipped = 1;
-        scale[r] = 1.0;
         return;
     }
-    scale[r] = sc;
-    for (int k = 0; k < 9; ++k) box.v[k] *= sc;
+    for (int a = 0; a < 3; ++a) scale[3 * r + a] = sc[a];
+    // barostat.rs:28-40: each box vector's component a scaled by sc[a].
+    for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) box.v[3 * k + a] *= sc[a];
     box_refresh(box);
 }
 
@@ -430,11 +471,11 @@ __global__ void k_baro_scale(int nmol, const int* __restrict__ mol_start, const 
             center[d] += pos[3 * a + d];
         }
     }
-    const double sc = scale[mol_sys[m]];
-    if (sc == 1.0) return;
+    const double* sc = scale + 3 * mol_sys[m];
+    if (sc[0] == 1.0 && sc[1] == 1.0 && sc[2] == 1.0) return;
     const double inv = 1.0 / (double)(hi - lo);
     double offset[3];
-    for (int d = 0; d < 3; ++d) offset[d] = (sc - 1.0) * (center[d] * inv);
+    for (int d = 0; d < 3; ++d) offset[d] = (sc[d] - 1.0) * (center[d] * inv);
     for (int k = lo; k < hi; ++k) {
         const int a = mol_atoms[k];
         for (int d = 0; d < 3; ++d) pos[3 * a + d] += offset[d];
@@ -455,7 +496,8 @@ __device__ double system_energy(const A* e, const SysInfo& S, double volume) {
 template <class A>
 __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* __restrict__ saved_box, int* rejected,
                               const A* __restrict__ e_old, const A* __restrict__ e_new,
-                              const int* __restrict__ nmol_sys, double pressure, double kt) {
+                              const int* __restrict__ nmol_sys, double pressure, int anisotropic, double px, double py,
+                              double pz, double kt) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nsys) return;
     BaroState& B = st[r];
@@ -466,7 +508,8 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
     SysInfo& S = sys[r];
     const double eo = system_energy(e_old + (size_t)r * E_SLOTS, S, saved_box[r].volume);
     const double en = system_energy(e_new + (size_t)r * E_SLOTS, S, S.box.volume);
-    // barostat.rs:160-178 (BAR_NM3_TO_KJ_PER_MOL = 0.0602214076)
+    // barostat.rs:160-178, anisotropic_barostat.rs:162-188 (BAR_NM3_TO_KJ_PER_MOL = 0.0602214076)
+    if (anisotropic) pressure = B.axis == 0 ? px : (B.axis == 1 ? py : pz);
     const double work = (en - eo) + pressure * 0.0602214076 * (B.v_new - B.v_old) -
                         (double)nmol_sys[r] * kt * log(B.v_new / B.v_old);
     bool accepted;
@@ -478,7 +521,7 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
     B.work = work;
     B.accepted_last = accepted ? 1 : 0;
     B.total_accepted += accepted ? 1 : 0;
-    baro_record(B, accepted, B.v_old);
+    baro_record(B, anisotropic, accepted, B.v_old);
     if (!accepted) S.box = saved_box[r];
     rejected[r] = accepted ? 0 : 1;
 }
@@ -833,7 +876,7 @@ void MdContext::enqueue_barostat(hipStream_t s) {
     }
     enqueue_forces_into(s, true, NbOperator::REBUILD_FLAG, B.force_fx.p, B.force_d.p, B.e_old_fx.p, B.e_old_d.p);
     hipLaunchKernelGGL(k_baro_propose, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p, B.saved_box.p,
-                       B.scale.p, B.seed, nb_->settings().cutoff, status_.p);
+                       B.scale.p, B.seed, nb_->settings().cutoff, B.anisotropic ? 1 : 0, B.scale_axes, status_.p);
     hipLaunchKernelGGL(k_baro_scale, dim3((B.nmol + 255) / 256), dim3(256), 0, s, B.nmol, B.mol_start.p,
                        B.mol_atoms.p, B.mol_sys.p, B.scale.p, pos_.p, B.pos_saved.p);
     nb_->force_rebuild(s);
@@ -841,10 +884,12 @@ void MdContext::enqueue_barostat(hipStream_t s) {
     const double kt = BOLTZMANN * B.temperature;
     if (mixed_)
         hipLaunchKernelGGL(k_baro_accept<long long>, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p,
-                           B.saved_box.p, B.rejected.p, B.e_old_fx.p, B.e_new_fx.p, B.nmol_sys.p, B.pressure, kt);
+                           B.saved_box.p, B.rejected.p, B.e_old_fx.p, B.e_new_fx.p, B.nmol_sys.p, B.pressure,
+                           B.anisotropic ? 1 : 0, B.axis_pressure[0], B.axis_pressure[1], B.axis_pressure[2], kt);
     else
         hipLaunchKernelGGL(k_baro_accept<double>, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p,
-                           B.saved_box.p, B.rejected.p, B.e_old_d.p, B.e_new_d.p, B.nmol_sys.p, B.pressure, kt);
+                           B.saved_box.p, B.rejected.p, B.e_old_d.p, B.e_new_d.p, B.nmol_sys.p, B.pressure,
+                           B.anisotropic ? 1 : 0, B.axis_pressure[0], B.axis_pressure[1], B.axis_pressure[2], kt);
     hipLaunchKernelGGL(k_baro_restore, dim3((n_ + 255) / 256), dim3(256), 0, s, n_, nb_->atom_sys_dev(),
                        B.rejected.p, pos_.p, B.pos_saved.p);
     nb_->force_rebuild(s);
@@ -866,6 +911,8 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
     B.pressure = pressure;
     B.temperature = temperature;
     B.seed = seed;
+    B.anisotropic = false;
+    B.scale_axes = 7;
     if (!B.enabled) return;
     // Molecules: connected components over the bonds (system.rs:100-141).
     std::vector<int> parent(n_);
@@ -874,6 +921,10 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
         while (parent[x] != x) x = parent[x] = parent[parent[x]];
         return x;
     };
+    for (size_t k = 0; k < cons1_h_.size(); ++k) {
+        const int a = root(cons1_h_[k]), b = root(cons2_h_[k]);
+        parent[std::max(a, b)] = std::min(a, b);
+    }
     for (size_t k = 0; k < bond1_h_.size(); ++k) {
         const int a = root(bond1_h_[k]), b = root(bond2_h_[k]);
         parent[std::max(a, b)] = std::min(a, b);
@@ -903,12 +954,13 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
     for (int r = 0; r < nsys_; ++r) {
         st[r] = BaroState{};
         st[r].volume_scale = 0.01 * sys[r].box.v[0] * sys[r].box.v[4] * sys[r].box.v[8];
+        for (int a = 0; a < 3; ++a) st[r].axis_volume_scale[a] = st[r].volume_scale;  // anisotropic_barostat.rs:221
     }
     B.st.upload(st, s_);
     B.snap_st.alloc(nsys_);
     B.saved_box.alloc(nsys_);
     B.rejected.alloc(nsys_);
-    B.scale.alloc(nsys_);
+    B.scale.alloc(3 * (size_t)nsys_);
     B.pos_saved.alloc(3 * (size_t)n_);
     const size_t es = (size_t)E_SLOTS * nsys_;
     if (mixed_) {
@@ -920,6 +972,19 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
         B.e_old_d.alloc(es);
         B.e_new_d.alloc(es);
     }
+}
+
+void MdContext::set_anisotropic_barostat(const double pressure[3], double temperature, int scale_axes, int frequency,
+                                         uint64_t seed) {
+    // anisotropic_barostat.rs:42-74, 217-230
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(pressure[a])) throw Error(OMM_ERR_INVALID, "pressure must be finite");
+    if ((scale_axes & 7) == 0 && frequency > 0) throw Error(OMM_ERR_INVALID, "no axis is enabled for scaling");
+    set_barostat(pressure[0], temperature, frequency, seed);
+    BaroDev& B = baro_;
+    B.anisotropic = true;
+    B.scale_axes = scale_axes & 7;
+    for (int a = 0; a < 3; ++a) B.axis_pressure[a] = pressure[a];
 }
 
 bool MdContext::set_parameter(const std::string& name, double value) {
@@ -1048,6 +1113,8 @@ void MdContext::setup_constraints(const omm_md_desc* descs, int nsys, const std:
                 throw Error(OMM_ERR_INVALID, "constraint distance must be positive and finite");
             if (!seen.insert({std::min(i, j), std::max(i, j)}).second)
                 throw Error(OMM_ERR_INVALID, "duplicate constraint between the same particles");
+            cons1_h_.push_back(off + i);  // molecules (system.rs:117-119) count every constraint
+            cons2_h_.push_back(off + j);
             if ((d->masses[i] == 0.0) != (d->masses[j] == 0.0))
                 throw Error(OMM_ERR_INVALID, "a constraint cannot connect a massive and massless particle");
             if (d->masses[i] != 0.0) {  // constraints between massless particles are dropped

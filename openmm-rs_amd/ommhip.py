@@ -69,6 +69,7 @@ class BarostatInfo(C.Structure):
         ("volume_old", C.c_double), ("volume_new", C.c_double),
         ("energy_old", C.c_double), ("energy_new", C.c_double), ("work", C.c_double),
         ("accepted_last", C.c_int32), ("skipped", C.c_int32),
+        ("axis", C.c_int32),
     ]
 
     def as_dict(self):
@@ -165,6 +166,10 @@ def library():
         lib.omm_water_box.argtypes = [C.c_int32, C.c_double, C.c_uint64, _dp]
         lib.omm_ctx_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
         lib.omm_ctx_apply_constraints.argtypes = [C.c_void_p, C.c_double]
+        lib.omm_ctx_set_anisotropic_barostat.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32, C.c_int32,
+                                                         C.c_uint64]
+        lib.omm_batch_set_anisotropic_barostat.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32, C.c_int32,
+                                                           C.c_uint64]
         lib.omm_ctx_apply_velocity_constraints.argtypes = [C.c_void_p, C.c_double]
         lib.omm_ctx_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         lib.omm_batch_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
@@ -690,6 +695,12 @@ class Context:
         """MonteCarloBarostat(pressure bar, temperature K, frequency) with rigid molecules."""
         _check(library().omm_ctx_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
 
+    def set_anisotropic_barostat(self, pressure, temperature, scale_axes=(True, True, True), frequency=25, seed=1):
+        """MonteCarloAnisotropicBarostat(pressure[3] bar, temperature K, scale_axes, frequency)."""
+        p = (C.c_double * 3)(*[float(x) for x in pressure])
+        mask = sum(1 << a for a in range(3) if scale_axes[a])
+        _check(library().omm_ctx_set_anisotropic_barostat(self._h, p, temperature, mask, int(frequency), int(seed)))
+
     def barostat_info(self):
         info = BarostatInfo()
         _check(library().omm_ctx_barostat_info(self._h, C.byref(info)))
@@ -803,6 +814,12 @@ class Batch:
 
     def set_barostat(self, pressure, temperature, frequency=25, seed=1):
         _check(library().omm_batch_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
+
+    def set_anisotropic_barostat(self, pressure, temperature, scale_axes=(True, True, True), frequency=25, seed=1):
+        p = (C.c_double * 3)(*[float(x) for x in pressure])
+        mask = sum(1 << a for a in range(3) if scale_axes[a])
+        _check(library().omm_batch_set_anisotropic_barostat(self._h, p, temperature, mask, int(frequency),
+                                                            int(seed)))
 
     def barostat_info(self, replica):
         info = BarostatInfo()

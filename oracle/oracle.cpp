@@ -1334,6 +1334,28 @@ void orc_scale_positions(int32_t n, double* pos, double scale, int32_t rigid, co
 }
 
 // barostat.rs:160-163
+// context.rs:976-1029 with a diagonal transform (the anisotropic barostat's
+// proposal, anisotropic_barostat.rs:148-157): rigid molecules move by
+// (s_a - 1) times their unweighted centroid along each axis a.
+void orc_scale_positions_axes(int32_t n, double* pos, const double scale[3], const int32_t* molecule_of,
+                              int32_t nmol) {
+    std::vector<double> center((size_t)nmol * 3, 0.0);
+    std::vector<int> count(nmol, 0);
+    for (int i = 0; i < n; ++i) {
+        int m = molecule_of[i];
+        for (int a = 0; a < 3; ++a) center[(size_t)m * 3 + a] += pos[3 * i + a];
+        ++count[m];
+    }
+    for (int m = 0; m < nmol; ++m) {
+        double inverse_count = 1.0 / (double)count[m];
+        for (int a = 0; a < 3; ++a) center[(size_t)m * 3 + a] *= inverse_count;
+    }
+    for (int i = 0; i < n; ++i) {
+        int m = molecule_of[i];
+        for (int a = 0; a < 3; ++a) pos[3 * i + a] += (scale[a] - 1.0) * center[(size_t)m * 3 + a];
+    }
+}
+
 double orc_barostat_work(double energy_change, double pressure_bar, double old_volume, double new_volume,
                          int32_t scaled_particles, double temperature) {
     double kt = BOLTZMANN * temperature;

@@ -72,6 +72,7 @@ def lib():
         L.orc_molecules.restype = C.c_int32
         L.orc_molecules.argtypes = [C.c_int32, C.c_int32, _ip, _ip, _ip]
         L.orc_scale_positions.argtypes = [C.c_int32, _dp, C.c_double, C.c_int32, _ip, C.c_int32]
+        L.orc_scale_positions_axes.argtypes = [C.c_int32, _dp, _dp, _ip, C.c_int32]
         L.orc_barostat_work.restype = C.c_double
         L.orc_barostat_work.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
                                         C.c_double]
@@ -295,6 +296,13 @@ def scale_positions(pos, scale, rigid, molecule_of, nmol):
     lib().orc_scale_positions(len(p), _p(p), scale, int(rigid), _p(_i(molecule_of), C.c_int32), nmol)
     return p
 
+
+def scale_positions_axes(pos, scale, molecule_of, nmol):
+    """Rigid-molecule scaling by a diagonal transform (context.rs:976-1029)."""
+    p = _f(pos).copy()
+    lib().orc_scale_positions_axes(len(p) // 3 if p.ndim == 1 else len(p), _p(p), _p(_f(scale)),
+                                   _p(_i(molecule_of), C.c_int32), int(nmol))
+    return p
 
 def barostat_work(de, pressure, v_old, v_new, scaled, temperature):
     return lib().orc_barostat_work(de, pressure, v_old, v_new, scaled, temperature)

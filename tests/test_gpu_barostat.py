@@ -106,3 +106,65 @@ def test_npt_run_stays_physical():
     v = box[0, 0] * box[1, 1] * box[2, 2]
     assert 0.8 * v0 < v < 1.2 * v0
     assert box[0, 0] == box[1, 1] == box[2, 2]  # isotropic
+
+
+@pytest.mark.parametrize("pressure,axes", [((1.0, 1.0, 1.0), (True, True, True)),
+                                           ((2.0e4, -2.0e4, 1.0), (True, True, False))])
+def test_anisotropic_attempts_match_oracle(pressure, axes):
+    """MonteCarloAnisotropicBarostat (anisotropic_barostat.rs:122-211): each
+    attempt scales one enabled axis by V'/V; energies, rigid scaling, box,
+    work (with that axis' pressure) and outcome against the oracle."""
+    s = water(500, 2.5, seed=32)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, seed=4)
+    ctx.set_anisotropic_barostat(pressure, 300.0, axes, 25, seed=78)
+    mol, nmol = orc.molecules(len(s.masses), s.bonds)
+    pos0, box0 = s.positions.copy(), s.box.copy()
+    seen = set()
+    for attempt in range(8):
+        ctx.barostat_attempt()
+        info = ctx.barostat_info()
+        st = ctx.get_state()
+        box1 = ctx.box()
+        a = info["axis"]
+        assert axes[a]
+        seen.add(a)
+        scale = np.ones(3)
+        scale[a] = info["volume_new"] / info["volume_old"]
+        pos_scaled = orc.scale_positions_axes(pos0, scale, mol, nmol).reshape(-1, 3)
+        box_scaled = box0 * scale[None, :]
+        e_old = oracle_energy(s, pos0, box0)
+        e_new = oracle_energy(s, pos_scaled, box_scaled)
+        print(f"P={pressure} attempt {attempt}: axis {a} dV={info['volume_new'] - info['volume_old']:+.4f} "
+              f"w={info['work']:+.3f} accepted={info['accepted_last']}")
+        assert abs(info["energy_old"] - e_old) <= 1e-6 * abs(e_old)
+        assert abs(info["energy_new"] - e_new) <= 1e-6 * abs(e_new)
+        w_ref = orc.barostat_work(info["energy_new"] - info["energy_old"], pressure[a], info["volume_old"],
+                                  info["volume_new"], nmol, 300.0)
+        assert info["work"] == pytest.approx(w_ref, rel=1e-12, abs=1e-9)
+        if info["accepted_last"]:
+            assert np.abs(st["positions"] - pos_scaled).max() < 1e-12
+            assert np.allclose(box1, box_scaled, rtol=1e-15, atol=0)
+        else:
+            assert np.array_equal(st["positions"], pos0)
+            assert np.array_equal(box1, box0)
+        pos0, box0 = st["positions"], box1
+    assert info["total_attempted"] == 8
+    assert seen == {k for k in range(3) if axes[k]}
+
+
+def test_anisotropic_npt_keeps_disabled_axis_and_batch_matches():
+    reps = [(216, 1.86), (350, 2.19)]
+    sys_ = [water(m, e, seed=210 + k) for k, (m, e) in enumerate(reps)]
+    batch = ommhip.Batch(sys_, temperature=300.0, friction=1.0, seeds=[41, 42])
+    batch.set_anisotropic_barostat((1.0, 1.0, 1.0), 300.0, (True, False, True), 5, seed=60)
+    batch.set_velocities_to_temperature(300.0, 70)
+    batch.step(25)
+    for r, s in enumerate(sys_):
+        single = ommhip.Context(s, temperature=300.0, friction=1.0, seed=41 + r)
+        single.set_anisotropic_barostat((1.0, 1.0, 1.0), 300.0, (True, False, True), 5, seed=60 + r)
+        single.set_velocities_to_temperature(300.0, 70 + r)
+        single.step(25)
+        assert batch.box(r)[1, 1] == s.box[1, 1]  # y never scaled
+        assert np.array_equal(batch.box(r), single.box())
+        assert np.array_equal(batch.get_state(r)["positions"], single.get_state()["positions"])
+        assert batch.barostat_info(r)["total_attempted"] == 5

@@ -474,3 +474,21 @@ def test_constrained_lfmiddle_keeps_water_rigid():
     mom0 = (masses[:, None] * v0).sum(0)
     mom1 = (masses[:, None] * v1.reshape(-1, 3)).sum(0)
     assert np.abs(mom1 - mom0).max() < 1e-12
+
+
+def test_rigid_scaling_by_axes():
+    # The anisotropic proposal's diagonal transform: equal factors are the
+    # isotropic scaling; one factor moves molecules along that axis only,
+    # rigidly.
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(0, 2.0, size=(30, 3))
+    bonds = np.array([[3 * m, 3 * m + 1] for m in range(10)] + [[3 * m, 3 * m + 2] for m in range(10)])
+    mol, nmol = orc.molecules(30, bonds)
+    s = 1.013
+    iso = orc.scale_positions(pos, s, True, mol, nmol)
+    assert np.abs(orc.scale_positions_axes(pos, [s, s, s], mol, nmol) - iso).max() < 1e-15
+    ax = orc.scale_positions_axes(pos, [1.0, s, 1.0], mol, nmol).reshape(-1, 3)
+    assert np.array_equal(ax[:, 0], pos[:, 0]) and np.array_equal(ax[:, 2], pos[:, 2])
+    d0 = pos[1::3] - pos[0::3]
+    d1 = ax[1::3] - ax[0::3]
+    assert np.abs(d1 - d0).max() < 1e-15
