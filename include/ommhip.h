@@ -313,7 +313,9 @@ typedef struct omm_barostat_info {
     double volume_old, volume_new;            /* nm^3 */
     double energy_old, energy_new, work;      /* kJ/mol */
     int32_t accepted_last, skipped;
-    int32_t axis;                             /* anisotropic barostat: axis of the last attempt */
+    int32_t axis;                             /* anisotropic / membrane: axis of the last attempt */
+    double scale[3];                          /* box scale factors of the last proposal */
+    double area_change;                       /* membrane: xy area change of the last proposal (nm^2) */
 } omm_barostat_info;
 
 omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
@@ -324,6 +326,12 @@ omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
  * barostat set before; frequency 0 removes it. */
 omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[3], double temperature,
                                             int32_t scale_axes, int32_t frequency, uint64_t seed);
+
+/* MonteCarloMembraneBarostat (forces/membrane_barostat.rs:61-292): pressure
+ * (bar), surface tension (bar nm), xy_mode 0 isotropic / 1 anisotropic,
+ * z_mode 0 free / 1 fixed / 2 constant volume. */
+omm_status omm_ctx_set_membrane_barostat(omm_ctx* ctx, double pressure, double surface_tension, double temperature,
+                                         int32_t xy_mode, int32_t z_mode, int32_t frequency, uint64_t seed);
 
 /* Current box vectors (row-major a, b, c); they change under a barostat
  * (Context::get_state box vectors, context.rs:930-1105). */
@@ -425,6 +433,9 @@ omm_status omm_batch_set_barostat(omm_batch* b, double pressure, double temperat
 omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info);
 omm_status omm_batch_set_anisotropic_barostat(omm_batch* b, const double pressure[3], double temperature,
                                               int32_t scale_axes, int32_t frequency, uint64_t seed);
+omm_status omm_batch_set_membrane_barostat(omm_batch* b, double pressure, double surface_tension,
+                                           double temperature, int32_t xy_mode, int32_t z_mode, int32_t frequency,
+                                           uint64_t seed);
 omm_status omm_batch_barostat_attempt(omm_batch* b);
 
 /* Context::get_state of replica r; any output pointer may be NULL. */

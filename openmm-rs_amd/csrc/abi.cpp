@@ -266,6 +266,8 @@ static void fill_info(const ommhip::BaroState& st, omm_barostat_info* info) {
     info->accepted_last = st.accepted_last;
     info->skipped = st.skipped;
     info->axis = st.axis;
+    for (int a = 0; a < 3; ++a) info->scale[a] = st.scale[a];
+    info->area_change = st.area_change;
 }
 
 omm_status omm_ctx_set_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
@@ -277,6 +279,11 @@ omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[
                                             int32_t scale_axes, int32_t frequency, uint64_t seed) {
     CTX_CALL(if (!pressure) throw Error(OMM_ERR_INVALID, "null pressure");
              ctx->ctx.set_anisotropic_barostat(pressure, temperature, scale_axes, frequency, seed));
+}
+
+omm_status omm_ctx_set_membrane_barostat(omm_ctx* ctx, double pressure, double surface_tension, double temperature,
+                                         int32_t xy_mode, int32_t z_mode, int32_t frequency, uint64_t seed) {
+    CTX_CALL(ctx->ctx.set_membrane_barostat(pressure, surface_tension, temperature, xy_mode, z_mode, frequency, seed));
 }
 
 omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info) {
@@ -380,6 +387,12 @@ omm_status omm_batch_set_anisotropic_barostat(omm_batch* b, const double pressur
                                               int32_t scale_axes, int32_t frequency, uint64_t seed) {
     BATCH_CALL(if (!pressure) throw Error(OMM_ERR_INVALID, "null pressure");
                b->ctx.set_anisotropic_barostat(pressure, temperature, scale_axes, frequency, seed));
+}
+
+omm_status omm_batch_set_membrane_barostat(omm_batch* b, double pressure, double surface_tension,
+                                           double temperature, int32_t xy_mode, int32_t z_mode, int32_t frequency,
+                                           uint64_t seed) {
+    BATCH_CALL(b->ctx.set_membrane_barostat(pressure, surface_tension, temperature, xy_mode, z_mode, frequency, seed));
 }
 
 omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info) {

@@ -35,6 +35,8 @@ struct BaroState {
     double axis_volume_scale[3];
     long long axis_attempted[3], axis_accepted[3];
     int axis;
+    double scale[3];     // box scale factors of the last proposal
+    double area_change;  // membrane barostat: change of the xy area (nm^2)
 };
 
 // Distance constraints (constraints.rs) of massive particles, grouped into
@@ -56,9 +58,14 @@ struct BaroDev {
     bool enabled = false;
     int frequency = 0, phase = 0;  // phase: steps since the last attempt slot
     double pressure = 0.0, temperature = 0.0;
-    bool anisotropic = false;      // MonteCarloAnisotropicBarostat: one axis per attempt
+    // 0 isotropic (MonteCarloBarostat), 1 anisotropic (one axis per
+    // attempt), 2 membrane (MonteCarloMembraneBarostat: xy coupled or not,
+    // z free / fixed / at constant volume, surface tension).
+    int kind = 0;
     double axis_pressure[3] = {};
-    int scale_axes = 7;            // bit d: axis d may be scaled
+    int scale_axes = 7;            // anisotropic: bit d lets axis d be scaled
+    int xy_mode = 0, z_mode = 0;   // membrane: xy 0 isotropic / 1 anisotropic; z 0 free / 1 fixed / 2 constant volume
+    double surface_tension = 0.0;  // membrane: bar nm
     uint64_t seed = 0;
     int nmol = 0;
     DevBuf<int> mol_start, mol_atoms, mol_sys, nmol_sys;
@@ -99,6 +106,9 @@ public:
     // pressures (bar); each attempt scales one enabled axis (scale_axes bit d).
     void set_anisotropic_barostat(const double pressure[3], double temperature, int scale_axes, int frequency,
                                   uint64_t seed);
+    // MonteCarloMembraneBarostat (membrane_barostat.rs:167-292).
+    void set_membrane_barostat(double pressure, double surface_tension, double temperature, int xy_mode, int z_mode,
+                               int frequency, uint64_t seed);
     // One barostat attempt now, outside the step sequence (test hook).
     void barostat_attempt();
     BaroState barostat_state(int r);

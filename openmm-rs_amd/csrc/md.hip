@@ -395,7 +395,8 @@ __device__ __forceinline__ void baro_record(BaroState& B, bool anisotropic, bool
 }
 
 __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved_box, double* scale, uint64_t seed,
-                               double cutoff, int anisotropic, int scale_axes, unsigned* status) {
+                               double cutoff, int kind, int scale_axes, int xy_mode, int z_mode, unsigned* status) {
+    const bool anisotropic = kind != 0;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nsys) return;
     BaroState& B = st[r];
@@ -408,10 +409,24 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
     B.u_accept = (double)c[1] * (1.0 / 4294967296.0);
     B.total_attempted += 1;
     double step = B.volume_scale;
-    if (anisotropic) {
+    const double u_axis = (double)c[2] * (1.0 / 4294967296.0);
+    if (kind == 2) {
+        // membrane_barostat.rs:178-190: x or y (one axis when coupled), or z
+        // when it is free, each draw of the three uniform.
+        int axis;
+        if (z_mode == 0) {
+            const int cand = min((int)(u_axis * 3.0), 2);
+            axis = cand < 2 ? (xy_mode == 0 ? 0 : cand) : 2;
+        } else {
+            const int cand = min((int)(u_axis * 2.0), 1);
+            axis = xy_mode == 0 ? 0 : cand;
+        }
+        B.axis = axis;
+        step = B.axis_volume_scale[axis];
+    } else if (anisotropic) {
         // The k-th enabled axis, k uniform (anisotropic_barostat.rs:133-139).
         const int enabled = __popc(scale_axes & 7);
-        int k = (int)((double)c[2] * (1.0 / 4294967296.0) * enabled);
+        int k = (int)(u_axis * enabled);
         if (k >= enabled) k = enabled - 1;
         int axis = 0;
         for (int a = 0; a < 3; ++a)
@@ -426,11 +441,12 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
         step = B.axis_volume_scale[axis];
     }
     const double v_old = box.v[0] * box.v[4] * box.v[8];
-    const double v_new = v_old + step * 2.0 * (B.u_volume - 0.5);
+    double v_new = v_old + step * 2.0 * (B.u_volume - 0.5);
     B.v_old = v_old;
     B.v_new = v_new;
     B.accepted_last = 0;
-    for (int a = 0; a < 3; ++a) scale[3 * r + a] = 1.0;
+    B.area_change = 0.0;
+    for (int a = 0; a < 3; ++a) B.scale[a] = scale[3 * r + a] = 1.0;
     if (!(isfinite(v_new) && v_new > 0.0)) {
         B.skipped = 1;
         baro_record(B, anisotropic, false, v_old);
@@ -438,7 +454,21 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
     }
     B.skipped = 0;
     double sc[3];
-    if (anisotropic) {
+    if (kind == 2) {
+        // membrane_barostat.rs:198-209
+        sc[0] = sc[1] = sc[2] = 1.0;
+        if (B.axis < 2 && xy_mode == 0) {
+            sc[0] = sc[1] = sqrt(v_new / v_old);
+        } else {
+            sc[B.axis] = v_new / v_old;
+        }
+        if (z_mode == 2) {
+            sc[2] = 1.0 / (sc[0] * sc[1]);
+            v_new = v_old;
+            B.v_new = v_new;
+        }
+        B.area_change = box.v[0] * box.v[4] * (sc[0] * sc[1] - 1.0);
+    } else if (anisotropic) {
         for (int a = 0; a < 3; ++a) sc[a] = a == B.axis ? v_new / v_old : 1.0;
     } else {
         sc[0] = sc[1] = sc[2] = cbrt(v_new / v_old);
@@ -450,7 +480,7 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
         B.skipped = 1;
         return;
     }
-    for (int a = 0; a < 3; ++a) scale[3 * r + a] = sc[a];
+    for (int a = 0; a < 3; ++a) B.scale[a] = scale[3 * r + a] = sc[a];
     // barostat.rs:28-40: each box vector's component a scaled by sc[a].
     for (int k = 0; k < 3; ++k)
         for (int a = 0; a < 3; ++a) box.v[3 * k + a] *= sc[a];
@@ -496,8 +526,9 @@ __device__ double system_energy(const A* e, const SysInfo& S, double volume) {
 template <class A>
 __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* __restrict__ saved_box, int* rejected,
                               const A* __restrict__ e_old, const A* __restrict__ e_new,
-                              const int* __restrict__ nmol_sys, double pressure, int anisotropic, double px, double py,
-                              double pz, double kt) {
+                              const int* __restrict__ nmol_sys, double pressure, int kind, double px, double py,
+                              double pz, double surface_tension, double kt) {
+    const bool anisotropic = kind != 0;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nsys) return;
     BaroState& B = st[r];
@@ -509,8 +540,10 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
     const double eo = system_energy(e_old + (size_t)r * E_SLOTS, S, saved_box[r].volume);
     const double en = system_energy(e_new + (size_t)r * E_SLOTS, S, S.box.volume);
     // barostat.rs:160-178, anisotropic_barostat.rs:162-188 (BAR_NM3_TO_KJ_PER_MOL = 0.0602214076)
-    if (anisotropic) pressure = B.axis == 0 ? px : (B.axis == 1 ? py : pz);
+    // membrane_barostat.rs:229-233: - surface tension x area change.
+    if (kind == 1) pressure = B.axis == 0 ? px : (B.axis == 1 ? py : pz);
     const double work = (en - eo) + pressure * 0.0602214076 * (B.v_new - B.v_old) -
+                        (kind == 2 ? surface_tension * 0.0602214076 * B.area_change : 0.0) -
                         (double)nmol_sys[r] * kt * log(B.v_new / B.v_old);
     bool accepted;
     if (isnan(work)) accepted = false;
@@ -876,7 +909,7 @@ void MdContext::enqueue_barostat(hipStream_t s) {
     }
     enqueue_forces_into(s, true, NbOperator::REBUILD_FLAG, B.force_fx.p, B.force_d.p, B.e_old_fx.p, B.e_old_d.p);
     hipLaunchKernelGGL(k_baro_propose, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p, B.saved_box.p,
-                       B.scale.p, B.seed, nb_->settings().cutoff, B.anisotropic ? 1 : 0, B.scale_axes, status_.p);
+                       B.scale.p, B.seed, nb_->settings().cutoff, B.kind, B.scale_axes, B.xy_mode, B.z_mode, status_.p);
     hipLaunchKernelGGL(k_baro_scale, dim3((B.nmol + 255) / 256), dim3(256), 0, s, B.nmol, B.mol_start.p,
                        B.mol_atoms.p, B.mol_sys.p, B.scale.p, pos_.p, B.pos_saved.p);
     nb_->force_rebuild(s);
@@ -884,12 +917,12 @@ void MdContext::enqueue_barostat(hipStream_t s) {
     const double kt = BOLTZMANN * B.temperature;
     if (mixed_)
         hipLaunchKernelGGL(k_baro_accept<long long>, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p,
-                           B.saved_box.p, B.rejected.p, B.e_old_fx.p, B.e_new_fx.p, B.nmol_sys.p, B.pressure,
-                           B.anisotropic ? 1 : 0, B.axis_pressure[0], B.axis_pressure[1], B.axis_pressure[2], kt);
+                           B.saved_box.p, B.rejected.p, B.e_old_fx.p, B.e_new_fx.p, B.nmol_sys.p, B.pressure, B.kind,
+                           B.axis_pressure[0], B.axis_pressure[1], B.axis_pressure[2], B.surface_tension, kt);
     else
         hipLaunchKernelGGL(k_baro_accept<double>, dim3(sb), dim3(64), 0, s, nsys_, nb_->systems_dev(), B.st.p,
-                           B.saved_box.p, B.rejected.p, B.e_old_d.p, B.e_new_d.p, B.nmol_sys.p, B.pressure,
-                           B.anisotropic ? 1 : 0, B.axis_pressure[0], B.axis_pressure[1], B.axis_pressure[2], kt);
+                           B.saved_box.p, B.rejected.p, B.e_old_d.p, B.e_new_d.p, B.nmol_sys.p, B.pressure, B.kind,
+                           B.axis_pressure[0], B.axis_pressure[1], B.axis_pressure[2], B.surface_tension, kt);
     hipLaunchKernelGGL(k_baro_restore, dim3((n_ + 255) / 256), dim3(256), 0, s, n_, nb_->atom_sys_dev(),
                        B.rejected.p, pos_.p, B.pos_saved.p);
     nb_->force_rebuild(s);
@@ -911,7 +944,7 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
     B.pressure = pressure;
     B.temperature = temperature;
     B.seed = seed;
-    B.anisotropic = false;
+    B.kind = 0;
     B.scale_axes = 7;
     if (!B.enabled) return;
     // Molecules: connected components over the bonds (system.rs:100-141).
@@ -982,9 +1015,24 @@ void MdContext::set_anisotropic_barostat(const double pressure[3], double temper
     if ((scale_axes & 7) == 0 && frequency > 0) throw Error(OMM_ERR_INVALID, "no axis is enabled for scaling");
     set_barostat(pressure[0], temperature, frequency, seed);
     BaroDev& B = baro_;
-    B.anisotropic = true;
+    B.kind = 1;
     B.scale_axes = scale_axes & 7;
     for (int a = 0; a < 3; ++a) B.axis_pressure[a] = pressure[a];
+}
+
+void MdContext::set_membrane_barostat(double pressure, double surface_tension, double temperature, int xy_mode,
+                                      int z_mode, int frequency, uint64_t seed) {
+    // membrane_barostat.rs:61-103, 278-292
+    if (!std::isfinite(surface_tension)) throw Error(OMM_ERR_INVALID, "surface tension must be finite");
+    if (xy_mode < 0 || xy_mode > 1) throw Error(OMM_ERR_INVALID, "xy mode must be 0 (isotropic) or 1 (anisotropic)");
+    if (z_mode < 0 || z_mode > 2)
+        throw Error(OMM_ERR_INVALID, "z mode must be 0 (free), 1 (fixed) or 2 (constant volume)");
+    set_barostat(pressure, temperature, frequency, seed);
+    BaroDev& B = baro_;
+    B.kind = 2;
+    B.xy_mode = xy_mode;
+    B.z_mode = z_mode;
+    B.surface_tension = surface_tension;
 }
 
 bool MdContext::set_parameter(const std::string& name, double value) {

@@ -70,10 +70,14 @@ class BarostatInfo(C.Structure):
         ("energy_old", C.c_double), ("energy_new", C.c_double), ("work", C.c_double),
         ("accepted_last", C.c_int32), ("skipped", C.c_int32),
         ("axis", C.c_int32),
+        ("scale", C.c_double * 3),
+        ("area_change", C.c_double),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["scale"] = tuple(self.scale)
+        return d
 
 
 class NbDesc(C.Structure):
@@ -170,6 +174,9 @@ def library():
                                                          C.c_uint64]
         lib.omm_batch_set_anisotropic_barostat.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32, C.c_int32,
                                                            C.c_uint64]
+        for fn in ("omm_ctx_set_membrane_barostat", "omm_batch_set_membrane_barostat"):
+            getattr(lib, fn).argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                         C.c_int32, C.c_uint64]
         lib.omm_ctx_apply_velocity_constraints.argtypes = [C.c_void_p, C.c_double]
         lib.omm_ctx_set_integrator.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
         lib.omm_batch_set_velocities_to_temperature.argtypes = [C.c_void_p, C.c_double, C.c_uint64]
@@ -695,6 +702,12 @@ class Context:
         """MonteCarloBarostat(pressure bar, temperature K, frequency) with rigid molecules."""
         _check(library().omm_ctx_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
 
+    def set_membrane_barostat(self, pressure, surface_tension, temperature, xy_mode=0, z_mode=0, frequency=25,
+                              seed=1):
+        """MonteCarloMembraneBarostat(pressure bar, surface tension bar nm, temperature K, xy/z modes)."""
+        _check(library().omm_ctx_set_membrane_barostat(self._h, pressure, surface_tension, temperature, int(xy_mode),
+                                                       int(z_mode), int(frequency), int(seed)))
+
     def set_anisotropic_barostat(self, pressure, temperature, scale_axes=(True, True, True), frequency=25, seed=1):
         """MonteCarloAnisotropicBarostat(pressure[3] bar, temperature K, scale_axes, frequency)."""
         p = (C.c_double * 3)(*[float(x) for x in pressure])
@@ -814,6 +827,11 @@ class Batch:
 
     def set_barostat(self, pressure, temperature, frequency=25, seed=1):
         _check(library().omm_batch_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
+
+    def set_membrane_barostat(self, pressure, surface_tension, temperature, xy_mode=0, z_mode=0, frequency=25,
+                              seed=1):
+        _check(library().omm_batch_set_membrane_barostat(self._h, pressure, surface_tension, temperature,
+                                                         int(xy_mode), int(z_mode), int(frequency), int(seed)))
 
     def set_anisotropic_barostat(self, pressure, temperature, scale_axes=(True, True, True), frequency=25, seed=1):
         p = (C.c_double * 3)(*[float(x) for x in pressure])

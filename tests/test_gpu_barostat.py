@@ -168,3 +168,51 @@ def test_anisotropic_npt_keeps_disabled_axis_and_batch_matches():
         assert np.array_equal(batch.box(r), single.box())
         assert np.array_equal(batch.get_state(r)["positions"], single.get_state()["positions"])
         assert batch.barostat_info(r)["total_attempted"] == 5
+
+
+@pytest.mark.parametrize("xy_mode,z_mode,tension", [(0, 0, 0.0), (1, 2, 500.0), (1, 1, -300.0)])
+def test_membrane_attempts_match_oracle(xy_mode, z_mode, tension):
+    """MonteCarloMembraneBarostat (membrane_barostat.rs:167-292): coupled or
+    independent x/y, z free / fixed / at constant volume, and the surface
+    tension term of the work; each attempt against the oracle."""
+    s = water(500, 2.5, seed=33)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, seed=5)
+    ctx.set_membrane_barostat(1.0, tension, 300.0, xy_mode, z_mode, 25, seed=79)
+    mol, nmol = orc.molecules(len(s.masses), s.bonds)
+    pos0, box0 = s.positions.copy(), s.box.copy()
+    axes = set()
+    for attempt in range(8):
+        ctx.barostat_attempt()
+        info = ctx.barostat_info()
+        st = ctx.get_state()
+        box1 = ctx.box()
+        a, sc = info["axis"], np.array(info["scale"])
+        axes.add(a)
+        if xy_mode == 0 and a < 2:
+            assert a == 0 and sc[0] == sc[1]
+        if z_mode == 1:
+            assert a < 2 and sc[2] == 1.0
+        if z_mode == 2:
+            assert a < 2 and sc[2] == 1.0 / (sc[0] * sc[1]) and info["volume_new"] == info["volume_old"]
+        area0 = box0[0, 0] * box0[1, 1]
+        assert info["area_change"] == pytest.approx(area0 * (sc[0] * sc[1] - 1.0), rel=1e-12, abs=1e-15)
+        pos_scaled = orc.scale_positions_axes(pos0, sc, mol, nmol).reshape(-1, 3)
+        box_scaled = box0 * sc[None, :]
+        e_old = oracle_energy(s, pos0, box0)
+        e_new = oracle_energy(s, pos_scaled, box_scaled)
+        assert abs(info["energy_old"] - e_old) <= 1e-6 * abs(e_old)
+        assert abs(info["energy_new"] - e_new) <= 1e-6 * abs(e_new)
+        w_ref = orc.barostat_work(info["energy_new"] - info["energy_old"], 1.0, info["volume_old"],
+                                  info["volume_new"], nmol, 300.0) - tension * 0.0602214076 * info["area_change"]
+        print(f"modes {xy_mode}/{z_mode}: axis {a} scale {sc} w={info['work']:+.3f} accepted={info['accepted_last']}")
+        assert info["work"] == pytest.approx(w_ref, rel=1e-12, abs=1e-9)
+        if info["accepted_last"]:
+            assert np.abs(st["positions"] - pos_scaled).max() < 1e-12
+            assert np.allclose(box1, box_scaled, rtol=1e-15, atol=0)
+        else:
+            assert np.array_equal(st["positions"], pos0)
+            assert np.array_equal(box1, box0)
+        pos0, box0 = st["positions"], box1
+    assert info["total_attempted"] == 8
+    if z_mode == 0 and xy_mode == 0:
+        assert axes <= {0, 2}
