@@ -316,6 +316,8 @@ typedef struct omm_barostat_info {
     int32_t axis;                             /* anisotropic / membrane: axis of the last attempt */
     double scale[3];                          /* box scale factors of the last proposal */
     double area_change;                       /* membrane: xy area change of the last proposal (nm^2) */
+    double log_shape;                         /* flexible: 2 ln s_x + ln s_y of the last proposal */
+    double box_new[9];                        /* the last proposal's box vectors */
 } omm_barostat_info;
 
 omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
@@ -326,6 +328,13 @@ omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
  * barostat set before; frequency 0 removes it. */
 omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[3], double temperature,
                                             int32_t scale_axes, int32_t frequency, uint64_t seed);
+
+/* MonteCarloFlexibleBarostat (forces/flexible_barostat.rs:132-238): every
+ * lower-triangle box element moves by up to an adaptive length step; the
+ * trial box is reduced and molecules scale by the diagonal ratios.
+ * (volume_scale in omm_barostat_info then reports the length step.) */
+omm_status omm_ctx_set_flexible_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
+                                         uint64_t seed);
 
 /* MonteCarloMembraneBarostat (forces/membrane_barostat.rs:61-292): pressure
  * (bar), surface tension (bar nm), xy_mode 0 isotropic / 1 anisotropic,
@@ -433,6 +442,8 @@ omm_status omm_batch_set_barostat(omm_batch* b, double pressure, double temperat
 omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_info* info);
 omm_status omm_batch_set_anisotropic_barostat(omm_batch* b, const double pressure[3], double temperature,
                                               int32_t scale_axes, int32_t frequency, uint64_t seed);
+omm_status omm_batch_set_flexible_barostat(omm_batch* b, double pressure, double temperature, int32_t frequency,
+                                           uint64_t seed);
 omm_status omm_batch_set_membrane_barostat(omm_batch* b, double pressure, double surface_tension,
                                            double temperature, int32_t xy_mode, int32_t z_mode, int32_t frequency,
                                            uint64_t seed);

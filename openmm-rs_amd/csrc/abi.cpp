@@ -268,6 +268,8 @@ static void fill_info(const ommhip::BaroState& st, omm_barostat_info* info) {
     info->axis = st.axis;
     for (int a = 0; a < 3; ++a) info->scale[a] = st.scale[a];
     info->area_change = st.area_change;
+    info->log_shape = st.log_shape;
+    for (int k = 0; k < 9; ++k) info->box_new[k] = st.box_new[k];
 }
 
 omm_status omm_ctx_set_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
@@ -279,6 +281,11 @@ omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[
                                             int32_t scale_axes, int32_t frequency, uint64_t seed) {
     CTX_CALL(if (!pressure) throw Error(OMM_ERR_INVALID, "null pressure");
              ctx->ctx.set_anisotropic_barostat(pressure, temperature, scale_axes, frequency, seed));
+}
+
+omm_status omm_ctx_set_flexible_barostat(omm_ctx* ctx, double pressure, double temperature, int32_t frequency,
+                                         uint64_t seed) {
+    CTX_CALL(ctx->ctx.set_flexible_barostat(pressure, temperature, frequency, seed));
 }
 
 omm_status omm_ctx_set_membrane_barostat(omm_ctx* ctx, double pressure, double surface_tension, double temperature,
@@ -387,6 +394,11 @@ omm_status omm_batch_set_anisotropic_barostat(omm_batch* b, const double pressur
                                               int32_t scale_axes, int32_t frequency, uint64_t seed) {
     BATCH_CALL(if (!pressure) throw Error(OMM_ERR_INVALID, "null pressure");
                b->ctx.set_anisotropic_barostat(pressure, temperature, scale_axes, frequency, seed));
+}
+
+omm_status omm_batch_set_flexible_barostat(omm_batch* b, double pressure, double temperature, int32_t frequency,
+                                           uint64_t seed) {
+    BATCH_CALL(b->ctx.set_flexible_barostat(pressure, temperature, frequency, seed));
 }
 
 omm_status omm_batch_set_membrane_barostat(omm_batch* b, double pressure, double surface_tension,

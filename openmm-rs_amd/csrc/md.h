@@ -37,6 +37,8 @@ struct BaroState {
     int axis;
     double scale[3];     // box scale factors of the last proposal
     double area_change;  // membrane barostat: change of the xy area (nm^2)
+    double log_shape;    // flexible barostat: 2 ln s_x + ln s_y of the last proposal
+    double box_new[9];   // the proposal's box vectors
 };
 
 // Distance constraints (constraints.rs) of massive particles, grouped into
@@ -60,7 +62,9 @@ struct BaroDev {
     double pressure = 0.0, temperature = 0.0;
     // 0 isotropic (MonteCarloBarostat), 1 anisotropic (one axis per
     // attempt), 2 membrane (MonteCarloMembraneBarostat: xy coupled or not,
-    // z free / fixed / at constant volume, surface tension).
+    // z free / fixed / at constant volume, surface tension), 3 flexible
+    // (MonteCarloFlexibleBarostat: every lower-triangle box element moves;
+    // volume_scale then holds the length step).
     int kind = 0;
     double axis_pressure[3] = {};
     int scale_axes = 7;            // anisotropic: bit d lets axis d be scaled
@@ -106,6 +110,8 @@ public:
     // pressures (bar); each attempt scales one enabled axis (scale_axes bit d).
     void set_anisotropic_barostat(const double pressure[3], double temperature, int scale_axes, int frequency,
                                   uint64_t seed);
+    // MonteCarloFlexibleBarostat (flexible_barostat.rs:132-238).
+    void set_flexible_barostat(double pressure, double temperature, int frequency, uint64_t seed);
     // MonteCarloMembraneBarostat (membrane_barostat.rs:167-292).
     void set_membrane_barostat(double pressure, double surface_tension, double temperature, int xy_mode, int z_mode,
                                int frequency, uint64_t seed);

@@ -382,8 +382,24 @@ __device__ __forceinline__ void adapt_step(double& volume_scale, long long& atte
     }
 }
 
-__device__ __forceinline__ void baro_record(BaroState& B, bool anisotropic, bool accepted, double previous_volume) {
-    if (anisotropic) {
+__device__ __forceinline__ void baro_record(BaroState& B, int kind, bool accepted, double previous_volume) {
+    if (kind == 3) {
+        // flexible_barostat.rs:200-215: a length step, capped at 0.3 V^(1/3).
+        B.attempted += 1;
+        B.accepted += accepted ? 1 : 0;
+        if (B.attempted >= 10) {
+            const double fraction = (double)B.accepted / (double)B.attempted;
+            if (fraction < 0.25) {
+                B.volume_scale /= 1.1;
+            } else if (fraction > 0.75) {
+                B.volume_scale = fmin(B.volume_scale * 1.1, cbrt(previous_volume) * 0.3);
+            } else {
+                return;
+            }
+            B.attempted = 0;
+            B.accepted = 0;
+        }
+    } else if (kind != 0) {
         const int a = B.axis;
         adapt_step(B.axis_volume_scale[a], B.axis_attempted[a], B.axis_accepted[a], accepted, previous_volume);
         B.volume_scale = B.axis_volume_scale[a];
@@ -446,10 +462,61 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
     B.v_new = v_new;
     B.accepted_last = 0;
     B.area_change = 0.0;
+    B.log_shape = 0.0;
     for (int a = 0; a < 3; ++a) B.scale[a] = scale[3 * r + a] = 1.0;
+    if (kind == 3) {
+        // flexible_barostat.rs:139-168: every lower-triangle element moves by
+        // up to the length step; the trial box is put back in reduced form.
+        uint32_t c2[4] = {(uint32_t)B.total_attempted, (uint32_t)((uint64_t)B.total_attempted >> 32), 1u,
+                          0x42415230u};
+        philox4x32_10(c2, (uint32_t)key, (uint32_t)(key >> 32));
+        const double u[6] = {B.u_volume, u_axis, (double)c[3] * (1.0 / 4294967296.0),
+                             (double)c2[0] * (1.0 / 4294967296.0), (double)c2[1] * (1.0 / 4294967296.0),
+                             (double)c2[2] * (1.0 / 4294967296.0)};
+        double t[9];
+        for (int k = 0; k < 9; ++k) t[k] = box.v[k];
+        int q = 0;
+        for (int row = 0; row < 3; ++row)
+            for (int col = 0; col <= row; ++col) t[3 * row + col] += B.volume_scale * 2.0 * (u[q++] - 0.5);
+        bool valid = true;
+        for (int k = 0; k < 9; ++k) valid = valid && isfinite(t[k]);
+        for (int row = 0; row < 3 && valid; ++row) {
+            if (t[3 * row + row] == 0.0) valid = false;
+            else if (t[3 * row + row] < 0.0)
+                for (int col = 0; col < 3; ++col) t[3 * row + col] = -t[3 * row + col];
+        }
+        const int red[3][3] = {{2, 1, 1}, {2, 0, 0}, {1, 0, 0}};  // (target, source, component)
+        for (int k = 0; k < 3 && valid; ++k) {
+            const int tg = red[k][0], sr = red[k][1], cp = red[k][2];
+            const double shift = rint(t[3 * tg + cp] / t[3 * sr + cp]);
+            for (int col = 0; col < 3; ++col) t[3 * tg + col] -= shift * t[3 * sr + col];
+        }
+        for (int k = 0; k < 9; ++k) valid = valid && isfinite(t[k]);
+        v_new = valid ? t[0] * t[4] * t[8] : NAN;
+        B.v_new = v_new;
+        for (int k = 0; k < 9; ++k) B.box_new[k] = valid ? t[k] : box.v[k];
+        if (!(isfinite(v_new) && v_new > 0.0)) {
+            B.skipped = 1;
+            baro_record(B, kind, false, v_old);
+            return;
+        }
+        B.skipped = 0;
+        double sc[3];
+        for (int a = 0; a < 3; ++a) sc[a] = t[4 * a] / box.v[4 * a];
+        B.log_shape = 2.0 * log(sc[0]) + log(sc[1]);
+        if (!(2.0 * cutoff <= t[0] && 2.0 * cutoff <= t[4] && 2.0 * cutoff <= t[8])) {
+            atomicOr(status, ST_CUTOFF_BOX);
+            B.skipped = 1;
+            return;
+        }
+        for (int a = 0; a < 3; ++a) B.scale[a] = scale[3 * r + a] = sc[a];
+        for (int k = 0; k < 9; ++k) box.v[k] = t[k];
+        box_refresh(box);
+        return;
+    }
     if (!(isfinite(v_new) && v_new > 0.0)) {
         B.skipped = 1;
-        baro_record(B, anisotropic, false, v_old);
+        baro_record(B, kind, false, v_old);
         return;
     }
     B.skipped = 0;
@@ -540,11 +607,13 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
     const double eo = system_energy(e_old + (size_t)r * E_SLOTS, S, saved_box[r].volume);
     const double en = system_energy(e_new + (size_t)r * E_SLOTS, S, S.box.volume);
     // barostat.rs:160-178, anisotropic_barostat.rs:162-188 (BAR_NM3_TO_KJ_PER_MOL = 0.0602214076)
-    // membrane_barostat.rs:229-233: - surface tension x area change.
+    // membrane_barostat.rs:229-233: - surface tension x area change;
+    // flexible_barostat.rs:177-183: (N - 2) kT ln(V'/V) and - kT x shape.
     if (kind == 1) pressure = B.axis == 0 ? px : (B.axis == 1 ? py : pz);
+    const double nmol = (double)nmol_sys[r] - (kind == 3 ? 2.0 : 0.0);
     const double work = (en - eo) + pressure * 0.0602214076 * (B.v_new - B.v_old) -
                         (kind == 2 ? surface_tension * 0.0602214076 * B.area_change : 0.0) -
-                        (double)nmol_sys[r] * kt * log(B.v_new / B.v_old);
+                        nmol * kt * log(B.v_new / B.v_old) - (kind == 3 ? kt * B.log_shape : 0.0);
     bool accepted;
     if (isnan(work)) accepted = false;
     else if (work <= 0.0) accepted = true;
@@ -554,7 +623,7 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
     B.work = work;
     B.accepted_last = accepted ? 1 : 0;
     B.total_accepted += accepted ? 1 : 0;
-    baro_record(B, anisotropic, accepted, B.v_old);
+    baro_record(B, kind, accepted, kind == 3 ? B.v_new : B.v_old);
     if (!accepted) S.box = saved_box[r];
     rejected[r] = accepted ? 0 : 1;
 }
@@ -1018,6 +1087,19 @@ void MdContext::set_anisotropic_barostat(const double pressure[3], double temper
     B.kind = 1;
     B.scale_axes = scale_axes & 7;
     for (int a = 0; a < 3; ++a) B.axis_pressure[a] = pressure[a];
+}
+
+void MdContext::set_flexible_barostat(double pressure, double temperature, int frequency, uint64_t seed) {
+    set_barostat(pressure, temperature, frequency, seed);
+    BaroDev& B = baro_;
+    B.kind = 3;
+    if (!B.enabled) return;
+    // flexible_barostat.rs:224-229: length step 0.01 V^(1/3).
+    std::vector<BaroState> st(nsys_);
+    OMM_HIP(hipMemcpyAsync(st.data(), B.st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToHost, s_));
+    OMM_HIP(hipStreamSynchronize(s_));
+    for (BaroState& b : st) b.volume_scale = 0.01 * std::cbrt(b.volume_scale / 0.01);
+    B.st.upload(st, s_);
 }
 
 void MdContext::set_membrane_barostat(double pressure, double surface_tension, double temperature, int xy_mode,

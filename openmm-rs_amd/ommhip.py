@@ -72,11 +72,14 @@ class BarostatInfo(C.Structure):
         ("axis", C.c_int32),
         ("scale", C.c_double * 3),
         ("area_change", C.c_double),
+        ("log_shape", C.c_double),
+        ("box_new", C.c_double * 9),
     ]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["scale"] = tuple(self.scale)
+        d["box_new"] = np.array(self.box_new).reshape(3, 3)
         return d
 
 
@@ -174,6 +177,8 @@ def library():
                                                          C.c_uint64]
         lib.omm_batch_set_anisotropic_barostat.argtypes = [C.c_void_p, _dp, C.c_double, C.c_int32, C.c_int32,
                                                            C.c_uint64]
+        for fn in ("omm_ctx_set_flexible_barostat", "omm_batch_set_flexible_barostat"):
+            getattr(lib, fn).argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_uint64]
         for fn in ("omm_ctx_set_membrane_barostat", "omm_batch_set_membrane_barostat"):
             getattr(lib, fn).argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                          C.c_int32, C.c_uint64]
@@ -702,6 +707,10 @@ class Context:
         """MonteCarloBarostat(pressure bar, temperature K, frequency) with rigid molecules."""
         _check(library().omm_ctx_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
 
+    def set_flexible_barostat(self, pressure, temperature, frequency=25, seed=1):
+        """MonteCarloFlexibleBarostat(pressure bar, temperature K, frequency)."""
+        _check(library().omm_ctx_set_flexible_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
+
     def set_membrane_barostat(self, pressure, surface_tension, temperature, xy_mode=0, z_mode=0, frequency=25,
                               seed=1):
         """MonteCarloMembraneBarostat(pressure bar, surface tension bar nm, temperature K, xy/z modes)."""
@@ -827,6 +836,9 @@ class Batch:
 
     def set_barostat(self, pressure, temperature, frequency=25, seed=1):
         _check(library().omm_batch_set_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
+
+    def set_flexible_barostat(self, pressure, temperature, frequency=25, seed=1):
+        _check(library().omm_batch_set_flexible_barostat(self._h, pressure, temperature, int(frequency), int(seed)))
 
     def set_membrane_barostat(self, pressure, surface_tension, temperature, xy_mode=0, z_mode=0, frequency=25,
                               seed=1):

@@ -1334,6 +1334,26 @@ void orc_scale_positions(int32_t n, double* pos, double scale, int32_t rigid, co
 }
 
 // barostat.rs:160-163
+// flexible_barostat.rs:15-41: flip vectors with a negative diagonal, then
+// reduce c by b (y), c by a (x) and b by a (x).  0 = a valid reduced box.
+int32_t orc_reduced_box(double* v) {
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(v[k])) return 1;
+    for (int row = 0; row < 3; ++row) {
+        if (v[3 * row + row] == 0.0) return 1;
+        if (v[3 * row + row] < 0.0)
+            for (int c = 0; c < 3; ++c) v[3 * row + c] = -v[3 * row + c];
+    }
+    const int steps[3][3] = {{2, 1, 1}, {2, 0, 0}, {1, 0, 0}};
+    for (const auto& st : steps) {
+        const double shift = std::round(v[3 * st[0] + st[2]] / v[3 * st[1] + st[2]]);
+        for (int c = 0; c < 3; ++c) v[3 * st[0] + c] -= shift * v[3 * st[1] + c];
+    }
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(v[k])) return 1;
+    return 0;
+}
+
 // context.rs:976-1029 with a diagonal transform (the anisotropic barostat's
 // proposal, anisotropic_barostat.rs:148-157): rigid molecules move by
 // (s_a - 1) times their unweighted centroid along each axis a.

@@ -131,6 +131,8 @@ int32_t orc_molecules(int32_t n, int32_t nbonds, const int32_t* p1, const int32_
 void orc_scale_positions(int32_t n, double* pos, double scale, int32_t rigid,
                          const int32_t* molecule_of, int32_t nmol);
 /* MonteCarloBarostat::accept work (barostat.rs:154-178). */
+/* flexible_barostat.rs:15-41: reduced form of box vectors in place; 0 = valid. */
+int32_t orc_reduced_box(double* v);
 /* Rigid-molecule scaling with a diagonal transform (anisotropic barostat). */
 void orc_scale_positions_axes(int32_t n, double* pos, const double scale[3], const int32_t* molecule_of,
                               int32_t nmol);

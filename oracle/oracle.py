@@ -73,6 +73,8 @@ def lib():
         L.orc_molecules.argtypes = [C.c_int32, C.c_int32, _ip, _ip, _ip]
         L.orc_scale_positions.argtypes = [C.c_int32, _dp, C.c_double, C.c_int32, _ip, C.c_int32]
         L.orc_scale_positions_axes.argtypes = [C.c_int32, _dp, _dp, _ip, C.c_int32]
+        L.orc_reduced_box.restype = C.c_int32
+        L.orc_reduced_box.argtypes = [_dp]
         L.orc_barostat_work.restype = C.c_double
         L.orc_barostat_work.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
                                         C.c_double]
@@ -295,6 +297,12 @@ def scale_positions(pos, scale, rigid, molecule_of, nmol):
     p = _f(pos).copy()
     lib().orc_scale_positions(len(p), _p(p), scale, int(rigid), _p(_i(molecule_of), C.c_int32), nmol)
     return p
+
+
+def reduced_box(box):
+    """flexible_barostat.rs:15-41; None when the box has no reduced form."""
+    v = _f(box).reshape(9).copy()
+    return None if lib().orc_reduced_box(_p(v)) else v.reshape(3, 3)
 
 
 def scale_positions_axes(pos, scale, molecule_of, nmol):

@@ -216,3 +216,68 @@ def test_membrane_attempts_match_oracle(xy_mode, z_mode, tension):
     assert info["total_attempted"] == 8
     if z_mode == 0 and xy_mode == 0:
         assert axes <= {0, 2}
+
+
+def test_flexible_attempts_match_oracle():
+    """MonteCarloFlexibleBarostat (flexible_barostat.rs:132-238): each lower-
+    triangle box element moves by up to the length step, the trial box is
+    reduced, molecules scale by the diagonal ratios, and the work carries
+    (N - 2) kT ln(V'/V) and kT (2 ln s_x + ln s_y)."""
+    s = water(500, 2.5, seed=34)
+    ctx = ommhip.Context(s, temperature=300.0, friction=1.0, seed=6)
+    ctx.set_flexible_barostat(1.0, 300.0, 25, seed=80)
+    mol, nmol = orc.molecules(len(s.masses), s.bonds)
+    pos0, box0 = s.positions.copy(), s.box.copy()
+    kt = orc.boltzmann() * 300.0
+    outcomes = set()
+    step = ctx.barostat_info()["volume_scale"]
+    assert step == pytest.approx(0.01 * np.cbrt(np.prod(np.diag(box0))), rel=1e-14)
+    for attempt in range(8):
+        ctx.barostat_attempt()
+        info = ctx.barostat_info()
+        st = ctx.get_state()
+        box1 = ctx.box()
+        bn = info["box_new"]
+        lower = np.tril(np.ones((3, 3), bool))
+        assert np.all(bn[~lower] == 0.0)
+        assert np.abs(bn - box0)[lower].max() <= step * (1 + 1e-12)
+        assert np.array_equal(orc.reduced_box(bn), bn)
+        sc = np.diag(bn) / np.diag(box0)
+        assert np.allclose(info["scale"], sc, rtol=1e-15, atol=0)
+        assert info["log_shape"] == pytest.approx(2 * np.log(sc[0]) + np.log(sc[1]), rel=1e-12, abs=1e-15)
+        assert info["volume_new"] == pytest.approx(np.prod(np.diag(bn)), rel=1e-15)
+        pos_scaled = orc.scale_positions_axes(pos0, sc, mol, nmol).reshape(-1, 3)
+        e_old = oracle_energy(s, pos0, box0)
+        e_new = oracle_energy(s, pos_scaled, bn)
+        assert abs(info["energy_old"] - e_old) <= 1e-6 * abs(e_old)
+        assert abs(info["energy_new"] - e_new) <= 1e-6 * abs(e_new)
+        de = info["energy_new"] - info["energy_old"]
+        w_ref = (de + 1.0 * 0.0602214076 * (info["volume_new"] - info["volume_old"])
+                 - (nmol - 2) * kt * np.log(info["volume_new"] / info["volume_old"]) - kt * info["log_shape"])
+        print(f"attempt {attempt}: box_new diag {np.diag(bn)} tilt {bn[1, 0]:+.4f} w={info['work']:+.3f} "
+              f"accepted={info['accepted_last']}")
+        assert info["work"] == pytest.approx(w_ref, rel=1e-12, abs=1e-9)
+        if info["accepted_last"]:
+            assert np.abs(st["positions"] - pos_scaled).max() < 1e-12
+            assert np.array_equal(box1, bn)
+        else:
+            assert np.array_equal(st["positions"], pos0)
+            assert np.array_equal(box1, box0)
+        outcomes.add(info["accepted_last"])
+        pos0, box0 = st["positions"], box1
+        step = info["volume_scale"]
+    assert info["total_attempted"] == 8
+
+
+def test_flexible_npt_run_stays_physical():
+    s = water(500, 2.5, seed=42)
+    ctx = ommhip.Context(s, temperature=300.0, friction=5.0, seed=10)
+    ctx.set_flexible_barostat(1.0, 300.0, 10, seed=6)
+    ctx.set_velocities_to_temperature(300.0, 12)
+    ctx.step(200)
+    box = ctx.box()
+    st = ctx.get_state()
+    assert ctx.barostat_info()["total_attempted"] == 20
+    assert np.isfinite(st["potential_energy"])
+    assert np.array_equal(orc.reduced_box(box), box)
+    assert 0.8 * 2.5 ** 3 < np.prod(np.diag(box)) < 1.2 * 2.5 ** 3

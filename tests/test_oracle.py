@@ -492,3 +492,20 @@ def test_rigid_scaling_by_axes():
     d0 = pos[1::3] - pos[0::3]
     d1 = ax[1::3] - ax[0::3]
     assert np.abs(d1 - d0).max() < 1e-15
+
+
+def test_reduced_box():
+    # flexible_barostat.rs:15-41 on closed-form cases: a vector with a
+    # negative diagonal flips; b_x = 0.7 a_x becomes -0.3 a_x; c is reduced by
+    # b then a; an already reduced box is unchanged; a zero diagonal fails.
+    a = 3.0
+    box = np.array([[a, 0, 0], [0.7 * a, 2.5, 0], [-1.9 * a + 0.1, 1.4 * 2.5, 2.8]])
+    r = orc.reduced_box(box)
+    assert r[1, 0] == pytest.approx(-0.3 * a)
+    assert r[2, 1] == pytest.approx(1.4 * 2.5 - 2.5 * round(1.4))
+    assert abs(r[2, 0]) <= 0.5 * a + 1e-12 and abs(r[1, 0]) <= 0.5 * a
+    flipped = orc.reduced_box(np.array([[a, 0, 0], [0.1, -2.0, 0], [0, 0, 2.0]]))
+    assert flipped[1, 1] == 2.0 and flipped[1, 0] == -0.1
+    ok = np.array([[2.0, 0, 0], [0.3, 2.0, 0], [-0.4, 0.5, 2.0]])
+    assert np.array_equal(orc.reduced_box(ok), ok)
+    assert orc.reduced_box(np.diag([2.0, 0.0, 2.0])) is None
