@@ -294,7 +294,8 @@ omm_status omm_ctx_set_graph_steps(omm_ctx* ctx, int32_t steps);
 /* MonteCarloBarostat (forces/barostat.rs:59-218) added to the context:
  * isotropic volume moves every `frequency` steps at `pressure` bar and
  * `temperature` K, molecules (bond-connected components, system.rs:100-141)
- * scaled as rigid bodies; frequency 0 removes it.  The attempt runs at the
+ * scaled as rigid bodies; at frequency 0 it makes no attempts but stays set
+ * for the pressure queries.  The attempt runs at the
  * start of the step that completes a period (context/program.rs:5-45), with
  * energies from the FP64 energy path.  The random stream (Philox keyed by
  * seed) differs from the reference's ChaCha12, as the reference allows for
@@ -322,10 +323,31 @@ typedef struct omm_barostat_info {
 
 omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info);
 
+/* Context::compute_current_pressure (context/pressure.rs:14-42) for the
+ * MonteCarloBarostat set on the context (OMM_ERR_STATE without one), in bar:
+ * the FP64 potential energies at the box scaled by 1.001 and 0.999 with the
+ * molecules moved rigidly (context.rs:942-1029), and the kinetic energy of
+ * the molecular momenta (pressure.rs:107-131).  Positions, velocities, box
+ * and step count are unchanged. */
+omm_status omm_ctx_compute_current_pressure(omm_ctx* ctx, double* pressure);
+
+/* Context::compute_current_anisotropic_pressure and
+ * compute_current_membrane_pressure (pressure.rs:44-105): the pressure along
+ * each Cartesian axis (bar), for the barostat of that kind. */
+omm_status omm_ctx_compute_current_anisotropic_pressure(omm_ctx* ctx, double pressure[3]);
+omm_status omm_ctx_compute_current_membrane_pressure(omm_ctx* ctx, double pressure[3]);
+
+/* Context::compute_current_flexible_pressure (pressure.rs:133-170) and
+ * compute_stress_tensor (pressure.rs:172-209) for the flexible barostat:
+ * components xx yy zz xy xz yz (bar), each from trial boxes deformed by
+ * +-0.1 % and put in reduced form. */
+omm_status omm_ctx_compute_current_flexible_pressure(omm_ctx* ctx, double pressure[6]);
+omm_status omm_ctx_compute_stress_tensor(omm_ctx* ctx, int32_t include_kinetic, double stress[6]);
+
 /* MonteCarloAnisotropicBarostat (forces/anisotropic_barostat.rs:42-243):
  * pressures per axis (bar), scale_axes bit d enables axis d; each attempt
  * scales one enabled axis by V'/V with its own adaptive step.  Replaces any
- * barostat set before; frequency 0 removes it. */
+ * barostat set before; frequency 0 stops the attempts. */
 omm_status omm_ctx_set_anisotropic_barostat(omm_ctx* ctx, const double pressure[3], double temperature,
                                             int32_t scale_axes, int32_t frequency, uint64_t seed);
 
@@ -448,6 +470,14 @@ omm_status omm_batch_set_membrane_barostat(omm_batch* b, double pressure, double
                                            double temperature, int32_t xy_mode, int32_t z_mode, int32_t frequency,
                                            uint64_t seed);
 omm_status omm_batch_barostat_attempt(omm_batch* b);
+
+/* The pressure queries of every replica at once (the same trial evaluations
+ * serve all replicas): count values, or 3 / 6 per replica, replica-major. */
+omm_status omm_batch_compute_current_pressure(omm_batch* b, double* pressure);
+omm_status omm_batch_compute_current_anisotropic_pressure(omm_batch* b, double* pressure);
+omm_status omm_batch_compute_current_membrane_pressure(omm_batch* b, double* pressure);
+omm_status omm_batch_compute_current_flexible_pressure(omm_batch* b, double* pressure);
+omm_status omm_batch_compute_stress_tensor(omm_batch* b, int32_t include_kinetic, double* stress);
 
 /* Context::get_state of replica r; any output pointer may be NULL. */
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,

@@ -299,6 +299,29 @@ omm_status omm_ctx_barostat_info(omm_ctx* ctx, omm_barostat_info* info) {
 
 omm_status omm_ctx_barostat_attempt(omm_ctx* ctx) { CTX_CALL(ctx->ctx.barostat_attempt()); }
 
+#define NULL_OUT(p) \
+    if (!(p)) throw Error(OMM_ERR_INVALID, "null output")
+
+omm_status omm_ctx_compute_current_pressure(omm_ctx* ctx, double* pressure) {
+    CTX_CALL(NULL_OUT(pressure); ctx->ctx.current_pressure(pressure));
+}
+
+omm_status omm_ctx_compute_current_anisotropic_pressure(omm_ctx* ctx, double pressure[3]) {
+    CTX_CALL(NULL_OUT(pressure); ctx->ctx.axis_pressures(1, pressure));
+}
+
+omm_status omm_ctx_compute_current_membrane_pressure(omm_ctx* ctx, double pressure[3]) {
+    CTX_CALL(NULL_OUT(pressure); ctx->ctx.axis_pressures(2, pressure));
+}
+
+omm_status omm_ctx_compute_current_flexible_pressure(omm_ctx* ctx, double pressure[6]) {
+    CTX_CALL(NULL_OUT(pressure); ctx->ctx.flexible_pressure(pressure));
+}
+
+omm_status omm_ctx_compute_stress_tensor(omm_ctx* ctx, int32_t include_kinetic, double stress[6]) {
+    CTX_CALL(NULL_OUT(stress); ctx->ctx.stress_tensor(include_kinetic != 0, stress));
+}
+
 omm_status omm_ctx_get_box(omm_ctx* ctx, double box[9]) {
     CTX_CALL(if (!box) throw Error(OMM_ERR_INVALID, "null output"); ctx->ctx.get_box(0, box));
 }
@@ -413,6 +436,26 @@ omm_status omm_batch_barostat_info(omm_batch* b, int32_t replica, omm_barostat_i
 }
 
 omm_status omm_batch_barostat_attempt(omm_batch* b) { BATCH_CALL(b->ctx.barostat_attempt()); }
+
+omm_status omm_batch_compute_current_pressure(omm_batch* b, double* pressure) {
+    BATCH_CALL(NULL_OUT(pressure); b->ctx.current_pressure(pressure));
+}
+
+omm_status omm_batch_compute_current_anisotropic_pressure(omm_batch* b, double* pressure) {
+    BATCH_CALL(NULL_OUT(pressure); b->ctx.axis_pressures(1, pressure));
+}
+
+omm_status omm_batch_compute_current_membrane_pressure(omm_batch* b, double* pressure) {
+    BATCH_CALL(NULL_OUT(pressure); b->ctx.axis_pressures(2, pressure));
+}
+
+omm_status omm_batch_compute_current_flexible_pressure(omm_batch* b, double* pressure) {
+    BATCH_CALL(NULL_OUT(pressure); b->ctx.flexible_pressure(pressure));
+}
+
+omm_status omm_batch_compute_stress_tensor(omm_batch* b, int32_t include_kinetic, double* stress) {
+    BATCH_CALL(NULL_OUT(stress); b->ctx.stress_tensor(include_kinetic != 0, stress));
+}
 
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
                                double* potential_energy, int64_t* step_count) {

@@ -57,7 +57,8 @@ struct ConstraintDev {
 };
 
 struct BaroDev {
-    bool enabled = false;
+    bool present = false;  // a barostat is set (its molecules are known), perhaps at frequency 0
+    bool enabled = false;  // attempts are made (frequency > 0)
     int frequency = 0, phase = 0;  // phase: steps since the last attempt slot
     double pressure = 0.0, temperature = 0.0;
     // 0 isotropic (MonteCarloBarostat), 1 anisotropic (one axis per
@@ -73,6 +74,8 @@ struct BaroDev {
     uint64_t seed = 0;
     int nmol = 0;
     DevBuf<int> mol_start, mol_atoms, mol_sys, nmol_sys;
+    std::vector<int> mol_start_h, mol_atoms_h, mol_sys_h;
+    DevBuf<double> transform, trial_box;  // pressure queries: [9 nsys] each
     DevBuf<BaroState> st, snap_st;
     DevBuf<Box> saved_box;
     DevBuf<int> rejected;
@@ -102,8 +105,9 @@ public:
     // global parameters; returns whether the force has the parameter.
     bool set_parameter(const std::string& name, double value);
     // MonteCarloBarostat (barostat.rs:59-218) on every replica: isotropic
-    // moves every `frequency` steps (0 removes it), molecules scaled as rigid
-    // bodies, replica r drawing from seed + r.  alpha and the PME grid stay
+    // moves every `frequency` steps (0: none, the barostat stays set for the
+    // pressure queries), molecules scaled as rigid bodies, replica r drawing
+    // from seed + r.  alpha and the PME grid stay
     // as initialised, as in the reference.
     void set_barostat(double pressure, double temperature, int frequency, uint64_t seed);
     // MonteCarloAnisotropicBarostat (anisotropic_barostat.rs:42-243): per-axis
@@ -115,6 +119,15 @@ public:
     // MonteCarloMembraneBarostat (membrane_barostat.rs:167-292).
     void set_membrane_barostat(double pressure, double surface_tension, double temperature, int xy_mode, int z_mode,
                                int frequency, uint64_t seed);
+    // Pressure queries of the barostat set on the replicas (context/pressure.rs),
+    // per replica: central differences of the FP64 potential energy over
+    // trial boxes deformed by +-0.1 % with the molecules moved rigidly, plus
+    // the kinetic term of the molecular momenta.  Positions, velocities, boxes
+    // and the step count are unchanged.
+    void current_pressure(double* out);                     // MonteCarloBarostat: 1 value per replica
+    void axis_pressures(int kind, double* out);             // anisotropic (1) / membrane (2): 3 per replica
+    void flexible_pressure(double* out);                    // flexible: 6 per replica (xx yy zz xy xz yz)
+    void stress_tensor(bool include_kinetic, double* out);  // flexible: 6 per replica
     // One barostat attempt now, outside the step sequence (test hook).
     void barostat_attempt();
     BaroState barostat_state(int r);
@@ -151,6 +164,15 @@ private:
                              long long* energy_fx, double* energy_d);
     void enqueue_step(hipStream_t s, int rebuild, bool barostat);
     void enqueue_barostat(hipStream_t s);
+    void require_barostat(int kind, const char* what) const;
+    // Context::energy_at over the replicas (context.rs:942-1029): replica r's
+    // molecules moved rigidly by transforms[9 r ..] (row-major), its box set to
+    // boxes[9 r ..], the potential energies evaluated, and positions and boxes
+    // restored bit for bit.
+    void energies_at(const std::vector<double>& transforms, const std::vector<double>& boxes, double* energies);
+    // pressure.rs:107-131 with rigid molecules: 1/2 P_i P_j / M summed over the
+    // molecules of mass M > 0, 6 components per replica.
+    void kinetic_tensor(std::vector<double>& out);
     void setup_constraints(const omm_md_desc* descs, int nsys, const std::vector<double>& inv);
     void launch_bonded(hipStream_t s, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d);
     int graph_length() const { return baro_.enabled ? baro_.frequency : graph_steps_; }

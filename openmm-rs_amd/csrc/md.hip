@@ -410,6 +410,29 @@ __device__ __forceinline__ void baro_record(BaroState& B, int kind, bool accepte
     }
 }
 
+// flexible_barostat.rs:15-41: the box vectors t (row-major) put in reduced
+// form in place: each diagonal element made positive by flipping its vector,
+// then c -= round(c_y / b_y) b, c -= round(c_x / a_x) a, b -= round(b_x / a_x) a.
+// False when an element is not finite or a diagonal element is zero.
+__host__ __device__ inline bool reduce_box(double t[9]) {
+#pragma clang fp contract(off)
+    bool valid = true;
+    for (int k = 0; k < 9; ++k) valid = valid && __builtin_isfinite(t[k]);
+    for (int row = 0; row < 3 && valid; ++row) {
+        if (t[3 * row + row] == 0.0) valid = false;
+        else if (t[3 * row + row] < 0.0)
+            for (int col = 0; col < 3; ++col) t[3 * row + col] = -t[3 * row + col];
+    }
+    const int red[3][3] = {{2, 1, 1}, {2, 0, 0}, {1, 0, 0}};  // (target, source, component)
+    for (int k = 0; k < 3 && valid; ++k) {
+        const int tg = red[k][0], sr = red[k][1], cp = red[k][2];
+        const double shift = round(t[3 * tg + cp] / t[3 * sr + cp]);  // half away from zero, as f64::round
+        for (int col = 0; col < 3; ++col) t[3 * tg + col] -= shift * t[3 * sr + col];
+    }
+    for (int k = 0; k < 9; ++k) valid = valid && __builtin_isfinite(t[k]);
+    return valid;
+}
+
 __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved_box, double* scale, uint64_t seed,
                                double cutoff, int kind, int scale_axes, int xy_mode, int z_mode, unsigned* status) {
     const bool anisotropic = kind != 0;
@@ -478,20 +501,7 @@ __global__ void k_baro_propose(int nsys, SysInfo* sys, BaroState* st, Box* saved
         int q = 0;
         for (int row = 0; row < 3; ++row)
             for (int col = 0; col <= row; ++col) t[3 * row + col] += B.volume_scale * 2.0 * (u[q++] - 0.5);
-        bool valid = true;
-        for (int k = 0; k < 9; ++k) valid = valid && isfinite(t[k]);
-        for (int row = 0; row < 3 && valid; ++row) {
-            if (t[3 * row + row] == 0.0) valid = false;
-            else if (t[3 * row + row] < 0.0)
-                for (int col = 0; col < 3; ++col) t[3 * row + col] = -t[3 * row + col];
-        }
-        const int red[3][3] = {{2, 1, 1}, {2, 0, 0}, {1, 0, 0}};  // (target, source, component)
-        for (int k = 0; k < 3 && valid; ++k) {
-            const int tg = red[k][0], sr = red[k][1], cp = red[k][2];
-            const double shift = rint(t[3 * tg + cp] / t[3 * sr + cp]);
-            for (int col = 0; col < 3; ++col) t[3 * tg + col] -= shift * t[3 * sr + col];
-        }
-        for (int k = 0; k < 9; ++k) valid = valid && isfinite(t[k]);
+        const bool valid = reduce_box(t);
         v_new = valid ? t[0] * t[4] * t[8] : NAN;
         B.v_new = v_new;
         for (int k = 0; k < 9; ++k) B.box_new[k] = valid ? t[k] : box.v[k];
@@ -626,6 +636,49 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
     baro_record(B, kind, accepted, kind == 3 ? B.v_new : B.v_old);
     if (!accepted) S.box = saved_box[r];
     rejected[r] = accepted ? 0 : 1;
+}
+
+// Rigid-molecule move by a per-system 3x3 transform T (row-major): every atom
+// of a molecule is shifted by (T - I) c, c the molecule's geometric centre
+// (context.rs:976-1029 with rigid molecules).  A diagonal T multiplies the
+// diagonal only; a general T sums each row in order, without contraction.
+__global__ void k_transform_molecules(int nmol, const int* __restrict__ mol_start, const int* __restrict__ mol_atoms,
+                                      const int* __restrict__ mol_sys, const double* __restrict__ transform,
+                                      double* pos) {
+#pragma clang fp contract(off)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const int lo = mol_start[m], hi = mol_start[m + 1];
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int k = lo; k < hi; ++k)
+        for (int d = 0; d < 3; ++d) c[d] += pos[3 * mol_atoms[k] + d];
+    const double inv = 1.0 / (double)(hi - lo);
+    for (int d = 0; d < 3; ++d) c[d] *= inv;
+    double t[9];
+    for (int k = 0; k < 9; ++k) t[k] = transform[9 * (size_t)mol_sys[m] + k];
+    for (int d = 0; d < 3; ++d) t[4 * d] -= 1.0;
+    const bool diagonal = t[1] == 0.0 && t[2] == 0.0 && t[3] == 0.0 && t[5] == 0.0 && t[6] == 0.0 && t[7] == 0.0;
+    double off[3];
+    for (int d = 0; d < 3; ++d)
+        off[d] = diagonal ? t[4 * d] * c[d] : (t[3 * d] * c[0] + t[3 * d + 1] * c[1]) + t[3 * d + 2] * c[2];
+    for (int k = lo; k < hi; ++k)
+        for (int d = 0; d < 3; ++d) pos[3 * mol_atoms[k] + d] += off[d];
+}
+
+// Trial boxes of the pressure queries: with `boxes`, each system's box is
+// saved and replaced by boxes[9 r ..]; without, the saved box is put back.
+__global__ void k_swap_boxes(int nsys, SysInfo* sys, Box* saved, const double* __restrict__ boxes) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nsys) return;
+    if (boxes) {
+        saved[r] = sys[r].box;
+        Box b = sys[r].box;
+        for (int k = 0; k < 9; ++k) b.v[k] = boxes[9 * (size_t)r + k];
+        box_refresh(b);
+        sys[r].box = b;
+    } else {
+        sys[r].box = saved[r];
+    }
 }
 
 __global__ void k_baro_restore(int n, const int* __restrict__ atom_sys, const int* __restrict__ rejected, double* pos,
@@ -1015,7 +1068,9 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
     B.seed = seed;
     B.kind = 0;
     B.scale_axes = 7;
-    if (!B.enabled) return;
+    // At frequency 0 the barostat makes no attempts but stays set, as a
+    // barostat force whose frequency is 0 does, for the pressure queries.
+    B.present = true;
     // Molecules: connected components over the bonds (system.rs:100-141).
     std::vector<int> parent(n_);
     for (int i = 0; i < n_; ++i) parent[i] = i;
@@ -1051,6 +1106,9 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
     B.mol_start.upload(start, s_);
     B.mol_atoms.upload(atoms, s_);
     B.mol_sys.upload(msys, s_);
+    B.mol_start_h = start;
+    B.mol_atoms_h = atoms;
+    B.mol_sys_h = msys;
     B.nmol_sys.upload(per_sys, s_);
     std::vector<BaroState> st(nsys_);
     for (int r = 0; r < nsys_; ++r) {
@@ -1090,10 +1148,13 @@ void MdContext::set_anisotropic_barostat(const double pressure[3], double temper
 }
 
 void MdContext::set_flexible_barostat(double pressure, double temperature, int frequency, uint64_t seed) {
+    // Its trial boxes are triclinic, which the Ewald sum does not take
+    // (nonbonded.rs:1087-1097).
+    if (nb_->settings().method == OMM_NB_EWALD)
+        throw Error(OMM_ERR_INVALID, "Ewald requires a rectangular periodic box; use PME for triclinic boxes");
     set_barostat(pressure, temperature, frequency, seed);
     BaroDev& B = baro_;
     B.kind = 3;
-    if (!B.enabled) return;
     // flexible_barostat.rs:224-229: length step 0.01 V^(1/3).
     std::vector<BaroState> st(nsys_);
     OMM_HIP(hipMemcpyAsync(st.data(), B.st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToHost, s_));
@@ -1155,11 +1216,217 @@ void MdContext::barostat_attempt() {
 
 BaroState MdContext::barostat_state(int r) {
     if (r < 0 || r >= nsys_) throw Error(OMM_ERR_INVALID, "replica index out of range");
-    if (!baro_.enabled) throw Error(OMM_ERR_STATE, "no barostat is set");
+    if (!baro_.present) throw Error(OMM_ERR_STATE, "no barostat is set");
     BaroState st;
     OMM_HIP(hipMemcpyAsync(&st, baro_.st.p + r, sizeof(BaroState), hipMemcpyDeviceToHost, s_));
     OMM_HIP(hipStreamSynchronize(s_));
     return st;
+}
+
+namespace {
+constexpr double BAR_NM3_TO_KJ_PER_MOL = 0.0602214076;  // barostat.rs
+// (row, column) of the six tensor components xx yy zz xy xz yz (pressure.rs:10).
+constexpr int COMPONENTS[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+}  // namespace
+
+void MdContext::require_barostat(int kind, const char* what) const {
+    if (!baro_.present || baro_.kind != kind)
+        throw Error(OMM_ERR_STATE, std::string("the context has no ") + what);
+}
+
+void MdContext::energies_at(const std::vector<double>& transforms, const std::vector<double>& boxes,
+                            double* energies) {
+    BaroDev& B = baro_;
+    prepare();
+    const bool ewald = nb_->settings().method == OMM_NB_EWALD;
+    for (int r = 0; r < nsys_; ++r) {
+        const double* b = boxes.data() + 9 * (size_t)r;
+        nb_->check_box(b);
+        if (ewald && !(b[3] == 0.0 && b[6] == 0.0 && b[7] == 0.0))
+            throw Error(OMM_ERR_INVALID, "Ewald requires a rectangular periodic box; use PME for triclinic boxes");
+    }
+    B.transform.upload(transforms, s_);
+    B.trial_box.upload(boxes, s_);
+    const int sb = (nsys_ + 63) / 64;
+    OMM_HIP(hipMemcpyAsync(B.pos_saved.p, pos_.p, 3 * (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    hipLaunchKernelGGL(k_swap_boxes, dim3(sb), dim3(64), 0, s_, nsys_, nb_->systems_dev(), B.saved_box.p,
+                       (const double*)B.trial_box.p);
+    if (B.nmol > 0)
+        hipLaunchKernelGGL(k_transform_molecules, dim3((B.nmol + 255) / 256), dim3(256), 0, s_, B.nmol,
+                           B.mol_start.p, B.mol_atoms.p, B.mol_sys.p, (const double*)B.transform.p, pos_.p);
+    OMM_HIP(hipGetLastError());
+    nb_->force_rebuild(s_);
+    auto restore = [&]() {
+        OMM_HIP(hipMemcpyAsync(pos_.p, B.pos_saved.p, 3 * (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        hipLaunchKernelGGL(k_swap_boxes, dim3(sb), dim3(64), 0, s_, nsys_, nb_->systems_dev(), B.saved_box.p,
+                           (const double*)nullptr);
+        OMM_HIP(hipGetLastError());
+        nb_->force_rebuild(s_);
+        OMM_HIP(hipStreamSynchronize(s_));
+    };
+    try {
+        evaluate(nullptr, energies, true);
+    } catch (...) {
+        restore();
+        throw;
+    }
+    restore();
+}
+
+void MdContext::kinetic_tensor(std::vector<double>& out) {
+    const BaroDev& B = baro_;
+    std::vector<double> vel(3 * (size_t)n_);
+    vel_.download(vel.data(), vel.size(), s_);
+    OMM_HIP(hipStreamSynchronize(s_));
+    out.assign(6 * (size_t)nsys_, 0.0);
+    for (size_t m = 0; m + 1 < B.mol_start_h.size(); ++m) {
+        double p[3] = {0.0, 0.0, 0.0}, mass = 0.0;
+        for (int k = B.mol_start_h[m]; k < B.mol_start_h[m + 1]; ++k) {
+            const int a = B.mol_atoms_h[k];
+            mass += masses_[a];
+            for (int d = 0; d < 3; ++d) p[d] += masses_[a] * vel[3 * (size_t)a + d];
+        }
+        if (mass > 0.0)
+            for (int c = 0; c < 6; ++c)
+                out[6 * (size_t)B.mol_sys_h[m] + c] += 0.5 * p[COMPONENTS[c][0]] * p[COMPONENTS[c][1]] / mass;
+    }
+}
+
+void MdContext::current_pressure(double* out) {
+    // pressure.rs:14-42
+    require_barostat(0, "MonteCarloBarostat");
+    std::vector<double> box(9 * (size_t)nsys_);
+    for (int r = 0; r < nsys_; ++r) get_box(r, box.data() + 9 * (size_t)r);
+    std::vector<double> T(9 * (size_t)nsys_), trial(box.size()), ep(nsys_), em(nsys_);
+    auto at = [&](double scale, double* e) {
+        for (int r = 0; r < nsys_; ++r)
+            for (int k = 0; k < 9; ++k) {
+                T[9 * r + k] = k % 4 == 0 ? scale : 0.0;
+                trial[9 * r + k] = box[9 * r + k] * scale;
+            }
+        energies_at(T, trial, e);
+    };
+    at(1.001, ep.data());
+    at(0.999, em.data());
+    std::vector<double> K;
+    kinetic_tensor(K);
+    for (int r = 0; r < nsys_; ++r) {
+        const double* b = box.data() + 9 * (size_t)r;
+        const double v = b[0] * b[4] * b[8];
+        const double kin = (K[6 * r] + K[6 * r + 1]) + K[6 * r + 2];
+        const double dv = v * (1.001 * 1.001 * 1.001 - 0.999 * 0.999 * 0.999);  // powi(3)
+        out[r] = ((2.0 / 3.0) * kin / v - (ep[r] - em[r]) / dv) / BAR_NM3_TO_KJ_PER_MOL;
+    }
+}
+
+void MdContext::axis_pressures(int kind, double* out) {
+    // pressure.rs:44-105
+    require_barostat(kind, kind == 1 ? "MonteCarloAnisotropicBarostat" : "MonteCarloMembraneBarostat");
+    std::vector<double> box(9 * (size_t)nsys_);
+    for (int r = 0; r < nsys_; ++r) get_box(r, box.data() + 9 * (size_t)r);
+    std::vector<double> K;
+    kinetic_tensor(K);
+    std::vector<double> T(9 * (size_t)nsys_), trial(box.size()), ep(nsys_), em(nsys_);
+    for (int axis = 0; axis < 3; ++axis) {
+        auto at = [&](double factor, double* e) {
+            double scale[3] = {1.0, 1.0, 1.0};
+            scale[axis] = factor;
+            for (int r = 0; r < nsys_; ++r)
+                for (int k = 0; k < 9; ++k) {
+                    T[9 * r + k] = k % 4 == 0 ? scale[k / 4] : 0.0;
+                    trial[9 * r + k] = box[9 * r + k] * scale[k % 3];  // barostat.rs:32-40
+                }
+            energies_at(T, trial, e);
+        };
+        at(1.001, ep.data());
+        at(0.999, em.data());
+        for (int r = 0; r < nsys_; ++r) {
+            const double* b = box.data() + 9 * (size_t)r;
+            const double v = b[0] * b[4] * b[8];
+            out[3 * r + axis] =
+                (2.0 * K[6 * r + axis] / v - (ep[r] - em[r]) / (0.002 * v)) / BAR_NM3_TO_KJ_PER_MOL;
+        }
+    }
+}
+
+void MdContext::flexible_pressure(double* out) {
+    // pressure.rs:133-170
+    require_barostat(3, "MonteCarloFlexibleBarostat");
+    std::vector<double> box(9 * (size_t)nsys_);
+    for (int r = 0; r < nsys_; ++r) get_box(r, box.data() + 9 * (size_t)r);
+    std::vector<double> K;
+    kinetic_tensor(K);
+    std::vector<double> T(9 * (size_t)nsys_), trial(box.size()), e[2] = {std::vector<double>(nsys_),
+                                                                         std::vector<double>(nsys_)};
+    for (int c = 0; c < 6; ++c) {
+        const int column = COMPONENTS[c][0], row = COMPONENTS[c][1];
+        std::vector<char> use(nsys_);
+        bool any = false;
+        for (int r = 0; r < nsys_; ++r) any |= (use[r] = c < 3 || box[9 * r + 3 * row + column] != 0.0);
+        for (int sign = 0; sign < 2; ++sign) {
+            std::fill(e[sign].begin(), e[sign].end(), 0.0);
+            if (!any) continue;
+            const double delta = sign == 0 ? 0.001 : -0.001;
+            for (int r = 0; r < nsys_; ++r) {
+                double* t = trial.data() + 9 * (size_t)r;
+                double* m = T.data() + 9 * (size_t)r;
+                for (int k = 0; k < 9; ++k) {
+                    t[k] = box[9 * r + k];
+                    m[k] = k % 4 == 0 ? 1.0 : 0.0;
+                }
+                if (!use[r]) continue;  // evaluated at its own box, result unused
+                t[3 * row + column] *= 1.0 + delta;
+                if (!reduce_box(t)) throw Error(OMM_ERR_INVALID, "invalid pressure trial box");
+                if (c < 3) m[4 * c] += delta;
+            }
+            std::vector<double> got(nsys_);
+            energies_at(T, trial, got.data());
+            for (int r = 0; r < nsys_; ++r)
+                if (use[r]) e[sign][r] = got[r];
+        }
+        for (int r = 0; r < nsys_; ++r) {
+            const double* b = box.data() + 9 * (size_t)r;
+            const double v = b[0] * b[4] * b[8];
+            out[6 * r + c] = (2.0 * K[6 * r + c] - (e[0][r] - e[1][r]) / 0.002) / (v * BAR_NM3_TO_KJ_PER_MOL);
+        }
+    }
+}
+
+void MdContext::stress_tensor(bool include_kinetic, double* out) {
+    // pressure.rs:172-209
+    require_barostat(3, "MonteCarloFlexibleBarostat");
+    std::vector<double> box(9 * (size_t)nsys_);
+    for (int r = 0; r < nsys_; ++r) get_box(r, box.data() + 9 * (size_t)r);
+    std::vector<double> K(6 * (size_t)nsys_, 0.0);
+    if (include_kinetic) kinetic_tensor(K);
+    std::vector<double> T(9 * (size_t)nsys_), trial(box.size()), e[2] = {std::vector<double>(nsys_),
+                                                                         std::vector<double>(nsys_)};
+    for (int c = 0; c < 6; ++c) {
+        const int i = COMPONENTS[c][0], j = COMPONENTS[c][1];
+        for (int sign = 0; sign < 2; ++sign) {
+            const double delta = sign == 0 ? 0.001 : -0.001;
+            for (int r = 0; r < nsys_; ++r) {
+                double* t = trial.data() + 9 * (size_t)r;
+                double* m = T.data() + 9 * (size_t)r;
+                for (int k = 0; k < 9; ++k) {
+                    t[k] = box[9 * r + k];
+                    m[k] = k % 4 == 0 ? 1.0 : 0.0;
+                }
+                for (int vec = 0; vec < 3; ++vec) {
+                    const double shift = delta * t[3 * vec + j];
+                    t[3 * vec + i] += shift;
+                }
+                if (!reduce_box(t)) throw Error(OMM_ERR_INVALID, "invalid stress trial box");
+                m[3 * i + j] += delta;
+            }
+            energies_at(T, trial, e[sign].data());
+        }
+        for (int r = 0; r < nsys_; ++r) {
+            const double* b = box.data() + 9 * (size_t)r;
+            const double v = b[0] * b[4] * b[8];
+            out[6 * r + c] = ((e[0][r] - e[1][r]) / 0.002 - 2.0 * K[6 * r + c]) / (v * BAR_NM3_TO_KJ_PER_MOL);
+        }
+    }
 }
 
 void MdContext::clear_accumulators(hipStream_t s) {

@@ -94,6 +94,8 @@ public:
     const int* lj_grid(int s = 0) const { return lj_grid_[s].data(); }
     bool ljpme() const { return cfg_[0].method == OMM_NB_LJPME; }
     const NbSettings& settings(int s = 0) const { return cfg_[s]; }
+    // Periodic box checks of every evaluation (nonbonded.rs:802-812).
+    void check_box(const double* box) const;
     bool mixed() const { return mixed_; }
     int num_particles(int s = 0) const { return n_[s]; }
     int total_particles() const { return ntotal_; }
@@ -185,7 +187,6 @@ private:
     // Grows the spread bins until the positions fit.
     void fit_bins(const double* pos_dev, unsigned* status, hipStream_t s);
     void upload_parameters(hipStream_t s);
-    void check_box(const double* box) const;
     // Device set-up of every system for these (validated) boxes: list,
     // mesh, kernel parameters and buffers.
     void setup_device(const std::vector<Box>& boxes);

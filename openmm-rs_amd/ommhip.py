@@ -190,6 +190,11 @@ def library():
         lib.omm_nb_set_global_parameter.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.POINTER(C.c_int32)]
         lib.omm_ctx_set_parameter.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.POINTER(C.c_int32)]
         lib.omm_batch_set_barostat.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int32, C.c_uint64]
+        for kind in ("ctx", "batch"):
+            for q in ("current_pressure", "current_anisotropic_pressure", "current_membrane_pressure",
+                      "current_flexible_pressure"):
+                getattr(lib, f"omm_{kind}_compute_{q}").argtypes = [C.c_void_p, _dp]
+            getattr(lib, f"omm_{kind}_compute_stress_tensor").argtypes = [C.c_void_p, C.c_int32, _dp]
         _lib = lib
     return _lib
 
@@ -731,6 +736,31 @@ class Context:
     def barostat_attempt(self):
         _check(library().omm_ctx_barostat_attempt(self._h))
 
+    def _query(self, fn, count, *args):
+        out = np.zeros(count)
+        _check(getattr(library(), fn)(self._h, *args, out.ctypes.data_as(_dp)))
+        return out
+
+    def compute_current_pressure(self):
+        """Context::compute_current_pressure of the MonteCarloBarostat (bar)."""
+        return float(self._query("omm_ctx_compute_current_pressure", 1)[0])
+
+    def compute_current_anisotropic_pressure(self):
+        """Pressure along x, y, z (bar) for the anisotropic barostat."""
+        return self._query("omm_ctx_compute_current_anisotropic_pressure", 3)
+
+    def compute_current_membrane_pressure(self):
+        """Pressure along x, y, z (bar) for the membrane barostat."""
+        return self._query("omm_ctx_compute_current_membrane_pressure", 3)
+
+    def compute_current_flexible_pressure(self):
+        """Pressure components xx yy zz xy xz yz (bar) for the flexible barostat."""
+        return self._query("omm_ctx_compute_current_flexible_pressure", 6)
+
+    def compute_stress_tensor(self, include_kinetic):
+        """Stress components xx yy zz xy xz yz (bar) for the flexible barostat."""
+        return self._query("omm_ctx_compute_stress_tensor", 6, int(bool(include_kinetic)))
+
     def box(self):
         b = np.zeros(9)
         _check(library().omm_ctx_get_box(self._h, b.ctypes.data_as(_dp)))
@@ -858,6 +888,27 @@ class Batch:
 
     def barostat_attempt(self):
         _check(library().omm_batch_barostat_attempt(self._h))
+
+    def _query(self, fn, per, *args):
+        out = np.zeros(len(self) * per)
+        _check(getattr(library(), fn)(self._h, *args, out.ctypes.data_as(_dp)))
+        return out if per == 1 else out.reshape(len(self), per)
+
+    def compute_current_pressure(self):
+        """Pressure of every replica (bar), MonteCarloBarostat."""
+        return self._query("omm_batch_compute_current_pressure", 1)
+
+    def compute_current_anisotropic_pressure(self):
+        return self._query("omm_batch_compute_current_anisotropic_pressure", 3)
+
+    def compute_current_membrane_pressure(self):
+        return self._query("omm_batch_compute_current_membrane_pressure", 3)
+
+    def compute_current_flexible_pressure(self):
+        return self._query("omm_batch_compute_current_flexible_pressure", 6)
+
+    def compute_stress_tensor(self, include_kinetic):
+        return self._query("omm_batch_compute_stress_tensor", 6, int(bool(include_kinetic)))
 
     def get_state(self, replica):
         n = self.sizes[replica]

@@ -3,8 +3,9 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py, always as the checker, never as the product path.
 The functions restate crates/openmm-core/src/forces/{nonbonded,pme,neighbors,
-ewald,harmonic_bond,harmonic_angle}.rs and integrators/langevin.rs; see
-oracle.h for the per-function citations and the unpinned last-ulp notes.
+ewald,harmonic_bond,harmonic_angle,barostat}.rs, integrators/langevin.rs and
+context/pressure.rs; see oracle.h for the per-function citations and the
+unpinned last-ulp notes.
 """
 from __future__ import annotations
 
@@ -311,6 +312,105 @@ def scale_positions_axes(pos, scale, molecule_of, nmol):
     lib().orc_scale_positions_axes(len(p) // 3 if p.ndim == 1 else len(p), _p(p), _p(_f(scale)),
                                    _p(_i(molecule_of), C.c_int32), int(nmol))
     return p
+
+BAR_NM3_TO_KJ_PER_MOL = 0.0602214076
+# (row, column) of the tensor components xx yy zz xy xz yz (pressure.rs:10).
+COMPONENTS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+def transform_positions(pos, transform, molecule_of, nmol):
+    """Rigid-molecule move by a 3x3 transform T (context.rs:976-1029): every
+    atom of a molecule is shifted by (T - I) c, c the molecule's geometric
+    centre."""
+    p = _f(pos).reshape(-1, 3)
+    mol = np.asarray(molecule_of)
+    sums = np.zeros((nmol, 3))
+    np.add.at(sums, mol, p)
+    centres = sums * (1.0 / np.bincount(mol, minlength=nmol))[:, None]
+    return p + (centres @ (np.asarray(transform, float) - np.eye(3)).T)[mol]
+
+
+def kinetic_tensor(vel, masses, molecule_of, nmol):
+    """pressure.rs:107-131 with rigid molecules: 1/2 P_i P_j / M summed over
+    the molecules of mass M > 0 (components xx yy zz xy xz yz)."""
+    v = _f(vel).reshape(-1, 3)
+    m = _f(masses)
+    mol = np.asarray(molecule_of)
+    P = np.zeros((nmol, 3))
+    np.add.at(P, mol, m[:, None] * v)
+    M = np.bincount(mol, weights=m, minlength=nmol)
+    keep = M > 0
+    return np.array([np.sum(0.5 * P[keep, i] * P[keep, j] / M[keep]) for i, j in COMPONENTS])
+
+
+def current_pressure(energy, pos, vel, masses, box, molecule_of, nmol):
+    """Context::compute_current_pressure (pressure.rs:14-42), bar; energy(pos,
+    box) is the potential energy of the system."""
+    box = _f(box).reshape(3, 3)
+    e = [energy(transform_positions(pos, s * np.eye(3), molecule_of, nmol), box * s) for s in (1.001, 0.999)]
+    kin = kinetic_tensor(vel, masses, molecule_of, nmol)[:3].sum()
+    v = box[0, 0] * box[1, 1] * box[2, 2]
+    dv = v * (1.001 ** 3 - 0.999 ** 3)
+    return ((2.0 / 3.0) * kin / v - (e[0] - e[1]) / dv) / BAR_NM3_TO_KJ_PER_MOL
+
+
+def axis_pressures(energy, pos, vel, masses, box, molecule_of, nmol):
+    """Anisotropic / membrane pressure per axis (pressure.rs:78-105), bar."""
+    box = _f(box).reshape(3, 3)
+    kin = kinetic_tensor(vel, masses, molecule_of, nmol)
+    v = box[0, 0] * box[1, 1] * box[2, 2]
+    out = np.zeros(3)
+    for axis in range(3):
+        e = []
+        for f in (1.001, 0.999):
+            scale = np.ones(3)
+            scale[axis] = f
+            e.append(energy(transform_positions(pos, np.diag(scale), molecule_of, nmol), box * scale[None, :]))
+        out[axis] = (2.0 * kin[axis] / v - (e[0] - e[1]) / (0.002 * v)) / BAR_NM3_TO_KJ_PER_MOL
+    return out
+
+
+def flexible_pressure(energy, pos, vel, masses, box, molecule_of, nmol):
+    """Context::compute_current_flexible_pressure (pressure.rs:133-170), bar."""
+    box = _f(box).reshape(3, 3)
+    kin = kinetic_tensor(vel, masses, molecule_of, nmol)
+    v = box[0, 0] * box[1, 1] * box[2, 2]
+    out = np.zeros(6)
+    for c, (column, row) in enumerate(COMPONENTS):
+        e = [0.0, 0.0]
+        if c < 3 or box[row, column] != 0.0:
+            for k, delta in enumerate((0.001, -0.001)):
+                trial = box.copy()
+                trial[row, column] *= 1.0 + delta
+                trial = reduced_box(trial)
+                if c < 3:
+                    t = np.eye(3)
+                    t[c, c] += delta
+                    e[k] = energy(transform_positions(pos, t, molecule_of, nmol), trial)
+                else:
+                    e[k] = energy(_f(pos).reshape(-1, 3), trial)
+        out[c] = (2.0 * kin[c] - (e[0] - e[1]) / 0.002) / (v * BAR_NM3_TO_KJ_PER_MOL)
+    return out
+
+
+def stress_tensor(energy, pos, vel, masses, box, molecule_of, nmol, include_kinetic):
+    """Context::compute_stress_tensor (pressure.rs:172-209), bar."""
+    box = _f(box).reshape(3, 3)
+    kin = kinetic_tensor(vel, masses, molecule_of, nmol) if include_kinetic else np.zeros(6)
+    v = box[0, 0] * box[1, 1] * box[2, 2]
+    out = np.zeros(6)
+    for c, (i, j) in enumerate(COMPONENTS):
+        e = []
+        for delta in (0.001, -0.001):
+            trial = box.copy()
+            trial[:, i] += delta * trial[:, j]
+            trial = reduced_box(trial)
+            t = np.eye(3)
+            t[i, j] += delta
+            e.append(energy(transform_positions(pos, t, molecule_of, nmol), trial))
+        out[c] = ((e[0] - e[1]) / 0.002 - 2.0 * kin[c]) / (v * BAR_NM3_TO_KJ_PER_MOL)
+    return out
+
 
 def barostat_work(de, pressure, v_old, v_new, scaled, temperature):
     return lib().orc_barostat_work(de, pressure, v_old, v_new, scaled, temperature)

@@ -30,3 +30,52 @@ def has_gpu():
         return ommhip.device_count() > 0
     except Exception:
         return False
+
+
+def lj_pair_system(box):
+    """Two Lennard-Jones atoms (no charge, no dispersion correction) within the
+    cutoff of a periodic box: the energy is 4 eps ((s/r)^12 - (s/r)^6)."""
+    f = ommhip.NonbondedForce()
+    f.set_nonbonded_method("CutoffPeriodic")
+    f.set_cutoff_distance(1.0)
+    f.set_use_dispersion_correction(False)
+    for _ in range(2):
+        f.add_particle(0.0, 0.3, 0.5)
+    pos = np.array([[0.2, 0.3, 0.4], [0.62, 0.47, 0.66]])
+    return f, pos, np.asarray(box, float)
+
+
+def lj_energy(d):
+    x = 0.3 / np.linalg.norm(d)
+    return 4 * 0.5 * (x**12 - x**6)
+
+
+LJ_BOX = [[3.0, 0, 0], [0.4, 4.0, 0], [-0.2, 0.3, 5.0]]
+
+
+def lj_pair_closed_forms(pos, vel, masses, box):
+    """The four pressure queries of two one-atom molecules from the pair
+    energy of the deformed separation (context/pressure.rs)."""
+    bar = orc.BAR_NM3_TO_KJ_PER_MOL
+    d = pos[1] - pos[0]
+    v = box[0, 0] * box[1, 1] * box[2, 2]
+    kin = sum(0.5 * m * np.array([vv[i] * vv[j] for i, j in orc.COMPONENTS]) for m, vv in zip(masses, vel))
+    iso = ((2 / 3) * kin[:3].sum() / v - (lj_energy(1.001 * d) - lj_energy(0.999 * d))
+           / (v * (1.001**3 - 0.999**3))) / bar
+    axis = np.array([(2 * kin[a] / v - (lj_energy(d * np.where(np.arange(3) == a, 1.001, 1.0))
+                                          - lj_energy(d * np.where(np.arange(3) == a, 0.999, 1.0))) / (0.002 * v)) / bar
+                     for a in range(3)])
+    flex = np.zeros(6)
+    stress = np.zeros(6)
+    for c, (i, j) in enumerate(orc.COMPONENTS):
+        e = []
+        for delta in (0.001, -0.001):
+            t = np.eye(3)
+            t[i, j] += delta
+            e.append(lj_energy(t @ d))
+        stress[c] = ((e[0] - e[1]) / 0.002 - 2 * kin[c]) / (v * bar)
+        if c < 3:
+            flex[c] = (2 * kin[c] - (e[0] - e[1]) / 0.002) / (v * bar)
+        else:
+            flex[c] = 2 * kin[c] / (v * bar)  # positions fixed: the pair keeps its separation
+    return iso, axis, flex, stress

@@ -15,6 +15,7 @@ import pytest
 
 import ommhip
 import oracle as orc
+from helpers import LJ_BOX, lj_energy, lj_pair_closed_forms, lj_pair_system
 from ommhip import NonbondedForce
 
 K = 138.93545764438198  # ONE_4PI_EPS0, nonbonded.rs:16
@@ -509,3 +510,58 @@ def test_reduced_box():
     ok = np.array([[2.0, 0, 0], [0.3, 2.0, 0], [-0.4, 0.5, 2.0]])
     assert np.array_equal(orc.reduced_box(ok), ok)
     assert orc.reduced_box(np.diag([2.0, 0.0, 2.0])) is None
+
+
+# --- context/pressure.rs -------------------------------------------------------
+
+def test_transform_positions_pins():
+    """The rigid transform: a diagonal one is the axis scaling of the C oracle,
+    one-atom molecules move by T x, and molecules keep their internal shape."""
+    s = ommhip.water_system(20, 1.2, seed=4)
+    mol, nmol = orc.molecules(len(s.masses), s.bonds)
+    sc = np.array([1.01, 0.98, 1.003])
+    assert np.allclose(orc.transform_positions(s.positions, np.diag(sc), mol, nmol),
+                       orc.scale_positions_axes(s.positions, sc, mol, nmol).reshape(-1, 3), rtol=0, atol=1e-15)
+    T = np.array([[1.01, 0.02, -0.01], [0.003, 0.99, 0.004], [0.0, -0.02, 1.002]])
+    singles = np.arange(len(s.masses))
+    assert np.allclose(orc.transform_positions(s.positions, T, singles, len(s.masses)), s.positions @ T.T,
+                       rtol=0, atol=1e-14)
+    moved = orc.transform_positions(s.positions, T, mol, nmol)
+    d0 = s.positions[1::3] - s.positions[0::3]
+    assert np.allclose(moved[1::3] - moved[0::3], d0, rtol=0, atol=1e-14)
+
+
+def test_pressure_kinetic_known_answer():
+    """pressure_uses_raw_molecular_momentum (tests/barostat.rs:112-149): with no
+    energy, P = 2 K / (3 V) of the molecular momenta; molecule {0, 1} of mass 4
+    carries P = (-2, 2, 6), the massless particle none; V = 60 nm^3."""
+    box = np.array([[3.0, 0, 0], [0.4, 4.0, 0], [-0.2, 0.3, 5.0]])
+    pos = np.array([[0.2, 0.3, 0.4], [0.9, 0.3, 0.4], [1.4, 1.1, 0.8]])
+    vel = np.array([[1.0, 2.0, 3.0], [-1.0, 0.0, 1.0], [10.0, 20.0, 30.0]])
+    masses = np.array([1.0, 3.0, 0.0])
+    mol, nmol = orc.molecules(3, np.array([[0, 1]]))
+    p = orc.current_pressure(lambda q, b: 0.0, pos, vel, masses, box, mol, nmol)
+    expected = 2.0 * (0.5 * (4.0 + 4.0 + 36.0) / 4.0) / (3.0 * 60.0 * orc.BAR_NM3_TO_KJ_PER_MOL)
+    assert p == pytest.approx(expected, rel=1e-14)
+    kin = orc.kinetic_tensor(vel, masses, mol, nmol)
+    assert np.allclose(kin, 0.5 * np.array([4, 4, 36, -4, -12, 12]) / 4.0, rtol=1e-15, atol=0)
+
+
+def test_pressure_queries_of_an_lj_pair_closed_form():
+    """The oracle's pressure queries, over its own nonbonded energy, against the
+    closed forms of an isolated Lennard-Jones pair in a triclinic box."""
+    f, pos, box = lj_pair_system(LJ_BOX)
+    vel = np.array([[0.3, -0.2, 0.5], [-0.1, 0.4, 0.2]])
+    masses = np.array([2.0, 3.0])
+    mol, nmol = np.arange(2), 2
+
+    def energy(q, b):
+        return orc.nb_evaluate(f.desc(), q, b)[1]
+
+    assert energy(pos, box) == pytest.approx(lj_energy(pos[1] - pos[0]), rel=1e-14)
+    iso, axis, flex, stress = lj_pair_closed_forms(pos, vel, masses, box)
+    assert orc.current_pressure(energy, pos, vel, masses, box, mol, nmol) == pytest.approx(iso, rel=1e-9)
+    assert np.allclose(orc.axis_pressures(energy, pos, vel, masses, box, mol, nmol), axis, rtol=1e-9, atol=1e-12)
+    assert np.allclose(orc.flexible_pressure(energy, pos, vel, masses, box, mol, nmol), flex, rtol=1e-9, atol=1e-12)
+    assert np.allclose(orc.stress_tensor(energy, pos, vel, masses, box, mol, nmol, True), stress, rtol=1e-9,
+                       atol=1e-12)
