@@ -605,7 +605,6 @@ __global__ void k_baro_accept(int nsys, SysInfo* sys, BaroState* st, const Box* 
                               const A* __restrict__ e_old, const A* __restrict__ e_new,
                               const int* __restrict__ nmol_sys, double pressure, int kind, double px, double py,
                               double pz, double surface_tension, double kt) {
-    const bool anisotropic = kind != 0;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nsys) return;
     BaroState& B = st[r];

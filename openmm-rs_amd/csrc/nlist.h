@@ -55,12 +55,14 @@ struct NlistDev {
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
     // Superblocks of SUPER consecutive blocks (systems with more than
-    // SUPER_MIN blocks): bounding boxes, per-system offsets and owners.
+    // SUPER_MIN blocks): bounding boxes and per-system offsets.
     int nsuper = 0;
     bool use_super = false;
-    DevBuf<int> super_off, super_sys;
+    DevBuf<int> super_off;
     DevBuf<double> super_center;
     DevBuf<float> super_half;
+    // Completion counter of the merged key-and-scan kernel (zero between builds).
+    DevBuf<int> done;
     DevBuf<int2> clist;         // cluster lists [nclusters][cap] of (sorted slot, atom)
     DevBuf<int> ccount;
     DevBuf<int> tasks, ntasks;  // pair-kernel tasks (cluster << 8 | first tile) and their count

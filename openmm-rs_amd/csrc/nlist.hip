@@ -65,13 +65,10 @@ __global__ void k_check(const double* __restrict__ pos, const double* __restrict
     }
 }
 
-__global__ void k_unit_keys(const double* __restrict__ pos, const int* __restrict__ unit_ptr,
-                            const int* __restrict__ unit_atoms, const int* __restrict__ unit_sys, int nunits,
-                            const SysInfo* __restrict__ sys, const int* __restrict__ cell_rank, int* unit_key,
-                            int* cell_count, const int* __restrict__ flag) {
-    if (!flag[0]) return;
-    int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nunits) return;
+__device__ __forceinline__ void unit_key_of(int u, const double* __restrict__ pos, const int* __restrict__ unit_ptr,
+                                            const int* __restrict__ unit_atoms, const int* __restrict__ unit_sys,
+                                            const SysInfo* __restrict__ sys, const int* __restrict__ cell_rank,
+                                            int* unit_key, int* cell_count, const int* __restrict__ flag) {
     const int si = unit_sys[u];
     if (!flag[1 + si]) {
         // Not rebuilt: the previous key reproduces the previous order.
@@ -93,6 +90,16 @@ __global__ void k_unit_keys(const double* __restrict__ pos, const int* __restric
     int key = S.cell_off + cell_rank[S.cell_off + (c[0] * S.ncell[1] + c[1]) * S.ncell[2] + c[2]];
     unit_key[u] = key;
     atomicAdd(&cell_count[key], 1);
+}
+
+__global__ void k_unit_keys(const double* __restrict__ pos, const int* __restrict__ unit_ptr,
+                            const int* __restrict__ unit_atoms, const int* __restrict__ unit_sys, int nunits,
+                            const SysInfo* __restrict__ sys, const int* __restrict__ cell_rank, int* unit_key,
+                            int* cell_count, const int* __restrict__ flag) {
+    if (!flag[0]) return;
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    unit_key_of(u, pos, unit_ptr, unit_atoms, unit_sys, sys, cell_rank, unit_key, cell_count, flag);
 }
 
 // Exclusive scan of in[0..n) into out[0..n], out[n] = total, in three
@@ -188,9 +195,7 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_down(int* in, int* out, int* c
 // for inputs of at most SCAN_SINGLE_MAX elements: one launch instead of three.
 constexpr int SCAN_SINGLE_MAX = 8 * SCAN_CHUNK;
 
-__global__ __launch_bounds__(SCAN_WG) void k_scan_single(int* in, int* out, int* copy, int n, int zero_input,
-                                                         const int* flag) {
-    if (!flag[0]) return;
+__device__ void scan_single_body(int* in, int* out, int* copy, int n, int zero_input) {
     __shared__ int s_wave[SCAN_WG / 64];
     int carry = 0;
     for (int base = 0; base < n; base += SCAN_CHUNK) {
@@ -221,6 +226,38 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_single(int* in, int* out, int*
     }
 }
 
+__global__ __launch_bounds__(SCAN_WG) void k_scan_single(int* in, int* out, int* copy, int n, int zero_input,
+                                                         const int* flag) {
+    if (!flag[0]) return;
+    scan_single_body(in, out, copy, n, zero_input);
+}
+
+// k_unit_keys and the one-workgroup scan of the cell counts in one launch:
+// the last workgroup to finish its keys (a completion counter, fenced on both
+// sides so the counts of every workgroup are visible) scans the counts and
+// resets the counter.  For at most SCAN_SINGLE_MAX cells.
+__global__ __launch_bounds__(SCAN_WG) void k_unit_keys_scan(const double* __restrict__ pos,
+                                                            const int* __restrict__ unit_ptr,
+                                                            const int* __restrict__ unit_atoms,
+                                                            const int* __restrict__ unit_sys, int nunits,
+                                                            const SysInfo* __restrict__ sys,
+                                                            const int* __restrict__ cell_rank, int* unit_key,
+                                                            int* cell_count, int* cell_start, int* cell_cursor,
+                                                            int ncells, int* done, const int* __restrict__ flag) {
+    if (!flag[0]) return;
+    const int u = blockIdx.x * SCAN_WG + threadIdx.x;
+    if (u < nunits) unit_key_of(u, pos, unit_ptr, unit_atoms, unit_sys, sys, cell_rank, unit_key, cell_count, flag);
+    __threadfence();
+    __syncthreads();
+    __shared__ int s_last;
+    if (threadIdx.x == 0) s_last = atomicAdd(done, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (threadIdx.x == 0) *done = 0;
+    scan_single_body(cell_count, cell_start, cell_cursor, ncells, 1);
+}
+
 __global__ void k_scatter_units(const int* __restrict__ unit_key, int nunits, int* cursor, int* sorted_units,
                                 const int* flag) {
     if (!flag[0]) return;
@@ -230,12 +267,9 @@ __global__ void k_scatter_units(const int* __restrict__ unit_key, int nunits, in
     sorted_units[slot] = u;
 }
 
-__global__ void k_sort_cells(const int* __restrict__ cell_start, int ncells, int* sorted_units,
-                             const int* __restrict__ unit_ptr, int* unit_size_sorted, const int* flag) {
-    if (!flag[0]) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    int lo = cell_start[c], hi = cell_start[c + 1];
+// Insertion sort of one cell's units by index (the scatter's atomics leave
+// them in arbitrary order; the index order makes the layout deterministic).
+__device__ __forceinline__ void sort_cell(int* sorted_units, int lo, int hi) {
     for (int i = lo + 1; i < hi; ++i) {
         int v = sorted_units[i];
         int j = i - 1;
@@ -245,6 +279,15 @@ __global__ void k_sort_cells(const int* __restrict__ cell_start, int ncells, int
         }
         sorted_units[j + 1] = v;
     }
+}
+
+__global__ void k_sort_cells(const int* __restrict__ cell_start, int ncells, int* sorted_units,
+                             const int* __restrict__ unit_ptr, int* unit_size_sorted, const int* flag) {
+    if (!flag[0]) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int lo = cell_start[c], hi = cell_start[c + 1];
+    sort_cell(sorted_units, lo, hi);
     if (unit_size_sorted)
         for (int i = lo; i < hi; ++i) {
             int u = sorted_units[i];
@@ -252,21 +295,13 @@ __global__ void k_sort_cells(const int* __restrict__ cell_start, int ncells, int
         }
 }
 
-// Places the atoms of sorted unit p at their slots.  Units are sorted by
-// (system, cell), so the atoms before unit p are those of the earlier systems
-// plus the earlier units of its own system; uniform > 0: every unit has that
-// many atoms (no size scan).  When every exception partner of an atom lies in
-// its own unit (closed units, e.g. rigid or flexible water), its sorted
-// exclusion list is written here as well.
-__global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* __restrict__ unit_offset, int uniform,
-                              const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
-                              const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
-                              const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
-                              const int* __restrict__ excl_idx, int closed, int nunits, int* sorted_atom,
-                              int* atom_slot, int* excl_sorted, const int* flag) {
-    if (!flag[0]) return;
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nunits) return;
+__device__ __forceinline__ void place_unit(int p, const int* __restrict__ sorted_units,
+                                           const int* __restrict__ unit_offset, int uniform,
+                                           const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
+                                           const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
+                                           const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
+                                           const int* __restrict__ excl_idx, int closed, int* sorted_atom,
+                                           int* atom_slot, int* excl_sorted, const int* flag) {
     int u = sorted_units[p];
     const int si = unit_sys[u];
     if (!flag[1 + si]) return;  // same keys, same slots: nothing changes
@@ -295,6 +330,43 @@ __global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* _
     }
 }
 
+// Places the atoms of sorted unit p at their slots.  Units are sorted by
+// (system, cell), so the atoms before unit p are those of the earlier systems
+// plus the earlier units of its own system; uniform > 0: every unit has that
+// many atoms (no size scan).  When every exception partner of an atom lies in
+// its own unit (closed units, e.g. rigid or flexible water), its sorted
+// exclusion list is written here as well.
+__global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* __restrict__ unit_offset, int uniform,
+                              const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
+                              const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
+                              const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
+                              const int* __restrict__ excl_idx, int closed, int nunits, int* sorted_atom,
+                              int* atom_slot, int* excl_sorted, const int* flag) {
+    if (!flag[0]) return;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nunits) return;
+    place_unit(p, sorted_units, unit_offset, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx,
+               closed, sorted_atom, atom_slot, excl_sorted, flag);
+}
+
+// Uniform units: one thread sorts the units of one cell and places them (no
+// size scan is needed between the two), one launch instead of two.
+__global__ void k_sort_place(const int* __restrict__ cell_start, int ncells, int* sorted_units, int uniform,
+                             const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
+                             const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
+                             const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
+                             const int* __restrict__ excl_idx, int closed, int* sorted_atom, int* atom_slot,
+                             int* excl_sorted, const int* flag) {
+    if (!flag[0]) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int lo = cell_start[c], hi = cell_start[c + 1];
+    sort_cell(sorted_units, lo, hi);
+    for (int p = lo; p < hi; ++p)
+        place_unit(p, sorted_units, nullptr, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx,
+                   closed, sorted_atom, atom_slot, excl_sorted, flag);
+}
+
 __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __restrict__ atom_slot,
                               const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx,
                               const int* __restrict__ block_sys, int npad, int* excl_sorted, const int* flag) {
@@ -318,6 +390,39 @@ __global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __
         }
     }
     for (int k = 0; k < MAXEX; ++k) excl_sorted[(size_t)s * MAXEX + k] = v[k];
+}
+
+// Bounding box of superblock sb (SUPER consecutive blocks of system si) from
+// its blocks' boxes; one wave, lane k reading block k.
+__device__ void super_bounds_of(int sb, int si, const SysInfo& S, const int* __restrict__ super_off,
+                                const double* center, const float* half, double* super_center, float* super_half) {
+    const int lane = threadIdx.x & 63;
+    const int first = S.block_off + (sb - super_off[si]) * SUPER;
+    const int J = first + lane;
+    double mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = 1e300;
+        mx[k] = -1e300;
+    }
+    if (J < S.block_off + S.nblocks) {
+        double d[3] = {center[3 * J] - center[3 * first], center[3 * J + 1] - center[3 * first + 1],
+                       center[3 * J + 2] - center[3 * first + 2]};
+        wrap_fast(d, S.box);
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = d[k] - (double)half[3 * J + k];
+            mx[k] = d[k] + (double)half[3 * J + k];
+        }
+    }
+    for (int off = 1; off < 64; off <<= 1)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
+            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
+        }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+            super_center[3 * sb + k] = center[3 * first + k] + 0.5 * (mn[k] + mx[k]);
+            super_half[3 * sb + k] = (float)(0.5 * (mx[k] - mn[k]) * (1.0 + 1e-6) + 1e-6);
+        }
 }
 
 // Two blocks per wave64: lanes 0-31 block 2*blockIdx.x, lanes 32-63 the next.
@@ -367,47 +472,18 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
     }
 }
 
-// Bounding box of each superblock (SUPER consecutive blocks of a system)
-// from its blocks' boxes, for flagged systems.  One wave per superblock.
-__global__ __launch_bounds__(64) void k_super_bounds(const int* __restrict__ super_off,
-                                                     const int* __restrict__ super_sys,
-                                                     const SysInfo* __restrict__ sys,
-                                                     const double* __restrict__ center,
+// Superblock bounds of flagged systems, one wave per superblock.
+__global__ __launch_bounds__(64) void k_super_bounds(int nsuper, const int* __restrict__ super_off, int nsys,
+                                                     const SysInfo* __restrict__ sys, const double* __restrict__ center,
                                                      const float* __restrict__ half, double* super_center,
                                                      float* super_half, const int* flag) {
     if (!flag[0]) return;
     const int sb = blockIdx.x;
-    const int si = super_sys[sb];
+    const int si = find_sys(super_off, nsys, sb);
     if (!flag[1 + si]) return;
-    const SysInfo& S = sys[si];
-    const int lane = threadIdx.x;
-    const int first = S.block_off + (sb - super_off[si]) * SUPER;
-    const int J = first + lane;
-    double mn[3], mx[3];
-    for (int k = 0; k < 3; ++k) {
-        mn[k] = 1e300;
-        mx[k] = -1e300;
-    }
-    if (J < S.block_off + S.nblocks) {
-        double d[3] = {center[3 * J] - center[3 * first], center[3 * J + 1] - center[3 * first + 1],
-                       center[3 * J + 2] - center[3 * first + 2]};
-        wrap_fast(d, S.box);
-        for (int k = 0; k < 3; ++k) {
-            mn[k] = d[k] - (double)half[3 * J + k];
-            mx[k] = d[k] + (double)half[3 * J + k];
-        }
-    }
-    for (int off = 1; off < 64; off <<= 1)
-        for (int k = 0; k < 3; ++k) {
-            mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
-            mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
-        }
-    if (lane == 0)
-        for (int k = 0; k < 3; ++k) {
-            super_center[3 * sb + k] = center[3 * first + k] + 0.5 * (mn[k] + mx[k]);
-            super_half[3 * sb + k] = (float)(0.5 * (mx[k] - mn[k]) * (1.0 + 1e-6) + 1e-6);
-        }
+    super_bounds_of(sb, si, sys[si], super_off, center, half, super_center, super_half);
 }
+
 
 // Cluster lists, one wave per cluster C of block I = C / CPB (CLIST_WAVES
 // clusters per workgroup).  Candidate blocks J > I of the same system pass
@@ -647,7 +723,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.cutoff = cutoff;
     nl.skin = skin;
     std::vector<int> eptr{0}, eidx, uptr{0}, uatoms, upos, usys, rank_all, block_sys, sorted, slot_of, atom_sys;
-    std::vector<int> block_off{0}, chunk_off{0}, super_off{0}, super_sys;
+    std::vector<int> block_off{0}, chunk_off{0}, super_off{0};
     nl.use_super = false;
     int natoms = 0, nblocks = 0, nchunks = 0, ncells = 0, max_npad = 0;
     nl.uniform = -1;
@@ -751,7 +827,6 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
         }
         for (int k = 0; k < nb; ++k) block_sys.push_back(si);
         const int nsb = (nb + SUPER - 1) / SUPER;
-        for (int k = 0; k < nsb; ++k) super_sys.push_back(si);
         super_off.push_back(super_off.back() + nsb);
         if (nb > SUPER_MIN) nl.use_super = true;
         atom_sys.insert(atom_sys.end(), n, si);
@@ -810,9 +885,10 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.block_minimg.alloc(nblocks);
     nl.nsuper = super_off.back();
     nl.super_off.upload(super_off, s);
-    nl.super_sys.upload(super_sys, s);
     nl.super_center.alloc(3 * (size_t)nl.nsuper);
     nl.super_half.alloc(3 * (size_t)nl.nsuper);
+    nl.done.alloc(1);
+    fill_async<int>(nl.done.p, 1, 0, s);
     nl.pos_build.alloc(3 * (size_t)natoms);
     nl.cap = std::min(nl.cap, 255 * JTILE);
     nl.clist.alloc((size_t)nl.nclusters * nl.cap);
@@ -850,20 +926,41 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     // unconditionally; otherwise every kernel exits early unless the
     // displacement check asked for a rebuild.
     const int* flag = forced ? nl.one.p : nl.flag.p;
-    hipLaunchKernelGGL(k_unit_keys, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, pos, nl.unit_ptr.p,
-                       nl.unit_atoms.p, nl.unit_sys.p, nl.nunits, sys_dev, nl.cell_rank.p, nl.unit_key.p,
-                       nl.cell_count.p, flag);
-    scan_async(nl, nl.cell_count.p, nl.cell_start.p, nl.cell_cursor.p, nl.ncells, 1, flag, s);
+    // Each kernel of the chain is a graph node that costs a few microseconds
+    // even when it exits at once, so steps without a rebuild pay for the
+    // chain's length: the cell-count scan rides on the key kernel and, for a
+    // single system, the sort of uniform units on their placement.  (Batches
+    // rebuild some replica on most steps, and there one thread per unit
+    // places faster than one thread per cell.)
+    if (nl.ncells <= SCAN_SINGLE_MAX) {
+        hipLaunchKernelGGL(k_unit_keys_scan, dim3(blocks_for(nl.nunits, SCAN_WG)), dim3(SCAN_WG), 0, s, pos,
+                           nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_sys.p, nl.nunits, sys_dev, nl.cell_rank.p,
+                           nl.unit_key.p, nl.cell_count.p, nl.cell_start.p, nl.cell_cursor.p, nl.ncells, nl.done.p,
+                           flag);
+    } else {
+        hipLaunchKernelGGL(k_unit_keys, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, pos, nl.unit_ptr.p,
+                           nl.unit_atoms.p, nl.unit_sys.p, nl.nunits, sys_dev, nl.cell_rank.p, nl.unit_key.p,
+                           nl.cell_count.p, flag);
+        scan_async(nl, nl.cell_count.p, nl.cell_start.p, nl.cell_cursor.p, nl.ncells, 1, flag, s);
+    }
     hipLaunchKernelGGL(k_scatter_units, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.unit_key.p, nl.nunits,
                        nl.cell_cursor.p, nl.sorted_units.p, flag);
-    hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p, nl.ncells,
-                       nl.sorted_units.p, nl.unit_ptr.p, nl.uniform > 0 ? (int*)nullptr : nl.unit_size_sorted.p, flag);
-    if (nl.uniform == 0)
-        scan_async(nl, nl.unit_size_sorted.p, nl.unit_offset.p, nullptr, nl.nunits, 0, flag, s);
-    hipLaunchKernelGGL(k_place_atoms, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.sorted_units.p,
-                       nl.unit_offset.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p, nl.unit_sys.p,
-                       sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.nunits, nl.sorted_atom.p, nl.atom_slot.p,
-                       nl.excl_sorted.p, flag);
+    if (nl.uniform > 0 && nl.nsys == 1) {
+        hipLaunchKernelGGL(k_sort_place, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p,
+                           nl.ncells, nl.sorted_units.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p,
+                           nl.unit_sys.p, sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.sorted_atom.p,
+                           nl.atom_slot.p, nl.excl_sorted.p, flag);
+    } else {
+        hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p,
+                           nl.ncells, nl.sorted_units.p, nl.unit_ptr.p,
+                           nl.uniform > 0 ? (int*)nullptr : nl.unit_size_sorted.p, flag);
+        if (nl.uniform == 0)
+            scan_async(nl, nl.unit_size_sorted.p, nl.unit_offset.p, nullptr, nl.nunits, 0, flag, s);
+        hipLaunchKernelGGL(k_place_atoms, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.sorted_units.p,
+                           nl.unit_offset.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p,
+                           nl.unit_sys.p, sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.nunits,
+                           nl.sorted_atom.p, nl.atom_slot.p, nl.excl_sorted.p, flag);
+    }
     if (!nl.closed)
         hipLaunchKernelGGL(k_excl_sorted, dim3(blocks_for(nl.nslots, 256)), dim3(256), 0, s, nl.sorted_atom.p,
                            nl.atom_slot.p, nl.excl_ptr.p, nl.excl_idx.p, nl.block_sys.p, nl.nslots,
@@ -873,8 +970,8 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
                        nl.block_minimg.p, nl.pos_build.p, nl.ntasks.p, flag);
     if (nl.use_super)
-        hipLaunchKernelGGL(k_super_bounds, dim3(nl.nsuper), dim3(64), 0, s, nl.super_off.p, nl.super_sys.p, sys_dev,
-                           nl.block_center.p, nl.block_half.p, nl.super_center.p, nl.super_half.p, flag);
+        hipLaunchKernelGGL(k_super_bounds, dim3(nl.nsuper), dim3(64), 0, s, nl.nsuper, nl.super_off.p, nl.nsys,
+                           sys_dev, nl.block_center.p, nl.block_half.p, nl.super_center.p, nl.super_half.p, flag);
     hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
                        nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p,
                        nl.super_off.p, nl.super_center.p, nl.super_half.p, reach, nl.cap, nl.nclusters, nl.clist.p,
