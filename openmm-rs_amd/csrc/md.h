@@ -162,7 +162,11 @@ private:
     void enqueue_forces(hipStream_t s, bool energy, int rebuild);
     void enqueue_forces_into(hipStream_t s, bool energy, int rebuild, long long* force_fx, double* force_d,
                              long long* energy_fx, double* energy_d);
-    void enqueue_step(hipStream_t s, int rebuild, bool barostat);
+    // One step; its integrator draws at step counter + offset, and `advance`
+    // > 0 ends a run of that many steps (advances the counter, clears the
+    // energies).  Eager steps use (1, 1); a captured graph (k + 1, 0) and
+    // (len, len) for its last step.
+    void enqueue_step(hipStream_t s, int rebuild, bool barostat, int offset = 1, int advance = 1);
     void enqueue_barostat(hipStream_t s);
     void require_barostat(int kind, const char* what) const;
     // Context::energy_at over the replicas (context.rs:942-1029): replica r's

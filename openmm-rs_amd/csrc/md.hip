@@ -135,10 +135,13 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
     }
 }
 
-// First kernel of every step: advance the RNG counter, clear the energies.
+// Last kernel of a run of `count` steps (one eager step, or a captured
+// graph): advance the step counter, clear the energies.  Inside the run, step
+// k's integrator draws at counter step[0] + k + 1, so a graph needs one such
+// node rather than one per step.
 template <class A>
-__global__ void k_step_begin(unsigned long long* step, A* energy, int nslots) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1ull;
+__global__ void k_step_advance(unsigned long long* step, int count, A* energy, int nslots) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += (unsigned long long)count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) energy[i] = (A)0;
 }
 
@@ -188,8 +191,9 @@ template <class A>
 __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__ vel, A* __restrict__ force,
                            const double* __restrict__ inv_mass, const int* __restrict__ atom_sys,
                            const SysInfo* __restrict__ sys, double dt, double decay, double variance,
-                           const unsigned long long* __restrict__ step, const double* __restrict__ pos_build,
-                           double lim2, int check, int* flag, const unsigned char* __restrict__ constrained) {
+                           const unsigned long long* __restrict__ step, int step_offset,
+                           const double* __restrict__ pos_build, double lim2, int check, int* flag,
+                           const unsigned char* __restrict__ constrained) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool moved = false;
     int si = 0;
@@ -203,7 +207,7 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
         if (im != 0.0) {  // massless particles do not move
             si = atom_sys[i];
             double xi[3];
-            step_normals(i, sys[si], step[0], xi);
+            step_normals(i, sys[si], step[0] + (unsigned long long)step_offset, xi);
             double noise = sqrt(variance * im);
             double half = 0.5 * dt;
             double d2 = 0.0;
@@ -288,7 +292,7 @@ __global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, 
                                    int mode, double* pos, double* vel, A* __restrict__ force,
                                    const double* __restrict__ inv_mass, const int* __restrict__ atom_sys,
                                    const SysInfo* __restrict__ sys, double dt, double decay, double variance,
-                                   const unsigned long long* __restrict__ step,
+                                   const unsigned long long* __restrict__ step, int step_offset,
                                    const double* __restrict__ pos_build, double lim2, int check, int* flag,
                                    double* prev, double* proposed, unsigned* status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,7 +324,7 @@ __global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, 
             for (int m = a0; m < a1; ++m) {
                 const int i = atoms[m];
                 double xi[3];
-                step_normals(i, sys[si], step[0], xi);
+                step_normals(i, sys[si], step[0] + (unsigned long long)step_offset, xi);
                 const double noise = sqrt(variance * inv_mass[i]);
                 for (int a = 0; a < 3; ++a) {
                     const int k = 3 * i + a;
@@ -1442,14 +1446,8 @@ void MdContext::clear_accumulators(hipStream_t s) {
 // One step.  Invariant: the force accumulator is zero on entry (the previous
 // integrator cleared it).  The integrator raises the list's rebuild flag when
 // an atom left the skin, and the next step's build kernels act on it.
-void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat) {
+void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat, int offset, int advance) {
     if (barostat) enqueue_barostat(s);
-    const int eslots = E_SLOTS * nsys_;
-    const int eb = std::min(64, (eslots + 255) / 256);
-    if (mixed_)
-        hipLaunchKernelGGL(k_step_begin<long long>, dim3(eb), dim3(256), 0, s, step_dev_.p, energy_fx_.p, eslots);
-    else
-        hipLaunchKernelGGL(k_step_begin<double>, dim3(eb), dim3(256), 0, s, step_dev_.p, energy_d_.p, eslots);
     enqueue_forces(s, false, rebuild);
     const double decay = std::exp(-friction_ * dt_);
     const double variance = BOLTZMANN * temperature_ * -std::expm1(-2.0 * friction_ * dt_);
@@ -1461,11 +1459,11 @@ void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat) {
     if (mixed_)
         hipLaunchKernelGGL(k_lfmiddle<long long>, dim3(blocks), dim3(256), 0, s, n_, pos_.p, vel_.p, force_fx_.p,
                            inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance, step_dev_.p,
-                           nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
+                           offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
     else
         hipLaunchKernelGGL(k_lfmiddle<double>, dim3(blocks), dim3(256), 0, s, n_, pos_.p, vel_.p, force_d_.p,
                            inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance, step_dev_.p,
-                           nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
+                           offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
     if (C.ncomp) {
         const int cb = (C.ncomp + 63) / 64;
         auto go = [&](auto* force) {
@@ -1473,11 +1471,21 @@ void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat) {
             hipLaunchKernelGGL(k_constrained_step<A>, dim3(cb), dim3(64), 0, s, C.ncomp, C.atom_off.p, C.atoms.p,
                                C.cons_off.p, C.c1.p, C.c2.p, C.d2.p, C.w1.p, C.w2.p, C.tolerance, 0, pos_.p, vel_.p,
                                force, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, nb_->pos_build(), lim * lim, check, nb_->list_flag(), C.prev.p,
+                               step_dev_.p, offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), C.prev.p,
                                C.proposed.p, status_.p);
         };
         if (mixed_) go(force_fx_.p);
         else go(force_d_.p);
+    }
+    if (advance > 0) {
+        const int eslots = E_SLOTS * nsys_;
+        const int eb = std::min(64, (eslots + 255) / 256);
+        if (mixed_)
+            hipLaunchKernelGGL(k_step_advance<long long>, dim3(eb), dim3(256), 0, s, step_dev_.p, advance,
+                               energy_fx_.p, eslots);
+        else
+            hipLaunchKernelGGL(k_step_advance<double>, dim3(eb), dim3(256), 0, s, step_dev_.p, advance, energy_d_.p,
+                               eslots);
     }
     OMM_HIP(hipGetLastError());
 }
@@ -1596,7 +1604,7 @@ void MdContext::project_constraints(double tolerance, bool velocities) {
         hipLaunchKernelGGL(k_constrained_step<A>, dim3(cb), dim3(64), 0, s_, C.ncomp, C.atom_off.p, C.atoms.p,
                            C.cons_off.p, C.c1.p, C.c2.p, C.d2.p, C.w1.p, C.w2.p, tolerance, velocities ? 2 : 1,
                            pos_.p, vel_.p, force, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, 1.0,
-                           0.0, step_dev_.p, nb_->pos_build(), 0.0, 0, nb_->list_flag(), C.prev.p, C.proposed.p,
+                           0.0, step_dev_.p, 0, nb_->pos_build(), 0.0, 0, nb_->list_flag(), C.prev.p, C.proposed.p,
                            status_.p);
     };
     if (mixed_) go(force_fx_.p);
@@ -1670,7 +1678,7 @@ void MdContext::capture_graph() {
         // last step (the phase is aligned before the graph is replayed).
         const int len = graph_length();
         for (int k = 0; k < len; ++k)
-            enqueue_step(s_, NbOperator::REBUILD_FLAG, baro_.enabled && k == len - 1);
+            enqueue_step(s_, NbOperator::REBUILD_FLAG, baro_.enabled && k == len - 1, k + 1, k == len - 1 ? len : 0);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s_, &g);
@@ -1868,11 +1876,11 @@ void MdContext::profile(int reps, double ms[8], int64_t* num_pairs) {
         if (mixed_)
             hipLaunchKernelGGL(k_lfmiddle<long long>, dim3((n_ + 255) / 256), dim3(256), 0, s_, n_, px.p, vx.p,
                                force_fx_.p, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
+                               step_dev_.p, 1, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
         else
             hipLaunchKernelGGL(k_lfmiddle<double>, dim3((n_ + 255) / 256), dim3(256), 0, s_, n_, px.p, vx.p,
                                force_d_.p, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
+                               step_dev_.p, 1, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
     });
     OMM_HIP(hipStreamSynchronize(s_));
     OMM_HIP(hipEventDestroy(t0));
