@@ -415,8 +415,13 @@ template <class R, class G, class A, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk_off, int nsys,
                                                      const SysInfo* __restrict__ sys, G* __restrict__ qgrid,
                                                      Cx<R>* cgrid, R* phi, const Cx<R>* __restrict__ roots_all,
-                                                     const R* __restrict__ moduli_all, A* energy, unsigned* status) {
+                                                     const R* __restrict__ moduli_all, A* energy, unsigned* status,
+                                                     int* clear, int nclear) {
     extern __shared__ unsigned char smem_raw[];
+    // The first pass also clears the spread's bin counts (every spread
+    // workgroup is done by now), so no fill kernel opens the next spread.
+    if (MODE == MODE_Z_R2C && clear)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nclear; i += gridDim.x * blockDim.x) clear[i] = 0;
     const bool zmode = MODE == MODE_Z_R2C || MODE == MODE_Z_C2R;
     const bool ymode = MODE == MODE_Y_FWD || MODE == MODE_Y_INV;
     const int axis = zmode ? 2 : (ymode ? 1 : 0);
@@ -634,14 +639,14 @@ int max_lines(int n, size_t csize) {
 
 template <class R, class G, class A, int MODE>
 void launch_fft(const PmeDev& pm, const SysInfo* sys, G* qgrid, Cx<R>* cgrid, R* phi, const R* roots, const R* moduli,
-                A* energy, unsigned* status, hipStream_t s) {
+                A* energy, unsigned* status, hipStream_t s, int* clear = nullptr, int nclear = 0) {
     const int prec = sizeof(R) == 8 ? 1 : 0;
     const bool zmode = MODE == MODE_Z_R2C || MODE == MODE_Z_C2R;
     const bool ymode = MODE == MODE_Y_FWD || MODE == MODE_Y_INV;
     const int axis = zmode ? 2 : (ymode ? 1 : 0);
     hipLaunchKernelGGL((k_fft<R, G, A, MODE>), dim3(pm.fft_blocks[prec][MODE]), dim3(FFT_THREADS),
                        pm.fft_shmem[prec][axis], s, pm.fft_off[prec][MODE].p, pm.nsys, sys, qgrid, cgrid, phi,
-                       reinterpret_cast<const Cx<R>*>(roots), moduli, energy, status);
+                       reinterpret_cast<const Cx<R>*>(roots), moduli, energy, status, clear, nclear);
 }
 
 template <class R>
@@ -659,19 +664,23 @@ template <class R, class G, class A>
 void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* charge, const SysInfo* sys, G* qgrid,
            R* cgrid, R* phi, const R* roots, const R* moduli, A* force, A* energy, unsigned* status, int phases,
            hipStream_t s) {
+    // Bin counts are zero between runs: set so at allocation, and cleared by
+    // the first FFT pass after each spread (or by a fill when a run spreads
+    // without transforming).
     if (phases & PME_SPREAD) {
         R* st_w = pme_weights<R>(pm);
-        fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
         hipLaunchKernelGGL((k_pme_bin<R>), dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.nslots,
                            nl.sorted_atom.p, pos, charge, pm.atom_sys, sys, pm.st_idx.p, st_w, pm.bin_count.p,
                            pm.bins.p, pm.bin_cap, status);
         hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
                            pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap, qgrid,
                            pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0, status);
+        if (!(phases & PME_FFT)) fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
     }
     if (phases & PME_FFT) {
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);
-        launch_fft<R, G, A, MODE_Z_R2C>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        launch_fft<R, G, A, MODE_Z_R2C>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s,
+                                        (phases & PME_SPREAD) ? pm.bin_count.p : nullptr, pm.ntiles);
         launch_fft<R, G, A, MODE_Y_FWD>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
         launch_fft<R, G, A, MODE_X_CONV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
         launch_fft<R, G, A, MODE_Y_INV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
@@ -791,6 +800,7 @@ int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStre
     pm.ntiles = toff.back();
     pm.tile_off.upload(toff, s);
     pm.bin_count.alloc((size_t)pm.ntiles);
+    fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
     pm_set_bin_cap(pm, bin_cap);
     pm.spread_edge = edge;
     return edge;
