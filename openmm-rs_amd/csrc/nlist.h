@@ -26,7 +26,7 @@ constexpr int IBLOCK = 32;  // atoms per block (the unit of the coarse list)
 constexpr int CLUSTER = 8;  // i-atoms per cluster (the unit of the pair kernel's lists)
 constexpr int CPB = IBLOCK / CLUSTER;  // clusters per block
 constexpr int SUPER = 64;   // blocks per superblock (the coarse level of the candidate search)
-constexpr int SUPER_MIN = 256;  // systems with more blocks search superblocks first
+constexpr int SUPER_MIN = 1024;  // systems with more blocks search superblocks first
 constexpr int JTILE = 64;   // j-slots per tile
 constexpr int TPT = 4;      // tiles of one cluster per pair-kernel task
 constexpr int MAXEX = 8;    // exclusion partners per atom handled by the pair kernel

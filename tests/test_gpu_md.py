@@ -148,6 +148,31 @@ def test_larger_box_multi_chunk_paths():
     assert np.isfinite(sa["potential_energy"])
 
 
+def test_superblock_search_paths():
+    # 12,000 waters in 7.1 nm: 1,125 list blocks, above the 1,024 from which
+    # clusters search 64-block superblocks first.  Forces against the oracle,
+    # graph steps against eager steps, and a batch that adds a small replica
+    # (which then takes the superblock path too) against lone contexts.
+    s = water(12000, 7.1, seed=41)
+    small = water(300, 2.1, seed=42)
+    a = ommhip.Context(s, temperature=300.0, friction=1.0, seed=8)
+    st = a.get_state()
+    F_ref, E_ref = orc.water_energy_forces(s)
+    assert force_rms_rel(st["forces"], F_ref) <= 2e-6
+    assert energy_rel(st["potential_energy"], E_ref) <= 1e-6
+    batch = ommhip.Batch([s, small], temperature=300.0, friction=1.0, seeds=[8, 9])
+    lone_small = ommhip.Context(small, temperature=300.0, friction=1.0, seed=9)
+    assert np.array_equal(batch.get_state(0)["forces"], st["forces"])
+    assert np.array_equal(batch.get_state(1)["forces"], lone_small.get_state()["forces"])
+    b = ommhip.Context(s, temperature=300.0, friction=1.0, seed=8)
+    a.set_velocities_to_temperature(300.0, 12)
+    b.set_velocities_to_temperature(300.0, 12)
+    a.step(20)
+    for _ in range(20):
+        b.step(1)
+    assert np.array_equal(a.get_state()["positions"], b.get_state()["positions"])
+
+
 @pytest.mark.parametrize("graph_steps", [10, 0])
 def test_fast_atom_keeps_the_list_exact(graph_steps):
     # Atoms launched at ~48 nm/ps (0.05 nm per step, half the list skin)
