@@ -71,7 +71,25 @@ def timed_steps(ctx, steps):
     return time.perf_counter() - t0
 
 
-GRAPH_STEPS = 50  # steps per captured graph (results do not depend on it); 0 with --eager
+GRAPH_STEPS = 50  # longest captured graph (results do not depend on it); 0 with --eager
+
+
+def graph_steps_for(steps):
+    """Steps per captured graph for a timed run of `steps`: the largest divisor
+    of `steps` not above GRAPH_STEPS, so every timed step is a graph replay."""
+    if GRAPH_STEPS <= 0 or steps <= 0:
+        return 0
+    return max(d for d in range(1, min(GRAPH_STEPS, steps) + 1) if steps % d == 0)
+
+
+def prime_graph(ctx, steps):
+    """Set the graph length for `steps` timed steps and run one untimed graph,
+    so capture and instantiation happen before the timed region."""
+    g = graph_steps_for(steps)
+    ctx.set_graph_steps(g)
+    if g:
+        ctx.step(g)
+    return g
 
 
 def run_single(steps, warmup, equilibrate, plain_steps, seed=2024):
@@ -82,7 +100,9 @@ def run_single(steps, warmup, equilibrate, plain_steps, seed=2024):
                          precision=ommhip.PRECISION_MIXED)
     ctx.set_graph_steps(GRAPH_STEPS)
     ctx.set_velocities_to_temperature(300.0, seed + 1)
-    equilibrate_then(ctx, equilibrate, warmup)
+    equilibrate_then(ctx, equilibrate, 0)
+    g = prime_graph(ctx, steps)
+    ctx.step(warmup)
     secs = timed_steps(ctx, steps)
     st = ctx.get_state()
     plain = ommhip.Context(s, temperature=300.0, friction=1.0, step_size=DT_PS, seed=seed,
@@ -92,7 +112,7 @@ def run_single(steps, warmup, equilibrate, plain_steps, seed=2024):
     plain.step(20)
     psecs = timed_steps(plain, plain_steps)
     plain.close()
-    return ctx, secs, psecs
+    return ctx, secs, psecs, g
 
 
 def batch_workload(count=64, seed=4096, lo=500, hi=6000, density=33.4):
@@ -112,6 +132,7 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
     batch.set_graph_steps(GRAPH_STEPS)
     batch.set_velocities_to_temperature(300.0, seed + 1)
     equilibrate_then(batch, equilibrate, warmup)
+    g = prime_graph(batch, steps)
     secs = timed_steps(batch, steps)
     phases, npairs = batch.profile(10)
     plain_secs = 0.0
@@ -135,6 +156,7 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
         "replica_steps_per_s": round(value, 2),
         "ms_per_batch_step": round(secs / steps * 1e3, 4),
         "steps": steps,
+        "graph_steps": g,
         "pairs_per_step": npairs,
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "launch_tuning": batch.launch_tuning(),
@@ -308,7 +330,7 @@ def main():
         rec = run_batch(args.batch_steps, 100, args.batch_equilibrate, args.batch_plain_steps)
         print(json.dumps(rec), flush=True)
         return
-    ctx, secs, psecs = run_single(args.steps, args.warmup, args.equilibrate, args.plain_steps)
+    ctx, secs, psecs, g = run_single(args.steps, args.warmup, args.equilibrate, args.plain_steps)
     value = ns_per_day(args.steps, secs)
     phases, npairs = ctx.profile(args.profile_reps)
     tiles, entries = ctx.list_stats()
@@ -325,6 +347,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(secs / args.steps * 1e3, 5),
+        "graph_steps": g,
+        "graph_replays_timed": args.steps // g if g else 0,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

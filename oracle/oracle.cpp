@@ -823,6 +823,10 @@ double orc_ewald_reciprocal(const double* pos, const double* charges, int32_t n,
     return energy;
 }
 
+void orc_dispersion_reciprocal_pair(double c6, double r2, double alpha, double* energy, double* force_over_r) {
+    dispersion_reciprocal_pair(c6, r2, alpha, *energy, *force_over_r);
+}
+
 double orc_dispersion_correction(const omm_nb_desc* desc, double volume) {
     Desc d(desc);
     return dispersion_coefficient(d) / volume;

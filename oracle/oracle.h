@@ -90,6 +90,11 @@ double orc_ewald_reciprocal(const double* pos, const double* charges, int32_t n,
                             double* forces);
 /* dispersion_correction (nonbonded.rs:660-719) energy for this volume. */
 double orc_dispersion_correction(const omm_nb_desc* desc, double volume);
+/* dispersion_reciprocal_pair (nonbonded.rs:1158-1191): the positive
+ * complement C6 (1 - e^-x (1 + x + x^2/2)) / r^6, x = alpha^2 r^2, and its
+ * dE/dr / r. */
+void orc_dispersion_reciprocal_pair(double c6, double r2, double alpha, double* energy,
+                                    double* force_over_r);
 
 /* apply_minimum_image_ortho (harmonic_bond.rs:152-159). */
 void orc_minimum_image(double delta[3], const double box[9]);

@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# OMM_ORACLE_LIB names another build of oracle.cpp: the mutation tests load
+# deliberately broken builds through it to show that the pins catch them.
+LIB_PATH = os.environ.get("OMM_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 _lib = None
@@ -56,6 +58,7 @@ def lib():
         L.orc_ewald_reciprocal.argtypes = [_dp, _dp, C.c_int32, _dp, C.c_double, _ip, _dp]
         L.orc_dispersion_correction.restype = C.c_double
         L.orc_dispersion_correction.argtypes = [C.c_void_p, C.c_double]
+        L.orc_dispersion_reciprocal_pair.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp]
         L.orc_minimum_image.argtypes = [_dp, _dp]
         L.orc_harmonic_bonds.restype = C.c_double
         L.orc_harmonic_bonds.argtypes = [C.c_int32, _ip, _ip, _dp, _dp, _dp, _dp, C.c_int32, _dp]
@@ -212,6 +215,13 @@ def ewald_reciprocal(pos, charges, widths, alpha, limits):
 
 def dispersion_correction(desc, volume):
     return lib().orc_dispersion_correction(C.byref(desc), volume)
+
+
+def dispersion_reciprocal_pair(c6, r2, alpha):
+    """(energy, dE/dr / r) of the dispersion complement (nonbonded.rs:1158-1191)."""
+    e, f = C.c_double(), C.c_double()
+    lib().orc_dispersion_reciprocal_pair(float(c6), float(r2), float(alpha), C.byref(e), C.byref(f))
+    return e.value, f.value
 
 
 def minimum_image(delta, box):
