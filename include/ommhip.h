@@ -211,6 +211,37 @@ omm_status omm_nb_update_parameters(omm_nb* nb, const omm_nb_desc* desc);
 omm_status omm_nb_destroy(omm_nb* nb);
 
 /* ------------------------------------------------------------------------ */
+/* The charge mesh as a second consumer of the device PME: the crate-internal
+ * PmeCache methods that ConstantPotentialForce calls (pme.rs:63-138; called at
+ * constant_potential.rs:642-644 for its energy and forces, and by its charge
+ * solver, constant_potential/solver.rs:24-31 and 219-226, for dE/dq).  Each
+ * call brings positions (n x 3), charges (n), box vectors (9, reduced form),
+ * alpha and the grid (each >= 5), as the reference's calls do; a workspace is
+ * set up for each (n, grid) and the two most recent are kept.  The mesh is
+ * plain FP64 (FP64 grid, transforms and sums): its users solve linear
+ * systems with it.  Uncharged particles do not spread (pme.rs:318). */
+typedef struct omm_pme omm_pme;
+
+omm_status omm_pme_create(omm_pme** out);
+
+/* PmeCache::evaluate_charges (pme.rs:63-86): the reciprocal mesh energy of
+ * these charges (*energy) and its forces, ADDED to forces (n x 3, nullable)
+ * as the reference adds into its force buffer. */
+omm_status omm_pme_evaluate_charges(omm_pme* pme, int32_t num_particles, const double* pos, const double* charges,
+                                    const double* box, double alpha, const int32_t grid[3], double* forces,
+                                    double* energy);
+
+/* PmeCache::charge_derivatives (pme.rs:88-138): for each of the count
+ * particle indices, the mesh potential at that particle's stencil (the
+ * derivative of the mesh energy with respect to its charge) is ADDED to
+ * derivatives[k]. */
+omm_status omm_pme_charge_derivatives(omm_pme* pme, int32_t num_particles, const double* pos, const double* charges,
+                                      const double* box, double alpha, const int32_t grid[3],
+                                      const int32_t* indices, int32_t count, double* derivatives);
+
+omm_status omm_pme_destroy(omm_pme* pme);
+
+/* ------------------------------------------------------------------------ */
 /* MD context: the step around the operator for the flexible-water workloads.
  * Stands in for Context::step (context.rs:795-928) with a LangevinMiddle
  * integrator (integrators/langevin.rs:103-142), HarmonicBondForce
@@ -378,6 +409,19 @@ omm_status omm_ctx_barostat_attempt(omm_ctx* ctx);
  * Pair-slot efficiency = pairs / (tiles * 512). */
 omm_status omm_ctx_list_stats(omm_ctx* ctx, int64_t* tiles, int64_t* entries);
 
+/* Storage test hooks (no counterpart in the reference, whose cell list
+ * grows on the host): omm_ctx_set_capacity reallocates the neighbour-list
+ * storage to list_cap entries per cluster (rounded up to whole 64-entry tiles)
+ * and rebuilds the lists from the positions they were built from, and the PME
+ * spread bins to bin_cap atoms per tile; 0 keeps a capacity.  A capacity
+ * below the need truncates without writing out of bounds and raises the
+ * overflow status; the next omm_ctx_step then restores the state it started
+ * from, grows the storage and replays, ending on the trajectory of a run that
+ * never overflowed.  omm_ctx_capacity reports the capacities and how many
+ * runs of steps were replayed so. */
+omm_status omm_ctx_set_capacity(omm_ctx* ctx, int32_t list_cap, int32_t bin_cap);
+omm_status omm_ctx_capacity(omm_ctx* ctx, int32_t* list_cap, int32_t* bin_cap, int64_t* overflow_replays);
+
 /* Launch shapes picked by the autotuner when the context is first prepared
  * (no counterpart in the reference): the spread tile edge (grid points), the
  * lines per FFT workgroup and the pair kernel's grid, each the fastest of its
@@ -396,6 +440,25 @@ omm_status omm_ctx_launch_tuning(omm_ctx* ctx, omm_launch_tuning* out);
  * and potential energy; any output pointer may be NULL. */
 omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* forces,
                              double* potential_energy, int64_t* step_count);
+
+/* Context::set_state (context.rs:274-293): step count, box vectors (9,
+ * NULL: kept), positions and velocities (N x 3, NULL: kept).  Global
+ * parameters are restored with omm_ctx_set_parameter, as the reference's
+ * set_state loops over set_parameter.  The step count keys the integrator's
+ * noise, so a restored context draws what the saved one would have drawn.
+ * list_positions (N x 3, NULL: pos) are the positions the saved context last
+ * built its neighbour list from (omm_ctx_get_list_positions); the list is
+ * rebuilt from them, which reproduces the saved list, so the restored context
+ * continues the saved trajectory bit for bit.  Without them the list is built
+ * from pos: the same physics, with FP32 pair sums taken around other block
+ * centres.  Barostat bookkeeping stays with the barostat, as in the reference. */
+omm_status omm_ctx_set_state(omm_ctx* ctx, int64_t step_count, const double* box, const double* pos,
+                             const double* vel, const double* list_positions);
+
+/* Positions the current neighbour list was built from (N x 3): the list part
+ * of a checkpoint, for omm_ctx_set_state (no counterpart in the reference,
+ * whose cell list is rebuilt on every evaluation). */
+omm_status omm_ctx_get_list_positions(omm_ctx* ctx, double* list_positions);
 
 /* Kinetic energy (context.rs:1107-1163), kJ/mol, of the current velocities. */
 omm_status omm_ctx_kinetic_energy(omm_ctx* ctx, double* kinetic_energy);
@@ -482,6 +545,13 @@ omm_status omm_batch_compute_stress_tensor(omm_batch* b, int32_t include_kinetic
 /* Context::get_state of replica r; any output pointer may be NULL. */
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
                                double* potential_energy, int64_t* step_count);
+
+/* Context::set_state of replica r (as omm_ctx_set_state).  The step count is
+ * shared by the replicas of a batch (it keys every replica's noise stream):
+ * the last value set holds for all of them. */
+omm_status omm_batch_set_state(omm_batch* b, int32_t replica, int64_t step_count, const double* box,
+                               const double* pos, const double* vel, const double* list_positions);
+omm_status omm_batch_get_list_positions(omm_batch* b, int32_t replica, double* list_positions);
 
 /* Potential energy of every replica (count values), one device evaluation. */
 omm_status omm_batch_energies(omm_batch* b, double* potential_energies);

@@ -1,7 +1,10 @@
 // extern "C" entry points of include/ommhip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <array>
 #include <cstring>
+#include <memory>
 #include <mutex>
 
 #include "../../include/ommhip.h"
@@ -30,6 +33,53 @@ struct omm_nb {
 struct omm_ctx {
     ommhip::MdContext ctx;
     omm_ctx(const omm_md_desc* d, int precision) : ctx(d, 1, precision) {}
+};
+
+// The charge mesh of PmeCache (pme.rs:8-138): plain-FP64 PME operators keyed
+// by particle count and grid, most recently used first.  Two are kept: the
+// constant-potential solver alternates between its system and a single unit
+// charge (its preconditioner, constant_potential/solver.rs:219-226).
+struct omm_pme {
+    struct Mesh {
+        int n;
+        std::array<int, 3> grid;
+        std::unique_ptr<ommhip::NbOperator> op;
+    };
+    std::vector<Mesh> meshes;
+    std::mutex mutex;
+
+    ommhip::NbOperator& mesh(int n, const int32_t* grid, double alpha, const double* box) {
+        if (n <= 0) throw Error(OMM_ERR_INVALID, "the charge mesh needs particles");
+        if (!grid || !box) throw Error(OMM_ERR_INVALID, "null grid or box");
+        const std::array<int, 3> g = {grid[0], grid[1], grid[2]};
+        for (size_t k = 0; k < meshes.size(); ++k)
+            if (meshes[k].n == n && meshes[k].grid == g) {
+                std::rotate(meshes.begin(), meshes.begin() + k, meshes.begin() + k + 1);
+                return *meshes[0].op;
+            }
+        // A reciprocal-only PME operator with the explicit (alpha, grid); its
+        // particles are placeholders, every call brings the charges.  The
+        // cutoff only has to fit the box.
+        std::vector<double> zero(n, 0.0), one(n, 1.0);
+        omm_nb_desc d;
+        omm_nb_desc_defaults(&d);
+        d.num_particles = n;
+        d.charge = zero.data();
+        d.sigma = one.data();
+        d.epsilon = zero.data();
+        d.method = OMM_NB_PME;
+        d.cutoff = std::min(0.3, 0.5 * std::min(box[0], std::min(box[4], box[8])));
+        d.pme_alpha = alpha;
+        for (int a = 0; a < 3; ++a) d.pme_grid[a] = grid[a];
+        d.use_dispersion_correction = 0;
+        d.include_direct_space = 0;
+        const omm_nb_desc* dp = &d;
+        std::unique_ptr<ommhip::NbOperator> op(new ommhip::NbOperator(&dp, 1, OMM_PRECISION_DOUBLE));
+        op->initialize(n, box);
+        meshes.insert(meshes.begin(), Mesh{n, g, std::move(op)});
+        if (meshes.size() > 2) meshes.pop_back();
+        return *meshes[0].op;
+    }
 };
 
 struct omm_batch {
@@ -351,6 +401,25 @@ omm_status omm_ctx_get_state(omm_ctx* ctx, double* pos, double* vel, double* for
     CTX_CALL(ctx->ctx.get_state(0, pos, vel, forces, potential_energy, step_count));
 }
 
+omm_status omm_ctx_set_state(omm_ctx* ctx, int64_t step_count, const double* box, const double* pos,
+                             const double* vel, const double* list_positions) {
+    CTX_CALL(ctx->ctx.set_state(0, step_count, box, pos, vel, list_positions));
+}
+
+omm_status omm_ctx_set_capacity(omm_ctx* ctx, int32_t list_cap, int32_t bin_cap) {
+    CTX_CALL(ctx->ctx.set_capacity(list_cap, bin_cap));
+}
+
+omm_status omm_ctx_capacity(omm_ctx* ctx, int32_t* list_cap, int32_t* bin_cap, int64_t* overflow_replays) {
+    CTX_CALL(if (!list_cap || !bin_cap || !overflow_replays) throw Error(OMM_ERR_INVALID, "null output");
+             ctx->ctx.capacity(list_cap, bin_cap, overflow_replays));
+}
+
+omm_status omm_ctx_get_list_positions(omm_ctx* ctx, double* list_positions) {
+    CTX_CALL(if (!list_positions) throw Error(OMM_ERR_INVALID, "null output");
+             ctx->ctx.list_positions(0, list_positions));
+}
+
 omm_status omm_ctx_kinetic_energy(omm_ctx* ctx, double* kinetic_energy) {
     CTX_CALL(if (!kinetic_energy) throw Error(OMM_ERR_INVALID, "null output"); *kinetic_energy = ctx->ctx.kinetic_energy(0));
 }
@@ -362,6 +431,48 @@ omm_status omm_ctx_profile(omm_ctx* ctx, int32_t reps, double phase_ms[8], int64
 omm_status omm_ctx_destroy(omm_ctx* ctx) {
     return guarded([&]() -> omm_status {
         delete ctx;
+        return OMM_OK;
+    });
+}
+
+omm_status omm_pme_create(omm_pme** out) {
+    return guarded([&]() -> omm_status {
+        if (!out) throw Error(OMM_ERR_INVALID, "null output handle");
+        *out = new omm_pme();
+        return OMM_OK;
+    });
+}
+
+omm_status omm_pme_evaluate_charges(omm_pme* pme, int32_t num_particles, const double* pos, const double* charges,
+                                    const double* box, double alpha, const int32_t grid[3], double* forces,
+                                    double* energy) {
+    return guarded([&]() -> omm_status {
+        if (!pme) throw Error(OMM_ERR_INVALID, "null handle");
+        if (!energy) throw Error(OMM_ERR_INVALID, "null energy output");
+        std::lock_guard<std::mutex> lock(pme->mutex);
+        ommhip::NbOperator& op = pme->mesh(num_particles, grid, alpha, box);
+        *energy = op.mesh_charges(pos, box, alpha, charges, forces, nullptr, 0, nullptr);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_pme_charge_derivatives(omm_pme* pme, int32_t num_particles, const double* pos, const double* charges,
+                                      const double* box, double alpha, const int32_t grid[3],
+                                      const int32_t* indices, int32_t count, double* derivatives) {
+    return guarded([&]() -> omm_status {
+        if (!pme) throw Error(OMM_ERR_INVALID, "null handle");
+        if (count < 0) throw Error(OMM_ERR_INVALID, "negative index count");
+        if (count == 0) return OMM_OK;
+        std::lock_guard<std::mutex> lock(pme->mutex);
+        ommhip::NbOperator& op = pme->mesh(num_particles, grid, alpha, box);
+        op.mesh_charges(pos, box, alpha, charges, nullptr, indices, count, derivatives);
+        return OMM_OK;
+    });
+}
+
+omm_status omm_pme_destroy(omm_pme* pme) {
+    return guarded([&]() -> omm_status {
+        delete pme;
         return OMM_OK;
     });
 }
@@ -460,6 +571,16 @@ omm_status omm_batch_compute_stress_tensor(omm_batch* b, int32_t include_kinetic
 omm_status omm_batch_get_state(omm_batch* b, int32_t replica, double* pos, double* vel, double* forces,
                                double* potential_energy, int64_t* step_count) {
     BATCH_CALL(b->ctx.get_state(replica, pos, vel, forces, potential_energy, step_count));
+}
+
+omm_status omm_batch_set_state(omm_batch* b, int32_t replica, int64_t step_count, const double* box,
+                               const double* pos, const double* vel, const double* list_positions) {
+    BATCH_CALL(b->ctx.set_state(replica, step_count, box, pos, vel, list_positions));
+}
+
+omm_status omm_batch_get_list_positions(omm_batch* b, int32_t replica, double* list_positions) {
+    BATCH_CALL(if (!list_positions) throw Error(OMM_ERR_INVALID, "null output");
+               b->ctx.list_positions(replica, list_positions));
 }
 
 omm_status omm_batch_energies(omm_batch* b, double* potential_energies) {

@@ -132,6 +132,24 @@ public:
     void barostat_attempt();
     BaroState barostat_state(int r);
     void get_state(int r, double* pos, double* vel, double* forces, double* energy, int64_t* step);
+    // Context::set_state (context.rs:274-293) for replica r: the step count
+    // (shared by the replicas of a batch: it keys every replica's noise), box
+    // vectors, positions and velocities, each optional (null: kept).  The
+    // neighbour list of replica r is rebuilt from list_pos when given (the
+    // positions the saved context last built its list from, list_positions()),
+    // else from pos, so a context restored with its list positions continues
+    // the saved trajectory bit for bit.
+    void set_state(int r, int64_t step, const double* box, const double* pos, const double* vel,
+                   const double* list_pos);
+    // Positions replica r's current neighbour list was built from.
+    void list_positions(int r, double* out);
+    // Test hook of the overflow recovery: list and spread-bin capacities (0:
+    // unchanged), see NbOperator::set_capacity; the lists are rebuilt from the
+    // positions they were built from.
+    void set_capacity(int list_cap, int bin_cap);
+    // Current capacities and the number of runs of steps replayed after an
+    // overflow since the context was created.
+    void capacity(int* list_cap, int* bin_cap, int64_t* replays);
     // Potential energy of every replica (nsys values).
     void energies(double* out);
     double kinetic_energy(int r);
@@ -179,7 +197,13 @@ private:
     void kinetic_tensor(std::vector<double>& out);
     void setup_constraints(const omm_md_desc* descs, int nsys, const std::vector<double>& inv);
     void launch_bonded(hipStream_t s, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d);
-    int graph_length() const { return baro_.enabled ? baro_.frequency : graph_steps_; }
+    // A barostat period of at most MAX_PERIOD_GRAPH steps is captured whole,
+    // its attempt opening the last step; longer periods replay attempt-free
+    // graphs of graph_steps_ steps and launch the attempt steps eagerly, so a
+    // graph never holds more than max(graph_steps_, MAX_PERIOD_GRAPH) steps.
+    static constexpr int MAX_PERIOD_GRAPH = 1000;
+    bool period_graph() const { return baro_.enabled && baro_.frequency <= MAX_PERIOD_GRAPH; }
+    int graph_length() const { return period_graph() ? baro_.frequency : graph_steps_; }
     void build_bonded(const std::vector<int>& e1, const std::vector<int>& e2, const std::vector<double>& eqq,
                       const std::vector<double>& esig, const std::vector<double>& eeps, const std::vector<int>& b1,
                       const std::vector<int>& b2, const std::vector<double>& blen, const std::vector<double>& bk,
@@ -192,7 +216,9 @@ private:
     void evaluate(double* forces, double* energy, bool want_energy);
     void check_status();
     unsigned read_status();
+    void enqueue_snapshot(hipStream_t s);
     void restore_snapshot();
+    void run_steps(int count, bool graph);
     void drop_graph();
     // Sizes the neighbour list for the current positions once they are all set.
     void prepare();
@@ -208,6 +234,11 @@ private:
     int64_t step_count_ = 0;
     int graph_steps_ = 10;
     DevBuf<double> pos_, vel_, inv_mass_, out_;
+    // Positions each system's neighbour list is built from at the next
+    // preparation, and which systems were set since the last one (the others
+    // rebuild from the positions of their current list, which reproduces it).
+    DevBuf<double> list_src_;
+    std::vector<char> touched_;
     BondedDev bonded_;
     ConstraintDev cons_;
     std::vector<int> bonded_exc_order_;  // exception k of the bonded arrays = global exception order[k]
@@ -219,7 +250,8 @@ private:
     DevBuf<double> force_d_, energy_d_;
     DevBuf<unsigned> status_;
     DevBuf<unsigned long long> step_dev_, snap_step_;
-    DevBuf<double> snap_pos_, snap_vel_;
+    DevBuf<double> snap_pos_, snap_vel_, snap_list_;
+    int64_t overflow_replays_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
 };

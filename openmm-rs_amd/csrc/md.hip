@@ -145,13 +145,18 @@ __global__ void k_step_advance(unsigned long long* step, int count, A* energy, i
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) energy[i] = (A)0;
 }
 
-__global__ void k_copy2(const double* __restrict__ a, double* __restrict__ a_out, const double* __restrict__ b,
-                        double* __restrict__ b_out, size_t n, const unsigned long long* __restrict__ step,
-                        unsigned long long* step_out) {
+// Snapshot of the state a run of steps starts from: positions, velocities,
+// the positions the neighbour list was built from (nullable when there is no
+// list) and the step counter.
+__global__ void k_snapshot(const double* __restrict__ pos, double* __restrict__ pos_out,
+                           const double* __restrict__ vel, double* __restrict__ vel_out,
+                           const double* __restrict__ built, double* __restrict__ built_out, size_t n,
+                           const unsigned long long* __restrict__ step, unsigned long long* step_out) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) {
-        a_out[i] = a[i];
-        b_out[i] = b[i];
+        pos_out[i] = pos[i];
+        vel_out[i] = vel[i];
+        if (built) built_out[i] = built[i];
     }
     if (i == 0) step_out[0] = step[0];
 }
@@ -210,7 +215,7 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
             step_normals(i, sys[si], step[0] + (unsigned long long)step_offset, xi);
             double noise = sqrt(variance * im);
             double half = 0.5 * dt;
-            double d2 = 0.0;
+            double d[3];
             for (int a = 0; a < 3; ++a) {
                 int k = 3 * i + a;
                 double v = vel[k] + f[a] * im * dt;
@@ -219,10 +224,9 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
                 vel[k] = v;
                 double x = mid + half * v;
                 pos[k] = x;
-                double d = x - pos_build[k];
-                d2 += d * d;
+                d[a] = x - pos_build[k];
             }
-            moved = check && !(d2 <= lim2);
+            moved = check && !(skin_d2(d[0], d[1], d[2]) <= lim2);
         }
     }
     wave_raise_flags(flag, moved, si);
@@ -339,14 +343,13 @@ __global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, 
             ok = ok && project_component(k0, k1, c1, c2, d2, w1, w2, prev, pos, false, tolerance);
             for (int m = a0; m < a1; ++m) {
                 const int i = atoms[m];
-                double dd = 0.0;
+                double d[3];
                 for (int a = 0; a < 3; ++a) {
                     const int k = 3 * i + a;
                     vel[k] += (pos[k] - proposed[k]) / dt;
-                    const double d = pos[k] - pos_build[k];
-                    dd += d * d;
+                    d[a] = pos[k] - pos_build[k];
                 }
-                moved = moved || (check && !(dd <= lim2));
+                moved = moved || (check && !(skin_d2(d[0], d[1], d[2]) <= lim2));
             }
         }
         if (!ok) atomicOr(status, ST_CONSTRAINTS);
@@ -792,6 +795,9 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     friction_ = descs[0].friction;
     pos_.alloc(3 * (size_t)n_);
     vel_.alloc(3 * (size_t)n_);
+    list_src_.alloc(3 * (size_t)n_);
+    touched_.assign(nsys, 1);
+    fill_async<double>(list_src_.p, 3 * (size_t)n_, 0.0, s_);
     fill_async<double>(pos_.p, 3 * (size_t)n_, 0.0, s_);
     fill_async<double>(vel_.p, 3 * (size_t)n_, 0.0, s_);
     const size_t eslots = (size_t)E_SLOTS * nsys;
@@ -809,6 +815,7 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     fill_async<unsigned long long>(step_dev_.p, 1, 0ull, s_);
     snap_pos_.alloc(3 * (size_t)n_);
     snap_vel_.alloc(3 * (size_t)n_);
+    snap_list_.alloc(3 * (size_t)n_);
     snap_step_.alloc(1);
     snap_sys_.alloc(nsys);
     clear_accumulators(s_);
@@ -825,18 +832,87 @@ void MdContext::set_positions(int r, const double* pos) {
     const int n = nb_->num_particles(r);
     for (int i = 0; i < 3 * n; ++i)
         if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
-    OMM_HIP(hipMemcpyAsync(pos_.p + 3 * (size_t)nb_->atom_offset(r), pos, 3 * (size_t)n * sizeof(double),
-                           hipMemcpyHostToDevice, s_));
+    const size_t off3 = 3 * (size_t)nb_->atom_offset(r), bytes = 3 * (size_t)n * sizeof(double);
+    OMM_HIP(hipMemcpyAsync(pos_.p + off3, pos, bytes, hipMemcpyHostToDevice, s_));
+    OMM_HIP(hipMemcpyAsync(list_src_.p + off3, pos, bytes, hipMemcpyHostToDevice, s_));
     OMM_HIP(hipStreamSynchronize(s_));
+    touched_[r] = 1;
     positions_dirty_ = true;
 }
 
+// Every system's list is a function of the positions it was built from and
+// the box.  A system set since the last preparation is built from its source
+// positions (those set, or the list positions of a restored state); the
+// others are rebuilt from the positions of their current lists, which
+// reproduces those lists.  The rebuild requests are then those the
+// integrators would raise at the current positions.
 void MdContext::prepare() {
     if (!positions_dirty_) return;
-    if (nb_->calibrate(pos_.p, status_.p, s_)) drop_graph();
-    nb_->force_rebuild(s_);
+    const double* built = nb_->pos_build();
+    for (int r = 0; r < nsys_; ++r)
+        if (!touched_[r] && built) {
+            const size_t off3 = 3 * (size_t)nb_->atom_offset(r);
+            OMM_HIP(hipMemcpyAsync(list_src_.p + off3, built + off3, 3 * (size_t)nb_->num_particles(r) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s_));
+        }
+    if (nb_->calibrate(list_src_.p, status_.p, s_)) drop_graph();
+    nb_->recheck_list(pos_.p, s_);
     OMM_HIP(hipStreamSynchronize(s_));
+    std::fill(touched_.begin(), touched_.end(), 0);
     positions_dirty_ = false;
+}
+
+void MdContext::set_state(int r, int64_t step, const double* box, const double* pos, const double* vel,
+                          const double* list_pos) {
+    if (r < 0 || r >= nsys_) throw Error(OMM_ERR_INVALID, "replica index out of range");
+    if (step < 0) throw Error(OMM_ERR_INVALID, "step count must be nonnegative");
+    if (list_pos && !pos) throw Error(OMM_ERR_INVALID, "list positions are restored together with positions");
+    const int n = nb_->num_particles(r);
+    for (const double* a : {pos, vel, list_pos})
+        if (a)
+            for (int i = 0; i < 3 * n; ++i)
+                if (!std::isfinite(a[i])) throw Error(OMM_ERR_NONFINITE, "state arrays must be finite");
+    OMM_HIP(hipStreamSynchronize(s_));
+    if (box) nb_->set_box(r, box, s_);
+    const size_t off3 = 3 * (size_t)nb_->atom_offset(r), bytes = 3 * (size_t)n * sizeof(double);
+    if (pos) {
+        OMM_HIP(hipMemcpyAsync(pos_.p + off3, pos, bytes, hipMemcpyHostToDevice, s_));
+        OMM_HIP(hipMemcpyAsync(list_src_.p + off3, list_pos ? list_pos : pos, bytes, hipMemcpyHostToDevice, s_));
+    } else if (box) {
+        // A new box with the current positions: the list follows the box.
+        OMM_HIP(hipMemcpyAsync(list_src_.p + off3, pos_.p + off3, bytes, hipMemcpyDeviceToDevice, s_));
+    }
+    if (pos || box) {
+        touched_[r] = 1;
+        positions_dirty_ = true;
+    }
+    if (vel) OMM_HIP(hipMemcpyAsync(vel_.p + off3, vel, bytes, hipMemcpyHostToDevice, s_));
+    step_count_ = step;
+    fill_async<unsigned long long>(step_dev_.p, 1, (unsigned long long)step, s_);
+    OMM_HIP(hipStreamSynchronize(s_));
+}
+
+void MdContext::set_capacity(int list_cap, int bin_cap) {
+    prepare();
+    drop_graph();
+    const double* built = nb_->pos_build();
+    if (built && nb_->settings().include_direct)
+        OMM_HIP(hipMemcpyAsync(list_src_.p, built, 3 * (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+    nb_->set_capacity(list_cap, bin_cap, list_src_.p, status_.p, s_);
+}
+
+void MdContext::capacity(int* list_cap, int* bin_cap, int64_t* replays) {
+    nb_->capacity(list_cap, bin_cap);
+    *replays = overflow_replays_;
+}
+
+void MdContext::list_positions(int r, double* out) {
+    if (r < 0 || r >= nsys_) throw Error(OMM_ERR_INVALID, "replica index out of range");
+    prepare();
+    const size_t off3 = 3 * (size_t)nb_->atom_offset(r), bytes = 3 * (size_t)nb_->num_particles(r) * sizeof(double);
+    const double* built = nb_->settings().include_direct ? nb_->pos_build() : nullptr;
+    OMM_HIP(hipMemcpyAsync(out, (built ? built : pos_.p) + off3, bytes, hipMemcpyDeviceToHost, s_));
+    OMM_HIP(hipStreamSynchronize(s_));
 }
 
 void MdContext::drop_graph() {
@@ -1644,6 +1720,20 @@ void MdContext::check_status() {
                                      "iterations (langevin.rs:215-277)");
 }
 
+void MdContext::enqueue_snapshot(hipStream_t s) {
+    const size_t n3 = 3 * (size_t)n_;
+    const double* built = nb_->settings().include_direct ? nb_->pos_build() : nullptr;
+    hipLaunchKernelGGL(k_snapshot, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, pos_.p, snap_pos_.p, vel_.p,
+                       snap_vel_.p, built, snap_list_.p, n3, step_dev_.p, snap_step_.p);
+    OMM_HIP(hipGetLastError());
+    OMM_HIP(hipMemcpyAsync(snap_sys_.p, nb_->systems_dev(), nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice, s));
+    if (baro_.enabled)
+        OMM_HIP(hipMemcpyAsync(baro_.snap_st.p, baro_.st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToDevice, s));
+}
+
+// Puts back the snapshot of enqueue_snapshot and rebuilds every list from the
+// positions it was built from at that time, which reproduces it, so the steps
+// replayed from here follow the trajectory of a run that never overflowed.
 void MdContext::restore_snapshot() {
     const size_t bytes = 3 * (size_t)n_ * sizeof(double);
     OMM_HIP(hipMemcpyAsync(nb_->systems_dev(), snap_sys_.p, nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice, s_));
@@ -1654,31 +1744,28 @@ void MdContext::restore_snapshot() {
     OMM_HIP(hipMemcpyAsync(step_dev_.p, snap_step_.p, sizeof(unsigned long long), hipMemcpyDeviceToDevice, s_));
     clear_accumulators(s_);
     fill_async<unsigned>(status_.p, 1, 0u, s_);
-    nb_->force_rebuild(s_);
+    if (nb_->settings().include_direct) {
+        OMM_HIP(hipMemcpyAsync(list_src_.p, snap_list_.p, bytes, hipMemcpyDeviceToDevice, s_));
+        nb_->calibrate(list_src_.p, status_.p, s_);  // sizes the storage for the rebuilt lists
+        nb_->recheck_list(pos_.p, s_);
+    }
+    OMM_HIP(hipStreamSynchronize(s_));
 }
 
-// The captured graph holds graph_steps_ steps and a snapshot of the state
-// taken before them.  Each step rebuilds the list only when the previous
-// integrator flagged it (the build kernels exit at once otherwise).  A list
-// or PME-bin overflow inside the graph restores the snapshot, grows the
-// storage and replays the steps eagerly (identical random numbers: the RNG is
-// counter-based).
+// The captured graph holds a snapshot of the state and graph_length() steps.
+// Each step rebuilds the list only when the previous integrator flagged it
+// (the build kernels exit at once otherwise).  With a barostat whose period
+// fits a graph, the graph spans one period, the attempt opening its last step
+// (the phase is aligned before the graph is replayed); longer periods use
+// attempt-free graphs of graph_steps_ steps between eager attempt steps.
 void MdContext::capture_graph() {
     OMM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     try {
-        const size_t n3 = 3 * (size_t)n_;
-        hipLaunchKernelGGL(k_copy2, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s_, pos_.p, snap_pos_.p, vel_.p,
-                           snap_vel_.p, n3, step_dev_.p, snap_step_.p);
-        OMM_HIP(hipMemcpyAsync(snap_sys_.p, nb_->systems_dev(), nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice,
-                               s_));
-        if (baro_.enabled)
-            OMM_HIP(hipMemcpyAsync(baro_.snap_st.p, baro_.st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToDevice,
-                                   s_));
-        // With a barostat the graph spans one period, the attempt opening its
-        // last step (the phase is aligned before the graph is replayed).
+        enqueue_snapshot(s_);
         const int len = graph_length();
+        const bool attempt = period_graph();
         for (int k = 0; k < len; ++k)
-            enqueue_step(s_, NbOperator::REBUILD_FLAG, baro_.enabled && k == len - 1, k + 1, k == len - 1 ? len : 0);
+            enqueue_step(s_, NbOperator::REBUILD_FLAG, attempt && k == len - 1, k + 1, k == len - 1 ? len : 0);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s_, &g);
@@ -1709,42 +1796,69 @@ void MdContext::set_integrator(double temperature, double friction, double step_
     dt_ = step_size;
 }
 
+// Runs `count` steps, as one replay of the captured graph (graph: count is
+// the graph length) or as eager launches after a snapshot.  A list or PME-bin
+// overflow during them restores the snapshot, grows the storage, rebuilds the
+// lists the snapshot had and replays the steps eagerly: the trajectory is that
+// of a run that never overflowed (the noise is counter-based).
+void MdContext::run_steps(int count, bool graph) {
+    BaroDev& B = baro_;
+    const int phase0 = B.phase;
+    // The attempt opens the step that completes a period (barostat.rs:129-135).
+    auto eager = [&]() {
+        for (int k = 0; k < count; ++k) {
+            enqueue_step(s_, NbOperator::REBUILD_FLAG, B.enabled && B.phase == B.frequency - 1);
+            if (B.enabled) B.phase = (B.phase + 1) % B.frequency;
+        }
+    };
+    if (graph) {
+        if (!graph_exec_) capture_graph();
+        OMM_HIP(hipGraphLaunch(graph_exec_, s_));
+        if (B.enabled) B.phase = (B.phase + count) % B.frequency;
+    } else {
+        enqueue_snapshot(s_);
+        eager();
+    }
+    unsigned st = read_status();
+    for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
+        if (attempt == 8) check_status();  // throws
+        ++overflow_replays_;
+        drop_graph();
+        nb_->grow(st & ST_BIN_OVERFLOW);  // the lists are sized by the rebuild from the snapshot
+        restore_snapshot();
+        B.phase = phase0;
+        eager();
+        st = read_status();
+    }
+}
+
 void MdContext::step(int nsteps) {
     if (nsteps < 0) throw Error(OMM_ERR_INVALID, "step count must be nonnegative");
     prepare();
-    BaroDev& B = baro_;
-    // Attempt before the step that completes a period (barostat.rs:129-135).
-    auto due = [&]() { return B.enabled && B.phase == B.frequency - 1; };
-    auto advance = [&](int k) {
-        if (B.enabled) B.phase = (B.phase + k) % B.frequency;
-    };
+    const BaroDev& B = baro_;
     const int len = graph_length();
     const bool graphs = mixed_ && graph_steps_ > 0 && len > 0;
+    const int chunk = graph_steps_ > 0 ? graph_steps_ : 10;  // eager steps per snapshot
     int done = 0;
     while (done < nsteps) {
-        const bool aligned = !B.enabled || B.phase == 0;
-        if (graphs && aligned && nsteps - done >= len) {
-            if (!graph_exec_) capture_graph();
-            OMM_HIP(hipGraphLaunch(graph_exec_, s_));
-            unsigned st = read_status();
-            for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
-                if (attempt == 4) check_status();  // throws
-                // Restore the snapshot, grow the storage and replay eagerly.
-                drop_graph();
-                restore_snapshot();
-                nb_->grow(st);
-                for (int k = 0; k < len; ++k) enqueue_step(s_, NbOperator::REBUILD_FLAG, B.enabled && k == len - 1);
-                st = read_status();
-                if (!(st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW))) capture_graph();
-            }
-            advance(len);
-            done += len;
+        const int left = nsteps - done;
+        // A period graph starts at an attempt boundary; an attempt-free graph
+        // ends before the next attempt step.
+        const bool fits = !B.enabled || (period_graph() ? B.phase == 0 : B.phase + len <= B.frequency - 1);
+        int k;
+        bool g = false;
+        if (graphs && fits && left >= len) {
+            k = len;
+            g = true;
+        } else if (graphs && B.enabled && period_graph()) {
+            k = std::min(left, B.frequency - B.phase);  // up to the next period boundary
+        } else if (graphs && B.enabled) {
+            k = std::min(left, B.phase < B.frequency - 1 ? B.frequency - 1 - B.phase : 1);  // up to and over the attempt
         } else {
-            // Plain FP64 form, alignment and remainders: one launch sequence per step.
-            enqueue_step(s_, NbOperator::REBUILD_FLAG, due());
-            advance(1);
-            ++done;
+            k = std::min(left, graphs ? len : chunk);
         }
+        run_steps(k, g);
+        done += k;
     }
     step_count_ += nsteps;
     check_status();

@@ -172,6 +172,36 @@ public:
     };
     const Tuning& tuning() const { return tuning_; }
     void force_rebuild(hipStream_t s);
+    // Rebuild requests recomputed from these positions: cleared, then raised
+    // for every system with an atom more than skin/2 from its position at
+    // the last build (the integrators' test).
+    void recheck_list(const double* pos_dev, hipStream_t s);
+    // Box of system s in the device table and the host copy, checked as at
+    // initialize (reduced form, cutoff at most half of each edge; Ewald:
+    // rectangular).  Set-state path (Context::set_periodic_box_vectors).
+    void set_box(int s, const double* box9, hipStream_t st);
+    // Test hook: reallocates the list storage to `list_cap` entries per
+    // cluster (0: unchanged; rounded up to whole tiles) and rebuilds the lists
+    // from src, and the PME spread bins to `bin_cap` atoms per tile (0:
+    // unchanged).  Capacities below the need raise the overflow status (at
+    // once for the lists, at the next spread for the bins); every write stays
+    // within the storage.
+    void set_capacity(int list_cap, int bin_cap, const double* src, unsigned* status, hipStream_t s);
+    // PmeCache::evaluate_charges / charge_derivatives (pme.rs:63-138) on a
+    // one-system plain-FP64 operator dedicated to the charge mesh: the
+    // Coulomb mesh of per-call charges at these positions, in this box, with
+    // this alpha (the grid is the operator's).  With count == 0 the mesh
+    // forces are added to forces (nullable) and the mesh energy is returned;
+    // with count > 0 the mesh potential at the stencil of each listed atom
+    // (dE/dq) is added to derivatives[k] instead.  The box and alpha stay in
+    // the operator's system table for the next call.
+    double mesh_charges(const double* pos, const double* box, double alpha, const double* charges, double* forces,
+                        const int* atoms, int count, double* derivatives);
+    // Current list capacity (entries per cluster) and spread-bin capacity.
+    void capacity(int* list_cap, int* bin_cap) const {
+        *list_cap = nl_.cap;
+        *bin_cap = pme_.bin_cap;
+    }
     // Tile count and total j-list length of the current list (host read-back).
     void list_stats(int64_t* tiles, int64_t* entries);
 
@@ -242,6 +272,8 @@ private:
 
     // Host-API scratch.
     DevBuf<double> pos_dev_;
+    DevBuf<double> mesh_q_, mesh_out_;  // mesh_charges: charges, potentials
+    DevBuf<int> mesh_atoms_;
     DevBuf<long long> force_fx_, energy_fx_;
     DevBuf<double> force_d_, energy_d_;
     DevBuf<unsigned> status_;

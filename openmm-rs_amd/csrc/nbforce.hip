@@ -953,6 +953,113 @@ void NbOperator::list_stats(int64_t* tiles, int64_t* entries) {
 
 void NbOperator::force_rebuild(hipStream_t s) { fill_async<int>(nl_.flag.p, (size_t)nl_.nsys + 1, 1, s); }
 
+void NbOperator::recheck_list(const double* pos_dev, hipStream_t s) {
+    if (ntotal_ == 0 || !cfg_[0].include_direct) return;
+    fill_async<int>(nl_.flag.p, (size_t)nl_.nsys + 1, 0, s);
+    nlist_check(nl_, pos_dev, 0, s);
+}
+
+void NbOperator::set_capacity(int list_cap, int bin_cap, const double* src, unsigned* status, hipStream_t s) {
+    if (list_cap < 0 || bin_cap < 0) throw Error(OMM_ERR_INVALID, "capacities must be nonnegative");
+    OMM_HIP(hipStreamSynchronize(s));
+    if (list_cap > 0 && cfg_[0].include_direct) {
+        const int cap = (list_cap + JTILE - 1) / JTILE * JTILE;
+        nlist_resize(nl_, cap);
+        pp_.cap = nl_.cap;
+        nlist_build(nl_, src, sys_dev_.p, status, s, true);
+    }
+    if (bin_cap > 0 && cfg_[0].pme_mesh()) {
+        pm_set_bin_cap(pme_, bin_cap);
+        if (ljpme()) pm_set_bin_cap(pme_lj_, bin_cap);
+    }
+    OMM_HIP(hipStreamSynchronize(s));
+}
+
+double NbOperator::mesh_charges(const double* pos, const double* box, double alpha, const double* charges,
+                                double* forces, const int* atoms, int count, double* derivatives) {
+    if (!initialized_ || !device_ready_) throw Error(OMM_ERR_STATE, "PME operator not initialized");
+    if (cfg_.size() != 1 || cfg_[0].method != OMM_NB_PME || mixed_)
+        throw Error(OMM_ERR_INVALID, "the charge mesh runs on a one-system plain-FP64 PME operator");
+    if (!pos || !box || !charges) throw Error(OMM_ERR_INVALID, "null positions, box or charges");
+    if (!(std::isfinite(alpha) && alpha > 0.0)) throw Error(OMM_ERR_INVALID, "alpha must be positive and finite");
+    if (count < 0 || (count > 0 && (!atoms || !derivatives)))
+        throw Error(OMM_ERR_INVALID, "charge derivatives need particle indices and an output array");
+    validate_box(box);
+    const int n = n_[0];
+    if (n <= 0) throw Error(OMM_ERR_INVALID, "the charge mesh needs particles");
+    double qmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(charges[i])) throw Error(OMM_ERR_NONFINITE, "charges must be finite");
+        qmax = std::max(qmax, std::fabs(charges[i]));
+    }
+    for (int i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
+    for (int k = 0; k < count; ++k)
+        if (atoms[k] < 0 || atoms[k] >= n) throw Error(OMM_ERR_INVALID, "particle index out of range");
+    hipStream_t s = stream_;
+    const Box b = make_box(box);
+    if (!have_box_ || std::memcmp(b.v, last_box_.v, sizeof(b.v)) != 0 || sys_h_[0].alpha != alpha) {
+        sys_h_[0].box = b;
+        sys_h_[0].alpha = alpha;
+        upload_systems(s);
+        last_box_ = b;
+        have_box_ = true;
+    }
+    pos_dev_.upload(pos, 3 * (size_t)n, s);
+    mesh_q_.upload(charges, (size_t)n, s);
+    const double saved_max = pme_.max_charge;
+    pme_.max_charge = qmax;  // selects the spread's staging; the FP64 grid is exact either way
+    const int phases = count > 0 ? (PME_SPREAD | PME_FFT) : PME_ALL;
+    unsigned st = 0;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        fill_async<unsigned>(status_.p, 1, 0u, s);
+        fill_async<double>(force_d_.p, 3 * (size_t)n, 0.0, s);
+        fill_async<double>(energy_d_.p, E_SLOTS, 0.0, s);
+        pme_run(pme_, MESH_F64, nl_, pos_dev_.p, mesh_q_.p, sys_dev_.p, nullptr, force_d_.p, nullptr, energy_d_.p,
+                status_.p, s, phases);
+        status_.download(&st, 1, s);
+        OMM_HIP(hipStreamSynchronize(s));
+        if (!(st & ST_BIN_OVERFLOW)) break;
+        grow(st);
+    }
+    pme_.max_charge = saved_max;
+    if (st & ST_BIN_OVERFLOW) throw Error(OMM_ERR_OVERFLOW, "PME spread bins kept overflowing");
+    if (st & ST_NONFINITE) throw Error(OMM_ERR_NONFINITE, "non-finite mesh result");
+    if (count > 0) {
+        mesh_atoms_.upload(atoms, (size_t)count, s);
+        mesh_out_.alloc((size_t)count);
+        pme_potential_at(pme_, sys_dev_.p, pos_dev_.p, mesh_atoms_.p, count, mesh_out_.p, s);
+        std::vector<double> pot(count);
+        mesh_out_.download(pot.data(), (size_t)count, s);
+        OMM_HIP(hipStreamSynchronize(s));
+        for (int k = 0; k < count; ++k) derivatives[k] += pot[k];
+        return 0.0;
+    }
+    double e[E_SLOTS];
+    energy_d_.download(e, E_SLOTS, s);
+    std::vector<double> f(forces ? 3 * (size_t)n : 0);
+    if (forces) force_d_.download(f.data(), 3 * (size_t)n, s);
+    OMM_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < f.size(); ++i) forces[i] += f[i];
+    return e[E_RECIP];
+}
+
+void NbOperator::set_box(int s, const double* box9, hipStream_t st) {
+    if (s < 0 || s >= (int)sys_h_.size()) throw Error(OMM_ERR_INVALID, "system index out of range");
+    if (!cfg_[0].periodic()) throw Error(OMM_ERR_INVALID, "non-periodic methods have no box vectors");
+    if (!box9) throw Error(OMM_ERR_INVALID, "null box vectors");
+    validate_box(box9);
+    for (int a = 0; a < 3; ++a)
+        if (!(cfg_[s].cutoff <= 0.5 * box9[a * 4]))
+            throw Error(OMM_ERR_CUTOFF_BOX, "cutoff must not exceed half the periodic box size");
+    if (cfg_[0].method == OMM_NB_EWALD && !(box9[3] == 0.0 && box9[6] == 0.0 && box9[7] == 0.0))
+        throw Error(OMM_ERR_INVALID, "Ewald requires a rectangular periodic box; use PME for triclinic boxes");
+    const Box b = make_box(box9);
+    sys_h_[s].box = b;
+    OMM_HIP(hipMemcpyAsync(&sys_dev_.p[s].box, &b, sizeof(Box), hipMemcpyHostToDevice, st));
+    OMM_HIP(hipStreamSynchronize(st));
+}
+
 int NbOperator::mesh_mode(bool energy) const {
     if (!mixed_) return MESH_F64;
     return energy ? MESH_F64_FIXED : MESH_F32;

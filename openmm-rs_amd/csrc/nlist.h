@@ -32,6 +32,14 @@ constexpr int TPT = 4;      // tiles of one cluster per pair-kernel task
 constexpr int MAXEX = 8;    // exclusion partners per atom handled by the pair kernel
 constexpr int PME_CHUNK = 64;  // sorted slots per PME spread/gather workgroup
 
+// Squared displacement of an atom from its position at the last list build,
+// summed x, y, z with fused multiply-adds.  The integrators and the list's own
+// displacement check both use this one form, so the same positions raise the
+// same rebuild requests wherever the test runs.
+__host__ __device__ __forceinline__ double skin_d2(double dx, double dy, double dz) {
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+}
+
 // Host description of one system for the list set-up (local atom indices).
 struct NlistSystem {
     int n = 0;

@@ -51,10 +51,8 @@ __global__ void k_check(const double* __restrict__ pos, const double* __restrict
     bool moved = false;
     int si = 0;
     if (i < n) {
-        double dx = pos[3 * i] - pos_build[3 * i];
-        double dy = pos[3 * i + 1] - pos_build[3 * i + 1];
-        double dz = pos[3 * i + 2] - pos_build[3 * i + 2];
-        double d2 = dx * dx + dy * dy + dz * dz;
+        double d2 = skin_d2(pos[3 * i] - pos_build[3 * i], pos[3 * i + 1] - pos_build[3 * i + 1],
+                            pos[3 * i + 2] - pos_build[3 * i + 2]);
         moved = !(d2 <= lim2);  // NaN also requests a rebuild
         si = atom_sys[i];
     }

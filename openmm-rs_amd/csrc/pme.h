@@ -54,6 +54,8 @@ int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStre
 
 // Doubles the per-tile bin capacity after ST_BIN_OVERFLOW.
 void pme_grow_bins(PmeDev& pm);
+// Reallocates the bins to `cap` atoms per spread tile.
+void pm_set_bin_cap(PmeDev& pm, int cap);
 
 // Spread, FFT, convolution (+energy into slot E_RECIP of each system), inverse
 // FFT, gather (forces added).  Atoms with zero charge are skipped as in the
@@ -73,6 +75,12 @@ enum MeshMode : int { MESH_F32 = 0, MESH_F64_FIXED = 1, MESH_F64 = 2 };
 void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
              const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
              unsigned* status, hipStream_t s, int phases = PME_ALL);
+
+// out[k] = the FP64 mesh potential at the stencil of atom atoms[k] of
+// system 0 (dE/dq of that atom, pme.rs:117-137), after an FP64 mesh run
+// (MESH_F64_FIXED or MESH_F64) of the spread and FFT phases.
+void pme_potential_at(const PmeDev& pm, const SysInfo* sys_dev, const double* pos, const int* atoms, int count,
+                      double* out, hipStream_t s);
 
 // B-spline moduli |sum_i M5(i+1) e^{2 pi i k i/n}|^2 with near-zero values
 // replaced by the mean of their neighbours (pme.rs:239-259).

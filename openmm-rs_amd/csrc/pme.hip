@@ -692,7 +692,41 @@ void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* char
     OMM_HIP(hipGetLastError());
 }
 
+// The mesh potential at the stencil of each listed atom of system 0,
+// sum_p phi_p w_x w_y w_z (PmeCache::charge_derivatives, pme.rs:117-137: the
+// derivative of the mesh energy with respect to that atom's charge), from the
+// FP64 potential of the last FP64 transform.  One thread per listed atom.
+__global__ void k_mesh_potential(int count, const int* __restrict__ atoms, const double* __restrict__ pos,
+                                 const SysInfo* __restrict__ sys, const double* __restrict__ phi, double* out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const SysInfo& S = sys[0];
+    const int nx = S.grid[0], ny = S.grid[1], nz = S.grid[2];
+    int base[3];
+    double w[3][5], dw[3][5];
+    stencil<double>(pos + 3 * (size_t)atoms[k], S.box, S.grid, base, w, dw);
+    const double* g = phi + S.qgrid_off;
+    double sum = 0.0;
+    for (int ix = 0; ix < 5; ++ix) {
+        const int gx = (base[0] + ix) % nx;
+        for (int iy = 0; iy < 5; ++iy) {
+            const int gy = (base[1] + iy) % ny;
+            const double* row = g + ((size_t)gx * ny + gy) * nz;
+            for (int iz = 0; iz < 5; ++iz) sum += row[(base[2] + iz) % nz] * w[0][ix] * w[1][iy] * w[2][iz];
+        }
+    }
+    out[k] = sum;
+}
+
 }  // namespace
+
+void pme_potential_at(const PmeDev& pm, const SysInfo* sys_dev, const double* pos, const int* atoms, int count,
+                      double* out, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_mesh_potential, dim3((count + 255) / 256), dim3(256), 0, s, count, atoms, pos, sys_dev,
+                       pm.phi_d.p, out);
+    OMM_HIP(hipGetLastError());
+}
 
 std::vector<double> bspline_moduli(int n) {
     // Order-5 spline at integer points: M5(1..4) = 1/24, 11/24, 11/24, 1/24

@@ -38,6 +38,7 @@ STATUS = {
     5: "DEVICE",
     6: "STATE",
 }
+STATUS_CODES = {name: code for code, name in STATUS.items()}
 
 # Water force field of the workloads (charges/LJ of a 3-site model; flexible
 # O-H bonds and H-O-H angle; masses in Da).
@@ -572,6 +573,64 @@ class GpuNonbondedForce:
             pass
 
 
+class PmeCache:
+    """The charge mesh of ``PmeCache`` (pme.rs:8-138) on the device: the two calls
+    ``ConstantPotentialForce`` makes, with per-call charges, alpha and grid."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(library().omm_pme_create(C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def _args(positions, charges, box, parameters):
+        pos = _f64(positions).reshape(-1, 3)
+        q = _f64(charges).reshape(-1)
+        if len(q) != len(pos):
+            raise ValueError("one charge per position is required")
+        b = _f64(box).reshape(9)
+        alpha, dims = parameters
+        g = (C.c_int32 * 3)(*[int(x) for x in dims])
+        return pos, q, b, C.c_double(float(alpha)), g
+
+    def evaluate_charges(self, positions, charges, box, parameters, forces=None):
+        """Mesh energy of the charges; the mesh forces are added into `forces` (N x 3) when given."""
+        pos, q, b, a, g = self._args(positions, charges, box, parameters)
+        e = C.c_double()
+        fp = None
+        if forces is not None:
+            if not (isinstance(forces, np.ndarray) and forces.dtype == np.float64 and forces.flags.c_contiguous
+                    and forces.size == pos.size):
+                raise ValueError("forces must be a C-contiguous float64 array of N x 3")
+            fp = forces.ctypes.data_as(_dp)
+        _check(library().omm_pme_evaluate_charges(self._h, len(pos), pos.ctypes.data_as(_dp), q.ctypes.data_as(_dp),
+                                                  b.ctypes.data_as(_dp), a, g, fp, C.byref(e)))
+        return e.value
+
+    def charge_derivatives(self, positions, charges, box, parameters, indices, derivatives):
+        """Adds dE_mesh/dq of each listed particle into `derivatives` (float64, len(indices))."""
+        pos, q, b, a, g = self._args(positions, charges, box, parameters)
+        idx = _i32(indices).reshape(-1)
+        if not (isinstance(derivatives, np.ndarray) and derivatives.dtype == np.float64
+                and derivatives.flags.c_contiguous and derivatives.size == idx.size):
+            raise ValueError("derivatives must be a C-contiguous float64 array with one value per index")
+        _check(library().omm_pme_charge_derivatives(self._h, len(pos), pos.ctypes.data_as(_dp),
+                                                    q.ctypes.data_as(_dp), b.ctypes.data_as(_dp), a, g,
+                                                    idx.ctypes.data_as(C.POINTER(C.c_int32)), len(idx),
+                                                    derivatives.ctypes.data_as(_dp)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().omm_pme_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 @dataclass
 class WaterSystem:
     """A flexible-water system: NonbondedForce (PME) + bonds + angles + masses."""
@@ -652,6 +711,19 @@ def _md_desc(system: WaterSystem, temperature, friction, step_size, seed):
     d.constraint_distance = _ptr(keep["cd"], C.c_double)
     d.constraint_tolerance = float(getattr(s, "constraint_tolerance", 0.0) or 0.0)
     return d, keep
+
+
+def _state_arrays(state, n):
+    """Contiguous FP64 (box, positions, velocities, list positions) of a state dict; absent keys are None."""
+    out = []
+    for key, shape in (("box", (9,)), ("positions", (n, 3)), ("velocities", (n, 3)), ("list_positions", (n, 3))):
+        v = state.get(key)
+        out.append(None if v is None else _f64(v).reshape(shape))
+    return out
+
+
+def _ptrs(arrays):
+    return [None if a is None else a.ctypes.data_as(_dp) for a in arrays]
 
 
 PHASES = ("nlist_build", "direct_pairs", "exceptions", "pme_spread", "pme_fft_conv", "pme_gather",
@@ -766,6 +838,16 @@ class Context:
         _check(library().omm_ctx_get_box(self._h, b.ctypes.data_as(_dp)))
         return b.reshape(3, 3)
 
+    def set_capacity(self, list_cap=0, bin_cap=0):
+        """Test hook: list entries per cluster and spread-bin atoms per tile (0: unchanged)."""
+        _check(library().omm_ctx_set_capacity(self._h, int(list_cap), int(bin_cap)))
+
+    def capacity(self):
+        """(list capacity, bin capacity, runs of steps replayed after an overflow)."""
+        lc, bc, r = C.c_int32(), C.c_int32(), C.c_int64()
+        _check(library().omm_ctx_capacity(self._h, C.byref(lc), C.byref(bc), C.byref(r)))
+        return lc.value, bc.value, r.value
+
     def list_stats(self):
         """(tiles, j-list entries) of the current neighbour list."""
         t, e = C.c_int64(), C.c_int64()
@@ -786,7 +868,17 @@ class Context:
         k = C.c_int64()
         _check(library().omm_ctx_get_state(self._h, pos.ctypes.data_as(_dp), vel.ctypes.data_as(_dp),
                                            frc.ctypes.data_as(_dp), C.byref(e), C.byref(k)))
-        return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value)
+        lp = np.zeros((self.n, 3))
+        _check(library().omm_ctx_get_list_positions(self._h, lp.ctypes.data_as(_dp)))
+        return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value,
+                    box=self.box(), list_positions=lp)
+
+    def set_state(self, state):
+        """Context::set_state (context.rs:274-293) from a get_state() dict: step count, box,
+        positions, velocities (each optional) and, when present, the positions the saved
+        neighbour list was built from, so the trajectory continues bit for bit."""
+        arrays = _state_arrays(state, self.n)
+        _check(library().omm_ctx_set_state(self._h, C.c_int64(int(state.get("step", 0))), *_ptrs(arrays)))
 
     def kinetic_energy(self):
         e = C.c_double()
@@ -918,7 +1010,16 @@ class Batch:
         _check(library().omm_batch_get_state(self._h, int(replica), pos.ctypes.data_as(_dp),
                                              vel.ctypes.data_as(_dp), frc.ctypes.data_as(_dp), C.byref(e),
                                              C.byref(k)))
-        return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value)
+        lp = np.zeros((n, 3))
+        _check(library().omm_batch_get_list_positions(self._h, int(replica), lp.ctypes.data_as(_dp)))
+        return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value,
+                    box=self.box(replica), list_positions=lp)
+
+    def set_state(self, replica, state):
+        """Context::set_state of one replica (the step count is shared by the batch)."""
+        arrays = _state_arrays(state, self.sizes[replica])
+        _check(library().omm_batch_set_state(self._h, int(replica), C.c_int64(int(state.get("step", 0))),
+                                             *_ptrs(arrays)))
 
     def energies(self):
         out = np.zeros(len(self.sizes))
