@@ -499,9 +499,18 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         // The FP32 mesh serves force-only evaluations: the influence function
         // in FP32 (exp2 of the hardware), no energy.  The FP64 meshes evaluate
         // it in FP64 and sum the energy.
+        //
+        // The half spectrum stands for each point k and its mirror -k.  The
+        // reference sums the influence over the full grid and keeps the real
+        // part of the inverse transform, which is the half spectrum weighted
+        // by the mean influence of k and its mirror.  The mirror's frequency
+        // is -m on every axis except at a Nyquist index (2 k = n), where it is
+        // m itself; in a triclinic box |k|^2 then differs between the two, so
+        // those points take the mean of both influences.
         constexpr bool F32 = sizeof(R) == 4;
         float rf[9];
         for (int k = 0; k < 9; ++k) rf[k] = (float)b.recip[k];
+        const bool tric = b.triclinic != 0;
         const float pi2_a2_log2e = (float)(pi2_a2 * 1.4426950408889634);
         const float pref_f = (float)pref;
         double eacc = 0.0;
@@ -515,36 +524,31 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
                 S = Cx<R>{0, 0};
                 continue;
             }
-            if (F32 && disp) {
-                const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
-                const float my = (float)(y < (ny + 1) / 2 ? y : y - ny);
-                const float mz = (float)(kz < (nz + 1) / 2 ? kz : kz - nz);
-                float k2 = 0.0f;
-                for (int a = 0; a < 3; ++a) {
-                    const float kv = mx * rf[a] + my * rf[3 + a] + mz * rf[6 + a];
-                    k2 += kv * kv;
-                }
-                const float kk = __builtin_sqrtf(k2);
-                float ec, ex;
-                Math<float>::erfc_exp((float)pi_a * kk, ec, ex);
-                const float mod = (float)mod_x[kx] * (float)mod_y[y] * (float)mod_z[kz];
-                const float infl =
-                    (float)dpref * ((float)dc1 * ec * kk * k2 + ex * ((float)da3 - (float)dc2 * k2)) / mod;
-                S.x = (R)((float)S.x * infl);
-                S.y = (R)((float)S.y * infl);
-                continue;
-            }
+            // Mirror signs: the mirror's frequency is -m except at a Nyquist index.
+            const bool nyq = tric && (2 * kx == nx || 2 * y == ny || 2 * kz == nz);
+            const int sx = 2 * kx == nx ? 1 : -1, sy = 2 * y == ny ? 1 : -1, sz = 2 * kz == nz ? 1 : -1;
             if (F32) {
                 const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
                 const float my = (float)(y < (ny + 1) / 2 ? y : y - ny);
                 const float mz = (float)(kz < (nz + 1) / 2 ? kz : kz - nz);
-                float k2 = 0.0f;
-                for (int a = 0; a < 3; ++a) {
-                    const float kv = mx * rf[a] + my * rf[3 + a] + mz * rf[6 + a];
-                    k2 += kv * kv;
-                }
                 const float mod = (float)mod_x[kx] * (float)mod_y[y] * (float)mod_z[kz];
-                const float infl = pref_f * __builtin_amdgcn_exp2f(-pi2_a2_log2e * k2) / (k2 * mod);
+                auto influence = [&](float ax, float ay, float az) {
+                    float k2 = 0.0f;
+                    for (int a = 0; a < 3; ++a) {
+                        const float kv = ax * rf[a] + ay * rf[3 + a] + az * rf[6 + a];
+                        k2 += kv * kv;
+                    }
+                    if (disp) {
+                        const float kk = __builtin_sqrtf(k2);
+                        float ec, ex;
+                        Math<float>::erfc_exp((float)pi_a * kk, ec, ex);
+                        return (float)dpref * ((float)dc1 * ec * kk * k2 + ex * ((float)da3 - (float)dc2 * k2)) /
+                               mod;
+                    }
+                    return pref_f * __builtin_amdgcn_exp2f(-pi2_a2_log2e * k2) / (k2 * mod);
+                };
+                float infl = influence(mx, my, mz);
+                if (nyq) infl = 0.5f * (infl + influence(-sx * mx, -sy * my, -sz * mz));
                 S.x = (R)((float)S.x * infl);
                 S.y = (R)((float)S.y * infl);
                 continue;
@@ -552,19 +556,21 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
             const double mx = kx < (nx + 1) / 2 ? kx : kx - nx;
             const double my = y < (ny + 1) / 2 ? y : y - ny;
             const double mz = kz < (nz + 1) / 2 ? kz : kz - nz;
-            double k2 = 0.0;
-            for (int a = 0; a < 3; ++a) {
-                double kv = mx * b.recip[0 * 3 + a] + my * b.recip[1 * 3 + a] + mz * b.recip[2 * 3 + a];
-                k2 += kv * kv;
-            }
             const double mod = (double)mod_x[kx] * (double)mod_y[y] * (double)mod_z[kz];
-            double infl;
-            if (disp) {
-                const double kk = sqrt(k2), bb = pi_a * kk;
-                infl = dpref / mod * (dc1 * erfc(bb) * kk * k2 + exp(-bb * bb) * (da3 - dc2 * k2));
-            } else {
-                infl = pref * exp(-pi2_a2 * k2) / (k2 * mod);
-            }
+            auto influence = [&](double ax, double ay, double az) {
+                double k2 = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    double kv = ax * b.recip[0 * 3 + a] + ay * b.recip[1 * 3 + a] + az * b.recip[2 * 3 + a];
+                    k2 += kv * kv;
+                }
+                if (disp) {
+                    const double kk = sqrt(k2), bb = pi_a * kk;
+                    return dpref / mod * (dc1 * erfc(bb) * kk * k2 + exp(-bb * bb) * (da3 - dc2 * k2));
+                }
+                return pref * exp(-pi2_a2 * k2) / (k2 * mod);
+            };
+            double infl = influence(mx, my, mz);
+            if (nyq) infl = 0.5 * (infl + influence(-sx * mx, -sy * my, -sz * mz));
             const double wz = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
             const double s2 = (double)S.x * S.x + (double)S.y * S.y;
             eacc += wz * 0.5 * infl * s2;

@@ -86,3 +86,39 @@ def test_rejects_bad_input():
         cache.evaluate_charges(pos, [1.0, 1.0], box, (2.4, (4, 12, 12)))
     with pytest.raises(ommhip.OmmError):
         cache.charge_derivatives(pos, [1.0, 1.0], box, (2.4, (12, 12, 12)), [2], np.zeros(1))
+
+
+@pytest.mark.parametrize("box", [
+    [[3.0, 0, 0], [0, 3.2, 0], [0, 0.1, 3.4]],
+    [[3.0, 0, 0], [0.35, 3.2, 0], [-0.35, 0.1, 3.4]],
+    [[3.1, 0, 0], [-0.9, 2.9, 0], [1.2, -1.1, 3.3]],
+])
+@pytest.mark.parametrize("grid", [(12, 14, 16), (13, 15, 17)])
+def test_triclinic_mesh_energy_is_exact(box, grid):
+    # In a triclinic box a point at a Nyquist index and its mirror have
+    # different |k|^2; the reference's full-grid sum averages their
+    # influence, which the half-spectrum convolution must reproduce.  The
+    # NonbondedForce operator's reciprocal energy in plain FP64 then matches
+    # the oracle to rounding, and the mixed form to its fixed-point grid.
+    box = np.array(box)
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(0, 3, size=(40, 3))
+    q = rng.normal(scale=0.5, size=40)
+    f = ommhip.NonbondedForce()
+    f.set_nonbonded_method("PME")
+    f.set_cutoff_distance(1.0)
+    f.set_use_dispersion_correction(False)
+    for x in q:
+        f.add_particle(float(x), 1.0, 0.0)
+    f.set_pme_parameters(2.4, *grid)
+    F_ref, e_ref = orc.pme_mesh(pos, q, box, 2.4, grid)
+    for precision, tol in ((ommhip.PRECISION_DOUBLE, 1e-12), (ommhip.PRECISION_MIXED, 1e-8)):
+        op = ommhip.GpuNonbondedForce(f, precision)
+        op.initialize(box)
+        e = op.energy_parts(pos, box)[2]
+        assert abs(e - e_ref) <= tol * abs(e_ref), (precision, e, e_ref)
+    cache = ommhip.PmeCache()
+    forces = np.zeros((40, 3))
+    e = cache.evaluate_charges(pos, q, box, (2.4, grid), forces)
+    assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+    assert np.sqrt(((forces - F_ref) ** 2).sum() / (F_ref ** 2).sum()) <= 1e-12
