@@ -241,6 +241,23 @@ omm_status omm_pme_charge_derivatives(omm_pme* pme, int32_t num_particles, const
 
 omm_status omm_pme_destroy(omm_pme* pme);
 
+/* No counterpart in the reference: omm_pme_charge_derivatives for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  It calls no hipMalloc, hipFree or allocating library routine.
+ * pos (n x 3), charges (n), indices (count) and derivatives (count, ADDED to)
+ * are the caller's DEVICE buffers; box (9), alpha and grid are host values.
+ * Every kernel is launched on `stream` (a hipStream_t; NULL: the default
+ * stream) and the call returns without waiting.  The only other device
+ * memory is a static FP64 complex grid of 64^3 points (4 MiB) per device,
+ * held until the process ends: grids above 64 points on an axis are rejected
+ * with OMM_ERR_INVALID.  Every call on a device shares that grid (each call
+ * clears it first), so calls that overlap on one device, from other streams
+ * or threads, race.  An index outside [0, n) sets its derivative to NaN. */
+omm_status omm_pme_charge_derivatives_noalloc(int32_t num_particles, const double* pos, const double* charges,
+                                              const double* box, double alpha, const int32_t grid[3],
+                                              const int32_t* indices, int32_t count, double* derivatives,
+                                              void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* MD context: the step around the operator for the flexible-water workloads.
  * Stands in for Context::step (context.rs:795-928) with a LangevinMiddle

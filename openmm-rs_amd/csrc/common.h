@@ -62,6 +62,9 @@ struct Box {
 };
 
 Box make_box(const double* box9);
+// Checks box vectors the way the reference does (system.rs:16-39): finite, a
+// along x, b in the x-y plane, reduced form.
+void validate_box(const double* v);
 
 // Recomputes the reciprocal rows, volume and triclinic flag of a box from its
 // vectors (pme.rs:176-199); used where a barostat rescales a box on the device.

@@ -21,16 +21,6 @@ void validate_particle(double q, double s, double e) {
         throw Error(OMM_ERR_INVALID, "nonbonded parameters must be finite with nonnegative sigma and epsilon");
 }
 
-// system.rs:16-39
-void validate_box(const double* v) {
-    for (int i = 0; i < 9; ++i)
-        if (!std::isfinite(v[i])) throw Error(OMM_ERR_NONFINITE, "periodic box vectors must be finite");
-    if (!(v[1] == 0.0 && v[2] == 0.0)) throw Error(OMM_ERR_INVALID, "first periodic box vector must be parallel to x");
-    if (!(v[5] == 0.0)) throw Error(OMM_ERR_INVALID, "second periodic box vector must be in the x-y plane");
-    if (!(v[0] > 0.0 && v[4] > 0.0 && v[8] > 0.0 && v[0] >= 2.0 * std::fabs(v[3]) && v[0] >= 2.0 * std::fabs(v[6]) &&
-          v[4] >= 2.0 * std::fabs(v[7])))
-        throw Error(OMM_ERR_INVALID, "periodic box vectors must be in reduced form");
-}
 
 // nonbonded.rs:1290-1336 — adaptive Simpson for the switched tail integral.
 template <class F>
@@ -46,6 +36,17 @@ double simpson(const F& f, double a, double b, double fa, double fm, double fb, 
 }
 
 }  // namespace
+
+// system.rs:16-39
+void validate_box(const double* v) {
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(v[i])) throw Error(OMM_ERR_NONFINITE, "periodic box vectors must be finite");
+    if (!(v[1] == 0.0 && v[2] == 0.0)) throw Error(OMM_ERR_INVALID, "first periodic box vector must be parallel to x");
+    if (!(v[5] == 0.0)) throw Error(OMM_ERR_INVALID, "second periodic box vector must be in the x-y plane");
+    if (!(v[0] > 0.0 && v[4] > 0.0 && v[8] > 0.0 && v[0] >= 2.0 * std::fabs(v[3]) && v[0] >= 2.0 * std::fabs(v[6]) &&
+          v[4] >= 2.0 * std::fabs(v[7])))
+        throw Error(OMM_ERR_INVALID, "periodic box vectors must be in reduced form");
+}
 
 Box make_box(const double* v) {
     Box b{};
