@@ -35,8 +35,22 @@ import ommhip  # noqa: E402
 
 METRIC = "ns/day (23.5k-atom PME water) + batched replica-steps/s, mixed vs plain FP64"
 DT_PS = 0.001
+# MI355X published peaks.  The FP32 vector peak counts packed FP32 (two
+# operations per lane per cycle); this build compiles with packed FP32 off,
+# so its attainable FP32 rate is half of it.
 FP32_PEAK_TFLOPS = 157.3
+FP32_NONPACKED_TFLOPS = 78.6
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic work per unit (the fixed convention of the roofline): FLOP per
+# in-cutoff pair, per exception, per atom for the spread and the gather;
+# bytes per grid point for the FFT passes and convolution, per atom for the
+# FP64 integrator with the bonded terms.
 FLOP_PER_PAIR = 75
+FLOP_PER_EXCEPTION = 40
+FLOP_SPREAD_PER_ATOM = 590
+FLOP_GATHER_PER_ATOM = 1140
+MESH_BYTES_PER_POINT = 24
+STEP_BYTES_PER_ATOM = 128
 TILE_SLOTS = 8 * 64  # pair slots of one k_pairs tile (8-atom cluster x 64 list entries)
 # HBM bytes per k_pairs launch on the 23,550-atom workload: rocprofv3 FETCH_SIZE
 # and WRITE_SIZE passes, each scaled by the same counter's reading for a plain
@@ -148,6 +162,7 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
     atoms = int(sum(len(s.masses) for s in systems))
     value = count * steps / secs
     plain = count * plain_steps / plain_secs
+    gpoints = sum(int(np.prod(pme_rule(0.9, 5e-4, float(s.box[0, 0]))[1])) for s in systems)
     return {
         "replicas": count,
         "atoms": atoms,
@@ -158,6 +173,8 @@ def run_batch(steps, warmup, equilibrate, plain_steps, count=64, seed=4096):
         "steps": steps,
         "graph_steps": g,
         "pairs_per_step": npairs,
+        "step_roofline": step_roofline(npairs, atoms, atoms, gpoints, secs / steps),
+        "phase_roofline": phase_roofline(phases, npairs, atoms, atoms, gpoints),
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "launch_tuning": batch.launch_tuning(),
         "plain_fp64_per_system": {"replica_steps_per_s": round(plain, 2), "steps_per_replica": plain_steps},
@@ -217,6 +234,9 @@ def run_large(steps, equilibrate, seed=30000):
     g = int(np.prod(grid))
     nbytes = algorithmic_bytes(n, g, entries)
     rates = {k: round(nbytes[k] / (phases[k] * 1e-3) / 1e9, 1) for k in nbytes if phases.get(k, 0) > 0}
+    gpts = int(np.prod(grid))
+    sroof = step_roofline(npairs, n, n, gpts, secs / steps)
+    proof = phase_roofline(phases, npairs, n, n, gpts)
     ctx.close()
     return {
         "workload": "100000 flexible 3-site waters (300000 atoms), 14.42 nm cubic box, PME cutoff 1.0 nm "
@@ -230,8 +250,14 @@ def run_large(steps, equilibrate, seed=30000):
         "pair_tflops": round(FLOP_PER_PAIR * npairs / (phases["direct_pairs"] * 1e-3) / 1e12, 3),
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "algorithmic_bytes": nbytes,
-        "achieved_hbm_gbps": rates,
-        "hbm_peak_gbps": 8000.0,
+        "effective_gbps": rates,
+        "effective_gbps_note": "algorithmic bytes / serial HIP-event time of each phase, repeated on the same "
+                               "buffers: the working sets fit the 256 MB Infinity Cache, so these are effective "
+                               "rates, not HBM traffic (measured HBM bytes per kernel: rocprofv3 FETCH_SIZE / "
+                               "WRITE_SIZE summaries)",
+        "hbm_peak_gbps": HBM_PEAK_GBPS,
+        "step_roofline": sroof,
+        "phase_roofline": proof,
     }
 
 
@@ -258,30 +284,136 @@ def algorithmic_bytes(n, g, entries):
     }
 
 
-def cpu_baseline(sample_steps):
+def step_roofline(npairs, natoms, nexc, gpoints, seconds_per_step):
+    """Time the step would take at the peaks for its algorithmic work, and the
+    fraction of the measured step time that is:
+    t_roof = (75 P + 40 E + 1730 N) / FP32 peak + (24 G + 128 N) / HBM peak."""
+    t_flop = (FLOP_PER_PAIR * npairs + FLOP_PER_EXCEPTION * nexc
+              + (FLOP_SPREAD_PER_ATOM + FLOP_GATHER_PER_ATOM) * natoms) / (FP32_PEAK_TFLOPS * 1e12)
+    t_mem = (MESH_BYTES_PER_POINT * gpoints + STEP_BYTES_PER_ATOM * natoms) / (HBM_PEAK_GBPS * 1e9)
+    t_roof = t_flop + t_mem
+    return {"t_roof_us": round(t_roof * 1e6, 3), "t_measured_us": round(seconds_per_step * 1e6, 2),
+            "frac": round(t_roof / seconds_per_step, 4),
+            "work": {"pairs": npairs, "exceptions": nexc, "atoms": natoms, "grid_points": gpoints}}
+
+
+def phase_roofline(phases, npairs, natoms, nexc, gpoints):
+    """Each phase's algorithmic rate (serial HIP-event time) against its
+    roofline: FLOP phases against the FP32 vector peak, byte phases against HBM."""
+    work = {
+        "direct_pairs": (FLOP_PER_PAIR * npairs, "TFLOP/s"),
+        "exceptions": (FLOP_PER_EXCEPTION * nexc, "TFLOP/s"),
+        "pme_spread": (FLOP_SPREAD_PER_ATOM * natoms, "TFLOP/s"),
+        "pme_gather": (FLOP_GATHER_PER_ATOM * natoms, "TFLOP/s"),
+        "pme_fft_conv": (MESH_BYTES_PER_POINT * gpoints, "GB/s"),
+    }
+    out = {}
+    for k, (w, unit) in work.items():
+        t = phases.get(k, 0.0) * 1e-3
+        if t <= 0:
+            continue
+        if unit == "TFLOP/s":
+            a = w / t / 1e12
+            out[k] = {"achieved": round(a, 3), "unit": unit, "peak": FP32_PEAK_TFLOPS,
+                      "frac": round(a / FP32_PEAK_TFLOPS, 5), "frac_of_nonpacked": round(a / FP32_NONPACKED_TFLOPS, 5)}
+        else:
+            a = w / t / 1e9
+            out[k] = {"achieved": round(a, 1), "unit": unit, "peak": HBM_PEAK_GBPS, "frac": round(a / HBM_PEAK_GBPS, 5)}
+    t = (phases.get("integrator", 0.0) + phases.get("bonded", 0.0)) * 1e-3
+    if t > 0:
+        a = STEP_BYTES_PER_ATOM * natoms / t / 1e9
+        out["integrator_and_bonded"] = {"achieved": round(a, 1), "unit": "GB/s", "peak": HBM_PEAK_GBPS,
+                                        "frac": round(a / HBM_PEAK_GBPS, 5)}
+    return out
+
+
+def host_info():
+    """Cores this process may use and the CPU model (lscpu's 'Model name')."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
+
+
+def _oracle_md_steps(s, steps, seed):
+    """Seconds for `steps` full MD steps of a water system with the FP64 oracle
+    (nonbonded + bonds + angles + LangevinMiddle), single-threaded."""
     import oracle as orc
 
-    s = workload()
-    box = s.box
     desc = s.nonbonded.desc()
-    rng = np.random.default_rng(0)
+    rng = np.random.default_rng(seed)
     pos = s.positions.copy()
     vel = rng.normal(size=pos.shape) * np.sqrt(orc.boltzmann() * 300.0 / s.masses)[:, None]
     t0 = time.perf_counter()
-    for _ in range(sample_steps):
-        F, _, _ = orc.nb_evaluate(desc, pos, box)
+    for _ in range(steps):
+        F, _, _ = orc.nb_evaluate(desc, pos, s.box)
         fb, _ = orc.harmonic_bonds(s.bonds, s.bond_params, pos)
         fa, _ = orc.harmonic_angles(s.angles, s.angle_params, pos)
         pos, vel = orc.lfmiddle_step(pos, vel, F + fb + fa, s.masses, DT_PS, 300.0, 1.0,
                                      rng.normal(size=pos.size))
-    dt = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def _replica_steps(args):
+    r, steps = args
+    s = batch_workload()[r]
+    return r, len(s.masses), _oracle_md_steps(s, steps, r)
+
+
+def cpu_baseline(sample_steps, c1_evals, c4_steps):
+    """The FP64 C++ restatement of the reference path (the oracle), timed on
+    this host: the 23,550-atom MD step (ns/day extrapolated from a bounded
+    sample), the 9,000-atom energy+force evaluation (median of c1_evals after
+    one warmup) and the 64-replica workload with one replica per usable core
+    (at most 16, the host share of one GPU) run side by side, beside the
+    one-core rate of the same replicas."""
+    import multiprocessing as mp
+
+    import oracle as orc
+
+    host = host_info()
+    dt = _oracle_md_steps(workload(), sample_steps, 0)
+    s1 = ommhip.water_system(3000, 4.48, 11, cutoff=0.9, tolerance=5e-4)
+    d1 = s1.nonbonded.desc()
+    orc.nb_evaluate(d1, s1.positions, s1.box)  # warmup
+    times = []
+    for _ in range(c1_evals):
+        t0 = time.perf_counter()
+        orc.nb_evaluate(d1, s1.positions, s1.box)
+        times.append(time.perf_counter() - t0)
+    c1 = 1.0 / float(np.median(times))
+    cores = max(1, min(16, host["usable_cores"], 64))
+    ctx = mp.get_context("fork")  # the children only run the CPU oracle
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_replica_steps, [(r, c4_steps) for r in range(cores)])
+    wall = time.perf_counter() - t0
+    per_replica = [sec / c4_steps for _, _, sec in res]
     return {
         "value": ns_per_day(sample_steps, dt),
         "unit": "ns/day",
         "cores": 1,
         "kind": "port",
+        "extrapolated": True,
         "sample": f"{sample_steps} full MD steps of the same 23,550-atom workload with the single-threaded "
-                  f"FP64 C++ restatement (cell list, exclusion hash set, c2c FFT), {dt:.1f} s",
+                  f"FP64 C++ restatement (cell list, exclusion hash set, c2c FFT), {dt:.1f} s, extrapolated to ns/day",
+        "host": host,
+        "c1_evaluations_per_s": {"value": round(c1, 4), "workload": "3000 waters (9000 atoms), 4.48 nm, PME 0.9 nm",
+                                 "protocol": f"median of {c1_evals} energy+force evaluations after one warmup",
+                                 "cores": 1},
+        "c4_replica_steps_per_s": {
+            "value": round(cores * c4_steps / wall, 3), "cores": cores,
+            "one_core": round(1.0 / float(np.mean(per_replica)), 3),
+            "sample": f"replicas 0..{cores - 1} of the 64-replica workload, one per core, {c4_steps} steps each, "
+                      f"{wall:.1f} s wall; one_core = 1 / mean seconds per replica-step",
+        },
     }
 
 
@@ -292,7 +424,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--equilibrate", type=int, default=2000)
     ap.add_argument("--plain-steps", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-c1-evals", type=int, default=5)
+    ap.add_argument("--cpu-c4-steps", type=int, default=2)
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-equilibrate", type=int, default=1000)
@@ -319,6 +453,11 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas would shard "
               "across GPUs with no communication; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    # The CPU baseline runs first: its worker processes are forked before this
+    # process touches the GPU.
+    cpu = None
+    if not (args.no_cpu or args.large_only or args.batch_only):
+        cpu = cpu_baseline(args.cpu_steps, args.cpu_c1_evals, args.cpu_c4_steps)
     if ommhip.device_count() < 1:
         print("no gfx950 device visible", file=sys.stderr)
         sys.exit(1)
@@ -338,6 +477,9 @@ def main():
     achieved = FLOP_PER_PAIR * npairs / t_pairs / 1e12
 
     plain = ns_per_day(args.plain_steps, psecs)
+    natoms = 23550
+    nexc = 23550  # three exclusions per water
+    gpoints = 55 ** 3
 
     line = {
         "metric": METRIC,
@@ -361,18 +503,25 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
+            # A FLOP-bound kernel; the schema's FLOP option is "mfma", but the
+            # work runs on the FP32 vector ALUs (no matrix cores): see "unit_note".
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "traffic": K_PAIRS_TRAFFIC_BYTES,
+            "unit_note": "FP32 vector (VALU) work, no MFMA; peak 157.3 TFLOP/s counts packed FP32, which this build "
+                         "disables, so 78.6 TFLOP/s is the attainable peak (frac_of_nonpacked)",
+            "frac_of_nonpacked": round(achieved / FP32_NONPACKED_TFLOPS, 4),
             "kernel": "k_pairs (direct-space LJ + erfc-Ewald), FP32 VALU work: 75 FLOP x in-cutoff pairs per launch",
             "pairs_per_launch": npairs,
             "ms_per_launch": round(phases["direct_pairs"], 5),
             "tiles_per_launch": tiles,
             "pair_slot_efficiency": round(npairs / max(1, tiles * TILE_SLOTS), 4),
         },
+        "step_roofline": step_roofline(npairs, natoms, nexc, gpoints, secs / args.steps),
+        "phase_roofline": phase_roofline(phases, npairs, natoms, nexc, gpoints),
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "launch_tuning": ctx.launch_tuning(),
         "plain_fp64": {"value": round(plain, 3), "unit": "ns/day", "steps": args.plain_steps,
@@ -384,8 +533,8 @@ def main():
         line["batched_npt"] = run_npt(args.npt_steps, args.npt_equilibrate)
     if not args.no_large:
         line["large_system"] = run_large(args.large_steps, args.large_equilibrate)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 
