@@ -251,6 +251,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;  // PME branch (default priority, like stream_)
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
+    hipEvent_t ev_list_ = nullptr;  // the list build is done (the gather may read the sorted order)
     bool joined_ = true;
 
     std::vector<SysInfo> sys_h_;
@@ -267,7 +268,7 @@ private:
     std::vector<SysInfo> sys_lj_h_;
     DevBuf<SysInfo> sys_lj_dev_;
     void lj_mesh(int mode, const double* pos_dev, long long* force_fx, double* force_d, long long* energy_fx,
-                 double* energy_d, unsigned* status, hipStream_t s, int phases = PME_ALL);
+                 double* energy_d, unsigned* status, hipStream_t s, int phases = PME_ALL, bool atom_order = false);
     PairParams pp_;
 
     // Host-API scratch.

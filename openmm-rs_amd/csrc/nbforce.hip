@@ -235,6 +235,7 @@ NbOperator::NbOperator(const omm_nb_desc* const* descs, int nsys, int precision)
 NbOperator::~NbOperator() {
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (ev_list_) (void)hipEventDestroy(ev_list_);
     if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -349,6 +350,7 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     if (!side_) OMM_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     if (!ev_fork_) OMM_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     if (!ev_join_) OMM_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
+    if (!ev_list_) OMM_HIP(hipEventCreateWithFlags(&ev_list_, hipEventDisableTiming));
     hipStream_t s = stream_;
     const int N = ntotal_;
     std::vector<int> atom_sys(N);
@@ -763,13 +765,15 @@ __global__ void k_copy_boxes(int nsys, const SysInfo* __restrict__ src, SysInfo*
 }  // namespace
 
 void NbOperator::lj_mesh(int mode, const double* pos_dev, long long* force_fx, double* force_d, long long* energy_fx,
-                         double* energy_d, unsigned* status, hipStream_t s, int phases) {
+                         double* energy_d, unsigned* status, hipStream_t s, int phases, bool atom_order) {
     // The boxes live in the Coulomb table (a barostat rescales them there).
     const int nsys = (int)sys_lj_h_.size();
-    hipLaunchKernelGGL(k_copy_boxes, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, sys_dev_.p, sys_lj_dev_.p);
-    OMM_HIP(hipGetLastError());
+    if (phases & PME_SPREAD) {
+        hipLaunchKernelGGL(k_copy_boxes, dim3((nsys + 63) / 64), dim3(64), 0, s, nsys, sys_dev_.p, sys_lj_dev_.p);
+        OMM_HIP(hipGetLastError());
+    }
     pme_run(pme_lj_, mode, nl_, pos_dev, part_.c6amp.p, sys_lj_dev_.p, force_fx, force_d, energy_fx, energy_d, status,
-            s, phases);
+            s, phases, atom_order);
 }
 
 void NbOperator::patch_layout(hipStream_t s) {
@@ -1070,6 +1074,25 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
                          long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
                          unsigned* status, hipStream_t s, bool join_now) {
     const NbSettings& c = cfg_[0];
+    // The mesh runs on the side stream, concurrent with the list build and
+    // the direct-space pairs; join() makes s wait for it.  It is forked
+    // before the list build: the spread bins the atoms in index order and the
+    // transforms never read the list, so only the gather (which walks the
+    // list's sorted order) waits for the build.
+    const bool mesh = recip && c.screened();
+    if (mesh) {
+        OMM_HIP(hipEventRecord(ev_fork_, s));
+        OMM_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
+        if (c.pme_mesh()) {
+            pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
+                    energy_d, status, side_, PME_SPREAD | PME_FFT, true);
+            if (ljpme())
+                lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_,
+                        PME_SPREAD | PME_FFT, true);
+        } else
+            ewald_run(ewald_, pos_dev, part_.charge.p, sys_dev_.p, alpha_[0], force_fx, force_d,
+                      energy ? energy_fx : nullptr, energy ? energy_d : nullptr, status, mixed_, side_);
+    }
     if (direct) {
         if (rebuild == REBUILD_CHECK) {
             nlist_check(nl_, pos_dev, 0, s);
@@ -1080,20 +1103,15 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
             nlist_build(nl_, pos_dev, sys_dev_.p, status, s, true);
         }
     }
-    // The mesh runs on the side stream, forked after the list build (the
-    // spread visits atoms in the list's sorted order) and concurrent with the
-    // direct-space pairs; join() makes s wait for it.
-    const bool mesh = recip && c.screened();
     if (mesh) {
-        OMM_HIP(hipEventRecord(ev_fork_, s));
-        OMM_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
         if (c.pme_mesh()) {
+            OMM_HIP(hipEventRecord(ev_list_, s));
+            OMM_HIP(hipStreamWaitEvent(side_, ev_list_, 0));
             pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
-                    energy_d, status, side_);
-            if (ljpme()) lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_);
-        } else
-            ewald_run(ewald_, pos_dev, part_.charge.p, sys_dev_.p, alpha_[0], force_fx, force_d,
-                      energy ? energy_fx : nullptr, energy ? energy_d : nullptr, status, mixed_, side_);
+                    energy_d, status, side_, PME_GATHER);
+            if (ljpme())
+                lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_, PME_GATHER);
+        }
         OMM_HIP(hipEventRecord(ev_join_, side_));
         joined_ = false;
     }

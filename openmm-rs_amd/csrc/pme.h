@@ -72,9 +72,12 @@ enum : int { PME_SPREAD = 1, PME_FFT = 2, PME_GATHER = 4, PME_ALL = 7 };
 enum MeshMode : int { MESH_F32 = 0, MESH_F64_FIXED = 1, MESH_F64 = 2 };
 // The atoms are visited in the list's sorted slot order (nl.sorted_atom),
 // which only sets locality: any permutation gives the same fixed-point sums.
+// atom_order: the spread's binning visits the atoms in index order instead,
+// so the spread and the transforms may run while a list build rewrites the
+// sorted order (the gather always reads it and must follow the build).
 void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
              const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
-             unsigned* status, hipStream_t s, int phases = PME_ALL);
+             unsigned* status, hipStream_t s, int phases = PME_ALL, bool atom_order = false);
 
 // out[k] = the FP64 mesh potential at the stencil of atom atoms[k] of
 // system 0 (dE/dq of that atom, pme.rs:117-137), after an FP64 mesh run

@@ -52,8 +52,11 @@ __device__ __forceinline__ int tile_of(int i, int tiles, int edge) { return min(
 inline int tiles_for(int n, int e) { return e == 8 ? std::max(1, (n + 4) / 8) : std::max(1, n / e); }
 
 // Atoms are visited in the neighbour list's sorted order, so the lanes of a
-// wave mostly fall into one or two tiles; each distinct tile of a wave takes
-// one atomic for all of its lanes (contended device-scope atomics serialise).
+// wave mostly fall into one or two tiles, or (order == nullptr) in atom
+// order, which lets the spread start before a list build has placed the
+// atoms; each distinct tile of a wave takes one atomic for all of its lanes
+// (contended device-scope atomics serialise).  The bins' order never changes
+// a result: the spread's fixed-point sums are order-independent.
 template <class R>
 __global__ void k_pme_bin(int nslots, const int* __restrict__ order, const double* __restrict__ pos,
                           const double* __restrict__ charge, const int* __restrict__ atom_sys,
@@ -61,7 +64,7 @@ __global__ void k_pme_bin(int nslots, const int* __restrict__ order, const doubl
                           int bin_cap, unsigned* status) {
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const int a = slot < nslots ? order[slot] : -1;
+    const int a = slot < nslots ? (order ? order[slot] : slot) : -1;
     const double q = a >= 0 ? charge[a] : 0.0;
     int tile = -1;
     if (q != 0.0) {  // uncharged atoms do not spread (pme.rs:315-330)
@@ -631,7 +634,8 @@ double* pme_weights<double>(PmeDev& pm) {
 }
 
 template <class R, class G, class A>
-void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* charge, const SysInfo* sys, G* qgrid,
+void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, const double* charge,
+           const SysInfo* sys, G* qgrid,
            R* cgrid, R* phi, const R* roots, const R* moduli, A* force, A* energy, unsigned* status, int phases,
            hipStream_t s) {
     // Bin counts are zero between runs: set so at allocation, and cleared by
@@ -639,9 +643,10 @@ void run_t(PmeDev& pm, const NlistDev& nl, const double* pos, const double* char
     // without transforming).
     if (phases & PME_SPREAD) {
         R* st_w = pme_weights<R>(pm);
-        hipLaunchKernelGGL((k_pme_bin<R>), dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.nslots,
-                           nl.sorted_atom.p, pos, charge, pm.atom_sys, sys, pm.st_idx.p, st_w, pm.bin_count.p,
-                           pm.bins.p, pm.bin_cap, status);
+        const int nitems = atom_order ? pm.natoms : nl.nslots;
+        hipLaunchKernelGGL((k_pme_bin<R>), dim3((nitems + 255) / 256), dim3(256), 0, s, nitems,
+                           atom_order ? (const int*)nullptr : nl.sorted_atom.p, pos, charge, pm.atom_sys, sys,
+                           pm.st_idx.p, st_w, pm.bin_count.p, pm.bins.p, pm.bin_cap, status);
         hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
                            pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap, qgrid,
                            pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0, status);
@@ -874,15 +879,15 @@ void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* ato
 
 void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
              const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
-             unsigned* status, hipStream_t s, int phases) {
+             unsigned* status, hipStream_t s, int phases, bool atom_order) {
     if (mode == MESH_F32)
-        run_t<float, long long, long long>(pm, nl, pos, charge, sys_dev, pm.qgrid_fx.p, pm.cgrid_f.p, pm.phi_f.p,
+        run_t<float, long long, long long>(pm, nl, atom_order, pos, charge, sys_dev, pm.qgrid_fx.p, pm.cgrid_f.p, pm.phi_f.p,
                                            pm.roots_f.p, pm.moduli_f.p, force_fx, energy_fx, status, phases, s);
     else if (mode == MESH_F64_FIXED)
-        run_t<double, long long, long long>(pm, nl, pos, charge, sys_dev, pm.qgrid_fx.p, pm.cgrid_d.p, pm.phi_d.p,
+        run_t<double, long long, long long>(pm, nl, atom_order, pos, charge, sys_dev, pm.qgrid_fx.p, pm.cgrid_d.p, pm.phi_d.p,
                                             pm.roots_d.p, pm.moduli_d.p, force_fx, energy_fx, status, phases, s);
     else
-        run_t<double, double, double>(pm, nl, pos, charge, sys_dev, pm.qgrid_d.p, pm.cgrid_d.p, pm.phi_d.p,
+        run_t<double, double, double>(pm, nl, atom_order, pos, charge, sys_dev, pm.qgrid_d.p, pm.cgrid_d.p, pm.phi_d.p,
                                       pm.roots_d.p, pm.moduli_d.p, force_d, energy_d, status, phases, s);
 }
 
