@@ -151,8 +151,10 @@ def test_larger_box_multi_chunk_paths():
 def test_superblock_search_paths():
     # 12,000 waters in 7.1 nm: 1,125 list blocks, above the 1,024 from which
     # clusters search 64-block superblocks first.  Forces against the oracle,
-    # graph steps against eager steps, and a batch that adds a small replica
-    # (which then takes the superblock path too) against lone contexts.
+    # graph steps against eager steps, and a batch against lone contexts: the
+    # batch builds superblock bounds for all systems, but the path is chosen
+    # per system, so the 300-water replica scans its blocks flat beside the
+    # large one's superblock search in the same launch.
     s = water(12000, 7.1, seed=41)
     small = water(300, 2.1, seed=42)
     a = ommhip.Context(s, temperature=300.0, friction=1.0, seed=8)
