@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define OMMHIP_ABI_VERSION 2
+#define OMMHIP_ABI_VERSION 3
 
 typedef enum omm_status {
     OMM_OK = 0,
@@ -441,17 +441,41 @@ omm_status omm_ctx_capacity(omm_ctx* ctx, int32_t* list_cap, int32_t* bin_cap, i
 
 /* Launch shapes picked by the autotuner when the context is first prepared
  * (no counterpart in the reference): the spread tile edge (grid points), the
- * lines per FFT workgroup and the pair kernel's grid, each the fastest of its
- * candidates on the context's own positions, with that candidate's time.
- * None of them changes a result bit. */
+ * FFT decomposition (1: the z and y transforms of each x plane in one
+ * workgroup, three launches per transform; 0: five line passes) with the
+ * lines per workgroup of the line passes, and the pair kernel's grid (the
+ * calibrated task count times pair_scale: 1, 2, or -2 for half), each the
+ * fastest of its candidates on the context's own positions, with that
+ * candidate's time.  None of them changes a result bit.  from_cache = 1 when
+ * the shapes came from the tuning cache (omm_set_tuning_cache) instead of
+ * being timed; the times are then those of the run that stored them. */
 typedef struct omm_launch_tuning {
     int32_t spread_tile_edge;
     int32_t fft_lines_per_workgroup;
     int32_t pair_grid;
     double spread_ms, fft_ms, pair_ms;
+    int32_t fft_plane;
+    int32_t pair_scale;
+    int32_t from_cache;
 } omm_launch_tuning;
 
 omm_status omm_ctx_launch_tuning(omm_ctx* ctx, omm_launch_tuning* out);
+
+/* Sets the launch shapes instead of the autotuner's (no counterpart in the
+ * reference): spread_tile_edge 6 or 8, fft_plane 0 or 1 (1 only where every
+ * system's x plane fits in LDS), fft_lines_per_workgroup 16, 8, 4 or 2,
+ * pair_scale 1, 2 or -2; the other fields are ignored.  Results do not
+ * change; OMM_ERR_INVALID for other values. */
+omm_status omm_ctx_set_launch_tuning(omm_ctx* ctx, const omm_launch_tuning* tuning);
+
+/* The process-wide tuning cache (no counterpart in the reference): a text
+ * file keyed by device (architecture and compute units) and workload
+ * (precision, method, cutoff, system sizes and grids).  A context or batch
+ * whose key is in the file applies the stored shapes without timing; every
+ * timed tuning appends its result.  NULL or "" turns the cache off.  Without
+ * this call the path comes from the environment variable
+ * OMMHIP_TUNING_CACHE, if set. */
+omm_status omm_set_tuning_cache(const char* path);
 
 /* Context::get_state (context.rs:930-1105) with positions, velocities, forces
  * and potential energy; any output pointer may be NULL. */
@@ -584,6 +608,7 @@ omm_status omm_batch_get_box(omm_batch* b, int32_t replica, double box[9]);
 omm_status omm_batch_list_stats(omm_batch* b, int64_t* tiles, int64_t* entries);
 omm_status omm_batch_profile(omm_batch* b, int32_t reps, double phase_ms[8], int64_t* num_pairs);
 omm_status omm_batch_launch_tuning(omm_batch* b, omm_launch_tuning* out);
+omm_status omm_batch_set_launch_tuning(omm_batch* b, const omm_launch_tuning* tuning);
 
 omm_status omm_batch_destroy(omm_batch* b);
 

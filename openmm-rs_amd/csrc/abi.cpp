@@ -389,8 +389,28 @@ void fill_tuning(const ommhip::NbOperator::Tuning& t, omm_launch_tuning* out) {
     out->spread_ms = t.spread_ms;
     out->fft_ms = t.fft_ms;
     out->pair_ms = t.pair_ms;
+    out->fft_plane = t.fft_plane;
+    out->pair_scale = t.pair_scale;
+    out->from_cache = t.from_cache;
+}
+
+void read_tuning(const omm_launch_tuning* in, ommhip::NbOperator::Tuning* t) {
+    if (!in) throw ommhip::Error(OMM_ERR_INVALID, "null tuning");
+    t->spread_edge = in->spread_tile_edge;
+    t->fft_cap = in->fft_lines_per_workgroup;
+    t->fft_plane = in->fft_plane;
+    t->pair_scale = in->pair_scale;
 }
 }  // namespace
+
+omm_status omm_ctx_set_launch_tuning(omm_ctx* ctx, const omm_launch_tuning* tuning) {
+    CTX_CALL(ommhip::NbOperator::Tuning t; read_tuning(tuning, &t); ctx->ctx.set_tuning(t));
+}
+
+omm_status omm_set_tuning_cache(const char* path) {
+    ommhip::set_tuning_cache_path(path);
+    return OMM_OK;
+}
 
 omm_status omm_ctx_launch_tuning(omm_ctx* ctx, omm_launch_tuning* out) {
     CTX_CALL(fill_tuning(ctx->ctx.tuning(), out));
@@ -603,6 +623,10 @@ omm_status omm_batch_list_stats(omm_batch* b, int64_t* tiles, int64_t* entries) 
 
 omm_status omm_batch_launch_tuning(omm_batch* b, omm_launch_tuning* out) {
     BATCH_CALL(fill_tuning(b->ctx.tuning(), out));
+}
+
+omm_status omm_batch_set_launch_tuning(omm_batch* b, const omm_launch_tuning* tuning) {
+    BATCH_CALL(ommhip::NbOperator::Tuning t; read_tuning(tuning, &t); b->ctx.set_tuning(t));
 }
 
 omm_status omm_batch_profile(omm_batch* b, int32_t reps, double phase_ms[8], int64_t* num_pairs) {

@@ -113,6 +113,9 @@ struct SysInfo {
     int dispersion;          // this table serves the LJPME dispersion mesh (its influence function)
     int roots_off[3], moduli_off[3];
     int fft_lines[2][3];  // lines per FFT workgroup along each axis, [0] FP32 / [1] FP64 transforms
+    // Plane transforms (one workgroup per x plane): z lines and y lines per
+    // fft_lds call, [0] FP32 / [1] FP64.
+    int plane_lines[2][2];
     int radix[3][MAX_STAGES];
     double alpha;
     // Volume-dependent host terms evaluated on the device:

@@ -164,6 +164,9 @@ public:
         prepare();
         return nb_->tuning();
     }
+    // Launch shapes set explicitly (NbOperator::apply_tuning); captured
+    // graphs are rebuilt.  Results do not change.
+    void set_tuning(const NbOperator::Tuning& t);
     // Per-phase average device times (ms): the six NbOperator::profile
     // phases, [6] bonds + angles, [7] LangevinMiddle update; plus the
     // in-cutoff pair count of the current positions.  State is preserved.

@@ -915,6 +915,13 @@ void MdContext::list_positions(int r, double* out) {
     OMM_HIP(hipStreamSynchronize(s_));
 }
 
+void MdContext::set_tuning(const NbOperator::Tuning& t) {
+    OMM_HIP(hipStreamSynchronize(s_));
+    prepare();
+    nb_->apply_tuning(t, pos_.p, status_.p, s_);
+    drop_graph();
+}
+
 void MdContext::drop_graph() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <mutex>
+#include <string>
 
 #include "ewald.h"
 #include "pme.h"
@@ -64,6 +65,10 @@ struct HostTerms {
     double self = 0.0;        // -k (alpha/sqrt(pi) sum q^2 + pi Q^2 / (2 alpha^2 V))
     double dispersion = 0.0;  // dispersion correction
 };
+
+// Path of the process-wide tuning cache (NULL or empty: none); replaces the
+// OMMHIP_TUNING_CACHE environment variable.
+void set_tuning_cache_path(const char* path);
 
 class NbOperator {
 public:
@@ -160,17 +165,28 @@ public:
     // longest j-list (host-side, outside graph capture).  Returns true when the
     // list storage was reallocated (captured graphs must then be rebuilt).
     bool calibrate(const double* pos_dev, unsigned* status, hipStream_t s);
+    // The meshes' own slot order from the current list (after a list build
+    // outside the step loop).
+    void take_mesh_order(hipStream_t s);
     // Launch-shape autotuner, run once by the first calibration: the spread
-    // tile edge, the FFT lines per workgroup and the pair kernel's grid, each
-    // picked by timing the candidates on the current positions.  None of
-    // them changes a result bit.
+    // tile edge, the FFT decomposition and lines per workgroup, and the pair
+    // kernel's grid, each picked by timing the candidates on the current
+    // positions, or taken from the tuning cache when it holds this device
+    // and workload.  None of them changes a result bit.
     void autotune(const double* pos_dev, unsigned* status, hipStream_t s);
     struct Tuning {
         int spread_edge = 0, fft_cap = 0, pair_grid = 0;
+        int fft_plane = 0;   // 1: plane passes around the x pass, 0: five line passes
         int pair_scale = 1;  // pair grid per calibrated task count: 1, 2 (doubled) or -2 (halved)
         double spread_ms = 0, fft_ms = 0, pair_ms = 0;
+        int from_cache = 0;  // taken from the tuning cache instead of timed
     };
     const Tuning& tuning() const { return tuning_; }
+    // Applies launch shapes (spread edge, FFT decomposition and lines, pair
+    // scale) without timing: the tuning cache's replay and a test hook.
+    // Throws OMM_ERR_INVALID for values outside the candidates, or for plane
+    // passes that do not fit this workload.
+    void apply_tuning(const Tuning& t, const double* pos_dev, unsigned* status, hipStream_t s);
     void force_rebuild(hipStream_t s);
     // Rebuild requests recomputed from these positions: cleared, then raised
     // for every system with an atom more than skin/2 from its position at
@@ -214,6 +230,10 @@ private:
     // Copies the launch layout fields of sys_h_ (spread tiles, FFT lines)
     // into the device table, keeping its boxes.
     void patch_layout(hipStream_t s);
+    // Key of this device and workload in the tuning cache.
+    std::string tuning_key() const;
+    // Pair grid for a calibrated task count and the tuned scale.
+    static int pair_grid_for(long long tasks, int scale);
     // Grows the spread bins until the positions fit.
     void fit_bins(const double* pos_dev, unsigned* status, hipStream_t s);
     void upload_parameters(hipStream_t s);
@@ -246,6 +266,7 @@ private:
     mutable std::vector<double> disp_coeff_;
     mutable std::vector<char> disp_cached_;
     int pair_grid_ = 1;
+    long long calibrated_tasks_ = 0;  // pair-kernel tasks of the calibrated list
     bool tuned_ = false;
     Tuning tuning_;
     hipStream_t stream_ = nullptr;

@@ -1,6 +1,8 @@
 // NonbondedForce operator: validation, parameter freezing and the evaluation
 // sequence (nonbonded.rs:771-1155) on one device.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <set>
 
@@ -789,12 +791,152 @@ void NbOperator::patch_layout(hipStream_t s) {
             d.tiles[a] = h.tiles[a];
             for (int p = 0; p < 2; ++p) d.fft_lines[p][a] = h.fft_lines[p][a];
         }
+        for (int p = 0; p < 2; ++p)
+            for (int k = 0; k < 2; ++k) d.plane_lines[p][k] = h.plane_lines[p][k];
     }
     sys_dev_.upload(dev, s);
 }
 
+int NbOperator::pair_grid_for(long long tasks, int scale) {
+    const long long scaled = scale > 1 ? tasks * scale : (scale < 0 ? tasks / 2 : tasks);
+    return (int)std::max<long long>(64, std::min<long long>(65535, scaled));
+}
+
+namespace {
+
+// The tuning cache: a text file of "key<TAB>edge cap plane scale spread_ms
+// fft_ms pair_ms" lines, appended by every timed autotune; the last line of
+// a key wins.  Its path is set by omm_set_tuning_cache, or taken from the
+// environment variable OMMHIP_TUNING_CACHE; empty: no cache.
+std::mutex g_cache_mutex;
+std::string g_cache_path;
+bool g_cache_path_set = false;
+
+std::string cache_path_locked() {
+    if (!g_cache_path_set) {
+        const char* env = std::getenv("OMMHIP_TUNING_CACHE");
+        g_cache_path = env ? env : "";
+        g_cache_path_set = true;
+    }
+    return g_cache_path;
+}
+
+bool cache_lookup(const std::string& key, NbOperator::Tuning& t) {
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    const std::string path = cache_path_locked();
+    if (path.empty()) return false;
+    std::FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    bool found = false;
+    char line[1024];
+    while (std::fgets(line, sizeof line, f)) {
+        char* tab = std::strchr(line, '\t');
+        if (!tab || std::string(line, tab) != key) continue;
+        NbOperator::Tuning v;
+        if (std::sscanf(tab + 1, "%d %d %d %d %lf %lf %lf", &v.spread_edge, &v.fft_cap, &v.fft_plane, &v.pair_scale,
+                        &v.spread_ms, &v.fft_ms, &v.pair_ms) == 7) {
+            t = v;
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void cache_store(const std::string& key, const NbOperator::Tuning& t) {
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    const std::string path = cache_path_locked();
+    if (path.empty()) return;
+    std::FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return;  // an unwritable cache only costs the next process a tuning run
+    std::fprintf(f, "%s\t%d %d %d %d %.6f %.6f %.6f\n", key.c_str(), t.spread_edge, t.fft_cap, t.fft_plane,
+                 t.pair_scale, t.spread_ms, t.fft_ms, t.pair_ms);
+    std::fclose(f);
+}
+
+}  // namespace
+
+void set_tuning_cache_path(const char* path) {
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    g_cache_path = path ? path : "";
+    g_cache_path_set = true;
+}
+
+std::string NbOperator::tuning_key() const {
+    int dev = 0;
+    OMM_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    OMM_HIP(hipGetDeviceProperties(&prop, dev));
+    // FNV-1a over the per-system sizes and grids.
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](long long v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (unsigned long long)((v >> (8 * b)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    for (size_t k = 0; k < sys_h_.size(); ++k) {
+        mix(sys_h_[k].n);
+        for (int a = 0; a < 3; ++a) mix(sys_h_[k].grid[a]);
+    }
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s/%d mixed=%d method=%d cutoff=%.6g systems=%zu atoms=%d layout=%016llx",
+                  prop.gcnArchName, prop.multiProcessorCount, mixed_ ? 1 : 0, (int)cfg_[0].method, cfg_[0].cutoff,
+                  sys_h_.size(), ntotal_, h);
+    return buf;
+}
+
+void NbOperator::apply_tuning(const Tuning& t, const double* pos_dev, unsigned* status, hipStream_t s) {
+    const NbSettings& c = cfg_[0];
+    // Shapes of phases this operator does not run are ignored.
+    if (c.pme_mesh()) {
+        if (t.spread_edge != 6 && t.spread_edge != 8)
+            throw Error(OMM_ERR_INVALID, "spread tile edge must be 6 or 8");
+        if (t.fft_cap != 16 && t.fft_cap != 8 && t.fft_cap != 4 && t.fft_cap != 2)
+            throw Error(OMM_ERR_INVALID, "FFT lines per workgroup must be 16, 8, 4 or 2");
+        if (t.fft_plane != 0 && t.fft_plane != 1) throw Error(OMM_ERR_INVALID, "FFT decomposition must be 0 or 1");
+    }
+    if (c.include_direct && t.pair_scale != 1 && t.pair_scale != 2 && t.pair_scale != -2)
+        throw Error(OMM_ERR_INVALID, "pair scale must be 1, 2 or -2");
+    if (c.pme_mesh()) {
+        const int prec = mesh_mode(false) == MESH_F32 ? 0 : 1;
+        if (!pme_set_fft_plane(pme_, sys_h_, prec, t.fft_plane) && t.fft_plane) {
+            pme_set_fft_plane(pme_, sys_h_, prec, tuning_.fft_plane);
+            throw Error(OMM_ERR_INVALID, "plane FFT passes do not fit this workload's grids in LDS");
+        }
+        pme_set_fft_lines(pme_, sys_h_, prec, t.fft_cap, s);
+        pme_set_spread_edge(pme_, sys_h_, t.spread_edge, s);
+        patch_layout(s);
+        fit_bins(pos_dev, status, s);
+    }
+    Tuning v = t;
+    if (!c.include_direct) v.pair_scale = 1;
+    v.pair_grid = pair_grid_ = pair_grid_for(calibrated_tasks_, v.pair_scale);
+    tuning_ = v;
+    tuned_ = true;
+    fill_async<unsigned>(status, 1, 0u, s);
+    OMM_HIP(hipStreamSynchronize(s));
+}
+
 void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s) {
     tuned_ = true;
+    {
+        const std::string key = tuning_key();
+        Tuning cached;
+        if (cache_lookup(key, cached)) {
+            cached.from_cache = 1;
+            bool ok = true;
+            try {
+                apply_tuning(cached, pos_dev, status, s);
+            } catch (const Error&) {
+                ok = false;  // an entry this build cannot apply: tune afresh
+            }
+            if (ok) {
+                tuning_.from_cache = 1;
+                return;
+            }
+        }
+    }
     hipEvent_t t0, t1;
     OMM_HIP(hipEventCreate(&t0));
     OMM_HIP(hipEventCreate(&t1));
@@ -846,18 +988,26 @@ void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s
         pme_set_spread_edge(pme_, sys_h_, tuning_.spread_edge, s);
         patch_layout(s);
         fit_bins(pos_dev, status, s);
-        // FFT lines per workgroup (the transforms are the same for any grouping).
+        // FFT decomposition (five line passes, or plane passes around the x
+        // pass) and lines per workgroup of the line passes: every candidate
+        // computes each line with the same arithmetic, so the results are
+        // the same bits.
         best = 1e30;
-        for (int cap : {16, 8, 4, 2}) {
-            pme_set_fft_lines(pme_, sys_h_, prec, cap, s);
-            patch_layout(s);
-            const double t = timed(phase(PME_FFT));
-            if (t < best) {
-                best = t;
-                tuning_.fft_cap = cap;
+        for (int plane : {0, 1}) {
+            if (!pme_set_fft_plane(pme_, sys_h_, prec, plane) && plane) continue;
+            for (int cap : {16, 8, 4, 2}) {
+                pme_set_fft_lines(pme_, sys_h_, prec, cap, s);
+                patch_layout(s);
+                const double t = timed(phase(PME_FFT));
+                if (t < best) {
+                    best = t;
+                    tuning_.fft_cap = cap;
+                    tuning_.fft_plane = plane;
+                }
             }
         }
         tuning_.fft_ms = best;
+        pme_set_fft_plane(pme_, sys_h_, prec, tuning_.fft_plane);
         pme_set_fft_lines(pme_, sys_h_, prec, tuning_.fft_cap, s);
         patch_layout(s);
     }
@@ -880,6 +1030,8 @@ void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s
         tuning_.pair_ms = best;
         pair_grid_ = tuning_.pair_grid;
     }
+    tuning_.from_cache = 0;
+    cache_store(tuning_key(), tuning_);
     fill_async<unsigned>(status, 1, 0u, s);
     OMM_HIP(hipStreamSynchronize(s));
     OMM_HIP(hipEventDestroy(t0));
@@ -910,9 +1062,8 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
         }
         // One wave per task of the calibrated list (grid-stride beyond it):
         // enough resident waves to hide the latency of the pair loop.
-        const long long scaled = tuning_.pair_scale > 1 ? tasks * tuning_.pair_scale
-                                                         : (tuning_.pair_scale < 0 ? tasks / 2 : tasks);
-        const int grid = (int)std::max<long long>(64, std::min<long long>(65535, scaled));
+        calibrated_tasks_ = tasks;
+        const int grid = pair_grid_for(tasks, tuning_.pair_scale);
         if (grid != pair_grid_) {
             pair_grid_ = grid;
             changed = true;  // captured graphs hold the old launch shape
@@ -929,6 +1080,7 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
         pp_.cap = nl_.cap;
         changed = true;
     }
+    take_mesh_order(s);
     if (!tuned_) {
         autotune(pos_dev, status, s);
         changed = true;
@@ -936,6 +1088,12 @@ bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t 
     fill_async<unsigned>(status, 1, 0u, s);
     OMM_HIP(hipStreamSynchronize(s));
     return changed;
+}
+
+void NbOperator::take_mesh_order(hipStream_t s) {
+    if (!cfg_[0].pme_mesh()) return;
+    pme_take_order(pme_, nl_, s);
+    if (ljpme()) pme_take_order(pme_lj_, nl_, s);
 }
 
 void NbOperator::list_stats(int64_t* tiles, int64_t* entries) {
@@ -972,6 +1130,7 @@ void NbOperator::set_capacity(int list_cap, int bin_cap, const double* src, unsi
         nlist_resize(nl_, cap);
         pp_.cap = nl_.cap;
         nlist_build(nl_, src, sys_dev_.p, status, s, true);
+        take_mesh_order(s);
     }
     if (bin_cap > 0 && cfg_[0].pme_mesh()) {
         pm_set_bin_cap(pme_, bin_cap);

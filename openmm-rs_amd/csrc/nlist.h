@@ -50,6 +50,7 @@ struct NlistSystem {
 
 struct NlistDev {
     int nsys = 0;
+    int generation = 0;  // advanced by every nlist_setup (slot layout changes)
     int natoms = 0, nslots = 0, nblocks = 0, nclusters = 0, nchunks = 0, cap = 0, nunits = 0, ncells = 0;
     int uniform = 0;  // atoms per unit when all units are the same size, else 0
     int closed = 0;   // every exception pair lies inside one unit

@@ -718,6 +718,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
                  double skin, hipStream_t s) {
     const int nsys = (int)systems.size();
     nl.nsys = nsys;
+    ++nl.generation;
     nl.cutoff = cutoff;
     nl.skin = skin;
     std::vector<int> eptr{0}, eidx, uptr{0}, uatoms, upos, usys, rank_all, block_sys, sorted, slot_of, atom_sys;

@@ -26,6 +26,15 @@ struct PmeDev {
     DevBuf<int> fft_off[2][5];
     int fft_blocks[2][5] = {};
     int fft_cap[2] = {};  // lines per FFT workgroup (at most), per precision
+    // Plane decomposition: the z and y transforms of each x plane in one
+    // workgroup (forward: z r2c then y; inverse: y then z c2r), so a
+    // transform is three launches instead of five.  Available per precision
+    // when every system's plane fits in LDS; results are the same bits as
+    // the five-pass form.
+    DevBuf<int> plane_off;  // nsys + 1 prefix of x planes
+    int plane_blocks = 0;
+    size_t plane_shmem[2] = {};  // dynamic LDS per plane workgroup, 0: unavailable
+    bool plane[2] = {};          // use the plane decomposition
     int spread_edge = 0;  // spread tile edge
     size_t fft_shmem[2][3] = {};  // dynamic LDS per axis
     // Spread: per-atom stencils (base point, weights) and per-tile atom bins.
@@ -37,6 +46,13 @@ struct PmeDev {
     DevBuf<int> st_idx;             // [natoms][4] stencil base point
     DevBuf<float> st_wf;            // [natoms][16] weights, FP32 mesh
     DevBuf<double> st_wd;           // [natoms][16] weights, FP64 mesh
+    // The mesh's own copy of a sorted slot order (slot -> atom, -1 padding),
+    // written by the gather from the list it follows and read by the next
+    // spread's binning.  The spread may then run while a list build rewrites
+    // nl.sorted_atom and still visit the atoms in a spatial order.  Valid for
+    // the list layout of generation order_gen.
+    DevBuf<int> order;
+    int order_gen = -1;
 };
 
 // Grid dimensions must each be >= 5 (pme.rs:222-225) and at most 1024.
@@ -48,6 +64,11 @@ void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* ato
 // lines per workgroup (cap <= 0: the default rule); writes sys[s].fft_lines,
 // which the caller uploads.  Returns the cap.  Results do not depend on it.
 int pme_set_fft_lines(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int cap, hipStream_t s);
+// FFT decomposition of precision prec: use = 1 the plane passes where
+// available, 0 the five line passes, < 0 the default (planes where
+// available).  Writes sys[s].plane_lines, which the caller uploads; returns
+// whether the plane passes are in use.  Results do not depend on it.
+bool pme_set_fft_plane(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int use);
 // Spread tile edge (6 or 8; <= 0: the default rule), with tiles and bins to
 // match; writes sys[s].ptile/tiles/tile_off.  Results do not depend on it.
 int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStream_t s);
@@ -56,6 +77,9 @@ int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStre
 void pme_grow_bins(PmeDev& pm);
 // Reallocates the bins to `cap` atoms per spread tile.
 void pm_set_bin_cap(PmeDev& pm, int cap);
+// Copies the list's sorted order into the mesh's own order (after a list
+// build outside the step loop).
+void pme_take_order(PmeDev& pm, const NlistDev& nl, hipStream_t s);
 
 // Spread, FFT, convolution (+energy into slot E_RECIP of each system), inverse
 // FFT, gather (forces added).  Atoms with zero charge are skipped as in the
@@ -72,9 +96,11 @@ enum : int { PME_SPREAD = 1, PME_FFT = 2, PME_GATHER = 4, PME_ALL = 7 };
 enum MeshMode : int { MESH_F32 = 0, MESH_F64_FIXED = 1, MESH_F64 = 2 };
 // The atoms are visited in the list's sorted slot order (nl.sorted_atom),
 // which only sets locality: any permutation gives the same fixed-point sums.
-// atom_order: the spread's binning visits the atoms in index order instead,
-// so the spread and the transforms may run while a list build rewrites the
-// sorted order (the gather always reads it and must follow the build).
+// atom_order: the spread's binning does not read nl.sorted_atom but the
+// mesh's own order (the sorted order of an earlier build, which any gather
+// refreshes), or index order before the first gather, so the spread and the
+// transforms may run while a list build rewrites the sorted order (the gather
+// always reads it and must follow the build).
 void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
              const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
              unsigned* status, hipStream_t s, int phases = PME_ALL, bool atom_order = false);

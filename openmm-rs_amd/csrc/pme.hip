@@ -256,7 +256,8 @@ __global__ __launch_bounds__(GATHER_ATOMS * 5) void k_gather5(const int* __restr
                                                               const double* __restrict__ pos,
                                                               const double* __restrict__ charge,
                                                               const SysInfo* __restrict__ sys,
-                                                              const R* __restrict__ phi, A* force, unsigned* status) {
+                                                              const R* __restrict__ phi, A* force, unsigned* status,
+                                                              int* order_out) {
     __shared__ R s_g[GATHER_ATOMS][5][3];
     const int si = find_sys(chunk_off, nsys, blockIdx.x);
     const SysInfo& S = sys[si];
@@ -266,6 +267,9 @@ __global__ __launch_bounds__(GATHER_ATOMS * 5) void k_gather5(const int* __restr
     const int k = t / 5, ix = t - k * 5;
     const int slot = S.slot_off + (blockIdx.x - S.chunk_off) * GATHER_ATOMS + k;
     const int a = slot < S.slot_off + S.nblocks * IBLOCK ? order[slot] : -1;
+    // The chunks of each system cover all of its slots, so the copy of the
+    // order is complete.
+    if (order_out && ix == 0 && slot < S.slot_off + S.nblocks * IBLOCK) order_out[slot] = a;
     const double q = a >= 0 ? charge[a] : 0.0;
     const Box& b = S.box;
     R g0 = 0, g1 = 0, g2 = 0;
@@ -338,6 +342,9 @@ __device__ __forceinline__ int div_small(int x, int d, float inv_d) {
 // k n/(Ns R) + q m stays below n, so the root index wraps with one conditional
 // subtraction.  Each output is computed by one thread in a fixed order, so
 // the result does not depend on T or on how lines are grouped.
+__device__ __forceinline__ float fused(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fused(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <class R>
 __device__ Cx<R>* fft_lds(Cx<R>* a, Cx<R>* b, int n, const int* radices, const Cx<R>* roots, bool inverse, int T) {
     const int li = threadIdx.x / T, t = threadIdx.x - li * T;
@@ -362,8 +369,10 @@ __device__ Cx<R>* fft_lds(Cx<R>* a, Cx<R>* b, int n, const int* radices, const C
                 const Cx<R> w = roots[idx];
                 const R wy = sgn * w.y;
                 const Cx<R> v = in[j + r * m];
-                ax += v.x * w.x - v.y * wy;
-                ay += v.x * wy + v.y * w.x;
+                // Explicit fused multiply-adds: every kernel that inlines
+                // this routine rounds the same way.
+                ax = fused(v.x, w.x, fused(-v.y, wy, ax));
+                ay = fused(v.x, wy, fused(v.y, w.x, ay));
                 idx += base;
                 if (idx >= n) idx -= n;
             }
@@ -580,6 +589,134 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
     }
 }
 
+// Plane passes: one workgroup per x plane of a system transforms the plane
+// along z and along y in LDS.  Forward (DIR 0): real z lines to the half
+// spectrum (kept as P[kz][y]), then y lines, stored as cgrid[x][ky][kz].
+// Inverse (DIR 1): y lines of the half spectrum, then the z lines rebuilt
+// from it by Hermitian symmetry and transformed to real values.  Every line
+// goes through fft_lds with the data the corresponding line pass would load,
+// so the results are the same bits as the five-pass form.
+//
+// LDS (complex elements): roots along z [nz] and y [ny], P and Q
+// [lyp x ny] (lyp = nzh rounded up to a multiple of the y lines per call),
+// and the z-line ping-pong buffers [2 x lz x nz], which share the one of P
+// and Q that does not hold live data when they fit in it.
+constexpr int PLANE_THREADS = 512;
+constexpr size_t PLANE_LDS_MAX = 160 * 1024 - 256;  // gfx950 LDS per workgroup, less the static arrays
+
+struct PlaneLayout {
+    int lyp;
+    size_t pq;      // elements of P (and of Q)
+    bool z_shared;  // the z buffers fit in the free one of P and Q
+    size_t elems;   // total complex elements
+};
+
+__host__ __device__ inline PlaneLayout plane_layout(int ny, int nz, int nzh, int lz, int ly) {
+    PlaneLayout L;
+    L.lyp = (nzh + ly - 1) / ly * ly;
+    L.pq = (size_t)L.lyp * ny;
+    const size_t zb = 2 * (size_t)lz * nz;
+    L.z_shared = zb <= L.pq;
+    L.elems = (size_t)nz + ny + 2 * L.pq + (L.z_shared ? 0 : zb);
+    return L;
+}
+
+template <class R, class G, int DIR>
+__global__ __launch_bounds__(PLANE_THREADS) void k_fft_plane(const int* __restrict__ plane_off, int nsys,
+                                                            const SysInfo* __restrict__ sys, G* __restrict__ qgrid,
+                                                            Cx<R>* cgrid, R* phi,
+                                                            const Cx<R>* __restrict__ roots_all, int* clear,
+                                                            int nclear) {
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ int s_rz[MAX_STAGES], s_ry[MAX_STAGES];
+    // The forward pass also clears the spread's bin counts (every spread
+    // workgroup is done by now), as the first line pass does.
+    if (DIR == 0 && clear)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nclear; i += gridDim.x * blockDim.x) clear[i] = 0;
+    const int si = find_sys(plane_off, nsys, blockIdx.x);
+    const SysInfo& S = sys[si];
+    const int ny = S.grid[1], nz = S.grid[2], nzh = S.nzh;
+    const int x = blockIdx.x - plane_off[si];
+    constexpr int prec = sizeof(R) == 8 ? 1 : 0;
+    const int lz = S.plane_lines[prec][0], ly = S.plane_lines[prec][1];
+    const PlaneLayout L = plane_layout(ny, nz, nzh, lz, ly);
+    Cx<R>* rz = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* ry = rz + nz;
+    Cx<R>* P = ry + ny;
+    Cx<R>* Q = P + L.pq;
+    Cx<R>* extra = Q + L.pq;
+    if (threadIdx.x < MAX_STAGES) {
+        s_rz[threadIdx.x] = S.radix[2][threadIdx.x];
+        s_ry[threadIdx.x] = S.radix[1][threadIdx.x];
+    }
+    for (int i = threadIdx.x; i < nz; i += PLANE_THREADS) rz[i] = roots_all[S.roots_off[2] + i];
+    for (int i = threadIdx.x; i < ny; i += PLANE_THREADS) ry[i] = roots_all[S.roots_off[1] + i];
+    qgrid += S.qgrid_off + (size_t)x * ny * nz;
+    phi += S.qgrid_off + (size_t)x * ny * nz;
+    cgrid += S.cgrid_off + (size_t)x * ny * nzh;
+    const int Tz = PLANE_THREADS / lz, Ty = PLANE_THREADS / ly;
+
+    if (DIR == 0) {
+        Cx<R>* Z = L.z_shared ? Q : extra;
+        __syncthreads();  // roots and radices
+        for (int y0 = 0; y0 < ny; y0 += lz) {
+            for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
+                const int li = e / nz, el = e - li * nz;
+                Cx<R> v = {0, 0};
+                if (y0 + li < ny) v.x = (R)grid_real(qgrid[(size_t)(y0 + li) * nz + el]);
+                Z[e] = v;
+            }
+            __syncthreads();
+            const Cx<R>* res = fft_lds<R>(Z, Z + (size_t)lz * nz, nz, s_rz, rz, false, Tz);
+            for (int e = threadIdx.x; e < lz * nzh; e += PLANE_THREADS) {
+                const int li = e / nzh, kz = e - li * nzh;
+                if (y0 + li < ny) P[(size_t)kz * ny + y0 + li] = res[(size_t)li * nz + kz];
+            }
+            __syncthreads();
+        }
+        const Cx<R>* res = P;
+        for (int k0 = 0; k0 < L.lyp; k0 += ly)
+            res = fft_lds<R>(P + (size_t)k0 * ny, Q + (size_t)k0 * ny, ny, s_ry, ry, false, Ty) - (size_t)k0 * ny;
+        for (int e = threadIdx.x; e < ny * nzh; e += PLANE_THREADS) {
+            const int ky = e / nzh, kz = e - ky * nzh;
+            cgrid[e] = res[(size_t)kz * ny + ky];
+        }
+    } else {
+        for (int e = threadIdx.x; e < ny * nzh; e += PLANE_THREADS) {
+            const int ky = e / nzh, kz = e - ky * nzh;
+            P[(size_t)kz * ny + ky] = cgrid[e];
+        }
+        __syncthreads();
+        const Cx<R>* res = P;
+        for (int k0 = 0; k0 < L.lyp; k0 += ly)
+            res = fft_lds<R>(P + (size_t)k0 * ny, Q + (size_t)k0 * ny, ny, s_ry, ry, true, Ty) - (size_t)k0 * ny;
+        Cx<R>* Z = L.z_shared ? (res == P ? Q : P) : extra;
+        for (int y0 = 0; y0 < ny; y0 += lz) {
+            for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
+                const int li = e / nz, el = e - li * nz;
+                const int y = y0 + li;
+                Cx<R> v = {0, 0};
+                if (y < ny) {
+                    if (el < nzh) {
+                        v = res[(size_t)el * ny + y];
+                    } else {
+                        v = res[(size_t)(nz - el) * ny + y];
+                        v.y = -v.y;
+                    }
+                }
+                Z[e] = v;
+            }
+            __syncthreads();
+            const Cx<R>* out = fft_lds<R>(Z, Z + (size_t)lz * nz, nz, s_rz, rz, true, Tz);
+            for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
+                const int li = e / nz, el = e - li * nz;
+                if (y0 + li < ny) phi[(size_t)(y0 + li) * nz + el] = out[e].x;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 std::vector<int> factorize(int n) {
     std::vector<int> r;
     while (n % 8 == 0) {
@@ -622,6 +759,15 @@ void launch_fft(const PmeDev& pm, const SysInfo* sys, G* qgrid, Cx<R>* cgrid, R*
                        reinterpret_cast<const Cx<R>*>(roots), moduli, energy, status, clear, nclear);
 }
 
+template <class R, class G, int DIR>
+void launch_plane(const PmeDev& pm, const SysInfo* sys, G* qgrid, Cx<R>* cgrid, R* phi, const R* roots,
+                  hipStream_t s, int* clear = nullptr, int nclear = 0) {
+    constexpr int prec = sizeof(R) == 8 ? 1 : 0;
+    hipLaunchKernelGGL((k_fft_plane<R, G, DIR>), dim3(pm.plane_blocks), dim3(PLANE_THREADS), pm.plane_shmem[prec], s,
+                       pm.plane_off.p, pm.nsys, sys, qgrid, cgrid, phi, reinterpret_cast<const Cx<R>*>(roots), clear,
+                       nclear);
+}
+
 template <class R>
 R* pme_weights(PmeDev& pm);
 template <>
@@ -641,12 +787,13 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
     // Bin counts are zero between runs: set so at allocation, and cleared by
     // the first FFT pass after each spread (or by a fill when a run spreads
     // without transforming).
+    const bool own_order = pm.order_gen == nl.generation && pm.order.n == (size_t)nl.nslots;
     if (phases & PME_SPREAD) {
         R* st_w = pme_weights<R>(pm);
-        const int nitems = atom_order ? pm.natoms : nl.nslots;
-        hipLaunchKernelGGL((k_pme_bin<R>), dim3((nitems + 255) / 256), dim3(256), 0, s, nitems,
-                           atom_order ? (const int*)nullptr : nl.sorted_atom.p, pos, charge, pm.atom_sys, sys,
-                           pm.st_idx.p, st_w, pm.bin_count.p, pm.bins.p, pm.bin_cap, status);
+        const int* order = atom_order ? (own_order ? pm.order.p : nullptr) : nl.sorted_atom.p;
+        const int nitems = order ? nl.nslots : pm.natoms;
+        hipLaunchKernelGGL((k_pme_bin<R>), dim3((nitems + 255) / 256), dim3(256), 0, s, nitems, order, pos, charge,
+                           pm.atom_sys, sys, pm.st_idx.p, st_w, pm.bin_count.p, pm.bins.p, pm.bin_cap, status);
         hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
                            pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap, qgrid,
                            pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0, status);
@@ -654,16 +801,29 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
     }
     if (phases & PME_FFT) {
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);
-        launch_fft<R, G, A, MODE_Z_R2C>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s,
-                                        (phases & PME_SPREAD) ? pm.bin_count.p : nullptr, pm.ntiles);
-        launch_fft<R, G, A, MODE_Y_FWD>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
-        launch_fft<R, G, A, MODE_X_CONV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
-        launch_fft<R, G, A, MODE_Y_INV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
-        launch_fft<R, G, A, MODE_Z_C2R>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        int* clear = (phases & PME_SPREAD) ? pm.bin_count.p : nullptr;
+        constexpr int prec = sizeof(R) == 8 ? 1 : 0;
+        if (pm.plane[prec]) {
+            launch_plane<R, G, 0>(pm, sys, qgrid, cg, phi, roots, s, clear, pm.ntiles);
+            launch_fft<R, G, A, MODE_X_CONV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+            launch_plane<R, G, 1>(pm, sys, qgrid, cg, phi, roots, s);
+        } else {
+            launch_fft<R, G, A, MODE_Z_R2C>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s, clear,
+                                            pm.ntiles);
+            launch_fft<R, G, A, MODE_Y_FWD>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+            launch_fft<R, G, A, MODE_X_CONV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+            launch_fft<R, G, A, MODE_Y_INV>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+            launch_fft<R, G, A, MODE_Z_C2R>(pm, sys, qgrid, cg, phi, roots, moduli, energy, status, s);
+        }
     }
-    if (phases & PME_GATHER)
+    if (phases & PME_GATHER) {
+        // The order buffer is allocated by pme_take_order (never here: this
+        // may run under graph capture).
+        const bool copy = pm.order.n == (size_t)nl.nslots;
         hipLaunchKernelGGL((k_gather5<R, A>), dim3(nl.nchunks), dim3(GATHER_ATOMS * 5), 0, s, nl.chunk_off.p, pm.nsys,
-                           nl.sorted_atom.p, pos, charge, sys, phi, force, status);
+                           nl.sorted_atom.p, pos, charge, sys, phi, force, status, copy ? pm.order.p : nullptr);
+        if (copy) pm.order_gen = nl.generation;
+    }
     OMM_HIP(hipGetLastError());
 }
 
@@ -729,6 +889,13 @@ void pm_set_bin_cap(PmeDev& pm, int cap) {
 
 void pme_grow_bins(PmeDev& pm) { pm_set_bin_cap(pm, 2 * pm.bin_cap); }
 
+void pme_take_order(PmeDev& pm, const NlistDev& nl, hipStream_t s) {
+    if (nl.nslots == 0 || nl.sorted_atom.n < (size_t)nl.nslots) return;
+    if (pm.order.n != (size_t)nl.nslots) pm.order.alloc(nl.nslots);
+    OMM_HIP(hipMemcpyAsync(pm.order.p, nl.sorted_atom.p, sizeof(int) * nl.nslots, hipMemcpyDeviceToDevice, s));
+    pm.order_gen = nl.generation;
+}
+
 namespace {
 const int PASS_AXIS[5] = {2, 1, 0, 1, 2};  // MODE_Z_R2C, Y_FWD, X_CONV, Y_INV, Z_C2R
 
@@ -776,6 +943,40 @@ int pme_set_fft_lines(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int cap, 
     }
     pm.fft_cap[prec] = cap;
     return cap;
+}
+
+bool pme_set_fft_plane(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int use) {
+    const size_t csize = prec ? 2 * sizeof(double) : 2 * sizeof(float);
+    size_t shmem = 0;
+    bool ok = use != 0 && !sys.empty();
+    for (SysInfo& S : sys) {
+        if (!ok) break;
+        const int ny = S.grid[1], nz = S.grid[2], nzh = S.nzh;
+        // y lines per call: all of the half spectrum's lines when the threads
+        // of a line stay at most ny, fewer calls otherwise; z lines per call:
+        // as many as fit, down to the fewest that keep the threads of a line
+        // at most nz.
+        auto pow2_at_least = [](int v) {
+            int p = 1;
+            while (p < v) p <<= 1;
+            return p;
+        };
+        const int ly = std::min(PLANE_THREADS, std::max(pow2_at_least(nzh), pow2_at_least((PLANE_THREADS + ny - 1) / ny)));
+        const int lz_min = std::min(PLANE_THREADS, pow2_at_least((PLANE_THREADS + nz - 1) / nz));
+        int lz = std::min(PLANE_THREADS, std::max(lz_min, pow2_at_least(ny)));
+        while (lz > lz_min && plane_layout(ny, nz, nzh, lz, ly).elems * csize > PLANE_LDS_MAX) lz >>= 1;
+        const size_t bytes = plane_layout(ny, nz, nzh, lz, ly).elems * csize;
+        if (bytes > PLANE_LDS_MAX) {
+            ok = false;
+            break;
+        }
+        S.plane_lines[prec][0] = lz;
+        S.plane_lines[prec][1] = ly;
+        shmem = std::max(shmem, bytes);
+    }
+    pm.plane[prec] = ok;
+    pm.plane_shmem[prec] = ok ? shmem : 0;
+    return ok;
 }
 
 int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStream_t s) {
@@ -854,6 +1055,27 @@ void pme_setup(PmeDev& pm, std::vector<SysInfo>& sys, int natoms, const int* ato
     // pass, and the largest spread tile edge with SPREAD_MIN_TILES tiles.
     pme_set_fft_lines(pm, sys, 0, 0, s);
     pme_set_fft_lines(pm, sys, 1, 0, s);
+    {
+        std::vector<int> poff{0};
+        for (const SysInfo& S : sys) poff.push_back(poff.back() + S.grid[0]);
+        pm.plane_off.upload(poff, s);
+        pm.plane_blocks = poff.back();
+        // Dynamic LDS above 64 KB per workgroup.
+        const int lds = (int)PLANE_LDS_MAX;
+        OMM_HIP(hipFuncSetAttribute((const void*)&k_fft_plane<float, long long, 0>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        OMM_HIP(hipFuncSetAttribute((const void*)&k_fft_plane<float, long long, 1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        OMM_HIP(hipFuncSetAttribute((const void*)&k_fft_plane<double, long long, 0>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        OMM_HIP(hipFuncSetAttribute((const void*)&k_fft_plane<double, long long, 1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        OMM_HIP(hipFuncSetAttribute((const void*)&k_fft_plane<double, double, 0>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        OMM_HIP(hipFuncSetAttribute((const void*)&k_fft_plane<double, double, 1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        for (int prec = 0; prec < 2; ++prec) pme_set_fft_plane(pm, sys, prec, -1);
+    }
     pme_set_spread_edge(pm, sys, 0, s);
     pm.st_idx.alloc(4 * (size_t)pm.natoms);
     if (mixed) pm.st_wf.alloc(16 * (size_t)pm.natoms);

@@ -53,11 +53,28 @@ _ip = C.POINTER(C.c_int32)
 
 
 class LaunchTuning(C.Structure):
+    """ctypes image of ``omm_launch_tuning``."""
+
     _fields_ = [("spread_tile_edge", C.c_int32), ("fft_lines_per_workgroup", C.c_int32), ("pair_grid", C.c_int32),
-                ("spread_ms", C.c_double), ("fft_ms", C.c_double), ("pair_ms", C.c_double)]
+                ("spread_ms", C.c_double), ("fft_ms", C.c_double), ("pair_ms", C.c_double),
+                ("fft_plane", C.c_int32), ("pair_scale", C.c_int32), ("from_cache", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def of(cls, spread_tile_edge, fft_lines_per_workgroup, fft_plane, pair_scale=1):
+        t = cls()
+        t.spread_tile_edge = spread_tile_edge
+        t.fft_lines_per_workgroup = fft_lines_per_workgroup
+        t.fft_plane = fft_plane
+        t.pair_scale = pair_scale
+        return t
+
+
+def set_tuning_cache(path):
+    """Process-wide tuning cache file (None: off); see omm_set_tuning_cache."""
+    _check(library().omm_set_tuning_cache(None if path is None else str(path).encode()))
 
 
 class BarostatInfo(C.Structure):
@@ -860,6 +877,11 @@ class Context:
         _check(library().omm_ctx_launch_tuning(self._h, C.byref(t)))
         return t.as_dict()
 
+    def set_launch_tuning(self, spread_tile_edge, fft_lines_per_workgroup, fft_plane, pair_scale=1):
+        """Launch shapes instead of the autotuner's (results do not change)."""
+        t = LaunchTuning.of(spread_tile_edge, fft_lines_per_workgroup, fft_plane, pair_scale)
+        _check(library().omm_ctx_set_launch_tuning(self._h, C.byref(t)))
+
     def get_state(self):
         pos = np.zeros((self.n, 3))
         vel = np.zeros((self.n, 3))
@@ -1045,6 +1067,10 @@ class Batch:
         t = LaunchTuning()
         _check(library().omm_batch_launch_tuning(self._h, C.byref(t)))
         return t.as_dict()
+
+    def set_launch_tuning(self, spread_tile_edge, fft_lines_per_workgroup, fft_plane, pair_scale=1):
+        t = LaunchTuning.of(spread_tile_edge, fft_lines_per_workgroup, fft_plane, pair_scale)
+        _check(library().omm_batch_set_launch_tuning(self._h, C.byref(t)))
 
     def profile(self, reps=20):
         ms = np.zeros(8)

@@ -121,3 +121,14 @@ def test_no_device_calls_fail_cleanly_without_gpu():
     with pytest.raises(ommhip.OmmError) as e:
         op.initialize(s.box)
     assert e.value.status == 5
+
+
+def test_tuning_cache_path_is_host_state(tmp_path):
+    # Setting and clearing the cache path touches no device.
+    ommhip.set_tuning_cache(tmp_path / "t.txt")
+    ommhip.set_tuning_cache(None)
+    assert ctypes_sizeof_tuning() == 3 * 4 + 4 + 3 * 8 + 3 * 4 + 4
+
+
+def ctypes_sizeof_tuning():
+    return C.sizeof(ommhip.LaunchTuning)
