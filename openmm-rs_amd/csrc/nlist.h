@@ -29,7 +29,6 @@ constexpr int SUPER = 64;   // blocks per superblock (the coarse level of the ca
 constexpr int SUPER_MIN = 1024;  // systems with more blocks search superblocks first
 constexpr int JTILE = 64;   // j-slots per tile
 constexpr int TPT = 4;      // tiles of one cluster per pair-kernel task
-constexpr int MAXEX = 8;    // exclusion partners per atom handled by the pair kernel
 constexpr int PME_CHUNK = 64;  // sorted slots per PME spread/gather workgroup
 
 // Squared displacement of an atom from its position at the last list build,
@@ -59,7 +58,12 @@ struct NlistDev {
     DevBuf<int> sorted_units, unit_size_sorted, unit_offset;
     DevBuf<int> sorted_atom, atom_slot, block_sys;
     DevBuf<int> block_off, chunk_off;  // per-system prefix tables (nsys + 1)
-    DevBuf<int> excl_ptr, excl_idx, excl_sorted;
+    // Exclusion partners (exception pairs, both directions) by atom, and per
+    // sorted slot the lowest and highest slot of its atom's partners ({INT_MAX,
+    // -1} for none), which lets the list build skip entries that cannot hold
+    // an exclusion with a cluster.
+    DevBuf<int> excl_ptr, excl_idx;
+    DevBuf<int2> excl_range;
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
     DevBuf<int> block_minimg;
@@ -73,6 +77,9 @@ struct NlistDev {
     // Completion counter of the merged key-and-scan kernel (zero between builds).
     DevBuf<int> done;
     DevBuf<int2> clist;         // cluster lists [nclusters][cap] of (sorted slot, atom)
+    // Per list entry, bit k: the cluster's k-th atom and the entry's atom are
+    // an excluded pair (any number of exclusions per atom).
+    DevBuf<unsigned char> clist_ex;
     DevBuf<int> ccount;
     DevBuf<int> tasks, ntasks;  // pair-kernel tasks (cluster << 8 | first tile) and their count
     // Direct-space forces by sorted slot, one component after another

@@ -299,7 +299,7 @@ __device__ __forceinline__ void place_unit(int p, const int* __restrict__ sorted
                                            const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
                                            const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
                                            const int* __restrict__ excl_idx, int closed, int* sorted_atom,
-                                           int* atom_slot, int* excl_sorted, const int* flag) {
+                                           int* atom_slot, int2* excl_range, const int* flag) {
     int u = sorted_units[p];
     const int si = unit_sys[u];
     if (!flag[1 + si]) return;  // same keys, same slots: nothing changes
@@ -311,19 +311,13 @@ __device__ __forceinline__ void place_unit(int p, const int* __restrict__ sorted
         sorted_atom[slot] = a;
         atom_slot[a] = slot;
         if (closed) {
-            int v[MAXEX];
-            for (int m = 0; m < MAXEX; ++m) v[m] = INT_MAX;
-            int cnt = 0;
-            for (int e = excl_ptr[a]; e < excl_ptr[a + 1] && cnt < MAXEX; ++e) {
-                int x = off + unit_pos[excl_idx[e]];
-                int j = cnt++;
-                while (j > 0 && v[j - 1] > x) {
-                    v[j] = v[j - 1];
-                    --j;
-                }
-                v[j] = x;
+            int lo = INT_MAX, hi = -1;
+            for (int e = excl_ptr[a]; e < excl_ptr[a + 1]; ++e) {
+                const int x = off + unit_pos[excl_idx[e]];
+                lo = min(lo, x);
+                hi = max(hi, x);
             }
-            for (int m = 0; m < MAXEX; ++m) excl_sorted[(size_t)slot * MAXEX + m] = v[m];
+            excl_range[slot] = make_int2(lo, hi);
         }
     }
 }
@@ -332,19 +326,19 @@ __device__ __forceinline__ void place_unit(int p, const int* __restrict__ sorted
 // (system, cell), so the atoms before unit p are those of the earlier systems
 // plus the earlier units of its own system; uniform > 0: every unit has that
 // many atoms (no size scan).  When every exception partner of an atom lies in
-// its own unit (closed units, e.g. rigid or flexible water), its sorted
-// exclusion list is written here as well.
+// its own unit (closed units, e.g. rigid or flexible water), the slot range of
+// its partners is written here as well.
 __global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* __restrict__ unit_offset, int uniform,
                               const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
                               const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
                               const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
                               const int* __restrict__ excl_idx, int closed, int nunits, int* sorted_atom,
-                              int* atom_slot, int* excl_sorted, const int* flag) {
+                              int* atom_slot, int2* excl_range, const int* flag) {
     if (!flag[0]) return;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nunits) return;
     place_unit(p, sorted_units, unit_offset, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx,
-               closed, sorted_atom, atom_slot, excl_sorted, flag);
+               closed, sorted_atom, atom_slot, excl_range, flag);
 }
 
 // Uniform units: one thread sorts the units of one cell and places them (no
@@ -354,7 +348,7 @@ __global__ void k_sort_place(const int* __restrict__ cell_start, int ncells, int
                              const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
                              const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
                              const int* __restrict__ excl_idx, int closed, int* sorted_atom, int* atom_slot,
-                             int* excl_sorted, const int* flag) {
+                             int2* excl_range, const int* flag) {
     if (!flag[0]) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
@@ -362,32 +356,27 @@ __global__ void k_sort_place(const int* __restrict__ cell_start, int ncells, int
     sort_cell(sorted_units, lo, hi);
     for (int p = lo; p < hi; ++p)
         place_unit(p, sorted_units, nullptr, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx,
-                   closed, sorted_atom, atom_slot, excl_sorted, flag);
+                   closed, sorted_atom, atom_slot, excl_range, flag);
 }
 
-__global__ void k_excl_sorted(const int* __restrict__ sorted_atom, const int* __restrict__ atom_slot,
-                              const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx,
-                              const int* __restrict__ block_sys, int npad, int* excl_sorted, const int* flag) {
+// Slot range of each slot's exclusion partners once every atom is placed
+// (units that do not hold all of their atoms' partners).
+__global__ void k_excl_range(const int* __restrict__ sorted_atom, const int* __restrict__ atom_slot,
+                             const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx,
+                             const int* __restrict__ block_sys, int npad, int2* excl_range, const int* flag) {
     if (!flag[0]) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= npad) return;
     if (!flag[1 + block_sys[s / IBLOCK]]) return;
-    int v[MAXEX];
-    for (int k = 0; k < MAXEX; ++k) v[k] = INT_MAX;
-    int a = sorted_atom[s];
-    if (a >= 0) {
-        int cnt = 0;
-        for (int k = excl_ptr[a]; k < excl_ptr[a + 1] && cnt < MAXEX; ++k) {
-            int x = atom_slot[excl_idx[k]];
-            int j = cnt++;
-            while (j > 0 && v[j - 1] > x) {
-                v[j] = v[j - 1];
-                --j;
-            }
-            v[j] = x;
+    int lo = INT_MAX, hi = -1;
+    const int a = sorted_atom[s];
+    if (a >= 0)
+        for (int k = excl_ptr[a]; k < excl_ptr[a + 1]; ++k) {
+            const int x = atom_slot[excl_idx[k]];
+            lo = min(lo, x);
+            hi = max(hi, x);
         }
-    }
-    for (int k = 0; k < MAXEX; ++k) excl_sorted[(size_t)s * MAXEX + k] = v[k];
+    excl_range[s] = make_int2(lo, hi);
 }
 
 // Bounding box of superblock sb (SUPER consecutive blocks of system si) from
@@ -495,9 +484,30 @@ __global__ __launch_bounds__(64) void k_super_bounds(int nsuper, const int* __re
 // sums; one atomic per workgroup reserves the runs of its clusters).  The
 // atoms of four passing blocks are loaded together, so their loads overlap.
 // Unflagged systems keep their lists and only append their tasks.
+// Exclusion data the list build reads to mark excluded pairs per entry.
+struct ExclLookup {
+    const int* atom_slot;
+    const int* ptr;
+    const int* idx;
+    const int2* range;
+    // Bit k set: the atom at slot first + k is an exclusion partner of atom
+    // a (at slot `slot`); every partner is checked, however many there are.
+    __device__ __forceinline__ unsigned bits(int slot, int a, int first) const {
+        const int2 r = range[slot];
+        if (r.y < first || r.x >= first + CLUSTER) return 0u;
+        unsigned m = 0u;
+        for (int k = ptr[a]; k < ptr[a + 1]; ++k) {
+            const int d = atom_slot[idx[k]] - first;
+            if (d >= 0 && d < CLUSTER) m |= 1u << d;
+        }
+        return m;
+    }
+};
+
 __device__ __forceinline__ void append_entry(int slot, int atom, const double p[3], int first, const double cq[3],
                                              const double hq[3], const Box& b, double reach2, int cap,
-                                             int2* out, int& count, unsigned long long below) {
+                                             int2* out, unsigned char* out_ex, const ExclLookup& ex, int& count,
+                                             unsigned long long below) {
     bool ok = false;
     if (atom >= 0 && slot >= first) {
         if (slot < first + CLUSTER) {
@@ -510,7 +520,10 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
     const unsigned long long m = __ballot(ok);
     if (ok) {
         const int w = count + __popcll(m & below);
-        if (w < cap) out[w] = make_int2(slot, atom);
+        if (w < cap) {
+            out[w] = make_int2(slot, atom);
+            out_ex[w] = (unsigned char)ex.bits(slot, atom, first);
+        }
     }
     count += __popcll(m);
 }
@@ -535,8 +548,10 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     const double* __restrict__ pos, const int* __restrict__ sorted_atom, const int* __restrict__ block_sys,
     const SysInfo* __restrict__ sys, const double* __restrict__ center, const float* __restrict__ half,
     const int* __restrict__ super_off, const double* __restrict__ super_center,
-    const float* __restrict__ super_half, double reach, int cap, int nclusters, int2* clist, int* ccount,
-    int* tasks, int* ntasks, unsigned* status, const int* flag) {
+    const float* __restrict__ super_half, const int* __restrict__ atom_slot, const int* __restrict__ excl_ptr,
+    const int* __restrict__ excl_idx, const int2* __restrict__ excl_range, double reach, int cap, int nclusters,
+    int2* clist, unsigned char* clist_ex, int* ccount, int* tasks, int* ntasks, unsigned* status,
+    const int* flag) {
     __shared__ int s_runs[CLIST_WAVES];
     __shared__ int s_base;
     if (!flag[0]) return;
@@ -556,6 +571,8 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
         const double reach2 = reach * reach;
         const unsigned long long below = (1ull << lane) - 1ull;
         int2* out = clist + (size_t)C * cap;
+        unsigned char* out_ex = clist_ex + (size_t)C * cap;
+        const ExclLookup ex{atom_slot, excl_ptr, excl_idx, excl_range};
 
         // The rest of block I from the cluster on: lanes 0 .. 31 - 8 q.
         const int rest = (I + 1) * IBLOCK - first;
@@ -587,7 +604,7 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
             cq[k] = rp[k] + 0.5 * (lo + hi);
             hq[k] = 0.5 * (hi - lo) * (1.0 + 1e-6) + 1e-6;
         }
-        append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, count, below);
+        append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below);
 
         // Blocks [J0, min(J0 + 64, jlim)) whose boxes come within reach.
         auto scan_blocks = [&](const int J0, const int jlim) {
@@ -616,8 +633,8 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
                     for (int k = 0; k < 3; ++k) pa[k] = pos[3 * atom_a + k];
                 if (atom_b >= 0)
                     for (int k = 0; k < 3; ++k) pb[k] = pos[3 * atom_b + k];
-                append_entry(slot_a, atom_a, pa, first, cq, hq, b, reach2, cap, out, count, below);
-                append_entry(slot_b, atom_b, pb, first, cq, hq, b, reach2, cap, out, count, below);
+                append_entry(slot_a, atom_a, pa, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below);
+                append_entry(slot_b, atom_b, pb, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below);
             }
         };
         if (S.nblocks > SUPER_MIN) {
@@ -740,9 +757,6 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
             adj[exc2[e]].push_back(exc1[e]);
         }
         for (int i = 0; i < n; ++i) {
-            if ((int)adj[i].size() > MAXEX)
-                throw Error(OMM_ERR_INVALID, "device path supports at most " + std::to_string(MAXEX) +
-                                                 " exceptions per particle");
             for (int j : adj[i]) eidx.push_back(natoms + j);
             eptr.push_back((int)eidx.size());
         }
@@ -878,7 +892,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.excl_ptr.upload(eptr, s);
     if (eidx.empty()) eidx.push_back(0);
     nl.excl_idx.upload(eidx, s);
-    nl.excl_sorted.alloc((size_t)nl.nslots * MAXEX);
+    nl.excl_range.alloc((size_t)nl.nslots);
     nl.block_center.alloc(3 * (size_t)nblocks);
     nl.block_half.alloc(3 * (size_t)nblocks);
     nl.block_minimg.alloc(nblocks);
@@ -891,6 +905,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     nl.pos_build.alloc(3 * (size_t)natoms);
     nl.cap = std::min(nl.cap, 255 * JTILE);
     nl.clist.alloc((size_t)nl.nclusters * nl.cap);
+    nl.clist_ex.alloc((size_t)nl.nclusters * nl.cap);
     nl.tasks.alloc((size_t)nl.nclusters * ((nl.cap / JTILE + TPT - 1) / TPT));
     nl.ntasks.alloc(1);
     nl.ccount.alloc(nl.nclusters);
@@ -909,6 +924,7 @@ void nlist_resize(NlistDev& nl, int cap) {
     if (cap / JTILE > 255) throw Error(OMM_ERR_INVALID, "neighbour list needs more than 255 tiles per cluster");
     nl.cap = cap;
     nl.clist.alloc((size_t)nl.nclusters * nl.cap);
+    nl.clist_ex.alloc((size_t)nl.nclusters * nl.cap);
     nl.tasks.alloc((size_t)nl.nclusters * ((nl.cap / JTILE + TPT - 1) / TPT));
 }
 
@@ -948,7 +964,7 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
         hipLaunchKernelGGL(k_sort_place, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p,
                            nl.ncells, nl.sorted_units.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p,
                            nl.unit_sys.p, sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.sorted_atom.p,
-                           nl.atom_slot.p, nl.excl_sorted.p, flag);
+                           nl.atom_slot.p, nl.excl_range.p, flag);
     } else {
         hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p,
                            nl.ncells, nl.sorted_units.p, nl.unit_ptr.p,
@@ -958,12 +974,12 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
         hipLaunchKernelGGL(k_place_atoms, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.sorted_units.p,
                            nl.unit_offset.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p,
                            nl.unit_sys.p, sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.nunits,
-                           nl.sorted_atom.p, nl.atom_slot.p, nl.excl_sorted.p, flag);
+                           nl.sorted_atom.p, nl.atom_slot.p, nl.excl_range.p, flag);
     }
     if (!nl.closed)
-        hipLaunchKernelGGL(k_excl_sorted, dim3(blocks_for(nl.nslots, 256)), dim3(256), 0, s, nl.sorted_atom.p,
+        hipLaunchKernelGGL(k_excl_range, dim3(blocks_for(nl.nslots, 256)), dim3(256), 0, s, nl.sorted_atom.p,
                            nl.atom_slot.p, nl.excl_ptr.p, nl.excl_idx.p, nl.block_sys.p, nl.nslots,
-                           nl.excl_sorted.p, flag);
+                           nl.excl_range.p, flag);
     double reach = nl.cutoff + nl.skin;
     hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p,
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
@@ -973,7 +989,8 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
                            sys_dev, nl.block_center.p, nl.block_half.p, nl.super_center.p, nl.super_half.p, flag);
     hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
                        nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p,
-                       nl.super_off.p, nl.super_center.p, nl.super_half.p, reach, nl.cap, nl.nclusters, nl.clist.p,
+                       nl.super_off.p, nl.super_center.p, nl.super_half.p, nl.atom_slot.p, nl.excl_ptr.p,
+                       nl.excl_idx.p, nl.excl_range.p, reach, nl.cap, nl.nclusters, nl.clist.p, nl.clist_ex.p,
                        nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
     OMM_HIP(hipGetLastError());
 }

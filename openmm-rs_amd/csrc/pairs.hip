@@ -160,7 +160,8 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                                               const R* __restrict__ sh, const R* __restrict__ e2,
                                               const double* __restrict__ qk_d, const double* __restrict__ sh_d,
                                               const double* __restrict__ e2_d, const int* __restrict__ sorted_atom,
-                                              const int* __restrict__ excl_sorted, const double* __restrict__ center,
+                                              const unsigned char* __restrict__ clist_ex,
+                                              const double* __restrict__ center,
                                               const int* __restrict__ minimg, const int2* __restrict__ clist,
                                               const int* __restrict__ ccount, const int* __restrict__ tasks,
                                               const int* __restrict__ ntasks, const int* __restrict__ block_sys,
@@ -176,6 +177,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
     __shared__ Q2<R> s_se[JTILE];
     __shared__ __align__(16) int s_js[JTILE];
     __shared__ int s_ja[JTILE];
+    __shared__ __align__(8) unsigned char s_ex[JTILE];  // exclusion bits of each entry (bit k: i-atom k)
     __shared__ Q4<double> s_pd[E64 ? JTILE : 1];
     __shared__ Q2<double> s_sd[E64 ? JTILE : 1];
 
@@ -207,14 +209,20 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
         const Box& bx = sys[sidx].box;
         const double c0 = center[3 * I], c1 = center[3 * I + 1], c2 = center[3 * I + 2];
         const int2* list = clist + (size_t)C * P.cap + (size_t)first * JTILE;
+        const unsigned char* list_ex = clist_ex + (size_t)C * P.cap + (size_t)first * JTILE;
         const int nentries = count - first * JTILE;
         auto entry = [&](int tl) {
             const int e = tl * JTILE + lane;
             return (tl < ntile && e < nentries) ? list[e] : make_int2(INT_MAX, -1);
         };
+        auto entry_ex = [&](int tl) -> unsigned {
+            const int e = tl * JTILE + lane;
+            return (tl < ntile && e < nentries) ? list_ex[e] : 0u;
+        };
         // Software pipeline: the entries of tile k + 2 and the coordinates of
         // tile k + 1 are in flight while tile k is evaluated.
         int2 e0 = entry(0), e1 = entry(1);
+        unsigned x0 = entry_ex(0), x1 = entry_ex(1);
         EntryRaw<R, E64> r0;
         load_entry<R, E64>(e0, pos, qk, sh, e2, qk_d, sh_d, e2_d, r0);
 
@@ -222,8 +230,6 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
         const int ia = sorted_atom[is];  // -1 in the padding of a system's last block
         R xi = 0, yi = 0, zi = 0, qi = 0, si = 0, ei = 0;
         double xid = 0, yid = 0, zid = 0, qid = 0, sid = 0, eid = 0;
-        int ex[MAXEX];
-        for (int m = 0; m < MAXEX; ++m) ex[m] = INT_MAX;
         if (ia >= 0) {
             double d[3] = {pos[3 * ia] - c0, pos[3 * ia + 1] - c1, pos[3 * ia + 2] - c2};
             if (P.periodic) wrap_nearest(d, bx);
@@ -241,14 +247,6 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 sid = sh_d[ia];
                 eid = e2_d[ia];
             }
-            const int4* ex4 = reinterpret_cast<const int4*>(excl_sorted + (size_t)is * MAXEX);
-            for (int q = 0; q < MAXEX / 4; ++q) {
-                const int4 v = ex4[q];
-                ex[4 * q] = v.x;
-                ex[4 * q + 1] = v.y;
-                ex[4 * q + 2] = v.z;
-                ex[4 * q + 3] = v.w;
-            }
         }
         const bool mi = P.periodic && minimg[I];
         R bv[9], ib0 = 0, ib4 = 0, ib8 = 0;
@@ -264,6 +262,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
         unsigned long long npair = 0, hsum = 0;
         for (int k = 0; k < ntile; ++k) {
             const int2 e2n = entry(k + 2);
+            const unsigned x2n = entry_ex(k + 2);
             EntryRaw<R, E64> r1;
             load_entry<R, E64>(e1, pos, qk, sh, e2, qk_d, sh_d, e2_d, r1);
 
@@ -287,6 +286,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 s_se[lane] = se;
                 s_js[lane] = e0.x;
                 s_ja[lane] = e0.y;
+                s_ex[lane] = (unsigned char)x0;
                 if (E64) {
                     s_pd[lane] = pd;
                     s_sd[lane] = sd;
@@ -303,16 +303,16 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
 #pragma unroll
                 for (int s8 = 0; s8 < CLUSTER; ++s8)
                     if (jv[s8] != INT_MAX && jv[s8] > is) vmask |= 1u << s8;
-                if (vmask) {
-                    for (int m = 0; m < MAXEX; ++m) {
-                        const int e = ex[m];
-                        if (e == INT_MAX) break;
-                        if (e <= is || e < jv[0] || e > jv[CLUSTER - 1]) continue;
-#pragma unroll
-                        for (int s8 = 0; s8 < CLUSTER; ++s8)
-                            if (jv[s8] == e) vmask &= ~(1u << s8);
-                    }
-                }
+                // Bit il of each of the group's eight exclusion bytes, gathered
+                // into bits 0-7: each byte's bit il moves to the byte's bit 0,
+                // and a multiply by 2^0 + 2^7 + 2^14 + 2^21 carries the bits of
+                // bytes 0-3 (positions 0, 8, 16, 24) to positions 21-24
+                // without collisions.
+                const unsigned long long xb = *reinterpret_cast<const unsigned long long*>(s_ex + grp * CLUSTER);
+                const unsigned long long sel = (xb >> il) & 0x0101010101010101ull;
+                const unsigned lo4 = (((unsigned)sel * 0x00204081u) >> 21) & 0xFu;
+                const unsigned hi4 = (((unsigned)(sel >> 32) * 0x00204081u) >> 21) & 0xFu;
+                vmask &= ~(lo4 | (hi4 << 4));
             }
 
             R gx = 0, gy = 0, gz = 0;
@@ -493,6 +493,8 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             __syncthreads();
             e0 = e1;
             e1 = e2n;
+            x0 = x1;
+            x1 = x2n;
             r0 = r1;
         }
         // i-forces: the eight groups' partial sums, in a fixed order.
@@ -582,7 +584,7 @@ void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* s
                 const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
                 unsigned* status, unsigned long long* stats, int grid, bool combine, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
-                       part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.excl_sorted.p, nl.block_center.p,
+                       part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.clist_ex.p, nl.block_center.p,
                        nl.block_minimg.p, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, nl.block_sys.p, sys_dev, p,
                        slot_buffer<A>(nl), energy, status, stats, nl.flag.p, nl.nsys + 1);
     if (combine)

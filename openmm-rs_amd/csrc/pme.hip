@@ -601,7 +601,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
 // [lyp x ny] (lyp = nzh rounded up to a multiple of the y lines per call),
 // and the z-line ping-pong buffers [2 x lz x nz], which share the one of P
 // and Q that does not hold live data when they fit in it.
-constexpr int PLANE_THREADS = 512;
+constexpr int PLANE_THREADS = 1024;
 constexpr size_t PLANE_LDS_MAX = 160 * 1024 - 256;  // gfx950 LDS per workgroup, less the static arrays
 
 struct PlaneLayout {

@@ -306,3 +306,78 @@ def test_ljpme_md_context_with_barostat():
     assert info["total_accepted"] > 0
     assert force_rms_rel(st["forces"], F_ref) <= FORCE_TOL
     assert energy_rel(st["potential_energy"], E_ref) <= ENERGY_TOL
+
+
+def _chain_system(nchains=60, length=24, edge=3.0, seed=5):
+    """Branched chains (bonds i-(i+1) and i-(i+2)): with 1-2, 1-3 and scaled
+    1-4 exceptions (nonbonded.rs:434-500) an inner atom has 12 exclusion
+    partners.  Chains are random walks of 0.14 nm steps that keep 0.18 nm
+    (minimum image) from every atom but the two before them, restarting a
+    chain that runs into a dead end."""
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((0, 3))
+
+    def clear_of(cand, others):
+        if not len(others):
+            return True
+        d = others - cand
+        d -= edge * np.round(d / edge)
+        return np.min(np.einsum("ij,ij->i", d, d)) >= 0.18 ** 2
+
+    while len(pos) < nchains * length:
+        chain = [rng.uniform(0, edge, 3)]
+        if not clear_of(chain[0], pos):
+            continue
+        tries = 0
+        while len(chain) < length and tries < 200:
+            tries += 1
+            step = rng.normal(size=3)
+            cand = chain[-1] + 0.14 * step / np.linalg.norm(step)
+            far = np.concatenate([pos, np.array(chain[:-1]).reshape(-1, 3)[:-1]])
+            if clear_of(cand, far):
+                chain.append(cand)
+        if len(chain) == length:
+            pos = np.concatenate([pos, np.array(chain)])
+    pos = pos % edge
+    n = len(pos)
+    f = ommhip.NonbondedForce()
+    f.set_nonbonded_method("PME")
+    f.set_cutoff_distance(0.9)
+    charges = rng.normal(scale=0.3, size=n)
+    charges -= charges.mean()
+    for i in range(n):
+        f.add_particle(float(charges[i]), 0.22, 0.4)
+    bonds = []
+    for c in range(nchains):
+        b = c * length
+        bonds += [(b + k, b + k + 1) for k in range(length - 1)]
+        bonds += [(b + k, b + k + 2) for k in range(length - 2)]
+    f.create_exceptions_from_bonds(bonds, 0.8333, 0.5)
+    box = np.eye(3) * edge
+    alpha, grid = orc.effective_pme_parameters(5e-4, 0.9, box)
+    f.set_pme_parameters(alpha, *grid)
+    return f, pos, box
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [ommhip.PRECISION_MIXED, ommhip.PRECISION_DOUBLE])
+def test_many_exclusions_per_particle(precision):
+    """More than eight exclusion partners per particle: the list marks every
+    excluded pair per entry, so the count per particle is not limited."""
+    f, pos, box = _chain_system()
+    partners = [set() for _ in range(len(pos))]
+    for a, b, *_ in f.exceptions:
+        partners[a].add(b)
+        partners[b].add(a)
+    assert max(len(p) for p in partners) == 12
+    desc = f.desc()
+    op = ommhip.GpuNonbondedForce(f, precision)
+    op.initialize(box)
+    F, E = op.calc_forces_and_energy(pos, box)
+    F_ref, E_ref, _ = orc.nb_evaluate(desc, pos, box)
+    frel, erel = force_rms_rel(F, F_ref), energy_rel(E, E_ref)
+    print(f"exceptions={len(f.exceptions)} force_rms_rel={frel:.3e} energy_rel={erel:.3e}")
+    ftol, etol = (FORCE_TOL, ENERGY_TOL) if precision == ommhip.PRECISION_MIXED else (1e-9, 1e-9)
+    assert frel <= ftol
+    assert erel <= etol
+    assert op.pair_stats(pos, box) == orc.nb_pairs(desc, pos, box)
