@@ -11,6 +11,8 @@
 
 #include "status.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #define OMM_HIP(call)                                                                           \
     do {                                                                                        \
         hipError_t e_ = (call);                                                                 \
@@ -19,6 +21,17 @@
     } while (0)
 
 namespace ommhip {
+
+// A roctx range over a host-side scope (rocprofv3 --marker-trace shows them
+// beside the kernels): step chunks, graph capture and replay, overflow
+// replays, preparation, autotuning, evaluations and queries.  Without a
+// profiler attached a range costs two calls that return at once.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 // 1/(4 pi eps0) in kJ/mol nm / e^2 (nonbonded.rs:16).
 constexpr double ONE_4PI_EPS0 = 138.93545764438198;

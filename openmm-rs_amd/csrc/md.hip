@@ -848,6 +848,7 @@ void MdContext::set_positions(int r, const double* pos) {
 // integrators would raise at the current positions.
 void MdContext::prepare() {
     if (!positions_dirty_) return;
+    TraceRange range("omm.prepare");
     const double* built = nb_->pos_build();
     for (int r = 0; r < nsys_; ++r)
         if (!touched_[r] && built) {
@@ -1379,6 +1380,7 @@ void MdContext::kinetic_tensor(std::vector<double>& out) {
 }
 
 void MdContext::current_pressure(double* out) {
+    TraceRange range("omm.pressure");
     // pressure.rs:14-42
     require_barostat(0, "MonteCarloBarostat");
     std::vector<double> box(9 * (size_t)nsys_);
@@ -1766,6 +1768,7 @@ void MdContext::restore_snapshot() {
 // (the phase is aligned before the graph is replayed); longer periods use
 // attempt-free graphs of graph_steps_ steps between eager attempt steps.
 void MdContext::capture_graph() {
+    TraceRange range("omm.capture_graph");
     OMM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     try {
         enqueue_snapshot(s_);
@@ -1809,6 +1812,7 @@ void MdContext::set_integrator(double temperature, double friction, double step_
 // lists the snapshot had and replays the steps eagerly: the trajectory is that
 // of a run that never overflowed (the noise is counter-based).
 void MdContext::run_steps(int count, bool graph) {
+    TraceRange range(graph ? "omm.graph_steps" : "omm.eager_steps");
     BaroDev& B = baro_;
     const int phase0 = B.phase;
     // The attempt opens the step that completes a period (barostat.rs:129-135).
@@ -1829,6 +1833,7 @@ void MdContext::run_steps(int count, bool graph) {
     unsigned st = read_status();
     for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
         if (attempt == 8) check_status();  // throws
+        TraceRange replay("omm.overflow_replay");
         ++overflow_replays_;
         drop_graph();
         nb_->grow(st & ST_BIN_OVERFLOW);  // the lists are sized by the rebuild from the snapshot
@@ -1841,6 +1846,7 @@ void MdContext::run_steps(int count, bool graph) {
 
 void MdContext::step(int nsteps) {
     if (nsteps < 0) throw Error(OMM_ERR_INVALID, "step count must be nonnegative");
+    TraceRange range("omm.step");
     prepare();
     const BaroDev& B = baro_;
     const int len = graph_length();
@@ -1872,6 +1878,7 @@ void MdContext::step(int nsteps) {
 }
 
 void MdContext::evaluate(double* forces, double* energy, bool want_energy) {
+    TraceRange range("omm.evaluate");
     prepare();
     for (int attempt = 0;; ++attempt) {
         clear_accumulators(s_);

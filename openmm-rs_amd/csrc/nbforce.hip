@@ -919,6 +919,7 @@ void NbOperator::apply_tuning(const Tuning& t, const double* pos_dev, unsigned* 
 }
 
 void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s) {
+    TraceRange range("omm.autotune");
     tuned_ = true;
     {
         const std::string key = tuning_key();
@@ -1039,6 +1040,7 @@ void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s
 }
 
 bool NbOperator::calibrate(const double* pos_dev, unsigned* status, hipStream_t s) {
+    TraceRange range("omm.calibrate");
     if (ntotal_ == 0) return false;
     fit_bins(pos_dev, status, s);
     if (!cfg_[0].include_direct) return false;
@@ -1141,6 +1143,7 @@ void NbOperator::set_capacity(int list_cap, int bin_cap, const double* src, unsi
 
 double NbOperator::mesh_charges(const double* pos, const double* box, double alpha, const double* charges,
                                 double* forces, const int* atoms, int count, double* derivatives) {
+    TraceRange range(count > 0 ? "omm.pme_charge_derivatives" : "omm.pme_evaluate_charges");
     if (!initialized_ || !device_ready_) throw Error(OMM_ERR_STATE, "PME operator not initialized");
     if (cfg_.size() != 1 || cfg_[0].method != OMM_NB_PME || mixed_)
         throw Error(OMM_ERR_INVALID, "the charge mesh runs on a one-system plain-FP64 PME operator");
@@ -1353,6 +1356,7 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
 
 void NbOperator::compute(const double* pos, const double* box, bool direct, bool recip, double* forces,
                          double* energy, double parts[5], int64_t* num_pairs, uint64_t* pair_hash) {
+    TraceRange range("omm.nb_evaluate");
     if (!initialized_) throw Error(OMM_ERR_STATE, "NonbondedForce used before initialize");
     if (cfg_.size() != 1) throw Error(OMM_ERR_STATE, "host-pointer evaluation needs a one-system operator");
     if (!pos) throw Error(OMM_ERR_INVALID, "positions must not be null");
