@@ -17,7 +17,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libommhip.so")
+# OMMHIP_LIBRARY selects another build of the same library (the sanitizer
+# build of the CPU tests); there is no fallback to anything else.
+LIB_PATH = os.environ.get("OMMHIP_LIBRARY") or os.path.join(_HERE, "libommhip.so")
 
 NO_CUTOFF, CUTOFF_NON_PERIODIC, CUTOFF_PERIODIC, EWALD, PME, LJPME = range(6)
 METHODS = {
