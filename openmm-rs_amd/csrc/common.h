@@ -52,6 +52,7 @@ enum : unsigned {
     ST_BIN_OVERFLOW = 8u,  // a PME spread tile received more atoms than its bin holds
     ST_CUTOFF_BOX = 16u,   // a box edge fell below twice the cutoff
     ST_CONSTRAINTS = 32u,  // a constraint projection was singular or did not converge
+    ST_VBOX = 64u,         // an atom left the region its non-periodic system's virtual box serves
 };
 
 // Energy slots of the device energy accumulator (fixed point).
@@ -135,6 +136,11 @@ struct SysInfo {
     // E_self = self_const + neutral_coeff / V, E_dispersion = disp_coeff / V.
     double self_const, neutral_coeff, disp_coeff;
     unsigned long long seed;  // integrator noise stream of this system
+    // Non-periodic methods on a virtual box (vcheck = 1): the integrators
+    // raise ST_VBOX when an atom leaves [vlo, vhi], the region within which
+    // every displacement the list and kernels see is the true one.
+    int vcheck;
+    double vlo[3], vhi[3];
 };
 
 // Largest s in [0, count) with off[s] <= x, for x in [off[0], off[count]).

@@ -164,6 +164,12 @@ public:
         prepare();
         return nb_->tuning();
     }
+    // Non-periodic methods: chooses the virtual box again when the current
+    // positions (and, when given, a second configuration on the device) do
+    // not fit it, and centres the integrators' region on them; true when the
+    // device was set up afresh (its lists must be built).
+    bool refit_virtual_box(const double* extra_dev);
+    void require_periodic_for_barostat() const;
     // Launch shapes set explicitly (NbOperator::apply_tuning); captured
     // graphs are rebuilt.  Results do not change.
     void set_tuning(const NbOperator::Tuning& t);
@@ -180,6 +186,7 @@ public:
     void project_constraints(double tolerance, bool velocities);
 
 private:
+    double default_box_[9] = {};  // a non-periodic system's default box vectors
     void enqueue_forces(hipStream_t s, bool energy, int rebuild);
     void enqueue_forces_into(hipStream_t s, bool energy, int rebuild, long long* force_fx, double* force_d,
                              long long* energy_fx, double* energy_d);

@@ -192,15 +192,23 @@ __device__ __forceinline__ void step_normals(int i, const SysInfo& S, unsigned l
 // neighbour list's rebuild flags (any, and the atom's system) when an atom
 // moved more than skin/2 from its position at the last build, so the next
 // step's build kernels rebuild that system.
+// A non-periodic system's atom outside the region its virtual box serves.
+__device__ __forceinline__ bool left_region(const SysInfo& S, const double* x) {
+    if (!S.vcheck) return false;
+    bool out = false;
+    for (int a = 0; a < 3; ++a) out = out || !(x[a] >= S.vlo[a] && x[a] <= S.vhi[a]);
+    return out;
+}
+
 template <class A>
 __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__ vel, A* __restrict__ force,
                            const double* __restrict__ inv_mass, const int* __restrict__ atom_sys,
                            const SysInfo* __restrict__ sys, double dt, double decay, double variance,
                            const unsigned long long* __restrict__ step, int step_offset,
                            const double* __restrict__ pos_build, double lim2, int check, int* flag,
-                           const unsigned char* __restrict__ constrained) {
+                           const unsigned char* __restrict__ constrained, unsigned* status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool moved = false;
+    bool moved = false, outside = false;
     int si = 0;
     if (i < n && !(constrained && constrained[i])) {  // constrained atoms: k_constrained_step
         double im = inv_mass[i];
@@ -227,9 +235,11 @@ __global__ void k_lfmiddle(int n, double* __restrict__ pos, double* __restrict__
                 d[a] = x - pos_build[k];
             }
             moved = check && !(skin_d2(d[0], d[1], d[2]) <= lim2);
+            outside = left_region(sys[si], pos + 3 * i);
         }
     }
     wave_raise_flags(flag, moved, si);
+    if (__ballot(outside) && (threadIdx.x & 63) == 0) atomicOr(status, ST_VBOX);
 }
 
 // One SHAKE (velocities = false) or RATTLE (velocities = true) projection of
@@ -341,6 +351,7 @@ __global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, 
                 }
             }
             ok = ok && project_component(k0, k1, c1, c2, d2, w1, w2, prev, pos, false, tolerance);
+            bool outside = false;
             for (int m = a0; m < a1; ++m) {
                 const int i = atoms[m];
                 double d[3];
@@ -350,7 +361,9 @@ __global__ void k_constrained_step(int ncomp, const int* __restrict__ atom_off, 
                     d[a] = pos[k] - pos_build[k];
                 }
                 moved = moved || (check && !(skin_d2(d[0], d[1], d[2]) <= lim2));
+                outside = outside || left_region(sys[si], pos + 3 * i);
             }
+            if (outside) atomicOr(status, ST_VBOX);
         }
         if (!ok) atomicOr(status, ST_CONSTRAINTS);
     }
@@ -723,10 +736,16 @@ MdContext::MdContext(const omm_md_desc* descs, int nsys, int precision) {
     }
     mixed_ = precision == OMM_PRECISION_MIXED;
     nb_.reset(new NbOperator(nbd.data(), nsys, precision));
-    if (!nb_->settings().periodic())
-        throw Error(OMM_ERR_INVALID, "the device MD step runs the periodic methods (PME, CutoffPeriodic); "
-                                     "NoCutoff and CutoffNonPeriodic are evaluated by the NonbondedForce operator");
     nb_->initialize(ns.data(), boxes.data());
+    if (nb_->is_virtual()) {
+        // NoCutoff / CutoffNonPeriodic: the periodic machinery on a virtual
+        // box, set up here for all atoms at the origin and chosen again from
+        // the positions when they are set (prepare).  The descriptor's box
+        // stays the system's default box vectors (context.rs:93).
+        std::vector<double> origin(3 * (size_t)ns[0], 0.0);
+        nb_->fit_virtual_box(origin.data(), ns[0]);
+        for (int k = 0; k < 9; ++k) default_box_[k] = descs[0].box[k];
+    }
     s_ = nb_->stream();
     n_ = nb_->total_particles();
     std::vector<double> inv;
@@ -846,9 +865,30 @@ void MdContext::set_positions(int r, const double* pos) {
 // others are rebuilt from the positions of their current lists, which
 // reproduces those lists.  The rebuild requests are then those the
 // integrators would raise at the current positions.
+void MdContext::require_periodic_for_barostat() const {
+    // context.rs:93-101
+    if (nb_->is_virtual()) throw Error(OMM_ERR_INVALID, "a barostat cannot be used with a non-periodic system");
+}
+
+bool MdContext::refit_virtual_box(const double* extra_dev) {
+    const size_t n3 = 3 * (size_t)n_;
+    std::vector<double> h(extra_dev ? 2 * n3 : n3);
+    OMM_HIP(hipMemcpyAsync(h.data(), pos_.p, n3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    if (extra_dev) OMM_HIP(hipMemcpyAsync(h.data() + n3, extra_dev, n3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    OMM_HIP(hipStreamSynchronize(s_));
+    if (!nb_->fit_virtual_box(h.data(), (int)(h.size() / 3))) return false;
+    drop_graph();  // the device was set up afresh
+    return true;
+}
+
 void MdContext::prepare() {
     if (!positions_dirty_) return;
     TraceRange range("omm.prepare");
+    if (nb_->is_virtual() && refit_virtual_box(nullptr)) {
+        // New lists, built from the current positions.
+        OMM_HIP(hipMemcpyAsync(list_src_.p, pos_.p, 3 * (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+        std::fill(touched_.begin(), touched_.end(), 1);
+    }
     const double* built = nb_->pos_build();
     for (int r = 0; r < nsys_; ++r)
         if (!touched_[r] && built) {
@@ -868,6 +908,7 @@ void MdContext::set_state(int r, int64_t step, const double* box, const double* 
     if (r < 0 || r >= nsys_) throw Error(OMM_ERR_INVALID, "replica index out of range");
     if (step < 0) throw Error(OMM_ERR_INVALID, "step count must be nonnegative");
     if (list_pos && !pos) throw Error(OMM_ERR_INVALID, "list positions are restored together with positions");
+    if (box && nb_->is_virtual()) throw Error(OMM_ERR_INVALID, "a non-periodic system takes no box vectors");
     const int n = nb_->num_particles(r);
     for (const double* a : {pos, vel, list_pos})
         if (a)
@@ -1139,6 +1180,7 @@ void MdContext::enqueue_barostat(hipStream_t s) {
 }
 
 void MdContext::set_barostat(double pressure, double temperature, int frequency, uint64_t seed) {
+    require_periodic_for_barostat();
     // barostat.rs:65-86, 205-218
     if (!std::isfinite(pressure)) throw Error(OMM_ERR_INVALID, "pressure must be finite");
     if (!(std::isfinite(temperature) && temperature >= 0.0))
@@ -1223,6 +1265,7 @@ void MdContext::set_barostat(double pressure, double temperature, int frequency,
 
 void MdContext::set_anisotropic_barostat(const double pressure[3], double temperature, int scale_axes, int frequency,
                                          uint64_t seed) {
+    require_periodic_for_barostat();
     // anisotropic_barostat.rs:42-74, 217-230
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(pressure[a])) throw Error(OMM_ERR_INVALID, "pressure must be finite");
@@ -1235,6 +1278,7 @@ void MdContext::set_anisotropic_barostat(const double pressure[3], double temper
 }
 
 void MdContext::set_flexible_barostat(double pressure, double temperature, int frequency, uint64_t seed) {
+    require_periodic_for_barostat();
     // Its trial boxes are triclinic, which the Ewald sum does not take
     // (nonbonded.rs:1087-1097).
     if (nb_->settings().method == OMM_NB_EWALD)
@@ -1252,6 +1296,7 @@ void MdContext::set_flexible_barostat(double pressure, double temperature, int f
 
 void MdContext::set_membrane_barostat(double pressure, double surface_tension, double temperature, int xy_mode,
                                       int z_mode, int frequency, uint64_t seed) {
+    require_periodic_for_barostat();
     // membrane_barostat.rs:61-103, 278-292
     if (!std::isfinite(surface_tension)) throw Error(OMM_ERR_INVALID, "surface tension must be finite");
     if (xy_mode < 0 || xy_mode > 1) throw Error(OMM_ERR_INVALID, "xy mode must be 0 (isotropic) or 1 (anisotropic)");
@@ -1544,11 +1589,11 @@ void MdContext::enqueue_step(hipStream_t s, int rebuild, bool barostat, int offs
     if (mixed_)
         hipLaunchKernelGGL(k_lfmiddle<long long>, dim3(blocks), dim3(256), 0, s, n_, pos_.p, vel_.p, force_fx_.p,
                            inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance, step_dev_.p,
-                           offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
+                           offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained, status_.p);
     else
         hipLaunchKernelGGL(k_lfmiddle<double>, dim3(blocks), dim3(256), 0, s, n_, pos_.p, vel_.p, force_d_.p,
                            inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance, step_dev_.p,
-                           offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained);
+                           offset, nb_->pos_build(), lim * lim, check, nb_->list_flag(), constrained, status_.p);
     if (C.ncomp) {
         const int cb = (C.ncomp + 63) / 64;
         auto go = [&](auto* force) {
@@ -1727,6 +1772,10 @@ void MdContext::check_status() {
     if (st & ST_CONSTRAINTS)
         throw Error(OMM_ERR_INVALID, "constraints failed: a projection was singular or did not converge in 1000 "
                                      "iterations (langevin.rs:215-277)");
+    if (st & ST_VBOX) {
+        fill_async<unsigned>(status_.p, 1, 0u, s_);
+        throw Error(OMM_ERR_OVERFLOW, "atoms of a non-periodic system kept leaving its virtual box");
+    }
 }
 
 void MdContext::enqueue_snapshot(hipStream_t s) {
@@ -1745,7 +1794,11 @@ void MdContext::enqueue_snapshot(hipStream_t s) {
 // replayed from here follow the trajectory of a run that never overflowed.
 void MdContext::restore_snapshot() {
     const size_t bytes = 3 * (size_t)n_ * sizeof(double);
-    OMM_HIP(hipMemcpyAsync(nb_->systems_dev(), snap_sys_.p, nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice, s_));
+    // (A virtual box's table is the operator's own: a replay after leaving
+    // its region runs in a new box, and nothing else changes the table.)
+    if (!nb_->is_virtual())
+        OMM_HIP(hipMemcpyAsync(nb_->systems_dev(), snap_sys_.p, nsys_ * sizeof(SysInfo), hipMemcpyDeviceToDevice,
+                               s_));
     if (baro_.enabled)
         OMM_HIP(hipMemcpyAsync(baro_.st.p, baro_.snap_st.p, nsys_ * sizeof(BaroState), hipMemcpyDeviceToDevice, s_));
     OMM_HIP(hipMemcpyAsync(pos_.p, snap_pos_.p, bytes, hipMemcpyDeviceToDevice, s_));
@@ -1831,12 +1884,20 @@ void MdContext::run_steps(int count, bool graph) {
         eager();
     }
     unsigned st = read_status();
-    for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW); ++attempt) {
+    for (int attempt = 0; st & (ST_LIST_OVERFLOW | ST_BIN_OVERFLOW | ST_VBOX); ++attempt) {
         if (attempt == 8) check_status();  // throws
         TraceRange replay("omm.overflow_replay");
         ++overflow_replays_;
         drop_graph();
-        nb_->grow(st & ST_BIN_OVERFLOW);  // the lists are sized by the rebuild from the snapshot
+        if (st & ST_VBOX) {
+            // An atom of a non-periodic system left the virtual box's region:
+            // a box for both the snapshot's and the current positions, and
+            // the steps replayed in it.
+            fill_async<unsigned>(status_.p, 1, 0u, s_);
+            refit_virtual_box(snap_pos_.p);  // a new box, or the region re-centred in the same one
+        } else {
+            nb_->grow(st & ST_BIN_OVERFLOW);  // the lists are sized by the rebuild from the snapshot
+        }
         restore_snapshot();
         B.phase = phase0;
         eager();
@@ -1965,6 +2026,13 @@ void MdContext::get_state(int r, double* pos, double* vel, double* forces, doubl
 
 void MdContext::get_box(int r, double* box) {
     if (r < 0 || r >= nsys_) throw Error(OMM_ERR_INVALID, "replica index out of range");
+    if (nb_->is_virtual()) {
+        bool any = false;
+        for (int k = 0; k < 9; ++k) any = any || default_box_[k] != 0.0;
+        if (!any) throw Error(OMM_ERR_STATE, "this non-periodic system has no box vectors");
+        for (int k = 0; k < 9; ++k) box[k] = default_box_[k];
+        return;
+    }
     SysInfo S;
     OMM_HIP(hipMemcpyAsync(&S, nb_->systems_dev() + r, sizeof(SysInfo), hipMemcpyDeviceToHost, s_));
     OMM_HIP(hipStreamSynchronize(s_));
@@ -2004,11 +2072,11 @@ void MdContext::profile(int reps, double ms[8], int64_t* num_pairs) {
         if (mixed_)
             hipLaunchKernelGGL(k_lfmiddle<long long>, dim3((n_ + 255) / 256), dim3(256), 0, s_, n_, px.p, vx.p,
                                force_fx_.p, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, 1, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
+                               step_dev_.p, 1, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr, status_.p);
         else
             hipLaunchKernelGGL(k_lfmiddle<double>, dim3((n_ + 255) / 256), dim3(256), 0, s_, n_, px.p, vx.p,
                                force_d_.p, inv_mass_.p, nb_->atom_sys_dev(), nb_->systems_dev(), dt_, decay, variance,
-                               step_dev_.p, 1, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr);
+                               step_dev_.p, 1, nb_->pos_build(), lim * lim, 1, scratch_flag.p, nullptr, status_.p);
     });
     OMM_HIP(hipStreamSynchronize(s_));
     OMM_HIP(hipEventDestroy(t0));

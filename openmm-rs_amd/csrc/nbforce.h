@@ -138,6 +138,14 @@ public:
     const double* charge_dev() const { return part_.charge.p; }
     const int* atom_sys_dev() const { return part_.atom_sys.p; }
     bool exceptions_pbc() const { return cfg_[0].exceptions_pbc; }
+    // Non-periodic method on a virtual box (one system).
+    bool is_virtual() const { return virtual_; }
+    // MD contexts of a non-periodic method: keeps the virtual box while the
+    // `count` host positions (any number of configurations of the system,
+    // concatenated) fit it, else chooses a new one and sets the device up for
+    // it.  Writes the region the integrators check.  Returns true when the
+    // device was set up (lists must be rebuilt, captured graphs dropped).
+    bool fit_virtual_box(const double* pos, int count);
     // Exceptions carry the reciprocal-space correction (PME or Ewald).
     bool screened() const { return cfg_[0].screened(); }
     const double* pos_build() const { return nl_.pos_build.p; }
@@ -245,7 +253,7 @@ private:
     // minimum image never changes a displacement the list or the kernels
     // look at, and never brings a far pair within reach.  Returns the box
     // for these positions, keeping the current one while it still fits.
-    Box virtual_box_for(const double* pos, bool* changed);
+    Box virtual_box_for(const double* pos, int count, bool* changed);
     double dispersion_coefficient(int s) const;
     // LJPME self term alpha^6 / 3 sum_i eps_i sigma_i^6 (nonbonded.rs:984-988).
     double lj_self_energy(int s) const;

@@ -357,7 +357,10 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     const int N = ntotal_;
     std::vector<int> atom_sys(N);
     std::vector<int> p1, p2;
+    std::vector<unsigned long long> seeds(nsys, 0ull);
+    for (int si = 0; si < nsys && si < (int)sys_h_.size(); ++si) seeds[si] = sys_h_[si].seed;
     sys_h_.assign(nsys, SysInfo{});
+    for (int si = 0; si < nsys; ++si) sys_h_[si].seed = seeds[si];
     std::vector<NlistSystem> nls(nsys);
     int off = 0;
     for (int si = 0; si < nsys; ++si) {
@@ -457,8 +460,7 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     device_ready_ = true;
 }
 
-Box NbOperator::virtual_box_for(const double* pos, bool* changed) {
-    const int n = n_[0];
+Box NbOperator::virtual_box_for(const double* pos, int n, bool* changed) {
     const NbSettings& c = cfg_[0];
     double lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]};
     for (int i = 1; i < n; ++i)
@@ -488,6 +490,47 @@ Box NbOperator::virtual_box_for(const double* pos, bool* changed) {
     vbox_ = make_box(v);
     *changed = true;
     return vbox_;
+}
+
+bool NbOperator::fit_virtual_box(const double* pos, int count) {
+    if (!virtual_) return false;
+    for (int i = 0; i < 3 * count; ++i)
+        if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
+    const NbSettings& c = cfg_[0];
+    const double reach = c.cutoff + 0.1;
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = hi[a] = pos[a];
+        for (int i = 1; i < count; ++i) {
+            lo[a] = std::min(lo[a], pos[3 * i + a]);
+            hi[a] = std::max(hi[a], pos[3 * i + a]);
+        }
+    }
+    // A box that still fits but leaves the atoms less than 0.2 nm to spread
+    // in some direction is replaced now rather than a few steps later.
+    if (device_ready_)
+        for (int a = 0; a < 3; ++a) {
+            const double L = vbox_.v[a * 4];
+            if ((c.use_cutoff() ? L - reach : 0.5 * L) - (hi[a] - lo[a]) < 0.4) vbox_.v[a * 4] = 0.0;
+        }
+    bool changed = false;
+    const Box b = virtual_box_for(pos, count, &changed);
+    const bool setup = changed || !device_ready_;
+    if (setup) setup_device(std::vector<Box>{b});
+    // The integrators' region: an interval of width L - reach (cutoff) or
+    // L / 2 (no cutoff) per axis, centred on these positions, in which any
+    // two atoms keep the true displacement and a far pair stays out of reach.
+    SysInfo& S = sys_h_[0];
+    for (int a = 0; a < 3; ++a) {
+        const double L = S.box.v[a * 4];
+        const double w = (c.use_cutoff() ? L - reach : 0.5 * L) * (1.0 - 1e-9);
+        const double mid = 0.5 * (lo[a] + hi[a]);
+        S.vlo[a] = mid - 0.5 * w;
+        S.vhi[a] = mid + 0.5 * w;
+    }
+    S.vcheck = 1;
+    sys_dev_.upload(sys_h_, stream_);
+    return setup;
 }
 
 void NbOperator::upload_systems(hipStream_t s) { sys_dev_.upload(sys_h_, s); }
@@ -1212,6 +1255,7 @@ double NbOperator::mesh_charges(const double* pos, const double* box, double alp
 }
 
 void NbOperator::set_box(int s, const double* box9, hipStream_t st) {
+    if (virtual_) throw Error(OMM_ERR_INVALID, "a non-periodic system takes no box vectors");
     if (s < 0 || s >= (int)sys_h_.size()) throw Error(OMM_ERR_INVALID, "system index out of range");
     if (!cfg_[0].periodic()) throw Error(OMM_ERR_INVALID, "non-periodic methods have no box vectors");
     if (!box9) throw Error(OMM_ERR_INVALID, "null box vectors");
@@ -1367,7 +1411,7 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
     Box b{};
     bool vchanged = false;
     if (n > 0 && virtual_) {
-        b = virtual_box_for(pos, &vchanged);
+        b = virtual_box_for(pos, n, &vchanged);
         if (!device_ready_) setup_device(std::vector<Box>{b});
     } else {
         b = make_box(box);

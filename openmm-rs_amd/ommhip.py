@@ -759,6 +759,8 @@ class Context:
                  seed=1, precision=PRECISION_MIXED):
         lib = library()
         self.n = len(system.masses)
+        # NoCutoff and CutoffNonPeriodic systems have no periodic box to save or restore.
+        self.periodic = system.nonbonded.method not in (NO_CUTOFF, CUTOFF_NON_PERIODIC)
         self._d, self._keep = _md_desc(system, temperature, friction, step_size, seed)
         h = C.c_void_p()
         _check(lib.omm_ctx_create(C.byref(self._d), precision, C.byref(h)))
@@ -895,7 +897,7 @@ class Context:
         lp = np.zeros((self.n, 3))
         _check(library().omm_ctx_get_list_positions(self._h, lp.ctypes.data_as(_dp)))
         return dict(positions=pos, velocities=vel, forces=frc, potential_energy=e.value, step=k.value,
-                    box=self.box(), list_positions=lp)
+                    box=self.box() if self.periodic else None, list_positions=lp)
 
     def set_state(self, state):
         """Context::set_state (context.rs:274-293) from a get_state() dict: step count, box,

@@ -197,12 +197,75 @@ def test_nonperiodic_virtual_box_follows_the_atoms(method):
         assert op.pair_stats(pos, None) == orc.nb_pairs(desc, pos, None)
 
 
-# Host-side validation: runs without a GPU.
-def test_md_context_rejects_nonperiodic_methods():
-    s = water(100, 1.9, seed=63)
-    s.nonbonded.set_nonbonded_method("CutoffNonPeriodic")
+def _droplet_energy_forces(s):
+    """The oracle's total for a non-periodic water system: nonbonded without a
+    box, harmonic bonds and angles."""
+    f, e, _ = orc.nb_evaluate(s.nonbonded.desc(), s.positions, None)
+    fb, eb = orc.harmonic_bonds(s.bonds, s.bond_params, s.positions)
+    fa, ea = orc.harmonic_angles(s.angles, s.angle_params, s.positions)
+    return f + fb + fa, e + eb + ea
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["NoCutoff", "CutoffNonPeriodic"])
+def test_nonperiodic_md_context(method):
+    """Context::step (context.rs:878-928) for the non-periodic methods: the
+    state against the oracle, graph replays against eager steps bit for bit,
+    and the state again against the oracle after the steps."""
+    s = droplet(method, molecules=200, seed=66)
+    a = ommhip.Context(s, temperature=300.0, friction=1.0, seed=4)
+    st = a.get_state()
+    F_ref, E_ref = _droplet_energy_forces(s)
+    assert force_rms_rel(st["forces"], F_ref) <= FORCE_TOL
+    assert energy_rel(st["potential_energy"], E_ref) <= ENERGY_TOL
+    assert st["box"] is None
+    b = ommhip.Context(s, temperature=300.0, friction=1.0, seed=4)
+    b.set_graph_steps(0)
+    for c in (a, b):
+        c.set_velocities_to_temperature(300.0, 2)
+        c.step(60)
+    sa, sb = a.get_state(), b.get_state()
+    assert np.array_equal(sa["positions"], sb["positions"])
+    assert np.array_equal(sa["velocities"], sb["velocities"])
+    moved = copy.copy(s)
+    moved.positions = sa["positions"]
+    F_ref, E_ref = _droplet_energy_forces(moved)
+    assert force_rms_rel(sa["forces"], F_ref) <= FORCE_TOL
+    assert energy_rel(sa["potential_energy"], E_ref) <= ENERGY_TOL
+    # context.rs:93-101: no barostat on a non-periodic system.
     with pytest.raises(ommhip.OmmError):
-        ommhip.Context(s, temperature=300.0, friction=1.0)
+        a.set_barostat(1.0, 300.0, 25)
+
+
+@pytest.mark.gpu
+def test_nonperiodic_context_follows_escaping_molecules():
+    """Two outer molecules leave the droplet at 15 nm/ps: they leave the region the
+    virtual box serves, and the steps since the last snapshot are replayed in
+    a new box.  Graph and eager runs replay at the same run boundaries and stay
+    bitwise equal; the final state matches the oracle."""
+    s = droplet("CutoffNonPeriodic", molecules=150, seed=67)
+    vel = np.zeros_like(s.positions)
+    com = s.positions.reshape(-1, 3, 3).mean(1)
+    fast_x, fast_y = int(np.argmax(com[:, 0])), int(np.argmin(com[:, 1]))  # outermost molecules
+    vel[3 * fast_x:3 * fast_x + 3] = [15.0, 0.0, 0.0]
+    vel[3 * fast_y:3 * fast_y + 3] = [0.0, -15.0, 0.0]
+    runs = []
+    for graph_steps in (10, 0):
+        c = ommhip.Context(s, temperature=0.0, friction=0.0, seed=4)
+        c.set_graph_steps(graph_steps)
+        c.set_velocities(vel)
+        c.step(200)  # 3 nm of flight
+        runs.append((c.get_state(), c.capacity()[2]))
+    (sa, ra), (sb, rb) = runs
+    assert ra > 0 and rb > 0
+    assert np.array_equal(sa["positions"], sb["positions"])
+    assert sa["positions"][3 * fast_x, 0] - s.positions[3 * fast_x, 0] > 2.0
+    assert s.positions[3 * fast_y, 1] - sa["positions"][3 * fast_y, 1] > 2.0
+    moved = copy.copy(s)
+    moved.positions = sa["positions"]
+    F_ref, E_ref = _droplet_energy_forces(moved)
+    assert force_rms_rel(sa["forces"], F_ref) <= FORCE_TOL
+    assert energy_rel(sa["potential_energy"], E_ref) <= ENERGY_TOL
 
 
 @pytest.mark.gpu
