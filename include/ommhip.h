@@ -171,6 +171,19 @@ omm_status omm_nb_calc_forces_and_energy_groups(omm_nb* nb, const double* pos, c
                                                 uint32_t groups, double* forces_out,
                                                 double* energy_out, int32_t* evaluated);
 
+/* The resident form of calc_forces_and_energy_groups (no counterpart in the
+ * reference, which takes host tensors; for callers whose positions already
+ * live on the device, e.g. a device-side integrator).  d_pos: device N x 3,
+ * read after the work already queued on `stream` (a hipStream_t; NULL: the
+ * null stream); d_forces: device N x 3 (overwritten, may be NULL); box and
+ * energy_out on the host as above; groups as above (0xFFFFFFFF: all).  The
+ * call returns once the outputs are complete.  Positions are not copied to
+ * the host except for NoCutoff / CutoffNonPeriodic, whose virtual box is
+ * chosen from them. */
+omm_status omm_nb_calc_forces_and_energy_device(omm_nb* nb, const double* d_pos, const double* box,
+                                                uint32_t groups, double* d_forces, double* energy_out,
+                                                int32_t* evaluated, void* stream);
+
 /* Energy split of the last evaluation (same call as above, with
  * include_direct = include_reciprocal = 1):
  *   parts[0] direct-space pairs (LJ + screened/plain Coulomb),

@@ -193,6 +193,27 @@ omm_status omm_nb_calc_forces_and_energy_groups(omm_nb* nb, const double* pos, c
     });
 }
 
+omm_status omm_nb_calc_forces_and_energy_device(omm_nb* nb, const double* d_pos, const double* box, uint32_t groups,
+                                                double* d_forces, double* energy_out, int32_t* evaluated,
+                                                void* stream) {
+    return guarded([&]() -> omm_status {
+        if (!nb) throw Error(OMM_ERR_INVALID, "null handle");
+        if (!d_pos) throw Error(OMM_ERR_INVALID, "positions must not be null");
+        std::lock_guard<std::mutex> lock(nb->op.mutex);
+        const ommhip::NbSettings& c = nb->op.settings();
+        // nonbonded.rs:1139-1149, as omm_nb_calc_forces_and_energy_groups.
+        bool direct = c.include_direct && (groups & (1u << c.force_group)) != 0;
+        uint32_t rg = c.reciprocal_group < 0 ? (uint32_t)c.force_group : (uint32_t)c.reciprocal_group;
+        bool recip = (c.method == OMM_NB_EWALD || c.method == OMM_NB_PME || c.method == OMM_NB_LJPME) &&
+                     (groups & (1u << rg)) != 0;
+        if (evaluated) *evaluated = (direct || recip) ? 1 : 0;
+        if (!(direct || recip)) return OMM_OK;
+        nb->op.compute(nullptr, box, direct, recip, nullptr, energy_out, nb->parts, nullptr, nullptr, d_pos, d_forces,
+                       static_cast<hipStream_t>(stream));
+        return OMM_OK;
+    });
+}
+
 omm_status omm_nb_energy_parts(omm_nb* nb, const double* pos, const double* box, double parts[5]) {
     return guarded([&]() -> omm_status {
         if (!nb || !parts) throw Error(OMM_ERR_INVALID, "null argument");

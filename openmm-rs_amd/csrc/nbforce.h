@@ -109,8 +109,12 @@ public:
     // Host-pointer evaluation of a one-system operator (calc_forces_and_energy
     // [_groups]).  parts receives the energy split; stats the pair count and
     // hash.
+    // Device form: pos_dev_in (device N x 3, read after the caller's stream
+    // reaches the call) replaces pos, and forces_dev_out (device N x 3,
+    // overwritten) receives the forces; the call returns with them complete.
     void compute(const double* pos, const double* box, bool direct, bool recip, double* forces, double* energy,
-                 double parts[5], int64_t* num_pairs, uint64_t* pair_hash);
+                 double parts[5], int64_t* num_pairs, uint64_t* pair_hash, const double* pos_dev_in = nullptr,
+                 double* forces_dev_out = nullptr, hipStream_t caller = nullptr);
 
     // Neighbour-list policy of one enqueued evaluation.
     enum Rebuild : int {
@@ -281,6 +285,7 @@ private:
     hipStream_t side_ = nullptr;  // PME branch (default priority, like stream_)
     hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     hipEvent_t ev_list_ = nullptr;  // the list build is done (the gather may read the sorted order)
+    hipEvent_t ev_caller_ = nullptr;  // device-pointer evaluations: the caller's stream reached the call
     bool joined_ = true;
 
     std::vector<SysInfo> sys_h_;

@@ -238,6 +238,7 @@ NbOperator::~NbOperator() {
     if (ev_fork_) (void)hipEventDestroy(ev_fork_);
     if (ev_join_) (void)hipEventDestroy(ev_join_);
     if (ev_list_) (void)hipEventDestroy(ev_list_);
+    if (ev_caller_) (void)hipEventDestroy(ev_caller_);
     if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -353,6 +354,7 @@ void NbOperator::setup_device(const std::vector<Box>& boxes) {
     if (!ev_fork_) OMM_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
     if (!ev_join_) OMM_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     if (!ev_list_) OMM_HIP(hipEventCreateWithFlags(&ev_list_, hipEventDisableTiming));
+    if (!ev_caller_) OMM_HIP(hipEventCreateWithFlags(&ev_caller_, hipEventDisableTiming));
     hipStream_t s = stream_;
     const int N = ntotal_;
     std::vector<int> atom_sys(N);
@@ -1399,15 +1401,27 @@ void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, d
 }
 
 void NbOperator::compute(const double* pos, const double* box, bool direct, bool recip, double* forces,
-                         double* energy, double parts[5], int64_t* num_pairs, uint64_t* pair_hash) {
-    TraceRange range("omm.nb_evaluate");
+                         double* energy, double parts[5], int64_t* num_pairs, uint64_t* pair_hash,
+                         const double* pos_dev_in, double* forces_dev_out, hipStream_t caller) {
+    TraceRange range(pos_dev_in ? "omm.nb_evaluate_device" : "omm.nb_evaluate");
     if (!initialized_) throw Error(OMM_ERR_STATE, "NonbondedForce used before initialize");
     if (cfg_.size() != 1) throw Error(OMM_ERR_STATE, "host-pointer evaluation needs a one-system operator");
-    if (!pos) throw Error(OMM_ERR_INVALID, "positions must not be null");
+    if (!pos && !pos_dev_in) throw Error(OMM_ERR_INVALID, "positions must not be null");
     check_box(box);
     const int n = n_[0];
-    for (int i = 0; i < 3 * n; ++i)
-        if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
+    // Device positions: a non-periodic method chooses its virtual box from a
+    // host copy; otherwise non-finite positions surface as non-finite forces.
+    std::vector<double> host_pos;
+    if (pos_dev_in && virtual_ && n > 0) {
+        host_pos.resize(3 * (size_t)n);
+        OMM_HIP(hipMemcpyAsync(host_pos.data(), pos_dev_in, host_pos.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               caller));
+        OMM_HIP(hipStreamSynchronize(caller));
+        pos = host_pos.data();
+    }
+    if (pos)
+        for (int i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(pos[i])) throw Error(OMM_ERR_NONFINITE, "particle positions must be finite");
     Box b{};
     bool vchanged = false;
     if (n > 0 && virtual_) {
@@ -1432,7 +1446,15 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
         last_box_ = b;
         have_box_ = true;
     }
-    pos_dev_.upload(pos, 3 * (size_t)n, s);
+    if (pos_dev_in) {
+        // Ordered after the caller's work on its stream; the call returns
+        // once the results are complete, as the host form does.
+        OMM_HIP(hipEventRecord(ev_caller_, caller));
+        OMM_HIP(hipStreamWaitEvent(s, ev_caller_, 0));
+        OMM_HIP(hipMemcpyAsync(pos_dev_.p, pos_dev_in, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        pos_dev_.upload(pos, 3 * (size_t)n, s);
+    }
     const bool want_stats = num_pairs || pair_hash;
     // Pair statistics are gathered by a force-only pass; energies by the
     // energy pass (FP64 pair energies and FP64 mesh in the mixed form).
@@ -1478,13 +1500,16 @@ void NbOperator::compute(const double* pos, const double* box, bool direct, bool
         if (mixed_) {
             long long efx[E_SLOTS];
             energy_fx_.download(efx, E_SLOTS, s);
-            launch_fixed_to_double(force_fx_.p, out_forces_.p, 3 * (size_t)n, s);
+            launch_fixed_to_double(force_fx_.p, forces_dev_out ? forces_dev_out : out_forces_.p, 3 * (size_t)n, s);
             if (forces) out_forces_.download(forces, 3 * (size_t)n, s);
             OMM_HIP(hipStreamSynchronize(s));
             for (int k = 0; k < E_SLOTS; ++k) e[k] = (double)efx[k] * (1.0 / ENERGY_SCALE);
         } else {
             energy_d_.download(e, E_SLOTS, s);
             if (forces) force_d_.download(forces, 3 * (size_t)n, s);
+            if (forces_dev_out)
+                OMM_HIP(hipMemcpyAsync(forces_dev_out, force_d_.p, 3 * (size_t)n * sizeof(double),
+                                       hipMemcpyDeviceToDevice, s));
             OMM_HIP(hipStreamSynchronize(s));
         }
         if (want_stats) {

@@ -529,6 +529,22 @@ class GpuNonbondedForce:
                                                        f.ctypes.data_as(_dp), C.byref(e)))
         return f, e.value
 
+    def calc_forces_and_energy_device(self, d_pos, box=None, d_forces=None, groups=0xFFFFFFFF, stream=None,
+                                      energy=True):
+        """The resident form (omm_nb_calc_forces_and_energy_device): d_pos and d_forces are device
+        addresses (ints or ctypes pointers, e.g. a torch tensor's data_ptr()), stream a hipStream_t
+        handle (None: the null stream).  Returns (energy or None, evaluated)."""
+        def addr(x):
+            return x.value if isinstance(x, C.c_void_p) else x
+
+        b, bp = _box_ptr(box)
+        e = C.c_double()
+        ev = C.c_int32()
+        _check(library().omm_nb_calc_forces_and_energy_device(
+            self._h, C.c_void_p(addr(d_pos)), bp, C.c_uint32(groups), C.c_void_p(addr(d_forces)),
+            C.byref(e) if energy else None, C.byref(ev), C.c_void_p(addr(stream))))
+        return (e.value if energy else None), bool(ev.value)
+
     def calc_forces(self, pos, box=None):
         """Forces only (energy_out = NULL): the MD step's FP32 path."""
         p = _f64(pos).reshape(-1, 3)

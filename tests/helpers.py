@@ -1,4 +1,7 @@
-"""Shared fixtures of the parity tests: water boxes and error metrics."""
+"""Shared fixtures of the parity tests: water boxes, error metrics and device
+buffers of the HIP runtime (for the entry points that take device pointers)."""
+import ctypes as C
+
 import numpy as np
 
 import ommhip
@@ -79,3 +82,38 @@ def lj_pair_closed_forms(pos, vel, masses, box):
         else:
             flex[c] = 2 * kin[c] / (v * bar)  # positions fixed: the pair keeps its separation
     return iso, axis, flex, stress
+
+
+H2D, D2H = 1, 2
+
+
+def hip_runtime():
+    """The HIP runtime the library links against, for the tests' own device buffers."""
+    ommhip.library()
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    hip.hipFree.argtypes = [C.c_void_p]
+    return hip
+
+
+class DeviceArray:
+    """A device copy of a host array (hipMalloc / hipMemcpy of the test, not of the library)."""
+
+    def __init__(self, hip, host):
+        self.hip, self.host = hip, np.ascontiguousarray(host)
+        self.p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(self.p), self.host.nbytes) == 0
+        assert hip.hipMemcpy(self.p, self.host.ctypes.data, self.host.nbytes, H2D) == 0
+
+    def get(self):
+        out = np.empty_like(self.host)
+        assert self.hip.hipMemcpy(out.ctypes.data, self.p, out.nbytes, D2H) == 0
+        return out
+
+    def free(self):
+        self.hip.hipFree(self.p)

@@ -9,38 +9,9 @@ import pytest
 
 import ommhip
 import oracle as orc
+from helpers import DeviceArray as _Dev, hip_runtime as _hip
 
 pytestmark = pytest.mark.gpu
-
-H2D, D2H = 1, 2
-
-
-def _hip():
-    ommhip.library()  # loads the HIP runtime libommhip.so links against
-    hip = C.CDLL("libamdhip64.so.7")
-    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    hip.hipFree.argtypes = [C.c_void_p]
-    return hip
-
-
-class _Dev:
-    """A device copy of a host array (hipMalloc / hipMemcpy of the test, not of the entry point)."""
-
-    def __init__(self, hip, host):
-        self.hip, self.host = hip, np.ascontiguousarray(host)
-        self.p = C.c_void_p()
-        assert hip.hipMalloc(C.byref(self.p), self.host.nbytes) == 0
-        assert hip.hipMemcpy(self.p, self.host.ctypes.data, self.host.nbytes, H2D) == 0
-
-    def get(self):
-        out = np.empty_like(self.host)
-        assert self.hip.hipMemcpy(out.ctypes.data, self.p, out.nbytes, D2H) == 0
-        return out
-
-    def free(self):
-        self.hip.hipFree(self.p)
-
 
 def _noalloc(hip, pos, charges, box, alpha, grid, idx, start):
     lib = ommhip.library()
