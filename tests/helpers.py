@@ -24,6 +24,11 @@ def force_rms_rel(f, ref):
     return float(np.sqrt(((f - ref) ** 2).sum() / (ref**2).sum()))
 
 
+def force_max_atom_rel(f, ref):
+    """Largest per-atom |F_i - F_i_ref| / |F_i_ref| (reported beside the RMS bar)."""
+    return float((np.linalg.norm(f - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)).max())
+
+
 def energy_rel(e, ref):
     return abs(e - ref) / abs(ref)
 

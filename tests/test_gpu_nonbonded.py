@@ -12,7 +12,7 @@ import pytest
 
 import ommhip
 import oracle as orc
-from helpers import energy_rel, force_rms_rel, water
+from helpers import energy_rel, force_max_atom_rel, force_rms_rel, water
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +34,8 @@ def check_against_oracle(system, precision=ommhip.PRECISION_MIXED, ftol=FORCE_TO
     parts = op.energy_parts(system.positions, system.box)
     frel = force_rms_rel(F, F_ref)
     erel = energy_rel(E, E_ref)
-    print(f"N={len(F)} force_rms_rel={frel:.3e} energy_rel={erel:.3e} E={E:.6f} E_ref={E_ref:.6f}")
+    print(f"N={len(F)} force_rms_rel={frel:.3e} force_max_atom_rel={force_max_atom_rel(F, F_ref):.3e} "
+          f"energy_rel={erel:.3e} E={E:.6f} E_ref={E_ref:.6f}")
     print("parts", parts, "ref", parts_ref)
     assert frel <= ftol
     assert erel <= etol
