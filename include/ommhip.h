@@ -453,8 +453,9 @@ omm_status omm_ctx_set_capacity(omm_ctx* ctx, int32_t list_cap, int32_t bin_cap)
 omm_status omm_ctx_capacity(omm_ctx* ctx, int32_t* list_cap, int32_t* bin_cap, int64_t* overflow_replays);
 
 /* Launch shapes picked by the autotuner when the context is first prepared
- * (no counterpart in the reference): the spread tile edge (grid points), the
- * FFT decomposition (1: the z and y transforms of each x plane in one
+ * (no counterpart in the reference): the spread's decomposition (the tile
+ * edge in grid points for tiles of the grid, or 0 for chunks of sorted atoms
+ * summed in LDS boxes, mixed precision only), the FFT decomposition (1: the z and y transforms of each x plane in one
  * workgroup, three launches per transform; 0: five line passes) with the
  * lines per workgroup of the line passes, and the pair kernel's grid (the
  * calibrated task count times pair_scale: 1, 2, or -2 for half), each the
@@ -475,7 +476,8 @@ typedef struct omm_launch_tuning {
 omm_status omm_ctx_launch_tuning(omm_ctx* ctx, omm_launch_tuning* out);
 
 /* Sets the launch shapes instead of the autotuner's (no counterpart in the
- * reference): spread_tile_edge 6 or 8, fft_plane 0 or 1 (1 only where every
+ * reference): spread_tile_edge 6 or 8, or 0 for the atom-owned spread (mixed
+ * precision only), fft_plane 0 or 1 (1 only where every
  * system's x plane fits in LDS), fft_lines_per_workgroup 16, 8, 4 or 2,
  * pair_scale 1, 2 or -2; the other fields are ignored.  Results do not
  * change; OMM_ERR_INVALID for other values. */

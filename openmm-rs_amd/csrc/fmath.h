@@ -34,6 +34,7 @@ struct Math<float> {
         erfc_v = p * ex;
     }
     static __device__ __forceinline__ float round(float x) { return rintf(x); }
+    static __device__ __forceinline__ float max(float a, float b) { return fmaxf(a, b); }
     // e^{-x} through the hardware exp2.
     static __device__ __forceinline__ float exp_neg(float x) { return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f); }
 };
@@ -47,6 +48,7 @@ struct Math<double> {
         ex = exp(-x * x);
     }
     static __device__ __forceinline__ double round(double x) { return rint(x); }
+    static __device__ __forceinline__ double max(double a, double b) { return fmax(a, b); }
     static __device__ __forceinline__ double exp_neg(double x) { return exp(-x); }
 };
 

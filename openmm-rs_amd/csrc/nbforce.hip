@@ -935,8 +935,10 @@ void NbOperator::apply_tuning(const Tuning& t, const double* pos_dev, unsigned* 
     const NbSettings& c = cfg_[0];
     // Shapes of phases this operator does not run are ignored.
     if (c.pme_mesh()) {
-        if (t.spread_edge != 6 && t.spread_edge != 8)
-            throw Error(OMM_ERR_INVALID, "spread tile edge must be 6 or 8");
+        if (t.spread_edge != 0 && t.spread_edge != 6 && t.spread_edge != 8)
+            throw Error(OMM_ERR_INVALID, "spread tile edge must be 0 (atom-owned spread), 6 or 8");
+        if (t.spread_edge == 0 && !mixed_)
+            throw Error(OMM_ERR_INVALID, "the atom-owned spread serves the mixed-precision mesh only");
         if (t.fft_cap != 16 && t.fft_cap != 8 && t.fft_cap != 4 && t.fft_cap != 2)
             throw Error(OMM_ERR_INVALID, "FFT lines per workgroup must be 16, 8, 4 or 2");
         if (t.fft_plane != 0 && t.fft_plane != 1) throw Error(OMM_ERR_INVALID, "FFT decomposition must be 0 or 1");
@@ -950,7 +952,8 @@ void NbOperator::apply_tuning(const Tuning& t, const double* pos_dev, unsigned* 
             throw Error(OMM_ERR_INVALID, "plane FFT passes do not fit this workload's grids in LDS");
         }
         pme_set_fft_lines(pme_, sys_h_, prec, t.fft_cap, s);
-        pme_set_spread_edge(pme_, sys_h_, t.spread_edge, s);
+        pme_.spread_atoms = t.spread_edge == 0;
+        if (t.spread_edge != 0) pme_set_spread_edge(pme_, sys_h_, t.spread_edge, s);
         patch_layout(s);
         fit_bins(pos_dev, status, s);
     }
@@ -1018,12 +1021,16 @@ void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s
                         s, ph);
             };
         };
-        // Spread tile edge.
+        // Spread: tiles of edge 8 or 6, or atom-owned chunks (edge 0).
         double best = 1e30;
-        for (int e : {8, 6}) {
-            pme_set_spread_edge(pme_, sys_h_, e, s);
-            patch_layout(s);
-            fit_bins(pos_dev, status, s);
+        for (int e : {8, 6, 0}) {
+            if (e == 0 && !mixed_) continue;  // fixed-point grids only
+            pme_.spread_atoms = e == 0;
+            if (e != 0) {
+                pme_set_spread_edge(pme_, sys_h_, e, s);
+                patch_layout(s);
+                fit_bins(pos_dev, status, s);
+            }
             const double t = timed(phase(PME_SPREAD));
             if (t < best) {
                 best = t;
@@ -1031,7 +1038,8 @@ void NbOperator::autotune(const double* pos_dev, unsigned* status, hipStream_t s
             }
         }
         tuning_.spread_ms = best;
-        pme_set_spread_edge(pme_, sys_h_, tuning_.spread_edge, s);
+        pme_.spread_atoms = tuning_.spread_edge == 0;
+        if (tuning_.spread_edge != 0) pme_set_spread_edge(pme_, sys_h_, tuning_.spread_edge, s);
         patch_layout(s);
         fit_bins(pos_dev, status, s);
         // FFT decomposition (five line passes, or plane passes around the x

@@ -21,6 +21,7 @@
 // energies remove it.  The MD step does not request energies and never pays
 // for this.
 #include <climits>
+#include <type_traits>
 
 #include "fmath.h"
 #include "pairs.h"
@@ -41,7 +42,9 @@ struct alignas(2 * sizeof(R)) Q2 {
 
 // The reference's cutoff decision for the pair (a, b): delta = x_j - x_i with
 // i < j, reduced-triclinic minimum image, r^2 = dx dx + dy dy + dz dz, no FMA.
-__device__ __noinline__ bool within_cutoff_exact(const double* __restrict__ pos, int a, int b, const Box& bx,
+// Inlined: a call would make the compiler wait for every load in flight
+// (the next tile's prefetch) before the band loop of each tile.
+__device__ __forceinline__ bool within_cutoff_exact(const double* __restrict__ pos, int a, int b, const Box& bx,
                                                  int periodic, double cutoff2) {
 #pragma clang fp contract(off)
     int i = min(a, b), j = max(a, b);
@@ -135,23 +138,24 @@ struct EntryRaw {
     double qd = 0, sd = 0, ed = 0;  // the energy path's FP64 parameters
 };
 
+// Loads the data of atom a >= 0 with no branch: the entries past the end of
+// a list load some real atom's data and are masked when their tile is staged,
+// so the loads of a tile never wait on a branch's merge.
 template <class R, bool E64>
-__device__ __forceinline__ void load_entry(const int2 en, const double* __restrict__ pos, const R* __restrict__ qk,
+__device__ __forceinline__ void load_entry(const int a, const double* __restrict__ pos, const R* __restrict__ qk,
                                            const R* __restrict__ sh, const R* __restrict__ e2,
                                            const double* __restrict__ qk_d, const double* __restrict__ sh_d,
                                            const double* __restrict__ e2_d, EntryRaw<R, E64>& r) {
-    if (en.y >= 0) {
-        r.x = pos[3 * en.y];
-        r.y = pos[3 * en.y + 1];
-        r.z = pos[3 * en.y + 2];
-        r.q = qk[en.y];
-        r.s = sh[en.y];
-        r.e = e2[en.y];
-        if (E64) {
-            r.qd = qk_d[en.y];
-            r.sd = sh_d[en.y];
-            r.ed = e2_d[en.y];
-        }
+    r.x = pos[3 * a];
+    r.y = pos[3 * a + 1];
+    r.z = pos[3 * a + 2];
+    r.q = qk[a];
+    r.s = sh[a];
+    r.e = e2[a];
+    if (E64) {
+        r.qd = qk_d[a];
+        r.sd = sh_d[a];
+        r.ed = e2_d[a];
     }
 }
 
@@ -211,43 +215,62 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
         const int2* list = clist + (size_t)C * P.cap + (size_t)first * JTILE;
         const unsigned char* list_ex = clist_ex + (size_t)C * P.cap + (size_t)first * JTILE;
         const int nentries = count - first * JTILE;
-        auto entry = [&](int tl) {
-            const int e = tl * JTILE + lane;
-            return (tl < ntile && e < nentries) ? list[e] : make_int2(INT_MAX, -1);
-        };
-        auto entry_ex = [&](int tl) -> unsigned {
-            const int e = tl * JTILE + lane;
-            return (tl < ntile && e < nentries) ? list_ex[e] : 0u;
-        };
+        // Entry e of tile tl is live when tl < ntile and e < nentries.  Every
+        // lane loads some real entry of the list (the last one past its end)
+        // and the liveness is applied when the tile is staged, so no load
+        // sits behind a branch.
+        const int last = max(nentries - 1, 0);
+        auto live = [&](int tl) { return tl < ntile && tl * JTILE + lane < nentries; };
+        auto entry = [&](int tl) { return list[min(tl * JTILE + lane, last)]; };
+        auto entry_ex = [&](int tl) -> unsigned { return list_ex[min(tl * JTILE + lane, last)]; };
+        // The task waits for two rounds of loads before its first tile: the
+        // i-atom's slot with the first two tiles' entries, then the i-atom's
+        // data with tile 0's coordinates and parameters.
+        const int is = C * CLUSTER + il;
+        const int ia = sorted_atom[is];  // -1 in the padding of a system's last block
         // Software pipeline: the entries of tile k + 2 and the coordinates of
         // tile k + 1 are in flight while tile k is evaluated.
         int2 e0 = entry(0), e1 = entry(1);
         unsigned x0 = entry_ex(0), x1 = entry_ex(1);
+        const int ia0 = max(ia, 0);
+        const double pxi = pos[3 * ia0], pyi = pos[3 * ia0 + 1], pzi = pos[3 * ia0 + 2];
+        const R qi_raw = qk[ia0], si_raw = sh[ia0], ei_raw = e2[ia0];
+        double qid_raw = 0, sid_raw = 0, eid_raw = 0;
+        if (E64) {
+            qid_raw = qk_d[ia0];
+            sid_raw = sh_d[ia0];
+            eid_raw = e2_d[ia0];
+        }
         EntryRaw<R, E64> r0;
-        load_entry<R, E64>(e0, pos, qk, sh, e2, qk_d, sh_d, e2_d, r0);
+        load_entry<R, E64>(max(e0.y, 0), pos, qk, sh, e2, qk_d, sh_d, e2_d, r0);
 
-        const int is = C * CLUSTER + il;
-        const int ia = sorted_atom[is];  // -1 in the padding of a system's last block
         R xi = 0, yi = 0, zi = 0, qi = 0, si = 0, ei = 0;
         double xid = 0, yid = 0, zid = 0, qid = 0, sid = 0, eid = 0;
         if (ia >= 0) {
-            double d[3] = {pos[3 * ia] - c0, pos[3 * ia + 1] - c1, pos[3 * ia + 2] - c2};
+            double d[3] = {pxi - c0, pyi - c1, pzi - c2};
             if (P.periodic) wrap_nearest(d, bx);
             xi = (R)d[0];
             yi = (R)d[1];
             zi = (R)d[2];
-            qi = qk[ia];
-            si = sh[ia];
-            ei = e2[ia];
+            qi = qi_raw;
+            si = si_raw;
+            ei = ei_raw;
             if (E64) {
                 xid = d[0];
                 yid = d[1];
                 zid = d[2];
-                qid = qk_d[ia];
-                sid = sh_d[ia];
-                eid = e2_d[ia];
+                qid = qid_raw;
+                sid = sid_raw;
+                eid = eid_raw;
             }
         }
+        // The pair steps read the i-atom's parameters inside divergent
+        // branches, where the compiler cannot prove an earlier wait and
+        // would wait for every load in flight, the next tile's prefetch
+        // included, in each step.  Reading them here once, on every path,
+        // completes their loads before the tiles start.
+        asm volatile("" ::"v"(qi), "v"(si), "v"(ei));
+        if (E64) asm volatile("" ::"v"(qid), "v"(sid), "v"(eid));
         const bool mi = P.periodic && minimg[I];
         R bv[9], ib0 = 0, ib4 = 0, ib8 = 0;
         if (mi) {
@@ -264,10 +287,14 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             const int2 e2n = entry(k + 2);
             const unsigned x2n = entry_ex(k + 2);
             EntryRaw<R, E64> r1;
-            load_entry<R, E64>(e1, pos, qk, sh, e2, qk_d, sh_d, e2_d, r1);
+            load_entry<R, E64>(max(e1.y, 0), pos, qk, sh, e2, qk_d, sh_d, e2_d, r1);
 
             // Stage tile k in LDS as offsets from the block centre.
             {
+                if (!live(k)) {
+                    e0 = make_int2(INT_MAX, -1);
+                    x0 = 0u;
+                }
                 Q4<R> pq = {0, 0, 0, 0};
                 Q2<R> se = {0, 0};
                 Q4<double> pd = {0, 0, 0, 0};
@@ -316,14 +343,13 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             }
 
             R gx = 0, gy = 0, gz = 0;
-            // Offset of entry `slot` from the lane's i-atom (FP32 minimum image
-            // when the block is near half the box).
-            auto delta = [&](const int slot, R& dx, R& dy, R& dz) {
-                const Q4<R> pj = s_pq[slot];
+            // Offset of the entry at pj from the lane's i-atom, with the FP32
+            // minimum image when MI (the block is near half the box).
+            auto delta = [&](const Q4<R>& pj, R& dx, R& dy, R& dz, auto MI) {
                 dx = pj.x - xi;
                 dy = pj.y - yi;
                 dz = pj.z - zi;
-                if (mi) {
+                if (decltype(MI)::value) {
                     R s = M::round(dz * ib8);
                     dx -= s * bv[6];
                     dy -= s * bv[7];
@@ -335,9 +361,9 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                     dx -= s * bv[0];
                 }
             };
-            // F/r of the pair (and its energy) for an accepted pair.
-            auto pair_force = [&](const int slot, const R dx, const R dy, const R dz, const R r2) {
-                const Q2<R> sj = s_se[slot];
+            // F/r of a pair at squared distance r2 with the entry's charge qj
+            // and LJ parameters sj; e receives the pair's working-precision energy.
+            auto pair_fe = [&](const R qj, const Q2<R> sj, const R r2, R& e) -> R {
                 const R inv_r = M::rsqrt(r2);
                 const R inv_r2 = inv_r * inv_r;
                 // Lennard-Jones with Lorentz-Berthelot combination (nonbonded.rs:869-874, 1203-1214).
@@ -346,7 +372,7 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                 const R s2 = sig * sig * inv_r2;
                 const R s6 = s2 * s2 * s2;
                 const R s12 = s6 * s6;
-                R e = eps4 * (s12 - s6);
+                e = eps4 * (s12 - s6);
                 R f = eps4 * ((R)6 * s6 - (R)12 * s12) * inv_r2;
                 if (LJPME) {
                     // nonbonded.rs:875-886 with C6 = eps4 (4 (sigma_i/2)(sigma_j/2))^3.
@@ -371,15 +397,15 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                     }
                 }
                 if (SWITCH) {
-                    const R r = r2 * inv_r;
-                    if (r > (R)P.switch_distance) {
-                        R sw, dsw;
-                        lj_switch<R>(r, P, sw, dsw);
-                        f = sw * f + e * dsw * inv_r;
-                        e *= sw;
-                    }
+                    // Inside the switching distance t = 0 gives S = 1 and
+                    // dS/dr = 0 exactly, so no branch is needed.
+                    const R r = M::max(r2 * inv_r, (R)P.switch_distance);
+                    R sw, dsw;
+                    lj_switch<R>(r, P, sw, dsw);
+                    f = sw * f + e * dsw * inv_r;
+                    e *= sw;
                 }
-                const R qq = qi * s_pq[slot].w;
+                const R qq = qi * qj;
                 if (ELEC == ELEC_EWALD) {
                     // screened_coulomb (nonbonded.rs:1263-1273)
                     const R r = r2 * inv_r;
@@ -398,6 +424,10 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                     e += ec;
                     f -= ec * inv_r2;
                 }
+                return f;
+            };
+            // Energy and statistics of an accepted pair (their variants only).
+            auto pair_extra = [&](const int slot, const R e) {
                 if (E64) {
                     const Q4<double> pjd = s_pd[slot];
                     const Q2<double> sjd = s_sd[slot];
@@ -419,48 +449,60 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                     ++npair;
                     hsum += pair_key(min(ia, ja), max(ia, ja));
                 }
-                return f;
             };
             // Step k: lane (grp, il) pairs i-atom il with entry grp * 8 + (il ^ k);
             // each entry meets every i-atom of the cluster once over the 8 steps.
             // Pairs whose FP32 r^2 falls in the band around the cutoff are only
-            // marked here and decided in FP64 after the steps.
+            // marked here and decided in FP64 after the steps.  Every lane
+            // evaluates its slot whether or not it holds an accepted pair: a
+            // wave nearly always holds one in each step, so a branch would skip
+            // nothing, and straight-line steps let the compiler issue the next
+            // step's LDS reads early.  A rejected slot's r^2 is replaced by 1
+            // (away from the singularity at 0) and its force selected to zero.
+            // The minimum-image choice is made once per tile, outside the
+            // steps, so the eight steps form one straight block.
             unsigned near = 0;
-            auto pair_step = [&](const int k) {
+            auto pair_step = [&](const int k, auto MI) {
                 const int s8 = il ^ k;
                 const int slot = grp * CLUSTER + s8;
-                if ((vmask >> s8) & 1u) {
-                    R dx, dy, dz;
-                    delta(slot, dx, dy, dz);
-                    const R r2 = dx * dx + dy * dy + dz * dz;
-                    const bool in = !P.use_cutoff || r2 < rc2_lo;
-                    if (!in && r2 < rc2_hi) near |= 1u << s8;
-                    if (in) {
-                        const R f = pair_force(slot, dx, dy, dz, r2);
-                        fx += f * dx;
-                        fy += f * dy;
-                        fz += f * dz;
-                        gx -= f * dx;
-                        gy -= f * dy;
-                        gz -= f * dz;
-                    }
-                }
+                const Q4<R> pj = s_pq[slot];
+                const Q2<R> sj = s_se[slot];
+                R dx, dy, dz;
+                delta(pj, dx, dy, dz, MI);
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const bool valid = (vmask >> s8) & 1u;
+                const bool in = valid && (!P.use_cutoff || r2 < rc2_lo);
+                if (valid && !in && r2 < rc2_hi) near |= 1u << s8;
+                R e;
+                R f = pair_fe(pj.w, sj, in ? r2 : (R)1, e);
+                f = in ? f : (R)0;
+                if ((ENERGY || STATS) && in) pair_extra(slot, e);
+                fx += f * dx;
+                fy += f * dy;
+                fz += f * dz;
+                gx -= f * dx;
+                gy -= f * dy;
+                gz -= f * dz;
             };
-            pair_step(0);
-            rotate_j<0>(gx, gy, gz);
-            pair_step(1);
-            rotate_j<1>(gx, gy, gz);
-            pair_step(2);
-            rotate_j<2>(gx, gy, gz);
-            pair_step(3);
-            rotate_j<3>(gx, gy, gz);
-            pair_step(4);
-            rotate_j<4>(gx, gy, gz);
-            pair_step(5);
-            rotate_j<5>(gx, gy, gz);
-            pair_step(6);
-            rotate_j<6>(gx, gy, gz);
-            pair_step(7);
+            auto eight_steps = [&](auto MI) {
+                pair_step(0, MI);
+                rotate_j<0>(gx, gy, gz);
+                pair_step(1, MI);
+                rotate_j<1>(gx, gy, gz);
+                pair_step(2, MI);
+                rotate_j<2>(gx, gy, gz);
+                pair_step(3, MI);
+                rotate_j<3>(gx, gy, gz);
+                pair_step(4, MI);
+                rotate_j<4>(gx, gy, gz);
+                pair_step(5, MI);
+                rotate_j<5>(gx, gy, gz);
+                pair_step(6, MI);
+                rotate_j<6>(gx, gy, gz);
+                pair_step(7, MI);
+            };
+            if (mi) eight_steps(std::true_type{});
+            else eight_steps(std::false_type{});
             // The cutoff band: the reference's FP64 decision (rare pairs; their
             // j-forces go straight to the slot-ordered accumulator).
             while (__ballot(near != 0u)) {
@@ -469,9 +511,12 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
                     near &= near - 1u;
                     const int slot = grp * CLUSTER + s8;
                     if (within_cutoff_exact(pos, ia, s_ja[slot], bx, P.periodic, cutoff2)) {
-                        R dx, dy, dz;
-                        delta(slot, dx, dy, dz);
-                        const R f = pair_force(slot, dx, dy, dz, dx * dx + dy * dy + dz * dz);
+                        const Q4<R> pj = s_pq[slot];
+                        R dx, dy, dz, e;
+                        if (mi) delta(pj, dx, dy, dz, std::true_type{});
+                        else delta(pj, dx, dy, dz, std::false_type{});
+                        const R f = pair_fe(pj.w, s_se[slot], dx * dx + dy * dy + dz * dz, e);
+                        if (ENERGY || STATS) pair_extra(slot, e);
                         fx += f * dx;
                         fy += f * dy;
                         fz += f * dz;

@@ -36,6 +36,9 @@ struct PmeDev {
     size_t plane_shmem[2] = {};  // dynamic LDS per plane workgroup, 0: unavailable
     bool plane[2] = {};          // use the plane decomposition
     int spread_edge = 0;  // spread tile edge
+    // Atom-owned spread (chunks of sorted atoms summed in LDS boxes) instead
+    // of the tile-owned one; both write the same bits.
+    bool spread_atoms = false;
     size_t fft_shmem[2][3] = {};  // dynamic LDS per axis
     // Spread: per-atom stencils (base point, weights) and per-tile atom bins.
     int natoms = 0, ntiles = 0, bin_cap = 0;

@@ -245,6 +245,211 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
     }
 }
 
+// Atom-owned spread (the autotuner's alternative to the tile-owned one): one
+// workgroup per chunk of PME_CHUNK sorted slots, or of PME_CHUNK atoms in
+// index order while no sorted order is available.  The chunk's stencils are
+// computed once.  When the chunk lies in one system and its stencil points
+// fit a box of at most SPREAD_BOX_BYTES of LDS, the contributions are summed
+// in that box and the box is added to the grid with one device atomic per
+// non-zero point; otherwise every contribution is added to the grid
+// directly.  Each contribution is the tile-owned spread's fixed-point value,
+// and integer sums do not depend on their order, so both spreads write the
+// same bits.  The grid is cleared before the launch.
+constexpr int SPREAD_BOX_BYTES = 32768;
+constexpr int ATOM_SPREAD_THREADS = 256;
+static_assert(PME_CHUNK == 64, "the atom-owned spread computes a chunk's stencils in one wave");
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wrap_index(int i, int n) {
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+__device__ __forceinline__ void grid_add(long long* p, unsigned long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), v);
+}
+__device__ __forceinline__ void grid_add(double* p, double v) { atomicAdd(p, v); }
+
+template <class R, class G>
+__global__ __launch_bounds__(ATOM_SPREAD_THREADS) void k_spread_atoms(const int* __restrict__ chunk_off, int nsys,
+                                                                      const int* __restrict__ order, int natoms,
+                                                                      const double* __restrict__ pos,
+                                                                      const double* __restrict__ charge,
+                                                                      const int* __restrict__ atom_sys,
+                                                                      const SysInfo* __restrict__ sys, G* grid,
+                                                                      int small_charges, unsigned* status) {
+    constexpr bool FIXED = sizeof(G) == sizeof(long long) && G(0.5) == G(0);  // int64 grid
+    using Acc = typename std::conditional<FIXED, unsigned long long, double>::type;
+    constexpr bool I32 = FIXED && sizeof(R) == 4;
+    const bool use32 = I32 && small_charges;
+    // The box: int32 sums at 2^-24 (at most PME_CHUNK contributions of
+    // |q| w <= 2 x 0.2153 per point stay below 2^31) or Acc sums.
+    __shared__ __align__(8) unsigned char s_raw[SPREAD_BOX_BYTES];
+    int* box32 = reinterpret_cast<int*>(s_raw);
+    Acc* box = reinterpret_cast<Acc*>(s_raw);
+    __shared__ int s_base[3][PME_CHUNK];  // box-local (fit) or grid base point
+    __shared__ R s_w[15][PME_CHUNK];
+    __shared__ double s_q[PME_CHUNK];
+    __shared__ int s_sys[PME_CHUNK];
+    __shared__ int s_org[3], s_ext[3], s_fit, s_lsys;
+    const int t = threadIdx.x;
+    if (t < PME_CHUNK) {
+        int a = -1;
+        if (chunk_off) {
+            const SysInfo& S = sys[find_sys(chunk_off, nsys, blockIdx.x)];
+            const int slot = S.slot_off + (blockIdx.x - S.chunk_off) * PME_CHUNK + t;
+            a = slot < S.slot_off + S.nblocks * IBLOCK ? order[slot] : -1;
+        } else {
+            const int i = blockIdx.x * PME_CHUNK + t;
+            a = i < natoms ? i : -1;
+        }
+        const double q = a >= 0 ? charge[a] : 0.0;
+        const bool on = q != 0.0;  // uncharged atoms do not spread (pme.rs:315-330)
+        int si = 0, idx[3] = {0, 0, 0};
+        if (on) {
+            si = atom_sys[a];
+            const SysInfo& S = sys[si];
+            R w[3][5], dw[3][5];
+            stencil<R>(pos + 3 * a, S.box, S.grid, idx, w, dw);
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 5; ++k) s_w[5 * d + k][t] = w[d][k];
+        }
+        s_q[t] = q;
+        s_sys[t] = si;
+        const unsigned long long live = __ballot(on);
+        int fit = -1;  // nothing to spread
+        if (live) {
+            const int lead = __ffsll((long long)live) - 1;
+            const int lsys = __shfl(si, lead, 64);
+            const SysInfo& L = sys[lsys];
+            int rel[3], lo[3], ext[3];
+            for (int d = 0; d < 3; ++d) {
+                // Offsets from the lead atom's base point, taken in (-n/2, n/2].
+                const int n = L.grid[d];
+                int r = idx[d] - __shfl(idx[d], lead, 64);
+                if (r > n / 2) r -= n;
+                else if (r <= -(n - n / 2)) r += n;
+                rel[d] = r;
+                lo[d] = wave_min(on ? r : INT_MAX);
+                ext[d] = wave_max(on ? r : INT_MIN) - lo[d] + 5;
+            }
+            const long long vol = (long long)ext[0] * ext[1] * ext[2];
+            const int cap = SPREAD_BOX_BYTES / (use32 ? (int)sizeof(int) : (int)sizeof(Acc));
+            fit = (__ballot(on && si != lsys) == 0 && vol <= cap) ? 1 : 0;
+            for (int d = 0; d < 3; ++d) s_base[d][t] = fit ? rel[d] - lo[d] : idx[d];
+            if (t == 0) {
+                for (int d = 0; d < 3; ++d) {
+                    s_org[d] = wrap_index(__shfl(idx[d], lead, 64) + lo[d], L.grid[d]);
+                    s_ext[d] = ext[d];
+                }
+                s_lsys = lsys;
+            }
+        }
+        if (t == 0) s_fit = fit;
+    }
+    __syncthreads();
+    const int fit = s_fit;
+    if (fit < 0) return;
+    if (fit) {
+        const int e1 = s_ext[1], e2 = s_ext[2];
+        const int vol = s_ext[0] * e1 * e2;
+        for (int i = t; i < vol; i += ATOM_SPREAD_THREADS) {
+            if (I32 && use32) box32[i] = 0;
+            else box[i] = (Acc)0;
+        }
+        __syncthreads();
+        for (int item = t; item < PME_CHUNK * 25; item += ATOM_SPREAD_THREADS) {
+            const int m = item / 25, r = item - 25 * m;
+            const int ix = r / 5, iy = r - 5 * ix;
+            const double q = s_q[m];
+            if (q == 0.0) continue;
+            const int row = ((s_base[0][m] + ix) * e1 + s_base[1][m] + iy) * e2 + s_base[2][m];
+            const R wx = s_w[ix][m];
+            if (I32 && use32) {
+                const float qx = (float)(q * SPREAD32_SCALE) * (float)wx;
+                const float qxy = qx * (float)s_w[5 + iy][m];
+                for (int iz = 0; iz < 5; ++iz) {
+                    const int v = __float2int_rn(qxy * (float)s_w[10 + iz][m]);
+                    if (v) atomicAdd(box32 + row + iz, v);
+                }
+            } else {
+                const R wxy = wx * s_w[5 + iy][m];
+                for (int iz = 0; iz < 5; ++iz) {
+                    const double c = q * (double)(wxy * s_w[10 + iz][m]);
+                    if (FIXED) lds_add(reinterpret_cast<unsigned long long*>(box) + row + iz,
+                                       (long long)to_fixed(c, GRID_SCALE, status));
+                    else lds_add(reinterpret_cast<double*>(box) + row + iz, c);
+                }
+            }
+        }
+        __syncthreads();
+        const SysInfo& L = sys[s_lsys];
+        const int nx = L.grid[0], ny = L.grid[1], nz = L.grid[2];
+        G* g = grid + L.qgrid_off;
+        for (int i = t; i < vol; i += ATOM_SPREAD_THREADS) {
+            Acc v;
+            if (I32 && use32) {
+                const int v32 = box32[i];
+                if (!v32) continue;
+                v = (Acc)((long long)v32 * 65536ll);
+            } else {
+                v = box[i];
+                if (v == (Acc)0) continue;
+            }
+            const int lz = i % e2, rr = i / e2;
+            const int ly = rr % e1, lx = rr / e1;
+            const int gx = wrap_index(s_org[0] + lx, nx), gy = wrap_index(s_org[1] + ly, ny),
+                      gz = wrap_index(s_org[2] + lz, nz);
+            grid_add(g + ((size_t)gx * ny + gy) * nz + gz, v);
+        }
+        return;
+    }
+    // Direct adds (a chunk spread over more than a box, or over systems).
+    for (int item = t; item < PME_CHUNK * 25; item += ATOM_SPREAD_THREADS) {
+        const int m = item / 25, r = item - 25 * m;
+        const int ix = r / 5, iy = r - 5 * ix;
+        const double q = s_q[m];
+        if (q == 0.0) continue;
+        const SysInfo& S = sys[s_sys[m]];
+        const int nx = S.grid[0], ny = S.grid[1], nz = S.grid[2];
+        int gx = s_base[0][m] + ix, gy = s_base[1][m] + iy;
+        if (gx >= nx) gx -= nx;
+        if (gy >= ny) gy -= ny;
+        G* grow = grid + S.qgrid_off + ((size_t)gx * ny + gy) * nz;
+        const int gz0 = s_base[2][m];
+        const R wx = s_w[ix][m];
+        if (I32 && use32) {
+            const float qx = (float)(q * SPREAD32_SCALE) * (float)wx;
+            const float qxy = qx * (float)s_w[5 + iy][m];
+            for (int iz = 0; iz < 5; ++iz) {
+                const int v = __float2int_rn(qxy * (float)s_w[10 + iz][m]);
+                int gz = gz0 + iz;
+                if (gz >= nz) gz -= nz;
+                if (v) grid_add(grow + gz, (unsigned long long)((long long)v * 65536ll));
+            }
+        } else {
+            const R wxy = wx * s_w[5 + iy][m];
+            for (int iz = 0; iz < 5; ++iz) {
+                const double c = q * (double)(wxy * s_w[10 + iz][m]);
+                int gz = gz0 + iz;
+                if (gz >= nz) gz -= nz;
+                if (FIXED) {
+                    const long long f = to_fixed(c, GRID_SCALE, status);
+                    if (f) grid_add(grow + gz, (unsigned long long)f);
+                } else if (c != 0.0) {
+                    grid_add(grow + gz, c);
+                }
+            }
+        }
+    }
+}
+
 // Gather with five threads per atom (one per x plane of the stencil), atoms
 // in the spatially sorted order; the five partial gradients are combined in
 // LDS.
@@ -789,15 +994,30 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
     // without transforming).
     const bool own_order = pm.order_gen == nl.generation && pm.order.n == (size_t)nl.nslots;
     if (phases & PME_SPREAD) {
-        R* st_w = pme_weights<R>(pm);
         const int* order = atom_order ? (own_order ? pm.order.p : nullptr) : nl.sorted_atom.p;
-        const int nitems = order ? nl.nslots : pm.natoms;
-        hipLaunchKernelGGL((k_pme_bin<R>), dim3((nitems + 255) / 256), dim3(256), 0, s, nitems, order, pos, charge,
-                           pm.atom_sys, sys, pm.st_idx.p, st_w, pm.bin_count.p, pm.bins.p, pm.bin_cap, status);
-        hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
-                           pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap, qgrid,
-                           pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0, status);
-        if (!(phases & PME_FFT)) fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
+        const int small = pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0;
+        // The atom-owned spread adds to the grid with device atomics, so it
+        // serves the fixed-point grids only (FP64 atomic sums would depend on
+        // their order).
+        constexpr bool fixed_grid = sizeof(G) == sizeof(long long) && G(0.5) == G(0);
+        if (pm.spread_atoms && fixed_grid) {
+            fill_async<G>(qgrid, (size_t)pm.qpoints, (G)0, s);
+            const int blocks = order ? nl.nchunks : (pm.natoms + PME_CHUNK - 1) / PME_CHUNK;
+            if (blocks > 0)
+                hipLaunchKernelGGL((k_spread_atoms<R, G>), dim3(blocks), dim3(ATOM_SPREAD_THREADS), 0, s,
+                                   order ? nl.chunk_off.p : nullptr, pm.nsys, order, pm.natoms, pos, charge,
+                                   pm.atom_sys, sys, qgrid, small, status);
+        } else {
+            R* st_w = pme_weights<R>(pm);
+            const int nitems = order ? nl.nslots : pm.natoms;
+            hipLaunchKernelGGL((k_pme_bin<R>), dim3((nitems + 255) / 256), dim3(256), 0, s, nitems, order, pos,
+                               charge, pm.atom_sys, sys, pm.st_idx.p, st_w, pm.bin_count.p, pm.bins.p, pm.bin_cap,
+                               status);
+            hipLaunchKernelGGL((k_spread_tiles<R, G>), dim3(pm.ntiles), dim3(SPREAD_THREADS), 0, s, pm.tile_off.p,
+                               pm.nsys, sys, pm.st_idx.p, st_w, charge, pm.bin_count.p, pm.bins.p, pm.bin_cap,
+                               qgrid, small, status);
+            if (!(phases & PME_FFT)) fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
+        }
     }
     if (phases & PME_FFT) {
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);

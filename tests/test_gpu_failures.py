@@ -52,6 +52,9 @@ def test_list_overflow_replays_bit_for_bit(graph_steps):
 def test_bin_overflow_replays_bit_for_bit():
     s = water(1000, 3.1, seed=52)
     ref, ctx = _pair(10, s)
+    # The bins belong to the tile-owned spread (the atom-owned one has none).
+    t = ctx.launch_tuning()
+    ctx.set_launch_tuning(8, t["fft_lines_per_workgroup"], t["fft_plane"], t["pair_scale"])
     ctx.set_capacity(bin_cap=8)
     ref.step(20)
     ctx.step(20)

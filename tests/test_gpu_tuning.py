@@ -9,8 +9,9 @@ from helpers import water
 
 pytestmark = pytest.mark.gpu
 
-# (spread edge, FFT lines per line-pass workgroup, plane passes, pair scale)
-SHAPES = [(8, 16, 0, 1), (6, 4, 1, 2), (8, 2, 1, -2), (6, 8, 0, 1)]
+# (spread edge: 0 for the atom-owned spread, FFT lines per line-pass
+# workgroup, plane passes, pair scale)
+SHAPES = [(8, 16, 0, 1), (6, 4, 1, 2), (8, 2, 1, -2), (6, 8, 0, 1), (0, 16, 1, 1), (0, 4, 0, 2)]
 
 
 def _run(system, shape, steps=12, seed=5, precision=ommhip.PRECISION_MIXED):
@@ -50,10 +51,62 @@ def test_plane_passes_in_plain_fp64():
     assert abs(a["potential_energy"] - b["potential_energy"]) <= 1e-10 * abs(a["potential_energy"])
 
 
-def test_batch_shapes_match_lone_context():
+def _scaled_charges(system, factor):
+    # Effective charges factor x q through a global parameter's offsets
+    # (largest |q| above 2: the spreads' int64 path instead of int32 rounds).
+    f = system.nonbonded
+    f.add_global_parameter("qscale", factor - 1.0)
+    w = ommhip.WATER
+    for i in range(f.num_particles()):
+        f.add_particle_parameter_offset("qscale", i, w["q_o"] if i % 3 == 0 else w["q_h"], 0.0, 0.0)
+    return system
+
+
+@pytest.mark.parametrize("case", ["small_grid", "large_charges", "flexible_box"])
+def test_atom_owned_spread_matches_tile_spread(case):
+    # The atom-owned spread (edge 0) adds the tile spread's fixed-point
+    # contributions in another order: the energy evaluation (FP64 stencils),
+    # the MD steps (FP32 stencils) and, with the flexible barostat, triclinic
+    # boxes all give the same bits.
+    s = water(500, 2.5, seed=31)
+    steps = 20
+    if case == "large_charges":
+        # |q_O| = 2.0016: just past the int32 rounds' bound; a few steps (the
+        # scaled charges pull hydrogens into neighbouring oxygens).
+        s = _scaled_charges(s, 2.4)
+        steps = 5
+    runs = []
+    for edge in (8, 0):
+        c = ommhip.Context(s, temperature=300.0, friction=1.0, seed=7)
+        c.set_launch_tuning(edge, 8, 0, 1)
+        if case == "flexible_box":
+            c.set_flexible_barostat(1.0, 300.0, 5, seed=3)
+        first = c.get_state()
+        c.set_velocities_to_temperature(300.0, 4)
+        c.step(steps)
+        runs.append((first, c.get_state(), c.box()))
+    (fa, ea, ba), (fb, eb, bb) = runs
+    print(case, "grid", s.nonbonded.desc().pme_grid, "box", ba.tolist())
+    assert np.array_equal(fa["forces"], fb["forces"])
+    assert fa["potential_energy"] == fb["potential_energy"]
+    assert np.array_equal(ea["positions"], eb["positions"])
+    assert np.array_equal(ea["velocities"], eb["velocities"])
+    assert np.array_equal(ba, bb)
+
+
+def test_atom_owned_spread_is_mixed_only():
+    c = ommhip.Context(water(216, 1.86, seed=1), temperature=300.0, friction=1.0, seed=1,
+                       precision=ommhip.PRECISION_DOUBLE)
+    with pytest.raises(ommhip.OmmError):
+        c.set_launch_tuning(0, 16, 0, 1)
+    assert c.launch_tuning()["spread_tile_edge"] in (6, 8)
+
+
+@pytest.mark.parametrize("edge", [6, 0])
+def test_batch_shapes_match_lone_context(edge):
     sys_ = [water(216, 1.86, seed=70), water(800, 2.88, seed=71), water(500, 2.465, seed=72)]
     batch = ommhip.Batch(sys_, temperature=300.0, friction=1.0, seeds=[1, 2, 3])
-    batch.set_launch_tuning(6, 2, 1, 1)
+    batch.set_launch_tuning(edge, 2, 1, 1)
     assert batch.launch_tuning()["fft_plane"] == 1
     batch.set_velocities_to_temperature(300.0, 8)
     batch.step(12)
