@@ -21,12 +21,16 @@ namespace {
 // BOLTZMANN in kJ/mol/K (langevin.rs:12).
 const double BOLTZMANN = 1.380649e-23 * 6.02214076e23 / 1000.0;
 
-// Bonded terms in one launch.  Threads [0, ngroups) take one small molecule
-// each (at most four atoms joined by bonds, angles or exceptions, e.g. a
-// water) and sum its exceptions, bonds and angles in registers before one
-// fixed-point add per atom and component; the remaining threads take the
-// terms outside such molecules one each.  Indices are global over the batch;
-// energies go to the term's system.
+// Bonded terms in one launch.  Each small molecule (at most four atoms
+// joined by bonds, angles or exceptions, e.g. a water) takes GROUP_LANES
+// consecutive lanes: lane k evaluates the molecule's terms k, k + GROUP_LANES,
+// ... (its exceptions, then bonds, then angles) into per-atom sums in
+// registers, the lanes' sums are combined by a fixed butterfly of shuffles,
+// and the molecule's atoms receive one fixed-point add per component.  The
+// threads after the molecules' lanes take the terms outside such molecules
+// one each.  Indices are global over the batch; energies go to the term's
+// system.
+constexpr int GROUP_LANES = 8;
 __device__ __forceinline__ int group_slot(const int at[4], int i) {
     return i == at[0] ? 0 : (i == at[1] ? 1 : (i == at[2] ? 2 : 3));
 }
@@ -51,53 +55,70 @@ __global__ __launch_bounds__(256) void k_bonded(
     const double* __restrict__ charge, int use_pbc, int has_pme, double alpha, const double* __restrict__ ec6,
     double lj_alpha, const double* __restrict__ pos,
     const int* __restrict__ atom_sys, const SysInfo* __restrict__ sys, A* force, A* energy, unsigned* status) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double ee = 0.0, eb = 0.0, ea = 0.0;
     int se = -1, sb = -1, sa = -1;
-    if (t < ngroups) {
+    const int grouped = ngroups * GROUP_LANES;  // a multiple of GROUP_LANES: a molecule's lanes share a branch
+    if (t < grouped) {
+        const int g = t / GROUP_LANES, lane = t % GROUP_LANES;
         int at[4];
-        for (int k = 0; k < 4; ++k) at[k] = grp_atoms[4 * t + k];
+        for (int k = 0; k < 4; ++k) at[k] = grp_atoms[4 * g + k];
         const int si = atom_sys[at[0]];
         const Box& box = sys[si].box;
-        const int e0 = grp_e[t], e_end = grp_e[t + 1];
-        const int b0 = grp_b[t], b_end = grp_b[t + 1];
-        const int a0 = grp_a[t], a_end = grp_a[t + 1];
+        const int e0 = grp_e[g], ne_g = grp_e[g + 1] - e0;
+        const int b0 = grp_b[g], nb_g = grp_b[g + 1] - b0;
+        const int a0 = grp_a[g], na_g = grp_a[g + 1] - a0;
         double f[4][3] = {};
-        for (int e = e0; e < e_end; ++e) {
-            const int i = e1[e], j = e2[e];
-            double fi[3];
-            ee += exception_pair(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, box, use_pbc, has_pme,
-                                 alpha, ec6 ? ec6[e] : 0.0, lj_alpha, fi);
-            const double mi[3] = {-fi[0], -fi[1], -fi[2]};
-            add_slot(f, group_slot(at, i), fi);
-            add_slot(f, group_slot(at, j), mi);
-        }
-        for (int b = b0; b < b_end; ++b) {
-            const int i = b1[b], j = b2[b];
-            double fi[3];
-            eb += bond_pair(i, j, blen[b], bk[b], pos, fi);
-            const double mi[3] = {-fi[0], -fi[1], -fi[2]};
-            add_slot(f, group_slot(at, i), fi);
-            add_slot(f, group_slot(at, j), mi);
-        }
-        for (int a = a0; a < a_end; ++a) {
-            const int i = a1[a], j = a2[a], k = a3[a];
-            double fi[3], fk[3];
-            ea += angle_triple(i, j, k, ath[a], ak[a], pos, fi, fk);
-            const double fj[3] = {-(fi[0] + fk[0]), -(fi[1] + fk[1]), -(fi[2] + fk[2])};
-            add_slot(f, group_slot(at, i), fi);
-            add_slot(f, group_slot(at, k), fk);
-            add_slot(f, group_slot(at, j), fj);
+        for (int k = lane; k < ne_g + nb_g + na_g; k += GROUP_LANES) {
+            if (k < ne_g) {
+                const int e = e0 + k, i = e1[e], j = e2[e];
+                double fi[3];
+                ee += exception_pair(i, j, eqq[e], esig[e], eeps[e], charge[i], charge[j], pos, box, use_pbc,
+                                     has_pme, alpha, ec6 ? ec6[e] : 0.0, lj_alpha, fi);
+                const double mi[3] = {-fi[0], -fi[1], -fi[2]};
+                add_slot(f, group_slot(at, i), fi);
+                add_slot(f, group_slot(at, j), mi);
+            } else if (k < ne_g + nb_g) {
+                const int b = b0 + k - ne_g, i = b1[b], j = b2[b];
+                double fi[3];
+                eb += bond_pair(i, j, blen[b], bk[b], pos, fi);
+                const double mi[3] = {-fi[0], -fi[1], -fi[2]};
+                add_slot(f, group_slot(at, i), fi);
+                add_slot(f, group_slot(at, j), mi);
+            } else {
+                const int a = a0 + k - ne_g - nb_g, i = a1[a], j = a2[a], l = a3[a];
+                double fi[3], fk[3];
+                ea += angle_triple(i, j, l, ath[a], ak[a], pos, fi, fk);
+                const double fj[3] = {-(fi[0] + fk[0]), -(fi[1] + fk[1]), -(fi[2] + fk[2])};
+                add_slot(f, group_slot(at, i), fi);
+                add_slot(f, group_slot(at, l), fk);
+                add_slot(f, group_slot(at, j), fj);
+            }
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (at[k] >= 0)
-                for (int c = 0; c < 3; ++c) acc_add(force + 3 * at[k] + c, f[k][c], FORCE_SCALE, status);
-        se = e0 < e_end ? si : -1;
-        sb = b0 < b_end ? si : -1;
-        sa = a0 < a_end ? si : -1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int o = 1; o < GROUP_LANES; o <<= 1) f[k][c] += __shfl_xor(f[k][c], o, 64);
+        // Lane k adds the components k and k + GROUP_LANES of the 4 x 3.
+#pragma unroll
+        for (int q = 0; q < 12; q += GROUP_LANES) {
+            const int kc = q + lane;
+            if (kc < 12 && at[kc / 3] >= 0) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) v = kc == 3 * k + c ? f[k][c] : v;
+                acc_add(force + 3 * at[kc / 3] + kc % 3, v, FORCE_SCALE, status);
+            }
+        }
+        se = ne_g > 0 ? si : -1;
+        sb = nb_g > 0 ? si : -1;
+        sa = na_g > 0 ? si : -1;
     } else {
-        int u = t - ngroups;
+        int u = t - grouped;
         const int le = ne - ge, lb = nb - gb, la = na - ga;
         if (u < le) {
             const int e = ge + u, i = e1[e], j = e2[e];
@@ -1110,7 +1131,7 @@ void MdContext::build_bonded(const std::vector<int>& e1, const std::vector<int>&
 void MdContext::launch_bonded(hipStream_t s, long long* force_fx, double* force_d, long long* energy_fx,
                               double* energy_d) {
     const BondedDev& B = bonded_;
-    const int total = B.ngroups + (B.ne - B.ge) + (B.nb - B.gb) + (B.na - B.ga);
+    const int total = B.ngroups * GROUP_LANES + (B.ne - B.ge) + (B.nb - B.gb) + (B.na - B.ga);
     if (total == 0) return;
     const int blocks = (total + 255) / 256;
     auto go = [&](auto* force, auto* energy) {
