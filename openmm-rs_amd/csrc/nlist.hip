@@ -504,10 +504,19 @@ struct ExclLookup {
     }
 };
 
+// The cluster's atoms as FP32 offsets from its box centre, for the per-atom
+// reach test (where the block needs no minimum-image search, so the nearest
+// image to the centre is the nearest to every atom within reach).
+struct ClusterAtoms {
+    float x[CLUSTER], y[CLUSTER], z[CLUSTER];
+    bool on;          // apply the per-atom test
+    float reach2;     // (reach + 1e-4)^2: a margin far above the FP32 rounding
+};
+
 __device__ __forceinline__ void append_entry(int slot, int atom, const double p[3], int first, const double cq[3],
                                              const double hq[3], const Box& b, double reach2, int cap,
                                              int2* out, unsigned char* out_ex, const ExclLookup& ex, int& count,
-                                             unsigned long long below) {
+                                             unsigned long long below, const ClusterAtoms& ca) {
     bool ok = false;
     if (atom >= 0 && slot >= first) {
         if (slot < first + CLUSTER) {
@@ -515,6 +524,19 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
         } else {
             double d[3] = {p[0] - cq[0], p[1] - cq[1], p[2] - cq[2]};
             ok = box_gap2(d, hq, b) < reach2;
+            if (ok && ca.on) {
+                // Within reach of the box: keep the entry only if it is
+                // within reach of one of the cluster's atoms.
+                wrap_fast(d, b);
+                const float dx = (float)d[0], dy = (float)d[1], dz = (float)d[2];
+                float m2 = 3.0e38f;
+#pragma unroll
+                for (int i = 0; i < CLUSTER; ++i) {
+                    const float ex0 = dx - ca.x[i], ex1 = dy - ca.y[i], ex2 = dz - ca.z[i];
+                    m2 = fminf(m2, ex0 * ex0 + ex1 * ex1 + ex2 * ex2);
+                }
+                ok = m2 < ca.reach2;
+            }
         }
     }
     const unsigned long long m = __ballot(ok);
@@ -549,9 +571,9 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     const SysInfo* __restrict__ sys, const double* __restrict__ center, const float* __restrict__ half,
     const int* __restrict__ super_off, const double* __restrict__ super_center,
     const float* __restrict__ super_half, const int* __restrict__ atom_slot, const int* __restrict__ excl_ptr,
-    const int* __restrict__ excl_idx, const int2* __restrict__ excl_range, double reach, int cap, int nclusters,
-    int2* clist, unsigned char* clist_ex, int* ccount, int* tasks, int* ntasks, unsigned* status,
-    const int* flag) {
+    const int* __restrict__ excl_idx, const int2* __restrict__ excl_range, const int* __restrict__ minimg,
+    double reach, int cap, int nclusters, int2* clist, unsigned char* clist_ex, int* ccount, int* tasks, int* ntasks,
+    unsigned* status, const int* flag) {
     __shared__ int s_runs[CLIST_WAVES];
     __shared__ int s_base;
     if (!flag[0]) return;
@@ -604,7 +626,25 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
             cq[k] = rp[k] + 0.5 * (lo + hi);
             hq[k] = 0.5 * (hi - lo) * (1.0 + 1e-6) + 1e-6;
         }
-        append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below);
+        ClusterAtoms ca;
+        ca.on = !minimg[I];
+        ca.reach2 = (float)((reach + 1e-4) * (reach + 1e-4));
+        {
+            // Padding atoms sit far away and never pass.
+            float o[3] = {1.0e18f, 1.0e18f, 1.0e18f};
+            if (lane < CLUSTER && own_atom >= 0) {
+                double d[3] = {own_p[0] - cq[0], own_p[1] - cq[1], own_p[2] - cq[2]};
+                wrap_fast(d, b);
+                for (int k = 0; k < 3; ++k) o[k] = (float)d[k];
+            }
+#pragma unroll
+            for (int i = 0; i < CLUSTER; ++i) {
+                ca.x[i] = __shfl(o[0], i, 64);
+                ca.y[i] = __shfl(o[1], i, 64);
+                ca.z[i] = __shfl(o[2], i, 64);
+            }
+        }
+        append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below, ca);
 
         // Blocks [J0, min(J0 + 64, jlim)) whose boxes come within reach.
         auto scan_blocks = [&](const int J0, const int jlim) {
@@ -633,8 +673,8 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
                     for (int k = 0; k < 3; ++k) pa[k] = pos[3 * atom_a + k];
                 if (atom_b >= 0)
                     for (int k = 0; k < 3; ++k) pb[k] = pos[3 * atom_b + k];
-                append_entry(slot_a, atom_a, pa, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below);
-                append_entry(slot_b, atom_b, pb, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below);
+                append_entry(slot_a, atom_a, pa, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below, ca);
+                append_entry(slot_b, atom_b, pb, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below, ca);
             }
         };
         if (S.nblocks > SUPER_MIN) {
@@ -990,7 +1030,8 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
                        nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p,
                        nl.super_off.p, nl.super_center.p, nl.super_half.p, nl.atom_slot.p, nl.excl_ptr.p,
-                       nl.excl_idx.p, nl.excl_range.p, reach, nl.cap, nl.nclusters, nl.clist.p, nl.clist_ex.p,
+                       nl.excl_idx.p, nl.excl_range.p, nl.block_minimg.p, reach, nl.cap, nl.nclusters, nl.clist.p,
+                       nl.clist_ex.p,
                        nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
     OMM_HIP(hipGetLastError());
 }
