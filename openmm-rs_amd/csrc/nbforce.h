@@ -180,6 +180,9 @@ public:
     // The meshes' own slot order from the current list (after a list build
     // outside the step loop).
     void take_mesh_order(hipStream_t s);
+    // Every mesh holds a slot order of the current list layout: the whole
+    // mesh, gather included, may then run beside a list build.
+    bool mesh_order_ready() const;
     // Launch-shape autotuner, run once by the first calibration: the spread
     // tile edge, the FFT decomposition and lines per workgroup, and the pair
     // kernel's grid, each picked by timing the candidates on the current
@@ -287,6 +290,7 @@ private:
     hipEvent_t ev_list_ = nullptr;  // the list build is done (the gather may read the sorted order)
     hipEvent_t ev_caller_ = nullptr;  // device-pointer evaluations: the caller's stream reached the call
     bool joined_ = true;
+    bool follow_order_ = false;  // join() copies the list's order into the meshes' own
 
     std::vector<SysInfo> sys_h_;
     DevBuf<SysInfo> sys_dev_;

@@ -1292,10 +1292,13 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
     const NbSettings& c = cfg_[0];
     // The mesh runs on the side stream, concurrent with the list build and
     // the direct-space pairs; join() makes s wait for it.  It is forked
-    // before the list build: the spread bins the atoms in index order and the
-    // transforms never read the list, so only the gather (which walks the
-    // list's sorted order) waits for the build.
+    // before the list build: the spread and the gather visit the atoms in the
+    // meshes' own copy of an earlier build's order (any order of the same
+    // slot layout gives the same sums), and join() refreshes that copy once
+    // both branches are done.  Only without such a copy (a new layout) does
+    // the gather wait for the build and read its order.
     const bool mesh = recip && c.screened();
+    const bool own_order = mesh_order_ready();
     if (mesh) {
         OMM_HIP(hipEventRecord(ev_fork_, s));
         OMM_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
@@ -1321,12 +1324,16 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
     }
     if (mesh) {
         if (c.pme_mesh()) {
-            OMM_HIP(hipEventRecord(ev_list_, s));
-            OMM_HIP(hipStreamWaitEvent(side_, ev_list_, 0));
+            if (!own_order) {
+                OMM_HIP(hipEventRecord(ev_list_, s));
+                OMM_HIP(hipStreamWaitEvent(side_, ev_list_, 0));
+            }
             pme_run(pme_, mesh_mode(energy), nl_, pos_dev, part_.charge.p, sys_dev_.p, force_fx, force_d, energy_fx,
-                    energy_d, status, side_, PME_GATHER);
+                    energy_d, status, side_, PME_GATHER, own_order);
             if (ljpme())
-                lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_, PME_GATHER);
+                lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_, PME_GATHER,
+                        own_order);
+            follow_order_ = follow_order_ || (direct && rebuild != REBUILD_NONE);
         }
         OMM_HIP(hipEventRecord(ev_join_, side_));
         joined_ = false;
@@ -1346,6 +1353,17 @@ void NbOperator::join(hipStream_t s) {
     if (joined_) return;
     OMM_HIP(hipStreamWaitEvent(s, ev_join_, 0));
     joined_ = true;
+    if (follow_order_) {
+        pme_follow_order(pme_, ljpme() ? &pme_lj_ : nullptr, nl_, s);
+        follow_order_ = false;
+    }
+}
+
+bool NbOperator::mesh_order_ready() const {
+    const auto ready = [&](const PmeDev& pm) {
+        return pm.order_gen == nl_.generation && pm.order.n == (size_t)nl_.nslots && nl_.nslots > 0;
+    };
+    return ready(pme_) && (!ljpme() || ready(pme_lj_));
 }
 
 void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, double* force_d, long long* energy_fx,

@@ -50,8 +50,9 @@ struct PmeDev {
     DevBuf<float> st_wf;            // [natoms][16] weights, FP32 mesh
     DevBuf<double> st_wd;           // [natoms][16] weights, FP64 mesh
     // The mesh's own copy of a sorted slot order (slot -> atom, -1 padding),
-    // written by the gather from the list it follows and read by the next
-    // spread's binning.  The spread may then run while a list build rewrites
+    // copied from the list after a step's mesh and list build have both
+    // finished (pme_follow_order) and read by the next step's spread and
+    // gather.  Those may then run while a list build rewrites
     // nl.sorted_atom and still visit the atoms in a spatial order.  Valid for
     // the list layout of generation order_gen.
     DevBuf<int> order;
@@ -83,6 +84,11 @@ void pm_set_bin_cap(PmeDev& pm, int cap);
 // Copies the list's sorted order into the mesh's own order (after a list
 // build outside the step loop).
 void pme_take_order(PmeDev& pm, const NlistDev& nl, hipStream_t s);
+// The same copy into pm's order and, when given and allocated, other's, by a
+// kernel (graph-capturable: allocates nothing; does nothing while pm's order
+// is unallocated or sized for another layout).  s must follow both the list
+// build and every mesh phase that reads the orders.
+void pme_follow_order(PmeDev& pm, PmeDev* other, const NlistDev& nl, hipStream_t s);
 
 // Spread, FFT, convolution (+energy into slot E_RECIP of each system), inverse
 // FFT, gather (forces added).  Atoms with zero charge are skipped as in the
@@ -99,11 +105,12 @@ enum : int { PME_SPREAD = 1, PME_FFT = 2, PME_GATHER = 4, PME_ALL = 7 };
 enum MeshMode : int { MESH_F32 = 0, MESH_F64_FIXED = 1, MESH_F64 = 2 };
 // The atoms are visited in the list's sorted slot order (nl.sorted_atom),
 // which only sets locality: any permutation gives the same fixed-point sums.
-// atom_order: the spread's binning does not read nl.sorted_atom but the
-// mesh's own order (the sorted order of an earlier build, which any gather
-// refreshes), or index order before the first gather, so the spread and the
-// transforms may run while a list build rewrites the sorted order (the gather
-// always reads it and must follow the build).
+// atom_order: the spread and the gather do not read nl.sorted_atom but the
+// mesh's own order (the sorted order of an earlier build of the same layout,
+// copied by pme_take_order or pme_follow_order), so the mesh may run while a
+// list build rewrites the sorted order.  Without a valid own order the spread
+// takes index order and the gather nl.sorted_atom (it must then follow the
+// build).
 void pme_run(PmeDev& pm, int mode, const NlistDev& nl, const double* pos, const double* charge,
              const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx, double* energy_d,
              unsigned* status, hipStream_t s, int phases = PME_ALL, bool atom_order = false);

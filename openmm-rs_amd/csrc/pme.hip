@@ -461,8 +461,7 @@ __global__ __launch_bounds__(GATHER_ATOMS * 5) void k_gather5(const int* __restr
                                                               const double* __restrict__ pos,
                                                               const double* __restrict__ charge,
                                                               const SysInfo* __restrict__ sys,
-                                                              const R* __restrict__ phi, A* force, unsigned* status,
-                                                              int* order_out) {
+                                                              const R* __restrict__ phi, A* force, unsigned* status) {
     __shared__ R s_g[GATHER_ATOMS][5][3];
     const int si = find_sys(chunk_off, nsys, blockIdx.x);
     const SysInfo& S = sys[si];
@@ -472,9 +471,6 @@ __global__ __launch_bounds__(GATHER_ATOMS * 5) void k_gather5(const int* __restr
     const int k = t / 5, ix = t - k * 5;
     const int slot = S.slot_off + (blockIdx.x - S.chunk_off) * GATHER_ATOMS + k;
     const int a = slot < S.slot_off + S.nblocks * IBLOCK ? order[slot] : -1;
-    // The chunks of each system cover all of its slots, so the copy of the
-    // order is complete.
-    if (order_out && ix == 0 && slot < S.slot_off + S.nblocks * IBLOCK) order_out[slot] = a;
     const double q = a >= 0 ? charge[a] : 0.0;
     const Box& b = S.box;
     R g0 = 0, g1 = 0, g2 = 0;
@@ -1037,12 +1033,11 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
         }
     }
     if (phases & PME_GATHER) {
-        // The order buffer is allocated by pme_take_order (never here: this
-        // may run under graph capture).
-        const bool copy = pm.order.n == (size_t)nl.nslots;
+        // With atom_order the gather follows the same order as the spread:
+        // it may then run while a list build rewrites nl.sorted_atom.
+        const int* order = atom_order && own_order ? pm.order.p : nl.sorted_atom.p;
         hipLaunchKernelGGL((k_gather5<R, A>), dim3(nl.nchunks), dim3(GATHER_ATOMS * 5), 0, s, nl.chunk_off.p, pm.nsys,
-                           nl.sorted_atom.p, pos, charge, sys, phi, force, status, copy ? pm.order.p : nullptr);
-        if (copy) pm.order_gen = nl.generation;
+                           order, pos, charge, sys, phi, force, status);
     }
     OMM_HIP(hipGetLastError());
 }
@@ -1108,6 +1103,27 @@ void pm_set_bin_cap(PmeDev& pm, int cap) {
 }
 
 void pme_grow_bins(PmeDev& pm) { pm_set_bin_cap(pm, 2 * pm.bin_cap); }
+
+namespace {
+__global__ void k_copy_order(int n, const int* __restrict__ src, int* a, int* b) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int v = src[k];
+    a[k] = v;
+    if (b) b[k] = v;
+}
+}  // namespace
+
+void pme_follow_order(PmeDev& pm, PmeDev* other, const NlistDev& nl, hipStream_t s) {
+    const size_t n = (size_t)nl.nslots;
+    if (n == 0 || pm.order.n != n) return;
+    const bool both = other && other->order.n == n;
+    hipLaunchKernelGGL(k_copy_order, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.nslots, nl.sorted_atom.p,
+                       pm.order.p, both ? other->order.p : nullptr);
+    OMM_HIP(hipGetLastError());
+    pm.order_gen = nl.generation;
+    if (both) other->order_gen = nl.generation;
+}
 
 void pme_take_order(PmeDev& pm, const NlistDev& nl, hipStream_t s) {
     if (nl.nslots == 0 || nl.sorted_atom.n < (size_t)nl.nslots) return;
