@@ -1294,9 +1294,10 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
     // the direct-space pairs; join() makes s wait for it.  It is forked
     // before the list build: the spread and the gather visit the atoms in the
     // meshes' own copy of an earlier build's order (any order of the same
-    // slot layout gives the same sums), and join() refreshes that copy once
-    // both branches are done.  Only without such a copy (a new layout) does
-    // the gather wait for the build and read its order.
+    // slot layout gives the same sums), and the pair forces' fold copies this
+    // step's order into the meshes' other half for the next step.  Only
+    // without such a copy (a new layout) does the gather wait for the build
+    // and read its order (join() then takes the copy).
     const bool mesh = recip && c.screened();
     const bool own_order = mesh_order_ready();
     if (mesh) {
@@ -1333,14 +1334,19 @@ void NbOperator::enqueue(const double* pos_dev, bool direct, bool recip, bool en
             if (ljpme())
                 lj_mesh(mesh_mode(energy), pos_dev, force_fx, force_d, energy_fx, energy_d, status, side_, PME_GATHER,
                         own_order);
-            follow_order_ = follow_order_ || (direct && rebuild != REBUILD_NONE);
+            if (!own_order) follow_order_ = true;
         }
         OMM_HIP(hipEventRecord(ev_join_, side_));
         joined_ = false;
     }
     if (direct) {
+        int* next[2] = {nullptr, nullptr};
+        if (mesh && own_order && c.pme_mesh()) {
+            next[0] = pme_next_order(pme_, nl_);
+            if (ljpme()) next[1] = pme_next_order(pme_lj_, nl_);
+        }
         launch_pairs(mixed_, energy, nl_, pp_, part_, pos_dev, sys_dev_.p, force_fx, force_d, energy_fx, energy_d,
-                     status, nullptr, pair_grid_, s);
+                     status, nullptr, pair_grid_, s, true, next[0], next[1]);
         if (exceptions)
             launch_exceptions(mixed_, exc_, part_, pos_dev, sys_dev_.p, c.exceptions_pbc ? 1 : 0,
                               c.screened() ? 1 : 0, alpha_[0], ljpme() ? lj_alpha_[0] : 0.0, force_fx, force_d,
@@ -1360,10 +1366,7 @@ void NbOperator::join(hipStream_t s) {
 }
 
 bool NbOperator::mesh_order_ready() const {
-    const auto ready = [&](const PmeDev& pm) {
-        return pm.order_gen == nl_.generation && pm.order.n == (size_t)nl_.nslots && nl_.nslots > 0;
-    };
-    return ready(pme_) && (!ljpme() || ready(pme_lj_));
+    return pme_order_ready(pme_, nl_) && (!ljpme() || pme_order_ready(pme_lj_, nl_));
 }
 
 void NbOperator::profile(const double* pos_dev, int reps, long long* force_fx, double* force_d, long long* energy_fx,

@@ -51,11 +51,13 @@ struct ExceptionDev {
 // (atom order) and clears it.  energy_on: also accumulate the energy (in the mixed
 // form each pair energy is evaluated in FP64).  stats (nullable): [0] pair
 // count, [1] pair hash per system (no energy is accumulated then).  Energies
-// go to slot E_DIRECT of each system's E_SLOTS block.
+// go to slot E_DIRECT of each system's E_SLOTS block.  order_a/order_b
+// (nullable, nl.nslots ints each): the second kernel also copies the list's
+// slot order there.
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
                   const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d,
                   long long* energy_fx, double* energy_d, unsigned* status, unsigned long long* stats,
-                  int grid, hipStream_t s, bool combine = true);
+                  int grid, hipStream_t s, bool combine = true, int* order_a = nullptr, int* order_b = nullptr);
 
 // The second kernel alone: adds the slot-ordered pair forces into force_fx
 // (mixed) or force_d and clears them (after launch_pairs with combine = false).

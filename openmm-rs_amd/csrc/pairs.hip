@@ -604,17 +604,21 @@ double* slot_buffer<double>(const NlistDev& nl) {
 
 // Adds the slot-ordered pair forces into the atom-ordered accumulator and
 // clears them.  Atomic, because the mesh may add into the same accumulator
-// from another stream meanwhile.
+// from another stream meanwhile.  order_a/order_b (nullable): copies of the
+// slot order it reads, for the meshes' next run.
 __device__ __forceinline__ void add_raw(long long* p, long long v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
 }
 __device__ __forceinline__ void add_raw(double* p, double v) { unsafeAtomicAdd(p, v); }
 
 template <class A>
-__global__ void k_slot_forces(const int* __restrict__ sorted_atom, int nslots, A* fslot, A* force) {
+__global__ void k_slot_forces(const int* __restrict__ sorted_atom, int nslots, A* fslot, A* force, int* order_a,
+                              int* order_b) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     const int a = sorted_atom[s];
+    if (order_a) order_a[s] = a;
+    if (order_b) order_b[s] = a;
     for (int c = 0; c < 3; ++c) {
         const A v = fslot[(size_t)c * nslots + s];
         if (v != (A)0) {
@@ -627,29 +631,31 @@ __global__ void k_slot_forces(const int* __restrict__ sorted_atom, int nslots, A
 template <class R, class A, int ELEC, bool STATS, bool ENERGY>
 void launch_one(const NlistDev& nl, const PairParams& p, const R* qk, const R* sh, const R* e2,
                 const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
-                unsigned* status, unsigned long long* stats, int grid, bool combine, hipStream_t s) {
+                unsigned* status, unsigned long long* stats, int grid, bool combine, int* const* order_out,
+                hipStream_t s) {
     hipLaunchKernelGGL((k_pairs<R, A, ELEC, STATS, ENERGY>), dim3(grid), dim3(64), 0, s, pos, qk, sh, e2,
                        part.qk_d.p, part.sh_d.p, part.e2_d.p, nl.sorted_atom.p, nl.clist_ex.p, nl.block_center.p,
                        nl.block_minimg.p, nl.clist.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, nl.block_sys.p, sys_dev, p,
                        slot_buffer<A>(nl), energy, status, stats, nl.flag.p, nl.nsys + 1);
     if (combine)
         hipLaunchKernelGGL(k_slot_forces<A>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p,
-                           nl.nslots, slot_buffer<A>(nl), force);
+                           nl.nslots, slot_buffer<A>(nl), force, order_out[0], order_out[1]);
 }
 
 template <class R, class A, int ELEC>
 void launch_pairs_t(bool stats_on, bool energy_on, const NlistDev& nl, const PairParams& p, const R* qk, const R* sh,
                     const R* e2, const ParticleDev& part, const double* pos, const SysInfo* sys_dev, A* force, A* energy,
-                    unsigned* status, unsigned long long* stats, int grid, bool combine, hipStream_t s) {
+                    unsigned* status, unsigned long long* stats, int grid, bool combine, int* const* order_out,
+                    hipStream_t s) {
     if (stats_on)
         launch_one<R, A, ELEC, true, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
-                                            combine, s);
+                                            combine, order_out, s);
     else if (energy_on)
         launch_one<R, A, ELEC, false, true>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
-                                            combine, s);
+                                            combine, order_out, s);
     else
         launch_one<R, A, ELEC, false, false>(nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status, stats, grid,
-                                             combine, s);
+                                             combine, order_out, s);
 }
 
 }  // namespace
@@ -662,32 +668,32 @@ template <class R, class A>
 void dispatch_kind(int kind, bool st, bool energy_on, const NlistDev& nl, const PairParams& p, const R* qk,
                    const R* sh, const R* e2, const ParticleDev& part, const double* pos, const SysInfo* sys_dev,
                    A* force, A* energy, unsigned* status, unsigned long long* stats, int grid, bool combine,
-                   hipStream_t s) {
+                   int* const* order_out, hipStream_t s) {
     switch (kind) {
         case ELEC_EWALD:
             launch_pairs_t<R, A, ELEC_EWALD>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force, energy, status,
-                                             stats, grid, combine, s);
+                                             stats, grid, combine, order_out, s);
             break;
         case ELEC_REACTION_FIELD:
             launch_pairs_t<R, A, ELEC_REACTION_FIELD>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force,
-                                                      energy, status, stats, grid, combine, s);
+                                                      energy, status, stats, grid, combine, order_out, s);
             break;
         case ELEC_COULOMB:
             launch_pairs_t<R, A, ELEC_COULOMB>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force, energy,
-                                               status, stats, grid, combine, s);
+                                               status, stats, grid, combine, order_out, s);
             break;
         case ELEC_EWALD | PAIR_SWITCH:
             launch_pairs_t<R, A, ELEC_EWALD | PAIR_SWITCH>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force,
-                                                           energy, status, stats, grid, combine, s);
+                                                           energy, status, stats, grid, combine, order_out, s);
             break;
         case ELEC_REACTION_FIELD | PAIR_SWITCH:
             launch_pairs_t<R, A, ELEC_REACTION_FIELD | PAIR_SWITCH>(st, energy_on, nl, p, qk, sh, e2, part, pos,
                                                                     sys_dev, force, energy, status, stats, grid,
-                                                                    combine, s);
+                                                                    combine, order_out, s);
             break;
         case ELEC_EWALD | PAIR_LJPME:
             launch_pairs_t<R, A, ELEC_EWALD | PAIR_LJPME>(st, energy_on, nl, p, qk, sh, e2, part, pos, sys_dev, force,
-                                                          energy, status, stats, grid, combine, s);
+                                                          energy, status, stats, grid, combine, order_out, s);
             break;
         default:
             throw Error(OMM_ERR_INVALID, "no pair kernel for this electrostatics / switching combination");
@@ -698,25 +704,26 @@ void dispatch_kind(int kind, bool st, bool energy_on, const NlistDev& nl, const 
 void launch_pairs(bool mixed, bool energy_on, const NlistDev& nl, const PairParams& p, const ParticleDev& part,
                   const double* pos, const SysInfo* sys_dev, long long* force_fx, double* force_d, long long* energy_fx,
                   double* energy_d, unsigned* status, unsigned long long* stats, int grid, hipStream_t s,
-                  bool combine) {
+                  bool combine, int* order_a, int* order_b) {
     const bool st = stats != nullptr;
+    int* const order_out[2] = {order_a, order_b};
     const int kind = p.elec | (p.switching ? PAIR_SWITCH : 0) | (p.ljpme ? PAIR_LJPME : 0);
     if (mixed)
         dispatch_kind<float, long long>(kind, st, energy_on, nl, p, part.qk.p, part.sh.p, part.e2.p, part, pos,
-                                        sys_dev, force_fx, energy_fx, status, stats, grid, combine, s);
+                                        sys_dev, force_fx, energy_fx, status, stats, grid, combine, order_out, s);
     else
         dispatch_kind<double, double>(kind, st, energy_on, nl, p, part.qk_d.p, part.sh_d.p, part.e2_d.p, part, pos,
-                                      sys_dev, force_d, energy_d, status, stats, grid, combine, s);
+                                      sys_dev, force_d, energy_d, status, stats, grid, combine, order_out, s);
     OMM_HIP(hipGetLastError());
 }
 
 void launch_slot_forces(bool mixed, const NlistDev& nl, long long* force_fx, double* force_d, hipStream_t s) {
     if (mixed)
         hipLaunchKernelGGL(k_slot_forces<long long>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p,
-                           nl.nslots, nl.fslot_fx.p, force_fx);
+                           nl.nslots, nl.fslot_fx.p, force_fx, nullptr, nullptr);
     else
         hipLaunchKernelGGL(k_slot_forces<double>, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.sorted_atom.p,
-                           nl.nslots, nl.fslot_d.p, force_d);
+                           nl.nslots, nl.fslot_d.p, force_d, nullptr, nullptr);
     OMM_HIP(hipGetLastError());
 }
 

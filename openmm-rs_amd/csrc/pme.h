@@ -49,14 +49,16 @@ struct PmeDev {
     DevBuf<int> st_idx;             // [natoms][4] stencil base point
     DevBuf<float> st_wf;            // [natoms][16] weights, FP32 mesh
     DevBuf<double> st_wd;           // [natoms][16] weights, FP64 mesh
-    // The mesh's own copy of a sorted slot order (slot -> atom, -1 padding),
-    // copied from the list after a step's mesh and list build have both
-    // finished (pme_follow_order) and read by the next step's spread and
-    // gather.  Those may then run while a list build rewrites
-    // nl.sorted_atom and still visit the atoms in a spatial order.  Valid for
-    // the list layout of generation order_gen.
+    // The mesh's own copies of a sorted slot order (slot -> atom, -1
+    // padding), two halves of nslots: a step's spread and gather read half
+    // order_sel while the pair kernels' fold copies the list's order into the
+    // other half (pme_next_order), which the next step reads.  The mesh may
+    // then run while a list build rewrites nl.sorted_atom and still visit
+    // the atoms in a spatial order.  Both halves hold orders of the list
+    // layout of generation order_gen (any such order gives the same sums).
     DevBuf<int> order;
     int order_gen = -1;
+    int order_sel = 0;
 };
 
 // Grid dimensions must each be >= 5 (pme.rs:222-225) and at most 1024.
@@ -81,14 +83,20 @@ int pme_set_spread_edge(PmeDev& pm, std::vector<SysInfo>& sys, int edge, hipStre
 void pme_grow_bins(PmeDev& pm);
 // Reallocates the bins to `cap` atoms per spread tile.
 void pm_set_bin_cap(PmeDev& pm, int cap);
-// Copies the list's sorted order into the mesh's own order (after a list
-// build outside the step loop).
+// Copies the list's sorted order into both halves of the mesh's own order
+// (after a list build outside the step loop; allocates).
 void pme_take_order(PmeDev& pm, const NlistDev& nl, hipStream_t s);
-// The same copy into pm's order and, when given and allocated, other's, by a
-// kernel (graph-capturable: allocates nothing; does nothing while pm's order
-// is unallocated or sized for another layout).  s must follow both the list
-// build and every mesh phase that reads the orders.
+// The same copy into pm's and, when given, other's orders by a kernel
+// (graph-capturable: allocates nothing; skips an order not allocated for
+// this layout's size).  s must follow both the list build and every mesh
+// phase that reads the orders.
 void pme_follow_order(PmeDev& pm, PmeDev* other, const NlistDev& nl, hipStream_t s);
+// Whether pm holds orders of the current list layout.
+bool pme_order_ready(const PmeDev& pm, const NlistDev& nl);
+// After a step's mesh phases have been enqueued: the half of pm's order they
+// do not read (where the step's list order is to be copied, on a stream that
+// follows the list build), which the next step's mesh then reads.
+int* pme_next_order(PmeDev& pm, const NlistDev& nl);
 
 // Spread, FFT, convolution (+energy into slot E_RECIP of each system), inverse
 // FFT, gather (forces added).  Atoms with zero charge are skipped as in the

@@ -988,9 +988,10 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
     // Bin counts are zero between runs: set so at allocation, and cleared by
     // the first FFT pass after each spread (or by a fill when a run spreads
     // without transforming).
-    const bool own_order = pm.order_gen == nl.generation && pm.order.n == (size_t)nl.nslots;
+    const bool own_order = pme_order_ready(pm, nl);
+    const int* own = own_order ? pm.order.p + (size_t)pm.order_sel * nl.nslots : nullptr;
     if (phases & PME_SPREAD) {
-        const int* order = atom_order ? (own_order ? pm.order.p : nullptr) : nl.sorted_atom.p;
+        const int* order = atom_order ? own : nl.sorted_atom.p;
         const int small = pm.max_charge <= SPREAD32_MAX_CHARGE ? 1 : 0;
         // The atom-owned spread adds to the grid with device atomics, so it
         // serves the fixed-point grids only (FP64 atomic sums would depend on
@@ -1035,7 +1036,7 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
     if (phases & PME_GATHER) {
         // With atom_order the gather follows the same order as the spread:
         // it may then run while a list build rewrites nl.sorted_atom.
-        const int* order = atom_order && own_order ? pm.order.p : nl.sorted_atom.p;
+        const int* order = atom_order && own_order ? own : nl.sorted_atom.p;
         hipLaunchKernelGGL((k_gather5<R, A>), dim3(nl.nchunks), dim3(GATHER_ATOMS * 5), 0, s, nl.chunk_off.p, pm.nsys,
                            order, pos, charge, sys, phi, force, status);
     }
@@ -1105,31 +1106,42 @@ void pm_set_bin_cap(PmeDev& pm, int cap) {
 void pme_grow_bins(PmeDev& pm) { pm_set_bin_cap(pm, 2 * pm.bin_cap); }
 
 namespace {
+// Both halves of each given order.
 __global__ void k_copy_order(int n, const int* __restrict__ src, int* a, int* b) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int v = src[k];
-    a[k] = v;
-    if (b) b[k] = v;
+    if (a) a[k] = a[n + k] = v;
+    if (b) b[k] = b[n + k] = v;
 }
 }  // namespace
 
 void pme_follow_order(PmeDev& pm, PmeDev* other, const NlistDev& nl, hipStream_t s) {
-    const size_t n = (size_t)nl.nslots;
-    if (n == 0 || pm.order.n != n) return;
-    const bool both = other && other->order.n == n;
-    hipLaunchKernelGGL(k_copy_order, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.nslots, nl.sorted_atom.p,
-                       pm.order.p, both ? other->order.p : nullptr);
+    const size_t n2 = 2 * (size_t)nl.nslots;
+    int* a = nl.nslots > 0 && pm.order.n == n2 ? pm.order.p : nullptr;
+    int* b = nl.nslots > 0 && other && other->order.n == n2 ? other->order.p : nullptr;
+    if (!a && !b) return;
+    hipLaunchKernelGGL(k_copy_order, dim3((nl.nslots + 255) / 256), dim3(256), 0, s, nl.nslots, nl.sorted_atom.p, a,
+                       b);
     OMM_HIP(hipGetLastError());
-    pm.order_gen = nl.generation;
-    if (both) other->order_gen = nl.generation;
+    if (a) pm.order_gen = nl.generation;
+    if (b) other->order_gen = nl.generation;
+}
+
+bool pme_order_ready(const PmeDev& pm, const NlistDev& nl) {
+    return nl.nslots > 0 && pm.order_gen == nl.generation && pm.order.n == 2 * (size_t)nl.nslots;
+}
+
+int* pme_next_order(PmeDev& pm, const NlistDev& nl) {
+    pm.order_sel ^= 1;
+    return pm.order.p + (size_t)pm.order_sel * nl.nslots;
 }
 
 void pme_take_order(PmeDev& pm, const NlistDev& nl, hipStream_t s) {
     if (nl.nslots == 0 || nl.sorted_atom.n < (size_t)nl.nslots) return;
-    if (pm.order.n != (size_t)nl.nslots) pm.order.alloc(nl.nslots);
-    OMM_HIP(hipMemcpyAsync(pm.order.p, nl.sorted_atom.p, sizeof(int) * nl.nslots, hipMemcpyDeviceToDevice, s));
-    pm.order_gen = nl.generation;
+    if (pm.order.n != 2 * (size_t)nl.nslots) pm.order.alloc(2 * (size_t)nl.nslots);
+    pm.order_gen = -1;  // pme_follow_order sets it
+    pme_follow_order(pm, nullptr, nl, s);
 }
 
 namespace {
