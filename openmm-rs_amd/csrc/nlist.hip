@@ -293,14 +293,13 @@ __global__ void k_sort_cells(const int* __restrict__ cell_start, int ncells, int
         }
 }
 
-__device__ __forceinline__ void place_unit(int p, const int* __restrict__ sorted_units,
-                                           const int* __restrict__ unit_offset, int uniform,
+// Places the atoms of unit u, at sorted position p.
+__device__ __forceinline__ void place_unit(int p, int u, const int* __restrict__ unit_offset, int uniform,
                                            const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
                                            const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
                                            const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
                                            const int* __restrict__ excl_idx, int closed, int* sorted_atom,
                                            int* atom_slot, int2* excl_range, const int* flag) {
-    int u = sorted_units[p];
     const int si = unit_sys[u];
     if (!flag[1 + si]) return;  // same keys, same slots: nothing changes
     const SysInfo& S = sys[si];
@@ -337,26 +336,30 @@ __global__ void k_place_atoms(const int* __restrict__ sorted_units, const int* _
     if (!flag[0]) return;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nunits) return;
-    place_unit(p, sorted_units, unit_offset, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx,
-               closed, sorted_atom, atom_slot, excl_range, flag);
+    place_unit(p, sorted_units[p], unit_offset, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr,
+               excl_idx, closed, sorted_atom, atom_slot, excl_range, flag);
 }
 
-// Uniform units: one thread sorts the units of one cell and places them (no
-// size scan is needed between the two), one launch instead of two.
-__global__ void k_sort_place(const int* __restrict__ cell_start, int ncells, int* sorted_units, int uniform,
-                             const int* __restrict__ unit_ptr, const int* __restrict__ unit_atoms,
-                             const int* __restrict__ unit_pos, const int* __restrict__ unit_sys,
-                             const SysInfo* __restrict__ sys, const int* __restrict__ excl_ptr,
-                             const int* __restrict__ excl_idx, int closed, int* sorted_atom, int* atom_slot,
-                             int2* excl_range, const int* flag) {
+// Uniform units: one thread per unit of the scatter's order finds the unit's
+// rank by index among the units of its cell (the position the index sort of
+// the cell would give it; no size scan is needed) and places it, one launch
+// instead of a sort and a placement.  The scatter's order is left as is.
+__global__ void k_rank_place(const int* __restrict__ unit_key, const int* __restrict__ cell_start, int nunits,
+                             const int* __restrict__ scattered, int uniform, const int* __restrict__ unit_ptr,
+                             const int* __restrict__ unit_atoms, const int* __restrict__ unit_pos,
+                             const int* __restrict__ unit_sys, const SysInfo* __restrict__ sys,
+                             const int* __restrict__ excl_ptr, const int* __restrict__ excl_idx, int closed,
+                             int* sorted_atom, int* atom_slot, int2* excl_range, const int* flag) {
     if (!flag[0]) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nunits) return;
+    const int u = scattered[q];
+    const int c = unit_key[u];
     const int lo = cell_start[c], hi = cell_start[c + 1];
-    sort_cell(sorted_units, lo, hi);
-    for (int p = lo; p < hi; ++p)
-        place_unit(p, sorted_units, nullptr, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx,
-                   closed, sorted_atom, atom_slot, excl_range, flag);
+    int p = lo;
+    for (int k = lo; k < hi; ++k) p += scattered[k] < u ? 1 : 0;
+    place_unit(p, u, nullptr, uniform, unit_ptr, unit_atoms, unit_pos, unit_sys, sys, excl_ptr, excl_idx, closed,
+               sorted_atom, atom_slot, excl_range, flag);
 }
 
 // Slot range of each slot's exclusion partners once every atom is placed
@@ -492,8 +495,8 @@ struct ExclLookup {
     const int2* range;
     // Bit k set: the atom at slot first + k is an exclusion partner of atom
     // a (at slot `slot`); every partner is checked, however many there are.
-    __device__ __forceinline__ unsigned bits(int slot, int a, int first) const {
-        const int2 r = range[slot];
+    // r: range[slot], loaded by the caller ahead of the distance tests.
+    __device__ __forceinline__ unsigned bits(int2 r, int a, int first) const {
         if (r.y < first || r.x >= first + CLUSTER) return 0u;
         unsigned m = 0u;
         for (int k = ptr[a]; k < ptr[a + 1]; ++k) {
@@ -517,6 +520,7 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
                                              const double hq[3], const Box& b, double reach2, int cap,
                                              int2* out, unsigned char* out_ex, const ExclLookup& ex, int& count,
                                              unsigned long long below, const ClusterAtoms& ca) {
+    const int2 r = atom >= 0 ? ex.range[slot] : make_int2(INT_MAX, -1);
     bool ok = false;
     if (atom >= 0 && slot >= first) {
         if (slot < first + CLUSTER) {
@@ -544,7 +548,7 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
         const int w = count + __popcll(m & below);
         if (w < cap) {
             out[w] = make_int2(slot, atom);
-            out_ex[w] = (unsigned char)ex.bits(slot, atom, first);
+            out_ex[w] = (unsigned char)ex.bits(r, atom, first);
         }
     }
     count += __popcll(m);
@@ -564,7 +568,7 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int k) {
     return pos;
 }
 
-constexpr int CLIST_WAVES = 16;  // clusters (one per wave) per workgroup of k_clist
+constexpr int CLIST_WAVES = 2;  // clusters (one per wave) per workgroup of k_clist
 
 __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     const double* __restrict__ pos, const int* __restrict__ sorted_atom, const int* __restrict__ block_sys,
@@ -1001,10 +1005,10 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     hipLaunchKernelGGL(k_scatter_units, dim3(blocks_for(nl.nunits, 256)), dim3(256), 0, s, nl.unit_key.p, nl.nunits,
                        nl.cell_cursor.p, nl.sorted_units.p, flag);
     if (nl.uniform > 0 && nl.nsys == 1) {
-        hipLaunchKernelGGL(k_sort_place, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p,
-                           nl.ncells, nl.sorted_units.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p, nl.unit_pos.p,
-                           nl.unit_sys.p, sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed, nl.sorted_atom.p,
-                           nl.atom_slot.p, nl.excl_range.p, flag);
+        hipLaunchKernelGGL(k_rank_place, dim3(blocks_for(nl.nunits, 64)), dim3(64), 0, s, nl.unit_key.p,
+                           nl.cell_start.p, nl.nunits, nl.sorted_units.p, nl.uniform, nl.unit_ptr.p, nl.unit_atoms.p,
+                           nl.unit_pos.p, nl.unit_sys.p, sys_dev, nl.excl_ptr.p, nl.excl_idx.p, nl.closed,
+                           nl.sorted_atom.p, nl.atom_slot.p, nl.excl_range.p, flag);
     } else {
         hipLaunchKernelGGL(k_sort_cells, dim3(blocks_for(nl.ncells, 256)), dim3(256), 0, s, nl.cell_start.p,
                            nl.ncells, nl.sorted_units.p, nl.unit_ptr.p,
