@@ -66,6 +66,10 @@ struct NlistDev {
     DevBuf<int2> excl_range;
     DevBuf<double> block_center, pos_build;
     DevBuf<float> block_half;
+    // Per sorted slot: the FP32 offset of its atom from its block's centre
+    // (minimum image within the block), .w the atom (int bits); padding
+    // slots sit far away with atom -1.  Written with the block bounds.
+    DevBuf<float4> slot_pos;
     DevBuf<int> block_minimg;
     // Superblocks of SUPER consecutive blocks (systems with more than
     // SUPER_MIN blocks): bounding boxes and per-system offsets.

@@ -423,7 +423,8 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
                                                      const int* __restrict__ block_sys,
                                                      const SysInfo* __restrict__ sys, double reach,
                                                      double* center, float* half, int* minimg,
-                                                     double* pos_build, int* ntasks, const int* flag) {
+                                                     double* pos_build, float4* slot_pos, int* ntasks,
+                                                     const int* flag) {
     if (!flag[0]) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ntasks[0] = 0;  // k_clist appends the tasks anew
     int lane = threadIdx.x;
@@ -438,8 +439,9 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
         mn[k] = 1e300;
         mx[k] = -1e300;
     }
+    double d[3] = {0.0, 0.0, 0.0};
     if (a >= 0) {
-        double d[3] = {pos[3 * a] - pos[3 * ref], pos[3 * a + 1] - pos[3 * ref + 1], pos[3 * a + 2] - pos[3 * ref + 2]};
+        for (int k = 0; k < 3; ++k) d[k] = pos[3 * a + k] - pos[3 * ref + k];
         wrap_fast(d, b);
         for (int k = 0; k < 3; ++k) mn[k] = mx[k] = d[k];
         for (int k = 0; k < 3; ++k) pos_build[3 * a + k] = pos[3 * a + k];
@@ -449,6 +451,13 @@ __global__ __launch_bounds__(64) void k_block_bounds(const double* __restrict__ 
             mn[k] = fmin(mn[k], __shfl_xor(mn[k], off, 64));
             mx[k] = fmax(mx[k], __shfl_xor(mx[k], off, 64));
         }
+    if (valid) {
+        float4 q = make_float4(1.0e18f, 1.0e18f, 1.0e18f, __int_as_float(-1));
+        if (a >= 0)
+            q = make_float4((float)(d[0] - 0.5 * (mn[0] + mx[0])), (float)(d[1] - 0.5 * (mn[1] + mx[1])),
+                            (float)(d[2] - 0.5 * (mn[2] + mx[2])), __int_as_float(a));
+        slot_pos[blk * IBLOCK + (lane & 31)] = q;
+    }
     if ((lane & 31) == 0 && valid) {
         int need = b.triclinic;
         for (int k = 0; k < 3; ++k) {
@@ -507,20 +516,29 @@ struct ExclLookup {
     }
 };
 
-// The cluster's atoms as FP32 offsets from its box centre, for the per-atom
-// reach test (where the block needs no minimum-image search, so the nearest
-// image to the centre is the nearest to every atom within reach).
+// The cluster's atoms a_i as FP32 offsets from its box centre, for the
+// per-atom reach test of an offset d from that centre (valid where the
+// nearest image of d to the centre is the nearest to every cluster atom
+// within reach):  min_i |d - a_i|^2 = |d|^2 + min_i (|a_i|^2 - 2 d.a_i),
+// three FMAs per atom.  The terms are wave-uniform (scalar registers).
 struct ClusterAtoms {
-    float x[CLUSTER], y[CLUSTER], z[CLUSTER];
-    bool on;          // apply the per-atom test
+    float mx2[CLUSTER], my2[CLUSTER], mz2[CLUSTER], aa[CLUSTER];  // -2 a_i, |a_i|^2
+    bool on;          // apply the per-atom test on the FP64 path
     float reach2;     // (reach + 1e-4)^2: a margin far above the FP32 rounding
 };
+
+__device__ __forceinline__ float nearest2(float dx, float dy, float dz, const ClusterAtoms& ca) {
+    float m = 3.0e38f;
+#pragma unroll
+    for (int i = 0; i < CLUSTER; ++i) m = fminf(m, fmaf(dz, ca.mz2[i], fmaf(dy, ca.my2[i], fmaf(dx, ca.mx2[i], ca.aa[i]))));
+    return m + (dx * dx + dy * dy + dz * dz);
+}
 
 __device__ __forceinline__ void append_entry(int slot, int atom, const double p[3], int first, const double cq[3],
                                              const double hq[3], const Box& b, double reach2, int cap,
                                              int2* out, unsigned char* out_ex, const ExclLookup& ex, int& count,
                                              unsigned long long below, const ClusterAtoms& ca) {
-    const int2 r = atom >= 0 ? ex.range[slot] : make_int2(INT_MAX, -1);
+    const int2 r = slot >= 0 ? ex.range[slot] : make_int2(INT_MAX, -1);
     bool ok = false;
     if (atom >= 0 && slot >= first) {
         if (slot < first + CLUSTER) {
@@ -532,17 +550,32 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
                 // Within reach of the box: keep the entry only if it is
                 // within reach of one of the cluster's atoms.
                 wrap_fast(d, b);
-                const float dx = (float)d[0], dy = (float)d[1], dz = (float)d[2];
-                float m2 = 3.0e38f;
-#pragma unroll
-                for (int i = 0; i < CLUSTER; ++i) {
-                    const float ex0 = dx - ca.x[i], ex1 = dy - ca.y[i], ex2 = dz - ca.z[i];
-                    m2 = fminf(m2, ex0 * ex0 + ex1 * ex1 + ex2 * ex2);
-                }
-                ok = m2 < ca.reach2;
+                ok = nearest2((float)d[0], (float)d[1], (float)d[2], ca) < ca.reach2;
             }
         }
     }
+    const unsigned long long m = __ballot(ok);
+    if (ok) {
+        const int w = count + __popcll(m & below);
+        if (w < cap) {
+            out[w] = make_int2(slot, atom);
+            out_ex[w] = (unsigned char)ex.bits(r, atom, first);
+        }
+    }
+    count += __popcll(m);
+}
+
+// The same for a slot of a later block J whose offset from the cluster's
+// centre is dc (J's centre, nearest image) + q.xyz (the slot's offset from
+// J's centre), where every atom of J within reach of the cluster has that
+// offset as its nearest image (J and the cluster fit in half the box with
+// the reach).  The per-atom test alone decides (it implies the box test).
+__device__ __forceinline__ void append_near(int slot, float4 q, float dcx, float dcy, float dcz, int first, int cap,
+                                            int2* out, unsigned char* out_ex, const ExclLookup& ex, int& count,
+                                            unsigned long long below, const ClusterAtoms& ca) {
+    const int2 r = slot >= 0 ? ex.range[slot] : make_int2(INT_MAX, -1);
+    const int atom = __float_as_int(q.w);
+    const bool ok = atom >= 0 && nearest2(dcx + q.x, dcy + q.y, dcz + q.z, ca) < ca.reach2;
     const unsigned long long m = __ballot(ok);
     if (ok) {
         const int w = count + __popcll(m & below);
@@ -576,8 +609,8 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     const int* __restrict__ super_off, const double* __restrict__ super_center,
     const float* __restrict__ super_half, const int* __restrict__ atom_slot, const int* __restrict__ excl_ptr,
     const int* __restrict__ excl_idx, const int2* __restrict__ excl_range, const int* __restrict__ minimg,
-    double reach, int cap, int nclusters, int2* clist, unsigned char* clist_ex, int* ccount, int* tasks, int* ntasks,
-    unsigned* status, const int* flag) {
+    const float4* __restrict__ slot_pos, double reach, int cap, int nclusters, int2* clist, unsigned char* clist_ex,
+    int* ccount, int* tasks, int* ntasks, unsigned* status, const int* flag) {
     __shared__ int s_runs[CLIST_WAVES];
     __shared__ int s_base;
     if (!flag[0]) return;
@@ -641,11 +674,14 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
                 wrap_fast(d, b);
                 for (int k = 0; k < 3; ++k) o[k] = (float)d[k];
             }
+            const float oo = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+            const float m0 = -2.0f * o[0], m1 = -2.0f * o[1], m2 = -2.0f * o[2];
 #pragma unroll
             for (int i = 0; i < CLUSTER; ++i) {
-                ca.x[i] = __shfl(o[0], i, 64);
-                ca.y[i] = __shfl(o[1], i, 64);
-                ca.z[i] = __shfl(o[2], i, 64);
+                ca.mx2[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m0), i));
+                ca.my2[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m1), i));
+                ca.mz2[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m2), i));
+                ca.aa[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(oo), i));
             }
         }
         append_entry(own_slot, own_atom, own_p, first, cq, hq, b, reach2, cap, out, out_ex, ex, count, below, ca);
@@ -653,17 +689,44 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
         // Blocks [J0, min(J0 + 64, jlim)) whose boxes come within reach.
         auto scan_blocks = [&](const int J0, const int jlim) {
             const int J = J0 + lane;
-            bool pass = false;
+            bool pass = false, near = false;
+            float dcf[3] = {0.0f, 0.0f, 0.0f};
             if (J < jlim) {
                 double dc[3], h[3];
                 for (int k = 0; k < 3; ++k) {
                     dc[k] = center[3 * J + k] - cq[k];
                     h[k] = hq[k] + (double)half[3 * J + k];
                 }
+                wrap_fast(dc, b);
                 pass = box_gap2(dc, h, b) < reach2;
+                near = !b.triclinic;
+                for (int k = 0; k < 3; ++k) {
+                    dcf[k] = (float)dc[k];
+                    if (h[k] + reach >= 0.5 * b.v[k * 4] - 1e-4) near = false;
+                }
             }
             const unsigned long long m = __ballot(pass);
             const int npass = __popcll(m);
+            if (m != 0 && __ballot(pass && near) == m) {
+                // Every passing block takes the slot-offset test, four per
+                // round (loads together): lanes 0-31 the first of each pair,
+                // lanes 32-63 the second.
+                const float4 none = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+                for (int r = 0; r < npass; r += 4) {
+                    const int ka = r + (lane >> 5), kb = r + 2 + (lane >> 5);
+                    const int sa = ka < npass ? nth_set_bit(m, ka) : 0;
+                    const int sb = kb < npass ? nth_set_bit(m, kb) : 0;
+                    const int slot_a = ka < npass ? (J0 + sa) * IBLOCK + (lane & 31) : -1;
+                    const int slot_b = kb < npass ? (J0 + sb) * IBLOCK + (lane & 31) : -1;
+                    const float4 qa = slot_a >= 0 ? slot_pos[slot_a] : none;
+                    const float4 qb = slot_b >= 0 ? slot_pos[slot_b] : none;
+                    append_near(slot_a, qa, __shfl(dcf[0], sa, 64), __shfl(dcf[1], sa, 64), __shfl(dcf[2], sa, 64),
+                                first, cap, out, out_ex, ex, count, below, ca);
+                    append_near(slot_b, qb, __shfl(dcf[0], sb, 64), __shfl(dcf[1], sb, 64), __shfl(dcf[2], sb, 64),
+                                first, cap, out, out_ex, ex, count, below, ca);
+                }
+                return;
+            }
             // Four passing blocks per round: lanes 0-31 take the first of
             // each pair, lanes 32-63 the second.
             for (int r = 0; r < npass; r += 4) {
@@ -937,6 +1000,7 @@ void nlist_setup(NlistDev& nl, const std::vector<NlistSystem>& systems, std::vec
     if (eidx.empty()) eidx.push_back(0);
     nl.excl_idx.upload(eidx, s);
     nl.excl_range.alloc((size_t)nl.nslots);
+    nl.slot_pos.alloc((size_t)nl.nslots);
     nl.block_center.alloc(3 * (size_t)nblocks);
     nl.block_half.alloc(3 * (size_t)nblocks);
     nl.block_minimg.alloc(nblocks);
@@ -1027,14 +1091,15 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     double reach = nl.cutoff + nl.skin;
     hipLaunchKernelGGL(k_block_bounds, dim3(blocks_for(nl.nblocks, 2)), dim3(64), 0, s, pos, nl.sorted_atom.p,
                        nl.nblocks, nl.block_sys.p, sys_dev, reach, nl.block_center.p, nl.block_half.p,
-                       nl.block_minimg.p, nl.pos_build.p, nl.ntasks.p, flag);
+                       nl.block_minimg.p, nl.pos_build.p, nl.slot_pos.p, nl.ntasks.p, flag);
     if (nl.use_super)
         hipLaunchKernelGGL(k_super_bounds, dim3(nl.nsuper), dim3(64), 0, s, nl.nsuper, nl.super_off.p, nl.nsys,
                            sys_dev, nl.block_center.p, nl.block_half.p, nl.super_center.p, nl.super_half.p, flag);
     hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
                        nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p,
                        nl.super_off.p, nl.super_center.p, nl.super_half.p, nl.atom_slot.p, nl.excl_ptr.p,
-                       nl.excl_idx.p, nl.excl_range.p, nl.block_minimg.p, reach, nl.cap, nl.nclusters, nl.clist.p,
+                       nl.excl_idx.p, nl.excl_range.p, nl.block_minimg.p, nl.slot_pos.p, reach, nl.cap,
+                       nl.nclusters, nl.clist.p,
                        nl.clist_ex.p,
                        nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
     OMM_HIP(hipGetLastError());
