@@ -485,12 +485,18 @@ __global__ __launch_bounds__(64) void k_super_bounds(int nsuper, const int* __re
 
 
 // Cluster lists, one wave per cluster C of block I = C / CPB (CLIST_WAVES
-// clusters per workgroup).  Candidate blocks J > I of the same system pass
-// when their bounding boxes come within reach of the cluster's box; the atoms
-// of the passing blocks and the later atoms of I itself are then tested
-// against the cluster's box.  The list keeps, in increasing slot order, the
-// (sorted slot, atom) of the cluster's own atoms and of every later atom
-// within reach, and the cluster appends its pair-kernel tasks: one run of up
+// clusters per workgroup).  Each pair of blocks of a system is taken by one
+// of the two: in a system of nb blocks scanned flat, block I takes the nb/2
+// blocks after it, cyclically (J = I + d mod nb for 0 < d < nb/2, and
+// d = nb/2 from the first half), so every cluster scans and tests about the
+// same number of blocks, wherever it sits in the sorted order; with
+// superblocks, the blocks J > I.  Candidate blocks pass when their bounding
+// boxes come within reach of the cluster's box; the atoms of the passing
+// blocks and the later atoms of I itself are then tested against the
+// cluster's box (or its atoms).  The list keeps the (sorted slot, atom) of
+// the cluster's own atoms and of every atom of those blocks within reach
+// (the pair kernel takes an entry of the cluster's own slots only after the
+// i-atom), and the cluster appends its pair-kernel tasks: one run of up
 // to TPT tiles of JTILE entries per task, packed as cluster * 256 + first
 // tile (their order in the table does not matter, as forces are fixed-point
 // sums; one atomic per workgroup reserves the runs of its clusters).  The
@@ -540,8 +546,8 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
                                              unsigned long long below, const ClusterAtoms& ca) {
     const int2 r = slot >= 0 ? ex.range[slot] : make_int2(INT_MAX, -1);
     bool ok = false;
-    if (atom >= 0 && slot >= first) {
-        if (slot < first + CLUSTER) {
+    if (atom >= 0) {
+        if (slot >= first && slot < first + CLUSTER) {
             ok = true;
         } else {
             double d[3] = {p[0] - cq[0], p[1] - cq[1], p[2] - cq[2]};
@@ -767,7 +773,12 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
                 }
             }
         } else {
-            for (int J0 = I + 1; J0 < jend; J0 += 64) scan_blocks(J0, jend);
+            const int nb = S.nblocks, rel = I - S.block_off;
+            const int dmax = (nb - 1) / 2 + ((nb % 2 == 0 && rel < nb / 2) ? 1 : 0);
+            const int fwd_end = min(I + 1 + dmax, jend);
+            for (int J0 = I + 1; J0 < fwd_end; J0 += 64) scan_blocks(J0, fwd_end);
+            const int wrap_end = S.block_off + max(0, I + 1 + dmax - jend);
+            for (int J0 = S.block_off; J0 < wrap_end; J0 += 64) scan_blocks(J0, wrap_end);
         }
         if (count > cap) {
             if (lane == 0) atomicOr(status, ST_LIST_OVERFLOW);

@@ -321,15 +321,19 @@ __global__ __launch_bounds__(64) void k_pairs(const double* __restrict__ pos, co
             }
             __syncthreads();
 
-            // Valid entries of the lane's group: present, j_sorted > i_sorted, not excluded.
+            // Valid entries of the lane's group: present, not excluded, and
+            // after the i-atom where they are of its own cluster (entries
+            // before the cluster's first slot come from blocks the cluster
+            // takes cyclically).
             unsigned vmask = 0;
             if (ia >= 0) {
                 const int4* g4 = reinterpret_cast<const int4*>(s_js + grp * CLUSTER);
                 const int4 lo = g4[0], hi = g4[1];
                 const int jv[CLUSTER] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                const int cfirst = is & ~(CLUSTER - 1);
 #pragma unroll
                 for (int s8 = 0; s8 < CLUSTER; ++s8)
-                    if (jv[s8] != INT_MAX && jv[s8] > is) vmask |= 1u << s8;
+                    if (jv[s8] != INT_MAX && (jv[s8] > is || jv[s8] < cfirst)) vmask |= 1u << s8;
                 // Bit il of each of the group's eight exclusion bytes, gathered
                 // into bits 0-7: each byte's bit il moves to the byte's bit 0,
                 // and a multiply by 2^0 + 2^7 + 2^14 + 2^21 carries the bits of
