@@ -254,7 +254,8 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_tiles(const int* __re
 // non-zero point; otherwise every contribution is added to the grid
 // directly.  Each contribution is the tile-owned spread's fixed-point value,
 // and integer sums do not depend on their order, so both spreads write the
-// same bits.  The grid is cleared before the launch.
+// same bits.  The grid is zero at the launch (the forward transform after
+// the previous spread cleared every point it read).
 constexpr int SPREAD_BOX_BYTES = 32768;
 constexpr int ATOM_SPREAD_THREADS = 256;
 static_assert(PME_CHUNK == 64, "the atom-owned spread computes a chunk's stencils in one wave");
@@ -635,7 +636,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         Cx<R> v = {0, 0};
         if (L < nlines) {
             if (MODE == MODE_Z_R2C) {
-                v.x = (R)grid_real(qgrid[(size_t)L * nz + el]);
+                // The grid is zero between runs: each point is cleared as it
+                // is read.
+                G& g = qgrid[(size_t)L * nz + el];
+                v.x = (R)grid_real(g);
+                g = (G)0;
             } else if (MODE == MODE_Z_C2R) {
                 if (el < nzh) {
                     v = cgrid[(size_t)L * nzh + el];
@@ -864,7 +869,11 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_fft_plane(const int* __restri
             for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
                 const int li = e / nz, el = e - li * nz;
                 Cx<R> v = {0, 0};
-                if (y0 + li < ny) v.x = (R)grid_real(qgrid[(size_t)(y0 + li) * nz + el]);
+                if (y0 + li < ny) {
+                    G& g = qgrid[(size_t)(y0 + li) * nz + el];  // cleared as read
+                    v.x = (R)grid_real(g);
+                    g = (G)0;
+                }
                 Z[e] = v;
             }
             __syncthreads();
@@ -985,9 +994,10 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
            const SysInfo* sys, G* qgrid,
            R* cgrid, R* phi, const R* roots, const R* moduli, A* force, A* energy, unsigned* status, int phases,
            hipStream_t s) {
-    // Bin counts are zero between runs: set so at allocation, and cleared by
-    // the first FFT pass after each spread (or by a fill when a run spreads
-    // without transforming).
+    // Bin counts and the charge grid are zero between runs: set so at
+    // allocation, and cleared by the first FFT pass after each spread (the
+    // grid point by point as the pass reads it), or by fills when a run
+    // spreads without transforming.
     const bool own_order = pme_order_ready(pm, nl);
     const int* own = own_order ? pm.order.p + (size_t)pm.order_sel * nl.nslots : nullptr;
     if (phases & PME_SPREAD) {
@@ -998,7 +1008,6 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
         // their order).
         constexpr bool fixed_grid = sizeof(G) == sizeof(long long) && G(0.5) == G(0);
         if (pm.spread_atoms && fixed_grid) {
-            fill_async<G>(qgrid, (size_t)pm.qpoints, (G)0, s);
             const int blocks = order ? nl.nchunks : (pm.natoms + PME_CHUNK - 1) / PME_CHUNK;
             if (blocks > 0)
                 hipLaunchKernelGGL((k_spread_atoms<R, G>), dim3(blocks), dim3(ATOM_SPREAD_THREADS), 0, s,
@@ -1015,6 +1024,7 @@ void run_t(PmeDev& pm, const NlistDev& nl, bool atom_order, const double* pos, c
                                qgrid, small, status);
             if (!(phases & PME_FFT)) fill_async<int>(pm.bin_count.p, (size_t)pm.ntiles, 0, s);
         }
+        if (!(phases & PME_FFT)) fill_async<G>(qgrid, (size_t)pm.qpoints, (G)0, s);
     }
     if (phases & PME_FFT) {
         Cx<R>* cg = reinterpret_cast<Cx<R>*>(cgrid);
