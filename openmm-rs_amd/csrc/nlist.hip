@@ -576,10 +576,10 @@ __device__ __forceinline__ void append_entry(int slot, int atom, const double p[
 // J's centre), where every atom of J within reach of the cluster has that
 // offset as its nearest image (J and the cluster fit in half the box with
 // the reach).  The per-atom test alone decides (it implies the box test).
-__device__ __forceinline__ void append_near(int slot, float4 q, float dcx, float dcy, float dcz, int first, int cap,
-                                            int2* out, unsigned char* out_ex, const ExclLookup& ex, int& count,
-                                            unsigned long long below, const ClusterAtoms& ca) {
-    const int2 r = slot >= 0 ? ex.range[slot] : make_int2(INT_MAX, -1);
+// r: ex.range[slot] (loaded by the caller with q).
+__device__ __forceinline__ void append_near(int slot, float4 q, int2 r, float dcx, float dcy, float dcz, int first,
+                                            int cap, int2* out, unsigned char* out_ex, const ExclLookup& ex,
+                                            int& count, unsigned long long below, const ClusterAtoms& ca) {
     const int atom = __float_as_int(q.w);
     const bool ok = atom >= 0 && nearest2(dcx + q.x, dcy + q.y, dcz + q.z, ca) < ca.reach2;
     const unsigned long long m = __ballot(ok);
@@ -714,22 +714,30 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
             const unsigned long long m = __ballot(pass);
             const int npass = __popcll(m);
             if (m != 0 && __ballot(pass && near) == m) {
-                // Every passing block takes the slot-offset test, four per
-                // round (loads together): lanes 0-31 the first of each pair,
-                // lanes 32-63 the second.
+                // Every passing block takes the slot-offset test, NEAR_ROUND
+                // per round with all their loads issued together: lanes 0-31
+                // the first of each pair, lanes 32-63 the second.
+                constexpr int NEAR_ROUND = 8;
                 const float4 none = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
-                for (int r = 0; r < npass; r += 4) {
-                    const int ka = r + (lane >> 5), kb = r + 2 + (lane >> 5);
-                    const int sa = ka < npass ? nth_set_bit(m, ka) : 0;
-                    const int sb = kb < npass ? nth_set_bit(m, kb) : 0;
-                    const int slot_a = ka < npass ? (J0 + sa) * IBLOCK + (lane & 31) : -1;
-                    const int slot_b = kb < npass ? (J0 + sb) * IBLOCK + (lane & 31) : -1;
-                    const float4 qa = slot_a >= 0 ? slot_pos[slot_a] : none;
-                    const float4 qb = slot_b >= 0 ? slot_pos[slot_b] : none;
-                    append_near(slot_a, qa, __shfl(dcf[0], sa, 64), __shfl(dcf[1], sa, 64), __shfl(dcf[2], sa, 64),
-                                first, cap, out, out_ex, ex, count, below, ca);
-                    append_near(slot_b, qb, __shfl(dcf[0], sb, 64), __shfl(dcf[1], sb, 64), __shfl(dcf[2], sb, 64),
-                                first, cap, out, out_ex, ex, count, below, ca);
+                for (int r = 0; r < npass; r += NEAR_ROUND) {
+                    int src[NEAR_ROUND / 2], sl[NEAR_ROUND / 2];
+                    float4 q[NEAR_ROUND / 2];
+                    int2 rg[NEAR_ROUND / 2];
+#pragma unroll
+                    for (int u = 0; u < NEAR_ROUND / 2; ++u) {
+                        const int kk = r + 2 * u + (lane >> 5);
+                        src[u] = kk < npass ? nth_set_bit(m, kk) : 0;
+                        sl[u] = kk < npass ? (J0 + src[u]) * IBLOCK + (lane & 31) : -1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < NEAR_ROUND / 2; ++u) {
+                        q[u] = sl[u] >= 0 ? slot_pos[sl[u]] : none;
+                        rg[u] = sl[u] >= 0 ? excl_range[sl[u]] : make_int2(INT_MAX, -1);
+                    }
+#pragma unroll
+                    for (int u = 0; u < NEAR_ROUND / 2; ++u)
+                        append_near(sl[u], q[u], rg[u], __shfl(dcf[0], src[u], 64), __shfl(dcf[1], src[u], 64),
+                                    __shfl(dcf[2], src[u], 64), first, cap, out, out_ex, ex, count, below, ca);
                 }
                 return;
             }
