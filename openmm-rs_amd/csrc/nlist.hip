@@ -484,8 +484,8 @@ __global__ __launch_bounds__(64) void k_super_bounds(int nsuper, const int* __re
 }
 
 
-// Cluster lists, one wave per cluster C of block I = C / CPB (CLIST_WAVES
-// clusters per workgroup).  Each pair of blocks of a system is taken by one
+// Cluster lists, one wave per cluster C of block I = C / CPB (W clusters per
+// workgroup).  Each pair of blocks of a system is taken by one
 // of the two: in a system of nb blocks scanned flat, block I takes the nb/2
 // blocks after it, cyclically (J = I + d mod nb for 0 < d < nb/2, and
 // d = nb/2 from the first half), so every cluster scans and tests about the
@@ -607,9 +607,14 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int k) {
     return pos;
 }
 
-constexpr int CLIST_WAVES = 2;  // clusters (one per wave) per workgroup of k_clist
+// Clusters (one per wave) per workgroup of k_clist: two for a single system
+// (small workgroups spread over every compute unit), sixteen for batches,
+// where the workgroups of unflagged systems only append their tasks and one
+// task reservation per workgroup keeps the atomics on the counter few.
+constexpr int CLIST_WAVES_SINGLE = 2, CLIST_WAVES_BATCH = 16;
 
-__global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_clist(
     const double* __restrict__ pos, const int* __restrict__ sorted_atom, const int* __restrict__ block_sys,
     const SysInfo* __restrict__ sys, const double* __restrict__ center, const float* __restrict__ half,
     const int* __restrict__ super_off, const double* __restrict__ super_center,
@@ -617,12 +622,12 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     const int* __restrict__ excl_idx, const int2* __restrict__ excl_range, const int* __restrict__ minimg,
     const float4* __restrict__ slot_pos, double reach, int cap, int nclusters, int2* clist, unsigned char* clist_ex,
     int* ccount, int* tasks, int* ntasks, unsigned* status, const int* flag) {
-    __shared__ int s_runs[CLIST_WAVES];
+    __shared__ int s_runs[W];
     __shared__ int s_base;
     if (!flag[0]) return;
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
-    const int C = blockIdx.x * CLIST_WAVES + wave;
+    const int C = blockIdx.x * W + wave;
     const int I = C / CPB;
     const int si = C < nclusters ? block_sys[I] : 0;
     const int first = C * CLUSTER;  // the cluster's first slot
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(64 * CLIST_WAVES) void k_clist(
     __syncthreads();
     if (threadIdx.x == 0) {
         int all = 0;
-        for (int w = 0; w < CLIST_WAVES; ++w) all += s_runs[w];
+        for (int w = 0; w < W; ++w) all += s_runs[w];
         s_base = all > 0 ? atomicAdd(ntasks, all) : 0;
     }
     __syncthreads();
@@ -1114,13 +1119,17 @@ void nlist_build(NlistDev& nl, const double* pos, const SysInfo* sys_dev, unsign
     if (nl.use_super)
         hipLaunchKernelGGL(k_super_bounds, dim3(nl.nsuper), dim3(64), 0, s, nl.nsuper, nl.super_off.p, nl.nsys,
                            sys_dev, nl.block_center.p, nl.block_half.p, nl.super_center.p, nl.super_half.p, flag);
-    hipLaunchKernelGGL(k_clist, dim3(blocks_for(nl.nclusters, CLIST_WAVES)), dim3(64 * CLIST_WAVES), 0, s, pos,
-                       nl.sorted_atom.p, nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p,
-                       nl.super_off.p, nl.super_center.p, nl.super_half.p, nl.atom_slot.p, nl.excl_ptr.p,
-                       nl.excl_idx.p, nl.excl_range.p, nl.block_minimg.p, nl.slot_pos.p, reach, nl.cap,
-                       nl.nclusters, nl.clist.p,
-                       nl.clist_ex.p,
-                       nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
+    auto clist = [&](auto kernel, int w) {
+        hipLaunchKernelGGL(kernel, dim3(blocks_for(nl.nclusters, w)), dim3(64 * w), 0, s, pos, nl.sorted_atom.p,
+                           nl.block_sys.p, sys_dev, nl.block_center.p, nl.block_half.p, nl.super_off.p,
+                           nl.super_center.p, nl.super_half.p, nl.atom_slot.p, nl.excl_ptr.p, nl.excl_idx.p,
+                           nl.excl_range.p, nl.block_minimg.p, nl.slot_pos.p, reach, nl.cap, nl.nclusters, nl.clist.p,
+                           nl.clist_ex.p, nl.ccount.p, nl.tasks.p, nl.ntasks.p, status, flag);
+    };
+    if (nl.nsys == 1)
+        clist(k_clist<CLIST_WAVES_SINGLE>, CLIST_WAVES_SINGLE);
+    else
+        clist(k_clist<CLIST_WAVES_BATCH>, CLIST_WAVES_BATCH);
     OMM_HIP(hipGetLastError());
 }
 
