@@ -589,6 +589,31 @@ __device__ Cx<R>* fft_lds(Cx<R>* a, Cx<R>* b, int n, const int* radices, const C
     return a;
 }
 
+// The z passes transform two real lines y = 2p, 2p + 1 of an x plane at once
+// as the complex line a + i b (the last line alone when ny is odd).  From
+// Z = FFT(a + i b) at k and at (n - k) mod n, the half spectra are
+//   A = (Z_k + conj(Z_{n-k})) / 2,  B = (Z_k - conj(Z_{n-k})) / (2i);
+// the inverse transforms A_full + i B_full (each rebuilt from its half by
+// Hermitian symmetry) and reads a and b as the real and imaginary parts.
+// Every kernel pairs the lines the same way and calls the same routines, so
+// the plane and line passes keep giving the same bits.
+template <class R>
+__device__ __forceinline__ void unpack_pair(Cx<R> U, Cx<R> W, Cx<R>& A, Cx<R>& B) {
+    A = Cx<R>{(R)0.5 * (U.x + W.x), (R)0.5 * (U.y - W.y)};
+    B = Cx<R>{(R)0.5 * (U.y + W.y), (R)0.5 * (W.x - U.x)};
+}
+template <class R>
+__device__ __forceinline__ Cx<R> pack_pair(Cx<R> A, Cx<R> B) {
+    return Cx<R>{A.x - B.y, A.y + B.x};
+}
+// Element el of the full spectrum of a real line from its half h[0 .. nz/2].
+template <class R>
+__device__ __forceinline__ Cx<R> hermitian_at(const Cx<R>* h, int el, int nz, int nzh) {
+    if (el < nzh) return h[el];
+    const Cx<R> v = h[nz - el];
+    return Cx<R>{v.x, -v.y};
+}
+
 template <class R, class G, class A, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk_off, int nsys,
                                                      const SysInfo* __restrict__ sys, G* __restrict__ qgrid,
@@ -609,7 +634,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
     const int n = S.grid[axis];
     const int LINES = S.fft_lines[sizeof(R) == 8 ? 1 : 0][axis];
     const int T = FFT_THREADS / LINES;
-    const int nlines = zmode ? nx * ny : (ymode ? nx * nzh : ny * nzh);
+    const int nyp = (ny + 1) / 2;  // z passes: line pairs per x plane
+    const int nlines = zmode ? nx * nyp : (ymode ? nx * nzh : ny * nzh);
     const int line0 = (blockIdx.x - blk_off[si]) * LINES;
     qgrid += S.qgrid_off;
     phi += S.qgrid_off;
@@ -635,18 +661,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         const int L = line0 + li;
         Cx<R> v = {0, 0};
         if (L < nlines) {
-            if (MODE == MODE_Z_R2C) {
-                // The grid is zero between runs: each point is cleared as it
-                // is read.
-                G& g = qgrid[(size_t)L * nz + el];
-                v.x = (R)grid_real(g);
-                g = (G)0;
-            } else if (MODE == MODE_Z_C2R) {
-                if (el < nzh) {
-                    v = cgrid[(size_t)L * nzh + el];
+            if (zmode) {
+                const int x = L / nyp, y = 2 * (L - x * nyp);
+                const size_t row = (size_t)x * ny + y;
+                if (MODE == MODE_Z_R2C) {
+                    // The grid is zero between runs: each point is cleared as
+                    // it is read.
+                    G& g0 = qgrid[row * nz + el];
+                    v.x = (R)grid_real(g0);
+                    g0 = (G)0;
+                    if (y + 1 < ny) {
+                        G& g1 = qgrid[(row + 1) * nz + el];
+                        v.y = (R)grid_real(g1);
+                        g1 = (G)0;
+                    }
                 } else {
-                    v = cgrid[(size_t)L * nzh + (nz - el)];
-                    v.y = -v.y;
+                    const Cx<R> a = hermitian_at(cgrid + row * nzh, el, nz, nzh);
+                    const Cx<R> b = y + 1 < ny ? hermitian_at(cgrid + (row + 1) * nzh, el, nz, nzh) : Cx<R>{0, 0};
+                    v = pack_pair(a, b);
                 }
             } else if (ymode) {
                 const int x = L / nzh, kz = L - x * nzh;
@@ -769,6 +801,20 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
     }
 
     // Store.
+    if (MODE == MODE_Z_R2C) {
+        for (int e = threadIdx.x; e < LINES * nzh; e += FFT_THREADS) {
+            const int li = e / nzh, el = e - li * nzh;
+            const int L = line0 + li;
+            if (L >= nlines) continue;
+            const int x = L / nyp, y = 2 * (L - x * nyp);
+            Cx<R> a, b;
+            unpack_pair(res[li * n + el], res[li * n + (el ? n - el : 0)], a, b);
+            Cx<R>* row = cgrid + ((size_t)x * ny + y) * nzh;
+            row[el] = a;
+            if (y + 1 < ny) row[nzh + el] = b;
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < LINES * n; e += FFT_THREADS) {
         int li, el;
         if (zmode) {
@@ -781,10 +827,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
         const int L = line0 + li;
         if (L >= nlines) continue;
         const Cx<R> v = res[li * n + el];
-        if (MODE == MODE_Z_R2C) {
-            if (el < nzh) cgrid[(size_t)L * nzh + el] = v;
-        } else if (MODE == MODE_Z_C2R) {
-            phi[(size_t)L * nz + el] = v.x;
+        if (MODE == MODE_Z_C2R) {
+            const int x = L / nyp, y = 2 * (L - x * nyp);
+            const size_t row = (size_t)x * ny + y;
+            phi[row * nz + el] = v.x;
+            if (y + 1 < ny) phi[(row + 1) * nz + el] = v.y;
         } else if (ymode) {
             const int x = L / nzh, kz = L - x * nzh;
             cgrid[((size_t)x * ny + el) * nzh + kz] = v;
@@ -796,17 +843,18 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft(const int* __restrict__ blk
 }
 
 // Plane passes: one workgroup per x plane of a system transforms the plane
-// along z and along y in LDS.  Forward (DIR 0): real z lines to the half
-// spectrum (kept as P[kz][y]), then y lines, stored as cgrid[x][ky][kz].
-// Inverse (DIR 1): y lines of the half spectrum, then the z lines rebuilt
+// along z and along y in LDS.  Forward (DIR 0): pairs of real z lines to their
+// half spectra (kept as P[kz][y]), then y lines, stored as cgrid[x][ky][kz].
+// Inverse (DIR 1): y lines of the half spectrum, then pairs of z lines rebuilt
 // from it by Hermitian symmetry and transformed to real values.  Every line
-// goes through fft_lds with the data the corresponding line pass would load,
-// so the results are the same bits as the five-pass form.
+// goes through fft_lds (and the pair packing) with the data the corresponding
+// line pass would load, so the results are the same bits as the five-pass
+// form.
 //
 // LDS (complex elements): roots along z [nz] and y [ny], P and Q
 // [lyp x ny] (lyp = nzh rounded up to a multiple of the y lines per call),
-// and the z-line ping-pong buffers [2 x lz x nz], which share the one of P
-// and Q that does not hold live data when they fit in it.
+// and the ping-pong buffers of lz z-line pairs [2 x lz x nz], which share
+// the one of P and Q that does not hold live data when they fit in it.
 constexpr int PLANE_THREADS = 1024;
 constexpr size_t PLANE_LDS_MAX = 160 * 1024 - 256;  // gfx950 LDS per workgroup, less the static arrays
 
@@ -845,6 +893,7 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_fft_plane(const int* __restri
     const int x = blockIdx.x - plane_off[si];
     constexpr int prec = sizeof(R) == 8 ? 1 : 0;
     const int lz = S.plane_lines[prec][0], ly = S.plane_lines[prec][1];
+    const int nyp = (ny + 1) / 2;  // z line pairs
     const PlaneLayout L = plane_layout(ny, nz, nzh, lz, ly);
     Cx<R>* rz = reinterpret_cast<Cx<R>*>(smem_raw);
     Cx<R>* ry = rz + nz;
@@ -865,14 +914,21 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_fft_plane(const int* __restri
     if (DIR == 0) {
         Cx<R>* Z = L.z_shared ? Q : extra;
         __syncthreads();  // roots and radices
-        for (int y0 = 0; y0 < ny; y0 += lz) {
+        for (int p0 = 0; p0 < nyp; p0 += lz) {
+            // Line pairs (y, y + 1) = (2p, 2p + 1), as the line passes pair them.
             for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
                 const int li = e / nz, el = e - li * nz;
+                const int y = 2 * (p0 + li);
                 Cx<R> v = {0, 0};
-                if (y0 + li < ny) {
-                    G& g = qgrid[(size_t)(y0 + li) * nz + el];  // cleared as read
-                    v.x = (R)grid_real(g);
-                    g = (G)0;
+                if (y < ny) {
+                    G& g0 = qgrid[(size_t)y * nz + el];  // cleared as read
+                    v.x = (R)grid_real(g0);
+                    g0 = (G)0;
+                    if (y + 1 < ny) {
+                        G& g1 = qgrid[(size_t)(y + 1) * nz + el];
+                        v.y = (R)grid_real(g1);
+                        g1 = (G)0;
+                    }
                 }
                 Z[e] = v;
             }
@@ -880,7 +936,13 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_fft_plane(const int* __restri
             const Cx<R>* res = fft_lds<R>(Z, Z + (size_t)lz * nz, nz, s_rz, rz, false, Tz);
             for (int e = threadIdx.x; e < lz * nzh; e += PLANE_THREADS) {
                 const int li = e / nzh, kz = e - li * nzh;
-                if (y0 + li < ny) P[(size_t)kz * ny + y0 + li] = res[(size_t)li * nz + kz];
+                const int y = 2 * (p0 + li);
+                if (y < ny) {
+                    Cx<R> a, b;
+                    unpack_pair(res[(size_t)li * nz + kz], res[(size_t)li * nz + (kz ? nz - kz : 0)], a, b);
+                    P[(size_t)kz * ny + y] = a;
+                    if (y + 1 < ny) P[(size_t)kz * ny + y + 1] = b;
+                }
             }
             __syncthreads();
         }
@@ -901,26 +963,29 @@ __global__ __launch_bounds__(PLANE_THREADS) void k_fft_plane(const int* __restri
         for (int k0 = 0; k0 < L.lyp; k0 += ly)
             res = fft_lds<R>(P + (size_t)k0 * ny, Q + (size_t)k0 * ny, ny, s_ry, ry, true, Ty) - (size_t)k0 * ny;
         Cx<R>* Z = L.z_shared ? (res == P ? Q : P) : extra;
-        for (int y0 = 0; y0 < ny; y0 += lz) {
+        // The half spectrum of line y at kz is res[kz][y].
+        auto half_at = [&](int y, int el) {
+            if (el < nzh) return res[(size_t)el * ny + y];
+            const Cx<R> v = res[(size_t)(nz - el) * ny + y];
+            return Cx<R>{v.x, -v.y};
+        };
+        for (int p0 = 0; p0 < nyp; p0 += lz) {
             for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
                 const int li = e / nz, el = e - li * nz;
-                const int y = y0 + li;
+                const int y = 2 * (p0 + li);
                 Cx<R> v = {0, 0};
-                if (y < ny) {
-                    if (el < nzh) {
-                        v = res[(size_t)el * ny + y];
-                    } else {
-                        v = res[(size_t)(nz - el) * ny + y];
-                        v.y = -v.y;
-                    }
-                }
+                if (y < ny) v = pack_pair(half_at(y, el), y + 1 < ny ? half_at(y + 1, el) : Cx<R>{0, 0});
                 Z[e] = v;
             }
             __syncthreads();
             const Cx<R>* out = fft_lds<R>(Z, Z + (size_t)lz * nz, nz, s_rz, rz, true, Tz);
             for (int e = threadIdx.x; e < lz * nz; e += PLANE_THREADS) {
                 const int li = e / nz, el = e - li * nz;
-                if (y0 + li < ny) phi[(size_t)(y0 + li) * nz + el] = out[e].x;
+                const int y = 2 * (p0 + li);
+                if (y < ny) {
+                    phi[(size_t)y * nz + el] = out[e].x;
+                    if (y + 1 < ny) phi[(size_t)(y + 1) * nz + el] = out[e].y;
+                }
             }
             __syncthreads();
         }
@@ -1158,7 +1223,8 @@ namespace {
 const int PASS_AXIS[5] = {2, 1, 0, 1, 2};  // MODE_Z_R2C, Y_FWD, X_CONV, Y_INV, Z_C2R
 
 int fft_nlines(const SysInfo& S, int a) {
-    return a == 0 ? S.grid[1] * S.nzh : (a == 1 ? S.grid[0] * S.nzh : S.grid[0] * S.grid[1]);
+    // z passes: pairs of real lines, (ny + 1) / 2 per x plane.
+    return a == 0 ? S.grid[1] * S.nzh : (a == 1 ? S.grid[0] * S.nzh : S.grid[0] * ((S.grid[1] + 1) / 2));
 }
 }  // namespace
 
@@ -1211,7 +1277,7 @@ bool pme_set_fft_plane(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int use)
         if (!ok) break;
         const int ny = S.grid[1], nz = S.grid[2], nzh = S.nzh;
         // y lines per call: all of the half spectrum's lines when the threads
-        // of a line stay at most ny, fewer calls otherwise; z lines per call:
+        // of a line stay at most ny, fewer calls otherwise; z line pairs per call:
         // as many as fit, down to the fewest that keep the threads of a line
         // at most nz.
         auto pow2_at_least = [](int v) {
@@ -1221,7 +1287,7 @@ bool pme_set_fft_plane(PmeDev& pm, std::vector<SysInfo>& sys, int prec, int use)
         };
         const int ly = std::min(PLANE_THREADS, std::max(pow2_at_least(nzh), pow2_at_least((PLANE_THREADS + ny - 1) / ny)));
         const int lz_min = std::min(PLANE_THREADS, pow2_at_least((PLANE_THREADS + nz - 1) / nz));
-        int lz = std::min(PLANE_THREADS, std::max(lz_min, pow2_at_least(ny)));
+        int lz = std::min(PLANE_THREADS, std::max(lz_min, pow2_at_least((ny + 1) / 2)));
         while (lz > lz_min && plane_layout(ny, nz, nzh, lz, ly).elems * csize > PLANE_LDS_MAX) lz >>= 1;
         const size_t bytes = plane_layout(ny, nz, nzh, lz, ly).elems * csize;
         if (bytes > PLANE_LDS_MAX) {
