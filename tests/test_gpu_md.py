@@ -177,8 +177,8 @@ def test_superblock_search_paths():
 
 @pytest.mark.parametrize("graph_steps", [10, 0])
 def test_fast_atom_keeps_the_list_exact(graph_steps):
-    # Atoms launched at ~48 nm/ps (0.05 nm per step, half the list skin)
-    # cross the skin/2 rebuild threshold within one or two steps, in graph
+    # Atoms launched at ~48 nm/ps (0.05 nm per step, about half the list
+    # skin) cross the skin/2 rebuild threshold within two steps, in graph
     # replays and in eager steps alike; the list must follow them, so the
     # forces after the run still match the oracle.
     s = water(500, 2.5, seed=31)
