@@ -55,7 +55,7 @@ TILE_SLOTS = 8 * 64  # pair slots of one k_pairs tile (8-atom cluster x 64 list 
 # HBM bytes per k_pairs launch on the 23,550-atom workload: rocprofv3 FETCH_SIZE
 # and WRITE_SIZE passes, each scaled by the same counter's reading for a plain
 # copy kernel of known size (profiles/k_pairs_md_step_water_23550_atoms.md).
-K_PAIRS_TRAFFIC_BYTES = 48.20e6
+K_PAIRS_TRAFFIC_BYTES = 49.33e6
 
 
 def workload(seed=2024):
